@@ -1,0 +1,148 @@
+/*
+ * pamr_mg_gpu.h -- C ABI of the MI355X (gfx950) multigrid hot path.
+ *
+ * Three layers, all plain C with device pointers and sizes:
+ *
+ *  1. m_util_r8 transfer routines on device memory.  Same arguments and
+ *     meaning as the Fortran routines (include/m_util_r8.h:10-43 of the
+ *     reference), passed by value, plus a HIP stream.  The reference's silent
+ *     bounds-check returns are kept (return 0, nothing written).
+ *
+ *  2. The application MG hooks (include/amrd.h:137-159): red-black
+ *     Gauss-Seidel relax, residual and L_op for the plain Poisson operator
+ *     (op 0, w/examples/nps/num_w.f:18-157) and a variable-coefficient
+ *     operator (op 1).  cmask/chr are double grid functions as in AMRD.
+ *
+ *  3. pmg_solver: a device-resident MG hierarchy over one base grid with
+ *     the FAS V-cycle of amrd/mg.c:84-201, graph-captured, in FP64 (the
+ *     reference method) or with FP32 inner cycles (mixed precision).
+ *
+ * Return codes: 0 success, negative on error (a message goes to stderr).
+ * The shared library is libpamr_mg.so (package directory pamr_amrd_amd/).
+ */
+#ifndef PAMR_MG_GPU_H
+#define PAMR_MG_GPU_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- 1. m_util_r8 on device ------------------------------------------- */
+
+/* replaces dmcopy3d_ (src/m_util_r8.f:20-50; include/m_util_r8.h:10-13) */
+int pmg_dmcopy3d(const double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, int ny2,
+                 int nz2, int i1, int j1, int k1, int i2, int j2, int k2, int ni, int nj,
+                 int nk, int si1, int sj1, int sk1, int si2, int sj2, int sk2,
+                 hipStream_t stream);
+
+/* replaces dmfwr3d_ incl. dmfwr3d_ex (src/m_util_r8.f:733-949;
+ * include/m_util_r8.h:27-31) */
+int pmg_dmfwr3d(const double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, int ny2,
+                int nz2, int i1, int j1, int k1, int i2, int j2, int k2, int ni, int nj, int nk,
+                int si1, int sj1, int sk1, int si2, int sj2, int sk2, const double *chr1,
+                double ex, int do_ex, hipStream_t stream);
+
+/* replaces dmhwr3d_ (src/m_util_r8.f:649-723; include/m_util_r8.h:21-25) */
+int pmg_dmhwr3d(const double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, int ny2,
+                int nz2, int i1, int j1, int k1, int i2, int j2, int k2, int ni, int nj, int nk,
+                int si1, int sj1, int sk1, int si2, int sj2, int sk2, const double *chr1,
+                double ex, int do_ex, hipStream_t stream);
+
+/* replaces dminterp3d_ with ord = 2 (dvi2q1d / dvi2q1d_ex, src/m_util_r8.f:
+ * 332-421,473-637; include/m_util_r8.h:15-19).  ord = 4 returns -2. */
+int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, int ny2, int nz2,
+                   int i1, int j1, int k1, int i2, int j2, int k2, int ni, int nj, int nk,
+                   int si1, int sj1, int sk1, int irho, int jrho, int krho, int ord,
+                   const double *chr1, double ex, int do_ex, hipStream_t stream);
+
+/* ---- 2. application MG hooks on device --------------------------------- */
+
+/* One red-black GS sweep; replaces the app_MG_relax hook body
+ * (include/amrd.h:148-152; w/examples/nps/num_w.f:101-157).  The RMS of the
+ * residuals evaluated during the sweep is written to *norm_dev. */
+int pmg_hook_relax(int op, double *V, const double *rhs, const double *rho, const double *a,
+                   const double *c, const double *cmask, const double *chr, double ex, int nx,
+                   int ny, int nz, double dx, double dy, double dz, double *norm_dev,
+                   hipStream_t stream);
+
+/* res = L[V] - rhs where cmask ON; replaces app_MG_residual
+ * (include/amrd.h:137-141; w/examples/nps/num_w.f:59-90) */
+int pmg_hook_residual(int op, double *res, const double *rhs, const double *V, const double *rho,
+                      const double *a, const double *c, const double *cmask, const double *chr,
+                      double ex, int nx, int ny, int nz, double dx, double dy, double dz,
+                      double *norm_dev, hipStream_t stream);
+
+/* LV = L[V] where cmask ON; replaces app_L_op (include/amrd.h:155-159;
+ * w/examples/nps/num_w.f:18-52) */
+int pmg_hook_lop(int op, double *LV, const double *V, const double *rho, const double *a,
+                 const double *c, const double *cmask, const double *chr, double ex, int nx,
+                 int ny, int nz, double dx, double dy, double dz, hipStream_t stream);
+
+/* ---- 3. uniform-hierarchy solver --------------------------------------- */
+
+typedef struct pmg_solver pmg_solver;
+
+/* grid-function selectors for pmg_field / pmg_upload / pmg_download */
+#define PMG_F_U 0   /* the elliptic variable f (amrd: AMRD_MG_f) */
+#define PMG_F_RHO 1 /* source term of the operator (nps: rho) */
+#define PMG_F_RHS 2 /* f_rhs */
+#define PMG_F_RES 3 /* f_res */
+#define PMG_F_LOP 4 /* f_lop */
+#define PMG_F_A 5   /* op 1: coefficient a */
+#define PMG_F_C 6   /* op 1: coefficient c */
+
+/* Builds the MG hierarchy of one nx x ny x nz vertex grid over bbox
+ * (x0,x1,y0,y1,z0,z1): coarsening by 2 while calc_island_no allows it with
+ * minimum coarse width min_cwidth (PAMR_build_mgh, src/gh.c:1222-1365,
+ * 1427-1467).  Replaces the MG-hierarchy part of PAMR_build_mgh
+ * (src/pamr.c:354-378) for a single-grid base level.  op: 0 Poisson,
+ * 1 variable coefficient; do_ex/ex: excision with chr == ex excised. */
+pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidth, int op,
+                       int do_ex, double ex);
+/* PAMR_destroy_mgh (src/gh.c:1371-1416) */
+void pmg_destroy(pmg_solver *S);
+int pmg_nlev(const pmg_solver *S);
+/* shape of level L (1 = coarsest) */
+int pmg_level_shape(const pmg_solver *S, int L, int *shape);
+/* AMRD MG parameters (amrd/io.c:97-108,269-285): MG_pre_swp, MG_pst_swp,
+ * MG_max_citer, MG_crtol, MG_w0_r, MG_w0_i; restriction tf type (1 straight,
+ * 2 half weight, 3 full weight) of f, f_res and the sources; interp order 2 */
+int pmg_set_params(pmg_solver *S, int pre, int pst, int max_citer, double crtol, double w0_r,
+                   double w0_i, int rop_u, int rop_res, int rop_src, int interp_ord);
+/* device pointer of a grid function on level L */
+double *pmg_field(pmg_solver *S, int L, int which);
+int pmg_upload(pmg_solver *S, int L, int which, const double *host);
+int pmg_download(pmg_solver *S, int L, int which, double *host);
+/* chr grid function of level L (values == ex are excised) */
+int pmg_set_chr(pmg_solver *S, int L, const double *chr_host);
+/* materialise AMRD's cmask of level L: kind 0 set_cmask_bdy, 1 set_cmask_child */
+int pmg_download_cmask(pmg_solver *S, int L, int kind, double *host);
+/* PAMR_build_mgh's initial injection (TF_MGH_INIT) and MG variable reset */
+int pmg_init(pmg_solver *S);
+/* one V-cycle (amrd/mg.c:84-201); *res_out = RMS residual on the finest
+ * level; lev_norms[L-1] = last relax norm per level (nlev doubles);
+ * *citer = iterations of the coarsest-level loop */
+int pmg_vcycle(pmg_solver *S, double *res_out, double *lev_norms, int *citer);
+int pmg_vcycle_async(pmg_solver *S);
+int pmg_read_norms(pmg_solver *S, double *res_out, double *lev_norms, int *citer);
+/* RMS residual of the finest level (writes f_res there) */
+int pmg_residual(pmg_solver *S, double *norm_out);
+/* solve_elliptics loop (amrd/mg.c:56-72), absolute tolerance */
+int pmg_solve(pmg_solver *S, int min_iter, int max_iter, double tol, double *norms, int *iters);
+/* relative solve ||r_k|| <= rtol ||r_0||; mode 0 FP64 V-cycles, 1 FP32
+ * inner V-cycles in FP64 defect correction; hist[k] = ||r_{k+1}|| */
+int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hist, int *iters,
+                  double *r0_out);
+int pmg_last_citer(pmg_solver *S, int which);
+/* execution tuning: sweep tile variant and z chunk; graph capture on/off */
+int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk);
+int pmg_num_sweep_variants(void);
+int pmg_set_graph(pmg_solver *S, int on);
+void *pmg_stream(pmg_solver *S);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,240 @@
+"""pamr_amrd_amd -- MI355X (gfx950) multigrid hot path of PAMR/AMRD.
+
+Python view of the C ABI declared in ``include/pamr_mg_gpu.h`` (and the
+PAMR/AMRD surface in ``include/pamr.h`` / ``include/amrd.h``), implemented by
+``libpamr_mg.so`` in this directory.  Device memory is owned either by the
+library (``Solver``) or by the caller (torch tensors passed as pointers to
+the m_util_r8 / hook entry points).  There is no CPU fallback: when the
+shared library is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpamr_mg.so")
+
+PMG_F_U, PMG_F_RHO, PMG_F_RHS, PMG_F_RES, PMG_F_LOP, PMG_F_A, PMG_F_C = range(7)
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+
+class NativeLibraryMissing(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libpamr_mg.so (raises NativeLibraryMissing when not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise NativeLibraryMissing(
+                f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'`")
+        L = C.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    i, d = C.c_int, C.c_double
+    m_args = [_vp, _vp] + [i] * 21
+    L.pmg_dmcopy3d.argtypes = m_args + [_vp]
+    L.pmg_dmfwr3d.argtypes = m_args + [_vp, d, i, _vp]
+    L.pmg_dmhwr3d.argtypes = m_args + [_vp, d, i, _vp]
+    L.pmg_dminterp3d.argtypes = [_vp, _vp] + [i] * 21 + [i, _vp, d, i, _vp]
+    L.pmg_hook_relax.argtypes = [i] + [_vp] * 7 + [d, i, i, i, d, d, d, _vp, _vp]
+    L.pmg_hook_residual.argtypes = [i] + [_vp] * 8 + [d, i, i, i, d, d, d, _vp, _vp]
+    L.pmg_hook_lop.argtypes = [i] + [_vp] * 7 + [d, i, i, i, d, d, d, _vp]
+    L.pmg_create.argtypes = [i, i, i, _dp, i, i, i, d]
+    L.pmg_create.restype = _vp
+    L.pmg_destroy.argtypes = [_vp]
+    L.pmg_destroy.restype = None
+    L.pmg_nlev.argtypes = [_vp]
+    L.pmg_level_shape.argtypes = [_vp, i, _ip]
+    L.pmg_set_params.argtypes = [_vp, i, i, i, d, d, d, i, i, i, i]
+    L.pmg_field.argtypes = [_vp, i, i]
+    L.pmg_field.restype = _vp
+    L.pmg_upload.argtypes = [_vp, i, i, _dp]
+    L.pmg_download.argtypes = [_vp, i, i, _dp]
+    L.pmg_set_chr.argtypes = [_vp, i, _dp]
+    L.pmg_download_cmask.argtypes = [_vp, i, i, _dp]
+    L.pmg_init.argtypes = [_vp]
+    L.pmg_vcycle.argtypes = [_vp, _dp, _dp, _ip]
+    L.pmg_vcycle_async.argtypes = [_vp]
+    L.pmg_read_norms.argtypes = [_vp, _dp, _dp, _ip]
+    L.pmg_residual.argtypes = [_vp, _dp]
+    L.pmg_solve.argtypes = [_vp, i, i, d, _dp, _ip]
+    L.pmg_solve_rel.argtypes = [_vp, i, d, i, _dp, _ip, _dp]
+    L.pmg_last_citer.argtypes = [_vp, i]
+    L.pmg_set_sweep_config.argtypes = [_vp, i, i]
+    L.pmg_num_sweep_variants.argtypes = []
+    L.pmg_set_graph.argtypes = [_vp, i]
+    L.pmg_stream.argtypes = [_vp]
+    L.pmg_stream.restype = _vp
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")
+
+
+def P(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def tptr(t):
+    """device pointer of a torch tensor (None -> NULL)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+class Solver:
+    """Owning wrapper of a ``pmg_solver`` (device-resident MG hierarchy)."""
+
+    def __init__(self, n, bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), min_cwidth=5, op=0, do_ex=0,
+                 ex=1.0, shape=None):
+        L = lib()
+        nx, ny, nz = shape if shape is not None else (n, n, n)
+        bb = np.ascontiguousarray(bbox, dtype=np.float64)
+        self._L = L
+        self.h = L.pmg_create(nx, ny, nz, P(bb), min_cwidth, op, do_ex, ex)
+        if not self.h:
+            raise RuntimeError("pmg_create failed")
+        self.nlev = L.pmg_nlev(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.pmg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, lev):
+        s = (C.c_int * 3)()
+        _check(self._L.pmg_level_shape(self.h, lev, s), "pmg_level_shape")
+        return (s[2], s[1], s[0])
+
+    def set_params(self, pre=2, pst=2, max_citer=5000, crtol=1e-3, w0_r=1.0, w0_i=1.0, rop_u=3,
+                   rop_res=3, rop_src=3, interp_ord=2):
+        _check(self._L.pmg_set_params(self.h, pre, pst, max_citer, crtol, w0_r, w0_i, rop_u,
+                                      rop_res, rop_src, interp_ord), "pmg_set_params")
+
+    def upload(self, lev, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.shape(lev)
+        _check(self._L.pmg_upload(self.h, lev, which, P(a)), "pmg_upload")
+
+    def download(self, lev, which):
+        a = np.zeros(self.shape(lev))
+        _check(self._L.pmg_download(self.h, lev, which, P(a)), "pmg_download")
+        return a
+
+    def set_chr(self, lev, chr_):
+        a = np.ascontiguousarray(chr_, dtype=np.float64)
+        _check(self._L.pmg_set_chr(self.h, lev, P(a)), "pmg_set_chr")
+
+    def cmask(self, lev, kind):
+        a = np.zeros(self.shape(lev))
+        _check(self._L.pmg_download_cmask(self.h, lev, kind, P(a)), "pmg_download_cmask")
+        return a
+
+    def init(self):
+        _check(self._L.pmg_init(self.h), "pmg_init")
+
+    def vcycle(self):
+        r = C.c_double(0)
+        norms = np.zeros(self.nlev)
+        ci = C.c_int(0)
+        _check(self._L.pmg_vcycle(self.h, C.byref(r), P(norms), C.byref(ci)), "pmg_vcycle")
+        return r.value, norms, ci.value
+
+    def residual(self):
+        r = C.c_double(0)
+        _check(self._L.pmg_residual(self.h, C.byref(r)), "pmg_residual")
+        return r.value
+
+    def solve(self, min_iter, max_iter, tol=0.0):
+        out = np.zeros(max(max_iter, 1))
+        it = C.c_int(0)
+        _check(self._L.pmg_solve(self.h, min_iter, max_iter, tol, P(out), C.byref(it)), "pmg_solve")
+        return out[: it.value]
+
+    def solve_rel(self, mode, rtol, max_iter):
+        hist = np.zeros(max(max_iter, 1))
+        it = C.c_int(0)
+        r0 = C.c_double(0)
+        _check(self._L.pmg_solve_rel(self.h, mode, rtol, max_iter, P(hist), C.byref(it),
+                                     C.byref(r0)), "pmg_solve_rel")
+        return r0.value, hist[: it.value]
+
+    def set_sweep_config(self, variant, zchunk):
+        _check(self._L.pmg_set_sweep_config(self.h, variant, zchunk), "pmg_set_sweep_config")
+
+    def set_graph(self, on):
+        _check(self._L.pmg_set_graph(self.h, 1 if on else 0), "pmg_set_graph")
+
+    def field_ptr(self, lev, which):
+        return self._L.pmg_field(self.h, lev, which)
+
+    def stream(self):
+        return self._L.pmg_stream(self.h)
+
+
+# --------------------------------------------------------------------------
+# synthetic workload generators (host side, deterministic)
+# --------------------------------------------------------------------------
+def splitmix64(seed):
+    """Generator of uint64 values, splitmix64 (Steele, Lea, Flood 2014)."""
+    state = np.uint64(seed)
+    M = np.uint64(0xFFFFFFFFFFFFFFFF)
+    while True:
+        with np.errstate(over="ignore"):
+            state = (state + np.uint64(0x9E3779B97F4A7C15)) & M
+            z = state
+            z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & M
+            z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & M
+            z = z ^ (z >> np.uint64(31))
+        yield z
+
+
+def uniforms(seed, n):
+    g = splitmix64(seed)
+    return np.array([float(int(next(g)) >> 11) * (1.0 / 9007199254740992.0) for _ in range(n)])
+
+
+def gaussian_params(seed, m=8):
+    """8 Gaussians: centres U[0.2,0.8]^3, widths U[0.05,0.15], amplitudes U[-1,1]."""
+    u = uniforms(seed, 5 * m).reshape(m, 5)
+    c = 0.2 + 0.6 * u[:, 0:3]
+    w = 0.05 + 0.10 * u[:, 3]
+    A = -1.0 + 2.0 * u[:, 4]
+    return c, w, A
+
+
+def gaussian_source(n, seed=1234, m=8, params=None, bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):
+    """f(x) = sum_m A_m exp(-(|x-c_m|/w_m)^2) on an n^3 vertex grid (numpy (z,y,x)).
+
+    This is the reference's gauss3d form (examples/nbs/num.f:13-39) with r0=0
+    and no eccentricity; boundary points are set to 0."""
+    c, w, A = params if params is not None else gaussian_params(seed, m)
+    x = np.linspace(bbox[0], bbox[1], n)
+    X, Y, Z = x[None, None, :], x[None, :, None], x[:, None, None]
+    f = np.zeros((n, n, n))
+    for k in range(len(A)):
+        r2 = (X - c[k, 0]) ** 2 + (Y - c[k, 1]) ** 2 + (Z - c[k, 2]) ** 2
+        f += A[k] * np.exp(-r2 / (w[k] * w[k]))
+    f[0, :, :] = f[-1, :, :] = 0
+    f[:, 0, :] = f[:, -1, :] = 0
+    f[:, :, 0] = f[:, :, -1] = 0
+    return f

@@ -1,0 +1,203 @@
+// pmg_hooks.hip -- device implementations of the application MG hooks.
+//
+// These are the grid-function operators the AMRD driver calls per grid
+// (include/amrd.h:137-159): relax (one red-black Gauss-Seidel sweep),
+// residual and L_op.  They act where cmask == AMRD_CMASK_ON (1.0) on points
+// 2..N-1 (1-based) and, when a chr array is given, skip excised points
+// (chr == ex).  op 0 is the plain Poisson operator of the nps example
+// (w/examples/nps/num_w.f:18-157), op 1 the variable-coefficient operator.
+// Arrays are device pointers; cmask and chr are double grid functions as in
+// the reference.  The norm (RMS of the residuals the hook evaluated) is
+// written to a device double, so the caller decides when to synchronise.
+#include "pmg_ops.h"
+#include "../../include/pamr_mg_gpu.h"
+
+using namespace pmg;
+
+namespace {
+
+constexpr int NT = 256;
+
+struct HookGeom {
+  int nx, ny, nz;
+  size_t sy, sz;
+};
+
+__device__ __forceinline__ bool hook_on(const double *cmask, const double *chr, double ex,
+                                        size_t p) {
+  if (cmask[p] != 1.0) return false;
+  if (chr && chr[p] == ex) return false;
+  return true;
+}
+
+// one colour pass of the red-black sweep (num_w.f:130-152): pass 0 updates
+// points with mod(i+j+k+pass,2)==0 in 1-based indices.
+template <int OP, bool P2>
+__global__ __launch_bounds__(NT) void k_hook_relax_pass(double *V, const double *rhs,
+                                                        const double *rho, const double *a,
+                                                        const double *c, const double *cmask,
+                                                        const double *chr, double ex, HookGeom g,
+                                                        Spacing<double> sp, int pass, NormWs ws,
+                                                        double *norm_out, int nb) {
+  const int mi = g.nx - 2, mj = g.ny - 2, mk = g.nz - 2;
+  size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
+  double s = 0.0, cnt = 0.0;
+  if (q < (size_t)mi * mj * mk) {
+    int i = 1 + (int)(q % mi), j = 1 + (int)((q / mi) % mj), k = 1 + (int)(q / ((size_t)mi * mj));
+    size_t p = ix3(i, j, k, g.nx, g.ny);
+    if (((i + j + k + 3 + pass) & 1) == 0 && hook_on(cmask, chr, ex, p)) {
+      double jac;
+      double res = lop_at<double, OP, P2>(V, rho, a, c, p, g.sy, g.sz, sp, &jac) - rhs[p];
+      V[p] = V[p] - res / jac;
+      s = res * res;
+      cnt = 1.0;
+    }
+  }
+  norm_epilogue<NT>(s, cnt, ws, norm_out, pass * nb + blockIdx.x, nb, 2 * nb, pass == 1);
+}
+
+template <int OP, bool P2>
+__global__ __launch_bounds__(NT) void k_hook_lop(double *LV, const double *V, const double *rho,
+                                                 const double *a, const double *c,
+                                                 const double *cmask, const double *chr,
+                                                 double ex, HookGeom g, Spacing<double> sp) {
+  const int mi = g.nx - 2, mj = g.ny - 2, mk = g.nz - 2;
+  size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
+  if (q >= (size_t)mi * mj * mk) return;
+  int i = 1 + (int)(q % mi), j = 1 + (int)((q / mi) % mj), k = 1 + (int)(q / ((size_t)mi * mj));
+  size_t p = ix3(i, j, k, g.nx, g.ny);
+  if (hook_on(cmask, chr, ex, p)) LV[p] = lop_at<double, OP, P2>(V, rho, a, c, p, g.sy, g.sz, sp, nullptr);
+}
+
+// residual (num_w.f:59-90): lop into res, then res -= rhs, norm
+template <int OP, bool P2>
+__global__ __launch_bounds__(NT) void k_hook_residual(double *res, const double *rhs,
+                                                      const double *V, const double *rho,
+                                                      const double *a, const double *c,
+                                                      const double *cmask, const double *chr,
+                                                      double ex, HookGeom g, Spacing<double> sp,
+                                                      NormWs ws, double *norm_out, int nb) {
+  const int mi = g.nx - 2, mj = g.ny - 2, mk = g.nz - 2;
+  size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
+  double s = 0.0, cnt = 0.0;
+  if (q < (size_t)mi * mj * mk) {
+    int i = 1 + (int)(q % mi), j = 1 + (int)((q / mi) % mj), k = 1 + (int)(q / ((size_t)mi * mj));
+    size_t p = ix3(i, j, k, g.nx, g.ny);
+    if (hook_on(cmask, chr, ex, p)) {
+      double r = lop_at<double, OP, P2>(V, rho, a, c, p, g.sy, g.sz, sp, nullptr) - rhs[p];
+      res[p] = r;
+      s = r * r;
+      cnt = 1.0;
+    }
+  }
+  norm_epilogue<NT>(s, cnt, ws, norm_out, blockIdx.x, nb, nb, true);
+}
+
+// Host helpers ---------------------------------------------------------------
+bool is_pow2(double v) {
+  int e;
+  double m = std::frexp(v, &e);
+  return m == 0.5 && v > 1e-300 && v < 1e300;
+}
+
+Spacing<double> make_spacing(double dx, double dy, double dz) {
+  Spacing<double> sp;
+  sp.dx2 = dx * dx;
+  sp.dy2 = dy * dy;
+  sp.dz2 = dz * dz;
+  sp.idx2 = 1.0 / sp.dx2;
+  sp.idy2 = 1.0 / sp.dy2;
+  sp.idz2 = 1.0 / sp.dz2;
+  sp.jac = -2 / sp.dx2 - 2 / sp.dy2 - 2 / sp.dz2;
+  sp.pow2 = is_pow2(sp.dx2) && is_pow2(sp.dy2) && is_pow2(sp.dz2);
+  return sp;
+}
+
+// Hook-layer norm workspace (one per process; the hook layer is
+// single-threaded like the reference driver).
+NormWs g_ws = {nullptr, nullptr, 0};
+
+int ensure_ws(int nparts) {
+  if (nparts <= g_ws.capacity) return 0;
+  if (g_ws.partials) {
+    PMG_HOST_CHECK(hipDeviceSynchronize());
+    PMG_HOST_CHECK(hipFree(g_ws.partials));
+  }
+  int cap = nparts < 4096 ? 4096 : nparts;
+  PMG_HOST_CHECK(hipMalloc(&g_ws.partials, sizeof(double) * 2 * cap));
+  if (!g_ws.ticket) {
+    PMG_HOST_CHECK(hipMalloc(&g_ws.ticket, sizeof(unsigned)));
+    PMG_HOST_CHECK(hipMemset(g_ws.ticket, 0, sizeof(unsigned)));
+  }
+  g_ws.capacity = cap;
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmg_hook_relax(int op, double *V, const double *rhs, const double *rho, const double *a,
+                   const double *c, const double *cmask, const double *chr, double ex, int nx,
+                   int ny, int nz, double dx, double dy, double dz, double *norm_dev,
+                   hipStream_t stream) {
+  if (nx < 3 || ny < 3 || nz < 3 || !V || !rhs || !rho || !cmask) return -1;
+  if (op == 1 && (!a || !c)) return -1;
+  HookGeom g{nx, ny, nz, (size_t)nx, (size_t)nx * ny};
+  Spacing<double> sp = make_spacing(dx, dy, dz);
+  size_t n = (size_t)(nx - 2) * (ny - 2) * (nz - 2);
+  int nb = (int)((n + NT - 1) / NT);
+  if (ensure_ws(2 * nb)) return -1;
+  for (int pass = 0; pass <= 1; pass++) {
+#define PMG_HR(OPV, P2V)                                                                         \
+  hipLaunchKernelGGL((k_hook_relax_pass<OPV, P2V>), dim3(nb), dim3(NT), 0, stream, V, rhs, rho, \
+                     a, c, cmask, chr, ex, g, sp, pass, g_ws, norm_dev, nb)
+    if (op == 0) { if (sp.pow2) PMG_HR(0, true); else PMG_HR(0, false); }
+    else { if (sp.pow2) PMG_HR(1, true); else PMG_HR(1, false); }
+#undef PMG_HR
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+int pmg_hook_residual(int op, double *res, const double *rhs, const double *V, const double *rho,
+                      const double *a, const double *c, const double *cmask, const double *chr,
+                      double ex, int nx, int ny, int nz, double dx, double dy, double dz,
+                      double *norm_dev, hipStream_t stream) {
+  if (nx < 3 || ny < 3 || nz < 3 || !V || !rhs || !rho || !cmask || !res) return -1;
+  if (op == 1 && (!a || !c)) return -1;
+  HookGeom g{nx, ny, nz, (size_t)nx, (size_t)nx * ny};
+  Spacing<double> sp = make_spacing(dx, dy, dz);
+  size_t n = (size_t)(nx - 2) * (ny - 2) * (nz - 2);
+  int nb = (int)((n + NT - 1) / NT);
+  if (ensure_ws(nb)) return -1;
+#define PMG_HS(OPV, P2V)                                                                        \
+  hipLaunchKernelGGL((k_hook_residual<OPV, P2V>), dim3(nb), dim3(NT), 0, stream, res, rhs, V, \
+                     rho, a, c, cmask, chr, ex, g, sp, g_ws, norm_dev, nb)
+  if (op == 0) { if (sp.pow2) PMG_HS(0, true); else PMG_HS(0, false); }
+  else { if (sp.pow2) PMG_HS(1, true); else PMG_HS(1, false); }
+#undef PMG_HS
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int pmg_hook_lop(int op, double *LV, const double *V, const double *rho, const double *a,
+                 const double *c, const double *cmask, const double *chr, double ex, int nx,
+                 int ny, int nz, double dx, double dy, double dz, hipStream_t stream) {
+  if (nx < 3 || ny < 3 || nz < 3 || !V || !LV || !rho || !cmask) return -1;
+  if (op == 1 && (!a || !c)) return -1;
+  HookGeom g{nx, ny, nz, (size_t)nx, (size_t)nx * ny};
+  Spacing<double> sp = make_spacing(dx, dy, dz);
+  size_t n = (size_t)(nx - 2) * (ny - 2) * (nz - 2);
+  int nb = (int)((n + NT - 1) / NT);
+#define PMG_HL(OPV, P2V)                                                                    \
+  hipLaunchKernelGGL((k_hook_lop<OPV, P2V>), dim3(nb), dim3(NT), 0, stream, LV, V, rho, a, \
+                     c, cmask, chr, ex, g, sp)
+  if (op == 0) { if (sp.pow2) PMG_HL(0, true); else PMG_HL(0, false); }
+  else { if (sp.pow2) PMG_HL(1, true); else PMG_HL(1, false); }
+#undef PMG_HL
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"

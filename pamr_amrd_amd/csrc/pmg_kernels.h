@@ -1,0 +1,407 @@
+// pmg_kernels.h -- templated device kernels of the uniform-hierarchy solver.
+//
+// The kernels implement, level by level, the operations of one FAS V-cycle
+// (amrd/mg.c:84-201) over a single grid per level.  They are templated on
+// the arithmetic type (FP64 for the reference method, FP32 for the inner
+// cycles of mixed-precision defect correction), on the operator (0 plain
+// Poisson, 1 variable coefficient) and on excision.
+#pragma once
+#include "pmg_ops.h"
+
+namespace pmg {
+
+// ---------------------------------------------------------------------------
+// Fused red-black Gauss-Seidel sweep (both colours in one pass).
+//
+// A block owns a TX x TY tile of interior (x,y) columns and a chunk of z
+// planes.  It marches in z with a 4-plane LDS ring holding u over the tile
+// plus a 2-point halo: pass 0 ("red", mod(i+j+k+pass,2)==0 in the 1-based
+// indices of num_w.f:134) is evaluated one plane ahead over tile+1, pass 1
+// on the tile, so each plane is read from HBM once and written once.  The
+// sweep reads u_in and writes u_out (ping-pong buffers): blocks never see a
+// neighbour's updated values, so the redundant halo updates are identical
+// to the owner's and results equal the sequential colour-by-colour sweep
+// bit for bit.  Boundary points are not written: both buffers keep the same
+// boundary shell (maintained by the kernels that write boundaries).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SweepP {
+  const T *uin;
+  T *uout;
+  const T *rho, *rhs, *a, *c;
+  const uint8_t *chr;
+  int nx, ny, nz;
+  int bx0, bx1, by0, by1, bz0, bz1;  // updatable box, inclusive, within the interior
+  int zchunk;
+  Spacing<T> sp;
+  NormWs ws;
+  double *norm_out;
+  int nb;
+};
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, bool HALO>
+__device__ __forceinline__ void sweep_update(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int x0,
+                                             int y0, int k, int pass, bool count, double &s,
+                                             double &cnt) {
+  constexpr int W = HALO ? TX + 2 : TX, H = HALO ? TY + 2 : TY, OFF = HALO ? 1 : 2;
+  constexpr int W2 = (W + 1) / 2;
+  T(*S0)[TX + 4] = sU[k & 3];
+  T(*Sm)[TX + 4] = sU[(k - 1) & 3];
+  T(*Sp)[TX + 4] = sU[(k + 1) & 3];
+  const size_t sy = (size_t)P.nx, sz = (size_t)P.nx * P.ny;
+  for (int q = threadIdx.x; q < W2 * H; q += blockDim.x) {
+    const int ly = OFF + q / W2;
+    const int j = y0 - 2 + ly;
+    // first lx >= OFF whose i has the colour's parity: (i+j+k+3+pass) even
+    const int want = (j + k + 3 + pass) & 1;             // parity of i
+    const int lx = OFF + (((want - (x0 - 2 + OFF)) & 1)) + 2 * (q % W2);
+    if (lx > OFF + W - 1) continue;
+    const int i = x0 - 2 + lx;
+    if (i < P.bx0 || i > P.bx1 || j < P.by0 || j > P.by1) continue;
+    const size_t p = ix3(i, j, k, P.nx, P.ny);
+    if constexpr (EX) {
+      if (P.chr[p]) continue;
+    }
+    const T uc = S0[ly][lx];
+    T jac, L;
+    if constexpr (OP == 0) {
+      L = lop_poisson<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
+                             Sp[ly][lx], Sm[ly][lx], P.rho[p], P.sp);
+      jac = P.sp.jac;
+    } else {
+      const T *a = P.a;
+      L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
+                             Sp[ly][lx], Sm[ly][lx], a[p], a[p + 1], a[p - 1], a[p + sy],
+                             a[p - sy], a[p + sz], a[p - sz], P.c[p], P.rho[p], P.sp, &jac);
+    }
+    T res;
+    if constexpr (RHS0) res = L - T(0); else res = L - P.rhs[p];
+    S0[ly][lx] = uc - res / jac;
+    if (count && lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2) {
+      s += (double)res * (double)res;
+      cnt += 1.0;
+    }
+  }
+}
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
+  constexpr int LX = TX + 4, LY = TY + 4;
+  __shared__ T sU[4][LY][LX];
+  const int x0 = 1 + blockIdx.x * TX, y0 = 1 + blockIdx.y * TY;
+  const int z0 = 1 + blockIdx.z * P.zchunk;
+  const int z1 = min(z0 + P.zchunk, P.nz - 1);
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  double s = 0.0, cnt = 0.0;
+
+  auto load = [&](int k) {
+    if (k < 0 || k >= nz) return;
+    T(*S)[LX] = sU[k & 3];
+    for (int q = threadIdx.x; q < LX * LY; q += NT) {
+      const int lx = q % LX, ly = q / LX;
+      const int i = x0 - 2 + lx, j = y0 - 2 + ly;
+      if (i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = P.uin[ix3(i, j, k, nx, ny)];
+    }
+  };
+  auto store = [&](int k) {
+    T(*S)[LX] = sU[k & 3];
+    for (int q = threadIdx.x; q < TX * TY; q += NT) {
+      const int tx = q % TX, ty = q / TX;
+      const int i = x0 + tx, j = y0 + ty;
+      if (i <= nx - 2 && j <= ny - 2) P.uout[ix3(i, j, k, nx, ny)] = S[ty + 2][tx + 2];
+    }
+  };
+
+  load(z0 - 2);
+  load(z0 - 1);
+  load(z0);
+  load(z0 + 1);
+  __syncthreads();
+  if (z0 - 1 >= P.bz0 && z0 - 1 <= P.bz1)
+    sweep_update<T, OP, EX, P2, RHS0, TX, TY, false>(P, sU, x0, y0, z0 - 1, 0, false, s, cnt);
+  if (z0 >= P.bz0 && z0 <= P.bz1)
+    sweep_update<T, OP, EX, P2, RHS0, TX, TY, true>(P, sU, x0, y0, z0, 0, true, s, cnt);
+  __syncthreads();
+  for (int k = z0; k < z1; k++) {
+    load(k + 2);
+    __syncthreads();
+    if (k + 1 >= P.bz0 && k + 1 <= P.bz1)
+      sweep_update<T, OP, EX, P2, RHS0, TX, TY, true>(P, sU, x0, y0, k + 1, 0, k + 1 < z1, s, cnt);
+    __syncthreads();
+    if (k >= P.bz0 && k <= P.bz1)
+      sweep_update<T, OP, EX, P2, RHS0, TX, TY, false>(P, sU, x0, y0, k, 1, true, s, cnt);
+    __syncthreads();
+    store(k);
+  }
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  norm_epilogue<NT>(s, cnt, P.ws, P.norm_out, bid, P.nb, P.nb, true);
+}
+
+// ---------------------------------------------------------------------------
+// Residual res = L[u] - rhs on updatable points (num_w.f:59-90).  Other
+// points: ZERO mode writes 0 (clear_MG_vars then residual, amrd/mg.c:91,104
+// on the finest level); KEEP mode leaves them (coarser levels, whose other
+// points already hold the injected values).
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ResidP {
+  const T *u, *rho, *rhs, *a, *c;
+  const uint8_t *chr;
+  T *res;
+  float *neg32;  // OUT32 mode: writes (float)(-res) here instead of res
+  int nx, ny, nz;
+  int bx0, bx1, by0, by1, bz0, bz1;
+  Spacing<T> sp;
+  NormWs ws;
+  double *norm_out;
+  int nb;
+};
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0, bool ZERO, bool OUT32, int NT>
+__global__ __launch_bounds__(NT) void k_residual(ResidP<T> P) {
+  const int nx = P.nx, ny = P.ny;
+  const size_t sy = nx, sz = (size_t)nx * ny;
+  // block covers NT consecutive x of one (j,k) row segment
+  const int segs = (nx + NT - 1) / NT;
+  const int row = blockIdx.x / segs, seg = blockIdx.x % segs;
+  const int j = row % ny, k = row / ny;
+  const int i = seg * NT + threadIdx.x;
+  double s = 0.0, cnt = 0.0;
+  if (i < nx && k < P.nz) {
+    const size_t p = ix3(i, j, k, nx, ny);
+    bool on = i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1 && k >= P.bz0 && k <= P.bz1;
+    if constexpr (EX) on = on && !P.chr[p];
+    if (on) {
+      T L = lop_at<T, OP, P2>(P.u, P.rho, P.a, P.c, p, sy, sz, P.sp, nullptr);
+      T r;
+      if constexpr (RHS0) r = L - T(0); else r = L - P.rhs[p];
+      if constexpr (OUT32) P.neg32[p] = (float)(-r); else P.res[p] = r;
+      s = (double)r * (double)r;
+      cnt = 1.0;
+    } else if constexpr (ZERO) {
+      if constexpr (OUT32) P.neg32[p] = 0.0f; else P.res[p] = T(0);
+    }
+  }
+  norm_epilogue<NT>(s, cnt, P.ws, P.norm_out, blockIdx.x, P.nb, P.nb, true);
+}
+
+// ---------------------------------------------------------------------------
+// Whole-grid restriction fine (nf) -> coarse (nc = (nf-1)/2+1), the
+// local_transfer coarsen call of a single-grid level (src/transfer.c:1126-
+// 1149 with i1=i2=1, stride 2).  kind: 1 straight, 2 half weight, 3 full
+// weight.  Boundary points are also written to `alt` (the second ping-pong
+// buffer of u) when given.
+// ---------------------------------------------------------------------------
+template <typename T, int KIND, bool EXC>
+__global__ __launch_bounds__(256) void k_restrict_whole(const T *u1, T *u2, T *alt,
+                                                        const T *chr1, T ex, int nx1, int ny1,
+                                                        int nz1, int nx2, int ny2, int nz2) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (size_t)nx2 * ny2 * nz2) return;
+  const int i = (int)(q % nx2), j = (int)((q / nx2) % ny2), k = (int)(q / ((size_t)nx2 * ny2));
+  const int i0 = 1 + 2 * i, j0 = 1 + 2 * j, k0 = 1 + 2 * k;
+  T v;
+  if constexpr (KIND == 1) {
+    v = u1[ix3(i0 - 1, j0 - 1, k0 - 1, nx1, ny1)];
+  } else if constexpr (KIND == 2) {
+    v = hw_point<T, EXC>(u1, chr1, ex, nx1, ny1, nz1, i0, j0, k0);
+  } else {
+    if constexpr (EXC) v = fw_point_ex<T>(u1, chr1, ex, nx1, ny1, nz1, i0, j0, k0, i + 1);
+    else v = fw_point<T>(u1, nx1, ny1, nz1, i0, j0, k0);
+  }
+  u2[q] = v;
+  if (alt && (i == 0 || j == 0 || k == 0 || i == nx2 - 1 || j == ny2 - 1 || k == nz2 - 1))
+    alt[q] = v;
+}
+
+// ---------------------------------------------------------------------------
+// L_op + compute_MG_rhs fused (amrd/mg.c:124-126, 271-304, save_tre = 0):
+//   lop = L[u] on interior points of the child-mask box (not excised), 0
+//   elsewhere (cleared by clear_MG_vars); rhs = -res*w0_r + lop inside the
+//   child-mask box, 0 outside; then lop = u.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LopRhsP {
+  const T *u, *rho, *res, *a, *c;
+  const uint8_t *chr;
+  T *rhs, *lop;
+  int nx, ny, nz;
+  int cx0, cx1, cy0, cy1, cz0, cz1;  // child-mask ON box (may include boundary)
+  T w0_r;
+  Spacing<T> sp;
+};
+
+template <typename T, int OP, bool EX, bool P2>
+__global__ __launch_bounds__(256) void k_lop_rhs(LopRhsP<T> P) {
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (size_t)nx * ny * nz) return;
+  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+  const bool on = i >= P.cx0 && i <= P.cx1 && j >= P.cy0 && j <= P.cy1 && k >= P.cz0 && k <= P.cz1;
+  const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
+  T lopv = T(0);
+  bool act = on && interior;
+  if constexpr (EX) act = act && !P.chr[q];
+  if (act) lopv = lop_at<T, OP, P2>(P.u, P.rho, P.a, P.c, q, (size_t)nx, (size_t)nx * ny, P.sp, nullptr);
+  P.rhs[q] = on ? (-P.res[q]) * P.w0_r + lopv : T(0);
+  P.lop[q] = P.u[q];
+}
+
+// ---------------------------------------------------------------------------
+// Coarse-grid correction, no excision (amrd/mg.c:323-355): with
+// R = (u_c - lop_c)/w0_i at coarse points, u_f += P(R) where P is the
+// dminterp3d trilinear composition (x, then y, then z phases; c0=c1=0.5 for
+// rho=2) -- computed per fine point, so the interpolated f_res is consumed
+// directly instead of being stored.  Boundary points also go to `alt`.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_cgc(T *u, T *alt, const T *uc, const T *lopc, T w0_i,
+                                             int nx, int ny, int nz, int cnx, int cny,
+                                             const uint8_t *chr) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (size_t)nx * ny * nz) return;
+  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+  const int ai = i >> 1, ri = i & 1, aj = j >> 1, rj = j & 1, ak = k >> 1, rk = k & 1;
+  const T c0 = T(1 - (1 * 1.0) / 2), c1 = T((1 * 1.0) / 2);
+  auto R = [&](int a, int b, int c) -> T {
+    const size_t pc = ix3(a, b, c, cnx, cny);
+    return (uc[pc] - lopc[pc]) / w0_i;
+  };
+  auto X = [&](int b, int c) -> T { return ri == 0 ? R(ai, b, c) : c0 * R(ai, b, c) + c1 * R(ai + 1, b, c); };
+  auto Y = [&](int c) -> T { return rj == 0 ? X(aj, c) : c0 * X(aj, c) + c1 * X(aj + 1, c); };
+  T v = rk == 0 ? Y(ak) : c0 * Y(ak) + c1 * Y(ak + 1);
+  if (chr && chr[q]) v = T(0);
+  const T nu = u[q] + v;
+  u[q] = nu;
+  if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) alt[q] = nu;
+}
+
+// coarse res = (u - lop)/w0_i (first loop of compute_cgc), for the
+// excision path where dminterp3d_ex runs as separate line passes
+template <typename T>
+__global__ __launch_bounds__(256) void k_cgc_diff(T *res, const T *u, const T *lop, T w0_i,
+                                                  size_t n) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) res[q] = (u[q] - lop[q]) / w0_i;
+}
+
+// clear_ex (src/transfer.c:930-935) on res, then u += res (whole grid)
+template <typename T>
+__global__ __launch_bounds__(256) void k_cgc_add_ex(T *u, T *alt, T *res, const uint8_t *chr,
+                                                    int nx, int ny, int nz) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= (size_t)nx * ny * nz) return;
+  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+  T r = res[q];
+  if (chr[q]) { r = T(0); res[q] = r; }
+  const T nu = u[q] + r;
+  u[q] = nu;
+  if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) alt[q] = nu;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_clear_ex(T *f, const uint8_t *chr, size_t n) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n && chr[q]) f[q] = T(0);
+}
+
+// ---------------------------------------------------------------------------
+// Coarsest level in one workgroup (amrd/mg.c:135-141,161-190 at L=1): the
+// pre-sweeps, then ctol = res_pre*MG_crtol, the post-sweeps, then relax
+// until res <= ctol or iter == MG_max_citer (iter continues from pst).
+// u, rho, rhs (and a, c, chr) live in LDS when they fit; sweeps update in
+// place colour by colour, as the hook does.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct CoarseP {
+  T *u;
+  const T *rho, *rhs, *a, *c;
+  const uint8_t *chr;
+  int nx, ny, nz;
+  int bx0, bx1, by0, by1, bz0, bz1;
+  Spacing<T> sp;
+  int npre, npst, max_citer;
+  double crtol;
+  double *norm_pre, *norm_post;  // device outputs (res[1] after pre / final)
+  int *citer;                    // device output: iteration count at exit
+};
+
+template <typename T, int OP, bool EX, bool P2, bool LDS, int NT>
+__global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
+  extern __shared__ unsigned char smem[];
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const size_t n = (size_t)nx * ny * nz, sy = nx, sz = (size_t)nx * ny;
+  T *u = P.u;
+  const T *rho = P.rho, *rhs = P.rhs, *a = P.a, *c = P.c;
+  if constexpr (LDS) {
+    T *su = reinterpret_cast<T *>(smem);
+    T *srho = su + n, *srhs = srho + n;
+    T *sa = srhs + n, *sc = sa + (OP == 1 ? n : 0);
+    for (size_t q = threadIdx.x; q < n; q += NT) {
+      su[q] = P.u[q];
+      srho[q] = P.rho[q];
+      srhs[q] = P.rhs[q];
+      if constexpr (OP == 1) { sa[q] = P.a[q]; sc[q] = P.c[q]; }
+    }
+    __syncthreads();
+    u = su; rho = srho; rhs = srhs;
+    if constexpr (OP == 1) { a = sa; c = sc; }
+  }
+  const int mi = P.bx1 - P.bx0 + 1, mj = P.by1 - P.by0 + 1, mk = P.bz1 - P.bz0 + 1;
+  const int npts = (mi > 0 && mj > 0 && mk > 0) ? mi * mj * mk : 0;
+  __shared__ double bres;
+  auto sweep = [&]() -> double {
+    double s = 0.0, cnt = 0.0;
+    for (int pass = 0; pass <= 1; pass++) {
+      for (int q = threadIdx.x; q < npts; q += NT) {
+        const int i = P.bx0 + q % mi, j = P.by0 + (q / mi) % mj, k = P.bz0 + q / (mi * mj);
+        if (((i + j + k + 3 + pass) & 1) != 0) continue;
+        const size_t p = ix3(i, j, k, nx, ny);
+        if constexpr (EX) { if (P.chr[p]) continue; }
+        T jac;
+        const T r = lop_at<T, OP, P2>(u, rho, a, c, p, sy, sz, P.sp, &jac) - rhs[p];
+        u[p] = u[p] - r / jac;
+        s += (double)r * (double)r;
+        cnt += 1.0;
+      }
+      __syncthreads();
+    }
+    block_reduce2<NT>(s, cnt);
+    if (threadIdx.x == 0) bres = sqrt(s / cnt);
+    __syncthreads();
+    return bres;
+  };
+  double res = 0.0;
+  for (int it = 0; it < P.npre; it++) res = sweep();
+  const double ctol = res * P.crtol;
+  if (threadIdx.x == 0 && P.norm_pre) P.norm_pre[0] = res;
+  int iter = 0;
+  for (iter = 0; iter < P.npst; iter++) res = sweep();
+  res = ctol + 1;
+  while (iter < P.max_citer && res > ctol) {
+    res = sweep();
+    iter++;
+  }
+  if constexpr (LDS) {
+    for (size_t q = threadIdx.x; q < n; q += NT) P.u[q] = u[q];
+  }
+  if (threadIdx.x == 0) {
+    if (P.norm_post) P.norm_post[0] = res;
+    if (P.citer) P.citer[0] = iter;
+  }
+}
+
+// elementwise helpers ---------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy(T *dst, const T *src, size_t n) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[q] = src[q];
+}
+
+template <typename TD, typename TS>
+__global__ __launch_bounds__(256) void k_convert(TD *dst, const TS *src, size_t n) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[q] = (TD)src[q];
+}
+
+}  // namespace pmg

@@ -1,0 +1,230 @@
+// pmg_ops.h -- pointwise device operators shared by every kernel.
+//
+// The stencil expressions reproduce, operation by operation, the reference
+// hooks and m_util_r8 routines cited at each function; with contraction
+// disabled they give the same IEEE result as the reference's Fortran.
+#pragma once
+#include "pmg_common.h"
+
+namespace pmg {
+
+// x/d2, or x*inv when d2 is an exact power of two (then identical bits).
+template <typename T, bool P2>
+__device__ __forceinline__ T divd(T x, T d2, T inv) {
+  if constexpr (P2) return x * inv; else return x / d2;
+}
+
+// Plain Poisson operator L[V] = lap_h V - rho at one point
+// (w/examples/nps/num_w.f:42-45; relax/residual terms at :136-140).
+// Neighbour naming: e/w = i+1/i-1, n/s = j+1/j-1, t/b = k+1/k-1.
+template <typename T, bool P2>
+__device__ __forceinline__ T lop_poisson(T uc, T ue, T uw, T un, T us, T ut, T ub,
+                                         T rho, const Spacing<T> &sp) {
+  T lx = divd<T, P2>((ue - 2 * uc) + uw, sp.dx2, sp.idx2);
+  T ly = divd<T, P2>((un - 2 * uc) + us, sp.dy2, sp.idy2);
+  T lz = divd<T, P2>((ut - 2 * uc) + ub, sp.dz2, sp.idz2);
+  return ((lx + ly) + lz) - rho;
+}
+
+// Variable-coefficient operator L[V] = div(a grad V) + c V - rho with
+// arithmetic-mean face coefficients (project-defined hook in the nps style;
+// the reference ships only constant-coefficient hooks).
+template <typename T, bool P2>
+__device__ __forceinline__ T lop_varcoef(T uc, T ue, T uw, T un, T us, T ut, T ub,
+                                         T ac, T ae, T aw, T an, T as, T at, T ab,
+                                         T cc, T rho, const Spacing<T> &sp, T *jac) {
+  T fe = T(0.5) * (ac + ae), fw = T(0.5) * (ac + aw);
+  T fn = T(0.5) * (ac + an), fs = T(0.5) * (ac + as);
+  T ft = T(0.5) * (ac + at), fb = T(0.5) * (ac + ab);
+  T lx = divd<T, P2>(fe * (ue - uc) - fw * (uc - uw), sp.dx2, sp.idx2);
+  T ly = divd<T, P2>(fn * (un - uc) - fs * (uc - us), sp.dy2, sp.idy2);
+  T lz = divd<T, P2>(ft * (ut - uc) - fb * (uc - ub), sp.dz2, sp.idz2);
+  if (jac)
+    *jac = ((divd<T, P2>(-(fe + fw), sp.dx2, sp.idx2) - divd<T, P2>(fn + fs, sp.dy2, sp.idy2)) -
+            divd<T, P2>(ft + fb, sp.dz2, sp.idz2)) + cc;
+  return (((lx + ly) + lz) + cc * uc) - rho;
+}
+
+// L at point p of a plain array (global memory neighbours).
+template <typename T, int OP, bool P2>
+__device__ __forceinline__ T lop_at(const T *V, const T *rho, const T *a, const T *c,
+                                    size_t p, size_t sy, size_t sz, const Spacing<T> &sp,
+                                    T *jac) {
+  if constexpr (OP == 0) {
+    if (jac) *jac = sp.jac;
+    return lop_poisson<T, P2>(V[p], V[p + 1], V[p - 1], V[p + sy], V[p - sy], V[p + sz],
+                              V[p - sz], rho[p], sp);
+  } else {
+    return lop_varcoef<T, P2>(V[p], V[p + 1], V[p - 1], V[p + sy], V[p - sy], V[p + sz],
+                              V[p - sz], a[p], a[p + 1], a[p - 1], a[p + sy], a[p - sy],
+                              a[p + sz], a[p - sz], c[p], rho[p], sp, jac);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Restriction point operators.  (i0,j0,k0) are 1-based fine indices as in
+// src/m_util_r8.f; U(a,b,c) reads the fine array.
+// ---------------------------------------------------------------------------
+#define PMG_U1(a, b, c) u1[ix3((a)-1, (b)-1, (c)-1, nx1, ny1)]
+#define PMG_CH(a, b, c) chr1[ix3((a)-1, (b)-1, (c)-1, nx1, ny1)]
+
+// dmfwr3d, src/m_util_r8.f:775-823
+template <typename T>
+__device__ __forceinline__ T fw_point(const T *u1, int nx1, int ny1, int nz1, int i0, int j0,
+                                      int k0) {
+  const int im = i0 - 1, jm = j0 - 1, km = k0 - 1, ip = i0 + 1, jp = j0 + 1, kp = k0 + 1;
+  const bool xi = i0 > 1 && i0 <= nx1 - 1, yi = j0 > 1 && j0 <= ny1 - 1,
+             zi = k0 > 1 && k0 <= nz1 - 1;
+  if (xi && yi && zi) {
+    return (PMG_U1(im, jm, km) + 2 * PMG_U1(i0, jm, km) + PMG_U1(ip, jm, km) +
+            2 * PMG_U1(im, j0, km) + 4 * PMG_U1(i0, j0, km) + 2 * PMG_U1(ip, j0, km) +
+            PMG_U1(im, jp, km) + 2 * PMG_U1(i0, jp, km) + PMG_U1(ip, jp, km) +
+            2 * PMG_U1(im, jm, k0) + 4 * PMG_U1(i0, jm, k0) + 2 * PMG_U1(ip, jm, k0) +
+            4 * PMG_U1(im, j0, k0) + 8 * PMG_U1(i0, j0, k0) + 4 * PMG_U1(ip, j0, k0) +
+            2 * PMG_U1(im, jp, k0) + 4 * PMG_U1(i0, jp, k0) + 2 * PMG_U1(ip, jp, k0) +
+            PMG_U1(im, jm, kp) + 2 * PMG_U1(i0, jm, kp) + PMG_U1(ip, jm, kp) +
+            2 * PMG_U1(im, j0, kp) + 4 * PMG_U1(i0, j0, kp) + 2 * PMG_U1(ip, j0, kp) +
+            PMG_U1(im, jp, kp) + 2 * PMG_U1(i0, jp, kp) + PMG_U1(ip, jp, kp)) / T(64.0);
+  } else if (xi && yi) {
+    return (1 * PMG_U1(im, jm, k0) + 2 * PMG_U1(i0, jm, k0) + 1 * PMG_U1(ip, jm, k0) +
+            2 * PMG_U1(im, j0, k0) + 4 * PMG_U1(i0, j0, k0) + 2 * PMG_U1(ip, j0, k0) +
+            1 * PMG_U1(im, jp, k0) + 2 * PMG_U1(i0, jp, k0) + 1 * PMG_U1(ip, jp, k0)) / T(16.0);
+  } else if (yi && zi) {
+    return (1 * PMG_U1(i0, jm, km) + 2 * PMG_U1(i0, jm, k0) + 1 * PMG_U1(i0, jm, kp) +
+            2 * PMG_U1(i0, j0, km) + 4 * PMG_U1(i0, j0, k0) + 2 * PMG_U1(i0, j0, kp) +
+            1 * PMG_U1(i0, jp, km) + 2 * PMG_U1(i0, jp, k0) + 1 * PMG_U1(i0, jp, kp)) / T(16.0);
+  } else if (xi && zi) {
+    return (1 * PMG_U1(im, j0, km) + 2 * PMG_U1(im, j0, k0) + 1 * PMG_U1(im, j0, kp) +
+            2 * PMG_U1(i0, j0, km) + 4 * PMG_U1(i0, j0, k0) + 2 * PMG_U1(i0, j0, kp) +
+            1 * PMG_U1(ip, j0, km) + 2 * PMG_U1(ip, j0, k0) + 1 * PMG_U1(ip, j0, kp)) / T(16.0);
+  } else if (xi) {
+    return (PMG_U1(im, j0, k0) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(ip, j0, k0)) / T(4.0);
+  } else if (yi) {
+    return (PMG_U1(i0, jm, k0) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(i0, jp, k0)) / T(4.0);
+  } else if (zi) {
+    return (PMG_U1(i0, j0, km) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(i0, j0, kp)) / T(4.0);
+  }
+  return PMG_U1(i0, j0, k0);
+}
+
+__device__ __forceinline__ int w3d(int a, int b, int c) {
+  // w3d(3,3,3) of dmfwr3d_ex, src/m_util_r8.f:851-853: (2-|a-2|)(2-|b-2|)(2-|c-2|)
+  return (2 - abs(a - 2)) * (2 - abs(b - 2)) * (2 - abs(c - 2)) / 1;
+}
+__device__ __forceinline__ int w3d_flat(int flat) {
+  int a = flat % 3 + 1, b = (flat / 3) % 3 + 1, c = flat / 9 + 1;
+  return w3d(a, b, c);
+}
+// 1D-k branch weight of dmfwr3d_ex (src/m_util_r8.f:931 reads w3d(i,1,kk),
+// i the coarse loop index): table entry while (i-1)+9(kk-1) < 27, the
+// intended w3d(1,1,kk) beyond it (there the reference read is undefined).
+__device__ __forceinline__ int w3d_kbranch(int i, int kk) {
+  int flat = (i - 1) + 9 * (kk - 1);
+  return (flat >= 0 && flat < 27) ? w3d_flat(flat) : w3d(1, 1, kk);
+}
+
+// dmfwr3d_ex, src/m_util_r8.f:855-943.  iloop = coarse loop index i.
+template <typename T>
+__device__ __forceinline__ T fw_point_ex(const T *u1, const T *chr1, T ex, int nx1, int ny1,
+                                         int nz1, int i0, int j0, int k0, int iloop) {
+  const int im = i0 - 1, jm = j0 - 1, km = k0 - 1, ip = i0 + 1, jp = j0 + 1, kp = k0 + 1;
+  const bool xi = i0 > 1 && i0 <= nx1 - 1, yi = j0 > 1 && j0 <= ny1 - 1,
+             zi = k0 > 1 && k0 <= nz1 - 1;
+  T sum = 0;
+  int w = 0;
+  if (xi && yi && zi) {
+    for (int a = im; a <= ip; a++)
+      for (int b = jm; b <= jp; b++)
+        for (int c = km; c <= kp; c++)
+          if (PMG_CH(a, b, c) != ex) {
+            int ww = w3d(a - im + 1, b - jm + 1, c - km + 1);
+            sum = sum + ww * PMG_U1(a, b, c);
+            w += ww;
+          }
+  } else if (xi && yi) {
+    for (int a = im; a <= ip; a++)
+      for (int b = jm; b <= jp; b++)
+        if (PMG_CH(a, b, k0) != ex) {
+          int ww = w3d(a - im + 1, b - jm + 1, 1);
+          sum = sum + ww * PMG_U1(a, b, k0);
+          w += ww;
+        }
+  } else if (yi && zi) {
+    for (int b = jm; b <= jp; b++)
+      for (int c = km; c <= kp; c++)
+        if (PMG_CH(i0, b, c) != ex) {
+          int ww = w3d(1, b - jm + 1, c - km + 1);
+          sum = sum + ww * PMG_U1(i0, b, c);
+          w += ww;
+        }
+  } else if (xi && zi) {
+    for (int a = im; a <= ip; a++)
+      for (int c = km; c <= kp; c++)
+        if (PMG_CH(a, j0, c) != ex) {
+          int ww = w3d(a - im + 1, 1, c - km + 1);
+          sum = sum + ww * PMG_U1(a, j0, c);
+          w += ww;
+        }
+  } else if (xi) {
+    for (int a = im; a <= ip; a++)
+      if (PMG_CH(a, j0, k0) != ex) {
+        int ww = w3d(a - im + 1, 1, 1);
+        sum = sum + ww * PMG_U1(a, j0, k0);
+        w += ww;
+      }
+  } else if (yi) {
+    for (int b = jm; b <= jp; b++)
+      if (PMG_CH(i0, b, k0) != ex) {
+        int ww = w3d(1, b - jm + 1, 1);
+        sum = sum + ww * PMG_U1(i0, b, k0);
+        w += ww;
+      }
+  } else if (zi) {
+    for (int c = km; c <= kp; c++)
+      if (PMG_CH(i0, j0, c) != ex) {
+        sum = sum + w3d(1, 1, c - km + 1) * PMG_U1(i0, j0, c);
+        w += w3d_kbranch(iloop, c - km + 1);
+      }
+  } else {
+    sum = PMG_U1(i0, j0, k0);
+    w = 1;
+  }
+  if (w == 0 || PMG_CH(i0, j0, k0) == ex) return T(0);
+  return sum / w;
+}
+
+// dmhwr3d, src/m_util_r8.f:675-717
+template <typename T, bool EXC>
+__device__ __forceinline__ T hw_point(const T *u1, const T *chr1, T ex, int nx1, int ny1,
+                                      int nz1, int i0, int j0, int k0) {
+  int w = 0;
+  T sum = 0;
+  if (i0 > 1 && i0 <= nx1 - 1) {
+    if (!EXC || (PMG_CH(i0 - 1, j0, k0) != ex && PMG_CH(i0, j0, k0) != ex &&
+                 PMG_CH(i0 + 1, j0, k0) != ex)) {
+      w += 4;
+      sum = sum + PMG_U1(i0 - 1, j0, k0) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(i0 + 1, j0, k0);
+    }
+  }
+  if (j0 > 1 && j0 <= ny1 - 1) {
+    if (!EXC || (PMG_CH(i0, j0 - 1, k0) != ex && PMG_CH(i0, j0, k0) != ex &&
+                 PMG_CH(i0, j0 + 1, k0) != ex)) {
+      w += 4;
+      sum = sum + PMG_U1(i0, j0 - 1, k0) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(i0, j0 + 1, k0);
+    }
+  }
+  if (k0 > 1 && k0 <= nz1 - 1) {
+    if (!EXC || (PMG_CH(i0, j0, k0 - 1) != ex && PMG_CH(i0, j0, k0) != ex &&
+                 PMG_CH(i0, j0, k0 + 1) != ex)) {
+      w += 4;
+      sum = sum + PMG_U1(i0, j0, k0 - 1) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(i0, j0, k0 + 1);
+    }
+  }
+  if (EXC && PMG_CH(i0, j0, k0) == ex) return T(0);
+  if (w > 0) return sum / w;
+  return PMG_U1(i0, j0, k0);
+}
+#undef PMG_U1
+#undef PMG_CH
+
+}  // namespace pmg

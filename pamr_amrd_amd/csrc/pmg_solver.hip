@@ -1,0 +1,917 @@
+// pmg_solver.hip -- the uniform-hierarchy FAS multigrid solver on one GPU.
+//
+// A pmg_solver is the device-resident counterpart of an AMRD MG hierarchy
+// built over a single base grid (PAMR_build_mgh, src/gh.c:1222-1365): every
+// MG level is one grid, coarsened by 2 while calc_island_no allows it
+// (src/gh.c:1427-1467).  pmg_vcycle executes amrd/mg.c:84-201 with the
+// plain hooks on the device; the whole cycle is enqueued on one stream with
+// no host synchronisation (the coarsest-grid loop and its stopping test run
+// inside one workgroup), so it can be captured once in a hipGraph and
+// replayed.  Mixed precision (FP32 inner V-cycles inside FP64 defect
+// correction) reuses the same kernels instantiated for float.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pmg_kernels.h"
+#include "pmg_solver.h"
+
+using namespace pmg;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// set_cmask_bdy / set_cmask_child for one grid (amrd/mg.c:362-488): the ON
+// index box (0-based, inclusive).  ok = false when the box is empty.
+// ---------------------------------------------------------------------------
+struct Box {
+  int lo[3], hi[3];
+  bool ok;
+};
+
+Box cmask_bdy_box(const int *shape, const double *gbbox, const double *base, const double *gdx) {
+  Box b;
+  for (int d = 0; d < 3; d++) {
+    b.lo[d] = 0;
+    b.hi[d] = shape[d] - 1;
+    if ((gbbox[2 * d] - base[2 * d]) > gdx[d] / 2) b.lo[d]++;
+    if ((base[2 * d + 1] - gbbox[2 * d + 1]) > gdx[d] / 2) b.hi[d]--;
+  }
+  b.ok = true;
+  for (int d = 0; d < 3; d++)
+    if (b.lo[d] < 0 || b.lo[d] > b.hi[d] || b.hi[d] >= shape[d]) b.ok = false;
+  return b;
+}
+
+Box cmask_child_box(const int *shape, const double *gbbox, const double *base, const double *gdx,
+                    const double *cb) {
+  Box b;
+  for (int d = 0; d < 3; d++) {
+    int s = (int)std::max((cb[2 * d] - gbbox[2 * d]) / gdx[d] + 0.5, 0.0);
+    int e = (int)std::min((cb[2 * d + 1] - gbbox[2 * d]) / gdx[d] + 0.5, (double)(shape[d] - 1));
+    if (s > 0 || (gbbox[2 * d] - base[2 * d]) > gdx[d] / 2) s++;
+    if (e < (shape[d] - 1) || (base[2 * d + 1] - gbbox[2 * d + 1]) > gdx[d] / 2) e--;
+    b.lo[d] = s;
+    b.hi[d] = e;
+  }
+  b.ok = true;
+  for (int d = 0; d < 3; d++)
+    if (b.lo[d] < 0 || b.lo[d] > b.hi[d] || b.hi[d] >= shape[d]) b.ok = false;
+  return b;
+}
+
+bool is_pow2(double v) {
+  int e;
+  double m = std::frexp(v, &e);
+  return m == 0.5 && v > 1e-300 && v < 1e300;
+}
+
+template <typename T>
+Spacing<T> make_spacing(const double *hdx) {
+  Spacing<T> sp;
+  T dx = (T)hdx[0], dy = (T)hdx[1], dz = (T)hdx[2];
+  sp.dx2 = dx * dx;
+  sp.dy2 = dy * dy;
+  sp.dz2 = dz * dz;
+  sp.idx2 = T(1) / sp.dx2;
+  sp.idy2 = T(1) / sp.dy2;
+  sp.idz2 = T(1) / sp.dz2;
+  sp.jac = -2 / sp.dx2 - 2 / sp.dy2 - 2 / sp.dz2;
+  sp.pow2 = is_pow2((double)sp.dx2) && is_pow2((double)sp.dy2) && is_pow2((double)sp.dz2);
+  return sp;
+}
+
+inline unsigned nblk256(size_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// solver state
+// ---------------------------------------------------------------------------
+template <typename T>
+struct LevelArrays {
+  T *u = nullptr, *ua = nullptr, *rho = nullptr, *rhs = nullptr, *res = nullptr, *lop = nullptr;
+  T *a = nullptr, *c = nullptr;
+};
+
+struct Level {
+  int n[3];
+  size_t size;
+  double hdx[3], gdx[3];
+  Spacing<double> sp64;
+  Spacing<float> sp32;
+  Box rbox;  // relax/residual box: cmask_bdy box clipped to the interior
+  Box cbox;  // child mask box (levels below the finest)
+  LevelArrays<double> d;
+  LevelArrays<float> f;
+  uint8_t *chr = nullptr;   // excision mask, 1 = excised (sweeps, hooks)
+  double *chrd = nullptr;   // the chr grid function itself (m_util_r8 _ex transfers)
+};
+
+struct pmg_solver {
+  int nlev = 0, op = 0, do_ex = 0;
+  double ex = 1.0;
+  double bbox[6];
+  Level lev[kMaxLev];
+  int pre = 2, pst = 2, max_citer = 5000, rop_u = 3, rop_res = 3, rop_src = 3, iord = 2;
+  double crtol = 1e-3, w0_r = 1.0, w0_i = 1.0;
+  hipStream_t stream = nullptr;
+  NormWs ws{nullptr, nullptr, 0};
+  double *d_norms = nullptr;  // kNormSlots doubles (FP64 cycle, outer test)
+  double *d_norms32 = nullptr;  // kNormSlots doubles (FP32 inner cycle)
+  int *d_citer = nullptr;     // 2 ints (FP64 cycle, FP32 cycle)
+  double *h_pinned = nullptr; // pinned readback
+  int sweep_variant = 1, zchunk = 32;
+  int use_graph = 1;
+  hipGraphExec_t g64 = nullptr;   // captured FP64 V-cycle
+  hipGraphExec_t g32 = nullptr;   // captured mixed-precision outer iteration
+  bool fp32_ready = false;
+  int last_citer = 0;
+};
+
+// norm slot layout in d_norms
+enum {
+  NS_RELAX = 0,                 // + L-1: last relax norm per level (FP64 cycle)
+  NS_RESID = kMaxLev,           // + L-1: residual norm per level
+  NS_FINAL = 2 * kMaxLev,       // final residual of the cycle
+  NS_CPRE = 2 * kMaxLev + 1,    // coarsest pre-sweep norm
+  NS_R0 = 2 * kMaxLev + 2,      // standalone finest residual
+  NS_OUTER = 2 * kMaxLev + 3,   // mixed: outer FP64 residual
+  kNormSlots = 2 * kMaxLev + 8
+};
+
+namespace {
+
+int alloc_level_arrays(pmg_solver *S, Level &Lv, bool f32) {
+  const size_t n = Lv.size;
+  if (!f32) {
+    auto &d = Lv.d;
+    for (double **p : {&d.u, &d.ua, &d.rho, &d.rhs, &d.res, &d.lop}) {
+      PMG_HOST_CHECK(hipMalloc(p, n * sizeof(double)));
+      PMG_HOST_CHECK(hipMemsetAsync(*p, 0, n * sizeof(double), S->stream));
+    }
+    if (S->op == 1) {
+      for (double **p : {&d.a, &d.c}) {
+        PMG_HOST_CHECK(hipMalloc(p, n * sizeof(double)));
+        PMG_HOST_CHECK(hipMemsetAsync(*p, 0, n * sizeof(double), S->stream));
+      }
+    }
+    if (S->do_ex) {
+      PMG_HOST_CHECK(hipMalloc(&Lv.chr, n));
+      PMG_HOST_CHECK(hipMemsetAsync(Lv.chr, 0, n, S->stream));
+      PMG_HOST_CHECK(hipMalloc(&Lv.chrd, n * sizeof(double)));
+    }
+  } else {
+    auto &f = Lv.f;
+    for (float **p : {&f.u, &f.ua, &f.rho, &f.rhs, &f.res, &f.lop}) {
+      PMG_HOST_CHECK(hipMalloc(p, n * sizeof(float)));
+      PMG_HOST_CHECK(hipMemsetAsync(*p, 0, n * sizeof(float), S->stream));
+    }
+    if (S->op == 1) {
+      for (float **p : {&f.a, &f.c}) {
+        PMG_HOST_CHECK(hipMalloc(p, n * sizeof(float)));
+      }
+    }
+  }
+  return 0;
+}
+
+template <typename T>
+void free_arrays(LevelArrays<T> &A) {
+  for (T **p : {&A.u, &A.ua, &A.rho, &A.rhs, &A.res, &A.lop, &A.a, &A.c}) {
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernel launchers (templated on precision)
+// ---------------------------------------------------------------------------
+template <typename T>
+LevelArrays<T> &arrays(Level &Lv);
+template <>
+LevelArrays<double> &arrays<double>(Level &Lv) { return Lv.d; }
+template <>
+LevelArrays<float> &arrays<float>(Level &Lv) { return Lv.f; }
+template <typename T>
+const Spacing<T> &spacing(const Level &Lv);
+template <>
+const Spacing<double> &spacing<double>(const Level &Lv) { return Lv.sp64; }
+template <>
+const Spacing<float> &spacing<float>(const Level &Lv) { return Lv.sp32; }
+
+// sweep tile variants (TX, TY, NT)
+#define PMG_SWEEP_VARIANTS(X) X(0, 64, 8, 256) X(1, 64, 16, 512) X(2, 32, 16, 512) X(3, 128, 8, 512)
+constexpr int kNumSweepVariants = 4;
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0>
+int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
+  int tx = 64, ty = 8;
+  switch (variant) {
+#define PMG_TXY(V, TXV, TYV, NTV) case V: tx = TXV; ty = TYV; break;
+    PMG_SWEEP_VARIANTS(PMG_TXY)
+#undef PMG_TXY
+    default: return -1;
+  }
+  dim3 grid((P.nx - 2 + tx - 1) / tx, (P.ny - 2 + ty - 1) / ty, (P.nz - 2 + P.zchunk - 1) / P.zchunk);
+  P.nb = (int)(grid.x * grid.y * grid.z);
+  if (P.nb > S->ws.capacity) return -3;
+  switch (variant) {
+#define PMG_LSW(V, TXV, TYV, NTV)                                                              \
+  case V:                                                                                      \
+    hipLaunchKernelGGL((k_sweep<T, OP, EX, P2, RHS0, TXV, TYV, NTV>), grid, dim3(NTV), 0,      \
+                       S->stream, P);                                                          \
+    break;
+    PMG_SWEEP_VARIANTS(PMG_LSW)
+#undef PMG_LSW
+  }
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int OP, bool EX>
+int launch_sweep(pmg_solver *S, SweepP<T> P, bool p2, bool rhs0) {
+  const int v = S->sweep_variant;
+  if (p2) return rhs0 ? launch_sweep_v<T, OP, EX, true, true>(S, v, P) : launch_sweep_v<T, OP, EX, true, false>(S, v, P);
+  return rhs0 ? launch_sweep_v<T, OP, EX, false, true>(S, v, P) : launch_sweep_v<T, OP, EX, false, false>(S, v, P);
+}
+
+// n red-black sweeps on level L (ping-pong u <-> ua; odd counts copy back)
+template <typename T>
+int sweeps(pmg_solver *S, int L, int n, double *norm_out, bool rhs0) {
+  Level &Lv = S->lev[L - 1];
+  LevelArrays<T> &A = arrays<T>(Lv);
+  if (n <= 0) return 0;
+  if (!Lv.rbox.ok) return 0;
+  SweepP<T> P;
+  P.rho = A.rho; P.rhs = A.rhs; P.a = A.a; P.c = A.c; P.chr = Lv.chr;
+  P.nx = Lv.n[0]; P.ny = Lv.n[1]; P.nz = Lv.n[2];
+  P.bx0 = Lv.rbox.lo[0]; P.bx1 = Lv.rbox.hi[0];
+  P.by0 = Lv.rbox.lo[1]; P.by1 = Lv.rbox.hi[1];
+  P.bz0 = Lv.rbox.lo[2]; P.bz1 = Lv.rbox.hi[2];
+  P.zchunk = std::max(1, std::min(S->zchunk, P.nz - 2));
+  P.sp = spacing<T>(Lv);
+  P.ws = S->ws;
+  P.norm_out = norm_out;
+  const bool p2 = P.sp.pow2 != 0;
+  for (int s = 0; s < n; s++) {
+    P.uin = (s % 2 == 0) ? A.u : A.ua;
+    P.uout = (s % 2 == 0) ? A.ua : A.u;
+    int rc;
+    if (S->op == 0) rc = S->do_ex ? launch_sweep<T, 0, true>(S, P, p2, rhs0) : launch_sweep<T, 0, false>(S, P, p2, rhs0);
+    else rc = S->do_ex ? launch_sweep<T, 1, true>(S, P, p2, rhs0) : launch_sweep<T, 1, false>(S, P, p2, rhs0);
+    if (rc) return rc;
+  }
+  if (n % 2 == 1) {
+    hipLaunchKernelGGL(k_copy<T>, dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, A.u, A.ua, Lv.size);
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+template <typename T, int OP, bool EX, bool P2>
+int launch_residual_t(pmg_solver *S, ResidP<T> P, bool rhs0, bool zero) {
+  const int segs = (P.nx + 255) / 256;
+  const int nb = segs * P.ny * P.nz;
+  P.nb = nb;
+  if (nb > S->ws.capacity) return -3;
+#define PMG_LR(R0, Z0, O32) hipLaunchKernelGGL((k_residual<T, OP, EX, P2, R0, Z0, O32, 256>), dim3(nb), dim3(256), 0, S->stream, P)
+  if (P.neg32) { PMG_LR(true, true, true); }
+  else if (rhs0) { if (zero) PMG_LR(true, true, false); else PMG_LR(true, false, false); }
+  else { if (zero) PMG_LR(false, true, false); else PMG_LR(false, false, false); }
+#undef PMG_LR
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int residual(pmg_solver *S, int L, double *norm_out, bool rhs0, bool zero, float *neg32 = nullptr) {
+  Level &Lv = S->lev[L - 1];
+  LevelArrays<T> &A = arrays<T>(Lv);
+  ResidP<T> P;
+  P.u = A.u; P.rho = A.rho; P.rhs = A.rhs; P.a = A.a; P.c = A.c; P.chr = Lv.chr;
+  P.res = A.res;
+  P.neg32 = neg32;
+  P.nx = Lv.n[0]; P.ny = Lv.n[1]; P.nz = Lv.n[2];
+  if (Lv.rbox.ok) {
+    P.bx0 = Lv.rbox.lo[0]; P.bx1 = Lv.rbox.hi[0];
+    P.by0 = Lv.rbox.lo[1]; P.by1 = Lv.rbox.hi[1];
+    P.bz0 = Lv.rbox.lo[2]; P.bz1 = Lv.rbox.hi[2];
+  } else {
+    P.bx0 = P.by0 = P.bz0 = 1;
+    P.bx1 = P.by1 = P.bz1 = 0;
+  }
+  P.sp = spacing<T>(Lv);
+  P.ws = S->ws;
+  P.norm_out = norm_out;
+  const bool p2 = P.sp.pow2 != 0;
+#define PMG_RS(OPV, EXV) \
+  (p2 ? launch_residual_t<T, OPV, EXV, true>(S, P, rhs0, zero) : launch_residual_t<T, OPV, EXV, false>(S, P, rhs0, zero))
+  if (S->op == 0) return S->do_ex ? PMG_RS(0, true) : PMG_RS(0, false);
+  return S->do_ex ? PMG_RS(1, true) : PMG_RS(1, false);
+#undef PMG_RS
+}
+
+// restriction of one field from level Lf to Lf-1 (whole grid)
+template <typename T>
+int restrict_field(pmg_solver *S, int Lfine, const T *src, T *dst, T *alt, int kind, bool use_ex) {
+  Level &F = S->lev[Lfine - 1], &Cl = S->lev[Lfine - 2];
+  if (kind == 0) return 0;
+  const size_t n = Cl.size;
+  (void)use_ex;
+#define PMG_RW(K) hipLaunchKernelGGL((k_restrict_whole<T, K, false>), dim3(nblk256(n)), dim3(256), 0, S->stream, src, dst, alt, (const T *)nullptr, T(0), F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2])
+  if (kind == 1) PMG_RW(1);
+  else if (kind == 2) PMG_RW(2);
+  else PMG_RW(3);
+#undef PMG_RW
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+namespace {
+__global__ void k_chr_bytes(uint8_t *dst, const double *chr, double ex, size_t n) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[q] = chr[q] == ex ? 1 : 0;
+}
+
+int restrict_field_ex64(pmg_solver *S, int Lfine, const double *src, double *dst, double *alt,
+                        int kind) {
+  Level &F = S->lev[Lfine - 1], &Cl = S->lev[Lfine - 2];
+  const size_t n = Cl.size;
+  const double *chrv = F.chrd;
+#define PMG_RWE(K) hipLaunchKernelGGL((k_restrict_whole<double, K, true>), dim3(nblk256(n)), dim3(256), 0, S->stream, src, dst, alt, chrv, S->ex, F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2])
+  if (kind == 1) PMG_RWE(1);
+  else if (kind == 2) PMG_RWE(2);
+  else PMG_RWE(3);
+#undef PMG_RWE
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int restrict_any(pmg_solver *S, int Lfine, const T *src, T *dst, T *alt, int kind) {
+  if (kind == 0) return 0;
+  if constexpr (std::is_same<T, double>::value) {
+    if (S->do_ex) return restrict_field_ex64(S, Lfine, src, dst, alt, kind);
+  }
+  return restrict_field<T>(S, Lfine, src, dst, alt, kind, false);
+}
+
+template <typename T>
+int lop_rhs(pmg_solver *S, int L) {
+  Level &Lv = S->lev[L - 1];
+  LevelArrays<T> &A = arrays<T>(Lv);
+  LopRhsP<T> P;
+  P.u = A.u; P.rho = A.rho; P.res = A.res; P.a = A.a; P.c = A.c; P.chr = Lv.chr;
+  P.rhs = A.rhs; P.lop = A.lop;
+  P.nx = Lv.n[0]; P.ny = Lv.n[1]; P.nz = Lv.n[2];
+  if (Lv.cbox.ok) {
+    P.cx0 = Lv.cbox.lo[0]; P.cx1 = Lv.cbox.hi[0];
+    P.cy0 = Lv.cbox.lo[1]; P.cy1 = Lv.cbox.hi[1];
+    P.cz0 = Lv.cbox.lo[2]; P.cz1 = Lv.cbox.hi[2];
+  } else {
+    P.cx0 = P.cy0 = P.cz0 = 1;
+    P.cx1 = P.cy1 = P.cz1 = 0;
+  }
+  P.w0_r = (T)S->w0_r;
+  P.sp = spacing<T>(Lv);
+  const bool p2 = P.sp.pow2 != 0;
+  const unsigned nb = nblk256(Lv.size);
+#define PMG_LO(OPV, EXV, P2V) hipLaunchKernelGGL((k_lop_rhs<T, OPV, EXV, P2V>), dim3(nb), dim3(256), 0, S->stream, P)
+  if (S->op == 0) {
+    if (S->do_ex) { if (p2) PMG_LO(0, true, true); else PMG_LO(0, true, false); }
+    else { if (p2) PMG_LO(0, false, true); else PMG_LO(0, false, false); }
+  } else {
+    if (S->do_ex) { if (p2) PMG_LO(1, true, true); else PMG_LO(1, true, false); }
+    else { if (p2) PMG_LO(1, false, true); else PMG_LO(1, false, false); }
+  }
+#undef PMG_LO
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, int ny2,
+                              int nz2, int i1, int j1, int k1, int i2, int j2, int k2, int ni,
+                              int nj, int nk, int si1, int sj1, int sk1, int irho, int jrho,
+                              int krho, int ord, const double *chr1, double ex, int do_ex,
+                              hipStream_t stream);
+
+template <typename T>
+int cgc(pmg_solver *S, int L) {
+  Level &F = S->lev[L - 1], &Cl = S->lev[L - 2];
+  LevelArrays<T> &AF = arrays<T>(F);
+  LevelArrays<T> &AC = arrays<T>(Cl);
+  if (S->iord != 2) return -4;
+  bool ex = false;
+  if constexpr (std::is_same<T, double>::value) ex = S->do_ex != 0;
+  if (!ex) {
+    hipLaunchKernelGGL(k_cgc<T>, dim3(nblk256(F.size)), dim3(256), 0, S->stream, AF.u, AF.ua, AC.u,
+                       AC.lop, (T)S->w0_i, F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1],
+                       (const uint8_t *)F.chr);
+    PMG_HOST_CHECK(hipGetLastError());
+    return 0;
+  }
+  if constexpr (std::is_same<T, double>::value) {
+    // excision: coarse res, dminterp3d_ex into the fine res, clear_ex, add
+    hipLaunchKernelGGL(k_cgc_diff<double>, dim3(nblk256(Cl.size)), dim3(256), 0, S->stream, AC.res,
+                       AC.u, AC.lop, S->w0_i, Cl.size);
+    PMG_HOST_CHECK(hipGetLastError());
+    int rc = pmg_dminterp3d(AC.res, AF.res, Cl.n[0], Cl.n[1], Cl.n[2], F.n[0], F.n[1], F.n[2], 1, 1, 1,
+                            1, 1, 1, Cl.n[0], Cl.n[1], Cl.n[2], 1, 1, 1, 2, 2, 2, 2,
+                            Cl.chrd, S->ex, 1, S->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_clear_ex<double>, dim3(nblk256(Cl.size)), dim3(256), 0, S->stream, AC.res,
+                       Cl.chr, Cl.size);
+    hipLaunchKernelGGL(k_cgc_add_ex<double>, dim3(nblk256(F.size)), dim3(256), 0, S->stream, AF.u,
+                       AF.ua, AF.res, F.chr, F.n[0], F.n[1], F.n[2]);
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+template <typename T>
+int coarse_solve(pmg_solver *S, int L, double *norm_pre, double *norm_post, int *citer,
+                 int max_citer) {
+  Level &Lv = S->lev[L - 1];
+  LevelArrays<T> &A = arrays<T>(Lv);
+  CoarseP<T> P;
+  P.u = A.u; P.rho = A.rho; P.rhs = A.rhs; P.a = A.a; P.c = A.c; P.chr = Lv.chr;
+  P.nx = Lv.n[0]; P.ny = Lv.n[1]; P.nz = Lv.n[2];
+  if (Lv.rbox.ok) {
+    P.bx0 = Lv.rbox.lo[0]; P.bx1 = Lv.rbox.hi[0];
+    P.by0 = Lv.rbox.lo[1]; P.by1 = Lv.rbox.hi[1];
+    P.bz0 = Lv.rbox.lo[2]; P.bz1 = Lv.rbox.hi[2];
+  } else {
+    P.bx0 = P.by0 = P.bz0 = 1;
+    P.bx1 = P.by1 = P.bz1 = 0;
+  }
+  P.sp = spacing<T>(Lv);
+  P.npre = S->pre; P.npst = S->pst; P.max_citer = max_citer; P.crtol = S->crtol;
+  P.norm_pre = norm_pre; P.norm_post = norm_post; P.citer = citer;
+  const int narr = (S->op == 1) ? 5 : 3;
+  const size_t lds = Lv.size * narr * sizeof(T);
+  const bool use_lds = lds <= 60 * 1024;
+  const bool p2 = P.sp.pow2 != 0;
+  constexpr int NT = 256;
+#define PMG_CS(OPV, EXV, P2V, LDSV) hipLaunchKernelGGL((k_coarse_solve<T, OPV, EXV, P2V, LDSV, NT>), dim3(1), dim3(NT), LDSV ? lds : 0, S->stream, P)
+#define PMG_CS2(OPV, EXV) \
+  if (p2) { if (use_lds) PMG_CS(OPV, EXV, true, true); else PMG_CS(OPV, EXV, true, false); } \
+  else { if (use_lds) PMG_CS(OPV, EXV, false, true); else PMG_CS(OPV, EXV, false, false); }
+  if (S->op == 0) { if (S->do_ex) { PMG_CS2(0, true) } else { PMG_CS2(0, false) } }
+  else { if (S->do_ex) { PMG_CS2(1, true) } else { PMG_CS2(1, false) } }
+#undef PMG_CS2
+#undef PMG_CS
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// one FAS V-cycle, enqueued (amrd/mg.c:84-201).  T = double is the
+// reference method.  T = float is the inner cycle of the mixed-precision
+// solver: its finest rho is the (negated) FP64 defect, rhs on the finest
+// level is zero and coarse-level rho are zero (for a linear operator the
+// coarse rho cancels between the FAS rhs and the coarse relaxation).
+// ---------------------------------------------------------------------------
+template <typename T>
+int enqueue_vcycle(pmg_solver *S, bool final_residual, double *norm_base, int *citer) {
+  const int Lf = S->nlev;
+  int rc;
+  for (int L = Lf; L >= 1; L--) {
+    if (L < Lf) {
+      LevelArrays<T> &AF = arrays<T>(S->lev[L]);
+      LevelArrays<T> &AC = arrays<T>(S->lev[L - 1]);
+      // residual on L+1; the finest level was cleared at the cycle start
+      if ((rc = residual<T>(S, L + 1, norm_base + NS_RESID + L, L + 1 == Lf, L + 1 == Lf))) return rc;
+      // PAMR_inject(L+1): u and u_res (sources are constant, restricted once)
+      if ((rc = restrict_any<T>(S, L + 1, AF.u, AC.u, AC.ua, S->rop_u))) return rc;
+      if ((rc = restrict_any<T>(S, L + 1, AF.res, AC.res, (T *)nullptr, S->rop_res))) return rc;
+      if ((rc = lop_rhs<T>(S, L))) return rc;
+    }
+    if (L == 1) {
+      if ((rc = coarse_solve<T>(S, 1, norm_base + NS_CPRE, norm_base + NS_RELAX + 0, citer, S->max_citer))) return rc;
+    } else {
+      if ((rc = sweeps<T>(S, L, S->pre, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
+    }
+  }
+  for (int L = 2; L <= Lf; L++) {
+    if ((rc = cgc<T>(S, L))) return rc;
+    if ((rc = sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
+  }
+  if (final_residual) {
+    if ((rc = residual<T>(S, Lf, norm_base + NS_FINAL, true, true))) return rc;
+  }
+  return 0;
+}
+
+int capture_or_run(pmg_solver *S, hipGraphExec_t *cache, int (*enq)(pmg_solver *)) {
+  if (!S->use_graph) return enq(S);
+  if (!*cache) {
+    hipGraph_t g;
+    PMG_HOST_CHECK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+    int rc = enq(S);
+    hipGraph_t g2 = nullptr;
+    hipError_t e = hipStreamEndCapture(S->stream, &g2);
+    if (rc) { if (g2) (void)hipGraphDestroy(g2); return rc; }
+    if (e != hipSuccess) { std::fprintf(stderr, "pamr_mg: graph capture failed: %s\n", hipGetErrorString(e)); return -1; }
+    g = g2;
+    PMG_HOST_CHECK(hipGraphInstantiate(cache, g, nullptr, nullptr, 0));
+    PMG_HOST_CHECK(hipGraphDestroy(g));
+  }
+  PMG_HOST_CHECK(hipGraphLaunch(*cache, S->stream));
+  return 0;
+}
+
+int enq_v64(pmg_solver *S) { return enqueue_vcycle<double>(S, true, S->d_norms, S->d_citer); }
+
+// Mixed precision outer iteration (defect correction).  On entry the FP32
+// source of the finest level holds -r (r = L64[u], the FP64 defect) and the
+// FP32 correction e is zero; the graph runs the FP32 V-cycle on A e = -r,
+// adds e to u in FP64 (zeroing e again), and recomputes the FP64 defect of
+// the updated u -- its norm is the convergence test, its negation in FP32
+// the next source.
+__global__ __launch_bounds__(256) void k_add_f32(double *u, float *e, size_t n) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) {
+    u[q] = u[q] + (double)e[q];
+    e[q] = 0.0f;
+  }
+}
+
+int enq_mixed(pmg_solver *S) {
+  const int Lf = S->nlev;
+  Level &F = S->lev[Lf - 1];
+  int rc;
+  if ((rc = enqueue_vcycle<float>(S, false, S->d_norms32, S->d_citer + 1))) return rc;
+  hipLaunchKernelGGL(k_add_f32, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.u, F.f.u, F.size);
+  PMG_HOST_CHECK(hipGetLastError());
+  return residual<double>(S, Lf, S->d_norms + NS_OUTER, true, true, F.f.rho);
+}
+
+int ensure_fp32(pmg_solver *S) {
+  if (S->fp32_ready) return 0;
+  for (int L = 1; L <= S->nlev; L++)
+    if (alloc_level_arrays(S, S->lev[L - 1], true)) return -1;
+  // coefficients in FP32 on every level (restricted FP64 values rounded)
+  for (int L = 1; L <= S->nlev; L++) {
+    Level &Lv = S->lev[L - 1];
+    if (S->op == 1) {
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, Lv.f.a, Lv.d.a, Lv.size);
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, Lv.f.c, Lv.d.c, Lv.size);
+    }
+  }
+  PMG_HOST_CHECK(hipGetLastError());
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  S->fp32_ready = true;
+  return 0;
+}
+
+void invalidate_graphs(pmg_solver *S) {
+  if (S->g64) { (void)hipGraphExecDestroy(S->g64); S->g64 = nullptr; }
+  if (S->g32) { (void)hipGraphExecDestroy(S->g32); S->g32 = nullptr; }
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidth, int op,
+                       int do_ex, double ex) {
+  if (nx < 3 || ny < 3 || nz < 3 || !bbox || (op != 0 && op != 1)) return nullptr;
+  pmg_solver *S = new pmg_solver();
+  S->op = op;
+  S->do_ex = do_ex ? 1 : 0;
+  S->ex = ex;
+  std::memcpy(S->bbox, bbox, 6 * sizeof(double));
+  if (hipStreamCreate(&S->stream) != hipSuccess) { delete S; return nullptr; }
+  // build_mgh coarsening (src/gh.c:1275-1340) via calc_island_no rules
+  int shapes[kMaxLev][3];
+  int nl = 1;
+  shapes[0][0] = nx; shapes[0][1] = ny; shapes[0][2] = nz;
+  double dx0[3];
+  for (int d = 0; d < 3; d++) dx0[d] = (bbox[2 * d + 1] - bbox[2 * d]) / (shapes[0][d] - 1);
+  for (int cf = 2; nl < kMaxLev; cf *= 2) {
+    int island = 1;
+    for (int d = 0; d < 3; d++) {
+      int sh = (int)((bbox[2 * d + 1] - bbox[2 * d]) / dx0[d] + 1.5);
+      if ((((sh - 1) % cf) && ((sh - 1) / cf + 1) >= min_cwidth) || (((sh - 1) / cf + 1) < min_cwidth))
+        island = 0;
+      else if (((sh - 1) / cf / 2 + 1) < min_cwidth && island != 0)
+        island = -1;
+    }
+    if (island == 0) break;
+    for (int d = 0; d < 3; d++) shapes[nl][d] = (int)((bbox[2 * d + 1] - bbox[2 * d]) / (dx0[d] * cf) + 1.5);
+    nl++;
+  }
+  S->nlev = nl;
+  for (int L = 1; L <= nl; L++) {
+    Level &Lv = S->lev[L - 1];
+    for (int d = 0; d < 3; d++) Lv.n[d] = shapes[nl - L][d];
+    Lv.size = (size_t)Lv.n[0] * Lv.n[1] * Lv.n[2];
+    for (int d = 0; d < 3; d++) {
+      // coordinates by repeated addition (src/gh.c:905-906)
+      double dx = (bbox[2 * d + 1] - bbox[2 * d]) / (Lv.n[d] - 1);
+      double x0 = bbox[2 * d], x1 = x0 + dx, x2 = x1 + dx;
+      Lv.hdx[d] = x1 - x0;
+      Lv.gdx[d] = x2 - x1;
+    }
+    Lv.sp64 = make_spacing<double>(Lv.hdx);
+    Lv.sp32 = make_spacing<float>(Lv.hdx);
+    Box b = cmask_bdy_box(Lv.n, bbox, bbox, Lv.gdx);
+    // relax/residual act on points 2..N-1 (1-based) where cmask is ON
+    for (int d = 0; d < 3; d++) {
+      b.lo[d] = std::max(b.lo[d], 1);
+      b.hi[d] = std::min(b.hi[d], Lv.n[d] - 2);
+      if (b.lo[d] > b.hi[d]) b.ok = false;
+    }
+    Lv.rbox = b;
+    Lv.cbox = cmask_child_box(Lv.n, bbox, bbox, Lv.gdx, bbox);
+    if (alloc_level_arrays(S, Lv, false)) { pmg_destroy(S); return nullptr; }
+  }
+  // norm workspace sized for the largest launch
+  const Level &F = S->lev[nl - 1];
+  int cap = (int)(((size_t)(F.n[0] + 255) / 256) * F.n[1] * F.n[2]) + 64;
+  cap = std::max(cap, 1 << 16);
+  S->ws.capacity = cap;
+  if (hipMalloc(&S->ws.partials, sizeof(double) * 2 * cap) != hipSuccess ||
+      hipMalloc(&S->ws.ticket, sizeof(unsigned)) != hipSuccess ||
+      hipMalloc(&S->d_norms, sizeof(double) * kNormSlots) != hipSuccess ||
+      hipMalloc(&S->d_norms32, sizeof(double) * kNormSlots) != hipSuccess ||
+      hipMalloc(&S->d_citer, sizeof(int) * 2) != hipSuccess ||
+      hipHostMalloc(&S->h_pinned, sizeof(double) * (kNormSlots + 4)) != hipSuccess) {
+    pmg_destroy(S);
+    return nullptr;
+  }
+  (void)hipMemsetAsync(S->ws.ticket, 0, sizeof(unsigned), S->stream);
+  (void)hipMemsetAsync(S->d_norms, 0, sizeof(double) * kNormSlots, S->stream);
+  (void)hipMemsetAsync(S->d_norms32, 0, sizeof(double) * kNormSlots, S->stream);
+  (void)hipMemsetAsync(S->d_citer, 0, sizeof(int) * 2, S->stream);
+  if (S->do_ex) {
+    // default chr: nothing excised (every value ex + 1)
+    for (int L = 1; L <= nl; L++) {
+      std::vector<double> h(S->lev[L - 1].size, ex + 1.0);
+      if (hipMemcpy(S->lev[L - 1].chrd, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+        pmg_destroy(S);
+        return nullptr;
+      }
+    }
+  }
+  if (hipStreamSynchronize(S->stream) != hipSuccess) { pmg_destroy(S); return nullptr; }
+  return S;
+}
+
+void pmg_destroy(pmg_solver *S) {
+  if (!S) return;
+  if (S->stream) (void)hipStreamSynchronize(S->stream);
+  invalidate_graphs(S);
+  for (int L = 0; L < S->nlev; L++) {
+    free_arrays(S->lev[L].d);
+    free_arrays(S->lev[L].f);
+    if (S->lev[L].chr) (void)hipFree(S->lev[L].chr);
+    if (S->lev[L].chrd) (void)hipFree(S->lev[L].chrd);
+  }
+  if (S->ws.partials) (void)hipFree(S->ws.partials);
+  if (S->ws.ticket) (void)hipFree(S->ws.ticket);
+  if (S->d_norms) (void)hipFree(S->d_norms);
+  if (S->d_norms32) (void)hipFree(S->d_norms32);
+  if (S->d_citer) (void)hipFree(S->d_citer);
+  if (S->h_pinned) (void)hipHostFree(S->h_pinned);
+  if (S->stream) (void)hipStreamDestroy(S->stream);
+  delete S;
+}
+
+int pmg_nlev(const pmg_solver *S) { return S ? S->nlev : 0; }
+
+int pmg_level_shape(const pmg_solver *S, int L, int *shape) {
+  if (!S || L < 1 || L > S->nlev) return -1;
+  for (int d = 0; d < 3; d++) shape[d] = S->lev[L - 1].n[d];
+  return 0;
+}
+
+int pmg_set_params(pmg_solver *S, int pre, int pst, int max_citer, double crtol, double w0_r,
+                   double w0_i, int rop_u, int rop_res, int rop_src, int interp_ord) {
+  if (!S || pre < 0 || pst < 0 || max_citer < 0 || interp_ord != 2) return -1;
+  if (rop_u < 0 || rop_u > 3 || rop_res < 0 || rop_res > 3 || rop_src < 0 || rop_src > 3) return -1;
+  S->pre = pre; S->pst = pst; S->max_citer = max_citer; S->crtol = crtol;
+  S->w0_r = w0_r; S->w0_i = w0_i; S->rop_u = rop_u; S->rop_res = rop_res; S->rop_src = rop_src;
+  S->iord = interp_ord;
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk) {
+  if (!S || variant < 0 || variant >= kNumSweepVariants || zchunk < 1) return -1;
+  S->sweep_variant = variant;
+  S->zchunk = zchunk;
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_num_sweep_variants(void) { return kNumSweepVariants; }
+
+int pmg_set_graph(pmg_solver *S, int on) {
+  if (!S) return -1;
+  S->use_graph = on ? 1 : 0;
+  invalidate_graphs(S);
+  return 0;
+}
+
+void *pmg_stream(pmg_solver *S) { return S ? (void *)S->stream : nullptr; }
+
+double *pmg_field(pmg_solver *S, int L, int which) {
+  if (!S || L < 1 || L > S->nlev) return nullptr;
+  LevelArrays<double> &d = S->lev[L - 1].d;
+  switch (which) {
+    case PMG_F_U: return d.u;
+    case PMG_F_RHO: return d.rho;
+    case PMG_F_RHS: return d.rhs;
+    case PMG_F_RES: return d.res;
+    case PMG_F_LOP: return d.lop;
+    case PMG_F_A: return d.a;
+    case PMG_F_C: return d.c;
+    default: return nullptr;
+  }
+}
+
+int pmg_upload(pmg_solver *S, int L, int which, const double *host) {
+  double *d = pmg_field(S, L, which);
+  if (!d || !host) return -1;
+  const size_t n = S->lev[L - 1].size;
+  PMG_HOST_CHECK(hipMemcpyAsync(d, host, n * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  if (which == PMG_F_U)
+    PMG_HOST_CHECK(hipMemcpyAsync(S->lev[L - 1].d.ua, host, n * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
+}
+
+int pmg_download(pmg_solver *S, int L, int which, double *host) {
+  double *d = pmg_field(S, L, which);
+  if (!d || !host) return -1;
+  PMG_HOST_CHECK(hipMemcpyAsync(host, d, S->lev[L - 1].size * sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
+}
+
+int pmg_set_chr(pmg_solver *S, int L, const double *chr_host) {
+  if (!S || !S->do_ex || L < 1 || L > S->nlev || !chr_host) return -1;
+  Level &Lv = S->lev[L - 1];
+  PMG_HOST_CHECK(hipMemcpyAsync(Lv.chrd, chr_host, Lv.size * sizeof(double), hipMemcpyHostToDevice, S->stream));
+  hipLaunchKernelGGL(k_chr_bytes, dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, Lv.chr, Lv.chrd,
+                     S->ex, Lv.size);
+  PMG_HOST_CHECK(hipGetLastError());
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
+}
+
+int pmg_download_cmask(pmg_solver *S, int L, int kind, double *host) {
+  // materialise the cmask the reference would hold: kind 0 = set_cmask_bdy
+  // (during sweeps), 1 = set_cmask_child (during L_op / compute_MG_rhs)
+  if (!S || L < 1 || L > S->nlev || !host) return -1;
+  Level &Lv = S->lev[L - 1];
+  Box b = kind == 0 ? cmask_bdy_box(Lv.n, S->bbox, S->bbox, Lv.gdx)
+                    : cmask_child_box(Lv.n, S->bbox, S->bbox, Lv.gdx, S->bbox);
+  for (size_t q = 0; q < Lv.size; q++) host[q] = 0.0;
+  if (!b.ok) return 0;
+  for (int k = b.lo[2]; k <= b.hi[2]; k++)
+    for (int j = b.lo[1]; j <= b.hi[1]; j++)
+      for (int i = b.lo[0]; i <= b.hi[0]; i++) host[ix3(i, j, k, Lv.n[0], Lv.n[1])] = 1.0;
+  return 0;
+}
+
+int pmg_init(pmg_solver *S) {
+  // PAMR_build_mgh's TF_MGH_INIT injection (src/gh.c:1237-1239,1334): u and
+  // the in-AMRH sources/coefficients restricted level by level; the MG
+  // variables res/rhs/lop start at zero
+  if (!S) return -1;
+  int rc;
+  for (int L = S->nlev; L > 1; L--) {
+    Level &F = S->lev[L - 1], &Cl = S->lev[L - 2];
+    if ((rc = restrict_any<double>(S, L, F.d.u, Cl.d.u, nullptr, S->rop_u))) return rc;
+    if ((rc = restrict_any<double>(S, L, F.d.rho, Cl.d.rho, nullptr, S->rop_src))) return rc;
+    if (S->op == 1) {
+      if ((rc = restrict_any<double>(S, L, F.d.a, Cl.d.a, nullptr, S->rop_src))) return rc;
+      if ((rc = restrict_any<double>(S, L, F.d.c, Cl.d.c, nullptr, S->rop_src))) return rc;
+    }
+  }
+  for (int L = 1; L <= S->nlev; L++) {
+    Level &Lv = S->lev[L - 1];
+    hipLaunchKernelGGL(k_copy<double>, dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, Lv.d.ua, Lv.d.u, Lv.size);
+    PMG_HOST_CHECK(hipMemsetAsync(Lv.d.rhs, 0, Lv.size * sizeof(double), S->stream));
+    PMG_HOST_CHECK(hipMemsetAsync(Lv.d.res, 0, Lv.size * sizeof(double), S->stream));
+    PMG_HOST_CHECK(hipMemsetAsync(Lv.d.lop, 0, Lv.size * sizeof(double), S->stream));
+  }
+  if (S->op == 1 && S->fp32_ready) {
+    for (int L = 1; L <= S->nlev; L++) {
+      Level &Lv = S->lev[L - 1];
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, Lv.f.a, Lv.d.a, Lv.size);
+      hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, Lv.f.c, Lv.d.c, Lv.size);
+    }
+  }
+  PMG_HOST_CHECK(hipGetLastError());
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
+}
+
+int pmg_vcycle_async(pmg_solver *S) {
+  if (!S) return -1;
+  return capture_or_run(S, &S->g64, enq_v64);
+}
+
+int pmg_read_norms(pmg_solver *S, double *res_out, double *lev_norms, int *citer) {
+  if (!S) return -1;
+  PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms, sizeof(double) * kNormSlots, hipMemcpyDeviceToHost, S->stream));
+  int ci[2];
+  PMG_HOST_CHECK(hipMemcpyAsync(ci, S->d_citer, sizeof(int) * 2, hipMemcpyDeviceToHost, S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  if (res_out) *res_out = S->h_pinned[NS_FINAL];
+  if (lev_norms) {
+    for (int L = 1; L <= S->nlev; L++) lev_norms[L - 1] = S->h_pinned[NS_RELAX + L - 1];
+    lev_norms[S->nlev - 1] = S->h_pinned[NS_FINAL];
+  }
+  if (citer) *citer = ci[0];
+  return 0;
+}
+
+int pmg_vcycle(pmg_solver *S, double *res_out, double *lev_norms, int *citer) {
+  int rc = pmg_vcycle_async(S);
+  if (rc) return rc;
+  return pmg_read_norms(S, res_out, lev_norms, citer);
+}
+
+int pmg_residual(pmg_solver *S, double *norm_out) {
+  if (!S) return -1;
+  int rc = residual<double>(S, S->nlev, S->d_norms + NS_R0, true, true);
+  if (rc) return rc;
+  PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_R0, sizeof(double), hipMemcpyDeviceToHost, S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  if (norm_out) *norm_out = S->h_pinned[0];
+  return 0;
+}
+
+int pmg_solve(pmg_solver *S, int min_iter, int max_iter, double tol, double *norms, int *iters) {
+  // solve_elliptics loop (amrd/mg.c:56-72): absolute RMS tolerance
+  if (!S) return -1;
+  int it = 0;
+  double res = tol + 1;
+  while ((it < min_iter || res > tol) && it < max_iter) {
+    int rc = pmg_vcycle(S, &res, nullptr, nullptr);
+    if (rc) return rc;
+    if (norms) norms[it] = res;
+    it++;
+  }
+  if (iters) *iters = it;
+  return 0;
+}
+
+int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hist, int *iters,
+                  double *r0_out) {
+  // Relative-residual solve: ||r_k|| / ||r_0|| <= rtol with ||.|| the RMS
+  // norm of the residual hook and r_0 the residual of the initial u.
+  // mode 0: plain FP64 V-cycles; mode 1: FP32 inner V-cycles in FP64 defect
+  // correction (the FP64 residual, the update and the test stay FP64).
+  if (!S || max_iter < 0) return -1;
+  double r0;
+  int rc = pmg_residual(S, &r0);
+  if (rc) return rc;
+  if (r0_out) *r0_out = r0;
+  int it = 0;
+  double r = r0;
+  if (mode == 0) {
+    while (it < max_iter && r > rtol * r0) {
+      if ((rc = pmg_vcycle(S, &r, nullptr, nullptr))) return rc;
+      if (hist) hist[it] = r;
+      it++;
+    }
+  } else if (mode == 1) {
+    if ((rc = ensure_fp32(S))) return rc;
+    // initial FP32 source -r0 (the residual above already gave ||r0||)
+    Level &F = S->lev[S->nlev - 1];
+    if ((rc = residual<double>(S, S->nlev, S->d_norms + NS_OUTER, true, true, F.f.rho))) return rc;
+    while (it < max_iter && r > rtol * r0) {
+      if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
+      PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost, S->stream));
+      PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+      r = S->h_pinned[0];
+      if (hist) hist[it] = r;
+      it++;
+    }
+  } else {
+    return -1;
+  }
+  if (iters) *iters = it;
+  return 0;
+}
+
+int pmg_last_citer(pmg_solver *S, int which) {
+  if (!S) return -1;
+  int ci[2] = {0, 0};
+  if (hipMemcpy(ci, S->d_citer, sizeof(int) * 2, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+  return ci[which ? 1 : 0];
+}
+
+}  // extern "C"
