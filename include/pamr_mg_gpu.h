@@ -136,8 +136,15 @@ int pmg_solve(pmg_solver *S, int min_iter, int max_iter, double tol, double *nor
 int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hist, int *iters,
                   double *r0_out);
 int pmg_last_citer(pmg_solver *S, int which);
+/* initial guess u = 0 on the finest level (device-side) */
+int pmg_zero_solution(pmg_solver *S);
+/* average duration (ms) of one launch of a solver kernel on level L, timed
+ * with HIP events on the solver's stream: which 0 = red-black sweep,
+ * 1 = residual; prec 0 = FP64, 1 = FP32 */
+int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double *ms_avg);
 /* execution tuning: sweep tile variant and z chunk; graph capture on/off */
 int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk);
+int pmg_set_level_sweep_config(pmg_solver *S, int L, int variant, int zchunk);
 int pmg_num_sweep_variants(void);
 int pmg_set_graph(pmg_solver *S, int on);
 void *pmg_stream(pmg_solver *S);

@@ -73,7 +73,10 @@ def _declare(L):
     L.pmg_solve.argtypes = [_vp, i, i, d, _dp, _ip]
     L.pmg_solve_rel.argtypes = [_vp, i, d, i, _dp, _ip, _dp]
     L.pmg_last_citer.argtypes = [_vp, i]
+    L.pmg_zero_solution.argtypes = [_vp]
+    L.pmg_time_kernel.argtypes = [_vp, i, i, i, i, _dp]
     L.pmg_set_sweep_config.argtypes = [_vp, i, i]
+    L.pmg_set_level_sweep_config.argtypes = [_vp, i, i, i]
     L.pmg_num_sweep_variants.argtypes = []
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_stream.argtypes = [_vp]
@@ -108,6 +111,19 @@ class Solver:
         if not self.h:
             raise RuntimeError("pmg_create failed")
         self.nlev = L.pmg_nlev(self.h)
+        self.apply_tuning()
+
+    def apply_tuning(self, prec="f64"):
+        """Per-level sweep tile/z-chunk from the autotuner table (tuning.json)."""
+        from .autotune import tuned_level_config
+        for lev in range(1, self.nlev + 1):
+            cfg = tuned_level_config(self.shape(lev)[2], prec)
+            if cfg is not None:
+                self.set_level_sweep_config(lev, *cfg)
+
+    def set_level_sweep_config(self, lev, variant, zchunk):
+        _check(self._L.pmg_set_level_sweep_config(self.h, lev, variant, zchunk),
+               "pmg_set_level_sweep_config")
 
     def close(self):
         if getattr(self, "h", None):
@@ -178,6 +194,14 @@ class Solver:
                                      C.byref(r0)), "pmg_solve_rel")
         return r0.value, hist[: it.value]
 
+    def zero_solution(self):
+        _check(self._L.pmg_zero_solution(self.h), "pmg_zero_solution")
+
+    def time_kernel(self, which, lev, prec, nrep):
+        ms = C.c_double(0)
+        _check(self._L.pmg_time_kernel(self.h, which, lev, prec, nrep, C.byref(ms)), "pmg_time_kernel")
+        return ms.value
+
     def set_sweep_config(self, variant, zchunk):
         _check(self._L.pmg_set_sweep_config(self.h, variant, zchunk), "pmg_set_sweep_config")
 
@@ -238,3 +262,6 @@ def gaussian_source(n, seed=1234, m=8, params=None, bbox=(0.0, 1.0, 0.0, 1.0, 0.
     f[:, 0, :] = f[:, -1, :] = 0
     f[:, :, 0] = f[:, :, -1] = 0
     return f
+
+
+from .autotune import tuned_sweep_config  # noqa: E402,F401

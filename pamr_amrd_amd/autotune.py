@@ -1,0 +1,88 @@
+"""Tile-shape autotuner for the fused red-black sweep.
+
+For a grid size it times the finest-level sweep (HIP events on the solver's
+stream, pmg_time_kernel) for every compiled tile variant (TX x TY x threads)
+and z-chunk length, in FP64 and FP32, and stores the fastest configuration
+in tuning.json next to this file.  bench.py and the solvers read the table
+through tuned_sweep_config(); sizes that were never tuned get the default.
+"""
+from __future__ import annotations
+
+import json
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, "tuning.json")
+DEFAULT = (1, 32)
+VARIANTS = {0: (64, 8, 256), 1: (64, 16, 512), 2: (32, 16, 512), 3: (128, 8, 512)}
+
+
+def load_table():
+    if not os.path.exists(TABLE):
+        return {}
+    with open(TABLE) as f:
+        return json.load(f)
+
+
+def tuned_level_config(n, prec="f64"):
+    """(variant, zchunk) tuned for a level of n^3 points, None when untuned."""
+    t = load_table().get(str(n), {}).get(prec)
+    if not t:
+        return None
+    return int(t["variant"]), int(t["zchunk"])
+
+
+def tuned_sweep_config(n, prec="f64"):
+    return tuned_level_config(n, prec) or DEFAULT
+
+
+def autotune(n, zchunks=(2, 4, 8, 16, 32, 64), nrep=20, write=True, verbose=True):
+    import numpy as np
+
+    import pamr_amrd_amd as pm
+    S = pm.Solver(n)
+    rho = -pm.gaussian_source(n, seed=1234) if n <= 129 else np.zeros((n, n, n))
+    S.upload(S.nlev, pm.PMG_F_RHO, rho)
+    S.init()
+    results = {}
+    for prec, name in ((0, "f64"), (1, "f32")):
+        best = None
+        rows = []
+        for v in range(pm.lib().pmg_num_sweep_variants()):
+            for z in zchunks:
+                if z > n - 2:
+                    continue
+                try:
+                    S.set_level_sweep_config(S.nlev, v, z)
+                    S.time_kernel(0, S.nlev, prec, 4)  # warm
+                    ms = S.time_kernel(0, S.nlev, prec, nrep)
+                except RuntimeError:
+                    continue
+                rows.append((v, z, ms))
+                if verbose:
+                    print(f"autotune n={n} {name} variant={v} {VARIANTS[v]} zchunk={z}: {ms:.4f} ms",
+                          flush=True)
+                if best is None or ms < best[2]:
+                    best = (v, z, ms)
+        results[name] = {"variant": best[0], "zchunk": best[1], "ms": best[2],
+                         "all": [list(r) for r in rows]}
+    S.close()
+    if write:
+        tab = load_table()
+        tab[str(n)] = results
+        with open(TABLE, "w") as f:
+            json.dump(tab, f, indent=1)
+    return results
+
+
+if __name__ == "__main__":
+    import argparse
+    import shutil
+    ap = argparse.ArgumentParser()
+    ap.add_argument("sizes", nargs="*", type=int, default=[257])
+    ap.add_argument("--copy-to", default=None, help="also copy tuning.json to this path")
+    a = ap.parse_args()
+    for n in a.sizes:
+        autotune(n)
+    if a.copy_to:
+        shutil.copy(TABLE, a.copy_to)

@@ -65,46 +65,31 @@ __device__ inline void block_reduce2(double &s, double &c) {
   __syncthreads();
 }
 
-// Norm epilogue shared by every norm-producing kernel: each block stores its
-// (sum res^2, count) partial; the last block to finish (ticket counter)
-// folds the partials in block order and writes sqrt(sum/count) -- the RMS
-// norm of the reference hooks (w/examples/nps/num_w.f:88,154).  The ticket
-// is reset by that last block, so the same workspace serves the next launch
-// and graph replays without a memset.
+// Norms of the reference hooks are RMS values sqrt(sum res^2 / count)
+// (w/examples/nps/num_w.f:88,154).  A norm-producing kernel stores one
+// (sum, count) partial per block in its slot of the workspace -- plain
+// stores, no atomics and no cross-block fences -- and k_norm_fold, launched
+// afterwards on the same stream, folds the partials of one or more slots in
+// block order.  The result is deterministic for a fixed launch shape.
 struct NormWs {
-  double *partials;  // 2 * capacity doubles
-  unsigned *ticket;
-  int capacity;
+  double *partials;  // [nslots][cap][2]
+  int *slot_nb;      // blocks that wrote each slot in its last launch
+  int cap;           // max blocks per launch
+  int nslots;
 };
 
 template <int NT>
-__device__ inline void norm_epilogue(double s, double c, NormWs ws, double *out, int slot,
-                                     int nblocks_launch, int nfold, bool final_launch) {
+__device__ inline void norm_partial(double s, double c, NormWs ws, int slot, int bid, int nb) {
   block_reduce2<NT>(s, c);
-  __shared__ int is_last;
   if (threadIdx.x == 0) {
-    ws.partials[2 * slot] = s;
-    ws.partials[2 * slot + 1] = c;
-    is_last = 0;
-    if (final_launch) {
-      __threadfence();
-      unsigned t = atomicAdd(ws.ticket, 1u);
-      is_last = (t == (unsigned)(nblocks_launch - 1));
-    }
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  double ts = 0.0, tc = 0.0;
-  for (int b = threadIdx.x; b < nfold; b += NT) {
-    ts += ws.partials[2 * b];
-    tc += ws.partials[2 * b + 1];
-  }
-  block_reduce2<NT>(ts, tc);
-  if (threadIdx.x == 0) {
-    if (out) out[0] = sqrt(ts / tc);
-    atomicExch(ws.ticket, 0u);
+    double *p = ws.partials + ((size_t)slot * ws.cap + bid) * 2;
+    p[0] = s;
+    p[1] = c;
+    if (bid == 0) ws.slot_nb[slot] = nb;
   }
 }
+
+// block b folds slots slot0 + b*fold ... slot0 + b*fold + fold-1 into out[b]
+__global__ __launch_bounds__(256) void k_norm_fold(NormWs ws, int slot0, int fold, double *out);
 
 }  // namespace pmg

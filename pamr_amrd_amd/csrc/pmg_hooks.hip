@@ -38,7 +38,7 @@ __global__ __launch_bounds__(NT) void k_hook_relax_pass(double *V, const double 
                                                         const double *c, const double *cmask,
                                                         const double *chr, double ex, HookGeom g,
                                                         Spacing<double> sp, int pass, NormWs ws,
-                                                        double *norm_out, int nb) {
+                                                        int nb) {
   const int mi = g.nx - 2, mj = g.ny - 2, mk = g.nz - 2;
   size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
   double s = 0.0, cnt = 0.0;
@@ -53,7 +53,7 @@ __global__ __launch_bounds__(NT) void k_hook_relax_pass(double *V, const double 
       cnt = 1.0;
     }
   }
-  norm_epilogue<NT>(s, cnt, ws, norm_out, pass * nb + blockIdx.x, nb, 2 * nb, pass == 1);
+  norm_partial<NT>(s, cnt, ws, pass, blockIdx.x, nb);
 }
 
 template <int OP, bool P2>
@@ -76,7 +76,7 @@ __global__ __launch_bounds__(NT) void k_hook_residual(double *res, const double 
                                                       const double *a, const double *c,
                                                       const double *cmask, const double *chr,
                                                       double ex, HookGeom g, Spacing<double> sp,
-                                                      NormWs ws, double *norm_out, int nb) {
+                                                      NormWs ws, int nb) {
   const int mi = g.nx - 2, mj = g.ny - 2, mk = g.nz - 2;
   size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
   double s = 0.0, cnt = 0.0;
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(NT) void k_hook_residual(double *res, const double 
       cnt = 1.0;
     }
   }
-  norm_epilogue<NT>(s, cnt, ws, norm_out, blockIdx.x, nb, nb, true);
+  norm_partial<NT>(s, cnt, ws, 0, blockIdx.x, nb);
 }
 
 // Host helpers ---------------------------------------------------------------
@@ -113,23 +113,24 @@ Spacing<double> make_spacing(double dx, double dy, double dz) {
   return sp;
 }
 
-// Hook-layer norm workspace (one per process; the hook layer is
-// single-threaded like the reference driver).
-NormWs g_ws = {nullptr, nullptr, 0};
+// Hook-layer norm workspace: two slots (the two colour passes of a relax
+// sweep are folded together).  The hook layer is single-threaded like the
+// reference driver.
+NormWs g_ws = {nullptr, nullptr, 0, 2};
 
-int ensure_ws(int nparts) {
-  if (nparts <= g_ws.capacity) return 0;
+int ensure_ws(int nb) {
+  if (nb <= g_ws.cap) return 0;
   if (g_ws.partials) {
     PMG_HOST_CHECK(hipDeviceSynchronize());
     PMG_HOST_CHECK(hipFree(g_ws.partials));
   }
-  int cap = nparts < 4096 ? 4096 : nparts;
-  PMG_HOST_CHECK(hipMalloc(&g_ws.partials, sizeof(double) * 2 * cap));
-  if (!g_ws.ticket) {
-    PMG_HOST_CHECK(hipMalloc(&g_ws.ticket, sizeof(unsigned)));
-    PMG_HOST_CHECK(hipMemset(g_ws.ticket, 0, sizeof(unsigned)));
+  int cap = nb < 4096 ? 4096 : nb;
+  PMG_HOST_CHECK(hipMalloc(&g_ws.partials, sizeof(double) * 2 * 2 * cap));
+  if (!g_ws.slot_nb) {
+    PMG_HOST_CHECK(hipMalloc(&g_ws.slot_nb, sizeof(int) * 2));
+    PMG_HOST_CHECK(hipMemset(g_ws.slot_nb, 0, sizeof(int) * 2));
   }
-  g_ws.capacity = cap;
+  g_ws.cap = cap;
   return 0;
 }
 
@@ -147,14 +148,18 @@ int pmg_hook_relax(int op, double *V, const double *rhs, const double *rho, cons
   Spacing<double> sp = make_spacing(dx, dy, dz);
   size_t n = (size_t)(nx - 2) * (ny - 2) * (nz - 2);
   int nb = (int)((n + NT - 1) / NT);
-  if (ensure_ws(2 * nb)) return -1;
+  if (ensure_ws(nb)) return -1;
   for (int pass = 0; pass <= 1; pass++) {
 #define PMG_HR(OPV, P2V)                                                                         \
   hipLaunchKernelGGL((k_hook_relax_pass<OPV, P2V>), dim3(nb), dim3(NT), 0, stream, V, rhs, rho, \
-                     a, c, cmask, chr, ex, g, sp, pass, g_ws, norm_dev, nb)
+                     a, c, cmask, chr, ex, g, sp, pass, g_ws, nb)
     if (op == 0) { if (sp.pow2) PMG_HR(0, true); else PMG_HR(0, false); }
     else { if (sp.pow2) PMG_HR(1, true); else PMG_HR(1, false); }
 #undef PMG_HR
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  if (norm_dev) {
+    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(256), 0, stream, g_ws, 0, 2, norm_dev);
     PMG_HOST_CHECK(hipGetLastError());
   }
   return 0;
@@ -173,11 +178,15 @@ int pmg_hook_residual(int op, double *res, const double *rhs, const double *V, c
   if (ensure_ws(nb)) return -1;
 #define PMG_HS(OPV, P2V)                                                                        \
   hipLaunchKernelGGL((k_hook_residual<OPV, P2V>), dim3(nb), dim3(NT), 0, stream, res, rhs, V, \
-                     rho, a, c, cmask, chr, ex, g, sp, g_ws, norm_dev, nb)
+                     rho, a, c, cmask, chr, ex, g, sp, g_ws, nb)
   if (op == 0) { if (sp.pow2) PMG_HS(0, true); else PMG_HS(0, false); }
   else { if (sp.pow2) PMG_HS(1, true); else PMG_HS(1, false); }
 #undef PMG_HS
   PMG_HOST_CHECK(hipGetLastError());
+  if (norm_dev) {
+    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(256), 0, stream, g_ws, 0, 1, norm_dev);
+    PMG_HOST_CHECK(hipGetLastError());
+  }
   return 0;
 }
 

@@ -35,48 +35,75 @@ struct SweepP {
   int zchunk;
   Spacing<T> sp;
   NormWs ws;
-  double *norm_out;
+  int slot;  // norm workspace slot
   int nb;
 };
 
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, bool HALO>
-__device__ __forceinline__ void sweep_update(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int x0,
-                                             int y0, int k, int pass, bool count, double &s,
-                                             double &cnt) {
+// Position of colour-slot q of a sweep phase on plane k.  HALO (pass 0):
+// tile + 1 ring; otherwise (pass 1) the tile.  Returns false when the slot
+// falls outside the region.  (lx, ly) index the LDS plane, whose origin is
+// the global point (x0-2, y0-2).
+template <int TX, int TY, bool HALO>
+__device__ __forceinline__ bool sweep_pos(int q, int x0, int y0, int k, int pass, int &lx,
+                                          int &ly) {
   constexpr int W = HALO ? TX + 2 : TX, H = HALO ? TY + 2 : TY, OFF = HALO ? 1 : 2;
   constexpr int W2 = (W + 1) / 2;
+  if (q >= W2 * H) return false;
+  ly = OFF + q / W2;
+  const int j = y0 - 2 + ly;
+  const int want = (j + k + 3 + pass) & 1;  // parity of i: (i+j+k+3+pass) even
+  lx = OFF + ((want - (x0 - 2 + OFF)) & 1) + 2 * (q % W2);
+  return lx <= OFF + W - 1;
+}
+
+template <typename T>
+__device__ __forceinline__ bool sweep_updatable(const SweepP<T> &P, int i, int j, int k) {
+  return i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1 && k >= P.bz0 && k <= P.bz1;
+}
+
+// One colour update at LDS position (lx, ly) of plane k.  rho/rhs are the
+// values at that point (prefetched by the caller).
+template <typename T, int OP, bool P2, bool RHS0, int TX, int TY>
+__device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int lx,
+                                         int ly, int k, size_t p, T rho, T rhs) {
   T(*S0)[TX + 4] = sU[k & 3];
   T(*Sm)[TX + 4] = sU[(k - 1) & 3];
   T(*Sp)[TX + 4] = sU[(k + 1) & 3];
-  const size_t sy = (size_t)P.nx, sz = (size_t)P.nx * P.ny;
-  for (int q = threadIdx.x; q < W2 * H; q += blockDim.x) {
-    const int ly = OFF + q / W2;
-    const int j = y0 - 2 + ly;
-    // first lx >= OFF whose i has the colour's parity: (i+j+k+3+pass) even
-    const int want = (j + k + 3 + pass) & 1;             // parity of i
-    const int lx = OFF + (((want - (x0 - 2 + OFF)) & 1)) + 2 * (q % W2);
-    if (lx > OFF + W - 1) continue;
-    const int i = x0 - 2 + lx;
-    if (i < P.bx0 || i > P.bx1 || j < P.by0 || j > P.by1) continue;
+  const T uc = S0[ly][lx];
+  T jac, L;
+  if constexpr (OP == 0) {
+    L = lop_poisson<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
+                           Sp[ly][lx], Sm[ly][lx], rho, P.sp);
+    jac = P.sp.jac;
+  } else {
+    const T *__restrict__ a = P.a;
+    const size_t sy = (size_t)P.nx, sz = (size_t)P.nx * P.ny;
+    L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
+                           Sp[ly][lx], Sm[ly][lx], a[p], a[p + 1], a[p - 1], a[p + sy], a[p - sy],
+                           a[p + sz], a[p - sz], P.c[p], rho, P.sp, &jac);
+  }
+  T res;
+  if constexpr (RHS0) res = L - T(0); else res = L - rhs;
+  S0[ly][lx] = uc - res / jac;
+  return res;
+}
+
+// Update every slot of one phase on plane k with values fetched directly
+// (prologue only).
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, bool HALO>
+__device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4],
+                                                   int x0, int y0, int k, int pass, bool count,
+                                                   double &s, double &cnt) {
+  constexpr int W = HALO ? TX + 2 : TX, H = HALO ? TY + 2 : TY;
+  for (int q = threadIdx.x; q < ((W + 1) / 2) * H; q += NT) {
+    int lx, ly;
+    if (!sweep_pos<TX, TY, HALO>(q, x0, y0, k, pass, lx, ly)) continue;
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly;
+    if (!sweep_updatable(P, i, j, k)) continue;
     const size_t p = ix3(i, j, k, P.nx, P.ny);
-    if constexpr (EX) {
-      if (P.chr[p]) continue;
-    }
-    const T uc = S0[ly][lx];
-    T jac, L;
-    if constexpr (OP == 0) {
-      L = lop_poisson<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
-                             Sp[ly][lx], Sm[ly][lx], P.rho[p], P.sp);
-      jac = P.sp.jac;
-    } else {
-      const T *a = P.a;
-      L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
-                             Sp[ly][lx], Sm[ly][lx], a[p], a[p + 1], a[p - 1], a[p + sy],
-                             a[p - sy], a[p + sz], a[p - sz], P.c[p], P.rho[p], P.sp, &jac);
-    }
-    T res;
-    if constexpr (RHS0) res = L - T(0); else res = L - P.rhs[p];
-    S0[ly][lx] = uc - res / jac;
+    if constexpr (EX) { if (P.chr[p]) continue; }
+    const T rhs = RHS0 ? T(0) : P.rhs[p];
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY>(P, sU, lx, ly, k, p, P.rho[p], rhs);
     if (count && lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2) {
       s += (double)res * (double)res;
       cnt += 1.0;
@@ -84,23 +111,100 @@ __device__ __forceinline__ void sweep_update(const SweepP<T> &P, T (*sU)[TY + 4]
   }
 }
 
+// Prefetch state of one phase: per slot the point index (or -1) and its
+// rho/rhs values, loaded one step before use.
+template <typename T, int N>
+struct PhaseRegs {
+  long long p[N];
+  int lx[N], ly[N];
+  T rho[N], rhs[N];
+};
+
+template <typename T, bool EX, bool RHS0, int TX, int TY, int NT, bool HALO, int N>
+__device__ __forceinline__ void phase_prefetch(const SweepP<T> &P, int x0, int y0, int k, int pass,
+                                               PhaseRegs<T, N> &R) {
+#pragma unroll
+  for (int m = 0; m < N; m++) {
+    R.p[m] = -1;
+    int lx, ly;
+    const int q = threadIdx.x + m * NT;
+    if (k < P.bz0 || k > P.bz1) continue;
+    if (!sweep_pos<TX, TY, HALO>(q, x0, y0, k, pass, lx, ly)) continue;
+    const int i = x0 - 2 + lx, j = y0 - 2 + ly;
+    if (!sweep_updatable(P, i, j, k)) continue;
+    const size_t p = ix3(i, j, k, P.nx, P.ny);
+    if constexpr (EX) { if (P.chr[p]) continue; }
+    R.p[m] = (long long)p;
+    R.lx[m] = lx;
+    R.ly[m] = ly;
+    R.rho[m] = P.rho[p];
+    if constexpr (!RHS0) R.rhs[m] = P.rhs[p];
+  }
+}
+
+template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, int N>
+__device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int k,
+                                          const PhaseRegs<T, N> &R, bool count, double &s,
+                                          double &cnt) {
+#pragma unroll
+  for (int m = 0; m < N; m++) {
+    if (R.p[m] < 0) continue;
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY>(P, sU, R.lx[m], R.ly[m], k, (size_t)R.p[m],
+                                                       R.rho[m], RHS0 ? T(0) : R.rhs[m]);
+    if (count && R.lx[m] >= 2 && R.lx[m] < TX + 2 && R.ly[m] >= 2 && R.ly[m] < TY + 2) {
+      s += (double)res * (double)res;
+      cnt += 1.0;
+    }
+  }
+}
+
+// Fused sweep.  Step k finalises plane k: pass 0 on plane k+1 (tile+1),
+// pass 1 on plane k (tile), store plane k.  While a step computes, the u
+// values of plane k+3 and the rho/rhs values of the next step are already
+// in flight in registers, so HBM latency overlaps the stencil work.
 template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
   constexpr int LX = TX + 4, LY = TY + 4;
+  constexpr int NLD = (LX * LY + NT - 1) / NT;
+  constexpr int N0 = (((TX + 3) / 2) * (TY + 2) + NT - 1) / NT;
+  constexpr int N1 = (((TX + 1) / 2) * TY + NT - 1) / NT;
   __shared__ T sU[4][LY][LX];
   const int x0 = 1 + blockIdx.x * TX, y0 = 1 + blockIdx.y * TY;
   const int z0 = 1 + blockIdx.z * P.zchunk;
   const int z1 = min(z0 + P.zchunk, P.nz - 1);
   const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const T *__restrict__ uin = P.uin;
+  T *__restrict__ uout = P.uout;
   double s = 0.0, cnt = 0.0;
 
-  auto load = [&](int k) {
+  auto load_direct = [&](int k) {
     if (k < 0 || k >= nz) return;
     T(*S)[LX] = sU[k & 3];
     for (int q = threadIdx.x; q < LX * LY; q += NT) {
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      if (i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = P.uin[ix3(i, j, k, nx, ny)];
+      if (i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = uin[ix3(i, j, k, nx, ny)];
+    }
+  };
+  T ld[NLD];
+  auto load_issue = [&](int k) {
+#pragma unroll
+    for (int m = 0; m < NLD; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int lx = q % LX, ly = q / LX;
+      const int i = x0 - 2 + lx, j = y0 - 2 + ly;
+      if (k < nz && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) ld[m] = uin[ix3(i, j, k, nx, ny)];
+    }
+  };
+  auto load_commit = [&](int k) {
+    if (k >= nz) return;
+    T(*S)[LX] = sU[k & 3];
+#pragma unroll
+    for (int m = 0; m < NLD; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int lx = q % LX, ly = q / LX;
+      const int i = x0 - 2 + lx, j = y0 - 2 + ly;
+      if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = ld[m];
     }
   };
   auto store = [&](int k) {
@@ -108,33 +212,44 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
     for (int q = threadIdx.x; q < TX * TY; q += NT) {
       const int tx = q % TX, ty = q / TX;
       const int i = x0 + tx, j = y0 + ty;
-      if (i <= nx - 2 && j <= ny - 2) P.uout[ix3(i, j, k, nx, ny)] = S[ty + 2][tx + 2];
+      if (i <= nx - 2 && j <= ny - 2) uout[ix3(i, j, k, nx, ny)] = S[ty + 2][tx + 2];
     }
   };
 
-  load(z0 - 2);
-  load(z0 - 1);
-  load(z0);
-  load(z0 + 1);
+  // prologue: planes z0-2 .. z0+1, pass 0 on z0-1 (tile) and z0 (tile+1)
+  load_direct(z0 - 2);
+  load_direct(z0 - 1);
+  load_direct(z0);
+  load_direct(z0 + 1);
   __syncthreads();
   if (z0 - 1 >= P.bz0 && z0 - 1 <= P.bz1)
-    sweep_update<T, OP, EX, P2, RHS0, TX, TY, false>(P, sU, x0, y0, z0 - 1, 0, false, s, cnt);
+    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, false>(P, sU, x0, y0, z0 - 1, 0, false, s, cnt);
   if (z0 >= P.bz0 && z0 <= P.bz1)
-    sweep_update<T, OP, EX, P2, RHS0, TX, TY, true>(P, sU, x0, y0, z0, 0, true, s, cnt);
+    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, true>(P, sU, x0, y0, z0, 0, true, s, cnt);
+  __syncthreads();
+  load_direct(z0 + 2);
+  PhaseRegs<T, N0> cur0, nxt0;
+  PhaseRegs<T, N1> cur1, nxt1;
+  phase_prefetch<T, EX, RHS0, TX, TY, NT, true, N0>(P, x0, y0, z0 + 1, 0, cur0);
+  phase_prefetch<T, EX, RHS0, TX, TY, NT, false, N1>(P, x0, y0, z0, 1, cur1);
   __syncthreads();
   for (int k = z0; k < z1; k++) {
-    load(k + 2);
+    // in flight during this step: u of plane k+3, rho/rhs of the next step
+    load_issue(k + 3);
+    phase_prefetch<T, EX, RHS0, TX, TY, NT, true, N0>(P, x0, y0, k + 2, 0, nxt0);
+    phase_prefetch<T, EX, RHS0, TX, TY, NT, false, N1>(P, x0, y0, k + 1, 1, nxt1);
+    phase_run<T, OP, P2, RHS0, TX, TY, N0>(P, sU, k + 1, cur0, k + 1 < z1, s, cnt);
     __syncthreads();
-    if (k + 1 >= P.bz0 && k + 1 <= P.bz1)
-      sweep_update<T, OP, EX, P2, RHS0, TX, TY, true>(P, sU, x0, y0, k + 1, 0, k + 1 < z1, s, cnt);
-    __syncthreads();
-    if (k >= P.bz0 && k <= P.bz1)
-      sweep_update<T, OP, EX, P2, RHS0, TX, TY, false>(P, sU, x0, y0, k, 1, true, s, cnt);
+    phase_run<T, OP, P2, RHS0, TX, TY, N1>(P, sU, k, cur1, true, s, cnt);
     __syncthreads();
     store(k);
+    load_commit(k + 3);
+    cur0 = nxt0;
+    cur1 = nxt1;
+    __syncthreads();
   }
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  norm_epilogue<NT>(s, cnt, P.ws, P.norm_out, bid, P.nb, P.nb, true);
+  norm_partial<NT>(s, cnt, P.ws, P.slot, bid, P.nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,38 +266,50 @@ struct ResidP {
   float *neg32;  // OUT32 mode: writes (float)(-res) here instead of res
   int nx, ny, nz;
   int bx0, bx1, by0, by1, bz0, bz1;
+  int zchunk;
   Spacing<T> sp;
   NormWs ws;
-  double *norm_out;
+  int slot;
   int nb;
 };
 
-template <typename T, int OP, bool EX, bool P2, bool RHS0, bool ZERO, bool OUT32, int NT>
-__global__ __launch_bounds__(NT) void k_residual(ResidP<T> P) {
+// block = RTX x RTY columns of all grid points, marching a z chunk
+constexpr int RTX = 64, RTY = 4;
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0, bool ZERO, bool OUT32>
+__global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
   const int nx = P.nx, ny = P.ny;
   const size_t sy = nx, sz = (size_t)nx * ny;
-  // block covers NT consecutive x of one (j,k) row segment
-  const int segs = (nx + NT - 1) / NT;
-  const int row = blockIdx.x / segs, seg = blockIdx.x % segs;
-  const int j = row % ny, k = row / ny;
-  const int i = seg * NT + threadIdx.x;
+  const int i = blockIdx.x * RTX + threadIdx.x % RTX;
+  const int j = blockIdx.y * RTY + threadIdx.x / RTX;
+  const int z0 = blockIdx.z * P.zchunk, z1 = min(z0 + P.zchunk, P.nz);
   double s = 0.0, cnt = 0.0;
-  if (i < nx && k < P.nz) {
-    const size_t p = ix3(i, j, k, nx, ny);
-    bool on = i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1 && k >= P.bz0 && k <= P.bz1;
-    if constexpr (EX) on = on && !P.chr[p];
-    if (on) {
-      T L = lop_at<T, OP, P2>(P.u, P.rho, P.a, P.c, p, sy, sz, P.sp, nullptr);
-      T r;
-      if constexpr (RHS0) r = L - T(0); else r = L - P.rhs[p];
-      if constexpr (OUT32) P.neg32[p] = (float)(-r); else P.res[p] = r;
-      s = (double)r * (double)r;
-      cnt = 1.0;
-    } else if constexpr (ZERO) {
-      if constexpr (OUT32) P.neg32[p] = 0.0f; else P.res[p] = T(0);
+  const T *__restrict__ u = P.u;
+  const T *__restrict__ rho = P.rho;
+  const T *__restrict__ rhs = P.rhs;
+  T *__restrict__ res = P.res;
+  float *__restrict__ neg32 = P.neg32;
+  if (i < nx && j < ny) {
+    const bool inxy = i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1;
+#pragma unroll 4
+    for (int k = z0; k < z1; k++) {
+      const size_t p = ix3(i, j, k, nx, ny);
+      bool on = inxy && k >= P.bz0 && k <= P.bz1;
+      if constexpr (EX) on = on && !P.chr[p];
+      if (on) {
+        T L = lop_at<T, OP, P2>(u, rho, P.a, P.c, p, sy, sz, P.sp, nullptr);
+        T r;
+        if constexpr (RHS0) r = L - T(0); else r = L - rhs[p];
+        if constexpr (OUT32) neg32[p] = (float)(-r); else res[p] = r;
+        s += (double)r * (double)r;
+        cnt += 1.0;
+      } else if constexpr (ZERO) {
+        if constexpr (OUT32) neg32[p] = 0.0f; else res[p] = T(0);
+      }
     }
   }
-  norm_epilogue<NT>(s, cnt, P.ws, P.norm_out, blockIdx.x, P.nb, P.nb, true);
+  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  norm_partial<RTX * RTY>(s, cnt, P.ws, P.slot, bid, P.nb);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,9 +382,10 @@ __global__ __launch_bounds__(256) void k_lop_rhs(LopRhsP<T> P) {
 // directly instead of being stored.  Boundary points also go to `alt`.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_cgc(T *u, T *alt, const T *uc, const T *lopc, T w0_i,
-                                             int nx, int ny, int nz, int cnx, int cny,
-                                             const uint8_t *chr) {
+__global__ __launch_bounds__(256) void k_cgc(T *__restrict__ u, T *__restrict__ alt,
+                                             const T *__restrict__ uc, const T *__restrict__ lopc,
+                                             T w0_i, int nx, int ny, int nz, int cnx, int cny,
+                                             const uint8_t *__restrict__ chr) {
   size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (q >= (size_t)nx * ny * nz) return;
   const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
@@ -396,6 +524,12 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_copy(T *dst, const T *src, size_t n) {
   size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (q < n) dst[q] = src[q];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill(T *dst, T v, size_t n) {
+  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[q] = v;
 }
 
 template <typename TD, typename TS>

@@ -117,12 +117,12 @@ struct pmg_solver {
   int pre = 2, pst = 2, max_citer = 5000, rop_u = 3, rop_res = 3, rop_src = 3, iord = 2;
   double crtol = 1e-3, w0_r = 1.0, w0_i = 1.0;
   hipStream_t stream = nullptr;
-  NormWs ws{nullptr, nullptr, 0};
-  double *d_norms = nullptr;  // kNormSlots doubles (FP64 cycle, outer test)
-  double *d_norms32 = nullptr;  // kNormSlots doubles (FP32 inner cycle)
+  NormWs ws{nullptr, nullptr, 0, 0};
+  double *d_norms = nullptr;  // folded norms: FP64 slots, FP32 slots, coarse-solve outputs
   int *d_citer = nullptr;     // 2 ints (FP64 cycle, FP32 cycle)
   double *h_pinned = nullptr; // pinned readback
   int sweep_variant = 1, zchunk = 32;
+  int lv_variant[kMaxLev], lv_zchunk[kMaxLev];  // per-level sweep configuration
   int use_graph = 1;
   hipGraphExec_t g64 = nullptr;   // captured FP64 V-cycle
   hipGraphExec_t g32 = nullptr;   // captured mixed-precision outer iteration
@@ -138,7 +138,10 @@ enum {
   NS_CPRE = 2 * kMaxLev + 1,    // coarsest pre-sweep norm
   NS_R0 = 2 * kMaxLev + 2,      // standalone finest residual
   NS_OUTER = 2 * kMaxLev + 3,   // mixed: outer FP64 residual
-  kNormSlots = 2 * kMaxLev + 8
+  kNormSlots = 2 * kMaxLev + 8,
+  NS_F32 = kNormSlots,          // FP32 inner-cycle slots start here
+  NS_COARSE = 2 * kNormSlots,   // coarse solve outputs: pre64, post64, pre32, post32
+  kNormTotal = 2 * kNormSlots + 8
 };
 
 namespace {
@@ -207,6 +210,7 @@ constexpr int kNumSweepVariants = 4;
 
 template <typename T, int OP, bool EX, bool P2, bool RHS0>
 int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
+  P.zchunk = std::max(1, std::min(P.zchunk, P.nz - 2));
   int tx = 64, ty = 8;
   switch (variant) {
 #define PMG_TXY(V, TXV, TYV, NTV) case V: tx = TXV; ty = TYV; break;
@@ -216,7 +220,7 @@ int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
   }
   dim3 grid((P.nx - 2 + tx - 1) / tx, (P.ny - 2 + ty - 1) / ty, (P.nz - 2 + P.zchunk - 1) / P.zchunk);
   P.nb = (int)(grid.x * grid.y * grid.z);
-  if (P.nb > S->ws.capacity) return -3;
+  if (P.nb > S->ws.cap) return -3;
   switch (variant) {
 #define PMG_LSW(V, TXV, TYV, NTV)                                                              \
   case V:                                                                                      \
@@ -231,15 +235,14 @@ int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
 }
 
 template <typename T, int OP, bool EX>
-int launch_sweep(pmg_solver *S, SweepP<T> P, bool p2, bool rhs0) {
-  const int v = S->sweep_variant;
+int launch_sweep(pmg_solver *S, int v, SweepP<T> P, bool p2, bool rhs0) {
   if (p2) return rhs0 ? launch_sweep_v<T, OP, EX, true, true>(S, v, P) : launch_sweep_v<T, OP, EX, true, false>(S, v, P);
   return rhs0 ? launch_sweep_v<T, OP, EX, false, true>(S, v, P) : launch_sweep_v<T, OP, EX, false, false>(S, v, P);
 }
 
 // n red-black sweeps on level L (ping-pong u <-> ua; odd counts copy back)
 template <typename T>
-int sweeps(pmg_solver *S, int L, int n, double *norm_out, bool rhs0) {
+int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
   Level &Lv = S->lev[L - 1];
   LevelArrays<T> &A = arrays<T>(Lv);
   if (n <= 0) return 0;
@@ -250,17 +253,18 @@ int sweeps(pmg_solver *S, int L, int n, double *norm_out, bool rhs0) {
   P.bx0 = Lv.rbox.lo[0]; P.bx1 = Lv.rbox.hi[0];
   P.by0 = Lv.rbox.lo[1]; P.by1 = Lv.rbox.hi[1];
   P.bz0 = Lv.rbox.lo[2]; P.bz1 = Lv.rbox.hi[2];
-  P.zchunk = std::max(1, std::min(S->zchunk, P.nz - 2));
+  P.zchunk = S->lv_zchunk[L - 1];
+  const int variant = S->lv_variant[L - 1];
   P.sp = spacing<T>(Lv);
   P.ws = S->ws;
-  P.norm_out = norm_out;
+  P.slot = slot;
   const bool p2 = P.sp.pow2 != 0;
   for (int s = 0; s < n; s++) {
     P.uin = (s % 2 == 0) ? A.u : A.ua;
     P.uout = (s % 2 == 0) ? A.ua : A.u;
     int rc;
-    if (S->op == 0) rc = S->do_ex ? launch_sweep<T, 0, true>(S, P, p2, rhs0) : launch_sweep<T, 0, false>(S, P, p2, rhs0);
-    else rc = S->do_ex ? launch_sweep<T, 1, true>(S, P, p2, rhs0) : launch_sweep<T, 1, false>(S, P, p2, rhs0);
+    if (S->op == 0) rc = S->do_ex ? launch_sweep<T, 0, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 0, false>(S, variant, P, p2, rhs0);
+    else rc = S->do_ex ? launch_sweep<T, 1, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 1, false>(S, variant, P, p2, rhs0);
     if (rc) return rc;
   }
   if (n % 2 == 1) {
@@ -272,11 +276,12 @@ int sweeps(pmg_solver *S, int L, int n, double *norm_out, bool rhs0) {
 
 template <typename T, int OP, bool EX, bool P2>
 int launch_residual_t(pmg_solver *S, ResidP<T> P, bool rhs0, bool zero) {
-  const int segs = (P.nx + 255) / 256;
-  const int nb = segs * P.ny * P.nz;
+  P.zchunk = std::max(1, std::min(16, P.nz));
+  dim3 grid((P.nx + RTX - 1) / RTX, (P.ny + RTY - 1) / RTY, (P.nz + P.zchunk - 1) / P.zchunk);
+  const int nb = (int)(grid.x * grid.y * grid.z);
   P.nb = nb;
-  if (nb > S->ws.capacity) return -3;
-#define PMG_LR(R0, Z0, O32) hipLaunchKernelGGL((k_residual<T, OP, EX, P2, R0, Z0, O32, 256>), dim3(nb), dim3(256), 0, S->stream, P)
+  if (nb > S->ws.cap) return -3;
+#define PMG_LR(R0, Z0, O32) hipLaunchKernelGGL((k_residual<T, OP, EX, P2, R0, Z0, O32>), grid, dim3(RTX * RTY), 0, S->stream, P)
   if (P.neg32) { PMG_LR(true, true, true); }
   else if (rhs0) { if (zero) PMG_LR(true, true, false); else PMG_LR(true, false, false); }
   else { if (zero) PMG_LR(false, true, false); else PMG_LR(false, false, false); }
@@ -286,7 +291,7 @@ int launch_residual_t(pmg_solver *S, ResidP<T> P, bool rhs0, bool zero) {
 }
 
 template <typename T>
-int residual(pmg_solver *S, int L, double *norm_out, bool rhs0, bool zero, float *neg32 = nullptr) {
+int residual(pmg_solver *S, int L, int slot, bool rhs0, bool zero, float *neg32 = nullptr) {
   Level &Lv = S->lev[L - 1];
   LevelArrays<T> &A = arrays<T>(Lv);
   ResidP<T> P;
@@ -304,7 +309,7 @@ int residual(pmg_solver *S, int L, double *norm_out, bool rhs0, bool zero, float
   }
   P.sp = spacing<T>(Lv);
   P.ws = S->ws;
-  P.norm_out = norm_out;
+  P.slot = slot;
   const bool p2 = P.sp.pow2 != 0;
 #define PMG_RS(OPV, EXV) \
   (p2 ? launch_residual_t<T, OPV, EXV, true>(S, P, rhs0, zero) : launch_residual_t<T, OPV, EXV, false>(S, P, rhs0, zero))
@@ -475,8 +480,15 @@ int coarse_solve(pmg_solver *S, int L, double *norm_pre, double *norm_post, int 
 // level is zero and coarse-level rho are zero (for a linear operator the
 // coarse rho cancels between the FAS rhs and the coarse relaxation).
 // ---------------------------------------------------------------------------
+int fold_norms(pmg_solver *S, int slot0, int nslots) {
+  hipLaunchKernelGGL(k_norm_fold, dim3(nslots), dim3(256), 0, S->stream, S->ws, slot0, 1,
+                     S->d_norms + slot0);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
 template <typename T>
-int enqueue_vcycle(pmg_solver *S, bool final_residual, double *norm_base, int *citer) {
+int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cnorm, int *citer) {
   const int Lf = S->nlev;
   int rc;
   for (int L = Lf; L >= 1; L--) {
@@ -491,7 +503,7 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, double *norm_base, int *c
       if ((rc = lop_rhs<T>(S, L))) return rc;
     }
     if (L == 1) {
-      if ((rc = coarse_solve<T>(S, 1, norm_base + NS_CPRE, norm_base + NS_RELAX + 0, citer, S->max_citer))) return rc;
+      if ((rc = coarse_solve<T>(S, 1, cnorm, cnorm + 1, citer, S->max_citer))) return rc;
     } else {
       if ((rc = sweeps<T>(S, L, S->pre, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
     }
@@ -524,7 +536,11 @@ int capture_or_run(pmg_solver *S, hipGraphExec_t *cache, int (*enq)(pmg_solver *
   return 0;
 }
 
-int enq_v64(pmg_solver *S) { return enqueue_vcycle<double>(S, true, S->d_norms, S->d_citer); }
+int enq_v64(pmg_solver *S) {
+  int rc = enqueue_vcycle<double>(S, true, 0, S->d_norms + NS_COARSE, S->d_citer);
+  if (rc) return rc;
+  return fold_norms(S, 0, kNormSlots);
+}
 
 // Mixed precision outer iteration (defect correction).  On entry the FP32
 // source of the finest level holds -r (r = L64[u], the FP64 defect) and the
@@ -544,10 +560,11 @@ int enq_mixed(pmg_solver *S) {
   const int Lf = S->nlev;
   Level &F = S->lev[Lf - 1];
   int rc;
-  if ((rc = enqueue_vcycle<float>(S, false, S->d_norms32, S->d_citer + 1))) return rc;
+  if ((rc = enqueue_vcycle<float>(S, false, NS_F32, S->d_norms + NS_COARSE + 2, S->d_citer + 1))) return rc;
   hipLaunchKernelGGL(k_add_f32, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.u, F.f.u, F.size);
   PMG_HOST_CHECK(hipGetLastError());
-  return residual<double>(S, Lf, S->d_norms + NS_OUTER, true, true, F.f.rho);
+  if ((rc = residual<double>(S, Lf, NS_OUTER, true, true, F.f.rho))) return rc;
+  return fold_norms(S, 0, 2 * kNormSlots);
 }
 
 int ensure_fp32(pmg_solver *S) {
@@ -633,23 +650,29 @@ pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidt
     Lv.cbox = cmask_child_box(Lv.n, bbox, bbox, Lv.gdx, bbox);
     if (alloc_level_arrays(S, Lv, false)) { pmg_destroy(S); return nullptr; }
   }
+  for (int L = 1; L <= nl; L++) {
+    const int n = S->lev[L - 1].n[2];
+    S->lv_variant[L - 1] = 1;
+    S->lv_zchunk[L - 1] = n >= 257 ? 32 : (n >= 129 ? 16 : (n >= 65 ? 8 : 4));
+  }
   // norm workspace sized for the largest launch
+  // partial-norm capacity: the largest launch over the finest level
   const Level &F = S->lev[nl - 1];
-  int cap = (int)(((size_t)(F.n[0] + 255) / 256) * F.n[1] * F.n[2]) + 64;
-  cap = std::max(cap, 1 << 16);
-  S->ws.capacity = cap;
-  if (hipMalloc(&S->ws.partials, sizeof(double) * 2 * cap) != hipSuccess ||
-      hipMalloc(&S->ws.ticket, sizeof(unsigned)) != hipSuccess ||
-      hipMalloc(&S->d_norms, sizeof(double) * kNormSlots) != hipSuccess ||
-      hipMalloc(&S->d_norms32, sizeof(double) * kNormSlots) != hipSuccess ||
+  const int rblk = ((F.n[0] + RTX - 1) / RTX) * ((F.n[1] + RTY - 1) / RTY) * ((F.n[2] + 15) / 16);
+  const int sblk = ((F.n[0] + 61) / 64) * ((F.n[1] + 5) / 8) * ((F.n[2] + 5) / 8);
+  int cap = std::max(1 << 14, std::max(rblk, sblk) + 64);
+  S->ws.cap = cap;
+  S->ws.nslots = 2 * kNormSlots;
+  if (hipMalloc(&S->ws.partials, sizeof(double) * 2 * (size_t)cap * S->ws.nslots) != hipSuccess ||
+      hipMalloc(&S->ws.slot_nb, sizeof(int) * S->ws.nslots) != hipSuccess ||
+      hipMalloc(&S->d_norms, sizeof(double) * kNormTotal) != hipSuccess ||
       hipMalloc(&S->d_citer, sizeof(int) * 2) != hipSuccess ||
-      hipHostMalloc(&S->h_pinned, sizeof(double) * (kNormSlots + 4)) != hipSuccess) {
+      hipHostMalloc(&S->h_pinned, sizeof(double) * (kNormTotal + 4)) != hipSuccess) {
     pmg_destroy(S);
     return nullptr;
   }
-  (void)hipMemsetAsync(S->ws.ticket, 0, sizeof(unsigned), S->stream);
-  (void)hipMemsetAsync(S->d_norms, 0, sizeof(double) * kNormSlots, S->stream);
-  (void)hipMemsetAsync(S->d_norms32, 0, sizeof(double) * kNormSlots, S->stream);
+  (void)hipMemsetAsync(S->ws.slot_nb, 0, sizeof(int) * S->ws.nslots, S->stream);
+  (void)hipMemsetAsync(S->d_norms, 0, sizeof(double) * kNormTotal, S->stream);
   (void)hipMemsetAsync(S->d_citer, 0, sizeof(int) * 2, S->stream);
   if (S->do_ex) {
     // default chr: nothing excised (every value ex + 1)
@@ -676,9 +699,8 @@ void pmg_destroy(pmg_solver *S) {
     if (S->lev[L].chrd) (void)hipFree(S->lev[L].chrd);
   }
   if (S->ws.partials) (void)hipFree(S->ws.partials);
-  if (S->ws.ticket) (void)hipFree(S->ws.ticket);
+  if (S->ws.slot_nb) (void)hipFree(S->ws.slot_nb);
   if (S->d_norms) (void)hipFree(S->d_norms);
-  if (S->d_norms32) (void)hipFree(S->d_norms32);
   if (S->d_citer) (void)hipFree(S->d_citer);
   if (S->h_pinned) (void)hipHostFree(S->h_pinned);
   if (S->stream) (void)hipStreamDestroy(S->stream);
@@ -705,9 +727,20 @@ int pmg_set_params(pmg_solver *S, int pre, int pst, int max_citer, double crtol,
 }
 
 int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk) {
+  // the same tile variant and z chunk for every level
   if (!S || variant < 0 || variant >= kNumSweepVariants || zchunk < 1) return -1;
-  S->sweep_variant = variant;
-  S->zchunk = zchunk;
+  for (int L = 0; L < S->nlev; L++) {
+    S->lv_variant[L] = variant;
+    S->lv_zchunk[L] = zchunk;
+  }
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_set_level_sweep_config(pmg_solver *S, int L, int variant, int zchunk) {
+  if (!S || L < 1 || L > S->nlev || variant < 0 || variant >= kNumSweepVariants || zchunk < 1) return -1;
+  S->lv_variant[L - 1] = variant;
+  S->lv_zchunk[L - 1] = zchunk;
   invalidate_graphs(S);
   return 0;
 }
@@ -824,13 +857,14 @@ int pmg_vcycle_async(pmg_solver *S) {
 
 int pmg_read_norms(pmg_solver *S, double *res_out, double *lev_norms, int *citer) {
   if (!S) return -1;
-  PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms, sizeof(double) * kNormSlots, hipMemcpyDeviceToHost, S->stream));
+  PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms, sizeof(double) * kNormTotal, hipMemcpyDeviceToHost, S->stream));
   int ci[2];
   PMG_HOST_CHECK(hipMemcpyAsync(ci, S->d_citer, sizeof(int) * 2, hipMemcpyDeviceToHost, S->stream));
   PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
   if (res_out) *res_out = S->h_pinned[NS_FINAL];
   if (lev_norms) {
     for (int L = 1; L <= S->nlev; L++) lev_norms[L - 1] = S->h_pinned[NS_RELAX + L - 1];
+    lev_norms[0] = S->h_pinned[NS_COARSE + 1];
     lev_norms[S->nlev - 1] = S->h_pinned[NS_FINAL];
   }
   if (citer) *citer = ci[0];
@@ -845,8 +879,9 @@ int pmg_vcycle(pmg_solver *S, double *res_out, double *lev_norms, int *citer) {
 
 int pmg_residual(pmg_solver *S, double *norm_out) {
   if (!S) return -1;
-  int rc = residual<double>(S, S->nlev, S->d_norms + NS_R0, true, true);
+  int rc = residual<double>(S, S->nlev, NS_R0, true, true);
   if (rc) return rc;
+  if ((rc = fold_norms(S, NS_R0, 1))) return rc;
   PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_R0, sizeof(double), hipMemcpyDeviceToHost, S->stream));
   PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
   if (norm_out) *norm_out = S->h_pinned[0];
@@ -891,7 +926,7 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
     if ((rc = ensure_fp32(S))) return rc;
     // initial FP32 source -r0 (the residual above already gave ||r0||)
     Level &F = S->lev[S->nlev - 1];
-    if ((rc = residual<double>(S, S->nlev, S->d_norms + NS_OUTER, true, true, F.f.rho))) return rc;
+    if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
     while (it < max_iter && r > rtol * r0) {
       if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
       PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost, S->stream));
@@ -905,6 +940,49 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
   }
   if (iters) *iters = it;
   return 0;
+}
+
+int pmg_zero_solution(pmg_solver *S) {
+  // initial guess u = 0 on the finest level (both sweep buffers); coarser u
+  // are overwritten by injection in every cycle
+  if (!S) return -1;
+  Level &F = S->lev[S->nlev - 1];
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.u, 0.0, F.size);
+  hipLaunchKernelGGL(k_fill<double>, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.ua, 0.0, F.size);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double *ms_avg) {
+  // Average duration of one launch of a solver kernel, timed with HIP
+  // events on the solver's stream over nrep back-to-back launches on the
+  // current data.  which: 0 = red-black sweep (ping-pong pairs, so u returns
+  // to its buffer), 1 = residual.  prec: 0 FP64, 1 FP32.
+  if (!S || L < 1 || L > S->nlev || nrep < 2 || !ms_avg) return -1;
+  if (prec == 1 && ensure_fp32(S)) return -1;
+  hipEvent_t e0, e1;
+  PMG_HOST_CHECK(hipEventCreate(&e0));
+  PMG_HOST_CHECK(hipEventCreate(&e1));
+  nrep -= nrep % 2;
+  const bool fine = (L == S->nlev);
+  int rc = 0;
+  PMG_HOST_CHECK(hipEventRecord(e0, S->stream));
+  if (which == 0) {
+    rc = prec ? sweeps<float>(S, L, nrep, NS_F32 + NS_RELAX + L - 1, fine)
+              : sweeps<double>(S, L, nrep, NS_RELAX + L - 1, fine);
+  } else {
+    for (int r = 0; r < nrep && !rc; r++)
+      rc = prec ? residual<float>(S, L, NS_F32 + NS_RESID + L - 1, fine, fine)
+                : residual<double>(S, L, NS_RESID + L - 1, fine, fine);
+  }
+  PMG_HOST_CHECK(hipEventRecord(e1, S->stream));
+  PMG_HOST_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_avg = (double)ms / nrep;
+  return rc;
 }
 
 int pmg_last_citer(pmg_solver *S, int which) {
