@@ -149,6 +149,46 @@ int pmg_num_sweep_variants(void);
 int pmg_set_graph(pmg_solver *S, int on);
 void *pmg_stream(pmg_solver *S);
 
+/* ---- 4. learned smoother / warm-start network (no reference counterpart:
+ * the north-star's learned component).  3x3x3 convolutions 1 -> 16 -> 16 -> 1,
+ * GELU, BF16 MFMA with FP32 accumulation. ---------------------------------- */
+typedef struct pmg_net pmg_net;
+pmg_net *pmg_net_create(int hidden, unsigned long long seed);
+void pmg_net_destroy(pmg_net *N);
+/* weights in the oracle layout: w1[16][1][27], b1[16], w2[16][16][27],
+ * b2[16], w3[1][16][27], b3[1] (host FP64) */
+int pmg_net_set_weights(pmg_net *N, const double *w1, const double *b1, const double *w2,
+                        const double *b2, const double *w3, const double *b3);
+int pmg_net_get_weights(pmg_net *N, double *w1, double *b1, double *w2, double *b2, double *w3,
+                        double *b3);
+/* y = N(x) on nb boxes of nx*ny*nz points (device FP32, zero padding) */
+int pmg_net_forward(pmg_net *N, const float *x, float *y, int nx, int ny, int nz, int nb);
+/* loss 0.5*mean((N(x)-t)^2) over points at least `margin` from a box face,
+ * and its gradients (host FP64, oracle layout) */
+int pmg_net_loss_grad(pmg_net *N, const float *x, const float *t, int nx, int ny, int nz, int nb,
+                      int margin, double *loss, double *gw1, double *gb1, double *gw2, double *gb2,
+                      double *gw3, double *gb3);
+/* one Adam step (lr) on a batch; *loss = loss before the step */
+int pmg_net_train_step(pmg_net *N, const float *x, const float *t, int nx, int ny, int nz, int nb,
+                       int margin, float lr, double *loss);
+void *pmg_net_stream(pmg_net *N);
+/* learned correction on one grid, on the caller's stream: y = N(scale*x),
+ * u += coef*y at interior points; pmg_net_recorrect repeats/undoes it */
+int pmg_net_correct(pmg_net *N, const float *x, float scale, double *u, double coef, int nx,
+                    int ny, int nz, void *stream);
+int pmg_net_recorrect(pmg_net *N, double *u, double coef, int nx, int ny, int nz, void *stream);
+/* training crops: out[c] = scale*(a - b) on side^3 cubes at host origins */
+int pmg_gather_crops(const double *a, const double *b, int nx, int ny, int nz, const int *origins_host,
+                     int ncrop, int side, double scale, float *out, void *stream);
+/* attach a network to a solver (pmg_solve_rel mode 2) and read its stats */
+int pmg_set_net(pmg_solver *S, pmg_net *N);
+int pmg_net_stats(pmg_solver *S, int *accepted, int *rejected, double *t_net_ms, double *t_vc_ms);
+/* finest-level solution u = ca*a + cb*b (device FP64; b may be NULL) */
+int pmg_set_solution_lincomb(pmg_solver *S, const double *a, const double *b, double ca, double cb);
+/* lane-layout self-test of the BF16 16x16x16 MFMA used by the network:
+ * D = bf16(A) * bf16(B) (host 16x16 row-major) */
+int pmg_net_mfma_selftest(const float *A, const float *B, float *D_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -81,6 +81,24 @@ def _declare(L):
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_stream.argtypes = [_vp]
     L.pmg_stream.restype = _vp
+    L.pmg_net_create.argtypes = [i, C.c_ulonglong]
+    L.pmg_net_create.restype = _vp
+    L.pmg_net_destroy.argtypes = [_vp]
+    L.pmg_net_destroy.restype = None
+    L.pmg_net_set_weights.argtypes = [_vp] + [_dp] * 6
+    L.pmg_net_get_weights.argtypes = [_vp] + [_dp] * 6
+    L.pmg_net_forward.argtypes = [_vp, _vp, _vp, i, i, i, i]
+    L.pmg_net_loss_grad.argtypes = [_vp, _vp, _vp, i, i, i, i, i, _dp] + [_dp] * 6
+    L.pmg_net_train_step.argtypes = [_vp, _vp, _vp, i, i, i, i, i, C.c_float, _dp]
+    L.pmg_net_stream.argtypes = [_vp]
+    L.pmg_net_stream.restype = _vp
+    L.pmg_net_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
+    L.pmg_net_correct.argtypes = [_vp, _vp, C.c_float, _vp, d, i, i, i, _vp]
+    L.pmg_net_recorrect.argtypes = [_vp, _vp, d, i, i, i, _vp]
+    L.pmg_gather_crops.argtypes = [_vp, _vp, i, i, i, _ip, i, i, d, _vp, _vp]
+    L.pmg_set_net.argtypes = [_vp, _vp]
+    L.pmg_net_stats.argtypes = [_vp, _ip, _ip, _dp, _dp]
+    L.pmg_set_solution_lincomb.argtypes = [_vp, _vp, _vp, d, d]
 
 
 def _check(rc, what):
@@ -213,6 +231,71 @@ class Solver:
 
     def stream(self):
         return self._L.pmg_stream(self.h)
+
+
+class Net:
+    """Owning wrapper of a ``pmg_net`` (the learned smoother network)."""
+
+    SHAPES = {"w1": (16, 1, 27), "b1": (16,), "w2": (16, 16, 27), "b2": (16,), "w3": (1, 16, 27),
+              "b3": (1,)}
+
+    def __init__(self, seed=1):
+        self._L = lib()
+        self.h = self._L.pmg_net_create(16, seed)
+        if not self.h:
+            raise RuntimeError("pmg_net_create failed")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.pmg_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get_weights(self):
+        ws = {k: np.zeros(v) for k, v in self.SHAPES.items()}
+        _check(self._L.pmg_net_get_weights(self.h, *[P(ws[k]) for k in ("w1", "b1", "w2", "b2", "w3", "b3")]),
+               "pmg_net_get_weights")
+        return ws
+
+    def set_weights(self, ws):
+        arr = [np.ascontiguousarray(ws[k], dtype=np.float64) for k in ("w1", "b1", "w2", "b2", "w3", "b3")]
+        _check(self._L.pmg_net_set_weights(self.h, *[P(a) for a in arr]), "pmg_net_set_weights")
+
+    def forward(self, x_dev, y_dev, shape, nb=1):
+        nz, ny, nx = shape
+        _check(self._L.pmg_net_forward(self.h, tptr(x_dev), tptr(y_dev), nx, ny, nz, nb), "pmg_net_forward")
+
+    def loss_grad(self, x_dev, t_dev, shape, nb=1, margin=0):
+        nz, ny, nx = shape
+        loss = C.c_double(0)
+        gs = {k: np.zeros(v) for k, v in self.SHAPES.items()}
+        _check(self._L.pmg_net_loss_grad(self.h, tptr(x_dev), tptr(t_dev), nx, ny, nz, nb, margin,
+                                         C.byref(loss),
+                                         *[P(gs[k]) for k in ("w1", "b1", "w2", "b2", "w3", "b3")]),
+               "pmg_net_loss_grad")
+        return loss.value, gs
+
+    def train_step(self, x_dev, t_dev, shape, nb, lr, margin=0):
+        nz, ny, nx = shape
+        loss = C.c_double(0)
+        _check(self._L.pmg_net_train_step(self.h, tptr(x_dev), tptr(t_dev), nx, ny, nz, nb, margin,
+                                          lr, C.byref(loss)), "pmg_net_train_step")
+        return loss.value
+
+
+def mfma_selftest(A, B):
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    D = np.zeros((16, 16), dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    _check(lib().pmg_net_mfma_selftest(A.ctypes.data_as(fp), B.ctypes.data_as(fp), D.ctypes.data_as(fp)),
+           "pmg_net_mfma_selftest")
+    return D
 
 
 # --------------------------------------------------------------------------

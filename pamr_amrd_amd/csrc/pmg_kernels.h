@@ -186,8 +186,9 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
       if (i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = uin[ix3(i, j, k, nx, ny)];
     }
   };
-  T ld[NLD];
-  auto load_issue = [&](int k) {
+  // u planes are prefetched two steps ahead into alternating register sets
+  T ldA[NLD], ldB[NLD];
+  auto load_issue = [&](T(&ld)[NLD], int k) {
 #pragma unroll
     for (int m = 0; m < NLD; m++) {
       const int q = threadIdx.x + m * NT;
@@ -196,7 +197,7 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
       if (k < nz && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) ld[m] = uin[ix3(i, j, k, nx, ny)];
     }
   };
-  auto load_commit = [&](int k) {
+  auto load_commit = [&](const T(&ld)[NLD], int k) {
     if (k >= nz) return;
     T(*S)[LX] = sU[k & 3];
 #pragma unroll
@@ -228,14 +229,16 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
     sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, true>(P, sU, x0, y0, z0, 0, true, s, cnt);
   __syncthreads();
   load_direct(z0 + 2);
+  load_issue(ldA, z0 + 3);
   PhaseRegs<T, N0> cur0, nxt0;
   PhaseRegs<T, N1> cur1, nxt1;
   phase_prefetch<T, EX, RHS0, TX, TY, NT, true, N0>(P, x0, y0, z0 + 1, 0, cur0);
   phase_prefetch<T, EX, RHS0, TX, TY, NT, false, N1>(P, x0, y0, z0, 1, cur1);
   __syncthreads();
-  for (int k = z0; k < z1; k++) {
-    // in flight during this step: u of plane k+3, rho/rhs of the next step
-    load_issue(k + 3);
+  // step k: issue plane k+4 and the next step's rho/rhs; pass 0 on k+1;
+  // pass 1 on k; store k; commit plane k+3 (issued one step earlier)
+  auto step = [&](int k, const T(&ldc)[NLD], T(&ldi)[NLD]) {
+    load_issue(ldi, k + 4);
     phase_prefetch<T, EX, RHS0, TX, TY, NT, true, N0>(P, x0, y0, k + 2, 0, nxt0);
     phase_prefetch<T, EX, RHS0, TX, TY, NT, false, N1>(P, x0, y0, k + 1, 1, nxt1);
     phase_run<T, OP, P2, RHS0, TX, TY, N0>(P, sU, k + 1, cur0, k + 1 < z1, s, cnt);
@@ -243,10 +246,14 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
     phase_run<T, OP, P2, RHS0, TX, TY, N1>(P, sU, k, cur1, true, s, cnt);
     __syncthreads();
     store(k);
-    load_commit(k + 3);
+    load_commit(ldc, k + 3);
     cur0 = nxt0;
     cur1 = nxt1;
     __syncthreads();
+  };
+  for (int k = z0; k < z1; k += 2) {
+    step(k, ldA, ldB);
+    if (k + 1 < z1) step(k + 1, ldB, ldA);
   }
   const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   norm_partial<NT>(s, cnt, P.ws, P.slot, bid, P.nb);
@@ -273,8 +280,9 @@ struct ResidP {
   int nb;
 };
 
-// block = RTX x RTY columns of all grid points, marching a z chunk
-constexpr int RTX = 64, RTY = 4;
+// block = RTX x RTY columns of all grid points, a z chunk of RZC planes
+// (unrolled: the loads of all RZC points are issued together)
+constexpr int RTX = 64, RTY = 4, RZC = 4;
 
 template <typename T, int OP, bool EX, bool P2, bool RHS0, bool ZERO, bool OUT32>
 __global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
   const size_t sy = nx, sz = (size_t)nx * ny;
   const int i = blockIdx.x * RTX + threadIdx.x % RTX;
   const int j = blockIdx.y * RTY + threadIdx.x / RTX;
-  const int z0 = blockIdx.z * P.zchunk, z1 = min(z0 + P.zchunk, P.nz);
+  const int z0 = blockIdx.z * RZC;
   double s = 0.0, cnt = 0.0;
   const T *__restrict__ u = P.u;
   const T *__restrict__ rho = P.rho;
@@ -291,8 +299,10 @@ __global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
   float *__restrict__ neg32 = P.neg32;
   if (i < nx && j < ny) {
     const bool inxy = i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1;
-#pragma unroll 4
-    for (int k = z0; k < z1; k++) {
+#pragma unroll
+    for (int kk = 0; kk < RZC; kk++) {
+      const int k = z0 + kk;
+      if (k >= P.nz) break;
       const size_t p = ix3(i, j, k, nx, ny);
       bool on = inxy && k >= P.bz0 && k <= P.bz1;
       if constexpr (EX) on = on && !P.chr[p];
@@ -319,13 +329,26 @@ __global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
 // weight.  Boundary points are also written to `alt` (the second ping-pong
 // buffer of u) when given.
 // ---------------------------------------------------------------------------
+// Elementwise kernels over a grid use 64 x 4 thread blocks over (x, y) and
+// one z plane per blockIdx.z: no 64-bit index division per point.
+constexpr int EX_ = 64, EY_ = 4;
+inline dim3 grid3(int nx, int ny, int nz) {
+  return dim3((nx + EX_ - 1) / EX_, (ny + EY_ - 1) / EY_, nz);
+}
+#define PMG_IJK(nx, ny)                                          \
+  const int i = blockIdx.x * EX_ + (int)(threadIdx.x % EX_);     \
+  const int j = blockIdx.y * EY_ + (int)(threadIdx.x / EX_);     \
+  const int k = blockIdx.z;                                      \
+  if (i >= (nx) || j >= (ny)) return;
+
 template <typename T, int KIND, bool EXC>
-__global__ __launch_bounds__(256) void k_restrict_whole(const T *u1, T *u2, T *alt,
-                                                        const T *chr1, T ex, int nx1, int ny1,
-                                                        int nz1, int nx2, int ny2, int nz2) {
-  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= (size_t)nx2 * ny2 * nz2) return;
-  const int i = (int)(q % nx2), j = (int)((q / nx2) % ny2), k = (int)(q / ((size_t)nx2 * ny2));
+__global__ __launch_bounds__(256) void k_restrict_whole(const T *__restrict__ u1,
+                                                        T *__restrict__ u2, T *__restrict__ alt,
+                                                        const T *__restrict__ chr1, T ex,
+                                                        int nx1, int ny1, int nz1, int nx2,
+                                                        int ny2, int nz2) {
+  PMG_IJK(nx2, ny2)
+  const size_t q = ix3(i, j, k, nx2, ny2);
   const int i0 = 1 + 2 * i, j0 = 1 + 2 * j, k0 = 1 + 2 * k;
   T v;
   if constexpr (KIND == 1) {
@@ -361,9 +384,8 @@ struct LopRhsP {
 template <typename T, int OP, bool EX, bool P2>
 __global__ __launch_bounds__(256) void k_lop_rhs(LopRhsP<T> P) {
   const int nx = P.nx, ny = P.ny, nz = P.nz;
-  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= (size_t)nx * ny * nz) return;
-  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+  PMG_IJK(nx, ny)
+  const size_t q = ix3(i, j, k, nx, ny);
   const bool on = i >= P.cx0 && i <= P.cx1 && j >= P.cy0 && j <= P.cy1 && k >= P.cz0 && k <= P.cz1;
   const bool interior = i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
   T lopv = T(0);
@@ -386,9 +408,8 @@ __global__ __launch_bounds__(256) void k_cgc(T *__restrict__ u, T *__restrict__ 
                                              const T *__restrict__ uc, const T *__restrict__ lopc,
                                              T w0_i, int nx, int ny, int nz, int cnx, int cny,
                                              const uint8_t *__restrict__ chr) {
-  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= (size_t)nx * ny * nz) return;
-  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+  PMG_IJK(nx, ny)
+  const size_t q = ix3(i, j, k, nx, ny);
   const int ai = i >> 1, ri = i & 1, aj = j >> 1, rj = j & 1, ak = k >> 1, rk = k & 1;
   const T c0 = T(1 - (1 * 1.0) / 2), c1 = T((1 * 1.0) / 2);
   auto R = [&](int a, int b, int c) -> T {
@@ -417,9 +438,8 @@ __global__ __launch_bounds__(256) void k_cgc_diff(T *res, const T *u, const T *l
 template <typename T>
 __global__ __launch_bounds__(256) void k_cgc_add_ex(T *u, T *alt, T *res, const uint8_t *chr,
                                                     int nx, int ny, int nz) {
-  size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
-  if (q >= (size_t)nx * ny * nz) return;
-  const int i = (int)(q % nx), j = (int)((q / nx) % ny), k = (int)(q / ((size_t)nx * ny));
+  PMG_IJK(nx, ny)
+  const size_t q = ix3(i, j, k, nx, ny);
   T r = res[q];
   if (chr[q]) { r = T(0); res[q] = r; }
   const T nu = u[q] + r;

@@ -1,0 +1,695 @@
+// pmg_learned.hip -- the learned smoother / warm-start network on gfx950.
+//
+// Network: 3 layers of 3x3x3 convolutions, 1 -> H -> H -> 1 channels
+// (H = 16), GELU (tanh form) after layers 1 and 2, zero padding.  It maps a
+// normalised residual r/s to a normalised error estimate; the solver applies
+// u += s * h^2 * N(r/s) and keeps the step only when the FP64 residual says
+// it contracts (pmg_solver's learned mode).
+//
+// Kernels are implicit GEMMs on the MFMA units: every layer is computed as a
+// 16 -> 16 channel convolution in BF16 with FP32 accumulation
+// (v_mfma_f32_16x16x16_bf16): a wavefront owns 16 consecutive points of an
+// x row and all 16 output channels, and accumulates the 27 taps as 27 MFMAs
+// with K = 16 input channels.  Layer 1's single input channel and layer 3's
+// single output channel are zero-padded to 16 (their padding rows/columns
+// stay exactly zero under training).  Activations are channel-last BF16
+// [point][16]; pre-activations FP32.
+//
+// Training kernels: backward-data is the same convolution with flipped taps
+// and transposed weights, fused with the GELU derivative; backward-weight is
+// a GEMM over points (K = points) per tap with per-block partials folded
+// deterministically; Adam on FP32 master weights.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "pmg_common.h"
+#include "../../include/pamr_mg_gpu.h"
+
+using namespace pmg;
+
+namespace {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+constexpr int NC = 16;      // channels (padded)
+constexpr int NTAP = 27;
+
+__host__ __device__ inline uint16_t f2bf(float f) {
+  uint32_t u;
+  __builtin_memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);  // inf / nan
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (uint16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t b) {
+  uint32_t u = (uint32_t)b << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+
+__device__ inline float gelu_f(float x) {
+  const float k = 0.7978845608028654f;
+  return 0.5f * x * (1.0f + tanhf(k * (x + 0.044715f * x * x * x)));
+}
+__device__ inline float gelu_grad_f(float x) {
+  const float k = 0.7978845608028654f;
+  const float x2 = x * x;
+  const float t = tanhf(k * (x + 0.044715f * x2 * x));
+  return 0.5f * (1.0f + t) + 0.5f * x * (1.0f - t * t) * k * (1.0f + 3.0f * 0.044715f * x2);
+}
+
+struct Geom {
+  int nx, ny, nz, nb;  // box shape and number of boxes (batch)
+};
+
+// ---------------------------------------------------------------------------
+// MFMA lane layout self-test: D = A * B for 16x16 bf16 operands.  Lane l
+// holds A[l%16][4*(l/16)+j], B[4*(l/16)+j][l%16] and D[4*(l/16)+i][l%16].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t *A, const uint16_t *B, float *D) {
+  const int l = threadIdx.x, m = l & 15, g = l >> 4;
+  bf16x4 a, b;
+  for (int j = 0; j < 4; j++) {
+    a[j] = (short)A[m * 16 + 4 * g + j];
+    b[j] = (short)B[(4 * g + j) * 16 + m];
+  }
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  for (int i = 0; i < 4; i++) D[(4 * g + i) * 16 + m] = acc[i];
+}
+
+// ---------------------------------------------------------------------------
+// 16 -> 16 convolution.  Grid (ceil(nx/16), ceil(ny/4), nz*nb), 4 waves per
+// block, one x row segment of 16 points per wave.
+//   EPI 0: z = conv + bias -> Z (fp32), A = bf16(gelu(z))
+//   EPI 1: z = conv + bias -> Z (fp32) only (last layer)
+//   EPI 2: backward data: v = conv * gelu'(aux) -> Z (fp32), A = bf16(v)
+// Wf: weight fragments [27][64 lanes][4] bf16 (B operand order).
+// ---------------------------------------------------------------------------
+template <int EPI>
+__global__ __launch_bounds__(256) void k_conv16(const uint16_t *__restrict__ X,
+                                                const uint16_t *__restrict__ Wf,
+                                                const float *__restrict__ bias,
+                                                const float *__restrict__ aux,
+                                                float *__restrict__ Zout,
+                                                uint16_t *__restrict__ Aout, Geom g) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int x0 = blockIdx.x * 16, y = blockIdx.y * 4 + w;
+  const int z = blockIdx.z % g.nz, b = blockIdx.z / g.nz;
+  if (y >= g.ny) return;  // uniform per wave; no block barrier below
+  const int m = lane & 15, grp = lane >> 4;
+  const size_t box = (size_t)g.nx * g.ny * g.nz;
+  const uint16_t *Xb = X + (size_t)b * box * NC;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 3
+  for (int tap = 0; tap < NTAP; tap++) {
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    const int xx = x0 + m + dx, yy = y + dy, zz = z + dz;
+    bf16x4 a = {0, 0, 0, 0};
+    if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
+      a = *(const bf16x4 *)(Xb + (((size_t)zz * g.ny + yy) * g.nx + xx) * NC + 4 * grp);
+    const bf16x4 bw = *(const bf16x4 *)(Wf + ((size_t)tap * 64 + lane) * 4);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int xx = x0 + 4 * grp + i;
+    if (xx >= g.nx) continue;
+    const size_t p = ((size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx) * NC + m;
+    float v = acc[i];
+    if constexpr (EPI == 0) {
+      v += bias[m];
+      Zout[p] = v;
+      Aout[p] = f2bf(gelu_f(v));
+    } else if constexpr (EPI == 1) {
+      Zout[p] = v + bias[m];
+    } else {
+      v *= gelu_grad_f(aux[p]);
+      Zout[p] = v;
+      Aout[p] = f2bf(v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weight gradient: dW[tap][ci][co] = sum_p X[p+off(tap)][ci] * dZ[p][co];
+// tap 27 is the bias (input channel 0 == 1).  Grid (nchunk, 28): block
+// (chunk, tap), 4 waves each summing every 4th group of 16 points of the
+// chunk with MFMA (M = ci, N = co, K = points), then a fixed-order fold of
+// the 4 waves into partial[chunk][tap][16][16].
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_conv16_wgrad(const uint16_t *__restrict__ X,
+                                                      const uint16_t *__restrict__ dZ,
+                                                      float *__restrict__ partial, Geom g,
+                                                      int chunk) {
+  __shared__ float red[4][256];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int tap = blockIdx.y;
+  const int m = lane & 15, grp = lane >> 4;
+  const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+  const size_t box = (size_t)g.nx * g.ny * g.nz;
+  const size_t total = box * g.nb;
+  const size_t p0 = (size_t)blockIdx.x * chunk;
+  const size_t p1 = p0 + chunk < total ? p0 + chunk : total;
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (size_t base = p0 + 16 * w; base < p1; base += 64) {
+    bf16x4 a = {0, 0, 0, 0}, bb = {0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const size_t p = base + 4 * grp + j;
+      if (p >= p1) continue;
+      bb[j] = (short)dZ[p * NC + m];
+      if (tap == NTAP) {
+        a[j] = (short)(m == 0 ? 0x3f80 : 0);  // bf16(1.0) on channel 0
+      } else {
+        const size_t q = p % box;
+        const size_t bidx = p / box;
+        const int x = (int)(q % g.nx), y = (int)((q / g.nx) % g.ny), z = (int)(q / ((size_t)g.nx * g.ny));
+        const int xx = x + dx, yy = y + dy, zz = z + dz;
+        if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
+          a[j] = (short)X[(bidx * box + ((size_t)zz * g.ny + yy) * g.nx + xx) * NC + m];
+      }
+    }
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bb, acc, 0, 0, 0);
+  }
+  // lane holds D[ci = 4*grp + i][co = m]
+#pragma unroll
+  for (int i = 0; i < 4; i++) red[w][(4 * grp + i) * 16 + m] = acc[i];
+  __syncthreads();
+  const int e = threadIdx.x;  // 256 entries
+  const float v = ((red[0][e] + red[1][e]) + red[2][e]) + red[3][e];
+  partial[((size_t)blockIdx.x * (NTAP + 1) + tap) * 256 + e] = v;
+}
+
+// fold partials over chunks (fixed order) -> grad[28][256]
+__global__ __launch_bounds__(256) void k_fold_wgrad(const float *__restrict__ partial,
+                                                    float *__restrict__ grad, int nchunk) {
+  const int tap = blockIdx.x, e = threadIdx.x;
+  float s = 0.f;
+  for (int c = 0; c < nchunk; c++) s += partial[((size_t)c * (NTAP + 1) + tap) * 256 + e];
+  grad[tap * 256 + e] = s;
+}
+
+// loss = 0.5 * mean over counted points of (Z[p][0] - T[p])^2 ; dZ[p][0] =
+// (Z - T)/count (bf16), other channels 0.  Points within `margin` of a box
+// face are not counted.  Per-block partial sums of the squared error.
+__global__ __launch_bounds__(256) void k_loss(const float *__restrict__ Z, const float *__restrict__ T,
+                                              uint16_t *__restrict__ dZ, double *__restrict__ part,
+                                              Geom g, int margin, double inv_count) {
+  __shared__ double red[256];
+  const size_t box = (size_t)g.nx * g.ny * g.nz;
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double e2 = 0.0;
+  if (p < box * g.nb) {
+    const size_t q = p % box;
+    const int x = (int)(q % g.nx), y = (int)((q / g.nx) % g.ny), z = (int)(q / ((size_t)g.nx * g.ny));
+    const bool on = x >= margin && x < g.nx - margin && y >= margin && y < g.ny - margin &&
+                    z >= margin && z < g.nz - margin;
+    float d = 0.f;
+    if (on) {
+      d = Z[p * NC] - T[p];
+      e2 = (double)d * (double)d;
+    }
+    dZ[p * NC] = f2bf((float)(d * inv_count));
+#pragma unroll
+    for (int c = 1; c < NC; c++) dZ[p * NC + c] = 0;
+  }
+  red[threadIdx.x] = e2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_fold_d(const double *__restrict__ part, int n, double *out) {
+  __shared__ double red[256];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// input packing: X[p][0] = bf16(src[p] * scale), channels 1..15 = 0
+template <typename TS>
+__global__ __launch_bounds__(256) void k_pack_input(const TS *__restrict__ src, uint16_t *__restrict__ X,
+                                                    size_t n, float scale) {
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (p >= n) return;
+  bf16x4 v = {0, 0, 0, 0};
+  v[0] = (short)f2bf((float)src[p] * scale);
+  const bf16x4 zero = {0, 0, 0, 0};
+  bf16x4 *d = (bf16x4 *)(X + p * NC);
+  d[0] = v;
+  d[1] = zero;
+  d[2] = zero;
+  d[3] = zero;
+}
+
+// Adam on FP32 master weights (only the entries `mask` marks trainable)
+__global__ __launch_bounds__(256) void k_adam(float *__restrict__ w, float *__restrict__ m1,
+                                              float *__restrict__ m2, const float *__restrict__ gr,
+                                              const uint8_t *__restrict__ mask, int n, float lr,
+                                              float b1, float b2, float c1, float c2, float eps) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n || !mask[i]) return;
+  const float g = gr[i];
+  const float a = b1 * m1[i] + (1.f - b1) * g;
+  const float b = b2 * m2[i] + (1.f - b2) * g * g;
+  m1[i] = a;
+  m2[i] = b;
+  w[i] -= lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
+// master layout per layer: W[tap][ci][co] (27*256 floats) then bias[co]
+// (16 floats), i.e. 28 rows of 256 with the bias in row 27, entries 0..15
+constexpr int LAYER_FLOATS = (NTAP + 1) * 256;
+
+// fragments: forward Wf[tap][lane][j] = W[tap][4*(lane/16)+j][lane%16];
+// backward-data (flipped taps, transposed): Wb[tap][lane][j] =
+// W[26-tap][lane%16][4*(lane/16)+j]
+__global__ __launch_bounds__(256) void k_pack_frags(const float *__restrict__ W, uint16_t *__restrict__ Wf,
+                                                    uint16_t *__restrict__ Wb) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;  // over 27*64*4
+  if (idx >= NTAP * 64 * 4) return;
+  const int tap = idx / 256, lane = (idx / 4) % 64, j = idx % 4;
+  const int m = lane & 15, grp = lane >> 4;
+  Wf[idx] = f2bf(W[tap * 256 + (4 * grp + j) * 16 + m]);
+  Wb[idx] = f2bf(W[(26 - tap) * 256 + m * 16 + (4 * grp + j)]);
+}
+
+// u += coef * Z[p][0] at interior points of the grid (box = the grid)
+template <typename TU>
+__global__ __launch_bounds__(256) void k_apply_correction(TU *__restrict__ u, const float *__restrict__ Z,
+                                                          int nx, int ny, int nz, double coef,
+                                                          TU *__restrict__ alt) {
+  const int i = blockIdx.x * 64 + (int)(threadIdx.x % 64);
+  const int j = blockIdx.y * 4 + (int)(threadIdx.x / 64);
+  const int k = blockIdx.z;
+  if (i < 1 || j < 1 || k < 1 || i >= nx - 1 || j >= ny - 1 || k >= nz - 1) return;
+  const size_t p = ix3(i, j, k, nx, ny);
+  u[p] = u[p] + (TU)(coef * (double)Z[p * NC]);
+  (void)alt;
+}
+
+}  // namespace
+
+// ===========================================================================
+// network object
+// ===========================================================================
+struct pmg_net {
+  int hidden = 16;
+  float *w = nullptr;       // 3 layers * LAYER_FLOATS master weights (device)
+  float *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
+  uint8_t *mask = nullptr;  // trainable entries
+  uint16_t *wf = nullptr, *wb = nullptr;  // 3 layers * 27*64*4 fragments each
+  int adam_t = 0;
+  // activation buffers (grown on demand): X0, A1, A2, dA (bf16 [P][16]); Z1, Z2, Z3, dZ (fp32)
+  size_t cap = 0;
+  uint16_t *x0 = nullptr, *a1 = nullptr, *a2 = nullptr, *d16 = nullptr, *d16b = nullptr;
+  float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *dz = nullptr;
+  float *partial = nullptr;
+  size_t partial_cap = 0;
+  double *lpart = nullptr, *lsum = nullptr;
+  size_t lpart_cap = 0;
+  hipStream_t stream = nullptr;
+  bool frags_valid = false;
+};
+
+namespace {
+
+int net_ensure(pmg_net *N, size_t P) {
+  if (P <= N->cap) return 0;
+  PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+  for (void *p : {(void *)N->x0, (void *)N->a1, (void *)N->a2, (void *)N->d16, (void *)N->d16b,
+                  (void *)N->z1, (void *)N->z2, (void *)N->z3, (void *)N->dz})
+    if (p) PMG_HOST_CHECK(hipFree(p));
+  const size_t b16 = P * NC * sizeof(uint16_t), b32 = P * NC * sizeof(float);
+  PMG_HOST_CHECK(hipMalloc(&N->x0, b16));
+  PMG_HOST_CHECK(hipMalloc(&N->a1, b16));
+  PMG_HOST_CHECK(hipMalloc(&N->a2, b16));
+  PMG_HOST_CHECK(hipMalloc(&N->d16, b16));
+  PMG_HOST_CHECK(hipMalloc(&N->d16b, b16));
+  PMG_HOST_CHECK(hipMalloc(&N->z1, b32));
+  PMG_HOST_CHECK(hipMalloc(&N->z2, b32));
+  PMG_HOST_CHECK(hipMalloc(&N->z3, b32));
+  PMG_HOST_CHECK(hipMalloc(&N->dz, b32));
+  N->cap = P;
+  return 0;
+}
+
+int net_frags(pmg_net *N) {
+  if (N->frags_valid) return 0;
+  for (int l = 0; l < 3; l++) {
+    hipLaunchKernelGGL(k_pack_frags, dim3((NTAP * 256 + 255) / 256), dim3(256), 0, N->stream,
+                       N->w + (size_t)l * LAYER_FLOATS, N->wf + (size_t)l * NTAP * 256,
+                       N->wb + (size_t)l * NTAP * 256);
+  }
+  PMG_HOST_CHECK(hipGetLastError());
+  N->frags_valid = true;
+  return 0;
+}
+
+dim3 conv_grid(const Geom &g) { return dim3((g.nx + 15) / 16, (g.ny + 3) / 4, g.nz * g.nb); }
+
+// forward of the 3 layers on X0 (already packed); Z3[p][0] is the output
+int net_forward(pmg_net *N, const Geom &g) {
+  int rc;
+  if ((rc = net_frags(N))) return rc;
+  const dim3 gr = conv_grid(g);
+  const float *b1 = N->w + 0 * LAYER_FLOATS + NTAP * 256;
+  const float *b2 = N->w + 1 * LAYER_FLOATS + NTAP * 256;
+  const float *b3 = N->w + 2 * LAYER_FLOATS + NTAP * 256;
+  hipLaunchKernelGGL(k_conv16<0>, gr, dim3(256), 0, N->stream, N->x0, N->wf, b1, (const float *)nullptr,
+                     N->z1, N->a1, g);
+  hipLaunchKernelGGL(k_conv16<0>, gr, dim3(256), 0, N->stream, N->a1, N->wf + NTAP * 256, b2,
+                     (const float *)nullptr, N->z2, N->a2, g);
+  hipLaunchKernelGGL(k_conv16<1>, gr, dim3(256), 0, N->stream, N->a2, N->wf + 2 * NTAP * 256, b3,
+                     (const float *)nullptr, N->z3, (uint16_t *)nullptr, g);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int net_wgrad(pmg_net *N, const Geom &g, const uint16_t *X, const uint16_t *dZ, float *grad_out) {
+  const size_t total = (size_t)g.nx * g.ny * g.nz * g.nb;
+  const int chunk = 4096;
+  const int nchunk = (int)((total + chunk - 1) / chunk);
+  const size_t need = (size_t)nchunk * (NTAP + 1) * 256;
+  if (need > N->partial_cap) {
+    PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+    if (N->partial) PMG_HOST_CHECK(hipFree(N->partial));
+    PMG_HOST_CHECK(hipMalloc(&N->partial, need * sizeof(float)));
+    N->partial_cap = need;
+  }
+  hipLaunchKernelGGL(k_conv16_wgrad, dim3(nchunk, NTAP + 1), dim3(256), 0, N->stream, X, dZ, N->partial, g,
+                     chunk);
+  hipLaunchKernelGGL(k_fold_wgrad, dim3(NTAP + 1), dim3(256), 0, N->stream, N->partial, grad_out, nchunk);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+// forward + loss + backward; gradients into N->grad (3 * LAYER_FLOATS)
+int net_loss_backward(pmg_net *N, const Geom &g, const float *T, int margin, double *loss_out) {
+  int rc;
+  if ((rc = net_forward(N, g))) return rc;
+  const size_t total = (size_t)g.nx * g.ny * g.nz * g.nb;
+  const int nblk = (int)((total + 255) / 256);
+  if ((size_t)nblk > N->lpart_cap) {
+    PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+    if (N->lpart) PMG_HOST_CHECK(hipFree(N->lpart));
+    PMG_HOST_CHECK(hipMalloc(&N->lpart, (size_t)nblk * sizeof(double)));
+    N->lpart_cap = nblk;
+  }
+  const long long cx = g.nx - 2 * margin, cy = g.ny - 2 * margin, cz = g.nz - 2 * margin;
+  const double count = (double)(cx > 0 ? cx : 0) * (cy > 0 ? cy : 0) * (cz > 0 ? cz : 0) * g.nb;
+  hipLaunchKernelGGL(k_loss, dim3(nblk), dim3(256), 0, N->stream, N->z3, T, N->d16, N->lpart, g, margin,
+                     1.0 / count);
+  hipLaunchKernelGGL(k_fold_d, dim3(1), dim3(256), 0, N->stream, N->lpart, nblk, N->lsum);
+  PMG_HOST_CHECK(hipGetLastError());
+  const dim3 gr = conv_grid(g);
+  // layer 3: dW3 from A2 and dZ3; dZ2 = conv^T(dZ3) * gelu'(Z2)
+  if ((rc = net_wgrad(N, g, N->a2, N->d16, N->grad + 2 * LAYER_FLOATS))) return rc;
+  hipLaunchKernelGGL(k_conv16<2>, gr, dim3(256), 0, N->stream, N->d16, N->wb + 2 * NTAP * 256,
+                     (const float *)nullptr, N->z2, N->dz, N->d16b, g);
+  // layer 2
+  if ((rc = net_wgrad(N, g, N->a1, N->d16b, N->grad + 1 * LAYER_FLOATS))) return rc;
+  hipLaunchKernelGGL(k_conv16<2>, gr, dim3(256), 0, N->stream, N->d16b, N->wb + 1 * NTAP * 256,
+                     (const float *)nullptr, N->z1, N->dz, N->d16, g);
+  // layer 1
+  if ((rc = net_wgrad(N, g, N->x0, N->d16, N->grad))) return rc;
+  PMG_HOST_CHECK(hipGetLastError());
+  if (loss_out) {
+    double l;
+    PMG_HOST_CHECK(hipMemcpyAsync(&l, N->lsum, sizeof(double), hipMemcpyDeviceToHost, N->stream));
+    PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+    *loss_out = 0.5 * l / count;
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmg_net_mfma_selftest(const float *A, const float *B, float *D_out) {
+  // host A (16x16), B (16x16) rounded to bf16; D_out = A*B from one MFMA
+  std::vector<uint16_t> ha(256), hb(256);
+  for (int i = 0; i < 256; i++) { ha[i] = f2bf(A[i]); hb[i] = f2bf(B[i]); }
+  uint16_t *dA, *dB;
+  float *dD;
+  PMG_HOST_CHECK(hipMalloc(&dA, 512));
+  PMG_HOST_CHECK(hipMalloc(&dB, 512));
+  PMG_HOST_CHECK(hipMalloc(&dD, 1024));
+  PMG_HOST_CHECK(hipMemcpy(dA, ha.data(), 512, hipMemcpyHostToDevice));
+  PMG_HOST_CHECK(hipMemcpy(dB, hb.data(), 512, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mfma_probe, dim3(1), dim3(64), 0, 0, dA, dB, dD);
+  PMG_HOST_CHECK(hipGetLastError());
+  PMG_HOST_CHECK(hipMemcpy(D_out, dD, 1024, hipMemcpyDeviceToHost));
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dD);
+  return 0;
+}
+
+pmg_net *pmg_net_create(int hidden, unsigned long long seed) {
+  if (hidden != 16) return nullptr;
+  pmg_net *N = new pmg_net();
+  if (hipStreamCreate(&N->stream) != hipSuccess) { delete N; return nullptr; }
+  const size_t nf = 3 * (size_t)LAYER_FLOATS;
+  std::vector<float> w(nf, 0.f);
+  std::vector<uint8_t> mask(nf, 0);
+  // splitmix64-driven uniform init, He-like scale; padding stays zero
+  unsigned long long st = seed;
+  auto rnd = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    unsigned long long z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+  };
+  const int cin[3] = {1, 16, 16}, cout[3] = {16, 16, 1};
+  for (int l = 0; l < 3; l++) {
+    const double sc = std::sqrt(3.0 / (27.0 * cin[l]));
+    for (int t = 0; t < NTAP; t++)
+      for (int ci = 0; ci < cin[l]; ci++)
+        for (int co = 0; co < cout[l]; co++) {
+          const size_t idx = (size_t)l * LAYER_FLOATS + t * 256 + ci * 16 + co;
+          w[idx] = (float)((2.0 * rnd() - 1.0) * sc);
+          mask[idx] = 1;
+        }
+    for (int co = 0; co < cout[l]; co++) mask[(size_t)l * LAYER_FLOATS + NTAP * 256 + co] = 1;
+  }
+  bool ok = hipMalloc(&N->w, nf * 4) == hipSuccess && hipMalloc(&N->m1, nf * 4) == hipSuccess &&
+            hipMalloc(&N->m2, nf * 4) == hipSuccess && hipMalloc(&N->grad, nf * 4) == hipSuccess &&
+            hipMalloc(&N->mask, nf) == hipSuccess &&
+            hipMalloc(&N->wf, 3 * NTAP * 256 * sizeof(uint16_t)) == hipSuccess &&
+            hipMalloc(&N->wb, 3 * NTAP * 256 * sizeof(uint16_t)) == hipSuccess &&
+            hipMalloc(&N->lsum, sizeof(double)) == hipSuccess;
+  if (!ok) { pmg_net_destroy(N); return nullptr; }
+  (void)hipMemcpy(N->w, w.data(), nf * 4, hipMemcpyHostToDevice);
+  (void)hipMemcpy(N->mask, mask.data(), nf, hipMemcpyHostToDevice);
+  (void)hipMemset(N->m1, 0, nf * 4);
+  (void)hipMemset(N->m2, 0, nf * 4);
+  (void)hipMemset(N->grad, 0, nf * 4);
+  return N;
+}
+
+void pmg_net_destroy(pmg_net *N) {
+  if (!N) return;
+  if (N->stream) (void)hipStreamSynchronize(N->stream);
+  for (void *p : {(void *)N->w, (void *)N->m1, (void *)N->m2, (void *)N->grad, (void *)N->mask,
+                  (void *)N->wf, (void *)N->wb, (void *)N->x0, (void *)N->a1, (void *)N->a2,
+                  (void *)N->d16, (void *)N->d16b, (void *)N->z1, (void *)N->z2, (void *)N->z3,
+                  (void *)N->dz, (void *)N->partial, (void *)N->lpart, (void *)N->lsum})
+    if (p) (void)hipFree(p);
+  if (N->stream) (void)hipStreamDestroy(N->stream);
+  delete N;
+}
+
+// Weights in the oracle's layout (oracle/mg_oracle.h): w1[16][1][27], b1[16],
+// w2[16][16][27], b2[16], w3[1][16][27], b3[1]; FP64 on the host.
+int pmg_net_set_weights(pmg_net *N, const double *w1, const double *b1, const double *w2,
+                        const double *b2, const double *w3, const double *b3) {
+  if (!N) return -1;
+  std::vector<float> w(3 * (size_t)LAYER_FLOATS, 0.f);
+  for (int co = 0; co < 16; co++) {
+    for (int t = 0; t < NTAP; t++) {
+      w[0 * LAYER_FLOATS + t * 256 + 0 * 16 + co] = (float)w1[co * 27 + t];
+      for (int ci = 0; ci < 16; ci++)
+        w[1 * LAYER_FLOATS + t * 256 + ci * 16 + co] = (float)w2[(co * 16 + ci) * 27 + t];
+    }
+    w[0 * LAYER_FLOATS + NTAP * 256 + co] = (float)b1[co];
+    w[1 * LAYER_FLOATS + NTAP * 256 + co] = (float)b2[co];
+  }
+  for (int ci = 0; ci < 16; ci++)
+    for (int t = 0; t < NTAP; t++) w[2 * LAYER_FLOATS + t * 256 + ci * 16 + 0] = (float)w3[ci * 27 + t];
+  w[2 * LAYER_FLOATS + NTAP * 256 + 0] = (float)b3[0];
+  PMG_HOST_CHECK(hipMemcpy(N->w, w.data(), w.size() * 4, hipMemcpyHostToDevice));
+  N->frags_valid = false;
+  return 0;
+}
+
+static void from_master(const std::vector<float> &w, double *w1, double *b1, double *w2, double *b2,
+                        double *w3, double *b3) {
+  for (int co = 0; co < 16; co++) {
+    for (int t = 0; t < NTAP; t++) {
+      w1[co * 27 + t] = w[0 * LAYER_FLOATS + t * 256 + co];
+      for (int ci = 0; ci < 16; ci++) w2[(co * 16 + ci) * 27 + t] = w[1 * LAYER_FLOATS + t * 256 + ci * 16 + co];
+    }
+    b1[co] = w[0 * LAYER_FLOATS + NTAP * 256 + co];
+    b2[co] = w[1 * LAYER_FLOATS + NTAP * 256 + co];
+  }
+  for (int ci = 0; ci < 16; ci++)
+    for (int t = 0; t < NTAP; t++) w3[ci * 27 + t] = w[2 * LAYER_FLOATS + t * 256 + ci * 16];
+  b3[0] = w[2 * LAYER_FLOATS + NTAP * 256];
+}
+
+int pmg_net_get_weights(pmg_net *N, double *w1, double *b1, double *w2, double *b2, double *w3,
+                        double *b3) {
+  if (!N) return -1;
+  std::vector<float> w(3 * (size_t)LAYER_FLOATS);
+  PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+  PMG_HOST_CHECK(hipMemcpy(w.data(), N->w, w.size() * 4, hipMemcpyDeviceToHost));
+  from_master(w, w1, b1, w2, b2, w3, b3);
+  return 0;
+}
+
+// Forward on nb boxes of nx*ny*nz points: x (device fp32 [nb][P]) -> y
+// (device fp32 [nb][P]).
+int pmg_net_forward(pmg_net *N, const float *x, float *y, int nx, int ny, int nz, int nb) {
+  if (!N || !x || !y || nx < 1 || ny < 1 || nz < 1 || nb < 1) return -1;
+  const size_t P = (size_t)nx * ny * nz * nb;
+  if (net_ensure(N, P)) return -1;
+  Geom g{nx, ny, nz, nb};
+  hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, N->stream, x, N->x0,
+                     P, 1.0f);
+  int rc = net_forward(N, g);
+  if (rc) return rc;
+  // y = channel 0 of Z3
+  PMG_HOST_CHECK(hipMemcpy2DAsync(y, sizeof(float), N->z3, NC * sizeof(float), sizeof(float), P,
+                                  hipMemcpyDeviceToDevice, N->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+  return 0;
+}
+
+// Loss and gradients (oracle layout, host FP64) for inputs x and targets t
+// (device fp32 [nb][P]); margin = points near box faces excluded from the loss.
+int pmg_net_loss_grad(pmg_net *N, const float *x, const float *t, int nx, int ny, int nz, int nb,
+                      int margin, double *loss, double *gw1, double *gb1, double *gw2, double *gb2,
+                      double *gw3, double *gb3) {
+  if (!N || !x || !t) return -1;
+  const size_t P = (size_t)nx * ny * nz * nb;
+  if (net_ensure(N, P)) return -1;
+  Geom g{nx, ny, nz, nb};
+  hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, N->stream, x, N->x0,
+                     P, 1.0f);
+  int rc = net_loss_backward(N, g, t, margin, loss);
+  if (rc) return rc;
+  std::vector<float> gr(3 * (size_t)LAYER_FLOATS);
+  PMG_HOST_CHECK(hipMemcpy(gr.data(), N->grad, gr.size() * 4, hipMemcpyDeviceToHost));
+  from_master(gr, gw1, gb1, gw2, gb2, gw3, gb3);
+  return 0;
+}
+
+// One Adam step on a batch (device fp32 inputs/targets).
+int pmg_net_train_step(pmg_net *N, const float *x, const float *t, int nx, int ny, int nz, int nb,
+                       int margin, float lr, double *loss) {
+  if (!N || !x || !t) return -1;
+  const size_t P = (size_t)nx * ny * nz * nb;
+  if (net_ensure(N, P)) return -1;
+  Geom g{nx, ny, nz, nb};
+  hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, N->stream, x, N->x0,
+                     P, 1.0f);
+  int rc = net_loss_backward(N, g, t, margin, loss);
+  if (rc) return rc;
+  N->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)N->adam_t), c2 = 1.f - std::pow(b2, (float)N->adam_t);
+  const int nf = 3 * LAYER_FLOATS;
+  hipLaunchKernelGGL(k_adam, dim3((nf + 255) / 256), dim3(256), 0, N->stream, N->w, N->m1, N->m2, N->grad,
+                     N->mask, nf, lr, b1, b2, c1, c2, 1e-8f);
+  PMG_HOST_CHECK(hipGetLastError());
+  N->frags_valid = false;
+  return 0;
+}
+
+void *pmg_net_stream(pmg_net *N) { return N ? (void *)N->stream : nullptr; }
+
+// Learned correction on one grid, on the caller's stream: y = N(scale * x)
+// (x device FP32, e.g. the negated defect the mixed solver keeps), then
+// u += coef * y at interior points.  The output stays in the network's
+// buffer so pmg_net_recorrect can undo or repeat the step.
+int pmg_net_correct(pmg_net *N, const float *x, float scale, double *u, double coef, int nx,
+                    int ny, int nz, void *stream) {
+  if (!N || !x || !u) return -1;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t P = (size_t)nx * ny * nz;
+  if (net_ensure(N, P)) return -1;
+  hipStream_t keep = N->stream;
+  N->stream = st;
+  Geom g{nx, ny, nz, 1};
+  hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, x, N->x0, P,
+                     scale);
+  int rc = net_forward(N, g);
+  N->stream = keep;
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_apply_correction<double>, dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0, st, u,
+                     N->z3, nx, ny, nz, coef, (double *)nullptr);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int pmg_net_recorrect(pmg_net *N, double *u, double coef, int nx, int ny, int nz, void *stream) {
+  if (!N || !u || (size_t)nx * ny * nz > N->cap) return -1;
+  hipLaunchKernelGGL(k_apply_correction<double>, dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0,
+                     (hipStream_t)stream, u, N->z3, nx, ny, nz, coef, (double *)nullptr);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+// Crops for training: out[c][z][y][x] = scale * (a[p] - b[p]) (b may be
+// NULL) for cubes of side `side` at host-given origins (ox, oy, oz).
+__global__ void k_gather_crops(const double *__restrict__ a, const double *__restrict__ b, int nx, int ny,
+                               const int *__restrict__ org, int side, double scale, float *__restrict__ out) {
+  const int c = blockIdx.y;
+  const size_t n = (size_t)side * side * side;
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  const int x = (int)(q % side), y = (int)((q / side) % side), z = (int)(q / ((size_t)side * side));
+  const size_t p = ix3(org[3 * c] + x, org[3 * c + 1] + y, org[3 * c + 2] + z, nx, ny);
+  const double v = b ? a[p] - b[p] : a[p];
+  out[(size_t)c * n + q] = (float)(scale * v);
+}
+
+int pmg_gather_crops(const double *a, const double *b, int nx, int ny, int nz, const int *origins_host,
+                     int ncrop, int side, double scale, float *out, void *stream) {
+  if (!a || !out || ncrop < 1 || side < 1) return -1;
+  for (int c = 0; c < ncrop; c++)
+    for (int d = 0; d < 3; d++) {
+      const int o = origins_host[3 * c + d], n = d == 0 ? nx : (d == 1 ? ny : nz);
+      if (o < 0 || o + side > n) return -1;
+    }
+  int *dorg;
+  PMG_HOST_CHECK(hipMalloc(&dorg, sizeof(int) * 3 * ncrop));
+  PMG_HOST_CHECK(hipMemcpyAsync(dorg, origins_host, sizeof(int) * 3 * ncrop, hipMemcpyHostToDevice,
+                                (hipStream_t)stream));
+  const size_t n = (size_t)side * side * side;
+  hipLaunchKernelGGL(k_gather_crops, dim3((unsigned)((n + 255) / 256), ncrop), dim3(256), 0, (hipStream_t)stream,
+                     a, b, nx, ny, dorg, side, scale, out);
+  PMG_HOST_CHECK(hipGetLastError());
+  PMG_HOST_CHECK(hipStreamSynchronize((hipStream_t)stream));
+  PMG_HOST_CHECK(hipFree(dorg));
+  return 0;
+}
+
+}  // extern "C"

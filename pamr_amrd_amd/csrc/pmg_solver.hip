@@ -17,6 +17,10 @@
 #include "pmg_kernels.h"
 #include "pmg_solver.h"
 
+extern "C" int pmg_net_correct(pmg_net *N, const float *x, float scale, double *u, double coef, int nx,
+                               int ny, int nz, void *stream);
+extern "C" int pmg_net_recorrect(pmg_net *N, double *u, double coef, int nx, int ny, int nz, void *stream);
+
 using namespace pmg;
 
 namespace {
@@ -128,6 +132,13 @@ struct pmg_solver {
   hipGraphExec_t g32 = nullptr;   // captured mixed-precision outer iteration
   bool fp32_ready = false;
   int last_citer = 0;
+  // learned mode
+  pmg_net *net = nullptr;
+  int net_enabled = 1;
+  int net_skip = 0;          // solves left before the network is tried again
+  int net_retry_period = 20;
+  double t_net_ms = -1, t_vc_ms = -1, rho_vc = 0.1;
+  int net_accepted = 0, net_rejected = 0;
 };
 
 // norm slot layout in d_norms
@@ -276,8 +287,8 @@ int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
 
 template <typename T, int OP, bool EX, bool P2>
 int launch_residual_t(pmg_solver *S, ResidP<T> P, bool rhs0, bool zero) {
-  P.zchunk = std::max(1, std::min(16, P.nz));
-  dim3 grid((P.nx + RTX - 1) / RTX, (P.ny + RTY - 1) / RTY, (P.nz + P.zchunk - 1) / P.zchunk);
+  P.zchunk = RZC;
+  dim3 grid((P.nx + RTX - 1) / RTX, (P.ny + RTY - 1) / RTY, (P.nz + RZC - 1) / RZC);
   const int nb = (int)(grid.x * grid.y * grid.z);
   P.nb = nb;
   if (nb > S->ws.cap) return -3;
@@ -325,7 +336,7 @@ int restrict_field(pmg_solver *S, int Lfine, const T *src, T *dst, T *alt, int k
   if (kind == 0) return 0;
   const size_t n = Cl.size;
   (void)use_ex;
-#define PMG_RW(K) hipLaunchKernelGGL((k_restrict_whole<T, K, false>), dim3(nblk256(n)), dim3(256), 0, S->stream, src, dst, alt, (const T *)nullptr, T(0), F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2])
+#define PMG_RW(K) hipLaunchKernelGGL((k_restrict_whole<T, K, false>), grid3(Cl.n[0], Cl.n[1], Cl.n[2]), dim3(256), 0, S->stream, src, dst, alt, (const T *)nullptr, T(0), F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2])
   if (kind == 1) PMG_RW(1);
   else if (kind == 2) PMG_RW(2);
   else PMG_RW(3);
@@ -347,7 +358,7 @@ int restrict_field_ex64(pmg_solver *S, int Lfine, const double *src, double *dst
   Level &F = S->lev[Lfine - 1], &Cl = S->lev[Lfine - 2];
   const size_t n = Cl.size;
   const double *chrv = F.chrd;
-#define PMG_RWE(K) hipLaunchKernelGGL((k_restrict_whole<double, K, true>), dim3(nblk256(n)), dim3(256), 0, S->stream, src, dst, alt, chrv, S->ex, F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2])
+#define PMG_RWE(K) hipLaunchKernelGGL((k_restrict_whole<double, K, true>), grid3(Cl.n[0], Cl.n[1], Cl.n[2]), dim3(256), 0, S->stream, src, dst, alt, chrv, S->ex, F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2])
   if (kind == 1) PMG_RWE(1);
   else if (kind == 2) PMG_RWE(2);
   else PMG_RWE(3);
@@ -384,8 +395,8 @@ int lop_rhs(pmg_solver *S, int L) {
   P.w0_r = (T)S->w0_r;
   P.sp = spacing<T>(Lv);
   const bool p2 = P.sp.pow2 != 0;
-  const unsigned nb = nblk256(Lv.size);
-#define PMG_LO(OPV, EXV, P2V) hipLaunchKernelGGL((k_lop_rhs<T, OPV, EXV, P2V>), dim3(nb), dim3(256), 0, S->stream, P)
+  const dim3 nb = grid3(Lv.n[0], Lv.n[1], Lv.n[2]);
+#define PMG_LO(OPV, EXV, P2V) hipLaunchKernelGGL((k_lop_rhs<T, OPV, EXV, P2V>), nb, dim3(256), 0, S->stream, P)
   if (S->op == 0) {
     if (S->do_ex) { if (p2) PMG_LO(0, true, true); else PMG_LO(0, true, false); }
     else { if (p2) PMG_LO(0, false, true); else PMG_LO(0, false, false); }
@@ -413,7 +424,7 @@ int cgc(pmg_solver *S, int L) {
   bool ex = false;
   if constexpr (std::is_same<T, double>::value) ex = S->do_ex != 0;
   if (!ex) {
-    hipLaunchKernelGGL(k_cgc<T>, dim3(nblk256(F.size)), dim3(256), 0, S->stream, AF.u, AF.ua, AC.u,
+    hipLaunchKernelGGL(k_cgc<T>, grid3(F.n[0], F.n[1], F.n[2]), dim3(256), 0, S->stream, AF.u, AF.ua, AC.u,
                        AC.lop, (T)S->w0_i, F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1],
                        (const uint8_t *)F.chr);
     PMG_HOST_CHECK(hipGetLastError());
@@ -430,7 +441,7 @@ int cgc(pmg_solver *S, int L) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_clear_ex<double>, dim3(nblk256(Cl.size)), dim3(256), 0, S->stream, AC.res,
                        Cl.chr, Cl.size);
-    hipLaunchKernelGGL(k_cgc_add_ex<double>, dim3(nblk256(F.size)), dim3(256), 0, S->stream, AF.u,
+    hipLaunchKernelGGL(k_cgc_add_ex<double>, grid3(F.n[0], F.n[1], F.n[2]), dim3(256), 0, S->stream, AF.u,
                        AF.ua, AF.res, F.chr, F.n[0], F.n[1], F.n[2]);
     PMG_HOST_CHECK(hipGetLastError());
   }
@@ -658,7 +669,7 @@ pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidt
   // norm workspace sized for the largest launch
   // partial-norm capacity: the largest launch over the finest level
   const Level &F = S->lev[nl - 1];
-  const int rblk = ((F.n[0] + RTX - 1) / RTX) * ((F.n[1] + RTY - 1) / RTY) * ((F.n[2] + 15) / 16);
+  const int rblk = ((F.n[0] + RTX - 1) / RTX) * ((F.n[1] + RTY - 1) / RTY) * ((F.n[2] + RZC - 1) / RZC);
   const int sblk = ((F.n[0] + 61) / 64) * ((F.n[1] + 5) / 8) * ((F.n[2] + 5) / 8);
   int cap = std::max(1 << 14, std::max(rblk, sblk) + 64);
   S->ws.cap = cap;
@@ -935,10 +946,112 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
       if (hist) hist[it] = r;
       it++;
     }
+  } else if (mode == 2) {
+    // mixed precision with the learned correction before each FP32
+    // V-cycle, kept only when its contraction per unit time beats the
+    // plain cycle's (measured); otherwise undone and retried later
+    if (!S->net) return -1;
+    if ((rc = ensure_fp32(S))) return rc;
+    Level &F = S->lev[S->nlev - 1];
+    if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
+    const double h2 = F.hdx[0] * F.hdx[0];
+    hipEvent_t e0, e1;
+    PMG_HOST_CHECK(hipEventCreate(&e0));
+    PMG_HOST_CHECK(hipEventCreate(&e1));
+    auto read_outer = [&](double *out) -> int {
+      PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost, S->stream));
+      PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+      *out = S->h_pinned[0];
+      return 0;
+    };
+    const bool try_net = S->net_enabled && S->net_skip == 0;
+    if (S->net_skip > 0) S->net_skip--;
+    bool net_on = try_net;
+    while (it < max_iter && r > rtol * r0) {
+      if (net_on && r > 0) {
+        PMG_HOST_CHECK(hipEventRecord(e0, S->stream));
+        if ((rc = pmg_net_correct(S->net, F.f.rho, (float)(-1.0 / r), F.d.u, r * h2, F.n[0], F.n[1], F.n[2],
+                                  S->stream))) return rc;
+        if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
+        if ((rc = fold_norms(S, NS_OUTER, 1))) return rc;
+        PMG_HOST_CHECK(hipEventRecord(e1, S->stream));
+        double rn;
+        if ((rc = read_outer(&rn))) return rc;
+        float ms = 0.f;
+        PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));
+        S->t_net_ms = S->t_net_ms < 0 ? ms : 0.5 * (S->t_net_ms + ms);
+        const double tv = S->t_vc_ms > 0 ? S->t_vc_ms : ms;
+        const bool better = rn < r && (std::log(rn / r) / S->t_net_ms) < (std::log(S->rho_vc) / tv);
+        if (better) {
+          S->net_accepted++;
+          r = rn;
+          if (hist) hist[it] = r;
+          it++;
+          continue;
+        }
+        // undo: u -= coef * y; the defect is recomputed for the V-cycle
+        S->net_rejected++;
+        if ((rc = pmg_net_recorrect(S->net, F.d.u, -r * h2, F.n[0], F.n[1], F.n[2], S->stream))) return rc;
+        if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
+        net_on = false;
+        S->net_skip = S->net_retry_period;
+      }
+      PMG_HOST_CHECK(hipEventRecord(e0, S->stream));
+      if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
+      PMG_HOST_CHECK(hipEventRecord(e1, S->stream));
+      double rn;
+      if ((rc = read_outer(&rn))) return rc;
+      float ms = 0.f;
+      PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      S->t_vc_ms = S->t_vc_ms < 0 ? ms : 0.5 * (S->t_vc_ms + ms);
+      if (r > 0 && rn > 0) S->rho_vc = 0.5 * S->rho_vc + 0.5 * (rn / r);
+      r = rn;
+      if (hist) hist[it] = r;
+      it++;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
   } else {
     return -1;
   }
   if (iters) *iters = it;
+  return 0;
+}
+
+int pmg_set_net(pmg_solver *S, pmg_net *N) {
+  if (!S) return -1;
+  S->net = N;
+  S->net_enabled = 1;
+  S->net_skip = 0;
+  S->t_net_ms = -1;
+  return 0;
+}
+
+int pmg_net_stats(pmg_solver *S, int *accepted, int *rejected, double *t_net_ms, double *t_vc_ms) {
+  if (!S) return -1;
+  if (accepted) *accepted = S->net_accepted;
+  if (rejected) *rejected = S->net_rejected;
+  if (t_net_ms) *t_net_ms = S->t_net_ms;
+  if (t_vc_ms) *t_vc_ms = S->t_vc_ms;
+  return 0;
+}
+
+// u (finest, both sweep buffers) = ca * a + cb * b (device FP64, b may be NULL)
+__global__ __launch_bounds__(256) void k_lincomb(double *u, double *ua, const double *a, const double *b,
+                                                 double ca, double cb, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  const double v = b ? ca * a[q] + cb * b[q] : ca * a[q];
+  u[q] = v;
+  ua[q] = v;
+}
+
+int pmg_set_solution_lincomb(pmg_solver *S, const double *a, const double *b, double ca, double cb) {
+  if (!S || !a) return -1;
+  Level &F = S->lev[S->nlev - 1];
+  hipLaunchKernelGGL(k_lincomb, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.u, F.d.ua, a, b, ca, cb,
+                     F.size);
+  PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
 
