@@ -101,6 +101,28 @@ def _declare(L):
     L.or_mg_solve.restype = i
     L.or_mg_residual_fine.argtypes = [C.c_void_p]
     L.or_mg_residual_fine.restype = d
+    vp = C.c_void_p
+    L.or_amr_create.argtypes = [_ip, _dp, i, _ip, _dp, i, i]
+    L.or_amr_create.restype = vp
+    L.or_amr_destroy.argtypes = [vp]
+    for f in ("or_amr_nlev", "or_amr_base_level"):
+        getattr(L, f).argtypes = [vp]
+    L.or_amr_ngrids.argtypes = [vp, i]
+    L.or_amr_grid_info.argtypes = [vp, i, i, _ip, _dp]
+    L.or_amr_field.argtypes = [vp, i, i, i]
+    L.or_amr_field.restype = _dp
+    L.or_amr_set_params.argtypes = [vp, i, i, i, d, d, d, i, i, i, i]
+    L.or_amr_init.argtypes = [vp]
+    L.or_amr_vcycle.argtypes = [vp, _dp, _ip]
+    L.or_amr_vcycle.restype = d
+    L.or_amr_residual_fine.argtypes = [vp]
+    L.or_amr_residual_fine.restype = d
+    L.or_amr_owned.argtypes = [vp, i, i, i, _ip, _dp, i]
+    L.or_amr_transfers.argtypes = [vp, i, i, i, _ip, i]
+    L.or_amr_set_cmask.argtypes = [vp, i, i]
+    L.or_amr_sync_field.argtypes = [vp, i, i, i]
+    L.or_amr_inject_field.argtypes = [vp, i, i, i]
+    L.or_amr_interp_field.argtypes = [vp, i, i]
     L.or_net_forward.argtypes = [i] + [_dp] * 8 + [i, i, i]
     L.or_net_loss_grad.argtypes = [i] + [_dp] * 8 + [i, i, i] + [_dp] * 6
     L.or_net_loss_grad.restype = d
@@ -195,6 +217,96 @@ class UniformMG:
 
     def residual_fine(self):
         return self.L.or_mg_residual_fine(self.h)
+
+
+OA_U, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C = range(8)
+
+
+class AmrMG:
+    """Owning wrapper of the oracle's multigrid driver over an AMR hierarchy.
+
+    ``levels`` is a list (refined AMR levels 2, 3, ...) of lists of physical
+    boxes (x0, x1, y0, y1, z0, z1); the base grid covers ``bbox``."""
+
+    def __init__(self, base_n, levels, bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), mcw=5, op=0):
+        self.L = lib()
+        shape = (C.c_int * 3)(base_n, base_n, base_n)
+        bb = np.array(bbox, dtype=np.float64)
+        nbox = (C.c_int * max(1, len(levels)))(*[len(l) for l in levels])
+        boxes = np.array([b for l in levels for b in l] or [[0.0] * 6], dtype=np.float64).reshape(-1)
+        self.h = self.L.or_amr_create(shape, P(bb), len(levels), nbox, P(boxes), mcw, op)
+        self.nlev = self.L.or_amr_nlev(self.h)
+        self.base = self.L.or_amr_base_level(self.h)
+
+    def __del__(self):
+        try:
+            if self.h:
+                self.L.or_amr_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    def ngrids(self, lev):
+        return self.L.or_amr_ngrids(self.h, lev)
+
+    def grid(self, lev, g):
+        """(shape (nz, ny, nx), bbox) of grid g of MG level lev."""
+        s = (C.c_int * 3)()
+        b = np.zeros(6)
+        self.L.or_amr_grid_info(self.h, lev, g, s, P(b))
+        return (s[2], s[1], s[0]), b
+
+    def field(self, lev, g, which):
+        p = self.L.or_amr_field(self.h, lev, g, which)
+        if not p:
+            return None
+        shape = self.grid(lev, g)[0]
+        return np.ctypeslib.as_array(p, shape=(int(np.prod(shape)),)).reshape(shape)
+
+    def set_params(self, pre=2, pst=2, max_citer=5000, crtol=1e-3, w0_r=1.0, w0_i=1.0,
+                   rop_u=3, rop_res=3, rop_src=3, interp_ord=2):
+        self.L.or_amr_set_params(self.h, pre, pst, max_citer, crtol, w0_r, w0_i, rop_u, rop_res,
+                                 rop_src, interp_ord)
+
+    def init(self):
+        self.L.or_amr_init(self.h)
+
+    def vcycle(self):
+        norms = np.zeros(self.nlev)
+        cit = C.c_int(0)
+        r = self.L.or_amr_vcycle(self.h, P(norms), C.byref(cit))
+        return r, norms, cit.value
+
+    def residual_fine(self):
+        return self.L.or_amr_residual_fine(self.h)
+
+    def owned(self, lev, interior, extend):
+        cap = 4096
+        gi = (C.c_int * cap)()
+        bb = np.zeros(6 * cap)
+        n = self.L.or_amr_owned(self.h, lev, interior, extend, gi, P(bb), cap)
+        assert n <= cap
+        return [(gi[k], bb[6 * k:6 * k + 6].copy()) for k in range(n)]
+
+    def transfers(self, kind, lev, arg=0):
+        """rows of 16 ints: src g, dst g, delta, irho, src ibbox[6], dst ibbox[6]"""
+        cap = 1 << 16
+        out = (C.c_int * (16 * cap))()
+        n = self.L.or_amr_transfers(self.h, kind, lev, arg, out, cap)
+        assert n <= cap
+        return np.array(out[:16 * n], dtype=np.int64).reshape(n, 16)
+
+    def set_cmask(self, lev, which):
+        self.L.or_amr_set_cmask(self.h, lev, which)
+
+    def sync_field(self, lev, bdy, which):
+        self.L.or_amr_sync_field(self.h, lev, bdy, which)
+
+    def inject_field(self, lev_fine, which, op=3):
+        self.L.or_amr_inject_field(self.h, lev_fine, which, op)
+
+    def interp_field(self, lev_coarse, which):
+        self.L.or_amr_interp_field(self.h, lev_coarse, which)
 
 
 def net_forward(w1, b1, w2, b2, w3, b3, x):

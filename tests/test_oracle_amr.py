@@ -1,0 +1,175 @@
+"""Pins of the multi-patch oracle (oracle/amr_oracle.c): the multigrid driver
+over an AMR hierarchy and PAMR's sync / inject / interp bookkeeping.
+
+* A refined level that covers the whole domain turns the AMR hierarchy into a
+  uniform one: per-cycle norms and the solution must equal the uniform
+  oracle's bit for bit (which is itself pinned in test_oracle_closed_forms).
+* sync (src/pamr.c:601-617): brute force over the lattice -- every point takes
+  the value of the first grid (level order) whose interior, reduced by the
+  AMR boundary width at non-physical faces, contains it.
+* interp (src/pamr.c:623-638): a trilinear polynomial with dyadic
+  coefficients on the coarse grids is reproduced exactly at every fine point.
+* inject (src/pamr.c:667-685): an affine field on the fine grids is
+  reproduced exactly at every coarse point inside a fine grid, and no other
+  coarse point is written.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from pamr_amrd_amd import hierarchy as H
+
+
+def small_levels(seed, base_n=17, nref=2):
+    return H.random_boxes(base_n=base_n, nref=nref, nmin=3, nmax=6, smin=5, smax=13, seed=seed)
+
+
+def lattice_box(bbox, dx):
+    """integer lattice box (x0, x1, y0, y1, z0, z1) of a physical bbox"""
+    return [int(round(v / dx)) for v in bbox]
+
+
+def test_full_domain_refinement_equals_uniform():
+    n = 33
+    m = O.AmrMG(17, [[(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)]])
+    assert m.nlev == 4 and m.base == 3
+    u = O.UniformMG(n)
+    assert u.nlev == 4
+    rho = -H.gaussian_on((n, n, n), (0, 1, 0, 1, 0, 1))
+    rho[0, :, :] = rho[-1, :, :] = rho[:, 0, :] = rho[:, -1, :] = rho[:, :, 0] = rho[:, :, -1] = 0
+    u.field(4, O.F_RHO)[:] = rho
+    m.field(4, 0, O.OA_RHO)[:] = rho
+    m.field(3, 0, O.OA_RHO)[:] = O.restrict(rho, 3)  # AMR data of the base level
+    for x in (u, m):
+        x.set_params()
+        x.init()
+    for c in range(5):
+        ru, nu, cu = u.vcycle()
+        ra, na, ca = m.vcycle()
+        assert ru == ra and cu == ca, (c, ru, ra, cu, ca)
+        assert np.array_equal(nu, na)
+    assert np.array_equal(u.field(4, O.F_U), m.field(4, 0, O.OA_U))
+    assert np.array_equal(u.field(2, O.F_U), m.field(2, 0, O.OA_U))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+@pytest.mark.parametrize("bdy", [0, 1])
+def test_sync_ownership_brute_force(seed, bdy):
+    levels = small_levels(seed)
+    m = O.AmrMG(17, levels)
+    L = m.nlev  # finest refined level
+    dx = 1.0 / 16 / 4
+    rng = np.random.default_rng(seed)
+    grids = []
+    for g in range(m.ngrids(L)):
+        shape, bb = m.grid(L, g)
+        f = m.field(L, g, O.OA_U)
+        f[:] = rng.standard_normal(shape)
+        grids.append((lattice_box(bb, dx), f.copy()))
+    lim = int(round(1.0 / dx))
+
+    def reduced(box):
+        r = list(box)
+        for d in range(3):
+            if r[2 * d] > 0:
+                r[2 * d] += bdy
+            if r[2 * d + 1] < lim:
+                r[2 * d + 1] -= bdy
+        return r
+
+    def inside(box, p):
+        return all(box[2 * d] <= p[d] <= box[2 * d + 1] for d in range(3))
+
+    m.sync_field(L, bdy, O.OA_U)
+    red = [reduced(b) for b, _ in grids]
+    for g, (box, before) in enumerate(grids):
+        after = m.field(L, g, O.OA_U)
+        for k in range(box[4], box[5] + 1):
+            for j in range(box[2], box[3] + 1):
+                for i in range(box[0], box[1] + 1):
+                    p = (i, j, k)
+                    owner = next((h for h in range(len(grids)) if inside(red[h], p)), None)
+                    loc = (k - box[4], j - box[2], i - box[0])
+                    if owner is None or owner == g:
+                        want = before[loc]
+                    else:
+                        ob = grids[owner][0]
+                        want = grids[owner][1][k - ob[4], j - ob[2], i - ob[0]]
+                    assert after[loc] == want, (g, p, owner)
+
+
+def poly_trilinear(X, Y, Z):
+    return 1.0 + 2.0 * X - 3.0 * Y + 0.5 * Z + 4.0 * X * Y - 2.0 * Y * Z + X * Z + 8.0 * X * Y * Z
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_interp_exact_for_trilinear(seed):
+    m = O.AmrMG(17, small_levels(seed))
+    for Lc in (m.base, m.base + 1):
+        for g in range(m.ngrids(Lc)):
+            shape, bb = m.grid(Lc, g)
+            m.field(Lc, g, O.OA_RES)[:] = poly_trilinear(*H.grid_points(shape, bb))
+        for g in range(m.ngrids(Lc + 1)):
+            m.field(Lc + 1, g, O.OA_RES)[:] = np.nan
+        m.interp_field(Lc, O.OA_RES)
+        for g in range(m.ngrids(Lc + 1)):
+            shape, bb = m.grid(Lc + 1, g)
+            got = m.field(Lc + 1, g, O.OA_RES)
+            assert not np.isnan(got).any(), "fine point not reached by interp"
+            assert np.array_equal(got, poly_trilinear(*H.grid_points(shape, bb)))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("op", [1, 2, 3])
+def test_inject_exact_for_affine(seed, op):
+    m = O.AmrMG(17, small_levels(seed))
+    aff = lambda X, Y, Z: 0.25 + 2.0 * X - 1.5 * Y + 3.0 * Z
+    for Lf in (m.base + 1, m.base + 2):
+        Lc = Lf - 1
+        dxc = 1.0 / 16 / (1 << (Lc - m.base))
+        fine_boxes = []
+        for g in range(m.ngrids(Lf)):
+            shape, bb = m.grid(Lf, g)
+            m.field(Lf, g, O.OA_U)[:] = aff(*H.grid_points(shape, bb))
+            fine_boxes.append(lattice_box(bb, dxc / 2))
+        for g in range(m.ngrids(Lc)):
+            m.field(Lc, g, O.OA_U)[:] = np.nan
+        m.inject_field(Lf, O.OA_U, op)
+        for g in range(m.ngrids(Lc)):
+            shape, bb = m.grid(Lc, g)
+            got = m.field(Lc, g, O.OA_U)
+            want = aff(*H.grid_points(shape, bb))
+            cb = lattice_box(bb, dxc)
+            for k in range(shape[0]):
+                for j in range(shape[1]):
+                    for i in range(shape[2]):
+                        p = (2 * (cb[0] + i), 2 * (cb[2] + j), 2 * (cb[4] + k))
+                        covered = any(all(b[2 * d] <= p[d] <= b[2 * d + 1] for d in range(3))
+                                      for b in fine_boxes)
+                        v = got[k, j, i]
+                        if covered:
+                            assert v == want[k, j, i], (Lf, g, i, j, k, v, want[k, j, i])
+                        else:
+                            assert np.isnan(v), (Lf, g, i, j, k)
+
+
+def test_owned_segments_partition():
+    """owned pieces are disjoint and cover the union of the grids"""
+    m = O.AmrMG(17, small_levels(5))
+    L = m.nlev
+    dx = 1.0 / 64
+    segs = m.owned(L, 1, 0)
+    cover = {}
+    for g, bb in segs:
+        b = lattice_box(bb, dx)
+        for k in range(b[4], b[5] + 1):
+            for j in range(b[2], b[3] + 1):
+                for i in range(b[0], b[1] + 1):
+                    assert (i, j, k) not in cover, "owned pieces overlap"
+                    cover[(i, j, k)] = g
+    union = set()
+    for g in range(m.ngrids(L)):
+        b = lattice_box(m.grid(L, g)[1], dx)
+        union |= {(i, j, k) for k in range(b[4], b[5] + 1) for j in range(b[2], b[3] + 1)
+                  for i in range(b[0], b[1] + 1)}
+    assert set(cover) == union
