@@ -74,6 +74,91 @@ def cpu_baseline(rho, n, cycles_needed, sample_cycles):
     }
 
 
+def sequence_leg(args, pm, torch):
+    """Learned + FP32 on the drifting sequence of 257^3 solves.
+
+    Training (not timed): (residual, error) crops from plain FP64 V-cycles of
+    solves 0 .. T-1, Adam on the GPU.  Evaluation: solves T .. T+K-1, each
+    from the same warm start (second-order extrapolation of the two previous
+    converged solutions) in four ways -- learned + FP32 (mode 2), FP32 inner
+    cycles (mode 1), plain FP64 -- and plain FP64 from u = 0 (the reference's
+    cold solve).  Timed per solve: setting the initial guess and the solve to
+    ||r|| <= rtol ||r(u=0)||."""
+    from pamr_amrd_amd import sequence as SQ
+    n = args.n
+    T, K = args.seq_train, args.steps
+    log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
+    seq = SQ.Sequence(n=n, seed=1234, speed=0.002)
+    t0 = time.perf_counter()
+    X, Y = seq.collect(nsolves=T, pairs=args.seq_pairs, side=16, cycles=8)
+    t_collect = time.perf_counter() - t0
+    log(f"sequence: {X.shape[0]} training pairs from {T} solves in {t_collect:.1f} s")
+    net = pm.Net(seed=3)
+    t0 = time.perf_counter()
+    losses = SQ.train(net, X, Y, side=16, steps=args.seq_train_steps, batch=32, lr=2e-3, margin=3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    log(f"sequence: trained {args.seq_train_steps} Adam steps in {t_train:.1f} s, "
+        f"loss {sum(losses[:20]) / 20:.3e} -> {sum(losses[-20:]) / 20:.3e}")
+    S = seq.S
+    S.set_net(net)
+    L = pm.lib()
+    u0 = torch.empty((n, n, n), dtype=torch.float64, device="cuda")
+    modes = (("learned_fp32", 2, True), ("fp32", 1, True), ("fp64_warm", 0, True), ("fp64_cold", 0, False))
+    times = {m[0]: [] for m in modes}
+    iters = {m[0]: [] for m in modes}
+    for s in range(T, T + args.seq_warmup + K):
+        rref = seq.prepare(s, warm=True)
+        SQ._copy_grid(pm.tptr(u0), S.field_ptr(seq.Lf, pm.PMG_F_U), n)
+        for name, mode, warm in modes:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            if warm:
+                L.pmg_set_solution_lincomb(S.h, pm.tptr(u0), None, 1.0, 0.0)
+            else:
+                S.zero_solution()
+            _, h = seq.solve(mode, args.rtol, rref)
+            torch.cuda.synchronize()
+            dt = 1e3 * (time.perf_counter() - t1)
+            if len(h) == 0 or h[-1] > args.rtol * rref:
+                raise RuntimeError(f"sequence solve {s} mode {name} did not converge: {h}")
+            if s >= T + args.seq_warmup:
+                times[name].append(dt)
+                iters[name].append(len(h))
+        seq.push_solution()
+    mean = {k: sum(v) / len(v) for k, v in times.items()}
+    return {
+        "ms_per_solve": {k: round(v, 4) for k, v in mean.items()},
+        "iterations": {k: round(sum(v) / len(v), 2) for k, v in iters.items()},
+        "speedup_learned_fp32_vs_fp64_cold": round(mean["fp64_cold"] / mean["learned_fp32"], 3),
+        "speedup_learned_fp32_vs_fp64_warm": round(mean["fp64_warm"] / mean["learned_fp32"], 3),
+        "network": S.net_stats(),
+        "training": {"solves": T, "pairs": int(X.shape[0]), "crop": 16, "adam_steps": args.seq_train_steps,
+                     "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
+                     "loss_first": sum(losses[:20]) / 20, "loss_last": sum(losses[-20:]) / 20},
+        "solves_timed": [T + args.seq_warmup, T + args.seq_warmup + K - 1],
+    }
+
+
+def probe_kernels(S, n, pm, torch):
+    """Launch each finest-level kernel of interest 4 times, nothing else
+    large: the counter runs attribute FETCH_SIZE / WRITE_SIZE per dispatch."""
+    Lf = S.nlev
+    for name, which, prec in (("sweep f64", 0, 0), ("residual f64", 1, 0), ("sweep f32", 0, 1)):
+        print(f"probe {name}: {S.time_kernel(which, Lf, prec, 4):.5f} ms", flush=True)
+    # calibration: pmg_dmcopy3d over the whole grid = plain copy, 8 B read + 8 B written per point
+    src = S.field_ptr(Lf, pm.PMG_F_RHO)
+    dst = torch.empty((n, n, n), dtype=torch.float64, device="cuda")
+    L = pm.lib()
+    for _ in range(4):
+        rc = L.pmg_dmcopy3d(src, pm.tptr(dst), n, n, n, n, n, n, 1, 1, 1, 1, 1, 1, n, n, n,
+                            1, 1, 1, 1, 1, 1, None)
+        if rc != 0:
+            raise RuntimeError(f"pmg_dmcopy3d returned {rc}")
+    torch.cuda.synchronize()
+    print(f"probe copy: {8 * n ** 3} bytes read and written per launch", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +170,15 @@ def main():
     ap.add_argument("--cpu-cycles", type=int, default=3)
     ap.add_argument("--variant", type=int, default=-1, help="sweep tile variant (-1: tuned)")
     ap.add_argument("--zchunk", type=int, default=-1)
+    ap.add_argument("--seq-train", type=int, default=50, help="training solves of the sequence")
+    ap.add_argument("--seq-pairs", type=int, default=4000)
+    ap.add_argument("--seq-train-steps", type=int, default=1500)
+    ap.add_argument("--seq-warmup", type=int, default=2)
+    ap.add_argument("--no-seq", action="store_true", help="skip the learned sequence leg")
+    ap.add_argument("--probe-kernels", action="store_true",
+                    help="only launch the finest-level kernels (for rocprofv3 --pmc runs): "
+                         "FP64 sweep, FP64 residual, FP32 sweep and a plain 8 B/point copy "
+                         "used to calibrate the counters, 4 launches each")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus N with N > 1 is not supported "
@@ -112,6 +206,10 @@ def main():
     S.upload(S.nlev, pm.PMG_F_RHO, rho)
     S.init()
     Lf = S.nlev
+    if args.probe_kernels:
+        probe_kernels(S, n, pm, torch)
+        S.close()
+        return
 
     def solve(mode):
         S.zero_solution()
@@ -158,28 +256,47 @@ def main():
     if not args.no_cpu:
         print("bench: CPU baseline (oracle, 1 core)", file=sys.stderr, flush=True)
         cpu = cpu_baseline(rho, n, len(hp), args.cpu_cycles)
+    single = {"workload": f"poisson{n}_gauss8_from_zero", "plain_fp64_ms_per_solve": round(ms_plain, 4),
+              "mixed_fp32_fp64_ms_per_solve": round(ms_mixed, 4),
+              "speedup_mixed_vs_plain_fp64": round(ms_plain / ms_mixed, 3),
+              "plain_cycles": len(hp), "mixed_outer_iters": len(hm),
+              "final_rel_residual": {"plain": hp[-1] / r0p, "mixed": hm[-1] / r0m}}
+    seqres = None
+    if not args.no_seq:
+        print("bench: learned sequence leg", file=sys.stderr, flush=True)
+        seqres = sequence_leg(args, pm, torch)
+    if seqres is not None:
+        value = seqres["ms_per_solve"]["learned_fp32"]
+        config = {"workload": f"poisson{n}_drift8_sequence_learned_fp32", "n": n, "levels": Lf,
+                  "cycle": "FAS V(2,2) RB-GS, FW restriction, trilinear prolongation",
+                  "rtol": args.rtol, "reference_norm": "||r(u=0)||",
+                  "sequence": f"8 Gaussians drifting 0.002/solve; train on solves 0-{args.seq_train - 1}, "
+                              f"timed solves {seqres['solves_timed'][0]}-{seqres['solves_timed'][1]}",
+                  "warm_start": "2 u_{s-1} - u_{s-2}", "sweep_variant": variant, "zchunk": zchunk}
+        dtype = "f32 inner / f64 outer; network bf16 (fp32 accumulate)"
+    else:
+        value = ms_mixed
+        config = {"workload": f"poisson{n}_gauss8_from_zero", "n": n, "levels": Lf,
+                  "cycle": "FAS V(2,2) RB-GS, FW restriction, trilinear prolongation",
+                  "rtol": args.rtol, "sweep_variant": variant, "zchunk": zchunk}
+        dtype = "f32 inner / f64 outer"
     out = {
         "metric": METRIC,
-        "value": round(ms_mixed, 4),
+        "value": round(value, 4),
         "unit": "ms/solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_mixed, 4),
+        "ms_per_step": round(value, 4),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 inner / f64 outer",
-        "data": "synthetic: seeded 8-Gaussian source (splitmix64 seed 1234)",
-        "config": {"workload": f"poisson{n}_gauss8_rtol1e-10", "n": n, "levels": Lf,
-                   "cycle": "FAS V(2,2) RB-GS, FW restriction, trilinear prolongation",
-                   "rtol": args.rtol, "sweep_variant": variant, "zchunk": zchunk},
-        "plain_fp64_ms_per_solve": round(ms_plain, 4),
-        "speedup_vs_plain_fp64": round(ms_plain / ms_mixed, 3),
-        "plain_cycles": len(hp),
-        "mixed_outer_iters": len(hm),
-        "final_rel_residual": {"plain": hp[-1] / r0p, "mixed": hm[-1] / r0m},
-        "solves_per_s": round(1e3 / ms_mixed, 2),
+        "dtype": dtype,
+        "data": "synthetic: seeded 8-Gaussian sources (splitmix64 seed 1234)",
+        "config": config,
+        "solves_per_s": round(1e3 / value, 2),
+        "sequence": seqres,
+        "single_solve": single,
         "roofline": roof,
         "roofline_fp32_sweep": roof32,
         "cpu_baseline": cpu,

@@ -1,0 +1,85 @@
+/*
+ * pamr_amr_gpu.h -- C ABI of the batched multigrid solve over an AMR
+ * hierarchy (several grids per level) on one MI355X (gfx950).
+ *
+ * This is the device-resident counterpart of PAMR's MG hierarchy with
+ * AMR levels: PAMR_compose_hierarchy + PAMR_build_mgh (src/pamr.c:257-378,
+ * src/gh.c:1222-1365) build it, PAMR_sync / PAMR_inject / PAMR_interp
+ * (src/pamr.c:601-685) move data with the reference's ownership and
+ * transfer-list rules (src/transfer.c:73-575, 946-1163), and AMRD's
+ * vcycle (amrd/mg.c:84-201) runs over it.  Every grid function is a device
+ * FP64 array in Fortran order (x fastest), one per grid, as the reference's
+ * gfs[] (src/gh.c:884-900).
+ *
+ * Execution: the transfer lists are resolved once per hierarchy into
+ * disjoint index boxes (later transfers of a list win where they overlap,
+ * as sequential execution gives), and each level operation -- a sweep over
+ * every grid of the level, a sync, an inject, an interp -- is one kernel
+ * launch.  A whole V-cycle is captured in one hipGraph.
+ *
+ * Scope: single MPI rank (ghost_width 0), refinement ratio 2, one base grid,
+ * no excision, FP64.  Return codes: 0 ok, < 0 error (as pamr_mg_gpu.h).
+ */
+#ifndef PAMR_AMR_GPU_H
+#define PAMR_AMR_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pmg_amr pmg_amr;
+
+/* PAMR_compose_hierarchy + PAMR_build_mgh (src/pamr.c:257-378).
+ * base_shape: points of the base grid over bbox (AMR level 1); namr refined
+ * AMR levels; nbox[l] boxes of AMR level l+2; boxes: 6 doubles per box
+ * (x0,x1,y0,y1,z0,z1), all levels concatenated in the caller's order (grids
+ * are then ordered by insert_grid, src/gh.c:235-257).  op: 0 Poisson
+ * (w/examples/nps/num_w.f), 1 div(a grad u) + c u. */
+pmg_amr *pmg_amr_create(const int *base_shape, const double *bbox, int namr, const int *nbox,
+                        const double *boxes, int min_mg_cwidth, int op);
+void pmg_amr_destroy(pmg_amr *A);
+/* PAMR_get_max_lev(PAMR_MGH) */
+int pmg_amr_nlev(pmg_amr *A);
+/* MG level that holds AMR level 1 */
+int pmg_amr_base_level(pmg_amr *A);
+int pmg_amr_ngrids(pmg_amr *A, int L);
+/* PAMR_get_g_attribs subset: shape and bbox of grid g (0-based) of level L */
+int pmg_amr_grid_info(pmg_amr *A, int L, int g, int *shape, double *bbox);
+/* device pointer of a grid function (which: PMG_F_U .. PMG_F_C of
+ * pamr_mg_gpu.h).  After writing u through it, call pmg_amr_commit_u. */
+double *pmg_amr_field(pmg_amr *A, int L, int g, int which);
+int pmg_amr_commit_u(pmg_amr *A);
+int pmg_amr_upload(pmg_amr *A, int L, int g, int which, const double *host);
+int pmg_amr_download(pmg_amr *A, int L, int g, int which, double *host);
+int pmg_amr_set_params(pmg_amr *A, int pre, int pst, int max_citer, double crtol, double w0_r,
+                       double w0_i, int rop_u, int rop_res, int rop_src, int interp_ord);
+/* TF_MGH_INIT injection into the levels below the base (src/gh.c:1334) */
+int pmg_amr_init(pmg_amr *A);
+/* one V-cycle (amrd/mg.c:84-201); res_out = res[Lf] (max over grids),
+ * lev_norms (nlev doubles) = last norm of every level */
+int pmg_amr_vcycle(pmg_amr *A, double *res_out, double *lev_norms, int *citer);
+/* solve_elliptics loop (amrd/mg.c:56-72) */
+int pmg_amr_solve(pmg_amr *A, int min_iter, int max_iter, double tol, double *norms, int *iters);
+/* residual of the finest level (max over grids) */
+int pmg_amr_residual(pmg_amr *A, double *norm_out);
+/* single transfers of one field: PAMR_sync(L, AMR_bdy_width = bdy),
+ * PAMR_inject(Lf -> Lf-1) with restriction op 1/2/3, PAMR_interp(Lc ->
+ * Lc+1) second order */
+int pmg_amr_sync(pmg_amr *A, int L, int bdy, int which);
+int pmg_amr_inject(pmg_amr *A, int Lf, int which, int op);
+int pmg_amr_interp(pmg_amr *A, int Lc, int which);
+/* set_cmask_bdy (which 0) / set_cmask_child (which 1) of grid g as the
+ * reference's double mask (1.0 ON, 0.0 OFF), host output */
+int pmg_amr_cmask(pmg_amr *A, int L, int g, int which, double *host_out);
+/* number of disjoint transfer boxes per level: out[0] sync(bdy 0),
+ * out[1] sync(bdy 1), out[2] inject into L, out[3] interp into L */
+int pmg_amr_plan_stats(pmg_amr *A, int L, int *out);
+/* average duration (ms) of one batched sweep launch over all grids of L */
+int pmg_amr_time_sweep(pmg_amr *A, int L, int nrep, double *ms_avg);
+int pmg_amr_set_graph(pmg_amr *A, int on);
+void *pmg_amr_stream(pmg_amr *A);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

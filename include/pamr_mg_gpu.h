@@ -132,7 +132,11 @@ int pmg_residual(pmg_solver *S, double *norm_out);
 /* solve_elliptics loop (amrd/mg.c:56-72), absolute tolerance */
 int pmg_solve(pmg_solver *S, int min_iter, int max_iter, double tol, double *norms, int *iters);
 /* relative solve ||r_k|| <= rtol ||r_0||; mode 0 FP64 V-cycles, 1 FP32
- * inner V-cycles in FP64 defect correction; hist[k] = ||r_{k+1}|| */
+ * inner V-cycles in FP64 defect correction, 2 as 1 with the learned
+ * correction (pmg_set_net); hist[k] = ||r_{k+1}||.  r0_out: on entry, a
+ * value > 0 is taken as the reference norm instead of ||r_0|| of the
+ * initial u (warm starts measure against the residual of u = 0); on exit
+ * the reference norm used. */
 int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hist, int *iters,
                   double *r0_out);
 int pmg_last_citer(pmg_solver *S, int which);

@@ -99,6 +99,32 @@ def _declare(L):
     L.pmg_set_net.argtypes = [_vp, _vp]
     L.pmg_net_stats.argtypes = [_vp, _ip, _ip, _dp, _dp]
     L.pmg_set_solution_lincomb.argtypes = [_vp, _vp, _vp, d, d]
+    # batched AMR hierarchy (include/pamr_amr_gpu.h)
+    L.pmg_amr_create.argtypes = [_ip, _dp, i, _ip, _dp, i, i]
+    L.pmg_amr_create.restype = _vp
+    L.pmg_amr_destroy.argtypes = [_vp]
+    L.pmg_amr_destroy.restype = None
+    for f in ("pmg_amr_nlev", "pmg_amr_base_level", "pmg_amr_commit_u", "pmg_amr_init"):
+        getattr(L, f).argtypes = [_vp]
+    L.pmg_amr_ngrids.argtypes = [_vp, i]
+    L.pmg_amr_grid_info.argtypes = [_vp, i, i, _ip, _dp]
+    L.pmg_amr_field.argtypes = [_vp, i, i, i]
+    L.pmg_amr_field.restype = _vp
+    L.pmg_amr_upload.argtypes = [_vp, i, i, i, _dp]
+    L.pmg_amr_download.argtypes = [_vp, i, i, i, _dp]
+    L.pmg_amr_set_params.argtypes = [_vp, i, i, i, d, d, d, i, i, i, i]
+    L.pmg_amr_vcycle.argtypes = [_vp, _dp, _dp, _ip]
+    L.pmg_amr_solve.argtypes = [_vp, i, i, d, _dp, _ip]
+    L.pmg_amr_residual.argtypes = [_vp, _dp]
+    L.pmg_amr_sync.argtypes = [_vp, i, i, i]
+    L.pmg_amr_inject.argtypes = [_vp, i, i, i]
+    L.pmg_amr_interp.argtypes = [_vp, i, i]
+    L.pmg_amr_cmask.argtypes = [_vp, i, i, i, _dp]
+    L.pmg_amr_plan_stats.argtypes = [_vp, i, _ip]
+    L.pmg_amr_time_sweep.argtypes = [_vp, i, i, _dp]
+    L.pmg_amr_set_graph.argtypes = [_vp, i]
+    L.pmg_amr_stream.argtypes = [_vp]
+    L.pmg_amr_stream.restype = _vp
 
 
 def _check(rc, what):
@@ -204,10 +230,12 @@ class Solver:
         _check(self._L.pmg_solve(self.h, min_iter, max_iter, tol, P(out), C.byref(it)), "pmg_solve")
         return out[: it.value]
 
-    def solve_rel(self, mode, rtol, max_iter):
+    def solve_rel(self, mode, rtol, max_iter, r_ref=0.0):
+        """(reference norm, residual history); r_ref > 0 replaces the
+        residual of the initial u as the reference of the relative test."""
         hist = np.zeros(max(max_iter, 1))
         it = C.c_int(0)
-        r0 = C.c_double(0)
+        r0 = C.c_double(r_ref)
         _check(self._L.pmg_solve_rel(self.h, mode, rtol, max_iter, P(hist), C.byref(it),
                                      C.byref(r0)), "pmg_solve_rel")
         return r0.value, hist[: it.value]
@@ -226,11 +254,116 @@ class Solver:
     def set_graph(self, on):
         _check(self._L.pmg_set_graph(self.h, 1 if on else 0), "pmg_set_graph")
 
+    def set_net(self, net):
+        """attach a trained network: solve_rel mode 2 applies its correction"""
+        _check(self._L.pmg_set_net(self.h, net.h if net is not None else None), "pmg_set_net")
+
+    def net_stats(self):
+        a, r = C.c_int(0), C.c_int(0)
+        tn, tv = C.c_double(0), C.c_double(0)
+        _check(self._L.pmg_net_stats(self.h, C.byref(a), C.byref(r), C.byref(tn), C.byref(tv)), "pmg_net_stats")
+        return {"accepted": a.value, "rejected": r.value, "t_net_ms": tn.value, "t_vcycle_ms": tv.value}
+
     def field_ptr(self, lev, which):
         return self._L.pmg_field(self.h, lev, which)
 
     def stream(self):
         return self._L.pmg_stream(self.h)
+
+
+class Amr:
+    """Owning wrapper of a ``pmg_amr``: the batched multigrid solve over an
+    AMR hierarchy.  ``levels`` lists, per refined AMR level, its physical
+    boxes (x0, x1, y0, y1, z0, z1); the base grid covers ``bbox``."""
+
+    def __init__(self, base_n, levels, bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), mcw=5, op=0):
+        self._L = lib()
+        shape = (C.c_int * 3)(base_n, base_n, base_n)
+        bb = np.array(bbox, dtype=np.float64)
+        nbox = (C.c_int * max(1, len(levels)))(*[len(l) for l in levels])
+        boxes = np.array([b for l in levels for b in l] or [[0.0] * 6], dtype=np.float64).reshape(-1)
+        self.h = self._L.pmg_amr_create(shape, P(bb), len(levels), nbox, P(boxes), mcw, op)
+        if not self.h:
+            raise RuntimeError("pmg_amr_create failed")
+        self.nlev = self._L.pmg_amr_nlev(self.h)
+        self.base = self._L.pmg_amr_base_level(self.h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._L.pmg_amr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def ngrids(self, lev):
+        return self._L.pmg_amr_ngrids(self.h, lev)
+
+    def grid(self, lev, g):
+        s = (C.c_int * 3)()
+        b = np.zeros(6)
+        _check(self._L.pmg_amr_grid_info(self.h, lev, g, s, P(b)), "pmg_amr_grid_info")
+        return (s[2], s[1], s[0]), b
+
+    def upload(self, lev, g, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == self.grid(lev, g)[0]
+        _check(self._L.pmg_amr_upload(self.h, lev, g, which, P(a)), "pmg_amr_upload")
+
+    def download(self, lev, g, which):
+        out = np.zeros(self.grid(lev, g)[0])
+        _check(self._L.pmg_amr_download(self.h, lev, g, which, P(out)), "pmg_amr_download")
+        return out
+
+    def set_params(self, pre=2, pst=2, max_citer=5000, crtol=1e-3, w0_r=1.0, w0_i=1.0,
+                   rop_u=3, rop_res=3, rop_src=3, interp_ord=2):
+        _check(self._L.pmg_amr_set_params(self.h, pre, pst, max_citer, crtol, w0_r, w0_i, rop_u,
+                                          rop_res, rop_src, interp_ord), "pmg_amr_set_params")
+
+    def init(self):
+        _check(self._L.pmg_amr_init(self.h), "pmg_amr_init")
+
+    def vcycle(self):
+        r = C.c_double(0)
+        norms = np.zeros(self.nlev)
+        cit = C.c_int(0)
+        _check(self._L.pmg_amr_vcycle(self.h, C.byref(r), P(norms), C.byref(cit)), "pmg_amr_vcycle")
+        return r.value, norms, cit.value
+
+    def residual(self):
+        r = C.c_double(0)
+        _check(self._L.pmg_amr_residual(self.h, C.byref(r)), "pmg_amr_residual")
+        return r.value
+
+    def sync(self, lev, bdy, which):
+        _check(self._L.pmg_amr_sync(self.h, lev, bdy, which), "pmg_amr_sync")
+
+    def inject(self, lev_fine, which, op=3):
+        _check(self._L.pmg_amr_inject(self.h, lev_fine, which, op), "pmg_amr_inject")
+
+    def interp(self, lev_coarse, which):
+        _check(self._L.pmg_amr_interp(self.h, lev_coarse, which), "pmg_amr_interp")
+
+    def cmask(self, lev, g, which):
+        out = np.zeros(self.grid(lev, g)[0])
+        _check(self._L.pmg_amr_cmask(self.h, lev, g, which, P(out)), "pmg_amr_cmask")
+        return out
+
+    def plan_stats(self, lev):
+        out = (C.c_int * 4)()
+        _check(self._L.pmg_amr_plan_stats(self.h, lev, out), "pmg_amr_plan_stats")
+        return list(out)
+
+    def time_sweep(self, lev, nrep=10):
+        ms = C.c_double(0)
+        _check(self._L.pmg_amr_time_sweep(self.h, lev, nrep, C.byref(ms)), "pmg_amr_time_sweep")
+        return ms.value
+
+    def set_graph(self, on):
+        _check(self._L.pmg_amr_set_graph(self.h, 1 if on else 0), "pmg_amr_set_graph")
 
 
 class Net:

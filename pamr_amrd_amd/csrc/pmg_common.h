@@ -25,6 +25,17 @@
 namespace pmg {
 
 constexpr int kWave = 64;        // CDNA wavefront width
+constexpr int kXcd = 8;          // MI355X: 8 XCDs, each with its own L2
+
+// The dispatcher hands workgroup b to XCD b % 8.  Map workgroups to tiles
+// so that each XCD owns one contiguous range of tile indices (a bijection
+// for any count): tiles that are neighbours in the linear tile order then
+// share an L2, and halo re-reads hit it instead of going to HBM.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb / kXcd, r = nb % kXcd;
+  const int x = b % kXcd, local = b / kXcd;
+  return x * q + (x < r ? x : r) + local;
+}
 constexpr int kMaxLev = 16;
 
 __host__ __device__ inline size_t ix3(int i, int j, int k, int nx, int ny) {
@@ -39,6 +50,7 @@ struct Spacing {
   T dx2, dy2, dz2;
   T idx2, idy2, idz2;
   T jac;     // -2/dx2 - 2/dy2 - 2/dz2 (Poisson Jacobian, host-computed)
+  T ijac;    // 1/jac (the FP32 inner cycles multiply instead of dividing)
   int pow2;  // 1 when inv_* are exact
 };
 
@@ -90,6 +102,7 @@ __device__ inline void norm_partial(double s, double c, NormWs ws, int slot, int
 }
 
 // block b folds slots slot0 + b*fold ... slot0 + b*fold + fold-1 into out[b]
-__global__ __launch_bounds__(256) void k_norm_fold(NormWs ws, int slot0, int fold, double *out);
+constexpr int kFoldThreads = 1024;
+__global__ __launch_bounds__(kFoldThreads) void k_norm_fold(NormWs ws, int slot0, int fold, double *out);
 
 }  // namespace pmg

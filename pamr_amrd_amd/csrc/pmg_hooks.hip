@@ -109,6 +109,7 @@ Spacing<double> make_spacing(double dx, double dy, double dz) {
   sp.idy2 = 1.0 / sp.dy2;
   sp.idz2 = 1.0 / sp.dz2;
   sp.jac = -2 / sp.dx2 - 2 / sp.dy2 - 2 / sp.dz2;
+  sp.ijac = 1 / sp.jac;
   sp.pow2 = is_pow2(sp.dx2) && is_pow2(sp.dy2) && is_pow2(sp.dz2);
   return sp;
 }
@@ -159,7 +160,7 @@ int pmg_hook_relax(int op, double *V, const double *rhs, const double *rho, cons
     PMG_HOST_CHECK(hipGetLastError());
   }
   if (norm_dev) {
-    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(256), 0, stream, g_ws, 0, 2, norm_dev);
+    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(kFoldThreads), 0, stream, g_ws, 0, 2, norm_dev);
     PMG_HOST_CHECK(hipGetLastError());
   }
   return 0;
@@ -184,7 +185,7 @@ int pmg_hook_residual(int op, double *res, const double *rhs, const double *V, c
 #undef PMG_HS
   PMG_HOST_CHECK(hipGetLastError());
   if (norm_dev) {
-    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(256), 0, stream, g_ws, 0, 1, norm_dev);
+    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(kFoldThreads), 0, stream, g_ws, 0, 1, norm_dev);
     PMG_HOST_CHECK(hipGetLastError());
   }
   return 0;

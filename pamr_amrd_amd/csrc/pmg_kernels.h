@@ -84,7 +84,10 @@ __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX 
   }
   T res;
   if constexpr (RHS0) res = L - T(0); else res = L - rhs;
-  S0[ly][lx] = uc - res / jac;
+  if constexpr (std::is_same<T, float>::value && OP == 0)
+    S0[ly][lx] = uc - res * P.sp.ijac;  // FP32 inner cycles: no IEEE division
+  else
+    S0[ly][lx] = uc - res / jac;  // FP64: the reference's V - res/Jac, bit for bit
   return res;
 }
 
@@ -120,23 +123,50 @@ struct PhaseRegs {
   T rho[N], rhs[N];
 };
 
-template <typename T, bool EX, bool RHS0, int TX, int TY, int NT, bool HALO, int N>
-__device__ __forceinline__ void phase_prefetch(const SweepP<T> &P, int x0, int y0, int k, int pass,
-                                               PhaseRegs<T, N> &R) {
+// Per-thread slot table of one phase for both plane parities, computed once
+// per block: LDS position, xy offset of the point, and whether the slot is
+// inside the tile region and the updatable x/y range.
+template <int N>
+struct SlotTab {
+  int lx[2][N], ly[2][N];
+  int off[2][N];  // i + nx*j, or -1
+};
+
+template <typename T, int TX, int TY, int NT, bool HALO, int N>
+__device__ __forceinline__ void slot_table(const SweepP<T> &P, int x0, int y0, int pass, SlotTab<N> &S) {
+#pragma unroll
+  for (int par = 0; par < 2; par++)
+#pragma unroll
+    for (int m = 0; m < N; m++) {
+      int lx = 0, ly = 0;
+      S.off[par][m] = -1;
+      // plane parity par: the colour test uses (j + k + 3 + pass), k = par mod 2
+      if (!sweep_pos<TX, TY, HALO>(threadIdx.x + m * NT, x0, y0, par, pass, lx, ly)) continue;
+      const int i = x0 - 2 + lx, j = y0 - 2 + ly;
+      S.lx[par][m] = lx;
+      S.ly[par][m] = ly;
+      if (i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1) S.off[par][m] = i + P.nx * j;
+    }
+}
+
+template <typename T, bool EX, bool RHS0, int N>
+__device__ __forceinline__ void phase_prefetch_t(const SweepP<T> &P, int k, const SlotTab<N> &S,
+                                                 PhaseRegs<T, N> &R) {
+  // the plane parity selects between the two register tables with a
+  // select, not an index (a run-time index would move them to scratch)
+  const bool odd = (k & 1) != 0;
+  const bool zok = k >= P.bz0 && k <= P.bz1;
+  const size_t base = (size_t)k * P.nx * P.ny;
 #pragma unroll
   for (int m = 0; m < N; m++) {
     R.p[m] = -1;
-    int lx, ly;
-    const int q = threadIdx.x + m * NT;
-    if (k < P.bz0 || k > P.bz1) continue;
-    if (!sweep_pos<TX, TY, HALO>(q, x0, y0, k, pass, lx, ly)) continue;
-    const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-    if (!sweep_updatable(P, i, j, k)) continue;
-    const size_t p = ix3(i, j, k, P.nx, P.ny);
+    const int off = odd ? S.off[1][m] : S.off[0][m];
+    if (!zok || off < 0) continue;
+    const size_t p = base + (size_t)off;
     if constexpr (EX) { if (P.chr[p]) continue; }
     R.p[m] = (long long)p;
-    R.lx[m] = lx;
-    R.ly[m] = ly;
+    R.lx[m] = odd ? S.lx[1][m] : S.lx[0][m];
+    R.ly[m] = odd ? S.ly[1][m] : S.ly[0][m];
     R.rho[m] = P.rho[p];
     if constexpr (!RHS0) R.rhs[m] = P.rhs[p];
   }
@@ -162,16 +192,17 @@ __device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX
 // pass 1 on plane k (tile), store plane k.  While a step computes, the u
 // values of plane k+3 and the rho/rhs values of the next step are already
 // in flight in registers, so HBM latency overlaps the stencil work.
+// One tile of the fused sweep: columns [x0, x0+TX) x [y0, y0+TY) of the
+// interior, planes [z0, z1).  `bid` is the tile's slot in the norm
+// workspace.  Shared by the single-grid and the batched (all grids of a
+// level) kernels.
 template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
+__device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, int z0, int z1, int bid) {
   constexpr int LX = TX + 4, LY = TY + 4;
   constexpr int NLD = (LX * LY + NT - 1) / NT;
   constexpr int N0 = (((TX + 3) / 2) * (TY + 2) + NT - 1) / NT;
   constexpr int N1 = (((TX + 1) / 2) * TY + NT - 1) / NT;
   __shared__ T sU[4][LY][LX];
-  const int x0 = 1 + blockIdx.x * TX, y0 = 1 + blockIdx.y * TY;
-  const int z0 = 1 + blockIdx.z * P.zchunk;
-  const int z1 = min(z0 + P.zchunk, P.nz - 1);
   const int nx = P.nx, ny = P.ny, nz = P.nz;
   const T *__restrict__ uin = P.uin;
   T *__restrict__ uout = P.uout;
@@ -230,17 +261,21 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
   __syncthreads();
   load_direct(z0 + 2);
   load_issue(ldA, z0 + 3);
+  SlotTab<N0> tab0;
+  SlotTab<N1> tab1;
+  slot_table<T, TX, TY, NT, true, N0>(P, x0, y0, 0, tab0);
+  slot_table<T, TX, TY, NT, false, N1>(P, x0, y0, 1, tab1);
   PhaseRegs<T, N0> cur0, nxt0;
   PhaseRegs<T, N1> cur1, nxt1;
-  phase_prefetch<T, EX, RHS0, TX, TY, NT, true, N0>(P, x0, y0, z0 + 1, 0, cur0);
-  phase_prefetch<T, EX, RHS0, TX, TY, NT, false, N1>(P, x0, y0, z0, 1, cur1);
+  phase_prefetch_t<T, EX, RHS0, N0>(P, z0 + 1, tab0, cur0);
+  phase_prefetch_t<T, EX, RHS0, N1>(P, z0, tab1, cur1);
   __syncthreads();
   // step k: issue plane k+4 and the next step's rho/rhs; pass 0 on k+1;
   // pass 1 on k; store k; commit plane k+3 (issued one step earlier)
   auto step = [&](int k, const T(&ldc)[NLD], T(&ldi)[NLD]) {
     load_issue(ldi, k + 4);
-    phase_prefetch<T, EX, RHS0, TX, TY, NT, true, N0>(P, x0, y0, k + 2, 0, nxt0);
-    phase_prefetch<T, EX, RHS0, TX, TY, NT, false, N1>(P, x0, y0, k + 1, 1, nxt1);
+    phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0);
+    phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1);
     phase_run<T, OP, P2, RHS0, TX, TY, N0>(P, sU, k + 1, cur0, k + 1 < z1, s, cnt);
     __syncthreads();
     phase_run<T, OP, P2, RHS0, TX, TY, N1>(P, sU, k, cur1, true, s, cnt);
@@ -255,8 +290,17 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
     step(k, ldA, ldB);
     if (k + 1 < z1) step(k + 1, ldB, ldA);
   }
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
   norm_partial<NT>(s, cnt, P.ws, P.slot, bid, P.nb);
+}
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int nty = (P.ny - 2 + TY - 1) / TY, ntx = (P.nx - 2 + TX - 1) / TX;
+  const int ty = tile % nty, tx = (tile / nty) % ntx, tz = tile / (nty * ntx);
+  const int z0 = 1 + tz * P.zchunk;
+  sweep_tile<T, OP, EX, P2, RHS0, TX, TY, NT>(P, 1 + tx * TX, 1 + ty * TY, z0, min(z0 + P.zchunk, P.nz - 1),
+                                              blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -288,38 +332,43 @@ template <typename T, int OP, bool EX, bool P2, bool RHS0, bool ZERO, bool OUT32
 __global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
   const int nx = P.nx, ny = P.ny;
   const size_t sy = nx, sz = (size_t)nx * ny;
-  const int i = blockIdx.x * RTX + threadIdx.x % RTX;
-  const int j = blockIdx.y * RTY + threadIdx.x / RTX;
-  const int z0 = blockIdx.z * RZC;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int nty = (ny + RTY - 1) / RTY, ntx = (nx + RTX - 1) / RTX;
+  const int ngz = (int)(gridDim.x / (nty * ntx));
+  const int tyi = tile % nty, txi = (tile / nty) % ntx, gz = tile / (nty * ntx);
+  const int i = txi * RTX + threadIdx.x % RTX;
+  const int j = tyi * RTY + threadIdx.x / RTX;
   double s = 0.0, cnt = 0.0;
   const T *__restrict__ u = P.u;
   const T *__restrict__ rho = P.rho;
   const T *__restrict__ rhs = P.rhs;
   T *__restrict__ res = P.res;
   float *__restrict__ neg32 = P.neg32;
+  const int nzc = (P.nz + RZC - 1) / RZC;
   if (i < nx && j < ny) {
     const bool inxy = i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1;
+    for (int zc = gz; zc < nzc; zc += ngz) {
 #pragma unroll
-    for (int kk = 0; kk < RZC; kk++) {
-      const int k = z0 + kk;
-      if (k >= P.nz) break;
-      const size_t p = ix3(i, j, k, nx, ny);
-      bool on = inxy && k >= P.bz0 && k <= P.bz1;
-      if constexpr (EX) on = on && !P.chr[p];
-      if (on) {
-        T L = lop_at<T, OP, P2>(u, rho, P.a, P.c, p, sy, sz, P.sp, nullptr);
-        T r;
-        if constexpr (RHS0) r = L - T(0); else r = L - rhs[p];
-        if constexpr (OUT32) neg32[p] = (float)(-r); else res[p] = r;
-        s += (double)r * (double)r;
-        cnt += 1.0;
-      } else if constexpr (ZERO) {
-        if constexpr (OUT32) neg32[p] = 0.0f; else res[p] = T(0);
+      for (int kk = 0; kk < RZC; kk++) {
+        const int k = zc * RZC + kk;
+        if (k >= P.nz) break;
+        const size_t p = ix3(i, j, k, nx, ny);
+        bool on = inxy && k >= P.bz0 && k <= P.bz1;
+        if constexpr (EX) on = on && !P.chr[p];
+        if (on) {
+          T L = lop_at<T, OP, P2>(u, rho, P.a, P.c, p, sy, sz, P.sp, nullptr);
+          T r;
+          if constexpr (RHS0) r = L - T(0); else r = L - rhs[p];
+          if constexpr (OUT32) neg32[p] = (float)(-r); else res[p] = r;
+          s += (double)r * (double)r;
+          cnt += 1.0;
+        } else if constexpr (ZERO) {
+          if constexpr (OUT32) neg32[p] = 0.0f; else res[p] = T(0);
+        }
       }
     }
   }
-  const int bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-  norm_partial<RTX * RTY>(s, cnt, P.ws, P.slot, bid, P.nb);
+  norm_partial<RTX * RTY>(s, cnt, P.ws, P.slot, blockIdx.x, P.nb);
 }
 
 // ---------------------------------------------------------------------------

@@ -82,6 +82,7 @@ Spacing<T> make_spacing(const double *hdx) {
   sp.idy2 = T(1) / sp.dy2;
   sp.idz2 = T(1) / sp.dz2;
   sp.jac = -2 / sp.dx2 - 2 / sp.dy2 - 2 / sp.dz2;
+  sp.ijac = 1 / sp.jac;
   sp.pow2 = is_pow2((double)sp.dx2) && is_pow2((double)sp.dy2) && is_pow2((double)sp.dz2);
   return sp;
 }
@@ -229,8 +230,8 @@ int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
 #undef PMG_TXY
     default: return -1;
   }
-  dim3 grid((P.nx - 2 + tx - 1) / tx, (P.ny - 2 + ty - 1) / ty, (P.nz - 2 + P.zchunk - 1) / P.zchunk);
-  P.nb = (int)(grid.x * grid.y * grid.z);
+  dim3 grid(((P.nx - 2 + tx - 1) / tx) * ((P.ny - 2 + ty - 1) / ty) * ((P.nz - 2 + P.zchunk - 1) / P.zchunk));
+  P.nb = (int)grid.x;
   if (P.nb > S->ws.cap) return -3;
   switch (variant) {
 #define PMG_LSW(V, TXV, TYV, NTV)                                                              \
@@ -288,8 +289,9 @@ int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
 template <typename T, int OP, bool EX, bool P2>
 int launch_residual_t(pmg_solver *S, ResidP<T> P, bool rhs0, bool zero) {
   P.zchunk = RZC;
-  dim3 grid((P.nx + RTX - 1) / RTX, (P.ny + RTY - 1) / RTY, (P.nz + RZC - 1) / RZC);
-  const int nb = (int)(grid.x * grid.y * grid.z);
+  const int nzc = (P.nz + RZC - 1) / RZC;
+  dim3 grid(((P.nx + RTX - 1) / RTX) * ((P.ny + RTY - 1) / RTY) * (nzc < 16 ? nzc : 16));
+  const int nb = (int)grid.x;
   P.nb = nb;
   if (nb > S->ws.cap) return -3;
 #define PMG_LR(R0, Z0, O32) hipLaunchKernelGGL((k_residual<T, OP, EX, P2, R0, Z0, O32>), grid, dim3(RTX * RTY), 0, S->stream, P)
@@ -492,7 +494,7 @@ int coarse_solve(pmg_solver *S, int L, double *norm_pre, double *norm_post, int 
 // coarse rho cancels between the FAS rhs and the coarse relaxation).
 // ---------------------------------------------------------------------------
 int fold_norms(pmg_solver *S, int slot0, int nslots) {
-  hipLaunchKernelGGL(k_norm_fold, dim3(nslots), dim3(256), 0, S->stream, S->ws, slot0, 1,
+  hipLaunchKernelGGL(k_norm_fold, dim3(nslots), dim3(kFoldThreads), 0, S->stream, S->ws, slot0, 1,
                      S->d_norms + slot0);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
@@ -920,13 +922,18 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
   // norm of the residual hook and r_0 the residual of the initial u.
   // mode 0: plain FP64 V-cycles; mode 1: FP32 inner V-cycles in FP64 defect
   // correction (the FP64 residual, the update and the test stay FP64).
+  // *r0_out > 0 on entry: that norm is the reference (e.g. the residual of
+  // u = 0 when the solve starts from a warm start); the test is then
+  // ||r_k|| <= rtol * reference.
   if (!S || max_iter < 0) return -1;
   double r0;
   int rc = pmg_residual(S, &r0);
   if (rc) return rc;
-  if (r0_out) *r0_out = r0;
+  const double rref = (r0_out && *r0_out > 0) ? *r0_out : r0;
+  if (r0_out) *r0_out = rref;
   int it = 0;
   double r = r0;
+  r0 = rref;
   if (mode == 0) {
     while (it < max_iter && r > rtol * r0) {
       if ((rc = pmg_vcycle(S, &r, nullptr, nullptr))) return rc;

@@ -1,0 +1,148 @@
+"""Batched multigrid over an AMR hierarchy on the GPU vs the multi-patch
+oracle (oracle/amr_oracle.c, pinned in test_oracle_amr.py).
+
+Bar: cmask rules, every transfer (sync with AMR boundary width 0 and 1,
+inject with straight / half-weight / full-weight restriction, second-order
+interp) and whole V-cycles bit for bit on every grid of every level; the
+per-cycle norms within 1e-12 relative (they differ only in summation order).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import pamr_amrd_amd as pm
+from pamr_amrd_amd import hierarchy as H
+
+pytestmark = pytest.mark.gpu
+
+FIELD = {O.OA_U: pm.PMG_F_U, O.OA_RHO: pm.PMG_F_RHO, O.OA_RES: pm.PMG_F_RES, O.OA_A: pm.PMG_F_A,
+         O.OA_C: pm.PMG_F_C}
+
+
+def layout(seed, base_n=17, nref=2, nmin=3, nmax=6, smin=5, smax=13):
+    return H.random_boxes(base_n=base_n, nref=nref, nmin=nmin, nmax=nmax, smin=smin, smax=smax, seed=seed)
+
+
+def pair(seed, op=0, base_n=17, **kw):
+    levels = layout(seed, base_n=base_n, **kw)
+    print(f"case amr seed={seed} op={op} base={base_n} boxes={[len(l) for l in levels]}", flush=True)
+    return O.AmrMG(base_n, levels, op=op), pm.Amr(base_n, levels, op=op)
+
+
+def each_grid(m):
+    for L in range(1, m.nlev + 1):
+        for g in range(m.ngrids(L)):
+            yield L, g
+
+
+def test_hierarchy_and_cmask(torch_cuda):
+    for seed in (1, 2, 3):
+        m, a = pair(seed)
+        assert a.nlev == m.nlev and a.base == m.base
+        for L, g in each_grid(m):
+            assert a.ngrids(L) == m.ngrids(L)
+            sm, bm = m.grid(L, g)
+            sa, ba = a.grid(L, g)
+            assert sm == sa and np.array_equal(bm, ba)
+        for L in range(1, m.nlev + 1):
+            for which in (0, 1):
+                m.set_cmask(L, which)
+                for g in range(m.ngrids(L)):
+                    assert np.array_equal(a.cmask(L, g, which), m.field(L, g, O.OA_CMASK)), (L, g, which)
+        a.close()
+
+
+def fill_random(m, a, levels, which, rng):
+    for L in levels:
+        for g in range(m.ngrids(L)):
+            v = rng.standard_normal(m.grid(L, g)[0])
+            m.field(L, g, which)[:] = v
+            a.upload(L, g, FIELD[which], v)
+
+
+def compare(m, a, levels, which):
+    for L in levels:
+        for g in range(m.ngrids(L)):
+            got = a.download(L, g, FIELD[which])
+            want = m.field(L, g, which)
+            assert np.array_equal(got, want), (L, g, np.nanmax(np.abs(got - want)))
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_transfers_bit_exact(torch_cuda, seed):
+    m, a = pair(seed)
+    rng = np.random.default_rng(seed)
+    Lf = m.nlev
+    for bdy in (0, 1):
+        fill_random(m, a, [Lf], O.OA_U, rng)
+        m.sync_field(Lf, bdy, O.OA_U)
+        a.sync(Lf, bdy, pm.PMG_F_U)
+        compare(m, a, [Lf], O.OA_U)
+    for op in (1, 2, 3):
+        for lf in (Lf, Lf - 1, m.base):
+            fill_random(m, a, [lf, lf - 1], O.OA_RES, rng)
+            m.inject_field(lf, O.OA_RES, op)
+            a.inject(lf, pm.PMG_F_RES, op)
+            compare(m, a, [lf - 1], O.OA_RES)
+    for lc in (m.base, m.base + 1):
+        fill_random(m, a, [lc, lc + 1], O.OA_RES, rng)
+        m.interp_field(lc, O.OA_RES)
+        a.interp(lc, pm.PMG_F_RES)
+        compare(m, a, [lc + 1], O.OA_RES)
+    a.close()
+
+
+def set_sources(m, a, op, seed):
+    for L in range(m.base, m.nlev + 1):
+        for g in range(m.ngrids(L)):
+            shape, bb = m.grid(L, g)
+            rho = -H.gaussian_on(shape, bb, seed=seed)
+            m.field(L, g, O.OA_RHO)[:] = rho
+            a.upload(L, g, pm.PMG_F_RHO, rho)
+            if op == 1:
+                av = H.lognormal_on(shape, bb)
+                cv = H.uniform_on(shape, bb)
+                m.field(L, g, O.OA_A)[:] = av
+                m.field(L, g, O.OA_C)[:] = cv
+                a.upload(L, g, pm.PMG_F_A, av)
+                a.upload(L, g, pm.PMG_F_C, cv)
+
+
+@pytest.mark.parametrize("seed,op,graph", [(1, 0, True), (2, 0, True), (3, 0, False), (4, 1, True), (5, 1, True)])
+def test_vcycles_bit_exact(torch_cuda, seed, op, graph):
+    m, a = pair(seed, op=op)
+    a.set_graph(graph)
+    set_sources(m, a, op, 1234 + seed)
+    for x in (m, a):
+        x.set_params()
+        x.init()
+    for L, g in each_grid(m):
+        for w in (O.OA_U, O.OA_RHO):
+            assert np.array_equal(a.download(L, g, FIELD[w]), m.field(L, g, w)), ("init", L, g, w)
+    for c in range(5):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = a.vcycle()
+        print(f"cycle {c}: oracle {rm:.15e} gpu {ra:.15e} citer {cm} {ca}", flush=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert cm == ca
+        assert np.allclose(na, nm, rtol=1e-12, atol=0), (c, na, nm)
+    for L, g in each_grid(m):
+        assert np.array_equal(a.download(L, g, pm.PMG_F_U), m.field(L, g, O.OA_U)), ("u", L, g)
+    a.close()
+
+
+def test_larger_layout_cycles(torch_cuda):
+    """base 33^3 with 6-10 boxes of 9-25 points per refined level"""
+    m, a = pair(11, base_n=33, nmin=6, nmax=10, smin=9, smax=25)
+    set_sources(m, a, 0, 99)
+    for x in (m, a):
+        x.set_params()
+        x.init()
+    for c in range(3):
+        rm, _, _ = m.vcycle()
+        ra, _, _ = a.vcycle()
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+    for L, g in each_grid(m):
+        assert np.array_equal(a.download(L, g, pm.PMG_F_U), m.field(L, g, O.OA_U)), ("u", L, g)
+    print("plan stats", [a.plan_stats(L) for L in range(1, a.nlev + 1)], flush=True)
+    a.close()
