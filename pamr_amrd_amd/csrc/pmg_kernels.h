@@ -305,9 +305,17 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
 
 // ---------------------------------------------------------------------------
 // Residual res = L[u] - rhs on updatable points (num_w.f:59-90).  Other
-// points: ZERO mode writes 0 (clear_MG_vars then residual, amrd/mg.c:91,104
-// on the finest level); KEEP mode leaves them (coarser levels, whose other
-// points already hold the injected values).
+// interior points: ZERO mode writes 0 (clear_MG_vars then residual,
+// amrd/mg.c:91,104 on the finest level); KEEP mode leaves them (coarser
+// levels, whose other points already hold the injected values).  Boundary
+// points are never written: on the levels where ZERO mode is used their
+// res (and the FP32 defect) are zero from allocation and no kernel writes
+// them, which is what clear_MG_vars leaves there.
+//
+// A block owns an RTX x RTY tile of interior (x, y) columns and marches in
+// z: the u plane k (tile + 1-point ring) sits in LDS for the x/y
+// neighbours, u(k-1), u(k), u(k+1) of the own column in registers, so each
+// u value is read from HBM once (plus the ring).
 // ---------------------------------------------------------------------------
 template <typename T>
 struct ResidP {
@@ -324,49 +332,83 @@ struct ResidP {
   int nb;
 };
 
-// block = RTX x RTY columns of all grid points, a z chunk of RZC planes
-// (unrolled: the loads of all RZC points are issued together)
-constexpr int RTX = 64, RTY = 4, RZC = 4;
+constexpr int RTX = 64, RTY = 8, RZC = 16;
+constexpr int kResRing = 2 * RTX + 2 * RTY + 4;
 
 template <typename T, int OP, bool EX, bool P2, bool RHS0, bool ZERO, bool OUT32>
 __global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
-  const int nx = P.nx, ny = P.ny;
+  __shared__ T sP[RTY + 2][RTX + 2];
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
   const size_t sy = nx, sz = (size_t)nx * ny;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int nty = (ny + RTY - 1) / RTY, ntx = (nx + RTX - 1) / RTX;
-  const int ngz = (int)(gridDim.x / (nty * ntx));
-  const int tyi = tile % nty, txi = (tile / nty) % ntx, gz = tile / (nty * ntx);
-  const int i = txi * RTX + threadIdx.x % RTX;
-  const int j = tyi * RTY + threadIdx.x / RTX;
-  double s = 0.0, cnt = 0.0;
+  const int nty = (ny - 2 + RTY - 1) / RTY, ntx = (nx - 2 + RTX - 1) / RTX;
+  const int ty = tile % nty, tx = (tile / nty) % ntx, tz = tile / (nty * ntx);
+  const int x0 = 1 + tx * RTX, y0 = 1 + ty * RTY;
+  const int z0 = 1 + tz * P.zchunk, z1 = min(z0 + P.zchunk, nz - 1);
+  const int lx = threadIdx.x % RTX, ly = threadIdx.x / RTX;
+  const int i = x0 + lx, j = y0 + ly;
+  const bool inside = i <= nx - 1 && j <= ny - 1;  // a grid point (interior or boundary)
+  const bool interior = i <= nx - 2 && j <= ny - 2;
+  const bool inxy = interior && i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1;
+  // ring point of this thread (threads 0 .. kResRing-1)
+  int hx = -1, hy = -1;
+  {
+    const int t = threadIdx.x;
+    if (t < RTX + 2) { hx = t; hy = 0; }
+    else if (t < 2 * (RTX + 2)) { hx = t - (RTX + 2); hy = RTY + 1; }
+    else if (t < 2 * (RTX + 2) + RTY) { hx = 0; hy = 1 + t - 2 * (RTX + 2); }
+    else if (t < kResRing) { hx = RTX + 1; hy = 1 + t - 2 * (RTX + 2) - RTY; }
+  }
+  const int hi = x0 - 1 + hx, hj = y0 - 1 + hy;
+  const bool hvalid = hx >= 0 && hi <= nx - 1 && hj <= ny - 1;
   const T *__restrict__ u = P.u;
-  const T *__restrict__ rho = P.rho;
-  const T *__restrict__ rhs = P.rhs;
-  T *__restrict__ res = P.res;
-  float *__restrict__ neg32 = P.neg32;
-  const int nzc = (P.nz + RZC - 1) / RZC;
-  if (i < nx && j < ny) {
-    const bool inxy = i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1;
-    for (int zc = gz; zc < nzc; zc += ngz) {
-#pragma unroll
-      for (int kk = 0; kk < RZC; kk++) {
-        const int k = zc * RZC + kk;
-        if (k >= P.nz) break;
-        const size_t p = ix3(i, j, k, nx, ny);
-        bool on = inxy && k >= P.bz0 && k <= P.bz1;
-        if constexpr (EX) on = on && !P.chr[p];
-        if (on) {
-          T L = lop_at<T, OP, P2>(u, rho, P.a, P.c, p, sy, sz, P.sp, nullptr);
-          T r;
-          if constexpr (RHS0) r = L - T(0); else r = L - rhs[p];
-          if constexpr (OUT32) neg32[p] = (float)(-r); else res[p] = r;
-          s += (double)r * (double)r;
-          cnt += 1.0;
-        } else if constexpr (ZERO) {
-          if constexpr (OUT32) neg32[p] = 0.0f; else res[p] = T(0);
-        }
+  double s = 0.0, cnt = 0.0;
+  T um = T(0), uc = T(0), up = T(0);
+  if (inside) {
+    um = u[ix3(i, j, z0 - 1, nx, ny)];
+    uc = u[ix3(i, j, z0, nx, ny)];
+  }
+  T hv = hvalid ? u[ix3(hi, hj, z0, nx, ny)] : T(0);
+  for (int k = z0; k < z1; k++) {
+    const size_t p = ix3(i, j, k, nx, ny);
+    if (inside) up = u[p + sz];
+    T hn = T(0);
+    if (hvalid && k + 1 < nz) hn = u[ix3(hi, hj, k + 1, nx, ny)];
+    T rhov = T(0), rhsv = T(0);
+    bool on = inxy && k >= P.bz0 && k <= P.bz1;
+    if constexpr (EX) { if (interior) on = on && !P.chr[p]; }
+    if (on) {
+      rhov = P.rho[p];
+      if constexpr (!RHS0) rhsv = P.rhs[p];
+    }
+    __syncthreads();  // the previous plane's readers are done
+    sP[ly + 1][lx + 1] = uc;
+    if (hvalid) sP[hy][hx] = hv;
+    __syncthreads();
+    if (on) {
+      T L;
+      if constexpr (OP == 0) {
+        L = lop_poisson<T, P2>(uc, sP[ly + 1][lx + 2], sP[ly + 1][lx], sP[ly + 2][lx + 1], sP[ly][lx + 1], up, um,
+                               rhov, P.sp);
+      } else {
+        const T *__restrict__ a = P.a;
+        L = lop_varcoef<T, P2>(uc, sP[ly + 1][lx + 2], sP[ly + 1][lx], sP[ly + 2][lx + 1], sP[ly][lx + 1], up, um,
+                               a[p], a[p + 1], a[p - 1], a[p + sy], a[p - sy], a[p + sz], a[p - sz], P.c[p], rhov,
+                               P.sp, nullptr);
+      }
+      T r;
+      if constexpr (RHS0) r = L - T(0); else r = L - rhsv;
+      if constexpr (OUT32) P.neg32[p] = (float)(-r); else P.res[p] = r;
+      s += (double)r * (double)r;
+      cnt += 1.0;
+    } else if constexpr (ZERO) {
+      if (interior) {
+        if constexpr (OUT32) P.neg32[p] = 0.0f; else P.res[p] = T(0);
       }
     }
+    um = uc;
+    uc = up;
+    hv = hn;
   }
   norm_partial<RTX * RTY>(s, cnt, P.ws, P.slot, blockIdx.x, P.nb);
 }
@@ -446,36 +488,61 @@ __global__ __launch_bounds__(256) void k_lop_rhs(LopRhsP<T> P) {
 }
 
 // ---------------------------------------------------------------------------
-// Coarse-grid correction, no excision (amrd/mg.c:323-355): with
-// R = (u_c - lop_c)/w0_i at coarse points, u_f += P(R) where P is the
+// Coarse-grid correction, no excision (amrd/mg.c:323-355): the coarse
+// correction R = (u_c - lop_c)/w0_i is first stored in the coarse f_res
+// (k_cgc_diff, as the reference does), then u_f += P(R) where P is the
 // dminterp3d trilinear composition (x, then y, then z phases; c0=c1=0.5 for
-// rho=2) -- computed per fine point, so the interpolated f_res is consumed
+// rho=2) evaluated per fine point -- the interpolated fine f_res is consumed
 // directly instead of being stored.  Boundary points also go to `alt`.
 // ---------------------------------------------------------------------------
+//
+// One thread per coarse point (ci, cj, ck): it reads the (up to) 8 corners
+// of its coarse cell once and updates the fine points 2c + r, r in {0,1}^3
+// inside the grid, with the per-point expression of the x/y/z phases.
 template <typename T>
 __global__ __launch_bounds__(256) void k_cgc(T *__restrict__ u, T *__restrict__ alt,
-                                             const T *__restrict__ uc, const T *__restrict__ lopc,
-                                             T w0_i, int nx, int ny, int nz, int cnx, int cny,
-                                             const uint8_t *__restrict__ chr) {
-  PMG_IJK(nx, ny)
-  const size_t q = ix3(i, j, k, nx, ny);
-  const int ai = i >> 1, ri = i & 1, aj = j >> 1, rj = j & 1, ak = k >> 1, rk = k & 1;
+                                             const T *__restrict__ Rc, int nx, int ny, int nz, int cnx,
+                                             int cny, int cnz, const uint8_t *__restrict__ chr) {
+  const int ci = blockIdx.x * EX_ + (int)(threadIdx.x % EX_);
+  const int cj = blockIdx.y * EY_ + (int)(threadIdx.x / EX_);
+  const int ck = blockIdx.z;
+  if (ci >= cnx || cj >= cny) return;
   const T c0 = T(1 - (1 * 1.0) / 2), c1 = T((1 * 1.0) / 2);
-  auto R = [&](int a, int b, int c) -> T {
-    const size_t pc = ix3(a, b, c, cnx, cny);
-    return (uc[pc] - lopc[pc]) / w0_i;
-  };
-  auto X = [&](int b, int c) -> T { return ri == 0 ? R(ai, b, c) : c0 * R(ai, b, c) + c1 * R(ai + 1, b, c); };
-  auto Y = [&](int c) -> T { return rj == 0 ? X(aj, c) : c0 * X(aj, c) + c1 * X(aj + 1, c); };
-  T v = rk == 0 ? Y(ak) : c0 * Y(ak) + c1 * Y(ak + 1);
-  if (chr && chr[q]) v = T(0);
-  const T nu = u[q] + v;
-  u[q] = nu;
-  if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) alt[q] = nu;
+  const bool xi = 2 * ci + 1 <= nx - 1, yi = 2 * cj + 1 <= ny - 1, zi = 2 * ck + 1 <= nz - 1;
+  T r[2][2][2];  // [c][b][a] corners (a along x)
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const bool need = (a == 0 || xi) && (b == 0 || yi) && (c == 0 || zi);
+        r[c][b][a] = need ? Rc[ix3(ci + a, cj + b, ck + c, cnx, cny)] : T(0);
+      }
+#pragma unroll
+  for (int rk = 0; rk < 2; rk++) {
+    if (rk == 1 && !zi) continue;
+#pragma unroll
+    for (int rj = 0; rj < 2; rj++) {
+      if (rj == 1 && !yi) continue;
+#pragma unroll
+      for (int ri = 0; ri < 2; ri++) {
+        if (ri == 1 && !xi) continue;
+        const int i = 2 * ci + ri, j = 2 * cj + rj, k = 2 * ck + rk;
+        auto X = [&](int b, int c) -> T { return ri == 0 ? r[c][b][0] : c0 * r[c][b][0] + c1 * r[c][b][1]; };
+        auto Y = [&](int c) -> T { return rj == 0 ? X(0, c) : c0 * X(0, c) + c1 * X(1, c); };
+        T v = rk == 0 ? Y(0) : c0 * Y(0) + c1 * Y(1);
+        const size_t q = ix3(i, j, k, nx, ny);
+        if (chr && chr[q]) v = T(0);
+        const T nu = u[q] + v;
+        u[q] = nu;
+        if (i == 0 || j == 0 || k == 0 || i == nx - 1 || j == ny - 1 || k == nz - 1) alt[q] = nu;
+      }
+    }
+  }
 }
 
-// coarse res = (u - lop)/w0_i (first loop of compute_cgc), for the
-// excision path where dminterp3d_ex runs as separate line passes
+// coarse res = (u - lop)/w0_i (first loop of compute_cgc)
 template <typename T>
 __global__ __launch_bounds__(256) void k_cgc_diff(T *res, const T *u, const T *lop, T w0_i,
                                                   size_t n) {

@@ -288,9 +288,11 @@ int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
 
 template <typename T, int OP, bool EX, bool P2>
 int launch_residual_t(pmg_solver *S, ResidP<T> P, bool rhs0, bool zero) {
-  P.zchunk = RZC;
-  const int nzc = (P.nz + RZC - 1) / RZC;
-  dim3 grid(((P.nx + RTX - 1) / RTX) * ((P.ny + RTY - 1) / RTY) * (nzc < 16 ? nzc : 16));
+  // z depth per block: long marches on big levels (HBM reuse), short ones on
+  // small levels (latency: a block's planes are processed one after another)
+  P.zchunk = std::max(1, std::min(std::min(RZC, std::max(2, (P.nz - 2) / 16)), P.nz - 2));
+  const int nzc = (P.nz - 2 + P.zchunk - 1) / P.zchunk;
+  dim3 grid(((P.nx - 2 + RTX - 1) / RTX) * ((P.ny - 2 + RTY - 1) / RTY) * nzc);
   const int nb = (int)grid.x;
   P.nb = nb;
   if (nb > S->ws.cap) return -3;
@@ -426,9 +428,10 @@ int cgc(pmg_solver *S, int L) {
   bool ex = false;
   if constexpr (std::is_same<T, double>::value) ex = S->do_ex != 0;
   if (!ex) {
-    hipLaunchKernelGGL(k_cgc<T>, grid3(F.n[0], F.n[1], F.n[2]), dim3(256), 0, S->stream, AF.u, AF.ua, AC.u,
-                       AC.lop, (T)S->w0_i, F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1],
-                       (const uint8_t *)F.chr);
+    hipLaunchKernelGGL(k_cgc_diff<T>, dim3(nblk256(Cl.size)), dim3(256), 0, S->stream, AC.res, AC.u, AC.lop,
+                       (T)S->w0_i, Cl.size);
+    hipLaunchKernelGGL(k_cgc<T>, grid3(Cl.n[0], Cl.n[1], Cl.n[2]), dim3(256), 0, S->stream, AF.u, AF.ua, AC.res,
+                       F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2], (const uint8_t *)F.chr);
     PMG_HOST_CHECK(hipGetLastError());
     return 0;
   }
