@@ -35,6 +35,23 @@ METRIC = "time to 1e-10 rel. residual per 257^3 solve; learned+FP32 vs plain FP6
 HBM_PEAK_GBS = 8000.0
 
 
+SWEEP_SHAPES = {0: (64, 8, 256), 1: (64, 16, 512), 2: (32, 16, 512), 3: (128, 8, 512)}
+
+
+def measured_traffic(n, variant, prec="double"):
+    """HBM bytes per launch of the finest-level sweep from the committed
+    rocprofv3 FETCH_SIZE/WRITE_SIZE summary (copy-kernel calibrated), when
+    it covers this grid size and tile shape; else None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_finest_{n}.json")
+    if not os.path.exists(path) or variant not in SWEEP_SHAPES:
+        return None
+    tx, ty, nt = SWEEP_SHAPES[variant]
+    name = f"pmg::k_sweep<{prec}, 0, false, true, true, {tx}, {ty}, {nt}>"
+    with open(path) as f:
+        entry = json.load(f).get("kernels", {}).get(name)
+    return None if entry is None else entry["traffic_bytes"]
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -140,6 +157,127 @@ def sequence_leg(args, pm, torch):
     }
 
 
+def amr_leg(args, pm, torch):
+    """Batched AMR hierarchy: base 129^3, refinement ratio 2, two refined
+    levels of 24-48 seeded boxes with odd sides in 17..97 clipped to their
+    parents, div(a grad u) + c u (a lognormal, sigma 0.5, correlation 0.1;
+    c uniform in [0,1]), FP64 V(2,2) cycles, one launch per level operation,
+    each cycle one hipGraph."""
+    import numpy as np
+    from pamr_amrd_amd import hierarchy as H
+    levels = H.random_boxes(base_n=129, nref=2, nmin=24, nmax=48, smin=17, smax=97, seed=7)
+    A = pm.Amr(129, levels, op=1)
+    npts = {}
+    for L in range(A.base, A.nlev + 1):
+        tot = 0
+        for g in range(A.ngrids(L)):
+            shape, bb = A.grid(L, g)
+            A.upload(L, g, pm.PMG_F_RHO, -H.gaussian_on(shape, bb))
+            A.upload(L, g, pm.PMG_F_A, H.lognormal_on_device(shape, bb, torch).cpu().numpy())
+            A.upload(L, g, pm.PMG_F_C, H.uniform_on(shape, bb))
+            tot += int(np.prod(shape))
+        npts[L] = tot
+    A.set_params(pre=2, pst=2, max_citer=5000, crtol=1e-3)
+    A.init()
+    hist = []
+    for _ in range(2):
+        hist.append(A.vcycle()[0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ncyc = args.amr_cycles
+    for _ in range(ncyc):
+        hist.append(A.vcycle()[0])
+    torch.cuda.synchronize()
+    ms_cycle = 1e3 * (time.perf_counter() - t0) / ncyc
+    Lf = A.nlev
+    t_sw = A.time_sweep(Lf, 10)
+    interior = sum(int(np.prod([s - 2 for s in A.grid(Lf, g)[0]])) for g in range(A.ngrids(Lf)))
+    # finest AMR level sweep: read u, rho, a, c, write u (rhs = 0 there)
+    bytes_sw = 40 * interior
+    stats = {L: A.plan_stats(L) for L in range(1, A.nlev + 1)}
+    out = {
+        "workload": "amr129_2lev_24to48boxes_varcoef",
+        "levels": A.nlev, "grids_per_level": [A.ngrids(L) for L in range(1, A.nlev + 1)],
+        "points_per_amr_level": npts,
+        "ms_per_vcycle": round(ms_cycle, 4), "vcycles_per_s": round(1e3 / ms_cycle, 1),
+        "residual_history": [float(f"{h:.6e}") for h in hist],
+        "finest_batched_sweep": {"grids": A.ngrids(Lf), "interior_points": interior, "avg_ms": round(t_sw, 5),
+                                 "algorithmic_bytes": bytes_sw,
+                                 "achieved_GBps": round(bytes_sw / (t_sw * 1e-3) / 1e9, 1)},
+        "transfer_boxes_per_level": {L: {"sync0": v[0], "sync1": v[1], "inject": v[2], "interp": v[3]}
+                                     for L, v in stats.items()},
+        "dtype": "f64",
+    }
+    A.close()
+    return out
+
+
+def excised_leg(args, pm, torch):
+    """Large excised problem: div(a grad u) + c u on n^3 (default 513^3), a
+    lognormal, c uniform in [0,1], 3 seeded excised spheres (chr on every
+    level, Dirichlet data g held on the excised points), 16 seeded Gaussian
+    right-hand sides, each solved to ||r|| <= rtol ||r(u0)|| with FP32 inner
+    V-cycles in FP64 defect correction; one right-hand side also in plain
+    FP64 for comparison."""
+    import numpy as np
+    from pamr_amrd_amd import hierarchy as H
+    from pamr_amrd_amd import sequence as SQ
+    n = args.c5_n
+    box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
+    spheres = H.excision_spheres(seed=21)
+    S = pm.Solver(n, op=1, do_ex=1, ex=1.0)
+    S.set_params(pre=2, pst=2, max_citer=5000, crtol=1e-3, rop_u=3, rop_res=3, rop_src=3)
+    Lf = S.nlev
+    for lev in range(1, Lf + 1):
+        S.set_chr(lev, H.chr_on(S.shape(lev), box, spheres))
+    shape = (n, n, n)
+    a = H.lognormal_on_device(shape, box, torch)
+    SQ._copy_grid(S.field_ptr(Lf, pm.PMG_F_A), pm.tptr(a), n)
+    del a
+    gen = torch.Generator(device="cuda").manual_seed(5)
+    c = torch.rand(shape, generator=gen, dtype=torch.float64, device="cuda")
+    SQ._copy_grid(S.field_ptr(Lf, pm.PMG_F_C), pm.tptr(c), n)
+    del c
+    ch = H.chr_on(shape, box, spheres)
+    excised = int((ch == 1.0).sum())
+    u0 = torch.from_numpy(np.where(ch == 1.0, H.excision_data(shape, box), 0.0)).cuda()
+    del ch
+    L = pm.lib()
+    times, iters, hists = [], [], []
+    plain = None
+    for r in range(args.c5_rhs):
+        src = SQ.DriftingSource(seed=1234 + r, speed=0.0)
+        rho = src.rho(n, 0, torch)
+        SQ._copy_grid(S.field_ptr(Lf, pm.PMG_F_RHO), pm.tptr(rho), n)
+        del rho
+        for mode in ((1, 0) if r == 0 else (1,)):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            L.pmg_set_solution_lincomb(S.h, pm.tptr(u0), None, 1.0, 0.0)
+            S.init()
+            r0, h = S.solve_rel(mode, args.rtol, args.c5_max_iter if mode == 1 else args.c5_plain_cycles)
+            torch.cuda.synchronize()
+            dt = 1e3 * (time.perf_counter() - t0)
+            if mode == 1:
+                if len(h) == 0 or h[-1] > args.rtol * r0:
+                    raise RuntimeError(f"excised rhs {r} did not converge in {len(h)} iterations: {h[-3:]}")
+                times.append(dt)
+                iters.append(len(h))
+                hists.append((h[-1] / r0) ** (1.0 / len(h)))
+            else:
+                # the reference's excision transfers make plain FP64 V-cycles
+                # contract slowly here; report where the capped run ended
+                plain = {"ms": round(dt, 2), "cycles": len(h), "final_rel_residual": float(h[-1] / r0),
+                         "converged": bool(h[-1] <= args.rtol * r0),
+                         "mean_contraction": float((h[-1] / r0) ** (1.0 / len(h)))}
+    S.close()
+    return {"workload": f"excised{n}_3spheres_varcoef_16rhs", "n": n, "excised_points": excised,
+            "rhs": args.c5_rhs, "mixed_ms_per_rhs": round(sum(times) / len(times), 2),
+            "mixed_total_ms": round(sum(times), 1), "mixed_outer_iters": iters,
+            "mixed_mean_contraction": None if not hists else float(np.mean(hists)),
+            "plain_fp64_first_rhs": plain, "rtol": args.rtol, "reference_norm": "||r(u0)||"}
+
+
 def probe_kernels(S, n, pm, torch):
     """Launch each finest-level kernel of interest 4 times, nothing else
     large: the counter runs attribute FETCH_SIZE / WRITE_SIZE per dispatch."""
@@ -175,6 +313,13 @@ def main():
     ap.add_argument("--seq-train-steps", type=int, default=1500)
     ap.add_argument("--seq-warmup", type=int, default=2)
     ap.add_argument("--no-seq", action="store_true", help="skip the learned sequence leg")
+    ap.add_argument("--no-amr", action="store_true", help="skip the batched AMR hierarchy leg")
+    ap.add_argument("--amr-cycles", type=int, default=10)
+    ap.add_argument("--no-excised", action="store_true", help="skip the large excised leg")
+    ap.add_argument("--c5-n", type=int, default=513)
+    ap.add_argument("--c5-rhs", type=int, default=16)
+    ap.add_argument("--c5-plain-cycles", type=int, default=60)
+    ap.add_argument("--c5-max-iter", type=int, default=300)
     ap.add_argument("--probe-kernels", action="store_true",
                     help="only launch the finest-level kernels (for rocprofv3 --pmc runs): "
                          "FP64 sweep, FP64 residual, FP32 sweep and a plain 8 B/point copy "
@@ -246,11 +391,12 @@ def main():
     ach64 = bytes64 / (t64 * 1e-3) / 1e9
     ach32 = bytes32 / (t32 * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": round(ach64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(ach64 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, variant),
             "kernel": "k_sweep<double> finest level (fused red-black sweep, 255^3 interior)",
             "algorithmic_bytes": bytes64, "avg_ms": round(t64, 5)}
     roof32 = {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-              "frac": round(ach32 / HBM_PEAK_GBS, 4), "algorithmic_bytes": bytes32,
+              "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, variant, "float"),
+              "algorithmic_bytes": bytes32,
               "avg_ms": round(t32, 5)}
     cpu = None
     if not args.no_cpu:
@@ -261,6 +407,14 @@ def main():
               "speedup_mixed_vs_plain_fp64": round(ms_plain / ms_mixed, 3),
               "plain_cycles": len(hp), "mixed_outer_iters": len(hm),
               "final_rel_residual": {"plain": hp[-1] / r0p, "mixed": hm[-1] / r0m}}
+    amrres = None
+    if not args.no_amr:
+        print("bench: batched AMR hierarchy leg", file=sys.stderr, flush=True)
+        amrres = amr_leg(args, pm, torch)
+    exres = None
+    if not args.no_excised:
+        print("bench: large excised leg", file=sys.stderr, flush=True)
+        exres = excised_leg(args, pm, torch)
     seqres = None
     if not args.no_seq:
         print("bench: learned sequence leg", file=sys.stderr, flush=True)
@@ -297,6 +451,8 @@ def main():
         "solves_per_s": round(1e3 / value, 2),
         "sequence": seqres,
         "single_solve": single,
+        "amr_hierarchy": amrres,
+        "excised": exres,
         "roofline": roof,
         "roofline_fp32_sweep": roof32,
         "cpu_baseline": cpu,

@@ -779,17 +779,21 @@ static void restrict_field(or_mg *m, int Lfine, int w, int op)
   const int *sf = m->shape[Lfine - 1], *sc = m->shape[Lfine - 2];
   const double *src = m->f[Lfine - 1][w];
   double *dst = m->f[Lfine - 2][w];
-  const double *chr = m->do_ex ? m->f[Lfine - 1][F_CHR] : src;
+  /* the coefficients of the variable-coefficient operator (a project-defined
+   * hook with no reference counterpart) have no excised value: they are
+   * restricted without the excision renormalisation */
+  const int do_ex = m->do_ex && w != F_A && w != F_C;
+  const double *chr = do_ex ? m->f[Lfine - 1][F_CHR] : src;
   if (!src || !dst || op == 0) return;
   if (op == 1)
     or_dmcopy3d(src, dst, sf[0], sf[1], sf[2], sc[0], sc[1], sc[2], 1, 1, 1, 1, 1, 1, sc[0], sc[1], sc[2],
                 2, 2, 2, 1, 1, 1);
   else if (op == 2)
     or_dmhwr3d(src, dst, sf[0], sf[1], sf[2], sc[0], sc[1], sc[2], 1, 1, 1, 1, 1, 1, sc[0], sc[1], sc[2],
-               2, 2, 2, 1, 1, 1, chr, m->ex, m->do_ex);
+               2, 2, 2, 1, 1, 1, chr, m->ex, do_ex);
   else
     or_dmfwr3d(src, dst, sf[0], sf[1], sf[2], sc[0], sc[1], sc[2], 1, 1, 1, 1, 1, 1, sc[0], sc[1], sc[2],
-               2, 2, 2, 1, 1, 1, chr, m->ex, m->do_ex);
+               2, 2, 2, 1, 1, 1, chr, m->ex, do_ex);
 }
 
 /* PAMR_inject(L+1 -> L) on a pure MG level: TF_INJECT_TO_MG_LEV selects

@@ -843,8 +843,11 @@ int pmg_init(pmg_solver *S) {
     if ((rc = restrict_any<double>(S, L, F.d.u, Cl.d.u, nullptr, S->rop_u))) return rc;
     if ((rc = restrict_any<double>(S, L, F.d.rho, Cl.d.rho, nullptr, S->rop_src))) return rc;
     if (S->op == 1) {
-      if ((rc = restrict_any<double>(S, L, F.d.a, Cl.d.a, nullptr, S->rop_src))) return rc;
-      if ((rc = restrict_any<double>(S, L, F.d.c, Cl.d.c, nullptr, S->rop_src))) return rc;
+      // the coefficients of the (project-defined) variable-coefficient
+      // operator have no excised value: restricted without the excision
+      // renormalisation, also when excision is on
+      if ((rc = restrict_field<double>(S, L, F.d.a, Cl.d.a, nullptr, S->rop_src, false))) return rc;
+      if ((rc = restrict_field<double>(S, L, F.d.c, Cl.d.c, nullptr, S->rop_src, false))) return rc;
     }
   }
   for (int L = 1; L <= S->nlev; L++) {

@@ -110,6 +110,43 @@ def lognormal_on(shape, bbox, seed=99, sigma=0.5, ell=0.1, modes=64):
     return np.exp(sigma * g)
 
 
+def lognormal_on_device(shape, bbox, torch, seed=99, sigma=0.5, ell=0.1, modes=64):
+    """lognormal_on evaluated with torch on the GPU (large grids); returns a
+    float64 device tensor of the same values up to the last bits."""
+    k, phase = _rff_params(seed, ell, modes)
+    X, Y, Z = (torch.from_numpy(np.ascontiguousarray(v)).cuda() for v in grid_points(shape, bbox))
+    g = torch.zeros(shape, dtype=torch.float64, device="cuda")
+    for m in range(modes):
+        g += torch.cos(float(k[m, 0]) * X + float(k[m, 1]) * Y + float(k[m, 2]) * Z + float(phase[m]))
+    g *= float(np.sqrt(2.0 / modes))
+    return torch.exp(sigma * g)
+
+
+def excision_spheres(seed=21, count=3, rmin=0.05, rmax=0.12):
+    """Seeded spheres (centre in [0.25, 0.75]^3, radius uniform in [rmin, rmax])."""
+    gen = splitmix64(seed)
+    out = []
+    for _ in range(count):
+        c = np.array([0.25 + 0.5 * _u01(gen) for _ in range(3)])
+        out.append((c, rmin + (rmax - rmin) * _u01(gen)))
+    return out
+
+
+def chr_on(shape, bbox, spheres, ex=1.0):
+    """chr grid function: ex inside any sphere (excised), 0 elsewhere."""
+    X, Y, Z = grid_points(shape, bbox)
+    inside = np.zeros(shape, dtype=bool)
+    for c, r in spheres:
+        inside |= (X - c[0]) ** 2 + (Y - c[1]) ** 2 + (Z - c[2]) ** 2 < r * r
+    return np.where(inside, ex, 0.0)
+
+
+def excision_data(shape, bbox):
+    """Dirichlet data held on the excised points: g = 0.1 (1 + x + y + z)."""
+    X, Y, Z = grid_points(shape, bbox)
+    return np.broadcast_to(0.1 * (1.0 + X + Y + Z), shape).copy()
+
+
 def _mix(z):
     with np.errstate(over="ignore"):
         z = (z + np.uint64(0x9E3779B97F4A7C15)) & _M64
