@@ -151,6 +151,10 @@ int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk);
 int pmg_set_level_sweep_config(pmg_solver *S, int L, int variant, int zchunk);
 int pmg_num_sweep_variants(void);
 int pmg_set_graph(pmg_solver *S, int on);
+/* run the small bottom levels (all dimensions <= 17 points, below the
+ * finest level, no excision) as one single-workgroup kernel (default on);
+ * results are identical either way */
+int pmg_set_bottom_fusion(pmg_solver *S, int on);
 void *pmg_stream(pmg_solver *S);
 
 /* ---- 4. learned smoother / warm-start network (no reference counterpart:

@@ -79,6 +79,7 @@ def _declare(L):
     L.pmg_set_level_sweep_config.argtypes = [_vp, i, i, i]
     L.pmg_num_sweep_variants.argtypes = []
     L.pmg_set_graph.argtypes = [_vp, i]
+    L.pmg_set_bottom_fusion.argtypes = [_vp, i]
     L.pmg_stream.argtypes = [_vp]
     L.pmg_stream.restype = _vp
     L.pmg_net_create.argtypes = [i, C.c_ulonglong]
@@ -253,6 +254,9 @@ class Solver:
 
     def set_graph(self, on):
         _check(self._L.pmg_set_graph(self.h, 1 if on else 0), "pmg_set_graph")
+
+    def set_bottom_fusion(self, on):
+        _check(self._L.pmg_set_bottom_fusion(self.h, 1 if on else 0), "pmg_set_bottom_fusion")
 
     def set_net(self, net):
         """attach a trained network: solve_rel mode 2 applies its correction"""

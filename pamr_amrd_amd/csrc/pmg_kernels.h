@@ -655,6 +655,187 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Bottom of the V-cycle in one workgroup.  Levels 1..nb (all small) are
+// processed from L_op + compute_MG_rhs on the top bottom level, through its
+// pre-sweeps, residual and injection, down to the coarsest solve and back up
+// through the coarse-grid corrections and post-sweeps (amrd/mg.c:84-201
+// restricted to those levels).  The caller has already computed the residual
+// of the level above and injected u and f_res into the top bottom level.
+// Sweeps update in place colour by colour, as the hook does (the fused
+// ping-pong sweep used on large levels gives the same values), so a whole
+// bottom V costs one launch instead of about ten per level.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct BotLev {
+  T *u;
+  const T *rho, *a, *c;
+  T *rhs, *res, *lop;
+  int nx, ny, nz;
+  int bx0, bx1, by0, by1, bz0, bz1;  // relax / residual box
+  int cx0, cx1, cy0, cy1, cz0, cz1;  // child-mask ON box
+  Spacing<T> sp;
+};
+
+constexpr int kBotMax = 6;
+constexpr int kBotThreads = 1024;
+
+template <typename T>
+struct BottomP {
+  BotLev<T> lev[kBotMax];  // lev[0] = MG level 1 (coarsest)
+  int nb;
+  int npre, npst, max_citer;
+  double crtol;
+  T w0_r, w0_i;
+  int rop_u, rop_res;
+  NormWs ws;
+  int relax_slot0;  // level L's last relax norm -> slot relax_slot0 + L - 1
+  int resid_slot0;  // level L's residual norm -> slot resid_slot0 + L - 1
+  double *norm_pre, *norm_post;
+  int *citer;
+};
+
+template <typename T, int OP, bool P2>
+__global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
+  constexpr int NT = kBotThreads;
+  __shared__ double bres;
+  auto put_norm = [&](int slot, double s, double c) {
+    block_reduce2<NT>(s, c);
+    if (threadIdx.x == 0) {
+      double *p = P.ws.partials + (size_t)slot * P.ws.cap * 2;
+      p[0] = s;
+      p[1] = c;
+      P.ws.slot_nb[slot] = 1;
+      bres = sqrt(s / c);
+    }
+    __syncthreads();
+    return bres;
+  };
+  auto box_pts = [&](const BotLev<T> &L, int &mi, int &mj, int &mk) {
+    mi = L.bx1 - L.bx0 + 1; mj = L.by1 - L.by0 + 1; mk = L.bz1 - L.bz0 + 1;
+    return (mi > 0 && mj > 0 && mk > 0) ? mi * mj * mk : 0;
+  };
+  // one red-black sweep in place; returns (sum res^2, count)
+  auto sweep = [&](const BotLev<T> &L, double &s, double &cnt) {
+    int mi, mj, mk;
+    const int npts = box_pts(L, mi, mj, mk);
+    const size_t sy = L.nx, sz = (size_t)L.nx * L.ny;
+    s = 0.0; cnt = 0.0;
+    for (int pass = 0; pass <= 1; pass++) {
+      for (int q = threadIdx.x; q < npts; q += NT) {
+        const int i = L.bx0 + q % mi, j = L.by0 + (q / mi) % mj, k = L.bz0 + q / (mi * mj);
+        if (((i + j + k + 3 + pass) & 1) != 0) continue;
+        const size_t p = ix3(i, j, k, L.nx, L.ny);
+        T jac;
+        const T r = lop_at<T, OP, P2>(L.u, L.rho, L.a, L.c, p, sy, sz, L.sp, &jac) - L.rhs[p];
+        if constexpr (std::is_same<T, float>::value && OP == 0) L.u[p] = L.u[p] - r * L.sp.ijac;
+        else L.u[p] = L.u[p] - r / jac;
+        s += (double)r * (double)r;
+        cnt += 1.0;
+      }
+      __syncthreads();
+    }
+  };
+  auto lop_rhs = [&](const BotLev<T> &L) {
+    const int n = L.nx * L.ny * L.nz;
+    const size_t sy = L.nx, sz = (size_t)L.nx * L.ny;
+    for (int q = threadIdx.x; q < n; q += NT) {
+      const int i = q % L.nx, j = (q / L.nx) % L.ny, k = q / (L.nx * L.ny);
+      const bool on = i >= L.cx0 && i <= L.cx1 && j >= L.cy0 && j <= L.cy1 && k >= L.cz0 && k <= L.cz1;
+      const bool interior = i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2 && k >= 1 && k <= L.nz - 2;
+      T lopv = T(0);
+      if (on && interior) lopv = lop_at<T, OP, P2>(L.u, L.rho, L.a, L.c, (size_t)q, sy, sz, L.sp, nullptr);
+      L.rhs[q] = on ? (-L.res[q]) * P.w0_r + lopv : T(0);
+      L.lop[q] = L.u[q];
+    }
+    __syncthreads();
+  };
+  auto residual = [&](const BotLev<T> &L, double &s, double &cnt) {
+    int mi, mj, mk;
+    const int npts = box_pts(L, mi, mj, mk);
+    const size_t sy = L.nx, sz = (size_t)L.nx * L.ny;
+    s = 0.0; cnt = 0.0;
+    for (int q = threadIdx.x; q < npts; q += NT) {
+      const int i = L.bx0 + q % mi, j = L.by0 + (q / mi) % mj, k = L.bz0 + q / (mi * mj);
+      const size_t p = ix3(i, j, k, L.nx, L.ny);
+      const T r = lop_at<T, OP, P2>(L.u, L.rho, L.a, L.c, p, sy, sz, L.sp, nullptr) - L.rhs[p];
+      L.res[p] = r;
+      s += (double)r * (double)r;
+      cnt += 1.0;
+    }
+    __syncthreads();
+  };
+  auto restrict_to = [&](const T *f, T *cdst, const BotLev<T> &F, const BotLev<T> &C, int kind) {
+    const int n = C.nx * C.ny * C.nz;
+    for (int q = threadIdx.x; q < n; q += NT) {
+      const int ci = q % C.nx, cj = (q / C.nx) % C.ny, ck = q / (C.nx * C.ny);
+      const int i0 = 1 + 2 * ci, j0 = 1 + 2 * cj, k0 = 1 + 2 * ck;
+      T v;
+      if (kind == 1) v = f[ix3(i0 - 1, j0 - 1, k0 - 1, F.nx, F.ny)];
+      else if (kind == 2) v = hw_point<T, false>(f, nullptr, T(0), F.nx, F.ny, F.nz, i0, j0, k0);
+      else v = fw_point<T>(f, F.nx, F.ny, F.nz, i0, j0, k0);
+      cdst[q] = v;
+    }
+  };
+  auto cgc = [&](const BotLev<T> &F, const BotLev<T> &C) {
+    const int nc = C.nx * C.ny * C.nz;
+    for (int q = threadIdx.x; q < nc; q += NT) C.res[q] = (C.u[q] - C.lop[q]) / P.w0_i;
+    __syncthreads();
+    const int n = F.nx * F.ny * F.nz;
+    const T c0 = T(1 - (1 * 1.0) / 2), c1 = T((1 * 1.0) / 2);
+    for (int q = threadIdx.x; q < n; q += NT) {
+      const int i = q % F.nx, j = (q / F.nx) % F.ny, k = q / (F.nx * F.ny);
+      const int ai = i >> 1, ri = i & 1, aj = j >> 1, rj = j & 1, ak = k >> 1, rk = k & 1;
+      auto R = [&](int a, int b, int c) -> T { return C.res[ix3(a, b, c, C.nx, C.ny)]; };
+      auto X = [&](int b, int c) -> T { return ri == 0 ? R(ai, b, c) : c0 * R(ai, b, c) + c1 * R(ai + 1, b, c); };
+      auto Y = [&](int c) -> T { return rj == 0 ? X(aj, c) : c0 * X(aj, c) + c1 * X(aj + 1, c); };
+      const T v = rk == 0 ? Y(ak) : c0 * Y(ak) + c1 * Y(ak + 1);
+      F.u[q] = F.u[q] + v;
+    }
+    __syncthreads();
+  };
+
+  const int top = P.nb - 1;
+  double s, cnt;
+  for (int l = top; l >= 1; l--) {
+    const BotLev<T> &L = P.lev[l];
+    lop_rhs(L);
+    for (int it = 0; it < P.npre; it++) sweep(L, s, cnt);
+    residual(L, s, cnt);
+    put_norm(P.resid_slot0 + l, s, cnt);
+    if (P.rop_u) restrict_to(L.u, P.lev[l - 1].u, L, P.lev[l - 1], P.rop_u);
+    if (P.rop_res) restrict_to(L.res, P.lev[l - 1].res, L, P.lev[l - 1], P.rop_res);
+    __syncthreads();
+  }
+  // coarsest level (amrd/mg.c:135-141, 161-190 at L = 1)
+  {
+    const BotLev<T> &L = P.lev[0];
+    lop_rhs(L);
+    double res = 0.0;
+    for (int it = 0; it < P.npre; it++) { sweep(L, s, cnt); res = put_norm(P.relax_slot0, s, cnt); }
+    const double ctol = res * P.crtol;
+    if (threadIdx.x == 0 && P.norm_pre) P.norm_pre[0] = res;
+    int iter;
+    for (iter = 0; iter < P.npst; iter++) { sweep(L, s, cnt); res = put_norm(P.relax_slot0, s, cnt); }
+    res = ctol + 1;
+    while (iter < P.max_citer && res > ctol) {
+      sweep(L, s, cnt);
+      res = put_norm(P.relax_slot0, s, cnt);
+      iter++;
+    }
+    if (threadIdx.x == 0) {
+      if (P.norm_post) P.norm_post[0] = res;
+      if (P.citer) P.citer[0] = iter;
+    }
+  }
+  for (int l = 1; l <= top; l++) {
+    const BotLev<T> &L = P.lev[l];
+    cgc(L, P.lev[l - 1]);
+    for (int it = 0; it < P.npst; it++) sweep(L, s, cnt);
+    if (P.npst > 0) put_norm(P.relax_slot0 + l, s, cnt);
+  }
+}
+
 // elementwise helpers ---------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(256) void k_copy(T *dst, const T *src, size_t n) {

@@ -129,6 +129,7 @@ struct pmg_solver {
   int sweep_variant = 1, zchunk = 32;
   int lv_variant[kMaxLev], lv_zchunk[kMaxLev];  // per-level sweep configuration
   int use_graph = 1;
+  int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   hipGraphExec_t g64 = nullptr;   // captured FP64 V-cycle
   hipGraphExec_t g32 = nullptr;   // captured mixed-precision outer iteration
   bool fp32_ready = false;
@@ -503,9 +504,67 @@ int fold_norms(pmg_solver *S, int slot0, int nslots) {
   return 0;
 }
 
+// The small levels at the bottom of the V (every dimension at most
+// kBottomMaxN points, below the finest level, no excision) run as one
+// single-workgroup kernel; returns the top level of that range or 0.
+constexpr int kBottomMaxN = 17;
+
+int bottom_top(const pmg_solver *S) {
+  if (S->do_ex || S->nlev < 3) return 0;
+  int Lb = 0;
+  for (int L = 1; L < S->nlev && L <= kBotMax; L++) {
+    const Level &Lv = S->lev[L - 1];
+    if (Lv.n[0] <= kBottomMaxN && Lv.n[1] <= kBottomMaxN && Lv.n[2] <= kBottomMaxN && Lv.rbox.ok) Lb = L;
+    else break;
+  }
+  return Lb >= 2 ? Lb : 0;
+}
+
+template <typename T>
+int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
+  BottomP<T> P;
+  P.nb = Lb;
+  for (int L = 1; L <= Lb; L++) {
+    Level &Lv = S->lev[L - 1];
+    LevelArrays<T> &A = arrays<T>(Lv);
+    BotLev<T> &B = P.lev[L - 1];
+    B.u = A.u; B.rho = A.rho; B.a = A.a; B.c = A.c; B.rhs = A.rhs; B.res = A.res; B.lop = A.lop;
+    B.nx = Lv.n[0]; B.ny = Lv.n[1]; B.nz = Lv.n[2];
+    B.bx0 = Lv.rbox.lo[0]; B.bx1 = Lv.rbox.hi[0];
+    B.by0 = Lv.rbox.lo[1]; B.by1 = Lv.rbox.hi[1];
+    B.bz0 = Lv.rbox.lo[2]; B.bz1 = Lv.rbox.hi[2];
+    if (Lv.cbox.ok) {
+      B.cx0 = Lv.cbox.lo[0]; B.cx1 = Lv.cbox.hi[0];
+      B.cy0 = Lv.cbox.lo[1]; B.cy1 = Lv.cbox.hi[1];
+      B.cz0 = Lv.cbox.lo[2]; B.cz1 = Lv.cbox.hi[2];
+    } else {
+      B.cx0 = B.cy0 = B.cz0 = 1;
+      B.cx1 = B.cy1 = B.cz1 = 0;
+    }
+    B.sp = spacing<T>(Lv);
+  }
+  P.npre = S->pre; P.npst = S->pst; P.max_citer = S->max_citer; P.crtol = S->crtol;
+  P.w0_r = (T)S->w0_r; P.w0_i = (T)S->w0_i;
+  P.rop_u = S->rop_u; P.rop_res = S->rop_res;
+  P.ws = S->ws;
+  P.relax_slot0 = norm_base + NS_RELAX;
+  P.resid_slot0 = norm_base + NS_RESID;
+  P.norm_pre = cnorm; P.norm_post = cnorm + 1; P.citer = citer;
+  const bool p2 = P.lev[0].sp.pow2 != 0;
+  for (int L = 2; L <= Lb; L++)
+    if ((P.lev[L - 1].sp.pow2 != 0) != p2) return -5;
+#define PMG_BOT(OPV, P2V) hipLaunchKernelGGL((k_bottom<T, OPV, P2V>), dim3(1), dim3(kBotThreads), 0, S->stream, P)
+  if (S->op == 0) { if (p2) PMG_BOT(0, true); else PMG_BOT(0, false); }
+  else { if (p2) PMG_BOT(1, true); else PMG_BOT(1, false); }
+#undef PMG_BOT
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
 template <typename T>
 int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cnorm, int *citer) {
   const int Lf = S->nlev;
+  const int Lb = S->use_bottom ? bottom_top(S) : 0;
   int rc;
   for (int L = Lf; L >= 1; L--) {
     if (L < Lf) {
@@ -516,6 +575,10 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
       // PAMR_inject(L+1): u and u_res (sources are constant, restricted once)
       if ((rc = restrict_any<T>(S, L + 1, AF.u, AC.u, AC.ua, S->rop_u))) return rc;
       if ((rc = restrict_any<T>(S, L + 1, AF.res, AC.res, (T *)nullptr, S->rop_res))) return rc;
+      if (L == Lb) {
+        if ((rc = bottom<T>(S, Lb, norm_base, cnorm, citer))) return rc;
+        break;
+      }
       if ((rc = lop_rhs<T>(S, L))) return rc;
     }
     if (L == 1) {
@@ -524,7 +587,7 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
       if ((rc = sweeps<T>(S, L, S->pre, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
     }
   }
-  for (int L = 2; L <= Lf; L++) {
+  for (int L = Lb ? Lb + 1 : 2; L <= Lf; L++) {
     if ((rc = cgc<T>(S, L))) return rc;
     if ((rc = sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
   }
@@ -766,6 +829,13 @@ int pmg_num_sweep_variants(void) { return kNumSweepVariants; }
 int pmg_set_graph(pmg_solver *S, int on) {
   if (!S) return -1;
   S->use_graph = on ? 1 : 0;
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_set_bottom_fusion(pmg_solver *S, int on) {
+  if (!S) return -1;
+  S->use_bottom = on ? 1 : 0;
   invalidate_graphs(S);
   return 0;
 }

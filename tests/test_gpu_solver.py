@@ -112,6 +112,34 @@ def test_graph_replays_match_eager(pm):
     Se.close()
 
 
+@pytest.mark.parametrize("n", [65, 129])
+def test_bottom_fusion_matches_unfused(pm, n):
+    """the single-workgroup bottom V (levels <= 17^3) gives the same u, the
+    same coarsest iteration counts and the same norms (up to summation order)
+    as the per-level launches"""
+    rho = -pm.gaussian_source(n, seed=5)
+    Sf, _ = _pair(pm, n, rho)
+    Su, _ = _pair(pm, n, rho)
+    Su.set_bottom_fusion(False)
+    for _ in range(4):
+        rf, nf, cf = Sf.vcycle()
+        ru, nu, cu = Su.vcycle()
+        assert cf == cu
+        assert abs(rf - ru) <= 1e-13 * ru
+        assert np.allclose(nf, nu, rtol=1e-12, atol=0)
+    for L in range(1, Sf.nlev + 1):
+        assert np.array_equal(Sf.download(L, pm.PMG_F_U), Su.download(L, pm.PMG_F_U)), L
+    # FP32 inner cycles take the fused path too (from u = 0 again)
+    Sf.zero_solution()
+    Su.zero_solution()
+    r0f, hf = Sf.solve_rel(1, 1e-10, 40)
+    r0u, hu = Su.solve_rel(1, 1e-10, 40)
+    assert hf[-1] <= 1e-10 * r0f and hu[-1] <= 1e-10 * r0u
+    assert abs(len(hf) - len(hu)) <= 1
+    Sf.close()
+    Su.close()
+
+
 def test_mixed_precision_reaches_1e10(pm):
     """FP32 inner V-cycles in FP64 defect correction reach ||r||/||r0|| <= 1e-10;
     the error against a 40-cycle FP64 solution is reported and bounded."""
