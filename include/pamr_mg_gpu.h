@@ -155,6 +155,10 @@ int pmg_set_graph(pmg_solver *S, int on);
  * finest level, no excision) as one single-workgroup kernel (default on);
  * results are identical either way */
 int pmg_set_bottom_fusion(pmg_solver *S, int on);
+/* compute the finest-level FP64 residual inside the full-weight injection of u
+ * and f_res instead of storing it (default on; no excision, full weighting
+ * for both); results are identical either way */
+int pmg_set_restrict_fusion(pmg_solver *S, int on);
 void *pmg_stream(pmg_solver *S);
 
 /* ---- 4. learned smoother / warm-start network (no reference counterpart:

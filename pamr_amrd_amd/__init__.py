@@ -80,6 +80,7 @@ def _declare(L):
     L.pmg_num_sweep_variants.argtypes = []
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_set_bottom_fusion.argtypes = [_vp, i]
+    L.pmg_set_restrict_fusion.argtypes = [_vp, i]
     L.pmg_stream.argtypes = [_vp]
     L.pmg_stream.restype = _vp
     L.pmg_net_create.argtypes = [i, C.c_ulonglong]
@@ -257,6 +258,9 @@ class Solver:
 
     def set_bottom_fusion(self, on):
         _check(self._L.pmg_set_bottom_fusion(self.h, 1 if on else 0), "pmg_set_bottom_fusion")
+
+    def set_restrict_fusion(self, on):
+        _check(self._L.pmg_set_restrict_fusion(self.h, 1 if on else 0), "pmg_set_restrict_fusion")
 
     def set_net(self, net):
         """attach a trained network: solve_rel mode 2 applies its correction"""

@@ -656,6 +656,158 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
 }
 
 // ---------------------------------------------------------------------------
+// Finest-level residual fused with PAMR_inject of u and f_res (full
+// weighting) into the next coarser level (amrd/mg.c:103-113 for L+1 = Lf).
+// On the finest level rhs = 0 and res is zero off the updatable points, so
+// res is computed where the restriction needs it and never stored: a block
+// owns CX x CY coarse columns and marches over its coarse z chunk, keeping
+// 4 fine u planes and 3 fine res planes (tile plus the restriction ring) in
+// LDS.  Per fine point u and rho are read once (plus the halo ring); the
+// coarse u (and its ping-pong twin on the boundary) and res are written.
+// The residual norm of the fine level is accumulated over the fine points
+// each block owns.  Values are identical to residual + dmfwr3d.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ResRestrP {
+  const T *u, *rho, *a, *c;
+  T *cu, *cua, *cres;
+  int nx, ny, nz, cnx, cny, cnz;
+  int czchunk;
+  Spacing<T> sp;
+  NormWs ws;
+  int slot;
+  int nb;
+};
+
+template <typename T, int OP, bool P2, int CX, int CY, int NT>
+__global__ __launch_bounds__(NT) void k_resid_restrict(ResRestrP<T> P) {
+  constexpr int UX = 2 * CX + 3, UY = 2 * CY + 3;  // u tile: fine x in [2cx0-2, 2cx0+2CX]
+  constexpr int RX = 2 * CX + 1, RY = 2 * CY + 1;  // res tile: fine x in [2cx0-1, 2cx0+2CX-1]
+  constexpr int NLU = (UX * UY + NT - 1) / NT, NLR = (RX * RY + NT - 1) / NT;
+  __shared__ T sU[4][UY][UX];
+  __shared__ T sR[3][RY][RX];
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ntx = (P.cnx + CX - 1) / CX, nty = (P.cny + CY - 1) / CY;
+  const int ty = tile % nty, tx = (tile / nty) % ntx, tz = tile / (nty * ntx);
+  const int cx0 = tx * CX, cy0 = ty * CY;
+  const int ck0 = tz * P.czchunk, ck1 = min(ck0 + P.czchunk, P.cnz);
+  const int ux0 = 2 * cx0 - 2, uy0 = 2 * cy0 - 2;  // global fine index of u tile origin
+  const int rx0 = 2 * cx0 - 1, ry0 = 2 * cy0 - 1;  // global fine index of res tile origin
+  double s = 0.0, cnt = 0.0;
+  // u planes: loaded into registers one step ahead, then committed to LDS
+  T ldU[NLU];
+  auto issue_u = [&](int k) {
+#pragma unroll
+    for (int m = 0; m < NLU; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int lx = q % UX, ly = q / UX;
+      const int i = ux0 + lx, j = uy0 + ly;
+      ldU[m] = (q < UX * UY && k >= 0 && k < nz && i >= 0 && i < nx && j >= 0 && j < ny)
+                   ? P.u[ix3(i, j, k, nx, ny)] : T(0);
+    }
+  };
+  auto commit_u = [&](int k) {
+    T(*S)[UX] = sU[(k + 4) & 3];
+#pragma unroll
+    for (int m = 0; m < NLU; m++) {
+      const int q = threadIdx.x + m * NT;
+      if (q < UX * UY) S[q / UX][q % UX] = ldU[m];
+    }
+  };
+  // rho of the res points of a plane, one step ahead
+  T ldR[NLR];
+  auto issue_rho = [&](int k) {
+    const bool kin = k >= 1 && k <= nz - 2;
+#pragma unroll
+    for (int m = 0; m < NLR; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int i = rx0 + q % RX, j = ry0 + q / RX;
+      ldR[m] = (q < RX * RY && kin && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2)
+                   ? P.rho[ix3(i, j, k, nx, ny)] : T(0);
+    }
+  };
+  auto res_plane = [&](int k, const T (&rho)[NLR]) {
+    T(*R)[RX] = sR[(k + 3) % 3];
+    T(*U0)[UX] = sU[(k + 4) & 3];
+    T(*Um)[UX] = sU[(k + 3) & 3];
+    T(*Up)[UX] = sU[(k + 5) & 3];
+    const bool kin = k >= 1 && k <= nz - 2;
+    const bool kown = k >= 2 * ck0 && k < 2 * ck1;
+#pragma unroll
+    for (int m = 0; m < NLR; m++) {
+      const int q = threadIdx.x + m * NT;
+      if (q >= RX * RY) continue;
+      const int lx = q % RX, ly = q / RX;
+      const int i = rx0 + lx, j = ry0 + ly;
+      T r = T(0);
+      if (kin && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) {
+        const int ax = lx + 1, ay = ly + 1;  // position in the u tile
+        if constexpr (OP == 0) {
+          r = lop_poisson<T, P2>(U0[ay][ax], U0[ay][ax + 1], U0[ay][ax - 1], U0[ay + 1][ax], U0[ay - 1][ax],
+                                 Up[ay][ax], Um[ay][ax], rho[m], P.sp) - T(0);
+        } else {
+          const size_t p = ix3(i, j, k, nx, ny);
+          const size_t sy = nx, sz = (size_t)nx * ny;
+          const T *__restrict__ a = P.a;
+          r = lop_varcoef<T, P2>(U0[ay][ax], U0[ay][ax + 1], U0[ay][ax - 1], U0[ay + 1][ax], U0[ay - 1][ax],
+                                 Up[ay][ax], Um[ay][ax], a[p], a[p + 1], a[p - 1], a[p + sy], a[p - sy],
+                                 a[p + sz], a[p - sz], P.c[p], rho[m], P.sp, nullptr) - T(0);
+        }
+        if (kown && i >= 2 * cx0 && i < 2 * cx0 + 2 * CX && j >= 2 * cy0 && j < 2 * cy0 + 2 * CY) {
+          s += (double)r * (double)r;
+          cnt += 1.0;
+        }
+      }
+      R[ly][lx] = r;
+    }
+  };
+  auto coarse_plane = [&](int kc) {
+    const int t = threadIdx.x;
+    if (t >= CX * CY) return;
+    const int ci = cx0 + t % CX, cj = cy0 + t / CX;
+    if (ci >= P.cnx || cj >= P.cny) return;
+    const int i0 = 2 * ci + 1, j0 = 2 * cj + 1, k0 = 2 * kc + 1;  // 1-based fine
+    auto Racc = [&](int a, int b, int c) -> T { return sR[(c - 1 + 3) % 3][b - 1 - ry0][a - 1 - rx0]; };
+    auto Uacc = [&](int a, int b, int c) -> T { return sU[(c - 1 + 4) & 3][b - 1 - uy0][a - 1 - ux0]; };
+    const T vr = fw_point_acc<T>(Racc, nx, ny, nz, i0, j0, k0);
+    const T vu = fw_point_acc<T>(Uacc, nx, ny, nz, i0, j0, k0);
+    const size_t q = ix3(ci, cj, kc, P.cnx, P.cny);
+    P.cres[q] = vr;
+    P.cu[q] = vu;
+    if (ci == 0 || cj == 0 || kc == 0 || ci == P.cnx - 1 || cj == P.cny - 1 || kc == P.cnz - 1) P.cua[q] = vu;
+  };
+  // fine planes needed: res at 2ck0-1 .. min(2ck1-1, nz-1); u one more each side
+  const int kf0 = 2 * ck0 - 1, kf1 = min(2 * ck1 - 1, nz - 1);
+  issue_u(kf0 - 1);
+  commit_u(kf0 - 1);
+  issue_u(kf0);
+  commit_u(kf0);
+  issue_u(kf0 + 1);
+  issue_rho(kf0);
+  for (int k = kf0; k <= kf1; k++) {
+    commit_u(k + 1);  // plane k+1 (its slot held plane k-3, no longer read)
+    T rho[NLR];
+#pragma unroll
+    for (int m = 0; m < NLR; m++) rho[m] = ldR[m];
+    if (k + 2 <= kf1 + 1) issue_u(k + 2);
+    if (k + 1 <= kf1) issue_rho(k + 1);
+    __syncthreads();
+    res_plane(k, rho);
+    __syncthreads();
+    // coarse plane kc is complete once fine plane min(2kc+1, nz-1) is done
+    if (k & 1) {
+      const int kc = (k - 1) / 2;  // k = 2kc+1
+      if (kc >= ck0 && kc < ck1) coarse_plane(kc);
+    } else if (k == nz - 1 && k / 2 >= ck0 && k / 2 < ck1) {
+      coarse_plane(k / 2);
+    }
+    __syncthreads();
+  }
+  norm_partial<NT>(s, cnt, P.ws, P.slot, blockIdx.x, P.nb);
+}
+
+// ---------------------------------------------------------------------------
 // Bottom of the V-cycle in one workgroup.  Levels 1..nb (all small) are
 // processed from L_op + compute_MG_rhs on the top bottom level, through its
 // pre-sweeps, residual and injection, down to the coarsest solve and back up

@@ -68,43 +68,52 @@ __device__ __forceinline__ T lop_at(const T *V, const T *rho, const T *a, const 
 #define PMG_U1(a, b, c) u1[ix3((a)-1, (b)-1, (c)-1, nx1, ny1)]
 #define PMG_CH(a, b, c) chr1[ix3((a)-1, (b)-1, (c)-1, nx1, ny1)]
 
-// dmfwr3d, src/m_util_r8.f:775-823
-template <typename T>
-__device__ __forceinline__ T fw_point(const T *u1, int nx1, int ny1, int nz1, int i0, int j0,
-                                      int k0) {
+// dmfwr3d, src/m_util_r8.f:775-823.  U(a, b, c) returns the fine value at
+// the 1-based indices (a, b, c); the branch (full, plane, line or copy
+// stencil) follows the position in the fine grid, as the Fortran does.
+template <typename T, typename Acc>
+__device__ __forceinline__ T fw_point_acc(const Acc &U, int nx1, int ny1, int nz1, int i0, int j0,
+                                          int k0) {
   const int im = i0 - 1, jm = j0 - 1, km = k0 - 1, ip = i0 + 1, jp = j0 + 1, kp = k0 + 1;
   const bool xi = i0 > 1 && i0 <= nx1 - 1, yi = j0 > 1 && j0 <= ny1 - 1,
              zi = k0 > 1 && k0 <= nz1 - 1;
   if (xi && yi && zi) {
-    return (PMG_U1(im, jm, km) + 2 * PMG_U1(i0, jm, km) + PMG_U1(ip, jm, km) +
-            2 * PMG_U1(im, j0, km) + 4 * PMG_U1(i0, j0, km) + 2 * PMG_U1(ip, j0, km) +
-            PMG_U1(im, jp, km) + 2 * PMG_U1(i0, jp, km) + PMG_U1(ip, jp, km) +
-            2 * PMG_U1(im, jm, k0) + 4 * PMG_U1(i0, jm, k0) + 2 * PMG_U1(ip, jm, k0) +
-            4 * PMG_U1(im, j0, k0) + 8 * PMG_U1(i0, j0, k0) + 4 * PMG_U1(ip, j0, k0) +
-            2 * PMG_U1(im, jp, k0) + 4 * PMG_U1(i0, jp, k0) + 2 * PMG_U1(ip, jp, k0) +
-            PMG_U1(im, jm, kp) + 2 * PMG_U1(i0, jm, kp) + PMG_U1(ip, jm, kp) +
-            2 * PMG_U1(im, j0, kp) + 4 * PMG_U1(i0, j0, kp) + 2 * PMG_U1(ip, j0, kp) +
-            PMG_U1(im, jp, kp) + 2 * PMG_U1(i0, jp, kp) + PMG_U1(ip, jp, kp)) / T(64.0);
+    return (U(im, jm, km) + 2 * U(i0, jm, km) + U(ip, jm, km) +
+            2 * U(im, j0, km) + 4 * U(i0, j0, km) + 2 * U(ip, j0, km) +
+            U(im, jp, km) + 2 * U(i0, jp, km) + U(ip, jp, km) +
+            2 * U(im, jm, k0) + 4 * U(i0, jm, k0) + 2 * U(ip, jm, k0) +
+            4 * U(im, j0, k0) + 8 * U(i0, j0, k0) + 4 * U(ip, j0, k0) +
+            2 * U(im, jp, k0) + 4 * U(i0, jp, k0) + 2 * U(ip, jp, k0) +
+            U(im, jm, kp) + 2 * U(i0, jm, kp) + U(ip, jm, kp) +
+            2 * U(im, j0, kp) + 4 * U(i0, j0, kp) + 2 * U(ip, j0, kp) +
+            U(im, jp, kp) + 2 * U(i0, jp, kp) + U(ip, jp, kp)) / T(64.0);
   } else if (xi && yi) {
-    return (1 * PMG_U1(im, jm, k0) + 2 * PMG_U1(i0, jm, k0) + 1 * PMG_U1(ip, jm, k0) +
-            2 * PMG_U1(im, j0, k0) + 4 * PMG_U1(i0, j0, k0) + 2 * PMG_U1(ip, j0, k0) +
-            1 * PMG_U1(im, jp, k0) + 2 * PMG_U1(i0, jp, k0) + 1 * PMG_U1(ip, jp, k0)) / T(16.0);
+    return (1 * U(im, jm, k0) + 2 * U(i0, jm, k0) + 1 * U(ip, jm, k0) +
+            2 * U(im, j0, k0) + 4 * U(i0, j0, k0) + 2 * U(ip, j0, k0) +
+            1 * U(im, jp, k0) + 2 * U(i0, jp, k0) + 1 * U(ip, jp, k0)) / T(16.0);
   } else if (yi && zi) {
-    return (1 * PMG_U1(i0, jm, km) + 2 * PMG_U1(i0, jm, k0) + 1 * PMG_U1(i0, jm, kp) +
-            2 * PMG_U1(i0, j0, km) + 4 * PMG_U1(i0, j0, k0) + 2 * PMG_U1(i0, j0, kp) +
-            1 * PMG_U1(i0, jp, km) + 2 * PMG_U1(i0, jp, k0) + 1 * PMG_U1(i0, jp, kp)) / T(16.0);
+    return (1 * U(i0, jm, km) + 2 * U(i0, jm, k0) + 1 * U(i0, jm, kp) +
+            2 * U(i0, j0, km) + 4 * U(i0, j0, k0) + 2 * U(i0, j0, kp) +
+            1 * U(i0, jp, km) + 2 * U(i0, jp, k0) + 1 * U(i0, jp, kp)) / T(16.0);
   } else if (xi && zi) {
-    return (1 * PMG_U1(im, j0, km) + 2 * PMG_U1(im, j0, k0) + 1 * PMG_U1(im, j0, kp) +
-            2 * PMG_U1(i0, j0, km) + 4 * PMG_U1(i0, j0, k0) + 2 * PMG_U1(i0, j0, kp) +
-            1 * PMG_U1(ip, j0, km) + 2 * PMG_U1(ip, j0, k0) + 1 * PMG_U1(ip, j0, kp)) / T(16.0);
+    return (1 * U(im, j0, km) + 2 * U(im, j0, k0) + 1 * U(im, j0, kp) +
+            2 * U(i0, j0, km) + 4 * U(i0, j0, k0) + 2 * U(i0, j0, kp) +
+            1 * U(ip, j0, km) + 2 * U(ip, j0, k0) + 1 * U(ip, j0, kp)) / T(16.0);
   } else if (xi) {
-    return (PMG_U1(im, j0, k0) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(ip, j0, k0)) / T(4.0);
+    return (U(im, j0, k0) + 2 * U(i0, j0, k0) + U(ip, j0, k0)) / T(4.0);
   } else if (yi) {
-    return (PMG_U1(i0, jm, k0) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(i0, jp, k0)) / T(4.0);
+    return (U(i0, jm, k0) + 2 * U(i0, j0, k0) + U(i0, jp, k0)) / T(4.0);
   } else if (zi) {
-    return (PMG_U1(i0, j0, km) + 2 * PMG_U1(i0, j0, k0) + PMG_U1(i0, j0, kp)) / T(4.0);
+    return (U(i0, j0, km) + 2 * U(i0, j0, k0) + U(i0, j0, kp)) / T(4.0);
   }
-  return PMG_U1(i0, j0, k0);
+  return U(i0, j0, k0);
+}
+
+template <typename T>
+__device__ __forceinline__ T fw_point(const T *u1, int nx1, int ny1, int nz1, int i0, int j0,
+                                      int k0) {
+  auto U = [&](int a, int b, int c) -> T { return PMG_U1(a, b, c); };
+  return fw_point_acc<T>(U, nx1, ny1, nz1, i0, j0, k0);
 }
 
 __device__ __forceinline__ int w3d(int a, int b, int c) {

@@ -130,6 +130,7 @@ struct pmg_solver {
   int lv_variant[kMaxLev], lv_zchunk[kMaxLev];  // per-level sweep configuration
   int use_graph = 1;
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
+  int use_fused_restrict = 1;  // finest residual + injection in one pass
   hipGraphExec_t g64 = nullptr;   // captured FP64 V-cycle
   hipGraphExec_t g32 = nullptr;   // captured mixed-precision outer iteration
   bool fp32_ready = false;
@@ -561,6 +562,52 @@ int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
   return 0;
 }
 
+// finest-level residual + full-weight injection of u and f_res in one pass
+// (no excision, full weighting for both, fine = 2 coarse - 1)
+// (FP64 only: in FP32 the separate residual and restriction are as fast)
+template <typename T>
+bool can_fuse_resid_restrict(const pmg_solver *S) {
+  if (!std::is_same<T, double>::value) return false;
+  if (!S->use_fused_restrict || S->do_ex || S->rop_u != 3 || S->rop_res != 3 || S->nlev < 2) return false;
+  const Level &F = S->lev[S->nlev - 1], &C = S->lev[S->nlev - 2];
+  for (int d = 0; d < 3; d++)
+    if (F.n[d] != 2 * C.n[d] - 1) return false;
+  // the residual box must be the whole interior (ZERO semantics off it)
+  return F.rbox.ok && F.rbox.lo[0] == 1 && F.rbox.lo[1] == 1 && F.rbox.lo[2] == 1 && F.rbox.hi[0] == F.n[0] - 2 &&
+         F.rbox.hi[1] == F.n[1] - 2 && F.rbox.hi[2] == F.n[2] - 2;
+}
+
+template <typename T, int OP, bool P2>
+int launch_resid_restrict_t(pmg_solver *S, ResRestrP<T> P) {
+  constexpr int CX = std::is_same<T, double>::value ? 16 : 32, CY = 8, NT = 256;
+  const int ncz = (P.cnz + P.czchunk - 1) / P.czchunk;
+  const dim3 grid(((P.cnx + CX - 1) / CX) * ((P.cny + CY - 1) / CY) * ncz);
+  P.nb = (int)grid.x;
+  if (P.nb > S->ws.cap) return -3;
+  hipLaunchKernelGGL((k_resid_restrict<T, OP, P2, CX, CY, NT>), grid, dim3(NT), 0, S->stream, P);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T>
+int resid_restrict(pmg_solver *S, int slot) {
+  Level &F = S->lev[S->nlev - 1], &C = S->lev[S->nlev - 2];
+  LevelArrays<T> &AF = arrays<T>(F);
+  LevelArrays<T> &AC = arrays<T>(C);
+  ResRestrP<T> P;
+  P.u = AF.u; P.rho = AF.rho; P.a = AF.a; P.c = AF.c;
+  P.cu = AC.u; P.cua = AC.ua; P.cres = AC.res;
+  P.nx = F.n[0]; P.ny = F.n[1]; P.nz = F.n[2];
+  P.cnx = C.n[0]; P.cny = C.n[1]; P.cnz = C.n[2];
+  P.czchunk = std::max(1, std::min(8, C.n[2]));
+  P.sp = spacing<T>(F);
+  P.ws = S->ws;
+  P.slot = slot;
+  const bool p2 = P.sp.pow2 != 0;
+  if (S->op == 0) return p2 ? launch_resid_restrict_t<T, 0, true>(S, P) : launch_resid_restrict_t<T, 0, false>(S, P);
+  return p2 ? launch_resid_restrict_t<T, 1, true>(S, P) : launch_resid_restrict_t<T, 1, false>(S, P);
+}
+
 template <typename T>
 int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cnorm, int *citer) {
   const int Lf = S->nlev;
@@ -570,11 +617,15 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
     if (L < Lf) {
       LevelArrays<T> &AF = arrays<T>(S->lev[L]);
       LevelArrays<T> &AC = arrays<T>(S->lev[L - 1]);
-      // residual on L+1; the finest level was cleared at the cycle start
-      if ((rc = residual<T>(S, L + 1, norm_base + NS_RESID + L, L + 1 == Lf, L + 1 == Lf))) return rc;
-      // PAMR_inject(L+1): u and u_res (sources are constant, restricted once)
-      if ((rc = restrict_any<T>(S, L + 1, AF.u, AC.u, AC.ua, S->rop_u))) return rc;
-      if ((rc = restrict_any<T>(S, L + 1, AF.res, AC.res, (T *)nullptr, S->rop_res))) return rc;
+      if (L + 1 == Lf && can_fuse_resid_restrict<T>(S)) {
+        if ((rc = resid_restrict<T>(S, norm_base + NS_RESID + L))) return rc;
+      } else {
+        // residual on L+1; the finest level was cleared at the cycle start
+        if ((rc = residual<T>(S, L + 1, norm_base + NS_RESID + L, L + 1 == Lf, L + 1 == Lf))) return rc;
+        // PAMR_inject(L+1): u and u_res (sources are constant, restricted once)
+        if ((rc = restrict_any<T>(S, L + 1, AF.u, AC.u, AC.ua, S->rop_u))) return rc;
+        if ((rc = restrict_any<T>(S, L + 1, AF.res, AC.res, (T *)nullptr, S->rop_res))) return rc;
+      }
       if (L == Lb) {
         if ((rc = bottom<T>(S, Lb, norm_base, cnorm, citer))) return rc;
         break;
@@ -836,6 +887,13 @@ int pmg_set_graph(pmg_solver *S, int on) {
 int pmg_set_bottom_fusion(pmg_solver *S, int on) {
   if (!S) return -1;
   S->use_bottom = on ? 1 : 0;
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_set_restrict_fusion(pmg_solver *S, int on) {
+  if (!S) return -1;
+  S->use_fused_restrict = on ? 1 : 0;
   invalidate_graphs(S);
   return 0;
 }

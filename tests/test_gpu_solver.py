@@ -112,6 +112,35 @@ def test_graph_replays_match_eager(pm):
     Se.close()
 
 
+@pytest.mark.parametrize("n", [33, 65, 129])
+def test_restrict_fusion_matches_unfused(pm, n):
+    """the finest residual computed inside the injection gives the same
+    coarse u and res, the same cycles and the same u as residual + dmfwr3d"""
+    rho = -pm.gaussian_source(n, seed=6)
+    Sf, _ = _pair(pm, n, rho)
+    Su, _ = _pair(pm, n, rho)
+    Su.set_restrict_fusion(False)
+    for _ in range(3):
+        rf, nf, cf = Sf.vcycle()
+        ru, nu, cu = Su.vcycle()
+        assert cf == cu
+        assert abs(rf - ru) <= 1e-13 * ru
+        assert np.allclose(nf, nu, rtol=1e-12, atol=0)
+    for L in range(1, Sf.nlev + 1):
+        for w in (pm.PMG_F_U, pm.PMG_F_RES):
+            if L == Sf.nlev and w == pm.PMG_F_RES:
+                continue
+            assert np.array_equal(Sf.download(L, w), Su.download(L, w)), (L, w)
+    Sf.zero_solution()
+    Su.zero_solution()
+    r0f, hf = Sf.solve_rel(1, 1e-10, 40)
+    r0u, hu = Su.solve_rel(1, 1e-10, 40)
+    assert hf[-1] <= 1e-10 * r0f and hu[-1] <= 1e-10 * r0u
+    assert abs(len(hf) - len(hu)) <= 1
+    Sf.close()
+    Su.close()
+
+
 @pytest.mark.parametrize("n", [65, 129])
 def test_bottom_fusion_matches_unfused(pm, n):
     """the single-workgroup bottom V (levels <= 17^3) gives the same u, the
