@@ -159,6 +159,11 @@ int pmg_set_bottom_fusion(pmg_solver *S, int on);
  * and f_res instead of storing it (default on; no excision, full weighting
  * for both); results are identical either way */
 int pmg_set_restrict_fusion(pmg_solver *S, int on);
+/* run pairs of red-black sweeps on levels of at least 65 points per side as
+ * one temporally blocked pass (default off: at 257^3 a pair takes 0.26 ms
+ * against 2 x 0.11 ms for single sweeps; zchunk > 0 sets its z depth);
+ * results are identical either way */
+int pmg_set_sweep_blocking(pmg_solver *S, int on, int zchunk);
 void *pmg_stream(pmg_solver *S);
 
 /* ---- 4. learned smoother / warm-start network (no reference counterpart:

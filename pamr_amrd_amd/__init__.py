@@ -81,6 +81,7 @@ def _declare(L):
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_set_bottom_fusion.argtypes = [_vp, i]
     L.pmg_set_restrict_fusion.argtypes = [_vp, i]
+    L.pmg_set_sweep_blocking.argtypes = [_vp, i, i]
     L.pmg_stream.argtypes = [_vp]
     L.pmg_stream.restype = _vp
     L.pmg_net_create.argtypes = [i, C.c_ulonglong]
@@ -261,6 +262,9 @@ class Solver:
 
     def set_restrict_fusion(self, on):
         _check(self._L.pmg_set_restrict_fusion(self.h, 1 if on else 0), "pmg_set_restrict_fusion")
+
+    def set_sweep_blocking(self, on, zchunk=0):
+        _check(self._L.pmg_set_sweep_blocking(self.h, 1 if on else 0, zchunk), "pmg_set_sweep_blocking")
 
     def set_net(self, net):
         """attach a trained network: solve_rel mode 2 applies its correction"""

@@ -11,6 +11,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <type_traits>
 
 #define PMG_HOST_CHECK(expr)                                                   \
   do {                                                                         \

@@ -304,6 +304,189 @@ __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
 }
 
 // ---------------------------------------------------------------------------
+// Two red-black sweeps in one pass over HBM (temporal blocking).  The four
+// colour passes -- sweep A pass 0 / pass 1, sweep B pass 0 / pass 1 -- run
+// as a pipeline in one z-march: at step k, phase p works on plane
+// k + 3 - p over the tile plus a (3 - p)-point ring, so every value a phase
+// reads was produced by the previous phase in this block.  The LDS holds a
+// 6-plane ring of the tile plus a 4-point margin.  Each u value crosses HBM
+// once per two sweeps (plus the margins); results equal two sequential
+// sweeps bit for bit, and the norm is that of sweep B (the last one).
+// ---------------------------------------------------------------------------
+constexpr int kS2Margin = 4, kS2Planes = 6;
+
+// colour slot q of phase region (tile + R ring) on plane k; LDS margin M
+template <int TX, int TY, int R, int M>
+__device__ __forceinline__ bool sweep2_pos(int q, int x0, int y0, int k, int pass, int &lx, int &ly) {
+  constexpr int W = TX + 2 * R, H = TY + 2 * R, OFF = M - R;
+  constexpr int W2 = (W + 1) / 2;
+  if (q >= W2 * H) return false;
+  ly = OFF + q / W2;
+  const int j = y0 - M + ly;
+  const int want = (j + k + 3 + pass) & 1;  // parity of i: (i+j+k+3+pass) even
+  lx = OFF + ((want - (x0 - M + OFF)) & 1) + 2 * (q % W2);
+  return lx <= OFF + W - 1;
+}
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
+__device__ __forceinline__ void sweep2_tile(const SweepP<T> &P, int x0, int y0, int z0, int z1, int bid) {
+  constexpr int M = kS2Margin, NPL = kS2Planes;
+  constexpr int LX = TX + 2 * M, LY = TY + 2 * M;
+  constexpr int NLD = (LX * LY + NT - 1) / NT;
+  constexpr int NS = ((((TX + 7) / 2) * (TY + 6)) + NT - 1) / NT;  // slots of the widest phase
+  __shared__ T sU[NPL][LY][LX];
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const T *__restrict__ uin = P.uin;
+  T *__restrict__ uout = P.uout;
+  auto ring = [](int k) { return (k + 2 * NPL) % NPL; };
+  double s = 0.0, cnt = 0.0;
+
+  // per-thread slot tables of the four phases for both plane parities
+  int tlx[4][2][NS], tly[4][2][NS], toff[4][2][NS];
+  auto table = [&](auto PHc) {
+    constexpr int ph = decltype(PHc)::value, R = 3 - ph;
+#pragma unroll
+    for (int par = 0; par < 2; par++)
+#pragma unroll
+      for (int m = 0; m < NS; m++) {
+        int lx = 0, ly = 0;
+        toff[ph][par][m] = -1;
+        tlx[ph][par][m] = 0;
+        tly[ph][par][m] = 0;
+        if (!sweep2_pos<TX, TY, R, M>(threadIdx.x + m * NT, x0, y0, par, ph & 1, lx, ly)) continue;
+        const int i = x0 - M + lx, j = y0 - M + ly;
+        tlx[ph][par][m] = lx;
+        tly[ph][par][m] = ly;
+        if (i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1) toff[ph][par][m] = i + nx * j;
+      }
+  };
+  table(std::integral_constant<int, 0>());
+  table(std::integral_constant<int, 1>());
+  table(std::integral_constant<int, 2>());
+  table(std::integral_constant<int, 3>());
+
+  auto load_direct = [&](int k) {
+    if (k < 0 || k >= nz) return;
+    T(*S)[LX] = sU[ring(k)];
+    for (int q = threadIdx.x; q < LX * LY; q += NT) {
+      const int lx = q % LX, ly = q / LX;
+      const int i = x0 - M + lx, j = y0 - M + ly;
+      if (i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = uin[ix3(i, j, k, nx, ny)];
+    }
+  };
+  T ldA[NLD], ldB[NLD];
+  auto load_issue = [&](T(&ld)[NLD], int k) {
+#pragma unroll
+    for (int m = 0; m < NLD; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int lx = q % LX, ly = q / LX;
+      const int i = x0 - M + lx, j = y0 - M + ly;
+      if (k >= 0 && k < nz && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) ld[m] = uin[ix3(i, j, k, nx, ny)];
+    }
+  };
+  auto load_commit = [&](const T(&ld)[NLD], int k) {
+    if (k < 0 || k >= nz) return;
+    T(*S)[LX] = sU[ring(k)];
+#pragma unroll
+    for (int m = 0; m < NLD; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int lx = q % LX, ly = q / LX;
+      const int i = x0 - M + lx, j = y0 - M + ly;
+      if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = ld[m];
+    }
+  };
+  // one colour phase on plane k (in place in the LDS ring); the phase index
+  // is a compile-time constant so the slot tables stay in registers
+  auto phase = [&](auto PHc, int k, bool count) {
+    constexpr int ph = decltype(PHc)::value;
+    if (k < P.bz0 || k > P.bz1) return;
+    const bool odd = (k & 1) != 0;
+    const size_t base = (size_t)k * nx * ny;
+    T(*S0)[LX] = sU[ring(k)];
+    T(*Sm)[LX] = sU[ring(k - 1)];
+    T(*Sp)[LX] = sU[ring(k + 1)];
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+      const int off = odd ? toff[ph][1][m] : toff[ph][0][m];
+      if (off < 0) continue;
+      const int lx = odd ? tlx[ph][1][m] : tlx[ph][0][m];
+      const int ly = odd ? tly[ph][1][m] : tly[ph][0][m];
+      const size_t p = base + (size_t)off;
+      if constexpr (EX) { if (P.chr[p]) continue; }
+      const T uc = S0[ly][lx];
+      T jac, L;
+      if constexpr (OP == 0) {
+        L = lop_poisson<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx], Sp[ly][lx],
+                               Sm[ly][lx], P.rho[p], P.sp);
+        jac = P.sp.jac;
+      } else {
+        const T *__restrict__ a = P.a;
+        const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+        L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx], Sp[ly][lx],
+                               Sm[ly][lx], a[p], a[p + 1], a[p - 1], a[p + sy], a[p - sy], a[p + sz], a[p - sz],
+                               P.c[p], P.rho[p], P.sp, &jac);
+      }
+      T res;
+      if constexpr (RHS0) res = L - T(0); else res = L - P.rhs[p];
+      if constexpr (std::is_same<T, float>::value && OP == 0) S0[ly][lx] = uc - res * P.sp.ijac;
+      else S0[ly][lx] = uc - res / jac;
+      if (count && lx >= M && lx < M + TX && ly >= M && ly < M + TY) {
+        s += (double)res * (double)res;
+        cnt += 1.0;
+      }
+    }
+  };
+  auto store = [&](int k) {
+    T(*S)[LX] = sU[ring(k)];
+    for (int q = threadIdx.x; q < TX * TY; q += NT) {
+      const int tx = q % TX, ty = q / TX;
+      const int i = x0 + tx, j = y0 + ty;
+      if (i <= nx - 2 && j <= ny - 2) uout[ix3(i, j, k, nx, ny)] = S[ty + M][tx + M];
+    }
+  };
+
+  // steps k = z0-6 .. z1-1: phase p on plane k+3-p when that plane is in
+  // [z0-(3-p), z1-1+(3-p)]; planes z0-4 .. z1+3 pass through the ring
+  const int kb = z0 - 6;
+  load_direct(kb + 2);
+  load_direct(kb + 3);
+  load_direct(kb + 4);
+  load_issue(ldB, kb + 5);
+  __syncthreads();
+  auto run = [&](auto PHc, int k) {
+    constexpr int ph = decltype(PHc)::value, d = 3 - ph;
+    const int kp = k + d;
+    if (kp >= z0 - d && kp <= z1 - 1 + d) phase(PHc, kp, ph >= 2 && kp >= z0 && kp < z1);
+    __syncthreads();
+  };
+  auto step = [&](int k, T(&ldi)[NLD], const T(&ldc)[NLD]) {
+    load_issue(ldi, k + 6);
+    run(std::integral_constant<int, 0>(), k);
+    run(std::integral_constant<int, 1>(), k);
+    run(std::integral_constant<int, 2>(), k);
+    run(std::integral_constant<int, 3>(), k);
+    if (k >= z0) store(k);
+    load_commit(ldc, k + 5);
+    __syncthreads();
+  };
+  for (int k = kb; k < z1; k += 2) {
+    step(k, ldA, ldB);
+    if (k + 1 < z1) step(k + 1, ldB, ldA);
+  }
+  norm_partial<NT>(s, cnt, P.ws, P.slot, bid, P.nb);
+}
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void k_sweep2(SweepP<T> P) {
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int nty = (P.ny - 2 + TY - 1) / TY, ntx = (P.nx - 2 + TX - 1) / TX;
+  const int ty = tile % nty, tx = (tile / nty) % ntx, tz = tile / (nty * ntx);
+  const int z0 = 1 + tz * P.zchunk;
+  sweep2_tile<T, OP, EX, P2, RHS0, TX, TY, NT>(P, 1 + tx * TX, 1 + ty * TY, z0, min(z0 + P.zchunk, P.nz - 1),
+                                               blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
 // Residual res = L[u] - rhs on updatable points (num_w.f:59-90).  Other
 // interior points: ZERO mode writes 0 (clear_MG_vars then residual,
 // amrd/mg.c:91,104 on the finest level); KEEP mode leaves them (coarser

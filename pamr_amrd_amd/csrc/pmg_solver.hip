@@ -110,6 +110,8 @@ struct Level {
   Box cbox;  // child mask box (levels below the finest)
   LevelArrays<double> d;
   LevelArrays<float> f;
+  int cur[2] = {0, 0};      // ping-pong buffer holding u (0: u, 1: ua), FP64 / FP32,
+                            // tracked while a cycle is enqueued
   uint8_t *chr = nullptr;   // excision mask, 1 = excised (sweeps, hooks)
   double *chrd = nullptr;   // the chr grid function itself (m_util_r8 _ex transfers)
 };
@@ -131,6 +133,9 @@ struct pmg_solver {
   int use_graph = 1;
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   int use_fused_restrict = 1;  // finest residual + injection in one pass
+  int use_sweep2 = 0;          // pairs of sweeps temporally blocked on large levels
+                               // (measured slower than single sweeps; off by default)
+  int sweep2_zchunk = 32;
   hipGraphExec_t g64 = nullptr;   // captured FP64 V-cycle
   hipGraphExec_t g32 = nullptr;   // captured mixed-precision outer iteration
   bool fp32_ready = false;
@@ -218,6 +223,14 @@ const Spacing<double> &spacing<double>(const Level &Lv) { return Lv.sp64; }
 template <>
 const Spacing<float> &spacing<float>(const Level &Lv) { return Lv.sp32; }
 
+template <typename T>
+int &curbuf(Level &Lv) { return Lv.cur[std::is_same<T, float>::value ? 1 : 0]; }
+// the buffer holding u and its ping-pong twin
+template <typename T>
+T *ucur(Level &Lv) { LevelArrays<T> &A = arrays<T>(Lv); return curbuf<T>(Lv) ? A.ua : A.u; }
+template <typename T>
+T *uoth(Level &Lv) { LevelArrays<T> &A = arrays<T>(Lv); return curbuf<T>(Lv) ? A.u : A.ua; }
+
 // sweep tile variants (TX, TY, NT)
 #define PMG_SWEEP_VARIANTS(X) X(0, 64, 8, 256) X(1, 64, 16, 512) X(2, 32, 16, 512) X(3, 128, 8, 512)
 constexpr int kNumSweepVariants = 4;
@@ -254,6 +267,34 @@ int launch_sweep(pmg_solver *S, int v, SweepP<T> P, bool p2, bool rhs0) {
   return rhs0 ? launch_sweep_v<T, OP, EX, false, true>(S, v, P) : launch_sweep_v<T, OP, EX, false, false>(S, v, P);
 }
 
+// two sweeps in one pass (temporal blocking), 64 x 16 tiles, 1024 threads,
+// on levels with at least kSweep2MinN points per side
+constexpr int kSweep2MinN = 65;
+constexpr int kS2TX = 64, kS2TY = 16, kS2NT = 1024;
+
+template <typename T, int OP, bool EX, bool P2, bool RHS0>
+int launch_sweep2_v(pmg_solver *S, SweepP<T> P) {
+  P.zchunk = std::max(1, std::min(P.zchunk, P.nz - 2));
+  dim3 grid(((P.nx - 2 + kS2TX - 1) / kS2TX) * ((P.ny - 2 + kS2TY - 1) / kS2TY) *
+            ((P.nz - 2 + P.zchunk - 1) / P.zchunk));
+  P.nb = (int)grid.x;
+  if (P.nb > S->ws.cap) return -3;
+  hipLaunchKernelGGL((k_sweep2<T, OP, EX, P2, RHS0, kS2TX, kS2TY, kS2NT>), grid, dim3(kS2NT), 0, S->stream, P);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename T, int OP, bool EX>
+int launch_sweep2(pmg_solver *S, SweepP<T> P, bool p2, bool rhs0) {
+  if (p2) return rhs0 ? launch_sweep2_v<T, OP, EX, true, true>(S, P) : launch_sweep2_v<T, OP, EX, true, false>(S, P);
+  return rhs0 ? launch_sweep2_v<T, OP, EX, false, true>(S, P) : launch_sweep2_v<T, OP, EX, false, false>(S, P);
+}
+
+// levels where pairs of sweeps run temporally blocked
+bool use_sweep2(const pmg_solver *S, const Level &Lv) {
+  return S->use_sweep2 && Lv.n[0] >= kSweep2MinN && Lv.n[1] >= kSweep2MinN && Lv.n[2] >= kSweep2MinN;
+}
+
 // n red-black sweeps on level L (ping-pong u <-> ua; odd counts copy back)
 template <typename T>
 int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
@@ -273,18 +314,42 @@ int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
   P.ws = S->ws;
   P.slot = slot;
   const bool p2 = P.sp.pow2 != 0;
-  for (int s = 0; s < n; s++) {
-    P.uin = (s % 2 == 0) ? A.u : A.ua;
-    P.uout = (s % 2 == 0) ? A.ua : A.u;
+  const bool two = use_sweep2(S, Lv);
+  int done = 0, launches = 0;
+  T *const b0 = ucur<T>(Lv), *const b1 = uoth<T>(Lv);
+  while (done < n) {
+    P.uin = (launches % 2 == 0) ? b0 : b1;
+    P.uout = (launches % 2 == 0) ? b1 : b0;
     int rc;
-    if (S->op == 0) rc = S->do_ex ? launch_sweep<T, 0, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 0, false>(S, variant, P, p2, rhs0);
-    else rc = S->do_ex ? launch_sweep<T, 1, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 1, false>(S, variant, P, p2, rhs0);
+    if (two && n - done >= 2) {
+      SweepP<T> Q = P;
+      Q.zchunk = S->sweep2_zchunk;
+      if (S->op == 0) rc = S->do_ex ? launch_sweep2<T, 0, true>(S, Q, p2, rhs0) : launch_sweep2<T, 0, false>(S, Q, p2, rhs0);
+      else rc = S->do_ex ? launch_sweep2<T, 1, true>(S, Q, p2, rhs0) : launch_sweep2<T, 1, false>(S, Q, p2, rhs0);
+      done += 2;
+    } else {
+      if (S->op == 0) rc = S->do_ex ? launch_sweep<T, 0, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 0, false>(S, variant, P, p2, rhs0);
+      else rc = S->do_ex ? launch_sweep<T, 1, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 1, false>(S, variant, P, p2, rhs0);
+      done += 1;
+    }
     if (rc) return rc;
+    launches++;
   }
-  if (n % 2 == 1) {
-    hipLaunchKernelGGL(k_copy<T>, dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, A.u, A.ua, Lv.size);
-    PMG_HOST_CHECK(hipGetLastError());
-  }
+  // an odd number of launches leaves u in the twin buffer: later kernels of
+  // this cycle read it there, settle() moves it back at the end of the level
+  curbuf<T>(Lv) ^= (launches & 1);
+  return 0;
+}
+
+// u back in its primary buffer (both buffers then agree)
+template <typename T>
+int settle(pmg_solver *S, int L) {
+  Level &Lv = S->lev[L - 1];
+  LevelArrays<T> &A = arrays<T>(Lv);
+  if (curbuf<T>(Lv) == 0) return 0;
+  hipLaunchKernelGGL(k_copy<T>, dim3(nblk256(Lv.size)), dim3(256), 0, S->stream, A.u, A.ua, Lv.size);
+  PMG_HOST_CHECK(hipGetLastError());
+  curbuf<T>(Lv) = 0;
   return 0;
 }
 
@@ -312,7 +377,7 @@ int residual(pmg_solver *S, int L, int slot, bool rhs0, bool zero, float *neg32 
   Level &Lv = S->lev[L - 1];
   LevelArrays<T> &A = arrays<T>(Lv);
   ResidP<T> P;
-  P.u = A.u; P.rho = A.rho; P.rhs = A.rhs; P.a = A.a; P.c = A.c; P.chr = Lv.chr;
+  P.u = ucur<T>(Lv); P.rho = A.rho; P.rhs = A.rhs; P.a = A.a; P.c = A.c; P.chr = Lv.chr;
   P.res = A.res;
   P.neg32 = neg32;
   P.nx = Lv.n[0]; P.ny = Lv.n[1]; P.nz = Lv.n[2];
@@ -432,7 +497,7 @@ int cgc(pmg_solver *S, int L) {
   if (!ex) {
     hipLaunchKernelGGL(k_cgc_diff<T>, dim3(nblk256(Cl.size)), dim3(256), 0, S->stream, AC.res, AC.u, AC.lop,
                        (T)S->w0_i, Cl.size);
-    hipLaunchKernelGGL(k_cgc<T>, grid3(Cl.n[0], Cl.n[1], Cl.n[2]), dim3(256), 0, S->stream, AF.u, AF.ua, AC.res,
+    hipLaunchKernelGGL(k_cgc<T>, grid3(Cl.n[0], Cl.n[1], Cl.n[2]), dim3(256), 0, S->stream, ucur<T>(F), uoth<T>(F), AC.res,
                        F.n[0], F.n[1], F.n[2], Cl.n[0], Cl.n[1], Cl.n[2], (const uint8_t *)F.chr);
     PMG_HOST_CHECK(hipGetLastError());
     return 0;
@@ -448,8 +513,8 @@ int cgc(pmg_solver *S, int L) {
     if (rc) return rc;
     hipLaunchKernelGGL(k_clear_ex<double>, dim3(nblk256(Cl.size)), dim3(256), 0, S->stream, AC.res,
                        Cl.chr, Cl.size);
-    hipLaunchKernelGGL(k_cgc_add_ex<double>, grid3(F.n[0], F.n[1], F.n[2]), dim3(256), 0, S->stream, AF.u,
-                       AF.ua, AF.res, F.chr, F.n[0], F.n[1], F.n[2]);
+    hipLaunchKernelGGL(k_cgc_add_ex<double>, grid3(F.n[0], F.n[1], F.n[2]), dim3(256), 0, S->stream, ucur<double>(F),
+                       uoth<double>(F), AF.res, F.chr, F.n[0], F.n[1], F.n[2]);
     PMG_HOST_CHECK(hipGetLastError());
   }
   return 0;
@@ -595,7 +660,7 @@ int resid_restrict(pmg_solver *S, int slot) {
   LevelArrays<T> &AF = arrays<T>(F);
   LevelArrays<T> &AC = arrays<T>(C);
   ResRestrP<T> P;
-  P.u = AF.u; P.rho = AF.rho; P.a = AF.a; P.c = AF.c;
+  P.u = ucur<T>(F); P.rho = AF.rho; P.a = AF.a; P.c = AF.c;
   P.cu = AC.u; P.cua = AC.ua; P.cres = AC.res;
   P.nx = F.n[0]; P.ny = F.n[1]; P.nz = F.n[2];
   P.cnx = C.n[0]; P.cny = C.n[1]; P.cnz = C.n[2];
@@ -623,7 +688,7 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
         // residual on L+1; the finest level was cleared at the cycle start
         if ((rc = residual<T>(S, L + 1, norm_base + NS_RESID + L, L + 1 == Lf, L + 1 == Lf))) return rc;
         // PAMR_inject(L+1): u and u_res (sources are constant, restricted once)
-        if ((rc = restrict_any<T>(S, L + 1, AF.u, AC.u, AC.ua, S->rop_u))) return rc;
+        if ((rc = restrict_any<T>(S, L + 1, ucur<T>(S->lev[L]), AC.u, AC.ua, S->rop_u))) return rc;
         if ((rc = restrict_any<T>(S, L + 1, AF.res, AC.res, (T *)nullptr, S->rop_res))) return rc;
       }
       if (L == Lb) {
@@ -641,6 +706,7 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
   for (int L = Lb ? Lb + 1 : 2; L <= Lf; L++) {
     if ((rc = cgc<T>(S, L))) return rc;
     if ((rc = sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
+    if ((rc = settle<T>(S, L))) return rc;
   }
   if (final_residual) {
     if ((rc = residual<T>(S, Lf, norm_base + NS_FINAL, true, true))) return rc;
@@ -887,6 +953,14 @@ int pmg_set_graph(pmg_solver *S, int on) {
 int pmg_set_bottom_fusion(pmg_solver *S, int on) {
   if (!S) return -1;
   S->use_bottom = on ? 1 : 0;
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_set_sweep_blocking(pmg_solver *S, int on, int zchunk) {
+  if (!S || zchunk < 0) return -1;
+  S->use_sweep2 = on ? 1 : 0;
+  if (zchunk > 0) S->sweep2_zchunk = zchunk;
   invalidate_graphs(S);
   return 0;
 }
@@ -1230,6 +1304,7 @@ int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double 
                 : residual<double>(S, L, NS_RESID + L - 1, fine, fine);
   }
   PMG_HOST_CHECK(hipEventRecord(e1, S->stream));
+  if (!rc && which == 0) rc = prec ? settle<float>(S, L) : settle<double>(S, L);
   PMG_HOST_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
   PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));

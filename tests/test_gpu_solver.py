@@ -112,6 +112,42 @@ def test_graph_replays_match_eager(pm):
     Se.close()
 
 
+@pytest.mark.parametrize("n,pre,pst", [(65, 2, 2), (129, 2, 2), (129, 3, 1), (257, 2, 2)])
+def test_sweep_blocking_matches_single(pm, n, pre, pst):
+    """pairs of sweeps run as one temporally blocked pass give the same u,
+    norms and coarsest iteration counts as single sweeps (also with odd
+    sweep counts, where u ends a level in the ping-pong twin)"""
+    rho = -pm.gaussian_source(n, seed=8)
+    Sb, _ = _pair(pm, n, rho, pre=pre, pst=pst) if n < 257 else (_solver_only(pm, n, rho, pre, pst), None)
+    Ss, _ = _pair(pm, n, rho, pre=pre, pst=pst) if n < 257 else (_solver_only(pm, n, rho, pre, pst), None)
+    Sb.set_sweep_blocking(True)
+    Ss.set_sweep_blocking(False)
+    for _ in range(3):
+        rb, nb, cb = Sb.vcycle()
+        rs, ns, cs = Ss.vcycle()
+        assert cb == cs
+        assert abs(rb - rs) <= 1e-13 * rs
+        assert np.allclose(nb, ns, rtol=1e-12, atol=0)
+    for L in range(1, Sb.nlev + 1):
+        assert np.array_equal(Sb.download(L, pm.PMG_F_U), Ss.download(L, pm.PMG_F_U)), L
+    Sb.zero_solution()
+    Ss.zero_solution()
+    r0b, hb = Sb.solve_rel(1, 1e-10, 40)
+    r0s, hs = Ss.solve_rel(1, 1e-10, 40)
+    assert hb[-1] <= 1e-10 * r0b and hs[-1] <= 1e-10 * r0s
+    assert abs(len(hb) - len(hs)) <= 1
+    Sb.close()
+    Ss.close()
+
+
+def _solver_only(pm, n, rho, pre, pst):
+    S = pm.Solver(n)
+    S.set_params(pre=pre, pst=pst)
+    S.upload(S.nlev, pm.PMG_F_RHO, rho)
+    S.init()
+    return S
+
+
 @pytest.mark.parametrize("n", [33, 65, 129])
 def test_restrict_fusion_matches_unfused(pm, n):
     """the finest residual computed inside the injection gives the same
