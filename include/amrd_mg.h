@@ -1,0 +1,78 @@
+/*
+ * amrd_mg.h -- AMRD's multigrid driver (amrd/mg.c) over pamr.h, on one
+ * MI355X (gfx950).
+ *
+ * The application supplies the MG hooks of amrd()'s argument list
+ * (include/amrd.h:17-34,137-159): app_MG_residual, app_MG_relax, app_L_op.
+ * As in AMRD they are called once per grid of a level inside a PAMR
+ * sequential iterator (call_app / call_rr_app, amrd/util.c:39-103); a hook
+ * fetches its grid with PAMR_get_g_attribs and its grid functions with
+ * PAMR_get_gfn -- device pointers here, so a hook launches kernels (on
+ * amrd_gpu_stream()) and returns its norm as a host double.  The norms of a
+ * level are folded with max, as call_rr_app's MPI_MAX on one rank.
+ *
+ * Return convention: 1 success, 0 failure with a message, as AMRD / PAMR.
+ */
+#ifndef AMRD_MG_GPU_H
+#define AMRD_MG_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef double (*amrd_real_hook)(void);
+typedef void (*amrd_void_hook)(void);
+typedef double (*amrd_level_relax_hook)(int L, int comm_only);
+typedef double (*amrd_level_real_hook)(int L);
+typedef void (*amrd_level_void_hook)(int L);
+
+/* The MG hooks of amrd() (include/amrd.h:137-159): app_MG_residual
+ * (RMS of res = L[f] - rhs where cmask is on), app_MG_relax (one sweep,
+ * RMS of the residuals it saw), app_L_op (f_lop = L[f] where cmask is on). */
+void amrd_set_app_MG_hooks(amrd_real_hook app_MG_residual, amrd_real_hook app_MG_relax,
+                           amrd_void_hook app_L_op);
+
+/* Optional level hooks, one call per MG level instead of one per grid; no
+ * counterpart in the reference, for applications that batch all grids of a
+ * level into one launch.  relax(L, comm_only) relaxes every grid of L (only
+ * those with PAMR_get_g_comm != 0 when comm_only) and returns the max norm;
+ * residual(L) / lop(L) likewise.  NULL entries keep the per-grid hooks. */
+void amrd_set_app_MG_level_hooks(amrd_level_relax_hook relax, amrd_level_real_hook residual,
+                                 amrd_level_void_hook lop);
+
+/* AMRD's MG variable wiring (amrd/io.c:843-937,988-1005): for every
+ * elliptic variable (names separated by spaces or commas, each already
+ * defined with PAMR_def_var_full, in_mgh = 1) define <v>_res (mg_inject
+ * half-weight, full-weight when <v>_res is in mg_fw_restr; mg_interp second
+ * order when <v> is in mg_interp2), <v>_lop and <v>_rhs (MG only), then
+ * cmask (and chr when do_ex).  Call before PAMR_compose_hierarchy. */
+int amrd_mg_def_vars(const char *elliptic_vars, const char *mg_fw_restr, const char *mg_interp2,
+                     int do_ex);
+
+/* AMRD's MG parameters (amrd/io.c:97-106,269-285); defaults 3, 3, 1, 50,
+ * 0.0, 1e-3, 5000, 1.0, 1.0 */
+void amrd_mg_set_params(int pre_swp, int pst_swp, int min_iter, int max_iter, double tol,
+                        double crtol, int max_citer, double w0_r, double w0_i);
+
+/* vcycle (amrd/mg.c:84-201) on the current MG hierarchy; returns the
+ * residual norm of the finest level (negative on error) */
+double amrd_vcycle(int giter);
+
+/* solve_elliptics (amrd/mg.c:20-79): PAMR_build_mgh(L, finest AMR level,
+ * tl), V-cycles while (iter < min_iter || res > tol) && iter < max_iter,
+ * PAMR_destroy_mgh.  Returns the number of cycles (negative on error);
+ * res_out (may be NULL) the last residual. */
+int amrd_solve_elliptics(int L, int tl, double *res_out);
+
+/* last norms of amrd_vcycle: lev_norms[L-1] for MG levels 1..Lf (the
+ * reference's res[] array) and the coarsest-level sweep count */
+int amrd_mg_last_norms(double *lev_norms, int *citer);
+
+/* the stream every device operation of pamr.h / amrd_mg.h is enqueued on
+ * (a hipStream_t); NULL before PAMR_compose_hierarchy */
+void *amrd_gpu_stream(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -79,6 +79,30 @@ int pmg_amr_time_sweep(pmg_amr *A, int L, int nrep, double *ms_avg);
 int pmg_amr_set_graph(pmg_amr *A, int on);
 void *pmg_amr_stream(pmg_amr *A);
 
+/* Further grid functions on every grid of every level (zeroed), for callers
+ * that hold more variables than the built-in ones (the PAMR variable table,
+ * src/pamr.c:1120-1175).  Returns the new field id (PMG_F_EXTRA, +1, ...)
+ * or -1; at most PMG_MAX_EXTRA fields. */
+#define PMG_F_EXTRA 16
+#define PMG_MAX_EXTRA 64
+int pmg_amr_add_field(pmg_amr *A);
+/* points of level L's field pool: the grids of L lie back to back from
+ * pmg_amr_field(A, L, 0, which), each padded to a multiple of 64 points
+ * (padding stays zero unless written) */
+long long pmg_amr_pool_points(pmg_amr *A, int L);
+/* PAMR_sync / PAMR_inject / PAMR_interp (src/pamr.c:601-685) of several
+ * fields at once (the tf-bit-selected gfns of communicate(), src/pamr.c:
+ * 533-593); ops[] per field 1 straight, 2 half-weight, 3 full-weight.
+ * Enqueued on pmg_amr_stream without a host synchronisation.  sync: bdy
+ * (AMR_bdy_width) 0 or 1, otherwise -2. */
+int pmg_amr_sync_fields(pmg_amr *A, int L, int bdy, int nf, const int *which);
+int pmg_amr_inject_fields(pmg_amr *A, int Lf, int nf, const int *which, const int *ops);
+int pmg_amr_interp_fields(pmg_amr *A, int Lc, int nf, const int *which);
+/* set_cmask_bdy (which 0) / set_cmask_child (which 1) of every grid of L
+ * (amrd/mg.c:362-488) written into a field as 1.0 / 0.0; enqueued on
+ * pmg_amr_stream */
+int pmg_amr_fill_cmask(pmg_amr *A, int L, int which, int field);
+
 #ifdef __cplusplus
 }
 #endif

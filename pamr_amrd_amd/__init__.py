@@ -1,7 +1,8 @@
 """pamr_amrd_amd -- MI355X (gfx950) multigrid hot path of PAMR/AMRD.
 
-Python view of the C ABI declared in ``include/pamr_mg_gpu.h`` (and the
-PAMR/AMRD surface in ``include/pamr.h`` / ``include/amrd.h``), implemented by
+Python view of the C ABI declared in ``include/pamr_mg_gpu.h`` and
+``include/pamr_amr_gpu.h`` (and the PAMR/AMRD surface in ``include/pamr.h`` /
+``include/amrd_mg.h``), implemented by
 ``libpamr_mg.so`` in this directory.  Device memory is owned either by the
 library (``Solver``) or by the caller (torch tensors passed as pointers to
 the m_util_r8 / hook entry points).  There is no CPU fallback: when the
@@ -128,6 +129,85 @@ def _declare(L):
     L.pmg_amr_set_graph.argtypes = [_vp, i]
     L.pmg_amr_stream.argtypes = [_vp]
     L.pmg_amr_stream.restype = _vp
+    L.pmg_amr_add_field.argtypes = [_vp]
+    L.pmg_amr_pool_points.argtypes = [_vp, i]
+    L.pmg_amr_pool_points.restype = C.c_longlong
+    L.pmg_amr_sync_fields.argtypes = [_vp, i, i, i, _ip]
+    L.pmg_amr_inject_fields.argtypes = [_vp, i, i, _ip, _ip]
+    L.pmg_amr_interp_fields.argtypes = [_vp, i, i, _ip]
+    L.pmg_amr_fill_cmask.argtypes = [_vp, i, i, i]
+    _declare_pamr(L)
+
+
+def _declare_pamr(L):
+    """include/pamr.h and include/amrd_mg.h (C strings are bytes)."""
+    i, d, s = C.c_int, C.c_double, C.c_char_p
+    L.PAMR_init_context.argtypes = [_dp, i, i, _ip, _dp, s, i]
+    L.PAMR_free_context.argtypes = [i]
+    L.PAMR_free_context.restype = None
+    L.PAMR_compose_hierarchy.argtypes = [i, i, i, _ip, _dp, d]
+    L.PAMR_build_mgh.argtypes = [i, i, i]
+    L.PAMR_destroy_mgh.argtypes = []
+    for f in ("PAMR_set_rho", "PAMR_get_rho"):
+        getattr(L, f).argtypes = [_ip, _ip, i]
+        getattr(L, f).restype = None
+    L.PAMR_set_lambda.argtypes = [d]
+    L.PAMR_set_lambda.restype = None
+    L.PAMR_get_lambda.argtypes = [_dp]
+    L.PAMR_get_lambda.restype = None
+    L.PAMR_get_dxdt.argtypes = [i, _dp, _dp]
+    L.PAMR_get_dxdt.restype = None
+    for f in ("PAMR_set_MG_coarse_width", "PAMR_get_MG_coarse_width", "PAMR_set_ghost_width",
+              "PAMR_get_ghost_width"):
+        getattr(L, f).argtypes = [_ip]
+        getattr(L, f).restype = None
+    L.PAMR_get_global_bbox.argtypes = [_dp]
+    L.PAMR_get_global_bbox.restype = None
+    L.PAMR_def_var_brief.argtypes = [s]
+    L.PAMR_def_var_full.argtypes = [s] + [i] * 12 + [_ip]
+    L.PAMR_get_var_attribs.argtypes = [s] + [_ip] * 13
+    L.PAMR_get_gfn.argtypes = [s, i, i]
+    for f in ("PAMR_freeze_tf_bits", "PAMR_thaw_tf_bits", "PAMR_clear_tf_bits"):
+        getattr(L, f).argtypes = []
+        getattr(L, f).restype = None
+    L.PAMR_set_tf_bit.argtypes = [i, i]
+    L.PAMR_set_tf_bit.restype = None
+    L.PAMR_set_tf_bits.argtypes = [i, i, i]
+    L.PAMR_set_tf_bits.restype = None
+    L.PAMR_sync.argtypes = [i, i, i, i]
+    L.PAMR_inject.argtypes = [i, i, i]
+    L.PAMR_interp.argtypes = [i, i, i]
+    L.PAMR_set_time.argtypes = [i, d]
+    L.PAMR_set_time.restype = None
+    L.PAMR_get_time.argtypes = [i]
+    L.PAMR_get_time.restype = d
+    L.PAMR_init_s_iter.argtypes = [i, i, i]
+    for f in ("PAMR_next_g", "PAMR_push_iter", "PAMR_pop_iter"):
+        getattr(L, f).argtypes = []
+    L.PAMR_get_g_attribs.argtypes = [_ip, _ip, _ip, _dp, _ip, _dp, _ip, C.POINTER(_dp), C.POINTER(_vp)]
+    for f in ("PAMR_get_g_rank", "PAMR_get_g_dim", "PAMR_get_g_shape", "PAMR_get_g_ghost_width",
+              "PAMR_get_g_ngfs", "PAMR_get_g_coarsest", "PAMR_get_g_level", "PAMR_get_g_comm"):
+        getattr(L, f).argtypes = [_ip]
+    L.PAMR_get_g_bbox.argtypes = [_dp]
+    L.PAMR_get_g_t.argtypes = [_dp]
+    L.PAMR_get_g_x.argtypes = [C.POINTER(_dp)]
+    L.PAMR_get_g_gfs.argtypes = [C.POINTER(_vp)]
+    L.PAMR_get_max_lev.argtypes = [i]
+    L.PAMR_get_min_lev.argtypes = [i]
+    L.PAMR_set_g_comm.argtypes = [i]
+    L.amrd_set_app_MG_hooks.argtypes = [_vp, _vp, _vp]
+    L.amrd_set_app_MG_hooks.restype = None
+    L.amrd_set_app_MG_level_hooks.argtypes = [_vp, _vp, _vp]
+    L.amrd_set_app_MG_level_hooks.restype = None
+    L.amrd_mg_def_vars.argtypes = [s, s, s, i]
+    L.amrd_mg_set_params.argtypes = [i, i, i, i, d, d, i, d, d]
+    L.amrd_mg_set_params.restype = None
+    L.amrd_vcycle.argtypes = [i]
+    L.amrd_vcycle.restype = d
+    L.amrd_solve_elliptics.argtypes = [i, i, _dp]
+    L.amrd_mg_last_norms.argtypes = [_dp, _ip]
+    L.amrd_gpu_stream.argtypes = []
+    L.amrd_gpu_stream.restype = _vp
 
 
 def _check(rc, what):

@@ -41,6 +41,28 @@ def _compile(src):
     return out
 
 
+APPS = os.path.join(HERE, "apps")
+INC = os.path.join(HERE, "..", "include")
+
+
+def build_apps() -> list:
+    """The example application hooks (plain C, built with gcc) as shared
+    libraries that link libpamr_mg.so from the package directory."""
+    out = []
+    for src in sorted(f for f in os.listdir(APPS) if f.endswith(".c")):
+        so = os.path.join(APPS, "lib" + src[:-2] + ".so")
+        cmd = ["gcc", "-std=gnu99", "-O2", "-fPIC", "-shared", "-Wall", "-D__HIP_PLATFORM_AMD__",
+               f"-I{INC}", "-I/opt/rocm/include", os.path.join(APPS, src), "-o", so + ".tmp",
+               f"-L{HERE}", "-lpamr_mg", "-L/opt/rocm/lib", "-lamdhip64",
+               "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"gcc failed for {src}:\n{r.stderr[-6000:]}")
+        os.replace(so + ".tmp", so)
+        out.append(so)
+    return out
+
+
 def build(jobs: int = 8) -> str:
     srcs = sources()
     with ThreadPoolExecutor(max_workers=min(jobs, len(srcs))) as ex:
@@ -50,6 +72,7 @@ def build(jobs: int = 8) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr[-6000:]}")
     os.replace(LIB + ".tmp", LIB)
+    build_apps()
     return LIB
 
 

@@ -343,6 +343,8 @@ struct ALevel {
   size_t pool = 0;
   double *u = nullptr, *ua = nullptr, *rho = nullptr, *rhs = nullptr, *res = nullptr, *lop = nullptr;
   double *a = nullptr, *c = nullptr;
+  std::vector<double *> extra;  // fields added by pmg_amr_add_field
+  double *cm[2] = {nullptr, nullptr};  // set_cmask_bdy / set_cmask_child as doubles (built on first use)
   uint8_t *child = nullptr;
   GridGeo *d_geo = nullptr;
   Spacing<double> sp;
@@ -360,6 +362,7 @@ struct ALevel {
 
 struct pmg_amr {
   int nlev = 0, nbase = 0, op = 0;
+  int nextra = 0;
   double cbbox[6];
   int periodic[3] = {0, 0, 0};
   std::vector<ALevel> lev;
@@ -783,7 +786,12 @@ double *field_pool(ALevel &Lv, int which) {
     case PMG_F_A: return Lv.a;
     case PMG_F_C: return Lv.c;
   }
+  if (which >= PMG_F_EXTRA && which - PMG_F_EXTRA < (int)Lv.extra.size()) return Lv.extra[which - PMG_F_EXTRA];
   return nullptr;
+}
+
+bool valid_field(const pmg_amr *A, int which) {
+  return (which >= 0 && which <= PMG_F_C) || (which >= PMG_F_EXTRA && which < PMG_F_EXTRA + A->nextra);
 }
 
 // one transfer plan with up to kMaxFields fields; kind 0 copy, 1-3 restrict, 4 interp
@@ -1220,6 +1228,8 @@ void pmg_amr_destroy(pmg_amr *A) {
   for (ALevel &Lv : A->lev) {
     for (double **p : {&Lv.u, &Lv.ua, &Lv.rho, &Lv.rhs, &Lv.res, &Lv.lop, &Lv.a, &Lv.c})
       if (*p) (void)hipFree(*p);
+    for (double *p : Lv.extra) if (p) (void)hipFree(p);
+    for (double *p : Lv.cm) if (p) (void)hipFree(p);
     if (Lv.child) (void)hipFree(Lv.child);
     if (Lv.d_geo) (void)hipFree(Lv.d_geo);
     for (BTile *p : {Lv.d_stiles, Lv.d_ptiles}) if (p) (void)hipFree(p);
@@ -1364,7 +1374,7 @@ int pmg_amr_residual(pmg_amr *A, double *norm_out) {
 }
 
 int pmg_amr_sync(pmg_amr *A, int L, int bdy, int which) {
-  if (!A || L < 1 || L > A->nlev || which < 0 || which > PMG_F_C || (bdy != 0 && bdy != 1)) return -1;
+  if (!A || L < 1 || L > A->nlev || !valid_field(A, which) || (bdy != 0 && bdy != 1)) return -1;
   int rc = do_sync(A, L, bdy, which);
   if (rc) return rc;
   PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
@@ -1372,7 +1382,7 @@ int pmg_amr_sync(pmg_amr *A, int L, int bdy, int which) {
 }
 
 int pmg_amr_inject(pmg_amr *A, int Lf, int which, int op) {
-  if (!A || Lf < 2 || Lf > A->nlev || which < 0 || which > PMG_F_C || op < 1 || op > 3) return -1;
+  if (!A || Lf < 2 || Lf > A->nlev || !valid_field(A, which) || op < 1 || op > 3) return -1;
   int rc = run_plan(A, A->lev[Lf - 2].inject, Lf, Lf - 1, op, &which, 1);
   if (rc) return rc;
   PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
@@ -1380,7 +1390,7 @@ int pmg_amr_inject(pmg_amr *A, int Lf, int which, int op) {
 }
 
 int pmg_amr_interp(pmg_amr *A, int Lc, int which) {
-  if (!A || Lc < 1 || Lc >= A->nlev || which < 0 || which > PMG_F_C) return -1;
+  if (!A || Lc < 1 || Lc >= A->nlev || !valid_field(A, which)) return -1;
   int rc = run_plan(A, A->lev[Lc].interp, Lc, Lc + 1, 4, &which, 1);
   if (rc) return rc;
   PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
@@ -1433,5 +1443,84 @@ int pmg_amr_set_graph(pmg_amr *A, int on) {
 }
 
 void *pmg_amr_stream(pmg_amr *A) { return A ? (void *)A->stream : nullptr; }
+
+int pmg_amr_add_field(pmg_amr *A) {
+  if (!A || A->nextra >= PMG_MAX_EXTRA) return -1;
+  PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
+  for (ALevel &Lv : A->lev) {
+    double *p = nullptr;
+    if (hipMalloc(&p, Lv.pool * sizeof(double)) != hipSuccess) {
+      std::fprintf(stderr, "pamr_amr: out of device memory for a new field\n");
+      return -1;
+    }
+    PMG_HOST_CHECK(hipMemset(p, 0, Lv.pool * sizeof(double)));
+    Lv.extra.push_back(p);
+  }
+  return PMG_F_EXTRA + A->nextra++;
+}
+
+long long pmg_amr_pool_points(pmg_amr *A, int L) {
+  if (!A || L < 1 || L > A->nlev) return -1;
+  return (long long)A->lev[L - 1].pool;
+}
+
+// the multi-field transfers below only enqueue on pmg_amr_stream; fields are
+// grouped kMaxFields per launch (each field reads and writes only itself)
+static int run_grouped(pmg_amr *A, const XPlan &P, int Ls, int Ld, int kind, int nf, const int *which) {
+  for (int f0 = 0; f0 < nf; f0 += kMaxFields) {
+    const int m = std::min(kMaxFields, nf - f0);
+    int rc = run_plan(A, P, Ls, Ld, kind, which + f0, m);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int pmg_amr_sync_fields(pmg_amr *A, int L, int bdy, int nf, const int *which) {
+  if (!A || L < 1 || L > A->nlev || nf < 0 || (nf > 0 && !which)) return -1;
+  if (bdy != 0 && bdy != 1) return -2;
+  for (int f = 0; f < nf; f++) if (!valid_field(A, which[f])) return -1;
+  ALevel &Lv = A->lev[L - 1];
+  return run_grouped(A, bdy ? Lv.sync1 : Lv.sync0, L, L, 0, nf, which);
+}
+
+int pmg_amr_inject_fields(pmg_amr *A, int Lf, int nf, const int *which, const int *ops) {
+  if (!A || Lf < 2 || Lf > A->nlev || nf < 0 || (nf > 0 && (!which || !ops))) return -1;
+  for (int f = 0; f < nf; f++) if (!valid_field(A, which[f]) || ops[f] < 1 || ops[f] > 3) return -1;
+  for (int op = 1; op <= 3; op++) {
+    std::vector<int> sel;
+    for (int f = 0; f < nf; f++) if (ops[f] == op) sel.push_back(which[f]);
+    int rc = run_grouped(A, A->lev[Lf - 2].inject, Lf, Lf - 1, op, (int)sel.size(), sel.data());
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int pmg_amr_fill_cmask(pmg_amr *A, int L, int which, int field) {
+  if (!A || L < 1 || L > A->nlev || (which != 0 && which != 1) || !valid_field(A, field)) return -1;
+  ALevel &Lv = A->lev[L - 1];
+  double *dst = field_pool(Lv, field);
+  if (!dst) return -1;
+  if (!Lv.cm[which]) {
+    std::vector<double> h(Lv.pool, 0.0);
+    std::vector<uint8_t> m;
+    for (int gi = 0; gi < (int)Lv.g.size(); gi++) {
+      cmask_bytes(A, L, gi, which, m);
+      for (size_t q = 0; q < m.size(); q++) h[Lv.g[gi].off + q] = m[q] ? 1.0 : 0.0;
+    }
+    if (hipMalloc(&Lv.cm[which], Lv.pool * sizeof(double)) != hipSuccess) {
+      Lv.cm[which] = nullptr;
+      return -1;
+    }
+    PMG_HOST_CHECK(hipMemcpy(Lv.cm[which], h.data(), Lv.pool * sizeof(double), hipMemcpyHostToDevice));
+  }
+  PMG_HOST_CHECK(hipMemcpyAsync(dst, Lv.cm[which], Lv.pool * sizeof(double), hipMemcpyDeviceToDevice, A->stream));
+  return 0;
+}
+
+int pmg_amr_interp_fields(pmg_amr *A, int Lc, int nf, const int *which) {
+  if (!A || Lc < 1 || Lc >= A->nlev || nf < 0 || (nf > 0 && !which)) return -1;
+  for (int f = 0; f < nf; f++) if (!valid_field(A, which[f])) return -1;
+  return run_grouped(A, A->lev[Lc].interp, Lc, Lc + 1, 4, nf, which);
+}
 
 }  // extern "C"

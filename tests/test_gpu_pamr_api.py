@@ -1,0 +1,203 @@
+"""The PAMR / AMRD calls (include/pamr.h, include/amrd_mg.h) driven as a C
+application drives them -- per-grid hooks written the reference's way
+(apps/nps_hooks.c) over device grid functions -- against the multi-patch
+oracle (oracle/amr_oracle.c, pinned in test_oracle_amr.py).
+
+Bar: grid geometry and the iterator's view of both hierarchies exact; the
+PAMR transfer entry points selected by transfer bits bit for bit; V-cycles
+(per-grid hooks and level hooks) bit for bit on every grid of every level
+with per-cycle norms within 1e-12 relative and the same coarsest-level
+sweep count; solve_elliptics leaves the solution in the AMR hierarchy's
+memory (shared with the MG hierarchy, src/gh.c:1139-1157).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pamr_amrd_amd as pm
+from pamr_amrd_amd import hierarchy as H
+from pamr_amrd_amd.pamr_app import PAMR_AMRH, PAMR_MGH, NpsContext
+
+pytestmark = pytest.mark.gpu
+
+NAME = {O.OA_U: "V", O.OA_RHO: "rho", O.OA_RES: "V_res", O.OA_A: "a", O.OA_C: "c"}
+
+
+def layout(seed, base_n=17):
+    return H.random_boxes(base_n=base_n, nref=2, nmin=3, nmax=6, smin=5, smax=13, seed=seed)
+
+
+def pair(seed, op=0, base_n=17):
+    levels = layout(seed, base_n)
+    print(f"case pamr api seed={seed} op={op} base={base_n} boxes={[len(l) for l in levels]}", flush=True)
+    return O.AmrMG(base_n, levels, op=op), NpsContext(base_n, levels, op=op)
+
+
+def load_sources(m, ctx, op, seed):
+    """rho (and a, c) on every AMR grid of both; the oracle's MG-only levels
+    are filled by its init, the context's by PAMR_build_mgh."""
+    nA = ctx.max_lev
+    for l in range(1, nA + 1):
+        Lm = m.base + l - 1
+        for g in range(m.ngrids(Lm)):
+            shape, bb = m.grid(Lm, g)
+            vals = {O.OA_RHO: -H.gaussian_on(shape, bb, seed=seed)}
+            if op == 1:
+                vals[O.OA_A] = H.lognormal_on(shape, bb)
+                vals[O.OA_C] = H.uniform_on(shape, bb)
+            for w, v in vals.items():
+                m.field(Lm, g, w)[:] = v
+                ctx.upload(PAMR_AMRH, l, g, NAME[w], v)
+
+
+def test_geometry_and_iterator(torch_cuda):
+    m, ctx = pair(1)
+    L = ctx.L
+    assert L.PAMR_get_max_lev(PAMR_AMRH) == ctx.max_lev
+    for l in range(1, ctx.max_lev + 1):
+        grids = ctx.grids(PAMR_AMRH, l)
+        Lm = m.base + l - 1
+        assert len(grids) == m.ngrids(Lm)
+        for g, (shape, bb, gfs) in enumerate(grids):
+            sm, bm = m.grid(Lm, g)
+            assert shape == sm and np.array_equal(bb, bm)
+            assert gfs[ctx.gfn("V", PAMR_AMRH) - 1] and gfs[ctx.gfn("rho", PAMR_AMRH) - 1]
+            assert not gfs[ctx.gfn("V_res", PAMR_MGH) - 1]   # no MG hierarchy yet
+    nl = ctx.build_mgh()
+    assert nl == m.nlev and L.PAMR_get_max_lev(PAMR_MGH) == nl
+    vmg, vamr = ctx.gfn("V", PAMR_MGH), ctx.gfn("V", PAMR_AMRH)
+    for Lm in range(1, nl + 1):
+        grids = ctx.grids(PAMR_MGH, Lm)
+        assert len(grids) == m.ngrids(Lm)
+        for g, (shape, bb, gfs) in enumerate(grids):
+            assert (shape, list(bb)) == (m.grid(Lm, g)[0], list(m.grid(Lm, g)[1]))
+            assert gfs[vmg - 1] and gfs[ctx.gfn("V_res", PAMR_MGH) - 1]
+            if Lm < m.base:
+                assert not gfs[vamr - 1]                      # AMR gfns do not exist on MG-only levels
+            else:
+                assert gfs[vmg - 1] == gfs[vamr - 1]          # shared memory at tl 1
+    # coarsest flag, comm flags, iterator stack, coordinates
+    assert L.PAMR_init_s_iter(1, PAMR_MGH, 0) == 1
+    c = C.c_int()
+    assert L.PAMR_get_g_coarsest(C.byref(c)) == 1 and c.value == 1
+    assert L.PAMR_push_iter() == 1
+    assert L.PAMR_init_s_iter(nl, PAMR_MGH, 0) == 1
+    L.PAMR_get_g_coarsest(C.byref(c))
+    assert c.value == 0
+    lev = C.c_int()
+    L.PAMR_get_g_level(C.byref(lev))
+    assert lev.value == nl
+    x = (C.POINTER(C.c_double) * 3)()
+    assert L.PAMR_get_g_x(x) == 1
+    shp = (C.c_int * 3)()
+    L.PAMR_get_g_shape(shp)
+    bb = np.zeros(6)
+    L.PAMR_get_g_bbox(pm.P(bb))
+    h = (bb[1] - bb[0]) / (shp[0] - 1)
+    assert x[0][1] - x[0][0] == h and x[0][shp[0] - 1] == bb[1]
+    assert L.PAMR_pop_iter() == 1
+    L.PAMR_get_g_level(C.byref(lev))
+    assert lev.value == 1
+    assert L.PAMR_destroy_mgh() == 1
+    assert ctx.L.PAMR_init_s_iter(1, PAMR_MGH, 0) == 0
+    ctx.close()
+
+
+@pytest.mark.parametrize("seed", [2, 3])
+def test_transfer_entry_points(torch_cuda, seed):
+    """PAMR_sync / PAMR_inject / PAMR_interp with hand-set transfer bits
+    against the oracle's transfers of the same field"""
+    m, ctx = pair(seed)
+    L = ctx.L
+    nl = ctx.build_mgh()
+    rng = np.random.default_rng(seed)
+
+    def fill(levels, which):
+        for Lm in levels:
+            for g in range(m.ngrids(Lm)):
+                v = rng.standard_normal(m.grid(Lm, g)[0])
+                m.field(Lm, g, which)[:] = v
+                ctx.upload(PAMR_MGH, Lm, g, NAME[which], v)
+
+    def compare(levels, which):
+        for Lm in levels:
+            for g in range(m.ngrids(Lm)):
+                got = ctx.download(PAMR_MGH, Lm, g, NAME[which])
+                assert np.array_equal(got, m.field(Lm, g, which)), (Lm, g, which)
+
+    def bit(which, val):
+        L.PAMR_clear_tf_bits()
+        L.PAMR_set_tf_bit(ctx.gfn(NAME[which], PAMR_MGH), val)   # also freezes
+
+    for bdy in (0, 1):
+        fill([nl], O.OA_U)
+        m.sync_field(nl, bdy, O.OA_U)
+        bit(O.OA_U, 1)
+        assert L.PAMR_sync(nl, 0, PAMR_MGH, bdy) == 1
+        compare([nl], O.OA_U)
+    for op in (1, 2, 3):
+        for lf in (nl, m.base):
+            fill([lf, lf - 1], O.OA_RES)
+            m.inject_field(lf, O.OA_RES, op)
+            bit(O.OA_RES, op)
+            assert L.PAMR_inject(lf, 0, PAMR_MGH) == 1
+            compare([lf - 1], O.OA_RES)
+    for lc in (m.base, m.base + 1):
+        fill([lc, lc + 1], O.OA_RES)
+        m.interp_field(lc, O.OA_RES)
+        bit(O.OA_RES, 2)
+        assert L.PAMR_interp(lc, 0, PAMR_MGH) == 1
+        compare([lc + 1], O.OA_RES)
+    bit(O.OA_RES, 4)
+    assert L.PAMR_interp(m.base, 0, PAMR_MGH) == 0              # fourth order: not supported
+    L.PAMR_thaw_tf_bits()
+    ctx.close()
+
+
+@pytest.mark.parametrize("seed,op,level_hooks", [(1, 0, False), (2, 0, True), (4, 1, False), (5, 1, True)])
+def test_vcycles_match_oracle(torch_cuda, seed, op, level_hooks):
+    m, ctx = pair(seed, op=op)
+    load_sources(m, ctx, op, 1234 + seed)
+    m.set_params()
+    m.init()
+    ctx.set_params(pre=2, pst=2)
+    ctx.app.nps_set_level_hooks(1 if level_hooks else 0)
+    nl = ctx.build_mgh()
+    assert nl == m.nlev
+    for Lm in range(1, nl + 1):
+        for g in range(m.ngrids(Lm)):
+            for w in (O.OA_U, O.OA_RHO):
+                assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, NAME[w]), m.field(Lm, g, w)), ("init", Lm, g, w)
+    for c in range(4):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = ctx.vcycle(c)
+        print(f"cycle {c}: oracle {rm:.15e} hooks {ra:.15e} citer {cm} {ca}", flush=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert ca == cm
+        assert np.allclose(na, nm, rtol=1e-12, atol=0), (c, na, nm)
+    for Lm in range(1, nl + 1):
+        for g in range(m.ngrids(Lm)):
+            assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, "V"), m.field(Lm, g, O.OA_U)), ("u", Lm, g)
+    ctx.close()
+
+
+def test_solve_elliptics_leaves_solution_in_amr_levels(torch_cuda):
+    m, ctx = pair(3)
+    load_sources(m, ctx, 0, 77)
+    m.set_params()
+    m.init()
+    ctx.set_params(pre=2, pst=2, min_iter=1, max_iter=6, tol=0.0)
+    res = C.c_double()
+    it = ctx.L.amrd_solve_elliptics(1, 1, C.byref(res))
+    assert it == 6
+    for _ in range(6):
+        rm, _, _ = m.vcycle()
+    assert abs(res.value - rm) <= 1e-12 * rm
+    assert ctx.L.PAMR_init_s_iter(1, PAMR_MGH, 0) == 0     # the MG hierarchy is gone
+    for l in range(1, ctx.max_lev + 1):
+        Lm = m.base + l - 1
+        for g in range(m.ngrids(Lm)):
+            assert np.array_equal(ctx.download(PAMR_AMRH, l, g, "V"), m.field(Lm, g, O.OA_U)), (l, g)
+    ctx.close()
