@@ -96,16 +96,17 @@ def sequence_leg(args, pm, torch):
 
     Training (not timed): (residual, error) crops from plain FP64 V-cycles of
     solves 0 .. T-1, Adam on the GPU.  Evaluation: solves T .. T+K-1, each
-    from the same warm start (second-order extrapolation of the two previous
-    converged solutions) in four ways -- learned + FP32 (mode 2), FP32 inner
-    cycles (mode 1), plain FP64 -- and plain FP64 from u = 0 (the reference's
-    cold solve).  Timed per solve: setting the initial guess and the solve to
-    ||r|| <= rtol ||r(u=0)||."""
+    solved in six ways from its own initial guess -- learned + FP32 (mode 2),
+    FP32 inner cycles (mode 1) and plain FP64 from the history projection warm
+    start (the last six solves), FP32 and plain FP64 from second-order
+    extrapolation of the two previous solutions, and plain FP64 from u = 0
+    (the reference's cold solve).  Timed per solve: computing and setting the
+    initial guess, and the solve to ||r|| <= rtol ||r(u=0)||."""
     from pamr_amrd_amd import sequence as SQ
     n = args.n
     T, K = args.seq_train, args.steps
     log = lambda m: print(f"bench: {m}", file=sys.stderr, flush=True)
-    seq = SQ.Sequence(n=n, seed=1234, speed=0.002)
+    seq = SQ.Sequence(n=n, seed=1234, speed=0.002, hist_k=args.seq_hist)
     t0 = time.perf_counter()
     X, Y = seq.collect(nsolves=T, pairs=args.seq_pairs, side=16, cycles=8)
     t_collect = time.perf_counter() - t0
@@ -119,21 +120,17 @@ def sequence_leg(args, pm, torch):
         f"loss {sum(losses[:20]) / 20:.3e} -> {sum(losses[-20:]) / 20:.3e}")
     S = seq.S
     S.set_net(net)
-    L = pm.lib()
-    u0 = torch.empty((n, n, n), dtype=torch.float64, device="cuda")
-    modes = (("learned_fp32", 2, True), ("fp32", 1, True), ("fp64_warm", 0, True), ("fp64_cold", 0, False))
+    modes = (("learned_fp32", 2, "proj"), ("fp32", 1, "proj"), ("fp64_warm", 0, "proj"),
+             ("fp32_extrap2", 1, "extrap2"), ("fp64_extrap2", 0, "extrap2"), ("fp64_cold", 0, "zero"))
     times = {m[0]: [] for m in modes}
     iters = {m[0]: [] for m in modes}
+    misfit = []
     for s in range(T, T + args.seq_warmup + K):
-        rref = seq.prepare(s, warm=True)
-        SQ._copy_grid(pm.tptr(u0), S.field_ptr(seq.Lf, pm.PMG_F_U), n)
-        for name, mode, warm in modes:
+        rref = seq.load_source(s)
+        for name, mode, method in modes:
             torch.cuda.synchronize()
             t1 = time.perf_counter()
-            if warm:
-                L.pmg_set_solution_lincomb(S.h, pm.tptr(u0), None, 1.0, 0.0)
-            else:
-                S.zero_solution()
+            seq.warm_start(method, misfit=False)
             _, h = seq.solve(mode, args.rtol, rref)
             torch.cuda.synchronize()
             dt = 1e3 * (time.perf_counter() - t1)
@@ -142,6 +139,10 @@ def sequence_leg(args, pm, torch):
             if s >= T + args.seq_warmup:
                 times[name].append(dt)
                 iters[name].append(len(h))
+        if s >= T + args.seq_warmup and seq.nhist:  # misfit of the projection (untimed)
+            seq.warm_start("proj", misfit=True)
+            misfit.append(seq.last_misfit)
+            _, h = seq.solve(0, args.rtol, rref)
         seq.push_solution()
     mean = {k: sum(v) / len(v) for k, v in times.items()}
     return {
@@ -149,6 +150,10 @@ def sequence_leg(args, pm, torch):
         "iterations": {k: round(sum(v) / len(v), 2) for k, v in iters.items()},
         "speedup_learned_fp32_vs_fp64_cold": round(mean["fp64_cold"] / mean["learned_fp32"], 3),
         "speedup_learned_fp32_vs_fp64_warm": round(mean["fp64_warm"] / mean["learned_fp32"], 3),
+        "speedup_learned_fp32_vs_fp64_extrap2": round(mean["fp64_extrap2"] / mean["learned_fp32"], 3),
+        "warm_start": {"method": f"history projection over the last {seq.hist_k} solves",
+                       "mean_relative_misfit": float(f"{sum(misfit) / len(misfit):.4e}"),
+                       "max_relative_misfit": float(f"{max(misfit):.4e}")},
         "network": S.net_stats(),
         "training": {"solves": T, "pairs": int(X.shape[0]), "crop": 16, "adam_steps": args.seq_train_steps,
                      "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
@@ -312,6 +317,7 @@ def main():
     ap.add_argument("--seq-pairs", type=int, default=4000)
     ap.add_argument("--seq-train-steps", type=int, default=1500)
     ap.add_argument("--seq-warmup", type=int, default=2)
+    ap.add_argument("--seq-hist", type=int, default=6, help="solves kept for the history warm start (<= 8)")
     ap.add_argument("--no-seq", action="store_true", help="skip the learned sequence leg")
     ap.add_argument("--no-amr", action="store_true", help="skip the batched AMR hierarchy leg")
     ap.add_argument("--amr-cycles", type=int, default=10)
@@ -426,7 +432,8 @@ def main():
                   "rtol": args.rtol, "reference_norm": "||r(u=0)||",
                   "sequence": f"8 Gaussians drifting 0.002/solve; train on solves 0-{args.seq_train - 1}, "
                               f"timed solves {seqres['solves_timed'][0]}-{seqres['solves_timed'][1]}",
-                  "warm_start": "2 u_{s-1} - u_{s-2}", "sweep_variant": variant, "zchunk": zchunk}
+                  "warm_start": f"history projection over the last {args.seq_hist} solves (timed)",
+                  "sweep_variant": variant, "zchunk": zchunk}
         dtype = "f32 inner / f64 outer; network bf16 (fp32 accumulate)"
     else:
         value = ms_mixed

@@ -202,6 +202,19 @@ int pmg_set_net(pmg_solver *S, pmg_net *N);
 int pmg_net_stats(pmg_solver *S, int *accepted, int *rejected, double *t_net_ms, double *t_vc_ms);
 /* finest-level solution u = ca*a + cb*b (device FP64; b may be NULL) */
 int pmg_set_solution_lincomb(pmg_solver *S, const double *a, const double *b, double ca, double cb);
+/* Warm start from earlier solves of a sequence (no counterpart in the
+ * reference): F[0..K-1] earlier finest-level sources (the rho field's
+ * convention, newest first) and U[k] their solutions, device FP64 arrays of
+ * the finest level's shape, 1 <= K <= 8.  Chooses c minimising
+ * ||rho - sum_k c_k F[k]||_2 (rho: the solver's current finest source) and
+ * sets the finest u (both sweep buffers) to sum_k c_k U[k]; L being linear,
+ * the new residual is that misfit plus sum_k c_k r_k (r_k: the residuals the
+ * earlier solves stopped at).  c_out: the K coefficients (may be NULL);
+ * rel_out: when not NULL, ||rho - sum c_k F[k]|| / ||rho|| measured by one
+ * more pass over the sources.  Synchronises the solver's stream; call
+ * pmg_init afterwards, as after any new initial guess. */
+int pmg_history_warm_start(pmg_solver *S, int K, const double *const *F, const double *const *U, double *c_out,
+                           double *rel_out);
 /* lane-layout self-test of the BF16 16x16x16 MFMA used by the network:
  * D = bf16(A) * bf16(B) (host 16x16 row-major) */
 int pmg_net_mfma_selftest(const float *A, const float *B, float *D_out);

@@ -103,6 +103,7 @@ def _declare(L):
     L.pmg_set_net.argtypes = [_vp, _vp]
     L.pmg_net_stats.argtypes = [_vp, _ip, _ip, _dp, _dp]
     L.pmg_set_solution_lincomb.argtypes = [_vp, _vp, _vp, d, d]
+    L.pmg_history_warm_start.argtypes = [_vp, i, C.POINTER(_vp), C.POINTER(_vp), _dp, _dp]
     # batched AMR hierarchy (include/pamr_amr_gpu.h)
     L.pmg_amr_create.argtypes = [_ip, _dp, i, _ip, _dp, i, i]
     L.pmg_amr_create.restype = _vp
@@ -325,6 +326,19 @@ class Solver:
 
     def zero_solution(self):
         _check(self._L.pmg_zero_solution(self.h), "pmg_zero_solution")
+
+    def history_warm_start(self, F, U, misfit=True):
+        """u = sum c_k U[k] minimising ||rho - sum c_k F[k]|| (device
+        pointers, newest first); returns (c, relative misfit or None)."""
+        K = len(F)
+        assert K == len(U) and 1 <= K <= 8
+        fa = (C.c_void_p * K)(*F)
+        ua = (C.c_void_p * K)(*U)
+        c = np.zeros(K)
+        rel = C.c_double(0)
+        _check(self._L.pmg_history_warm_start(self.h, K, fa, ua, P(c), C.byref(rel) if misfit else None),
+               "pmg_history_warm_start")
+        return c, (rel.value if misfit else None)
 
     def time_kernel(self, which, lev, prec, nrep):
         ms = C.c_double(0)

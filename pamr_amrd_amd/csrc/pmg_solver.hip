@@ -147,6 +147,9 @@ struct pmg_solver {
   int net_retry_period = 20;
   double t_net_ms = -1, t_vc_ms = -1, rho_vc = 0.1;
   int net_accepted = 0, net_rejected = 0;
+  // history warm start: per-block Gram partials and their fold
+  double *d_hist = nullptr;
+  double *h_hist = nullptr;
 };
 
 // norm slot layout in d_norms
@@ -899,6 +902,8 @@ void pmg_destroy(pmg_solver *S) {
   if (S->d_norms) (void)hipFree(S->d_norms);
   if (S->d_citer) (void)hipFree(S->d_citer);
   if (S->h_pinned) (void)hipHostFree(S->h_pinned);
+  if (S->d_hist) (void)hipFree(S->d_hist);
+  if (S->h_hist) (void)hipHostFree(S->h_hist);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
 }
@@ -1267,6 +1272,248 @@ int pmg_set_solution_lincomb(pmg_solver *S, const double *a, const double *b, do
   hipLaunchKernelGGL(k_lincomb, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.u, F.d.ua, a, b, ca, cb,
                      F.size);
   PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// History warm start (a minimal-residual projection onto earlier solves, in
+// the spirit of P. F. Fischer, "Projection techniques for iterative solution
+// of Ax = b with successive right-hand sides", CMAME 163 (1998) 193-204).
+// Earlier sources F_0 (newest) .. F_{K-1} and their solutions U_k are given;
+// since L is linear, u0 = sum c_k U_k has the residual
+// (rho - sum c_k F_k) + sum c_k r_k, so c is chosen to minimise
+// ||rho - sum c_k F_k||_2.  The F_k of a smooth sequence are nearly
+// collinear, so the normal equations are formed in the backward-difference
+// basis B_j = nabla^j F_0 = sum_i (-1)^i C(j,i) F_i, whose vectors shrink
+// geometrically but stay far from collinear: each Gram entry keeps its
+// relative accuracy, and a diagonal scaling leaves a well-conditioned
+// system.  The K x K solve runs on the host in long double.
+// ---------------------------------------------------------------------------
+constexpr int kHistMax = 8;
+constexpr int kHistBlocks = 1024;
+constexpr int kHistM = kHistMax * (kHistMax + 1) / 2 + kHistMax + 1;
+
+struct HistPtrs {
+  const double *p[kHistMax];
+};
+
+__host__ __device__ constexpr int binom_c(int n, int k) {
+  int r = 1;
+  for (int i = 1; i <= k; i++) r = r * (n - k + i) / i;
+  return r;
+}
+
+// per block: G (upper triangle, row-major), g = B^T rho, rho^T rho
+template <int K>
+__global__ __launch_bounds__(256) void k_hist_gram(HistPtrs F, const double *rho, size_t n, double *partials) {
+  constexpr int M = K * (K + 1) / 2 + K + 1;
+  double acc[M];
+#pragma unroll
+  for (int m = 0; m < M; m++) acc[m] = 0.0;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+    double f[K], b[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) f[k] = F.p[k][q];
+#pragma unroll
+    for (int j = 0; j < K; j++) {
+      double v = 0.0;
+#pragma unroll
+      for (int i = 0; i <= j; i++) v += ((i & 1) ? -1.0 : 1.0) * (double)binom_c(j, i) * f[i];
+      b[j] = v;
+    }
+    const double r = rho[q];
+    int m = 0;
+#pragma unroll
+    for (int j = 0; j < K; j++)
+#pragma unroll
+      for (int l = j; l < K; l++) acc[m++] += b[j] * b[l];
+#pragma unroll
+    for (int j = 0; j < K; j++) acc[m++] += b[j] * r;
+    acc[m] += r * r;
+  }
+  __shared__ double red[256 / kWave][M];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    double v = acc[m];
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    if (lane == 0) red[wid][m] = v;
+  }
+  __syncthreads();
+  for (int m = threadIdx.x; m < M; m += 256) {
+    double v = 0.0;
+    for (int w = 0; w < 256 / kWave; w++) v += red[w][m];
+    partials[(size_t)blockIdx.x * M + m] = v;
+  }
+}
+
+// block-order fold of the partials: out[m] = sum_b partials[b][m]
+__global__ __launch_bounds__(64) void k_hist_fold(const double *partials, int nb, int M, double *out) {
+  const int m = threadIdx.x;
+  if (m >= M) return;
+  double v = 0.0;
+  for (int b = 0; b < nb; b++) v += partials[(size_t)b * M + m];
+  out[m] = v;
+}
+
+// per block: sum (rho - sum_k c_k F_k)^2 and sum rho^2
+template <int K>
+__global__ __launch_bounds__(256) void k_hist_misfit(HistPtrs F, const double *rho, const double *c, size_t n,
+                                                     double *partials) {
+  double e2 = 0.0, r2 = 0.0;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+    const double r = rho[q];
+    double v = r;
+#pragma unroll
+    for (int k = 0; k < K; k++) v -= c[k] * F.p[k][q];
+    e2 += v * v;
+    r2 += r * r;
+  }
+  block_reduce2<256>(e2, r2);
+  if (threadIdx.x == 0) {
+    partials[(size_t)blockIdx.x * 2] = e2;
+    partials[(size_t)blockIdx.x * 2 + 1] = r2;
+  }
+}
+
+// u (finest, both sweep buffers) = sum_k c_k U_k
+template <int K>
+__global__ __launch_bounds__(256) void k_hist_lincomb(double *u, double *ua, HistPtrs U, const double *c,
+                                                      size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; k++) v += c[k] * U.p[k][q];
+  u[q] = v;
+  ua[q] = v;
+}
+
+// Solves G a = g (G symmetric positive semi-definite, K x K) after scaling by
+// the diagonal, with a relative ridge for exactly dependent histories, by
+// Gaussian elimination with partial pivoting in long double.
+void hist_solve(int K, const double *Gup, const double *g, long double *a) {
+  long double A[kHistMax][kHistMax + 1], d[kHistMax];
+  int m = 0;
+  for (int j = 0; j < K; j++)
+    for (int l = j; l < K; l++) {
+      A[j][l] = A[l][j] = Gup[m];
+      m++;
+    }
+  for (int j = 0; j < K; j++) d[j] = A[j][j] > 0 ? 1.0L / std::sqrt((long double)A[j][j]) : 0.0L;
+  for (int j = 0; j < K; j++) {
+    for (int l = 0; l < K; l++) A[j][l] *= d[j] * d[l];
+    A[j][j] += 1e-15L;  // ridge on the unit diagonal: exactly dependent histories
+    A[j][K] = g[j] * d[j];
+  }
+  for (int col = 0; col < K; col++) {
+    int piv = col;
+    for (int r = col + 1; r < K; r++)
+      if (std::fabs((double)A[r][col]) > std::fabs((double)A[piv][col])) piv = r;
+    if (piv != col)
+      for (int l = 0; l <= K; l++) std::swap(A[col][l], A[piv][l]);
+    if (A[col][col] == 0) continue;
+    for (int r = col + 1; r < K; r++) {
+      const long double f = A[r][col] / A[col][col];
+      for (int l = col; l <= K; l++) A[r][l] -= f * A[col][l];
+    }
+  }
+  for (int r = K - 1; r >= 0; r--) {
+    long double v = A[r][K];
+    for (int l = r + 1; l < K; l++) v -= A[r][l] * a[l];
+    a[r] = A[r][r] != 0 ? v / A[r][r] : 0.0L;
+  }
+  for (int j = 0; j < K; j++) a[j] *= d[j];
+}
+
+enum { HIST_GRAM = 0, HIST_LINCOMB = 1, HIST_MISFIT = 2 };
+
+template <int K>
+int hist_launch(pmg_solver *S, const HistPtrs &F, const HistPtrs &U, double *dc, int what) {
+  Level &Lf = S->lev[S->nlev - 1];
+  constexpr int M = K * (K + 1) / 2 + K + 1;
+  if (what == HIST_MISFIT) {
+    hipLaunchKernelGGL(k_hist_misfit<K>, dim3(kHistBlocks), dim3(256), 0, S->stream, F, Lf.d.rho, dc, Lf.size,
+                       S->d_hist);
+    PMG_HOST_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_hist_fold, dim3(1), dim3(64), 0, S->stream, S->d_hist, kHistBlocks, 2,
+                       S->d_hist + (size_t)kHistBlocks * kHistM);
+  } else if (what == HIST_GRAM) {
+    hipLaunchKernelGGL(k_hist_gram<K>, dim3(kHistBlocks), dim3(256), 0, S->stream, F, Lf.d.rho, Lf.size, S->d_hist);
+    PMG_HOST_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(k_hist_fold, dim3(1), dim3(64), 0, S->stream, S->d_hist, kHistBlocks, M,
+                       S->d_hist + (size_t)kHistBlocks * kHistM);
+  } else {
+    hipLaunchKernelGGL(k_hist_lincomb<K>, dim3(nblk256(Lf.size)), dim3(256), 0, S->stream, Lf.d.u, Lf.d.ua, U, dc,
+                       Lf.size);
+  }
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int hist_dispatch(pmg_solver *S, int K, const HistPtrs &F, const HistPtrs &U, double *dc, int what) {
+  switch (K) {
+    case 1: return hist_launch<1>(S, F, U, dc, what);
+    case 2: return hist_launch<2>(S, F, U, dc, what);
+    case 3: return hist_launch<3>(S, F, U, dc, what);
+    case 4: return hist_launch<4>(S, F, U, dc, what);
+    case 5: return hist_launch<5>(S, F, U, dc, what);
+    case 6: return hist_launch<6>(S, F, U, dc, what);
+    case 7: return hist_launch<7>(S, F, U, dc, what);
+    case 8: return hist_launch<8>(S, F, U, dc, what);
+  }
+  return -1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmg_history_warm_start(pmg_solver *S, int K, const double *const *F, const double *const *U, double *c_out,
+                           double *rel_out) {
+  if (!S || K < 1 || K > kHistMax || !F || !U) return -1;
+  for (int k = 0; k < K; k++)
+    if (!F[k] || !U[k]) return -1;
+  if (!S->d_hist) {
+    PMG_HOST_CHECK(hipMalloc(&S->d_hist, sizeof(double) * ((size_t)kHistBlocks * kHistM + kHistM + kHistMax)));
+    PMG_HOST_CHECK(hipHostMalloc(&S->h_hist, sizeof(double) * (kHistM + kHistMax), hipHostMallocDefault));
+  }
+  HistPtrs hf{}, hu{};
+  for (int k = 0; k < K; k++) {
+    hf.p[k] = F[k];
+    hu.p[k] = U[k];
+  }
+  const int M = K * (K + 1) / 2 + K + 1;
+  double *d_fold = S->d_hist + (size_t)kHistBlocks * kHistM;
+  if (hist_dispatch(S, K, hf, hu, nullptr, HIST_GRAM)) return -1;
+  PMG_HOST_CHECK(hipMemcpyAsync(S->h_hist, d_fold, sizeof(double) * M, hipMemcpyDeviceToHost, S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  const double *Gup = S->h_hist, *g = S->h_hist + K * (K + 1) / 2;
+  long double a[kHistMax];
+  hist_solve(K, Gup, g, a);
+  // back to the F basis: B_j = sum_i (-1)^i C(j,i) F_i
+  long double c[kHistMax];
+  for (int i = 0; i < K; i++) {
+    c[i] = 0.0L;
+    for (int j = i; j < K; j++) c[i] += a[j] * (long double)(((i & 1) ? -1 : 1) * binom_c(j, i));
+  }
+  double *hc = S->h_hist + kHistM;
+  for (int k = 0; k < K; k++) hc[k] = (double)c[k];
+  double *dc = d_fold + kHistM;
+  PMG_HOST_CHECK(hipMemcpyAsync(dc, hc, sizeof(double) * K, hipMemcpyHostToDevice, S->stream));
+  if (hist_dispatch(S, K, hf, hu, dc, HIST_LINCOMB)) return -1;
+  if (rel_out) {  // the misfit measured directly (one more pass over the sources)
+    if (hist_dispatch(S, K, hf, hu, dc, HIST_MISFIT)) return -1;
+    PMG_HOST_CHECK(hipMemcpyAsync(S->h_hist, d_fold, sizeof(double) * 2, hipMemcpyDeviceToHost, S->stream));
+  }
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  if (rel_out) *rel_out = S->h_hist[1] > 0 ? std::sqrt(S->h_hist[0] / S->h_hist[1]) : 0.0;
+  if (c_out)
+    for (int k = 0; k < K; k++) c_out[k] = hc[k];
   return 0;
 }
 

@@ -8,8 +8,12 @@ so the source stays in the box).  Every solve runs to
 ||r_k|| <= rtol * ||r(u = 0)|| (RMS norms of the residual hook), so warm
 starts are measured against the same reference as cold solves.
 
-* warm start: u_0 = 2 u_{s-1} - u_{s-2} (second-order extrapolation of the
-  previous solutions), u_{s-1} for s = 1, 0 for s = 0;
+* warm starts: "proj" (default) -- the history projection of
+  pmg_history_warm_start: u_0 = sum_k c_k u_{s-k} over the last K solves,
+  c minimising ||f_s - sum_k c_k f_{s-k}|| (the residual of u_0 up to the
+  earlier solves' own residuals, L being linear); "extrap2" -- second-order
+  extrapolation u_0 = 2 u_{s-1} - u_{s-2} (u_{s-1} for s = 1); "zero" --
+  u_0 = 0, the cold solve;
 * training data: (residual, error) crops from plain FP64 V-cycles of the
   first solves, the error taken against a tightly converged FP64 solution;
   inputs are scaled by 1/||r||, targets by 1/(||r|| h^2) -- the scaling the
@@ -78,7 +82,7 @@ def _copy_grid(dst_ptr, src_ptr, n, stream=None):
 class Sequence:
     """Runs the drifting sequence on one Solver, keeping the last two solutions."""
 
-    def __init__(self, n=257, seed=1234, speed=0.002, torch=None):
+    def __init__(self, n=257, seed=1234, speed=0.002, torch=None, hist_k=6):
         import torch as _t
         self.torch = torch or _t
         self.n = n
@@ -87,34 +91,62 @@ class Sequence:
         self.S.set_params(pre=2, pst=2, max_citer=5000, crtol=1e-3, rop_u=3, rop_res=3, rop_src=3)
         self.Lf = self.S.nlev
         t = self.torch
-        self.hist = [t.zeros((n, n, n), dtype=t.float64, device="cuda") for _ in range(2)]
+        self.hist_k = max(2, hist_k)
+        self.hist = [t.zeros((n, n, n), dtype=t.float64, device="cuda") for _ in range(self.hist_k)]
+        self.hist_f = [t.zeros((n, n, n), dtype=t.float64, device="cuda") for _ in range(self.hist_k)]
         self.nhist = 0
         self.h = 1.0 / (n - 1)
+        self.rho_cur = None
+        self.last_misfit = None
 
-    def prepare(self, s, warm=True):
-        """Load f_s, set the initial guess and restrict the sources; returns
+    def load_source(self, s):
+        """Load f_s into the solver (finest rho) and restrict it; returns
         ||r(u = 0)|| (the reference norm of the relative test)."""
         t = self.torch
         rho = self.src.rho(self.n, s, t)
         _copy_grid(self.S.field_ptr(self.Lf, PMG_F_RHO), tptr(rho), self.n)
+        self.rho_cur = rho
         rref = float(t.sqrt(t.mean(rho[1:-1, 1:-1, 1:-1] ** 2)))
-        L = lib()
-        t.cuda.synchronize()
-        if not warm or self.nhist == 0:
-            self.S.zero_solution()
-        elif self.nhist == 1:
-            L.pmg_set_solution_lincomb(self.S.h, tptr(self.hist[0]), None, 1.0, 0.0)
-        else:
-            L.pmg_set_solution_lincomb(self.S.h, tptr(self.hist[0]), tptr(self.hist[1]), 2.0, -1.0)
         self.S.init()
         t.cuda.synchronize()
         return rref
 
+    def warm_start(self, method="proj", misfit=True):
+        """Set the initial guess on the finest level (see the module text);
+        for "proj" with misfit, last_misfit = ||f_s - sum c_k f_{s-k}|| /
+        ||f_s|| (one more pass over the sources)."""
+        L = lib()
+        self.last_misfit = None
+        if method == "zero" or self.nhist == 0:
+            self.S.zero_solution()
+        elif method == "proj":
+            K = min(self.nhist, self.hist_k)
+            self.last_c, self.last_misfit = self.S.history_warm_start(
+                [tptr(f) for f in self.hist_f[:K]], [tptr(u) for u in self.hist[:K]], misfit=misfit)
+        elif method == "extrap2" and self.nhist == 1:
+            L.pmg_set_solution_lincomb(self.S.h, tptr(self.hist[0]), None, 1.0, 0.0)
+        elif method == "extrap2":
+            L.pmg_set_solution_lincomb(self.S.h, tptr(self.hist[0]), tptr(self.hist[1]), 2.0, -1.0)
+        else:
+            raise ValueError(method)
+
+    def prepare(self, s, warm=True, method="proj"):
+        """load_source(s) and the initial guess; returns ||r(u = 0)||."""
+        rref = self.load_source(s)
+        self.warm_start(method if warm else "zero")
+        self.torch.cuda.synchronize()
+        return rref
+
     def push_solution(self):
-        """u_{s} becomes the newest history entry."""
-        self.hist[0], self.hist[1] = self.hist[1], self.hist[0]
-        _copy_grid(tptr(self.hist[0]), self.S.field_ptr(self.Lf, PMG_F_U), self.n)
-        self.nhist = min(2, self.nhist + 1)
+        """(f_s, u_s) becomes the newest history entry."""
+        t = self.torch
+        u, f = self.hist.pop(), self.hist_f.pop()
+        _copy_grid(tptr(u), self.S.field_ptr(self.Lf, PMG_F_U), self.n)
+        f.copy_(self.rho_cur)
+        t.cuda.synchronize()
+        self.hist.insert(0, u)
+        self.hist_f.insert(0, f)
+        self.nhist = min(self.hist_k, self.nhist + 1)
 
     def solve(self, mode, rtol, rref, max_iter=100):
         return self.S.solve_rel(mode, rtol, max_iter, r_ref=rref)
