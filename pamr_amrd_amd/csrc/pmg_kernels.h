@@ -773,6 +773,34 @@ struct CoarseP {
   int *citer;                    // device output: iteration count at exit
 };
 
+// q / m and q % m for 0 <= q < 2^22: a float reciprocal estimate, off by
+// at most one, corrected by one integer step (exact).  A few instructions
+// instead of the integer-division sequence, which dominates the index
+// arithmetic of the single-workgroup kernels below.
+struct FastDiv {
+  int m;
+  float inv;
+};
+__device__ __forceinline__ FastDiv make_fdiv(int m) { return FastDiv{m, 1.0f / (float)(m > 0 ? m : 1)}; }
+__device__ __forceinline__ int fdivmod(int q, const FastDiv &d, int &rem) {
+  int t = (int)((float)q * d.inv);
+  int r = q - t * d.m;
+  if (r < 0) {
+    t--;
+    r += d.m;
+  } else if (r >= d.m) {
+    t++;
+    r -= d.m;
+  }
+  rem = r;
+  return t;
+}
+// (i, j, k) of the linear index q of an mi x mj x * box, i fastest
+__device__ __forceinline__ void box_ijk(int q, const FastDiv &di, const FastDiv &dj, int &i, int &j, int &k) {
+  const int t = fdivmod(q, di, i);
+  k = fdivmod(t, dj, j);
+}
+
 template <typename T, int OP, bool EX, bool P2, bool LDS, int NT>
 __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
   extern __shared__ unsigned char smem[];
@@ -796,12 +824,15 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
   }
   const int mi = P.bx1 - P.bx0 + 1, mj = P.by1 - P.by0 + 1, mk = P.bz1 - P.bz0 + 1;
   const int npts = (mi > 0 && mj > 0 && mk > 0) ? mi * mj * mk : 0;
+  const FastDiv dmi = make_fdiv(mi), dmj = make_fdiv(mj);
   __shared__ double bres;
   auto sweep = [&]() -> double {
     double s = 0.0, cnt = 0.0;
     for (int pass = 0; pass <= 1; pass++) {
       for (int q = threadIdx.x; q < npts; q += NT) {
-        const int i = P.bx0 + q % mi, j = P.by0 + (q / mi) % mj, k = P.bz0 + q / (mi * mj);
+        int i, j, k;
+        box_ijk(q, dmi, dmj, i, j, k);
+        i += P.bx0; j += P.by0; k += P.bz0;
         if (((i + j + k + 3 + pass) & 1) != 0) continue;
         const size_t p = ix3(i, j, k, nx, ny);
         if constexpr (EX) { if (P.chr[p]) continue; }
@@ -1056,9 +1087,12 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
     const int npts = box_pts(L, mi, mj, mk);
     const size_t sy = L.nx, sz = (size_t)L.nx * L.ny;
     s = 0.0; cnt = 0.0;
+    const FastDiv dmi = make_fdiv(mi), dmj = make_fdiv(mj);
     for (int pass = 0; pass <= 1; pass++) {
       for (int q = threadIdx.x; q < npts; q += NT) {
-        const int i = L.bx0 + q % mi, j = L.by0 + (q / mi) % mj, k = L.bz0 + q / (mi * mj);
+        int i, j, k;
+        box_ijk(q, dmi, dmj, i, j, k);
+        i += L.bx0; j += L.by0; k += L.bz0;
         if (((i + j + k + 3 + pass) & 1) != 0) continue;
         const size_t p = ix3(i, j, k, L.nx, L.ny);
         T jac;
@@ -1074,8 +1108,10 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
   auto lop_rhs = [&](const BotLev<T> &L) {
     const int n = L.nx * L.ny * L.nz;
     const size_t sy = L.nx, sz = (size_t)L.nx * L.ny;
+    const FastDiv dnx = make_fdiv(L.nx), dny = make_fdiv(L.ny);
     for (int q = threadIdx.x; q < n; q += NT) {
-      const int i = q % L.nx, j = (q / L.nx) % L.ny, k = q / (L.nx * L.ny);
+      int i, j, k;
+      box_ijk(q, dnx, dny, i, j, k);
       const bool on = i >= L.cx0 && i <= L.cx1 && j >= L.cy0 && j <= L.cy1 && k >= L.cz0 && k <= L.cz1;
       const bool interior = i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2 && k >= 1 && k <= L.nz - 2;
       T lopv = T(0);
@@ -1090,8 +1126,11 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
     const int npts = box_pts(L, mi, mj, mk);
     const size_t sy = L.nx, sz = (size_t)L.nx * L.ny;
     s = 0.0; cnt = 0.0;
+    const FastDiv dmi = make_fdiv(mi), dmj = make_fdiv(mj);
     for (int q = threadIdx.x; q < npts; q += NT) {
-      const int i = L.bx0 + q % mi, j = L.by0 + (q / mi) % mj, k = L.bz0 + q / (mi * mj);
+      int i, j, k;
+      box_ijk(q, dmi, dmj, i, j, k);
+      i += L.bx0; j += L.by0; k += L.bz0;
       const size_t p = ix3(i, j, k, L.nx, L.ny);
       const T r = lop_at<T, OP, P2>(L.u, L.rho, L.a, L.c, p, sy, sz, L.sp, nullptr) - L.rhs[p];
       L.res[p] = r;
@@ -1102,8 +1141,10 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
   };
   auto restrict_to = [&](const T *f, T *cdst, const BotLev<T> &F, const BotLev<T> &C, int kind) {
     const int n = C.nx * C.ny * C.nz;
+    const FastDiv dnx = make_fdiv(C.nx), dny = make_fdiv(C.ny);
     for (int q = threadIdx.x; q < n; q += NT) {
-      const int ci = q % C.nx, cj = (q / C.nx) % C.ny, ck = q / (C.nx * C.ny);
+      int ci, cj, ck;
+      box_ijk(q, dnx, dny, ci, cj, ck);
       const int i0 = 1 + 2 * ci, j0 = 1 + 2 * cj, k0 = 1 + 2 * ck;
       T v;
       if (kind == 1) v = f[ix3(i0 - 1, j0 - 1, k0 - 1, F.nx, F.ny)];
@@ -1118,8 +1159,10 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
     __syncthreads();
     const int n = F.nx * F.ny * F.nz;
     const T c0 = T(1 - (1 * 1.0) / 2), c1 = T((1 * 1.0) / 2);
+    const FastDiv dnx = make_fdiv(F.nx), dny = make_fdiv(F.ny);
     for (int q = threadIdx.x; q < n; q += NT) {
-      const int i = q % F.nx, j = (q / F.nx) % F.ny, k = q / (F.nx * F.ny);
+      int i, j, k;
+      box_ijk(q, dnx, dny, i, j, k);
       const int ai = i >> 1, ri = i & 1, aj = j >> 1, rj = j & 1, ak = k >> 1, rk = k & 1;
       auto R = [&](int a, int b, int c) -> T { return C.res[ix3(a, b, c, C.nx, C.ny)]; };
       auto X = [&](int b, int c) -> T { return ri == 0 ? R(ai, b, c) : c0 * R(ai, b, c) + c1 * R(ai + 1, b, c); };

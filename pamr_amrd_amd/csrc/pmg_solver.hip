@@ -1140,8 +1140,20 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
   // ||r_k|| <= rtol * reference.
   if (!S || max_iter < 0) return -1;
   double r0;
-  int rc = pmg_residual(S, &r0);
-  if (rc) return rc;
+  int rc;
+  if (mode == 1 || mode == 2) {
+    // the FP64 defect of the initial u: its norm is r0 and, in FP32, it is
+    // the source of the first inner cycle
+    if ((rc = ensure_fp32(S))) return rc;
+    Level &F = S->lev[S->nlev - 1];
+    if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
+    if ((rc = fold_norms(S, NS_OUTER, 1))) return rc;
+    PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+    r0 = S->h_pinned[0];
+  } else if ((rc = pmg_residual(S, &r0))) {
+    return rc;
+  }
   const double rref = (r0_out && *r0_out > 0) ? *r0_out : r0;
   if (r0_out) *r0_out = rref;
   int it = 0;
@@ -1154,10 +1166,6 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
       it++;
     }
   } else if (mode == 1) {
-    if ((rc = ensure_fp32(S))) return rc;
-    // initial FP32 source -r0 (the residual above already gave ||r0||)
-    Level &F = S->lev[S->nlev - 1];
-    if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
     while (it < max_iter && r > rtol * r0) {
       if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
       PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost, S->stream));
@@ -1171,9 +1179,7 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
     // V-cycle, kept only when its contraction per unit time beats the
     // plain cycle's (measured); otherwise undone and retried later
     if (!S->net) return -1;
-    if ((rc = ensure_fp32(S))) return rc;
     Level &F = S->lev[S->nlev - 1];
-    if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
     const double h2 = F.hdx[0] * F.hdx[0];
     hipEvent_t e0, e1;
     PMG_HOST_CHECK(hipEventCreate(&e0));
