@@ -167,7 +167,7 @@ int pmg_set_sweep_blocking(pmg_solver *S, int on, int zchunk);
 void *pmg_stream(pmg_solver *S);
 
 /* ---- 4. learned smoother / warm-start network (no reference counterpart:
- * the north-star's learned component).  3x3x3 convolutions 1 -> 16 -> 16 -> 1,
+ * the learned part of the accelerated solve).  3x3x3 convolutions 1 -> 16 -> 16 -> 1,
  * GELU, BF16 MFMA with FP32 accumulation. ---------------------------------- */
 typedef struct pmg_net pmg_net;
 pmg_net *pmg_net_create(int hidden, unsigned long long seed);
@@ -197,7 +197,12 @@ int pmg_net_recorrect(pmg_net *N, double *u, double coef, int nx, int ny, int nz
 /* training crops: out[c] = scale*(a - b) on side^3 cubes at host origins */
 int pmg_gather_crops(const double *a, const double *b, int nx, int ny, int nz, const int *origins_host,
                      int ncrop, int side, double scale, float *out, void *stream);
-/* attach a network to a solver (pmg_solve_rel mode 2) and read its stats */
+/* attach a network to a solver (pmg_solve_rel mode 2) and read its stats.
+ * Mode 2 applies the network before an FP32 cycle and keeps the correction
+ * only when its measured contraction per unit time beats the cycle's;
+ * after a rejection it is not tried for 20 solves, a period that doubles
+ * with every further rejection (at most 1280) and returns to 20 on an
+ * acceptance or a new pmg_set_net. */
 int pmg_set_net(pmg_solver *S, pmg_net *N);
 int pmg_net_stats(pmg_solver *S, int *accepted, int *rejected, double *t_net_ms, double *t_vc_ms);
 /* finest-level solution u = ca*a + cb*b (device FP64; b may be NULL) */

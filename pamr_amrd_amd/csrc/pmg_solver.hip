@@ -144,7 +144,7 @@ struct pmg_solver {
   pmg_net *net = nullptr;
   int net_enabled = 1;
   int net_skip = 0;          // solves left before the network is tried again
-  int net_retry_period = 20;
+  int net_retry_period = 20; // doubles after every rejection (up to 1280), back to 20 on acceptance
   double t_net_ms = -1, t_vc_ms = -1, rho_vc = 0.1;
   int net_accepted = 0, net_rejected = 0;
   // history warm start: per-block Gram partials and their fold
@@ -1210,6 +1210,7 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
         const bool better = rn < r && (std::log(rn / r) / S->t_net_ms) < (std::log(S->rho_vc) / tv);
         if (better) {
           S->net_accepted++;
+          S->net_retry_period = 20;
           r = rn;
           if (hist) hist[it] = r;
           it++;
@@ -1221,6 +1222,7 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
         if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
         net_on = false;
         S->net_skip = S->net_retry_period;
+        S->net_retry_period = std::min(2 * S->net_retry_period, 1280);
       }
       PMG_HOST_CHECK(hipEventRecord(e0, S->stream));
       if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
@@ -1249,6 +1251,7 @@ int pmg_set_net(pmg_solver *S, pmg_net *N) {
   S->net = N;
   S->net_enabled = 1;
   S->net_skip = 0;
+  S->net_retry_period = 20;
   S->t_net_ms = -1;
   return 0;
 }

@@ -82,7 +82,7 @@ def _copy_grid(dst_ptr, src_ptr, n, stream=None):
 class Sequence:
     """Runs the drifting sequence on one Solver, keeping the last two solutions."""
 
-    def __init__(self, n=257, seed=1234, speed=0.002, torch=None, hist_k=6):
+    def __init__(self, n=257, seed=1234, speed=0.002, torch=None, hist_k=8):
         import torch as _t
         self.torch = torch or _t
         self.n = n
