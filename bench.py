@@ -101,12 +101,13 @@ def sequence_leg(args, pm, torch):
 
     Training (not timed): (residual, error) crops from plain FP64 V-cycles of
     solves 0 .. T-1, Adam on the GPU.  Evaluation: solves T .. T+K-1, each
-    solved in six ways from its own initial guess -- learned + FP32 (mode 2),
-    FP32 inner cycles (mode 1) and plain FP64 from the history projection warm
-    start (the last six solves), FP32 and plain FP64 from second-order
-    extrapolation of the two previous solutions, and plain FP64 from u = 0
-    (the reference's cold solve).  Timed per solve: computing and setting the
-    initial guess, and the solve to ||r|| <= rtol ||r(u=0)||."""
+    solved in seven ways from its own initial guess -- learned + FP32 (mode
+    2), FP32 inner cycles (mode 1), flexible GMRES(8) preconditioned by the
+    FP32 cycle and plain FP64 from the history projection warm start (the last
+    --seq-hist solves), FP32 and plain FP64 from second-order extrapolation of
+    the two previous solutions, and plain FP64 from u = 0 (the reference's
+    cold solve).  Timed per solve: computing and setting the initial guess,
+    and the solve to ||r|| <= rtol ||r(u=0)||."""
     from pamr_amrd_amd import sequence as SQ
     n = args.n
     T, K = args.seq_train, args.steps
@@ -125,8 +126,10 @@ def sequence_leg(args, pm, torch):
         f"loss {sum(losses[:20]) / 20:.3e} -> {sum(losses[-20:]) / 20:.3e}")
     S = seq.S
     S.set_net(net)
-    modes = (("learned_fp32", 2, "proj"), ("fp32", 1, "proj"), ("fp64_warm", 0, "proj"),
-             ("fp32_extrap2", 1, "extrap2"), ("fp64_extrap2", 0, "extrap2"), ("fp64_cold", 0, "zero"))
+    # mode -1: flexible GMRES(8) with the FP32 V-cycle as preconditioner
+    modes = (("learned_fp32", 2, "proj"), ("fp32", 1, "proj"), ("fgmres_fp32", -1, "proj"),
+             ("fp64_warm", 0, "proj"), ("fp32_extrap2", 1, "extrap2"), ("fp64_extrap2", 0, "extrap2"),
+             ("fp64_cold", 0, "zero"))
     times = {m[0]: [] for m in modes}
     iters = {m[0]: [] for m in modes}
     misfit = []
@@ -136,7 +139,10 @@ def sequence_leg(args, pm, torch):
             torch.cuda.synchronize()
             t1 = time.perf_counter()
             seq.warm_start(method, misfit=False)
-            _, h = seq.solve(mode, args.rtol, rref)
+            if mode < 0:
+                _, h = seq.S.solve_fgmres(8, args.rtol, 100, r_ref=rref)
+            else:
+                _, h = seq.solve(mode, args.rtol, rref)
             torch.cuda.synchronize()
             dt = 1e3 * (time.perf_counter() - t1)
             # no iteration: the initial guess already met the test
@@ -157,6 +163,7 @@ def sequence_leg(args, pm, torch):
         "speedup_learned_fp32_vs_fp64_cold": round(mean["fp64_cold"] / mean["learned_fp32"], 3),
         "speedup_learned_fp32_vs_fp64_warm": round(mean["fp64_warm"] / mean["learned_fp32"], 3),
         "speedup_learned_fp32_vs_fp64_extrap2": round(mean["fp64_extrap2"] / mean["learned_fp32"], 3),
+        "speedup_fgmres_fp32_vs_fp64_cold": round(mean["fp64_cold"] / mean["fgmres_fp32"], 3),
         "warm_start": {"method": f"history projection over the last {seq.hist_k} solves",
                        "mean_relative_misfit": float(f"{sum(misfit) / len(misfit):.4e}"),
                        "max_relative_misfit": float(f"{max(misfit):.4e}")},
@@ -227,9 +234,10 @@ def excised_leg(args, pm, torch):
     """Large excised problem: div(a grad u) + c u on n^3 (default 513^3), a
     lognormal, c uniform in [0,1], 3 seeded excised spheres (chr on every
     level, Dirichlet data g held on the excised points), 16 seeded Gaussian
-    right-hand sides, each solved to ||r|| <= rtol ||r(u0)|| with FP32 inner
-    V-cycles in FP64 defect correction; one right-hand side also in plain
-    FP64 for comparison."""
+    right-hand sides, each solved to ||r|| <= rtol ||r(u0)|| by flexible
+    GMRES(8) with the FP32 V-cycle as preconditioner; the first right-hand
+    sides also by FP32 inner V-cycles in FP64 defect correction, the first in
+    plain FP64 V-cycles (capped) for comparison."""
     import numpy as np
     from pamr_amrd_amd import hierarchy as H
     from pamr_amrd_amd import sequence as SQ
@@ -255,21 +263,30 @@ def excised_leg(args, pm, torch):
     del ch
     L = pm.lib()
     times, iters, hists = [], [], []
+    gtimes, giters = [], []
     plain = None
     for r in range(args.c5_rhs):
         src = SQ.DriftingSource(seed=1234 + r, speed=0.0)
         rho = src.rho(n, 0, torch)
         SQ._copy_grid(S.field_ptr(Lf, pm.PMG_F_RHO), pm.tptr(rho), n)
         del rho
-        for mode in ((1, 0) if r == 0 else (1,)):
+        for mode in ((-1, 1, 0) if r == 0 else (-1, 1) if r < args.c5_mixed_rhs else (-1,)):
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             L.pmg_set_solution_lincomb(S.h, pm.tptr(u0), None, 1.0, 0.0)
             S.init()
-            r0, h = S.solve_rel(mode, args.rtol, args.c5_max_iter if mode == 1 else args.c5_plain_cycles)
+            if mode < 0:
+                r0, h = S.solve_fgmres(args.c5_restart, args.rtol, args.c5_max_iter)
+            else:
+                r0, h = S.solve_rel(mode, args.rtol, args.c5_max_iter if mode == 1 else args.c5_plain_cycles)
             torch.cuda.synchronize()
             dt = 1e3 * (time.perf_counter() - t0)
-            if mode == 1:
+            if mode < 0:
+                if len(h) == 0 or h[-1] > args.rtol * r0:
+                    raise RuntimeError(f"excised rhs {r} (fgmres) did not converge in {len(h)} steps: {h[-3:]}")
+                gtimes.append(dt)
+                giters.append(len(h))
+            elif mode == 1:
                 if len(h) == 0 or h[-1] > args.rtol * r0:
                     raise RuntimeError(f"excised rhs {r} did not converge in {len(h)} iterations: {h[-3:]}")
                 times.append(dt)
@@ -283,8 +300,11 @@ def excised_leg(args, pm, torch):
                          "mean_contraction": float((h[-1] / r0) ** (1.0 / len(h)))}
     S.close()
     return {"workload": f"excised{n}_3spheres_varcoef_16rhs", "n": n, "excised_points": excised,
-            "rhs": args.c5_rhs, "mixed_ms_per_rhs": round(sum(times) / len(times), 2),
-            "mixed_total_ms": round(sum(times), 1), "mixed_outer_iters": iters,
+            "rhs": args.c5_rhs,
+            "fgmres_ms_per_rhs": round(sum(gtimes) / len(gtimes), 2), "fgmres_total_ms": round(sum(gtimes), 1),
+            "fgmres_restart": args.c5_restart, "fgmres_preconditioner_applications": giters,
+            "mixed_rhs_timed": len(times),
+            "mixed_ms_per_rhs": round(sum(times) / len(times), 2), "mixed_outer_iters": iters,
             "mixed_mean_contraction": None if not hists else float(np.mean(hists)),
             "plain_fp64_first_rhs": plain, "rtol": args.rtol, "reference_norm": "||r(u0)||"}
 
@@ -332,6 +352,8 @@ def main():
     ap.add_argument("--c5-rhs", type=int, default=16)
     ap.add_argument("--c5-plain-cycles", type=int, default=60)
     ap.add_argument("--c5-max-iter", type=int, default=300)
+    ap.add_argument("--c5-restart", type=int, default=8, help="FGMRES restart length of the excised leg")
+    ap.add_argument("--c5-mixed-rhs", type=int, default=2, help="right-hand sides also solved by defect correction")
     ap.add_argument("--probe-kernels", action="store_true",
                     help="only launch the finest-level kernels (for rocprofv3 --pmc runs): "
                          "FP64 sweep, FP64 residual, FP32 sweep and a plain 8 B/point copy "

@@ -139,6 +139,15 @@ int pmg_solve(pmg_solver *S, int min_iter, int max_iter, double tol, double *nor
  * the reference norm used. */
 int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hist, int *iters,
                   double *r0_out);
+/* Relative solve as pmg_solve_rel with restarted flexible GMRES(m)
+ * (1 <= m <= 15; no counterpart in the reference): the operator is the
+ * finest level's FP64 residual hook with a zero source, the right
+ * preconditioner one FP32 V-cycle from zero (the mixed mode's inner cycle).
+ * hist[k]: the residual estimate after preconditioner application k+1, the
+ * FP64 residual at the end of each restart; iters counts preconditioner
+ * applications.  Holds 2m + 3 finest-level FP64 vectors. */
+int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hist, int *iters,
+                     double *r0_out);
 int pmg_last_citer(pmg_solver *S, int which);
 /* initial guess u = 0 on the finest level (device-side) */
 int pmg_zero_solution(pmg_solver *S);

@@ -103,6 +103,7 @@ def _declare(L):
     L.pmg_set_net.argtypes = [_vp, _vp]
     L.pmg_net_stats.argtypes = [_vp, _ip, _ip, _dp, _dp]
     L.pmg_set_solution_lincomb.argtypes = [_vp, _vp, _vp, d, d]
+    L.pmg_solve_fgmres.argtypes = [_vp, i, d, i, _dp, _ip, _dp]
     L.pmg_history_warm_start.argtypes = [_vp, i, C.POINTER(_vp), C.POINTER(_vp), _dp, _dp]
     # batched AMR hierarchy (include/pamr_amr_gpu.h)
     L.pmg_amr_create.argtypes = [_ip, _dp, i, _ip, _dp, i, i]
@@ -326,6 +327,15 @@ class Solver:
 
     def zero_solution(self):
         _check(self._L.pmg_zero_solution(self.h), "pmg_zero_solution")
+
+    def solve_fgmres(self, m, rtol, max_iter, r_ref=0.0):
+        """(reference norm, residual history) of pmg_solve_fgmres."""
+        hist = np.zeros(max(max_iter, 1))
+        it = C.c_int(0)
+        r0 = C.c_double(r_ref)
+        _check(self._L.pmg_solve_fgmres(self.h, m, rtol, max_iter, P(hist), C.byref(it), C.byref(r0)),
+               "pmg_solve_fgmres")
+        return r0.value, hist[: it.value]
 
     def history_warm_start(self, F, U, misfit=True):
         """u = sum c_k U[k] minimising ||rho - sum c_k F[k]|| (device

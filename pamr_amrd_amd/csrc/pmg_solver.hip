@@ -150,6 +150,12 @@ struct pmg_solver {
   // history warm start: per-block Gram partials and their fold
   double *d_hist = nullptr;
   double *h_hist = nullptr;
+  // flexible GMRES: V (m+1 vectors), Z (m), w, a zero source, reductions
+  int kry_m = 0;
+  double *kry = nullptr;
+  double *kry_red = nullptr;
+  double *kry_host = nullptr;
+  hipGraphExec_t gprec = nullptr;  // captured FP32 V-cycle (the preconditioner)
 };
 
 // norm slot layout in d_norms
@@ -787,6 +793,7 @@ int ensure_fp32(pmg_solver *S) {
 void invalidate_graphs(pmg_solver *S) {
   if (S->g64) { (void)hipGraphExecDestroy(S->g64); S->g64 = nullptr; }
   if (S->g32) { (void)hipGraphExecDestroy(S->g32); S->g32 = nullptr; }
+  if (S->gprec) { (void)hipGraphExecDestroy(S->gprec); S->gprec = nullptr; }
 }
 
 }  // namespace
@@ -904,6 +911,9 @@ void pmg_destroy(pmg_solver *S) {
   if (S->h_pinned) (void)hipHostFree(S->h_pinned);
   if (S->d_hist) (void)hipFree(S->d_hist);
   if (S->h_hist) (void)hipHostFree(S->h_hist);
+  if (S->kry) (void)hipFree(S->kry);
+  if (S->kry_red) (void)hipFree(S->kry_red);
+  if (S->kry_host) (void)hipHostFree(S->kry_host);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
 }
@@ -1360,13 +1370,15 @@ __global__ __launch_bounds__(256) void k_hist_gram(HistPtrs F, const double *rho
   }
 }
 
-// block-order fold of the partials: out[m] = sum_b partials[b][m]
-__global__ __launch_bounds__(64) void k_hist_fold(const double *partials, int nb, int M, double *out) {
-  const int m = threadIdx.x;
-  if (m >= M) return;
-  double v = 0.0;
-  for (int b = 0; b < nb; b++) v += partials[(size_t)b * M + m];
-  out[m] = v;
+// fold of per-block partials: out[m] = sum_b partials[b][m], one workgroup
+// per m; thread t sums b = t, t + 256, ... and the workgroup adds the 256
+// sums in a fixed tree (deterministic for fixed nb)
+__global__ __launch_bounds__(256) void k_hist_fold(const double *partials, int nb, int M, double *out) {
+  const int m = blockIdx.x;
+  double v = 0.0, unused = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) v += partials[(size_t)b * M + m];
+  block_reduce2<256>(v, unused);
+  if (threadIdx.x == 0) out[m] = v;
 }
 
 // per block: sum (rho - sum_k c_k F_k)^2 and sum rho^2
@@ -1449,12 +1461,12 @@ int hist_launch(pmg_solver *S, const HistPtrs &F, const HistPtrs &U, double *dc,
     hipLaunchKernelGGL(k_hist_misfit<K>, dim3(kHistBlocks), dim3(256), 0, S->stream, F, Lf.d.rho, dc, Lf.size,
                        S->d_hist);
     PMG_HOST_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_hist_fold, dim3(1), dim3(64), 0, S->stream, S->d_hist, kHistBlocks, 2,
+    hipLaunchKernelGGL(k_hist_fold, dim3(2), dim3(256), 0, S->stream, S->d_hist, kHistBlocks, 2,
                        S->d_hist + (size_t)kHistBlocks * kHistM);
   } else if (what == HIST_GRAM) {
     hipLaunchKernelGGL(k_hist_gram<K>, dim3(kHistBlocks), dim3(256), 0, S->stream, F, Lf.d.rho, Lf.size, S->d_hist);
     PMG_HOST_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(k_hist_fold, dim3(1), dim3(64), 0, S->stream, S->d_hist, kHistBlocks, M,
+    hipLaunchKernelGGL(k_hist_fold, dim3(M), dim3(256), 0, S->stream, S->d_hist, kHistBlocks, M,
                        S->d_hist + (size_t)kHistBlocks * kHistM);
   } else {
     hipLaunchKernelGGL(k_hist_lincomb<K>, dim3(nblk256(Lf.size)), dim3(256), 0, S->stream, Lf.d.u, Lf.d.ua, U, dc,
@@ -1523,6 +1535,356 @@ int pmg_history_warm_start(pmg_solver *S, int K, const double *const *F, const d
   if (rel_out) *rel_out = S->h_hist[1] > 0 ? std::sqrt(S->h_hist[0] / S->h_hist[1]) : 0.0;
   if (c_out)
     for (int k = 0; k < K; k++) c_out[k] = hc[k];
+  return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Flexible GMRES(m) on the finest level, right-preconditioned by one FP32
+// V-cycle from zero (Y. Saad, "A flexible inner-outer preconditioned GMRES
+// algorithm", SIAM J. Sci. Comput. 14 (1993) 461-469).  The operator is the
+// linear part of the FP64 residual hook: A v = R(v) with a zero source (the
+// residual kernel with rho = 0; excised and boundary points give 0, so every
+// Krylov vector stays on the unknowns).  The preconditioner is the mixed
+// mode's inner cycle: writing v as the FP32 source and cycling from zero
+// gives z ~ A^{-1} v.  Orthogonalisation: classical Gram-Schmidt applied
+// twice, each pass one fused kernel over all basis vectors (dots, then the
+// update with the new norm).  All reductions are per block in a fixed grid
+// and folded in block order: deterministic.
+// ---------------------------------------------------------------------------
+constexpr int kKryMax = 16;
+constexpr int kKryBlocks = 2048;
+
+struct VecPtrs {
+  const double *p[kKryMax + 1];
+};
+
+// partials[b][i] = sum_q w[q] V_i[q], i < nv
+__global__ __launch_bounds__(256) void k_kry_dots(const double *w, VecPtrs V, int nv, size_t n, double *partials) {
+  double acc[kKryMax + 1];
+#pragma unroll
+  for (int i = 0; i <= kKryMax; i++) acc[i] = 0.0;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+    const double wv = w[q];
+#pragma unroll
+    for (int i = 0; i <= kKryMax; i++)
+      if (i < nv) acc[i] += wv * V.p[i][q];
+  }
+  __shared__ double red[256 / kWave][kKryMax + 1];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i <= kKryMax; i++) {
+    double v = acc[i];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    if (lane == 0) red[wid][i] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < nv; i += 256) {
+    double v = 0.0;
+    for (int w2 = 0; w2 < 256 / kWave; w2++) v += red[w2][i];
+    partials[(size_t)blockIdx.x * nv + i] = v;
+  }
+}
+
+// w -= sum_i c_i V_i (c on the device); partials[b] = sum of the new w^2
+__global__ __launch_bounds__(256) void k_kry_update(double *w, VecPtrs V, int nv, const double *c, size_t n,
+                                                    double *partials) {
+  double cc[kKryMax + 1];
+#pragma unroll
+  for (int i = 0; i <= kKryMax; i++) cc[i] = i < nv ? c[i] : 0.0;
+  double s = 0.0;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+    double v = w[q];
+#pragma unroll
+    for (int i = 0; i <= kKryMax; i++)
+      if (i < nv) v -= cc[i] * V.p[i][q];
+    w[q] = v;
+    s += v * v;
+  }
+  double cnt = 0.0;
+  block_reduce2<256>(s, cnt);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// w -= sum_i c_i V_i, and partials[b][i] = sum_q V_i[q] w_new[q] (i < nv),
+// partials[b][nv] = sum w_new^2: the first Gram-Schmidt update fused with the
+// dot products of the second pass
+__global__ __launch_bounds__(256) void k_kry_update_dots(double *w, VecPtrs V, int nv, const double *c, size_t n,
+                                                         double *partials) {
+  double cc[kKryMax + 1], acc[kKryMax + 2];
+#pragma unroll
+  for (int i = 0; i <= kKryMax; i++) cc[i] = i < nv ? c[i] : 0.0;
+#pragma unroll
+  for (int i = 0; i <= kKryMax + 1; i++) acc[i] = 0.0;
+  for (size_t q = (size_t)blockIdx.x * 256 + threadIdx.x; q < n; q += (size_t)gridDim.x * 256) {
+    double vi[kKryMax + 1];
+    double v = w[q];
+#pragma unroll
+    for (int i = 0; i <= kKryMax; i++) {
+      vi[i] = i < nv ? V.p[i][q] : 0.0;
+      v -= cc[i] * vi[i];
+    }
+    w[q] = v;
+#pragma unroll
+    for (int i = 0; i <= kKryMax; i++) acc[i] += vi[i] * v;
+    acc[kKryMax + 1] += v * v;
+  }
+  __shared__ double red[256 / kWave][kKryMax + 2];
+  const int lane = threadIdx.x % kWave, wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int i = 0; i <= kKryMax + 1; i++) {
+    double v = acc[i];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    if (lane == 0) red[wid][i] = v;
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i <= nv; i += 256) {
+    const int src = i < nv ? i : kKryMax + 1;
+    double v = 0.0;
+    for (int w2 = 0; w2 < 256 / kWave; w2++) v += red[w2][src];
+    partials[(size_t)blockIdx.x * (nv + 1) + i] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_kry_scale(double *dst, const double *src, double f, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[q] = f * src[q];
+}
+
+// preconditioner input: FP32 source = v, FP32 solution = 0
+__global__ __launch_bounds__(256) void k_kry_prep(float *rho32, float *u32, const double *v, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) {
+    rho32[q] = (float)v[q];
+    u32[q] = 0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_kry_to64(double *dst, const float *src, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[q] = (double)src[q];
+}
+
+// u (both sweep buffers) += sum_j y_j Z_j
+__global__ __launch_bounds__(256) void k_kry_xupdate(double *u, double *ua, VecPtrs Z, int nz, const double *y,
+                                                     size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  double v = u[q];
+  for (int j = 0; j < nz; j++) v += y[j] * Z.p[j][q];
+  u[q] = v;
+  ua[q] = v;
+}
+
+int enq_prec(pmg_solver *S) {
+  return enqueue_vcycle<float>(S, false, NS_F32, S->d_norms + NS_COARSE + 2, S->d_citer + 1);
+}
+
+// FP64 residual hook of an arbitrary u with source rho into res (finest level)
+int resid_vec(pmg_solver *S, const double *u, const double *rho, double *res, int slot) {
+  Level &Lv = S->lev[S->nlev - 1];
+  LevelArrays<double> &A = arrays<double>(Lv);
+  ResidP<double> P;
+  P.u = u; P.rho = rho; P.rhs = A.rhs; P.a = A.a; P.c = A.c; P.chr = Lv.chr;
+  P.res = res;
+  P.neg32 = nullptr;
+  P.nx = Lv.n[0]; P.ny = Lv.n[1]; P.nz = Lv.n[2];
+  if (Lv.rbox.ok) {
+    P.bx0 = Lv.rbox.lo[0]; P.bx1 = Lv.rbox.hi[0];
+    P.by0 = Lv.rbox.lo[1]; P.by1 = Lv.rbox.hi[1];
+    P.bz0 = Lv.rbox.lo[2]; P.bz1 = Lv.rbox.hi[2];
+  } else {
+    P.bx0 = P.by0 = P.bz0 = 1;
+    P.bx1 = P.by1 = P.bz1 = 0;
+  }
+  P.sp = spacing<double>(Lv);
+  P.ws = S->ws;
+  P.slot = slot;
+  const bool p2 = P.sp.pow2 != 0;
+#define PMG_RV(OPV, EXV) \
+  (p2 ? launch_residual_t<double, OPV, EXV, true>(S, P, true, true) : launch_residual_t<double, OPV, EXV, false>(S, P, true, true))
+  if (S->op == 0) return S->do_ex ? PMG_RV(0, true) : PMG_RV(0, false);
+  return S->do_ex ? PMG_RV(1, true) : PMG_RV(1, false);
+#undef PMG_RV
+}
+
+int kry_fold(pmg_solver *S, int M, double *host) {
+  double *fold = S->kry_red + (size_t)kKryBlocks * (kKryMax + 1);
+  hipLaunchKernelGGL(k_hist_fold, dim3(M), dim3(256), 0, S->stream, S->kry_red, kKryBlocks, M, fold);
+  PMG_HOST_CHECK(hipGetLastError());
+  PMG_HOST_CHECK(hipMemcpyAsync(host, fold, sizeof(double) * M, hipMemcpyDeviceToHost, S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
+}
+
+int kry_dots(pmg_solver *S, const double *w, const VecPtrs &V, int nv, double *host) {
+  Level &F = S->lev[S->nlev - 1];
+  hipLaunchKernelGGL(k_kry_dots, dim3(kKryBlocks), dim3(256), 0, S->stream, w, V, nv, F.size, S->kry_red);
+  PMG_HOST_CHECK(hipGetLastError());
+  return kry_fold(S, nv, host);
+}
+
+int kry_update(pmg_solver *S, double *w, const VecPtrs &V, int nv, const double *c_host, double *norm2) {
+  Level &F = S->lev[S->nlev - 1];
+  double *dc = S->kry_red + (size_t)kKryBlocks * (kKryMax + 1) + (kKryMax + 1);
+  double *hc = S->kry_host + 2 * (kKryMax + 1);
+  for (int i = 0; i < nv; i++) hc[i] = c_host[i];
+  PMG_HOST_CHECK(hipMemcpyAsync(dc, hc, sizeof(double) * nv, hipMemcpyHostToDevice, S->stream));
+  hipLaunchKernelGGL(k_kry_update, dim3(kKryBlocks), dim3(256), 0, S->stream, w, V, nv, dc, F.size, S->kry_red);
+  PMG_HOST_CHECK(hipGetLastError());
+  return kry_fold(S, 1, norm2);
+}
+
+// w -= V c_host; h_out[0..nv-1] = V^T w_new, *norm2 = w_new . w_new
+int kry_update_dots(pmg_solver *S, double *w, const VecPtrs &V, int nv, const double *c_host, double *h_out,
+                    double *norm2) {
+  Level &F = S->lev[S->nlev - 1];
+  double *dc = S->kry_red + (size_t)kKryBlocks * (kKryMax + 1) + (kKryMax + 1);
+  double *hc = S->kry_host + 2 * (kKryMax + 1);
+  for (int i = 0; i < nv; i++) hc[i] = c_host[i];
+  PMG_HOST_CHECK(hipMemcpyAsync(dc, hc, sizeof(double) * nv, hipMemcpyHostToDevice, S->stream));
+  hipLaunchKernelGGL(k_kry_update_dots, dim3(kKryBlocks), dim3(256), 0, S->stream, w, V, nv, dc, F.size,
+                     S->kry_red);
+  PMG_HOST_CHECK(hipGetLastError());
+  double tmp[kKryMax + 2];
+  int rc = kry_fold(S, nv + 1, tmp);
+  if (rc) return rc;
+  for (int i = 0; i < nv; i++) h_out[i] = tmp[i];
+  *norm2 = tmp[nv];
+  return 0;
+}
+
+int ensure_krylov(pmg_solver *S, int m) {
+  if (S->kry && S->kry_m >= m) return 0;
+  if (S->kry) { PMG_HOST_CHECK(hipStreamSynchronize(S->stream)); (void)hipFree(S->kry); S->kry = nullptr; }
+  const size_t n = S->lev[S->nlev - 1].size;
+  const size_t nvec = (size_t)(2 * m + 3);
+  PMG_HOST_CHECK(hipMalloc(&S->kry, nvec * n * sizeof(double)));
+  PMG_HOST_CHECK(hipMemsetAsync(S->kry, 0, nvec * n * sizeof(double), S->stream));
+  if (!S->kry_red) {
+    PMG_HOST_CHECK(hipMalloc(&S->kry_red, sizeof(double) * ((size_t)kKryBlocks * (kKryMax + 1) + 4 * (kKryMax + 1))));
+    PMG_HOST_CHECK(hipHostMalloc(&S->kry_host, sizeof(double) * 4 * (kKryMax + 1), hipHostMallocDefault));
+  }
+  S->kry_m = m;
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hist, int *iters, double *r0_out) {
+  if (!S || m < 1 || m > kKryMax - 1 || max_iter < 0) return -1;
+  int rc;
+  if ((rc = ensure_fp32(S))) return rc;
+  if ((rc = ensure_krylov(S, m))) return rc;
+  Level &F = S->lev[S->nlev - 1];
+  const size_t n = F.size;
+  const unsigned nb = nblk256(n);
+  double *V[kKryMax + 1], *Z[kKryMax];
+  for (int i = 0; i <= m; i++) V[i] = S->kry + (size_t)i * n;
+  for (int i = 0; i < m; i++) Z[i] = S->kry + (size_t)(m + 1 + i) * n;
+  double *w = S->kry + (size_t)(2 * m + 1) * n;
+  const double *zero = S->kry + (size_t)(2 * m + 2) * n;
+  double *hh = S->kry_host;  // [0, kKryMax]: dots / norms
+  auto true_residual = [&](double *rms) -> int {
+    int r;
+    if ((r = resid_vec(S, ucur<double>(F), F.d.rho, w, NS_R0))) return r;
+    if ((r = fold_norms(S, NS_R0, 1))) return r;
+    PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_R0, sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+    *rms = S->h_pinned[0];
+    return 0;
+  };
+  double r_rms;
+  if ((rc = true_residual(&r_rms))) return rc;
+  const double rref = (r0_out && *r0_out > 0) ? *r0_out : r_rms;
+  if (r0_out) *r0_out = rref;
+  int it = 0;
+  std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  while (it < max_iter && r_rms > rtol * rref) {
+    // b = -R(u) (in w): 2-norm, and the RMS-to-2-norm factor of this grid
+    VecPtrs Pw{};
+    Pw.p[0] = w;
+    if ((rc = kry_dots(S, w, Pw, 1, hh))) return rc;
+    const double beta = std::sqrt(hh[0]);
+    if (!(beta > 0)) break;
+    const double scale = beta / r_rms;
+    hipLaunchKernelGGL(k_kry_scale, dim3(nb), dim3(256), 0, S->stream, V[0], w, -1.0 / beta, n);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    const double target = rtol * rref * scale;
+    int k = 0;
+    for (int j = 0; j < m && it < max_iter; j++) {
+      // Z_j = M V_j: the FP32 V-cycle from zero with V_j as its source
+      hipLaunchKernelGGL(k_kry_prep, dim3(nb), dim3(256), 0, S->stream, F.f.rho, F.f.u, V[j], n);
+      PMG_HOST_CHECK(hipGetLastError());
+      if ((rc = capture_or_run(S, &S->gprec, enq_prec))) return rc;
+      hipLaunchKernelGGL(k_kry_to64, dim3(nb), dim3(256), 0, S->stream, Z[j], F.f.u, n);
+      PMG_HOST_CHECK(hipGetLastError());
+      // V_{j+1} = A Z_j, orthogonalised against V_0..V_j by classical
+      // Gram-Schmidt, repeated when the first pass removed more than 1 - 1/2
+      // of w's squared norm (the criterion of Daniel, Gragg, Kaufman and
+      // Stewart, Math. Comp. 30 (1976) 772-795)
+      if ((rc = resid_vec(S, Z[j], zero, V[j + 1], NS_OUTER + 1))) return rc;
+      VecPtrs PV{};
+      for (int i = 0; i <= j; i++) PV.p[i] = V[i];
+      PV.p[j + 1] = V[j + 1];
+      double h1[kKryMax + 2], h2[kKryMax + 1], nrm2 = 0.0;
+      if ((rc = kry_dots(S, V[j + 1], PV, j + 2, h1))) return rc;
+      const double before2 = h1[j + 1];
+      PV.p[j + 1] = nullptr;
+      if ((rc = kry_update_dots(S, V[j + 1], PV, j + 1, h1, h2, &nrm2))) return rc;
+      const bool again = nrm2 < 0.5 * before2;
+      if (again && (rc = kry_update(S, V[j + 1], PV, j + 1, h2, &nrm2))) return rc;
+      double col[kKryMax + 1];
+      for (int i = 0; i <= j; i++) col[i] = h1[i] + (again ? h2[i] : 0.0);
+      col[j + 1] = std::sqrt(nrm2);
+      if (col[j + 1] > 0)
+        hipLaunchKernelGGL(k_kry_scale, dim3(nb), dim3(256), 0, S->stream, V[j + 1], V[j + 1], 1.0 / col[j + 1], n);
+      // Givens rotations
+      for (int i = 0; i < j; i++) {
+        const double t = cs[i] * col[i] + sn[i] * col[i + 1];
+        col[i + 1] = -sn[i] * col[i] + cs[i] * col[i + 1];
+        col[i] = t;
+      }
+      const double den = std::hypot(col[j], col[j + 1]);
+      cs[j] = den > 0 ? col[j] / den : 1.0;
+      sn[j] = den > 0 ? col[j + 1] / den : 0.0;
+      col[j] = den;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = col[i];
+      k = j + 1;
+      if (hist) hist[it] = std::fabs(g[j + 1]) / scale;
+      it++;
+      if (std::fabs(g[j + 1]) <= target || !(col[j] > 0) || nrm2 == 0.0) break;
+    }
+    // y = H^{-1} g, u += Z y
+    for (int i = k - 1; i >= 0; i--) {
+      double v = g[i];
+      for (int l = i + 1; l < k; l++) v -= H[(size_t)i * m + l] * y[l];
+      y[i] = H[(size_t)i * m + i] != 0 ? v / H[(size_t)i * m + i] : 0.0;
+    }
+    VecPtrs PZ{};
+    for (int i = 0; i < k; i++) {
+      PZ.p[i] = Z[i];
+      hh[kKryMax + 1 + i] = y[i];
+    }
+    double *dy = S->kry_red + (size_t)kKryBlocks * (kKryMax + 1) + 2 * (kKryMax + 1);
+    PMG_HOST_CHECK(hipMemcpyAsync(dy, hh + kKryMax + 1, sizeof(double) * k, hipMemcpyHostToDevice, S->stream));
+    hipLaunchKernelGGL(k_kry_xupdate, dim3(nb), dim3(256), 0, S->stream, F.d.u, F.d.ua, PZ, k, dy, n);
+    PMG_HOST_CHECK(hipGetLastError());
+    if ((rc = true_residual(&r_rms))) return rc;
+    if (hist && it > 0) hist[it - 1] = r_rms;
+  }
+  if (iters) *iters = it;
   return 0;
 }
 
