@@ -152,8 +152,9 @@ def restrict(u_f, op=3, chr_f=None, ex=1.0):
     return out
 
 
-def interp(u_c, chr_c=None, ex=1.0):
-    """Whole-grid trilinear prolongation coarse->fine via or_dminterp3d."""
+def interp(u_c, chr_c=None, ex=1.0, ord=2):
+    """Whole-grid prolongation coarse->fine via or_dminterp3d: trilinear
+    (ord 2) or cubic (ord 4, no excision)."""
     cz, cy, cx = u_c.shape
     nx, ny, nz = 2 * cx - 1, 2 * cy - 1, 2 * cz - 1
     out = np.zeros((nz, ny, nx))
@@ -161,8 +162,17 @@ def interp(u_c, chr_c=None, ex=1.0):
     do_ex = 0 if chr_c is None else 1
     ch = src if chr_c is None else chr_c
     lib().or_dminterp3d(P(src), P(out), cx, cy, cz, nx, ny, nz, 1, 1, 1, 1, 1, 1, cx, cy, cz,
-                        1, 1, 1, 2, 2, 2, 2, P(ch), ex, do_ex)
+                        1, 1, 1, 2, 2, 2, ord, P(ch), ex, do_ex)
     return out
+
+
+def dminterp3d(u1, u2, args, chr1=None, ex=1.0, do_ex=0):
+    """or_dminterp3d with the reference's argument list (args: the 22 ints
+    nx1..nz2, i1..k2, ni..nk, si1..sk1, irho..krho, ord); u2 is updated in
+    place and returned."""
+    ch = u1 if chr1 is None else chr1
+    lib().or_dminterp3d(P(u1), P(u2), *[int(a) for a in args], P(ch), ex, do_ex)
+    return u2
 
 
 class UniformMG:

@@ -10,7 +10,8 @@
  *   src/m_util_r8.f:332-360  dvi2q1d             -> or_dvi2q1d
  *   src/m_util_r8.f:368-421  dvi2q1d_ex          -> or_dvi2q1d_ex
  *   src/m_util_r8.f:426-456  init_chrt/and_chrt  -> chrt_init/chrt_and
- *   src/m_util_r8.f:473-637  dminterp3d (ord 2)  -> or_dminterp3d
+ *   src/m_util_r8.f:65-137   dvi4q1d             -> or_dvi4q1d
+ *   src/m_util_r8.f:473-637  dminterp3d (ord 2; ord 4 without excision) -> or_dminterp3d
  *   src/m_util_r8.f:649-723  dmhwr3d             -> or_dmhwr3d
  *   src/m_util_r8.f:733-949  dmfwr3d(_ex)        -> or_dmfwr3d
  *   w/examples/nps/num_w.f:18-157 lop/residual/relax (plain Poisson hooks,
@@ -87,6 +88,66 @@ void or_dvi2q1d(double *v1, double *v2, int i1, int i2, int n, int irho,
   }
 }
 
+/* src/m_util_r8.f:65-137: cubic (4-point Lagrange) irho:1 interpolation;
+ * n1 is the length of v1's array (the right-boundary test), n < 4 falls
+ * back to dvi2q1d.  Interior cells take the anchors i-1..i+2 (also outside
+ * the segment when they exist), the first / last cell the four anchors
+ * inside when no anchor lies beyond. */
+void or_dvi4q1d(double *v1, double *v2, int i1, int i2, int n1, int n, int irho, int s1, int s2)
+{
+  int i, j, is, ie;
+  double c0, c1, c2, c3, sg;
+  /* the reference's literals are 0.1666 6666 6666 6667 d0: fixed-form blanks
+   * are not significant, so 16 digits (a different double than 1/6's) */
+  const double half = 0.5, mhalf = -0.5, sixth = 0.1666666666666667, msixth = -0.1666666666666667;
+#define V1(q) v1[(q)-1]
+#define V2(q) v2[(q)-1]
+  if (n < 4) {
+    or_dvi2q1d(v1, v2, i1, i2, n, irho, s1, s2);
+    return;
+  }
+  for (i = 1; i <= n; i++) V2(irho * (i - 1) * s2 + i2) = V1((i - 1) * s1 + i1);
+  if (i1 <= s1) {
+    for (j = 1; j <= irho - 1; j++) {
+      sg = (j * 1.0) / irho;
+      c0 = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+      c1 = half * sg * (sg - 2.0) * (sg - 3.0);
+      c2 = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+      c3 = sixth * sg * (sg - 1.0) * (sg - 2.0);
+      V2(j * s2 + i2) = c0 * V1(i1) + c1 * V1(s1 + i1) + c2 * V1(2 * s1 + i1) + c3 * V1(3 * s1 + i1);
+    }
+    is = 1;
+  } else {
+    is = 0;
+  }
+  if ((i1 + n * s1) > (n1 - 2 * s1)) {
+    ie = n - 3;
+    for (j = 1; j <= irho - 1; j++) {
+      sg = 2.0 + (j * 1.0) / irho;
+      c0 = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+      c1 = half * sg * (sg - 2.0) * (sg - 3.0);
+      c2 = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+      c3 = sixth * sg * (sg - 1.0) * (sg - 2.0);
+      V2((irho * (n - 2) + j) * s2 + i2) = c0 * V1((n - 4) * s1 + i1) + c1 * V1((n - 3) * s1 + i1) +
+                                          c2 * V1((n - 2) * s1 + i1) + c3 * V1((n - 1) * s1 + i1);
+    }
+  } else {
+    ie = n - 2;
+  }
+  for (j = 1; j <= irho - 1; j++) {
+    sg = 1.0 + (j * 1.0) / irho;
+    c0 = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+    c1 = half * sg * (sg - 2.0) * (sg - 3.0);
+    c2 = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+    c3 = sixth * sg * (sg - 1.0) * (sg - 2.0);
+    for (i = is; i <= ie; i++)
+      V2((irho * i + j) * s2 + i2) = c0 * V1((i - 1) * s1 + i1) + c1 * V1(i * s1 + i1) + c2 * V1((i + 1) * s1 + i1) +
+                                     c3 * V1((i + 2) * s1 + i1);
+  }
+#undef V1
+#undef V2
+}
+
 /* src/m_util_r8.f:368-421 (extrapolates excised anchors of v1, in place) */
 void or_dvi2q1d_ex(double *v1, double *v2, int i1, int i2, int n1, int n,
                    int irho, int s1, int s2, const double *chr1, int i3,
@@ -146,7 +207,8 @@ static void chrt_and(double *chrt, const double *chr1, int n, int s3, double ex)
   }
 }
 
-/* src/m_util_r8.f:473-637, ord == 2 only (dvi4q1d is not on this path) */
+/* src/m_util_r8.f:473-637: ord 2 (with or without excision) and ord 4
+ * without excision (dvi4q1d_ex is not restated) */
 void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
                    int nx2, int ny2, int nz2, int i1, int j1, int k1,
                    int i2, int j2, int k2, int ni, int nj, int nk,
@@ -162,8 +224,8 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
   if (j2 < 1 || (j2 + (nj - 1) * jrho) > ny2) return;
   if (k1 < 1 || (k1 + nk - 1) * sk1 > nz1) return;
   if (k2 < 1 || (k2 + (nk - 1) * krho) > nz2) return;
-  if (ord != 2) {
-    fprintf(stderr, "or_dminterp3d: only ord=2 is restated\n");
+  if (!(ord == 2 || (ord == 4 && do_ex != 1))) {
+    fprintf(stderr, "or_dminterp3d: ord=%d with do_ex=%d is not restated\n", ord, do_ex);
     abort();
   }
   if (do_ex == 1 && (nx1 > nmax || ny1 > nmax || nz1 > nmax)) {
@@ -179,6 +241,8 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
       if (do_ex == 1) {
         const double *cl = &F3(chr1, 1, j1 + sj1 * (j - j2) / jrho, k1 + sk1 * (k - k2) / krho, nx1, ny1);
         or_dvi2q1d_ex(line, line, i2, i2, nx2, ni, irho, irho, 1, cl, i1, si1, ex);
+      } else if (ord == 4) {
+        or_dvi4q1d(line, line, i2, i2, nx2, ni, irho, irho, 1);
       } else {
         or_dvi2q1d(line, line, i2, i2, ni, irho, irho, 1);
       }
@@ -194,6 +258,8 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
           chrt_and(chrt, &F3(chr1, i1 + si1 * ((i - i2) / irho + 1), 1, k1 + sk1 * ((k - k2) / krho), nx1, ny1), ny1, nx1, ex);
         or_dvi2q1d_ex(col, col, 1 + (j2 - 1) * nx2, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho,
                       jrho * nx2, nx2, chrt, j1, sj1, ex);
+      } else if (ord == 4) {
+        or_dvi4q1d(col, col, 1 + (j2 - 1) * nx2, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho, jrho * nx2, nx2);
       } else {
         or_dvi2q1d(col, col, 1 + (j2 - 1) * nx2, 1 + (j2 - 1) * nx2, nj, jrho, jrho * nx2, nx2);
       }
@@ -216,6 +282,9 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
                    nz1, nxy, ex);
         or_dvi2q1d_ex(col, col, (k2 - 1) * nx2 * ny2 + 1, (k2 - 1) * nx2 * ny2 + 1,
                       nx2 * ny2 * (nz2 - 1) + 1, nk, krho, krho * nx2 * ny2, nx2 * ny2, chrt, k1, sk1, ex);
+      } else if (ord == 4) {
+        or_dvi4q1d(col, col, (k2 - 1) * nx2 * ny2 + 1, (k2 - 1) * nx2 * ny2 + 1, nx2 * ny2 * (nz2 - 1) + 1, nk,
+                   krho, krho * nx2 * ny2, nx2 * ny2);
       } else {
         or_dvi2q1d(col, col, (k2 - 1) * nx2 * ny2 + 1, (k2 - 1) * nx2 * ny2 + 1, nk, krho,
                    krho * nx2 * ny2, nx2 * ny2);

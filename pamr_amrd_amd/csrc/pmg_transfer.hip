@@ -90,6 +90,84 @@ __global__ __launch_bounds__(NT) void k_interp3d(const T *u1, T *u2, int nx1, in
   u2[ix3(i2 - 1 + fi, j2 - 1 + fj, k2 - 1 + fk, nx2, ny2)] = v;
 }
 
+// ---- fourth-order interpolation (dminterp3d, ord = 4, no excision) ------
+// One fine point of dvi4q1d (src/m_util_r8.f:65-137) on a line whose anchors
+// are A(a), a = -1 .. n, at the 1-based line positions p0 + a*rho of a line
+// of N points; f = a*rho + r (0 < r < rho) is the point's offset from the
+// first anchor.  The first cell uses anchors 0..3 when p0 <= rho (no anchor
+// to its left), the last cell anchors n-4..n-1 when p0 + (n+2)*rho > N,
+// every other cell a-1..a+2; n < 4 falls back to dvi2q1d (:332-360).  The
+// literals are the reference's 0.1666 6666 6666 6667 d0 (16 digits: fixed-
+// form blanks are not significant) and the operations keep its order.
+template <typename F>
+__device__ __forceinline__ double vi4_point(int n, int rho, int p0, int N, int f, const F &A) {
+  const int a = f / rho, r = f % rho;
+  if (n < 4) {
+    const double sg = (r * 1.0) / rho;
+    const double c0 = 1 - sg, c1 = sg;
+    return c0 * A(a) + c1 * A(a + 1);
+  }
+  const double half = 0.5, mhalf = -0.5, sixth = 0.1666666666666667, msixth = -0.1666666666666667;
+  double sg;
+  int b;
+  if (p0 <= rho && a == 0) {
+    sg = (r * 1.0) / rho;
+    b = 0;
+  } else if (p0 + (n + 2) * rho > N && a == n - 2) {
+    sg = 2.0 + (r * 1.0) / rho;
+    b = n - 4;
+  } else {
+    sg = 1.0 + (r * 1.0) / rho;
+    b = a - 1;
+  }
+  const double c0 = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+  const double c1 = half * sg * (sg - 2.0) * (sg - 3.0);
+  const double c2 = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+  const double c3 = sixth * sg * (sg - 1.0) * (sg - 2.0);
+  return c0 * A(b) + c1 * A(b + 1) + c2 * A(b + 2) + c3 * A(b + 3);
+}
+
+// The three line phases of dminterp3d (src/m_util_r8.f:518-634), one launch
+// each, one thread per written point.  A phase writes only points off the
+// anchor lattice of its axis and reads only points on it (inside or just
+// outside the segment), so its points are independent; phases run in the
+// reference's order x, y, z on u2 after the common points were copied.
+template <int AX>
+__global__ __launch_bounds__(NT) void k_interp4(double *u2, int nx2, int ny2, int nz2, int i2, int j2, int k2, int ni,
+                                                int nj, int nk, int irho, int jrho, int krho) {
+  const int mi = (ni - 1) * irho + 1, mj = (nj - 1) * jrho + 1, mk = (nk - 1) * krho + 1;
+  const size_t q = (size_t)blockIdx.x * NT + threadIdx.x;
+  int fi, fj, fk;  // offsets from (i2, j2, k2); the phase's own axis is the fine one
+  if (AX == 0) {
+    if (q >= (size_t)mi * nj * nk) return;
+    fi = (int)(q % mi);
+    const int line = (int)(q / mi);
+    fj = (line % nj) * jrho;
+    fk = (line / nj) * krho;
+    if (fi % irho == 0) return;
+    auto A = [&](int a) -> double { return u2[ix3(i2 - 1 + a * irho, j2 - 1 + fj, k2 - 1 + fk, nx2, ny2)]; };
+    u2[ix3(i2 - 1 + fi, j2 - 1 + fj, k2 - 1 + fk, nx2, ny2)] = vi4_point(ni, irho, i2, nx2, fi, A);
+  } else if (AX == 1) {
+    if (q >= (size_t)mi * mj * nk) return;
+    fj = (int)(q % mj);
+    const size_t line = q / mj;
+    fi = (int)(line % mi);
+    fk = (int)(line / mi) * krho;
+    if (fj % jrho == 0) return;
+    auto A = [&](int a) -> double { return u2[ix3(i2 - 1 + fi, j2 - 1 + a * jrho, k2 - 1 + fk, nx2, ny2)]; };
+    u2[ix3(i2 - 1 + fi, j2 - 1 + fj, k2 - 1 + fk, nx2, ny2)] = vi4_point(nj, jrho, j2, ny2, fj, A);
+  } else {
+    if (q >= (size_t)mi * mj * mk) return;
+    fk = (int)(q / ((size_t)mi * mj));
+    const size_t rest = q % ((size_t)mi * mj);
+    fi = (int)(rest % mi);
+    fj = (int)(rest / mi);
+    if (fk % krho == 0) return;
+    auto A = [&](int a) -> double { return u2[ix3(i2 - 1 + fi, j2 - 1 + fj, k2 - 1 + a * krho, nx2, ny2)]; };
+    u2[ix3(i2 - 1 + fi, j2 - 1 + fj, k2 - 1 + fk, nx2, ny2)] = vi4_point(nk, krho, k2, nz2, fk, A);
+  }
+}
+
 // ---- excision interpolation: one thread per line (dvi2q1d_ex) ----------
 // Line view: v(q) for q = 1..n1 (1-based) at stride sv elements.
 template <typename T, typename CHR>
@@ -280,8 +358,12 @@ int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, i
   if (j2 < 1 || (j2 + (nj - 1) * jrho) > ny2) return 0;
   if (k1 < 1 || (k1 + nk - 1) * sk1 > nz1) return 0;
   if (k2 < 1 || (k2 + (nk - 1) * krho) > nz2) return 0;
-  if (ord != 2) {
-    std::fprintf(stderr, "pmg_dminterp3d: only ord=2 is implemented (got %d)\n", ord);
+  if (ord != 2 && ord != 4) {
+    std::fprintf(stderr, "pmg_dminterp3d: only ord=2,4 are supported (got %d)\n", ord);
+    return -2;
+  }
+  if (ord == 4 && do_ex == 1) {
+    std::fprintf(stderr, "pmg_dminterp3d: ord=4 with excision (dvi4q1d_ex) is not implemented\n");
     return -2;
   }
   if (do_ex == 1 && (nx1 > 3000 || ny1 > 3000 || nz1 > 3000)) {
@@ -289,6 +371,26 @@ int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, i
     return -2;
   }
   const int mi = (ni - 1) * irho + 1, mj = (nj - 1) * jrho + 1, mk = (nk - 1) * krho + 1;
+  if (ord == 4) {
+    int rc = pmg_dmcopy3d(u1, u2, nx1, ny1, nz1, nx2, ny2, nz2, i1, j1, k1, i2, j2, k2, ni, nj, nk, si1, sj1, sk1,
+                          irho, jrho, krho, stream);
+    if (rc) return rc;
+    size_t n = (size_t)mi * nj * nk;
+    hipLaunchKernelGGL(k_interp4<0>, dim3(nblk(n)), dim3(NT), 0, stream, u2, nx2, ny2, nz2, i2, j2, k2, ni, nj, nk,
+                       irho, jrho, krho);
+    PMG_HOST_CHECK(hipGetLastError());
+    if (nj == 1 && nk == 1) return 0;
+    n = (size_t)mi * mj * nk;
+    hipLaunchKernelGGL(k_interp4<1>, dim3(nblk(n)), dim3(NT), 0, stream, u2, nx2, ny2, nz2, i2, j2, k2, ni, nj, nk,
+                       irho, jrho, krho);
+    PMG_HOST_CHECK(hipGetLastError());
+    if (nk == 1) return 0;
+    n = (size_t)mi * mj * mk;
+    hipLaunchKernelGGL(k_interp4<2>, dim3(nblk(n)), dim3(NT), 0, stream, u2, nx2, ny2, nz2, i2, j2, k2, ni, nj, nk,
+                       irho, jrho, krho);
+    PMG_HOST_CHECK(hipGetLastError());
+    return 0;
+  }
   if (do_ex != 1) {
     size_t n = (size_t)mi * mj * mk;
     hipLaunchKernelGGL(k_interp3d<double>, dim3(nblk(n)), dim3(NT), 0, stream, u1, u2, nx1,

@@ -103,6 +103,25 @@ def main():
     out["hook_nbs_phr"] = phr
     out["hook_nbs_relax_V"] = Vb
     out["hook_nbs_relax_norm"] = np.array([nrm.value])
+    # fourth-order interpolation (dvi4q1d, no excision) into a pre-filled
+    # target: an interior segment (anchors outside it are read) and the
+    # whole grid (both boundary stencils)
+    rng4 = np.random.default_rng(20261203)
+    for tag, (ni, nj, nk, i2, j2, k2, rho, extra) in {"a": (6, 4, 5, 3, 3, 5, 2, 3),
+                                                       "b": (5, 5, 5, 1, 1, 1, 2, 0)}.items():
+        cx, cy, cz = ni + 2, nj + 1, nk + 3
+        u1 = rng4.standard_normal((cz, cy, cx))
+        nx = i2 + (ni - 1) * rho + extra
+        ny = j2 + (nj - 1) * rho + extra
+        nz = k2 + (nk - 1) * rho + extra
+        u2 = rng4.standard_normal((nz, ny, nx))
+        args = [cx, cy, cz, nx, ny, nz, 2, 1, 3, i2, j2, k2, ni, nj, nk, 1, 1, 1, rho, rho, rho, 4]
+        o = u2.copy()
+        R.dminterp3d_(P(u1), P(o), *[ci(v) for v in args], P(u1), cd(1.0), ci(0))
+        out[f"interp4_{tag}_u1"] = u1
+        out[f"interp4_{tag}_u2"] = u2
+        out[f"interp4_{tag}_args"] = np.array(args, dtype=np.int64)
+        out[f"interp4_{tag}_out"] = o
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
 

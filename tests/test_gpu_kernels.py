@@ -75,6 +75,37 @@ def test_interpolation(pm, torch_cuda, cshape, ex):
     assert np.array_equal(dout.cpu().numpy(), ref)
 
 
+@pytest.mark.parametrize("seg", [
+    # (ni, nj, nk), (i2, j2, k2), rho, extra fine points past the segment
+    ((5, 5, 5), (1, 1, 1), 2, 0),
+    ((6, 4, 5), (3, 3, 5), 2, 3),
+    ((4, 7, 3), (2, 1, 3), 2, 1),
+    ((5, 4, 4), (5, 9, 1), 4, 8),
+    ((33, 33, 33), (1, 1, 1), 2, 0),
+    ((30, 21, 17), (4, 3, 5), 2, 4)])
+def test_cubic_interpolation(pm, torch_cuda, seg):
+    """dminterp3d ord = 4 (dvi4q1d, no excision) into a pre-filled target,
+    bit for bit against the oracle (pinned to the reference Fortran)"""
+    torch = torch_cuda
+    (ni, nj, nk), (i2, j2, k2), rho, extra = seg
+    print(f"case interp4 seg={seg}", flush=True)
+    rng = np.random.default_rng(ni * 97 + nj * 13 + nk + rho)
+    cx, cy, cz = ni + 2, nj + 1, nk + 3
+    u1 = rng.standard_normal((cz, cy, cx))
+    nx, ny, nz = i2 + (ni - 1) * rho + extra, j2 + (nj - 1) * rho + extra, k2 + (nk - 1) * rho + extra
+    u2 = rng.standard_normal((nz, ny, nx))
+    args = [cx, cy, cz, nx, ny, nz, 2, 1, 3, i2, j2, k2, ni, nj, nk, 1, 1, 1, rho, rho, rho, 4]
+    ref = O.dminterp3d(u1.copy(), u2.copy(), args)
+    du, dout = _dev(torch, u1), _dev(torch, u2)
+    rc = pm.lib().pmg_dminterp3d(pm.tptr(du), pm.tptr(dout), *args, None, 1.0, 0, None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(dout.cpu().numpy(), ref)
+    # with excision the fourth-order path is refused, not approximated
+    dchr = torch.zeros_like(du)
+    assert pm.lib().pmg_dminterp3d(pm.tptr(du), pm.tptr(dout), *args, pm.tptr(dchr), 1.0, 1, None) == -2
+
+
 @pytest.mark.parametrize("shape", [(5, 5, 5), (12, 9, 7), (33, 33, 33), (65, 40, 17)])
 @pytest.mark.parametrize("op", [0, 1])
 @pytest.mark.parametrize("ex", [0, 1])

@@ -98,6 +98,18 @@ def test_interpolation_exact_for_trilinear():
     assert np.allclose(O.interp(g), gf, rtol=0, atol=1e-14)
 
 
+def test_cubic_interpolation_exact_for_tricubic():
+    """dvi4q1d is 4-point Lagrange interpolation: exact for polynomials of
+    degree <= 3 in each variable (the premise of test/test1.c:287-288)"""
+    X, Y, Z = grid(9)
+    p = lambda x, y, z: (1 + x - 2 * x ** 3) * (0.5 - y ** 2 + y ** 3) * (2 + z + 3 * z ** 3) + x * y * z ** 2
+    g = np.broadcast_to(p(X, Y, Z), (9, 9, 9)).copy()
+    Xf, Yf, Zf = grid(17)
+    assert np.allclose(O.interp(g, ord=4), p(Xf, Yf, Zf), rtol=0, atol=1e-13)
+    # a cubic term breaks the trilinear interpolation, as it must
+    assert not np.allclose(O.interp(g, ord=2), p(Xf, Yf, Zf), rtol=0, atol=1e-6)
+
+
 def _setup(n, rho):
     m = O.UniformMG(n)
     m.set_params(pre=2, pst=2, max_citer=5000, crtol=1e-3, rop_u=3, rop_res=3, rop_src=3)

@@ -64,3 +64,13 @@ def test_hooks_match_reference(g):
     n = L.or_relax(0, O.P(Vb), O.P(rhs), O.P(src), None, None, O.P(cm), None, 1.0, nx, ny, nz, dx, dy, dz)
     assert np.array_equal(Vb, g["hook_nbs_relax_V"])
     assert n == g["hook_nbs_relax_norm"][0]
+
+
+@pytest.mark.parametrize("tag", ["a", "b"])
+def test_cubic_interpolation_matches_reference(g, tag):
+    """ord = 4 (dvi4q1d) into a pre-filled target: interior segment (tag a)
+    and whole grid (tag b)"""
+    u1 = g[f"interp4_{tag}_u1"].copy()
+    u2 = g[f"interp4_{tag}_u2"].copy()
+    O.dminterp3d(u1, u2, g[f"interp4_{tag}_args"])
+    assert np.array_equal(u2, g[f"interp4_{tag}_out"])

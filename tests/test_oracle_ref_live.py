@@ -60,6 +60,32 @@ def test_interp_live(cshape, ex):
     assert np.array_equal(ua, ub)
 
 
+@pytest.mark.parametrize("seg", [
+    # (ni, nj, nk), (i2, j2, k2) offsets in u2, rho, extra fine points past the segment
+    ((5, 5, 5), (1, 1, 1), 2, 0),       # whole grid: both boundary stencils everywhere
+    ((6, 4, 5), (3, 3, 5), 2, 3),       # interior segment: anchors outside it are read
+    ((4, 7, 3), (2, 1, 3), 2, 1),       # nk = 3 < 4: that axis falls back to dvi2q1d
+    ((5, 4, 4), (5, 9, 1), 4, 8),       # rho 4
+    ((9, 5, 6), (3, 1, 3), 2, 2)])
+def test_interp4_live(seg):
+    """ord = 4 (dvi4q1d), no excision, with u2 pre-filled: the interior
+    cubic stencils read u2's anchors outside the interpolated segment"""
+    (ni, nj, nk), (i2, j2, k2), rho, extra = seg
+    rng = np.random.default_rng(ni * 97 + nj * 13 + nk + rho)
+    cx, cy, cz = ni + 2, nj + 1, nk + 3
+    u1 = rng.standard_normal((cz, cy, cx))
+    nx = i2 + (ni - 1) * rho + extra
+    ny = j2 + (nj - 1) * rho + extra
+    nz = k2 + (nk - 1) * rho + extra
+    u2 = rng.standard_normal((nz, ny, nx))
+    a, b = u2.copy(), u2.copy()
+    args = [cx, cy, cz, nx, ny, nz, 2, 1, 3, i2, j2, k2, ni, nj, nk, 1, 1, 1, rho, rho, rho, 4]
+    O.lib().or_dminterp3d(O.P(u1), O.P(a), *args, O.P(u1), 1.0, 0)
+    R.dminterp3d_(O.P(u1), O.P(b), *[ci(v) for v in args], O.P(u1), cd(1.0), ci(0))
+    assert np.array_equal(a, b)
+    assert not np.array_equal(a, u2)
+
+
 @pytest.mark.parametrize("shape", [(5, 5, 5), (12, 9, 7), (17, 17, 17)])
 def test_hooks_live(shape):
     nx, ny, nz = shape
