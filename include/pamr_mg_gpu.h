@@ -51,8 +51,8 @@ int pmg_dmhwr3d(const double *u1, double *u2, int nx1, int ny1, int nz1, int nx2
                 double ex, int do_ex, hipStream_t stream);
 
 /* replaces dminterp3d_ (src/m_util_r8.f:473-637; include/m_util_r8.h:15-19):
- * ord = 2 (dvi2q1d / dvi2q1d_ex, :332-421) and ord = 4 without excision
- * (dvi4q1d, :65-137); ord = 4 with do_ex = 1 (dvi4q1d_ex) returns -2. */
+ * ord = 2 (dvi2q1d / dvi2q1d_ex, :332-421) and ord = 4 (dvi4q1d /
+ * dvi4q1d_ex, :65-319; with excision rho <= 20, else -2). */
 int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, int ny2, int nz2,
                    int i1, int j1, int k1, int i2, int j2, int k2, int ni, int nj, int nk,
                    int si1, int sj1, int sk1, int irho, int jrho, int krho, int ord,

@@ -11,7 +11,8 @@
  *   src/m_util_r8.f:368-421  dvi2q1d_ex          -> or_dvi2q1d_ex
  *   src/m_util_r8.f:426-456  init_chrt/and_chrt  -> chrt_init/chrt_and
  *   src/m_util_r8.f:65-137   dvi4q1d             -> or_dvi4q1d
- *   src/m_util_r8.f:473-637  dminterp3d (ord 2; ord 4 without excision) -> or_dminterp3d
+ *   src/m_util_r8.f:146-319  dvi4q1d_ex          -> or_dvi4q1d_ex
+ *   src/m_util_r8.f:473-637  dminterp3d (ord 2, 4) -> or_dminterp3d
  *   src/m_util_r8.f:649-723  dmhwr3d             -> or_dmhwr3d
  *   src/m_util_r8.f:733-949  dmfwr3d(_ex)        -> or_dmfwr3d
  *   w/examples/nps/num_w.f:18-157 lop/residual/relax (plain Poisson hooks,
@@ -148,6 +149,130 @@ void or_dvi4q1d(double *v1, double *v2, int i1, int i2, int n1, int n, int irho,
 #undef V2
 }
 
+/* src/m_util_r8.f:146-319: dvi4q1d with excision.  Extrapolates excised
+ * anchors next to non-excised runs (in place in v1), copies the common
+ * points, then writes the first / last / interior cubic cells whose four
+ * anchors are all non-excised; a non-excised run shorter than 4 anchors
+ * first sends the whole line through dvi2q1d_ex.  chr1/i3/s3 index the
+ * coarse mask as in or_dvi2q1d_ex.  The conditions and the '(i1s-4*s1)'
+ * bound of the right-side extrapolation are kept as written. */
+void or_dvi4q1d_ex(double *v1, double *v2, int i1, int i2, int n1, int n, int irho, int s1, int s2,
+                   const double *chr1, int i3, int s3, double ex)
+{
+  double c0[20], c1[20], c2[20], c3[20], sg;
+  int i, j, cn, j0, i1s, i1e, i10, i30, init_2;
+  const double half = 0.5, mhalf = -0.5, sixth = 0.1666666666666667, msixth = -0.1666666666666667;
+#define V1(q) v1[(q)-1]
+#define V2(q) v2[(q)-1]
+#define C1(q) chr1[(q)-1]
+  if (irho > 20) {
+    fprintf(stderr, "or_dvi4q1d_ex: irho expected to be less than 20\n");
+    abort();
+  }
+  i = 3;
+  i1s = i1;
+  while (i > 0 && i1s > s1) {
+    i1s -= s1;
+    i--;
+  }
+  i = 3;
+  i1e = i1 + (n - 1) * s1;
+  while (i > 0 && i1e < (n1 - s1)) {
+    i1e += s1;
+    i--;
+  }
+  cn = 0;
+  init_2 = 0;
+  for (i = i1s; i <= i1e; i += s1) {
+    if (C1((i - i1) * s3 / s1 + i3) == ex || i == i1e) {
+      if (cn > 0 && cn < 4)
+        init_2 = 1;
+      else
+        cn = 0;
+    } else {
+      cn++;
+    }
+  }
+  if (init_2) or_dvi2q1d_ex(v1, v2, i1, i2, n1, n, irho, s1, s2, chr1, i3, s3, ex);
+  for (i = i1s; i <= i1e; i += s1) {
+    if (C1(i3 + s3 * (i - i1) / s1) == ex) {
+      if (i >= (i1s + 3 * s1) && C1(i3 + s3 * (i - s1 - i1) / s1) != ex && C1(i3 + s3 * (i - 2 * s1 - i1) / s1) != ex &&
+          C1(i3 + s3 * (i - 3 * s1 - i1) / s1) != ex) {
+        if (i >= (i1s + 4 * s1) && C1(i3 + s3 * (i - 4 * s1 - i1) / s1) != ex)
+          V1(i) = 4 * (V1(i - s1) + V1(i - 3 * s1)) - 6 * V1(i - 2 * s1) - V1(i - 4 * s1);
+        else
+          V1(i) = 3 * (V1(i - s1) - V1(i - 2 * s1)) + V1(i - 3 * s1);
+      } else if (i <= (i1e - 3 * s1) && C1(i3 + s3 * (i + s1 - i1) / s1) != ex &&
+                 C1(i3 + s3 * (i + 2 * s1 - i1) / s1) != ex && C1(i3 + s3 * (i + 3 * s1 - i1) / s1) != ex) {
+        if (i <= (i1s - 4 * s1) && C1(i3 + s3 * (i + 4 * s1 - i1) / s1) != ex)
+          V1(i) = 4 * (V1(i + s1) + V1(i + 3 * s1)) - 6 * V1(i + 2 * s1) - V1(i + 4 * s1);
+        else
+          V1(i) = 3 * (V1(i + s1) - V1(i + 2 * s1)) + V1(i + 3 * s1);
+      }
+    }
+    if (i >= i1 && i <= (i1 + (n - 1) * s1)) V2(irho * ((i - i1) / s1) * s2 + i2) = V1(i);
+  }
+  /* first cells of non-excised runs */
+  for (j = 1; j <= irho - 1; j++) {
+    sg = (j * 1.0) / irho;
+    c0[j - 1] = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+    c1[j - 1] = half * sg * (sg - 2.0) * (sg - 3.0);
+    c2[j - 1] = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+    c3[j - 1] = sixth * sg * (sg - 1.0) * (sg - 2.0);
+  }
+  for (i = 1; i <= n - 1; i++) {
+    i10 = (i - 1) * s1 + i1;
+    i30 = (i - 1) * s3 + i3;
+    j0 = (i - 1) * irho * s2 + i2;
+    if (i10 <= (i1e - 3 * s1) && C1(i30 + s3) != ex && C1(i30 + 2 * s3) != ex &&
+        ((i == 1 && i10 == i1s) || (i10 >= (i1s + s1) && C1(i30) != ex && C1(i30 - s3) == ex) || C1(i30) == ex)) {
+      for (j = 1; j <= irho - 1; j++)
+        V2(j * s2 + j0) = c0[j - 1] * V1(i10) + c1[j - 1] * V1(s1 + i10) + c2[j - 1] * V1(2 * s1 + i10) +
+                          c3[j - 1] * V1(3 * s1 + i10);
+    }
+  }
+  /* last cells of non-excised runs */
+  for (j = 1; j <= irho - 1; j++) {
+    sg = 2.0 + (j * 1.0) / irho;
+    c0[j - 1] = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+    c1[j - 1] = half * sg * (sg - 2.0) * (sg - 3.0);
+    c2[j - 1] = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+    c3[j - 1] = sixth * sg * (sg - 1.0) * (sg - 2.0);
+  }
+  for (i = 1; i <= n - 1; i++) {
+    i10 = (i - 1) * s1 + i1;
+    i30 = (i - 1) * s3 + i3;
+    j0 = (i - 1) * irho * s2 + i2;
+    if (i10 >= (i1s + 2 * s1) && C1(i30 - s3) != ex &&
+        ((i == (n - 1) && i10 == (i1e - s1)) || (i10 <= (i1e - s1) && C1(i30) != ex && C1(i30 + s3) == ex))) {
+      for (j = 1; j <= irho - 1; j++)
+        V2(j * s2 + j0) = c0[j - 1] * V1(i10 - 2 * s1) + c1[j - 1] * V1(i10 - s1) + c2[j - 1] * V1(i10) +
+                          c3[j - 1] * V1(i10 + s1);
+    }
+  }
+  /* interior cells */
+  for (j = 1; j <= irho - 1; j++) {
+    sg = 1.0 + (j * 1.0) / irho;
+    c0[j - 1] = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+    c1[j - 1] = half * sg * (sg - 2.0) * (sg - 3.0);
+    c2[j - 1] = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+    c3[j - 1] = sixth * sg * (sg - 1.0) * (sg - 2.0);
+  }
+  for (i = 1; i <= n; i++) {
+    i10 = (i - 1) * s1 + i1;
+    i30 = (i - 1) * s3 + i3;
+    j0 = (i - 1) * irho * s2 + i2;
+    if (C1(i30) != ex && i10 >= (i1s + s1) && C1(i30 - s3) != ex && i10 <= (i1e - 2 * s1) && C1(i30 + s3) != ex) {
+      for (j = 1; j <= irho - 1; j++)
+        V2(j0 + j * s2) = c0[j - 1] * V1(i10 - s1) + c1[j - 1] * V1(i10) + c2[j - 1] * V1(i10 + s1) +
+                          c3[j - 1] * V1(i10 + 2 * s1);
+    }
+  }
+#undef V1
+#undef V2
+#undef C1
+}
+
 /* src/m_util_r8.f:368-421 (extrapolates excised anchors of v1, in place) */
 void or_dvi2q1d_ex(double *v1, double *v2, int i1, int i2, int n1, int n,
                    int irho, int s1, int s2, const double *chr1, int i3,
@@ -207,8 +332,7 @@ static void chrt_and(double *chrt, const double *chr1, int n, int s3, double ex)
   }
 }
 
-/* src/m_util_r8.f:473-637: ord 2 (with or without excision) and ord 4
- * without excision (dvi4q1d_ex is not restated) */
+/* src/m_util_r8.f:473-637: ord 2 and 4, with or without excision */
 void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
                    int nx2, int ny2, int nz2, int i1, int j1, int k1,
                    int i2, int j2, int k2, int ni, int nj, int nk,
@@ -224,8 +348,8 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
   if (j2 < 1 || (j2 + (nj - 1) * jrho) > ny2) return;
   if (k1 < 1 || (k1 + nk - 1) * sk1 > nz1) return;
   if (k2 < 1 || (k2 + (nk - 1) * krho) > nz2) return;
-  if (!(ord == 2 || (ord == 4 && do_ex != 1))) {
-    fprintf(stderr, "or_dminterp3d: ord=%d with do_ex=%d is not restated\n", ord, do_ex);
+  if (!(ord == 2 || ord == 4)) {
+    fprintf(stderr, "or_dminterp3d: only ord=2,4 currently supported\n");
     abort();
   }
   if (do_ex == 1 && (nx1 > nmax || ny1 > nmax || nz1 > nmax)) {
@@ -240,7 +364,10 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
       double *line = &F3(u2, 1, j, k, nx2, ny2);
       if (do_ex == 1) {
         const double *cl = &F3(chr1, 1, j1 + sj1 * (j - j2) / jrho, k1 + sk1 * (k - k2) / krho, nx1, ny1);
-        or_dvi2q1d_ex(line, line, i2, i2, nx2, ni, irho, irho, 1, cl, i1, si1, ex);
+        if (ord == 4)
+          or_dvi4q1d_ex(line, line, i2, i2, nx2, ni, irho, irho, 1, cl, i1, si1, ex);
+        else
+          or_dvi2q1d_ex(line, line, i2, i2, nx2, ni, irho, irho, 1, cl, i1, si1, ex);
       } else if (ord == 4) {
         or_dvi4q1d(line, line, i2, i2, nx2, ni, irho, irho, 1);
       } else {
@@ -256,8 +383,12 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
         chrt_init(chrt, &F3(chr1, i1 + si1 * ((i - i2) / irho), 1, k1 + sk1 * ((k - k2) / krho), nx1, ny1), ny1, nx1);
         if ((i - i2) % irho != 0)
           chrt_and(chrt, &F3(chr1, i1 + si1 * ((i - i2) / irho + 1), 1, k1 + sk1 * ((k - k2) / krho), nx1, ny1), ny1, nx1, ex);
-        or_dvi2q1d_ex(col, col, 1 + (j2 - 1) * nx2, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho,
-                      jrho * nx2, nx2, chrt, j1, sj1, ex);
+        if (ord == 4)
+          or_dvi4q1d_ex(col, col, 1 + (j2 - 1) * nx2, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho,
+                        jrho * nx2, nx2, chrt, j1, sj1, ex);
+        else
+          or_dvi2q1d_ex(col, col, 1 + (j2 - 1) * nx2, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho,
+                        jrho * nx2, nx2, chrt, j1, sj1, ex);
       } else if (ord == 4) {
         or_dvi4q1d(col, col, 1 + (j2 - 1) * nx2, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho, jrho * nx2, nx2);
       } else {
@@ -280,8 +411,12 @@ void or_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1,
         if ((j - j2) % jrho != 0 && (i - i2) % irho != 0)
           chrt_and(chrt, &F3(chr1, i1 + si1 * ((i - i2) / irho + 1), j1 + sj1 * ((j - j2) / jrho + 1), 1, nx1, ny1),
                    nz1, nxy, ex);
-        or_dvi2q1d_ex(col, col, (k2 - 1) * nx2 * ny2 + 1, (k2 - 1) * nx2 * ny2 + 1,
-                      nx2 * ny2 * (nz2 - 1) + 1, nk, krho, krho * nx2 * ny2, nx2 * ny2, chrt, k1, sk1, ex);
+        if (ord == 4)
+          or_dvi4q1d_ex(col, col, (k2 - 1) * nx2 * ny2 + 1, (k2 - 1) * nx2 * ny2 + 1,
+                        nx2 * ny2 * (nz2 - 1) + 1, nk, krho, krho * nx2 * ny2, nx2 * ny2, chrt, k1, sk1, ex);
+        else
+          or_dvi2q1d_ex(col, col, (k2 - 1) * nx2 * ny2 + 1, (k2 - 1) * nx2 * ny2 + 1,
+                        nx2 * ny2 * (nz2 - 1) + 1, nk, krho, krho * nx2 * ny2, nx2 * ny2, chrt, k1, sk1, ex);
       } else if (ord == 4) {
         or_dvi4q1d(col, col, (k2 - 1) * nx2 * ny2 + 1, (k2 - 1) * nx2 * ny2 + 1, nx2 * ny2 * (nz2 - 1) + 1, nk,
                    krho, krho * nx2 * ny2, nx2 * ny2);

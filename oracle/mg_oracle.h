@@ -28,6 +28,8 @@ void or_dmcopy3d(const double *u1, double *u2, int nx1, int ny1, int nz1,
                  int i2, int j2, int k2, int ni, int nj, int nk,
                  int si1, int sj1, int sk1, int si2, int sj2, int sk2);
 void or_dvi4q1d(double *v1, double *v2, int i1, int i2, int n1, int n, int irho, int s1, int s2);
+void or_dvi4q1d_ex(double *v1, double *v2, int i1, int i2, int n1, int n, int irho, int s1, int s2,
+                   const double *chr1, int i3, int s3, double ex);
 void or_dvi2q1d(double *v1, double *v2, int i1, int i2, int n, int irho,
                 int s1, int s2);
 void or_dvi2q1d_ex(double *v1, double *v2, int i1, int i2, int n1, int n,

@@ -203,6 +203,97 @@ __device__ void line_vi2q1d_ex(T *v, int i1, int n1, int n, int irho, int s1, in
 #undef LC
 }
 
+// dvi4q1d_ex (src/m_util_r8.f:146-319) on one line, in place (v1 = v2 with
+// s1 == irho*s2, so the common-point copy is a no-op): the extension of the
+// anchor range by up to 3 anchors each side, the dvi2q1d_ex pass when a
+// non-excised run is shorter than 4 anchors, extrapolation into excised
+// anchors, then the first / last / interior cubic cells -- in the
+// reference's order, with its conditions as written.
+template <typename T, typename CHR>
+__device__ void line_vi4q1d_ex(T *v, int i1, int n1, int n, int irho, int s1, int s2, const CHR &chr, int i3,
+                               int s3, T ex) {
+#define LV(q) v[((size_t)(q)-1)]
+#define LC(q) chr((q)-1)
+  const double half = 0.5, mhalf = -0.5, sixth = 0.1666666666666667, msixth = -0.1666666666666667;
+  int i = 3, i1s = i1;
+  while (i > 0 && i1s > s1) {
+    i1s -= s1;
+    i--;
+  }
+  i = 3;
+  int i1e = i1 + (n - 1) * s1;
+  while (i > 0 && i1e < (n1 - s1)) {
+    i1e += s1;
+    i--;
+  }
+  int cn = 0;
+  bool init_2 = false;
+  for (i = i1s; i <= i1e; i += s1) {
+    if (LC((i - i1) * s3 / s1 + i3) == ex || i == i1e) {
+      if (cn > 0 && cn < 4) init_2 = true;
+      else cn = 0;
+    } else {
+      cn++;
+    }
+  }
+  if (init_2) line_vi2q1d_ex<T>(v, i1, n1, n, irho, s1, s2, chr, i3, s3, ex);
+  for (i = i1s; i <= i1e; i += s1) {
+    if (LC(i3 + s3 * (i - i1) / s1) == ex) {
+      if (i >= (i1s + 3 * s1) && LC(i3 + s3 * (i - s1 - i1) / s1) != ex && LC(i3 + s3 * (i - 2 * s1 - i1) / s1) != ex &&
+          LC(i3 + s3 * (i - 3 * s1 - i1) / s1) != ex) {
+        if (i >= (i1s + 4 * s1) && LC(i3 + s3 * (i - 4 * s1 - i1) / s1) != ex)
+          LV(i) = 4 * (LV(i - s1) + LV(i - 3 * s1)) - 6 * LV(i - 2 * s1) - LV(i - 4 * s1);
+        else
+          LV(i) = 3 * (LV(i - s1) - LV(i - 2 * s1)) + LV(i - 3 * s1);
+      } else if (i <= (i1e - 3 * s1) && LC(i3 + s3 * (i + s1 - i1) / s1) != ex &&
+                 LC(i3 + s3 * (i + 2 * s1 - i1) / s1) != ex && LC(i3 + s3 * (i + 3 * s1 - i1) / s1) != ex) {
+        if (i <= (i1s - 4 * s1) && LC(i3 + s3 * (i + 4 * s1 - i1) / s1) != ex)
+          LV(i) = 4 * (LV(i + s1) + LV(i + 3 * s1)) - 6 * LV(i + 2 * s1) - LV(i + 4 * s1);
+        else
+          LV(i) = 3 * (LV(i + s1) - LV(i + 2 * s1)) + LV(i + 3 * s1);
+      }
+    }
+  }
+  auto coef = [&](double sg, double *c) {
+    c[0] = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
+    c[1] = half * sg * (sg - 2.0) * (sg - 3.0);
+    c[2] = mhalf * sg * (sg - 1.0) * (sg - 3.0);
+    c[3] = sixth * sg * (sg - 1.0) * (sg - 2.0);
+  };
+  double c[4];
+  for (i = 1; i <= n - 1; i++) {  // first cells of non-excised runs
+    const int i10 = (i - 1) * s1 + i1, i30 = (i - 1) * s3 + i3, j0 = (i - 1) * irho * s2 + i1;
+    if (i10 <= (i1e - 3 * s1) && LC(i30 + s3) != ex && LC(i30 + 2 * s3) != ex &&
+        ((i == 1 && i10 == i1s) || (i10 >= (i1s + s1) && LC(i30) != ex && LC(i30 - s3) == ex) || LC(i30) == ex)) {
+      for (int j = 1; j <= irho - 1; j++) {
+        coef((j * 1.0) / irho, c);
+        LV(j * s2 + j0) = c[0] * LV(i10) + c[1] * LV(s1 + i10) + c[2] * LV(2 * s1 + i10) + c[3] * LV(3 * s1 + i10);
+      }
+    }
+  }
+  for (i = 1; i <= n - 1; i++) {  // last cells of non-excised runs
+    const int i10 = (i - 1) * s1 + i1, i30 = (i - 1) * s3 + i3, j0 = (i - 1) * irho * s2 + i1;
+    if (i10 >= (i1s + 2 * s1) && LC(i30 - s3) != ex &&
+        ((i == (n - 1) && i10 == (i1e - s1)) || (i10 <= (i1e - s1) && LC(i30) != ex && LC(i30 + s3) == ex))) {
+      for (int j = 1; j <= irho - 1; j++) {
+        coef(2.0 + (j * 1.0) / irho, c);
+        LV(j * s2 + j0) = c[0] * LV(i10 - 2 * s1) + c[1] * LV(i10 - s1) + c[2] * LV(i10) + c[3] * LV(i10 + s1);
+      }
+    }
+  }
+  for (i = 1; i <= n; i++) {  // interior cells
+    const int i10 = (i - 1) * s1 + i1, i30 = (i - 1) * s3 + i3, j0 = (i - 1) * irho * s2 + i1;
+    if (LC(i30) != ex && i10 >= (i1s + s1) && LC(i30 - s3) != ex && i10 <= (i1e - 2 * s1) && LC(i30 + s3) != ex) {
+      for (int j = 1; j <= irho - 1; j++) {
+        coef(1.0 + (j * 1.0) / irho, c);
+        LV(j0 + j * s2) = c[0] * LV(i10 - s1) + c[1] * LV(i10) + c[2] * LV(i10 + s1) + c[3] * LV(i10 + 2 * s1);
+      }
+    }
+  }
+#undef LV
+#undef LC
+}
+
 // chrt of src/m_util_r8.f:426-456 evaluated on demand: init_chrt copies
 // column c0, each and_chrt folds one more column (ex only if both are ex).
 template <typename T>
@@ -220,7 +311,7 @@ struct ChrtCols {
 };
 
 // x lines through anchor (j,k): in place on u2
-template <typename T>
+template <typename T, int ORD>
 __global__ void k_interp_ex_x(T *u2, const T *chr1, T ex, int nx1, int ny1, int nx2, int ny2,
                               int i1, int j1, int k1, int i2, int j2, int k2, int ni, int nj,
                               int nk, int si1, int sj1, int sk1, int irho, int jrho, int krho) {
@@ -231,11 +322,12 @@ __global__ void k_interp_ex_x(T *u2, const T *chr1, T ex, int nx1, int ny1, int 
   T *line = u2 + ix3(0, j - 1, k - 1, nx2, ny2);
   ChrtCols<T> cl{{chr1 + ix3(0, j1 + sj1 * (j - j2) / jrho - 1, k1 + sk1 * (k - k2) / krho - 1,
                              nx1, ny1), nullptr, nullptr, nullptr}, 1, ex};
-  line_vi2q1d_ex<T>(line, i2, nx2, ni, irho, irho, 1, cl, i1, si1, ex);
+  if (ORD == 4) line_vi4q1d_ex<T>(line, i2, nx2, ni, irho, irho, 1, cl, i1, si1, ex);
+  else line_vi2q1d_ex<T>(line, i2, nx2, ni, irho, irho, 1, cl, i1, si1, ex);
 }
 
 // y lines for every fine i at anchor k; chrt staged in a per-thread column
-template <typename T>
+template <typename T, int ORD>
 __global__ void k_interp_ex_y(T *u2, const T *chr1, T ex, int nx1, int ny1,
                               int nx2, int ny2, int i1, int j1, int k1, int i2, int j2, int k2,
                               int ni, int nj, int nk, int si1, int sj1, int sk1, int irho,
@@ -251,12 +343,15 @@ __global__ void k_interp_ex_y(T *u2, const T *chr1, T ex, int nx1, int ny1,
                   nullptr, nullptr},
                  (size_t)nx1, ex};
   T *col = u2 + ix3(i - 1, 0, k - 1, nx2, ny2);
-  line_vi2q1d_ex<T>(col, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho, jrho * nx2, nx2, ch,
-                    j1, sj1, ex);
+  if (ORD == 4)
+    line_vi4q1d_ex<T>(col, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho, jrho * nx2, nx2, ch, j1, sj1, ex);
+  else
+    line_vi2q1d_ex<T>(col, 1 + (j2 - 1) * nx2, nx2 * (ny2 - 1) + 1, nj, jrho, jrho * nx2, nx2, ch,
+                      j1, sj1, ex);
 }
 
 // z lines for every fine (i,j)
-template <typename T>
+template <typename T, int ORD>
 __global__ void k_interp_ex_z(T *u2, const T *chr1, T ex, int nx1, int ny1,
                               int nz1, int nx2, int ny2, int nz2, int i1, int j1, int k1, int i2,
                               int j2, int k2, int ni, int nj, int nk, int si1, int sj1, int sk1,
@@ -276,8 +371,11 @@ __global__ void k_interp_ex_z(T *u2, const T *chr1, T ex, int nx1, int ny1,
   (void)nz1;
   T *col = u2 + ix3(i - 1, j - 1, 0, nx2, ny2);
   const int s2 = nx2 * ny2;
-  line_vi2q1d_ex<T>(col, (k2 - 1) * s2 + 1, s2 * (nz2 - 1) + 1, nk, krho, krho * s2, s2, ch, k1,
-                    sk1, ex);
+  if (ORD == 4)
+    line_vi4q1d_ex<T>(col, (k2 - 1) * s2 + 1, s2 * (nz2 - 1) + 1, nk, krho, krho * s2, s2, ch, k1, sk1, ex);
+  else
+    line_vi2q1d_ex<T>(col, (k2 - 1) * s2 + 1, s2 * (nz2 - 1) + 1, nk, krho, krho * s2, s2, ch, k1,
+                      sk1, ex);
 }
 
 }  // namespace
@@ -362,8 +460,8 @@ int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, i
     std::fprintf(stderr, "pmg_dminterp3d: only ord=2,4 are supported (got %d)\n", ord);
     return -2;
   }
-  if (ord == 4 && do_ex == 1) {
-    std::fprintf(stderr, "pmg_dminterp3d: ord=4 with excision (dvi4q1d_ex) is not implemented\n");
+  if (ord == 4 && do_ex == 1 && (irho > 20 || jrho > 20 || krho > 20)) {
+    std::fprintf(stderr, "pmg_dminterp3d: dvi4q1d_ex expects rho <= 20\n");
     return -2;
   }
   if (do_ex == 1 && (nx1 > 3000 || ny1 > 3000 || nz1 > 3000)) {
@@ -371,7 +469,7 @@ int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, i
     return -2;
   }
   const int mi = (ni - 1) * irho + 1, mj = (nj - 1) * jrho + 1, mk = (nk - 1) * krho + 1;
-  if (ord == 4) {
+  if (ord == 4 && do_ex != 1) {
     int rc = pmg_dmcopy3d(u1, u2, nx1, ny1, nz1, nx2, ny2, nz2, i1, j1, k1, i2, j2, k2, ni, nj, nk, si1, sj1, sk1,
                           irho, jrho, krho, stream);
     if (rc) return rc;
@@ -400,21 +498,24 @@ int pmg_dminterp3d(double *u1, double *u2, int nx1, int ny1, int nz1, int nx2, i
     return 0;
   }
   if (!chr1) return -1;
+  auto kex_x = ord == 4 ? k_interp_ex_x<double, 4> : k_interp_ex_x<double, 2>;
+  auto kex_y = ord == 4 ? k_interp_ex_y<double, 4> : k_interp_ex_y<double, 2>;
+  auto kex_z = ord == 4 ? k_interp_ex_z<double, 4> : k_interp_ex_z<double, 2>;
   // common points, then the three line phases in place on u2
   int rc = pmg_dmcopy3d(u1, u2, nx1, ny1, nz1, nx2, ny2, nz2, i1, j1, k1, i2, j2, k2, ni, nj, nk,
                         si1, sj1, sk1, irho, jrho, krho, stream);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_interp_ex_x<double>, dim3((nj * nk + 63) / 64), dim3(64), 0, stream, u2,
+  hipLaunchKernelGGL(kex_x, dim3((nj * nk + 63) / 64), dim3(64), 0, stream, u2,
                      chr1, ex, nx1, ny1, nx2, ny2, i1, j1, k1, i2, j2, k2, ni, nj, nk, si1, sj1,
                      sk1, irho, jrho, krho);
   PMG_HOST_CHECK(hipGetLastError());
   if (nj == 1 && nk == 1) return 0;
-  hipLaunchKernelGGL(k_interp_ex_y<double>, dim3((mi * nk + 63) / 64), dim3(64), 0, stream, u2,
+  hipLaunchKernelGGL(kex_y, dim3((mi * nk + 63) / 64), dim3(64), 0, stream, u2,
                      chr1, ex, nx1, ny1, nx2, ny2, i1, j1, k1, i2, j2, k2, ni, nj, nk,
                      si1, sj1, sk1, irho, jrho, krho);
   PMG_HOST_CHECK(hipGetLastError());
   if (nk == 1) return 0;
-  hipLaunchKernelGGL(k_interp_ex_z<double>, dim3((mi * mj + 63) / 64), dim3(64), 0, stream, u2,
+  hipLaunchKernelGGL(kex_z, dim3((mi * mj + 63) / 64), dim3(64), 0, stream, u2,
                      chr1, ex, nx1, ny1, nz1, nx2, ny2, nz2, i1, j1, k1, i2, j2, k2,
                      ni, nj, nk, si1, sj1, sk1, irho, jrho, krho);
   PMG_HOST_CHECK(hipGetLastError());

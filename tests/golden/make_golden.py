@@ -122,6 +122,21 @@ def main():
         out[f"interp4_{tag}_u2"] = u2
         out[f"interp4_{tag}_args"] = np.array(args, dtype=np.int64)
         out[f"interp4_{tag}_out"] = o
+    # ... and with excision (dvi4q1d_ex): whole grid, random coarse masks
+    for tag, (cx, cy, cz, dens) in {"a": (9, 7, 6, 0.3), "b": (9, 9, 9, 0.1)}.items():
+        u1 = rng4.standard_normal((cz, cy, cx))
+        chr_ = np.where(rng4.random((cz, cy, cx)) < dens, 1.0, 0.0)
+        nx, ny, nz = 2 * cx - 1, 2 * cy - 1, 2 * cz - 1
+        u2 = rng4.standard_normal((nz, ny, nx))
+        args = [cx, cy, cz, nx, ny, nz, 1, 1, 1, 1, 1, 1, cx, cy, cz, 1, 1, 1, 2, 2, 2, 4]
+        o, uu = u2.copy(), u1.copy()
+        R.dminterp3d_(P(uu), P(o), *[ci(v) for v in args], P(chr_), cd(1.0), ci(1))
+        out[f"interp4ex_{tag}_u1"] = u1
+        out[f"interp4ex_{tag}_chr"] = chr_
+        out[f"interp4ex_{tag}_u2"] = u2
+        out[f"interp4ex_{tag}_args"] = np.array(args, dtype=np.int64)
+        out[f"interp4ex_{tag}_out"] = o
+        out[f"interp4ex_{tag}_u1_after"] = uu
     np.savez_compressed(OUT, **out)
     print("wrote", OUT, os.path.getsize(OUT), "bytes")
 

@@ -101,9 +101,37 @@ def test_cubic_interpolation(pm, torch_cuda, seg):
     assert rc == 0
     torch.cuda.synchronize()
     assert np.array_equal(dout.cpu().numpy(), ref)
-    # with excision the fourth-order path is refused, not approximated
-    dchr = torch.zeros_like(du)
-    assert pm.lib().pmg_dminterp3d(pm.tptr(du), pm.tptr(dout), *args, pm.tptr(dchr), 1.0, 1, None) == -2
+
+
+@pytest.mark.parametrize("case", [
+    # coarse shape, coarse offset, segment, fine offset, rho, mask density
+    ((5, 5, 5), (1, 1, 1), (5, 5, 5), (1, 1, 1), 2, 0.15),
+    ((9, 7, 6), (1, 1, 1), (9, 7, 6), (1, 1, 1), 2, 0.3),
+    ((33, 33, 33), (1, 1, 1), (33, 33, 33), (1, 1, 1), 2, 0.1),
+    ((14, 13, 15), (5, 5, 5), (6, 5, 7), (9, 9, 9), 2, 0.2),
+    ((13, 13, 13), (5, 5, 5), (5, 5, 5), (17, 17, 17), 4, 0.25)])
+def test_cubic_interpolation_excised(pm, torch_cuda, case):
+    """dminterp3d ord = 4 with excision (dvi4q1d_ex) bit for bit against
+    the oracle; segments keep 4 coarse points of margin, the reach of the
+    reference's anchor extension"""
+    torch = torch_cuda
+    (cx, cy, cz), (i1, j1, k1), (ni, nj, nk), (i2, j2, k2), rho, dens = case
+    print(f"case interp4 ex {case}", flush=True)
+    rng = np.random.default_rng(cx * 7 + cy * 3 + cz + rho + int(dens * 100))
+    u1 = rng.standard_normal((cz, cy, cx))
+    chr_ = np.where(rng.random((cz, cy, cx)) < dens, 1.0, 0.0)
+    if (i1, j1, k1) == (1, 1, 1):
+        nx, ny, nz = (ni - 1) * rho + 1, (nj - 1) * rho + 1, (nk - 1) * rho + 1
+    else:
+        nx, ny, nz = [o + (m + 3) * rho + 1 for o, m in ((i2, ni), (j2, nj), (k2, nk))]
+    u2 = rng.standard_normal((nz, ny, nx))
+    args = [cx, cy, cz, nx, ny, nz, i1, j1, k1, i2, j2, k2, ni, nj, nk, 1, 1, 1, rho, rho, rho, 4]
+    ref = O.dminterp3d(u1.copy(), u2.copy(), args, chr1=chr_, ex=1.0, do_ex=1)
+    du, dout, dchr = _dev(torch, u1), _dev(torch, u2), _dev(torch, chr_)
+    rc = pm.lib().pmg_dminterp3d(pm.tptr(du), pm.tptr(dout), *args, pm.tptr(dchr), 1.0, 1, None)
+    assert rc == 0
+    torch.cuda.synchronize()
+    assert np.array_equal(dout.cpu().numpy(), ref)
 
 
 @pytest.mark.parametrize("shape", [(5, 5, 5), (12, 9, 7), (33, 33, 33), (65, 40, 17)])

@@ -74,3 +74,14 @@ def test_cubic_interpolation_matches_reference(g, tag):
     u2 = g[f"interp4_{tag}_u2"].copy()
     O.dminterp3d(u1, u2, g[f"interp4_{tag}_args"])
     assert np.array_equal(u2, g[f"interp4_{tag}_out"])
+
+
+@pytest.mark.parametrize("tag", ["a", "b"])
+def test_cubic_interpolation_with_excision_matches_reference(g, tag):
+    """ord = 4 with excision (dvi4q1d_ex): the fine result and the source
+    array, which the reference also modifies (dminterp3d's NOTE)"""
+    u1 = g[f"interp4ex_{tag}_u1"].copy()
+    u2 = g[f"interp4ex_{tag}_u2"].copy()
+    O.dminterp3d(u1, u2, g[f"interp4ex_{tag}_args"], chr1=g[f"interp4ex_{tag}_chr"], ex=1.0, do_ex=1)
+    assert np.array_equal(u2, g[f"interp4ex_{tag}_out"])
+    assert np.array_equal(u1, g[f"interp4ex_{tag}_u1_after"])

@@ -86,6 +86,34 @@ def test_interp4_live(seg):
     assert not np.array_equal(a, u2)
 
 
+@pytest.mark.parametrize("case", [
+    # coarse shape, coarse offset (i1,j1,k1), segment (ni,nj,nk), fine offset, rho, mask density
+    ((5, 5, 5), (1, 1, 1), (5, 5, 5), (1, 1, 1), 2, 0.15),
+    ((9, 7, 6), (1, 1, 1), (9, 7, 6), (1, 1, 1), 2, 0.3),
+    ((9, 9, 9), (1, 1, 1), (9, 9, 9), (1, 1, 1), 2, 0.05),
+    ((14, 13, 15), (5, 5, 5), (6, 5, 7), (9, 9, 9), 2, 0.2),
+    ((13, 13, 13), (5, 5, 5), (5, 5, 5), (17, 17, 17), 4, 0.25)])
+def test_interp4_ex_live(case):
+    """ord = 4 with excision (dvi4q1d_ex): extrapolation into excised
+    anchors, run starts / ends, the dvi2q1d_ex fallback for short runs"""
+    (cx, cy, cz), (i1, j1, k1), (ni, nj, nk), (i2, j2, k2), rho, dens = case
+    rng = np.random.default_rng(cx * 7 + cy * 3 + cz + rho + int(dens * 100))
+    u1 = rng.standard_normal((cz, cy, cx))
+    chr_ = np.where(rng.random((cz, cy, cx)) < dens, 1.0, 0.0)
+    if (i1, j1, k1) == (1, 1, 1):
+        nx, ny, nz = (ni - 1) * rho + 1, (nj - 1) * rho + 1, (nk - 1) * rho + 1
+    else:
+        nx, ny, nz = [o + (m + 3) * rho + 1 for o, m in ((i2, ni), (j2, nj), (k2, nk))]
+    u2 = rng.standard_normal((nz, ny, nx))
+    a, b = u2.copy(), u2.copy()
+    ua, ub = u1.copy(), u1.copy()
+    args = [cx, cy, cz, nx, ny, nz, i1, j1, k1, i2, j2, k2, ni, nj, nk, 1, 1, 1, rho, rho, rho, 4]
+    O.lib().or_dminterp3d(O.P(ua), O.P(a), *args, O.P(chr_), 1.0, 1)
+    R.dminterp3d_(O.P(ub), O.P(b), *[ci(v) for v in args], O.P(chr_), cd(1.0), ci(1))
+    assert np.array_equal(a, b)
+    assert np.array_equal(ua, ub)
+
+
 @pytest.mark.parametrize("shape", [(5, 5, 5), (12, 9, 7), (17, 17, 17)])
 def test_hooks_live(shape):
     nx, ny, nz = shape
