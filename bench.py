@@ -40,7 +40,8 @@ METRIC = "time to 1e-10 rel. residual per 257^3 solve; learned+FP32 vs plain FP6
 HBM_PEAK_GBS = 8000.0
 
 
-SWEEP_SHAPES = {0: (64, 8, 256), 1: (64, 16, 512), 2: (32, 16, 512), 3: (128, 8, 512)}
+SWEEP_SHAPES = {0: (64, 8, 256), 1: (64, 16, 512), 2: (32, 16, 512), 3: (128, 8, 512), 4: (64, 32, 512),
+                5: (128, 16, 1024)}
 
 
 def measured_traffic(n, variant, prec="double"):

@@ -241,8 +241,9 @@ template <typename T>
 T *uoth(Level &Lv) { LevelArrays<T> &A = arrays<T>(Lv); return curbuf<T>(Lv) ? A.u : A.ua; }
 
 // sweep tile variants (TX, TY, NT)
-#define PMG_SWEEP_VARIANTS(X) X(0, 64, 8, 256) X(1, 64, 16, 512) X(2, 32, 16, 512) X(3, 128, 8, 512)
-constexpr int kNumSweepVariants = 4;
+#define PMG_SWEEP_VARIANTS(X) \
+  X(0, 64, 8, 256) X(1, 64, 16, 512) X(2, 32, 16, 512) X(3, 128, 8, 512) X(4, 64, 32, 512) X(5, 128, 16, 1024)
+constexpr int kNumSweepVariants = 6;
 
 template <typename T, int OP, bool EX, bool P2, bool RHS0>
 int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
