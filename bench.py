@@ -198,6 +198,10 @@ def amr_leg(args, pm, torch):
         npts[L] = tot
     A.set_params(pre=2, pst=2, max_citer=5000, crtol=1e-3)
     A.init()
+    tuned = {}
+    if not args.amr_no_tune:
+        from pamr_amrd_amd import autotune as AT
+        tuned = AT.autotune_amr(A, nrep=5)
     hist = []
     for _ in range(2):
         hist.append(A.vcycle()[0])
@@ -225,6 +229,8 @@ def amr_leg(args, pm, torch):
                                  "achieved_GBps": round(bytes_sw / (t_sw * 1e-3) / 1e9, 1)},
         "transfer_boxes_per_level": {L: {"sync0": v[0], "sync1": v[1], "inject": v[2], "interp": v[3]}
                                      for L, v in stats.items()},
+        "sweep_tiling_per_level": {L: {"variant": v, "zchunk": z, "tuned_ms": round(ms, 5),
+                                       "default_ms": round(ms0, 5)} for L, (v, z, ms, ms0) in tuned.items()},
         "dtype": "f64",
     }
     A.close()
@@ -348,6 +354,7 @@ def main():
     ap.add_argument("--no-seq", action="store_true", help="skip the learned sequence leg")
     ap.add_argument("--no-amr", action="store_true", help="skip the batched AMR hierarchy leg")
     ap.add_argument("--amr-cycles", type=int, default=10)
+    ap.add_argument("--amr-no-tune", action="store_true", help="keep the default AMR sweep tiling")
     ap.add_argument("--no-excised", action="store_true", help="skip the large excised leg")
     ap.add_argument("--c5-n", type=int, default=513)
     ap.add_argument("--c5-rhs", type=int, default=16)

@@ -102,6 +102,14 @@ int pmg_amr_interp_fields(pmg_amr *A, int Lc, int nf, const int *which);
  * (amrd/mg.c:362-488) written into a field as 1.0 / 0.0; enqueued on
  * pmg_amr_stream */
 int pmg_amr_fill_cmask(pmg_amr *A, int L, int which, int field);
+/* Sweep tiling of level L for the autotuner: tile variant (0 .. count-1:
+ * 128x16, 64x16, 32x16, 32x8, 16x16 columns) and z-chunk length.  Results
+ * are identical for every tiling (only the order of the norm's partial sums
+ * changes); -3 when the level would have more tiles than the norm
+ * workspace holds (the previous tiling is kept). */
+int pmg_amr_num_sweep_variants(void);
+int pmg_amr_set_level_sweep_config(pmg_amr *A, int L, int variant, int zchunk);
+int pmg_amr_sweep_config(pmg_amr *A, int L, int *variant, int *zchunk);
 
 #ifdef __cplusplus
 }

@@ -138,6 +138,9 @@ def _declare(L):
     L.pmg_amr_inject_fields.argtypes = [_vp, i, i, _ip, _ip]
     L.pmg_amr_interp_fields.argtypes = [_vp, i, i, _ip]
     L.pmg_amr_fill_cmask.argtypes = [_vp, i, i, i]
+    L.pmg_amr_num_sweep_variants.argtypes = []
+    L.pmg_amr_set_level_sweep_config.argtypes = [_vp, i, i, i]
+    L.pmg_amr_sweep_config.argtypes = [_vp, i, _ip, _ip]
     _declare_pamr(L)
 
 
@@ -480,6 +483,17 @@ class Amr:
 
     def set_graph(self, on):
         _check(self._L.pmg_amr_set_graph(self.h, 1 if on else 0), "pmg_amr_set_graph")
+
+    def set_level_sweep_config(self, lev, variant, zchunk):
+        """Sweep tiling of level ``lev``; RuntimeError (code -3) when the
+        level would need more tiles than the norm workspace holds."""
+        _check(self._L.pmg_amr_set_level_sweep_config(self.h, lev, variant, zchunk),
+               "pmg_amr_set_level_sweep_config")
+
+    def sweep_config(self, lev):
+        v, z = C.c_int(0), C.c_int(0)
+        _check(self._L.pmg_amr_sweep_config(self.h, lev, C.byref(v), C.byref(z)), "pmg_amr_sweep_config")
+        return v.value, z.value
 
 
 class Net:

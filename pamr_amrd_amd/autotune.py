@@ -76,6 +76,48 @@ def autotune(n, zchunks=(2, 4, 8, 16, 32, 64), nrep=20, write=True, verbose=True
     return results
 
 
+AMR_VARIANTS = {0: (128, 16, 1024), 1: (64, 16, 512), 2: (32, 16, 512), 3: (32, 8, 256), 4: (16, 16, 256)}
+
+
+def autotune_amr(A, levels=None, zchunks=(4, 8, 16, 32), nrep=10, verbose=False):
+    """Per-level tile selection for the batched sweeps of an AMR hierarchy
+    (``pamr_amrd_amd.Amr``): every compiled tile variant and z-chunk is timed
+    with pmg_amr_time_sweep on the level's own patches and the fastest is
+    kept.  Timing relaxes u, so each level's u is saved and restored around
+    it.  Returns {level: (variant, zchunk, ms, default_ms)}."""
+    import pamr_amrd_amd as pm
+    nvar = pm.lib().pmg_amr_num_sweep_variants()
+    levels = levels or range(2, A.nlev + 1)
+    out = {}
+    for L in levels:
+        saved = [A.download(L, g, pm.PMG_F_U) for g in range(A.ngrids(L))]
+        v0, z0 = A.sweep_config(L)
+        A.time_sweep(L, 2)
+        t0 = A.time_sweep(L, nrep)
+        best = (v0, z0, t0)
+        nmax = max(max(A.grid(L, g)[0]) for g in range(A.ngrids(L)))
+        for v in range(nvar):
+            for z in zchunks:
+                if z > nmax - 2 or (v, z) == (v0, z0):
+                    continue
+                try:
+                    A.set_level_sweep_config(L, v, z)
+                except RuntimeError:
+                    continue
+                A.time_sweep(L, 2)
+                ms = A.time_sweep(L, nrep)
+                if verbose:
+                    print(f"amr autotune level {L} variant={v} {AMR_VARIANTS.get(v)} zchunk={z}: {ms:.4f} ms",
+                          flush=True)
+                if ms < best[2]:
+                    best = (v, z, ms)
+        A.set_level_sweep_config(L, best[0], best[1])
+        for g, u in enumerate(saved):
+            A.upload(L, g, pm.PMG_F_U, u)
+        out[L] = (best[0], best[1], best[2], t0)
+    return out
+
+
 if __name__ == "__main__":
     import argparse
     import shutil

@@ -730,16 +730,21 @@ void cmask_bytes(const pmg_amr *A, int L, int gi, int which, std::vector<uint8_t
   }
 }
 
-// sweep tile shapes (TX, TY, NT): the uniform solver's variants
-#define PMG_AMR_VARIANTS(X) X(1, 64, 16, 512) X(2, 32, 16, 512)
+// sweep tile shapes (TX, TY, NT); 1 and 2 are the defaults of single-grid
+// and patch levels, the others are for the per-level autotuner
+#define PMG_AMR_VARIANTS(X) \
+  X(0, 128, 16, 1024) X(1, 64, 16, 512) X(2, 32, 16, 512) X(3, 32, 8, 256) X(4, 16, 16, 256)
+constexpr int kAmrVariants = 5;
 
-void variant_txy(int v, int &tx, int &ty) {
+bool variant_txy(int v, int &tx, int &ty) {
   switch (v) {
-#define PMG_TXY(V, TXV, TYV, NTV) case V: tx = TXV; ty = TYV; break;
+#define PMG_TXY(V, TXV, TYV, NTV) case V: tx = TXV; ty = TYV; return true;
     PMG_AMR_VARIANTS(PMG_TXY)
 #undef PMG_TXY
-    default: tx = 32; ty = 16;
   }
+  tx = 32;
+  ty = 16;
+  return false;
 }
 
 int build_tiles(pmg_amr *A, ALevel &Lv) {
@@ -1492,6 +1497,40 @@ int pmg_amr_inject_fields(pmg_amr *A, int Lf, int nf, const int *which, const in
     int rc = run_grouped(A, A->lev[Lf - 2].inject, Lf, Lf - 1, op, (int)sel.size(), sel.data());
     if (rc) return rc;
   }
+  return 0;
+}
+
+int pmg_amr_num_sweep_variants(void) { return kAmrVariants; }
+
+int pmg_amr_set_level_sweep_config(pmg_amr *A, int L, int variant, int zchunk) {
+  int tx, ty;
+  if (!A || L < 1 || L > A->nlev || zchunk < 1 || !variant_txy(variant, tx, ty)) return -1;
+  ALevel &Lv = A->lev[L - 1];
+  PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
+  const int ov = Lv.variant, oz = Lv.zchunk;
+  Lv.variant = variant;
+  Lv.zchunk = zchunk;
+  int rc = build_tiles(A, Lv);
+  if (!rc && std::max(Lv.nstiles, Lv.nptiles) + 64 > A->ws.cap) rc = -3;  // more tiles than norm slots
+  if (rc) {
+    Lv.variant = ov;
+    Lv.zchunk = oz;
+    if (build_tiles(A, Lv)) return -1;
+  }
+  // the fold table points at the tile ranges just rebuilt
+  if (A->d_fold) {
+    (void)hipFree(A->d_fold);
+    A->d_fold = nullptr;
+  }
+  if (build_folds(A)) return -1;
+  drop_graphs(A);
+  return rc;
+}
+
+int pmg_amr_sweep_config(pmg_amr *A, int L, int *variant, int *zchunk) {
+  if (!A || L < 1 || L > A->nlev) return -1;
+  if (variant) *variant = A->lev[L - 1].variant;
+  if (zchunk) *zchunk = A->lev[L - 1].zchunk;
   return 0;
 }
 

@@ -146,3 +146,64 @@ def test_larger_layout_cycles(torch_cuda):
         assert np.array_equal(a.download(L, g, pm.PMG_F_U), m.field(L, g, O.OA_U)), ("u", L, g)
     print("plan stats", [a.plan_stats(L) for L in range(1, a.nlev + 1)], flush=True)
     a.close()
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
+def test_sweep_tilings_bit_exact(torch_cuda, variant):
+    """Every compiled tile variant, with z-chunks 2/4/8 spread over the
+    levels, gives the oracle's V-cycles bit for bit (the tiling changes
+    only the summation order of the norms)."""
+    m, a = pair(21 + variant, op=variant % 2, base_n=33, nmin=4, nmax=8, smin=9, smax=25)
+    for L in range(1, a.nlev + 1):
+        a.set_level_sweep_config(L, variant, (2, 4, 8)[L % 3])
+        assert a.sweep_config(L) == (variant, (2, 4, 8)[L % 3])
+    set_sources(m, a, variant % 2, 500 + variant)
+    for x in (m, a):
+        x.set_params()
+        x.init()
+    for c in range(3):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = a.vcycle()
+        print(f"variant {variant} cycle {c}: oracle {rm:.15e} gpu {ra:.15e}", flush=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert cm == ca
+        assert np.allclose(na, nm, rtol=1e-12, atol=0), (c, na, nm)
+    for L, g in each_grid(m):
+        assert np.array_equal(a.download(L, g, pm.PMG_F_U), m.field(L, g, O.OA_U)), ("u", L, g)
+    a.close()
+
+
+def test_autotune_keeps_state(torch_cuda):
+    """The per-level autotuner restores u around its timing runs: cycles
+    after tuning still equal the oracle's bit for bit."""
+    from pamr_amrd_amd import autotune as AT
+    m, a = pair(31, op=1, base_n=33, nmin=6, nmax=10, smin=9, smax=25)
+    set_sources(m, a, 1, 77)
+    for x in (m, a):
+        x.set_params()
+        x.init()
+    rm, _, _ = m.vcycle()
+    ra, _, _ = a.vcycle()
+    assert abs(ra - rm) <= 1e-12 * rm
+    chosen = AT.autotune_amr(a, zchunks=(2, 4, 8), nrep=2)
+    print(f"autotuned {chosen}", flush=True)
+    assert set(chosen) == set(range(2, a.nlev + 1))
+    for L, (v, z, _, _) in chosen.items():
+        assert a.sweep_config(L) == (v, z)
+    for c in range(2):
+        rm, _, _ = m.vcycle()
+        ra, _, _ = a.vcycle()
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+    for L, g in each_grid(m):
+        assert np.array_equal(a.download(L, g, pm.PMG_F_U), m.field(L, g, O.OA_U)), ("u", L, g)
+    a.close()
+
+
+def test_sweep_config_rejects(torch_cuda):
+    m, a = pair(3)
+    v0 = a.sweep_config(a.nlev)
+    for bad in ((99, 4), (0, 0)):
+        with pytest.raises(RuntimeError):
+            a.set_level_sweep_config(a.nlev, *bad)
+        assert a.sweep_config(a.nlev) == v0
+    a.close()
