@@ -54,15 +54,34 @@ int amrd_mg_def_vars(const char *elliptic_vars, const char *mg_fw_restr, const c
 void amrd_mg_set_params(int pre_swp, int pst_swp, int min_iter, int max_iter, double tol,
                         double crtol, int max_citer, double w0_r, double w0_i);
 
+/* AMRD's truncation-error variables (amrd/io.c:810-838): for every name in
+ * tre_vars (AMRD_TRE_vars) "<name>_tre" is defined in the AMR hierarchy
+ * with one time level; the first one also lives in the MG hierarchy and
+ * receives the MG estimate.  Call before PAMR_compose_hierarchy.
+ * Returns 1 / 0. */
+int amrd_def_tre_vars(const char *tre_vars);
+
 /* vcycle (amrd/mg.c:84-201) on the current MG hierarchy; returns the
- * residual norm of the finest level (negative on error) */
+ * residual norm of the finest level (negative on error).  amrd_vcycle is
+ * vcycle(giter, 0). */
 double amrd_vcycle(int giter);
+
+/* vcycle(giter, save_tre) (amrd/mg.c:84): with save_tre set,
+ * compute_MG_rhs (amrd/mg.c:271-314) also writes max_i |lop_i| *
+ * sqrt(num_f_tre_vars) into the first "<name>_tre" variable on each coarse
+ * level, syncs it and interpolates it (second order) to the next finer
+ * level, so that every level above the coarsest ends with the estimate of
+ * the level below it.  An error (negative return) without tre variables. */
+double amrd_vcycle_tre(int giter, int save_tre);
 
 /* solve_elliptics (amrd/mg.c:20-79): PAMR_build_mgh(L, finest AMR level,
  * tl), V-cycles while (iter < min_iter || res > tol) && iter < max_iter,
  * PAMR_destroy_mgh.  Returns the number of cycles (negative on error);
  * res_out (may be NULL) the last residual. */
 int amrd_solve_elliptics(int L, int tl, double *res_out);
+/* solve_elliptics(L, tl, save_tre) (amrd/mg.c:20): the same with every
+ * V-cycle run as amrd_vcycle_tre(iter, save_tre) */
+int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out);
 
 /* last norms of amrd_vcycle: lev_norms[L-1] for MG levels 1..Lf (the
  * reference's res[] array) and the coarsest-level sweep count */

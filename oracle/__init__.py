@@ -113,6 +113,7 @@ def _declare(L):
     L.or_amr_field.restype = _dp
     L.or_amr_set_params.argtypes = [vp, i, i, i, d, d, d, i, i, i, i]
     L.or_amr_init.argtypes = [vp]
+    L.or_amr_set_tre.argtypes = [vp, i, i]
     L.or_amr_vcycle.argtypes = [vp, _dp, _ip]
     L.or_amr_vcycle.restype = d
     L.or_amr_residual_fine.argtypes = [vp]
@@ -229,7 +230,7 @@ class UniformMG:
         return self.L.or_mg_residual_fine(self.h)
 
 
-OA_U, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C = range(8)
+OA_U, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_TRE = range(9)
 
 
 class AmrMG:
@@ -280,6 +281,9 @@ class AmrMG:
 
     def init(self):
         self.L.or_amr_init(self.h)
+
+    def set_tre(self, save_tre, ntre=1):
+        self.L.or_amr_set_tre(self.h, 1 if save_tre else 0, ntre)
 
     def vcycle(self):
         norms = np.zeros(self.nlev)

@@ -46,6 +46,7 @@ struct or_amr {
   int periodic[3];
   int pre, pst, max_citer, rop_u, rop_res, rop_src, iord;
   double crtol, w0_r, w0_i;
+  int save_tre, ntre;
   int tf[OA_NUM]; /* transfer bits per field for the current communicate() */
 };
 
@@ -532,6 +533,12 @@ void or_amr_set_params(or_amr *m, int pre, int pst, int max_citer, double crtol,
   m->rop_src = rop_src; m->iord = interp_ord;
 }
 
+void or_amr_set_tre(or_amr *m, int save_tre, int ntre)
+{
+  m->save_tre = save_tre;
+  m->ntre = ntre;
+}
+
 /* TF_MGH_INIT (src/gh.c:1237-1239,1334; src/pamr.c:1898): the in_amrh
  * fields with mg_inject -- u and the sources -- into each new MG level */
 void or_amr_init(or_amr *m)
@@ -620,7 +627,8 @@ static void hook_lop_level(or_amr *m, int L)
   }
 }
 
-/* amrd/mg.c:271-314 (save_tre = 0) */
+/* amrd/mg.c:271-314; one elliptic variable, so the TRE's running max
+ * starts from 0 at every point (i == 0, :290) */
 static void compute_mg_rhs(or_amr *m, int L)
 {
   int gi;
@@ -628,6 +636,10 @@ static void compute_mg_rhs(or_amr *m, int L)
   for (gi = 0; gi < m->lev[L - 1].ng; gi++) {
     ag_grid *g = &m->lev[L - 1].g[gi];
     for (j = 0; j < gsize(g); j++) {
+      if (m->save_tre) {
+        double t = 0, v = fabs(g->f[OA_LOP][j]) * sqrt((double)m->ntre);
+        g->f[OA_TRE][j] = t > v ? t : v; /* max (amrd/util.h:12) */
+      }
       if (g->f[OA_CMASK][j] == 1.0)
         g->f[OA_RHS][j] = -g->f[OA_RES][j] * m->w0_r + g->f[OA_LOP][j];
       else
@@ -690,6 +702,14 @@ double or_amr_vcycle(or_amr *m, double *lev_norms, int *citer)
       or_amr_set_cmask(m, L, 1);
       hook_lop_level(m, L);
       compute_mg_rhs(m, L);
+      if (m->save_tre) { /* amrd/mg.c:306-313: sync on L, interp to L+1 */
+        tf_clear(m);
+        m->tf[OA_TRE] = 1;
+        do_sync(m, L, 0);
+        tf_clear(m);
+        m->tf[OA_TRE] = 2;
+        communicate(m, L, L + 1, 0, 1);
+      }
     }
     or_amr_set_cmask(m, L, 0);
     for (iter = 0; iter < m->pre; iter++) {

@@ -26,7 +26,7 @@ extern "C" {
 typedef struct or_amr or_amr;
 
 /* field ids of or_amr_field */
-enum { OA_U = 0, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_NUM };
+enum { OA_U = 0, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_TRE, OA_NUM };
 
 /* base_shape: points of the base grid over bbox; namr: number of refined
  * AMR levels; nbox[l]: boxes of refined level l+2; boxes: 6 doubles per box
@@ -46,6 +46,11 @@ void or_amr_set_params(or_amr *m, int pre, int pst, int max_citer,
                        int rop_res, int rop_src, int interp_ord);
 /* TF_MGH_INIT injection into the MG levels below the base */
 void or_amr_init(or_amr *m);
+/* compute_MG_rhs's truncation-error estimate (amrd/mg.c:288-295,306-313):
+ * with save_tre set, every V-cycle writes |lop| * sqrt(ntre) into OA_TRE
+ * on each coarse level, syncs it and interpolates it to the next finer
+ * level.  ntre is AMRD_num_f_tre_vars. */
+void or_amr_set_tre(or_amr *m, int save_tre, int ntre);
 /* one FAS V-cycle over the hierarchy; returns res[Lf] (max over the grids
  * of Lf); lev_norms (nlev doubles) gets the last norm of every level */
 double or_amr_vcycle(or_amr *m, double *lev_norms, int *citer);

@@ -210,6 +210,10 @@ def _declare_pamr(L):
     L.amrd_vcycle.argtypes = [i]
     L.amrd_vcycle.restype = d
     L.amrd_solve_elliptics.argtypes = [i, i, _dp]
+    L.amrd_vcycle_tre.argtypes = [i, i]
+    L.amrd_vcycle_tre.restype = d
+    L.amrd_solve_elliptics_tre.argtypes = [i, i, i, _dp]
+    L.amrd_def_tre_vars.argtypes = [s]
     L.amrd_mg_last_norms.argtypes = [_dp, _ip]
     L.amrd_gpu_stream.argtypes = []
     L.amrd_gpu_stream.restype = _vp

@@ -140,9 +140,9 @@ int field_of_gfn(int gfn) {
   Var *v = var_of_gfn(gfn, &tl);
   if (!v) return -1;
   if (tl > 0) return v->amr_field[tl - 1];
-  if (v->a.in_amrh) {
-    if (C->mgh_tl < 1 || C->mgh_tl > (int)v->amr_field.size()) return -1;
-    return v->amr_field[C->mgh_tl - 1];
+  if (v->a.in_amrh) {  // the MG gfn aliases AMR time level min(tl, num_tl) (src/gh.c:1143-1146)
+    if (C->mgh_tl < 1 || v->amr_field.empty()) return -1;
+    return v->amr_field[std::min<size_t>(C->mgh_tl, v->amr_field.size()) - 1];
   }
   return v->mg_field;
 }
@@ -229,6 +229,20 @@ __global__ __launch_bounds__(kNT) void k_mg_rhs(double *rhs, double *lop, const 
   lop[i] = f[i];
 }
 
+// compute_MG_rhs with save_tre (amrd/mg.c:286-303): the running max of
+// |lop| * sqrt(num_f_tre_vars) over the elliptic variables, taken before lop
+// is overwritten, restarted at 0 by the first variable
+__global__ __launch_bounds__(kNT) void k_mg_rhs_tre(double *rhs, double *lop, const double *res, const double *f,
+                                                    const double *cmask, double w0_r, double *tre, int first,
+                                                    double scale, long long n) {
+  const long long i = (long long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= n) return;
+  const double t = first ? 0.0 : tre[i], v = fabs(lop[i]) * scale;
+  tre[i] = t > v ? t : v;  // max (amrd/util.h:12)
+  rhs[i] = cmask[i] == 1.0 ? -res[i] * w0_r + lop[i] : 0.0;
+  lop[i] = f[i];
+}
+
 // compute_cgc, coarse part (amrd/mg.c:336-339)
 __global__ __launch_bounds__(kNT) void k_cgc_res(double *res, const double *f, const double *lop, double w0_i,
                                                  long long n) {
@@ -254,6 +268,8 @@ struct Amrd {
   amrd_level_real_hook lresidual = nullptr;
   amrd_level_void_hook llop = nullptr;
   std::vector<std::string> ell;
+  std::string tre;  // "<v>_tre" of the first truncation-error variable (also in the MG hierarchy)
+  int ntre = 0;     // AMRD_num_f_tre_vars
   int do_ex = 0;
   int pre = 3, pst = 3, min_iter = 1, max_iter = 50, max_citer = 5000;
   double tol = 0.0, crtol = 1e-3, w0_r = 1.0, w0_i = 1.0;
@@ -321,15 +337,41 @@ bool clear_mg_vars(int L, const std::vector<EllGfns> &E) {
   return true;
 }
 
-// compute_MG_rhs (amrd/mg.c:271-314, save_tre = 0)
-bool compute_mg_rhs(int L, const std::vector<EllGfns> &E, int cmask) {
+// compute_MG_rhs (amrd/mg.c:271-314)
+bool compute_mg_rhs(int L, const std::vector<EllGfns> &E, int cmask, int save_tre) {
   long long n;
   const double *cm = pool(L, cmask, n);
+  int tre_gfn = 0;
+  double *tre = nullptr;
+  if (save_tre) {
+    if (!D.ntre || (tre_gfn = PAMR_get_gfn((char *)D.tre.c_str(), PAMR_MGH, 0)) <= 0 ||
+        !(tre = pool(L, tre_gfn, n))) {
+      std::printf("compute_MG_rhs: error ... save_tre=1, but no tre variables\n");
+      return false;
+    }
+  }
+  bool first = true;
   for (const EllGfns &e : E) {
     double *rhs = pool(L, e.rhs, n), *lop = pool(L, e.lop, n), *res = pool(L, e.res, n), *f = pool(L, e.f, n);
     if (!cm || !rhs || !lop || !res || !f) return false;
-    hipLaunchKernelGGL(k_mg_rhs, dim3(nblocks(n)), dim3(kNT), 0, C->stream, rhs, lop, res, f, cm, D.w0_r, n);
+    if (save_tre)
+      hipLaunchKernelGGL(k_mg_rhs_tre, dim3(nblocks(n)), dim3(kNT), 0, C->stream, rhs, lop, res, f, cm, D.w0_r, tre,
+                         first ? 1 : 0, std::sqrt((double)D.ntre), n);
+    else
+      hipLaunchKernelGGL(k_mg_rhs, dim3(nblocks(n)), dim3(kNT), 0, C->stream, rhs, lop, res, f, cm, D.w0_r, n);
     if (hipGetLastError() != hipSuccess) return false;
+    first = false;
+  }
+  if (save_tre) {  // set_gfn_sync, sync on L; set_gfn_in, interp to L+1 (amrd/mg.c:306-313)
+    PAMR_freeze_tf_bits();
+    PAMR_clear_tf_bits();
+    PAMR_set_tf_bit(tre_gfn, PAMR_SYNC);
+    if (!PAMR_sync(L, 0, PAMR_MGH, 0)) return false;
+    PAMR_freeze_tf_bits();
+    PAMR_clear_tf_bits();
+    PAMR_set_tf_bit(tre_gfn, PAMR_SECOND_ORDER);
+    if (!PAMR_interp(L, 0, PAMR_MGH)) return false;
+    PAMR_thaw_tf_bits();
   }
   return true;
 }
@@ -453,6 +495,9 @@ void PAMR_free_context(int num) {
   if (C->A) pmg_amr_destroy(C->A);
   delete C;
   C = nullptr;
+  D.ell.clear();  // AMRD's variable tables belong to the context's variables
+  D.tre.clear();
+  D.ntre = 0;
 }
 
 void PAMR_set_lambda(double lambda) {
@@ -734,17 +779,16 @@ int PAMR_build_mgh(int min_lev, int max_lev, int tl) {
                 C->namr + 1);
     return 0;
   }
-  for (const Var &v : C->vars)
-    if (v.a.in_amrh && v.a.in_mgh && (tl < 1 || tl > v.a.num_tl)) {
-      std::printf("PAMR_build_mgh: error ... time level %i of <%s> does not exist\n", tl, v.name.c_str());
-      return 0;
-    }
+  if (tl < 1) {
+    std::printf("PAMR_build_mgh: error ... invalid time level %i\n", tl);
+    return 0;
+  }
   C->mgh = true;
   C->mgh_tl = tl;
   // freshly allocated MG grid functions are zero (src/gh.c:895-898)
   for (const Var &v : C->vars) {
     if (!v.a.in_mgh) continue;
-    const int f = v.a.in_amrh ? v.amr_field[tl - 1] : v.mg_field;
+    const int f = v.a.in_amrh ? v.amr_field[std::min<size_t>(tl, v.amr_field.size()) - 1] : v.mg_field;
     const int Ltop = v.a.in_amrh ? C->nbase - 1 : C->nlev;
     for (int L = 1; L <= Ltop; L++) {
       double *p = pmg_amr_field(C->A, L, 0, f);
@@ -1123,7 +1167,27 @@ void amrd_mg_set_params(int pre_swp, int pst_swp, int min_iter, int max_iter, do
 }
 
 // amrd/mg.c:84-201
-double amrd_vcycle(int giter) {
+int amrd_def_tre_vars(const char *tre_vars) {
+  if (!C) {
+    std::printf("amrd_def_tre_vars: error ... no current context\n");
+    return 0;
+  }
+  int pbt[6] = {PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN};
+  for (const std::string &v : split_list(tre_vars)) {
+    const std::string name = v + "_tre";
+    const int in_mgh = D.ntre == 0 ? 1 : 0;  // the first one carries the MG estimate (amrd/io.c:823-827)
+    if (!PAMR_def_var_full((char *)name.c_str(), 1, in_mgh, 1, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP,
+                           PAMR_NO_SYNC, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_SYNC, 1, PAMR_SECOND_ORDER, pbt))
+      return 0;
+    if (in_mgh) D.tre = name;
+    D.ntre++;
+  }
+  return 1;
+}
+
+double amrd_vcycle(int giter) { return amrd_vcycle_tre(giter, 0); }
+
+double amrd_vcycle_tre(int giter, int save_tre) {
   (void)giter;
   if (!C || !C->mgh) {
     std::printf("amrd_vcycle: error ... no MG hierarchy\n");
@@ -1155,7 +1219,7 @@ double amrd_vcycle(int giter) {
       if (!PAMR_sync(L, 0, PAMR_MGH, 0)) return -1.0;
       if (!set_cmask(L, 1, cmask)) return -1.0;
       lop_level(L);
-      if (!compute_mg_rhs(L, E, cmask)) return -1.0;
+      if (!compute_mg_rhs(L, E, cmask, save_tre)) return -1.0;
     }
     if (!set_cmask(L, 0, cmask)) return -1.0;
     for (iter = 0; iter < D.pre; iter++) {
@@ -1193,8 +1257,10 @@ double amrd_vcycle(int giter) {
   return res[Lf];
 }
 
+int amrd_solve_elliptics(int L, int tl, double *res_out) { return amrd_solve_elliptics_tre(L, tl, 0, res_out); }
+
 // amrd/mg.c:20-79 (no t0 initial-data or constraint hooks)
-int amrd_solve_elliptics(int L, int tl, double *res_out) {
+int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out) {
   if (!C || !C->A) {
     std::printf("amrd_solve_elliptics: error ... no AMR hierarchy\n");
     return -1;
@@ -1208,7 +1274,7 @@ int amrd_solve_elliptics(int L, int tl, double *res_out) {
   double res = D.tol + 1;
   int it = 0;
   while ((it < D.min_iter || res > D.tol) && it < D.max_iter) {
-    res = amrd_vcycle(it);
+    res = amrd_vcycle_tre(it, save_tre);
     if (res < 0) {
       PAMR_destroy_mgh();
       return -1;

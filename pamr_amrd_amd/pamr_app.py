@@ -49,7 +49,7 @@ class NpsContext:
     refined levels' boxes (``levels[l]``: boxes of AMR level l + 2)."""
 
     def __init__(self, base_n, levels, op=0, rop_u=3, rop_res=3, rop_src=3, mcw=5,
-                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):
+                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tre_vars=None):
         self.L, self.app = lib(), nps_lib()
         L = self.L
         shape = (C.c_int * 3)(base_n, base_n, base_n)
@@ -61,6 +61,8 @@ class NpsContext:
         L.PAMR_set_MG_coarse_width(w)
         if not self.app.nps_define_vars(op, rop_u, rop_res, rop_src):
             raise RuntimeError("nps_define_vars failed")
+        if tre_vars and not L.amrd_def_tre_vars(tre_vars.encode()):
+            raise RuntimeError("amrd_def_tre_vars failed")
         boxes = [list(self.bbox)] + [list(b) for lv in levels for b in lv]
         levs = [1] + [l + 2 for l, lv in enumerate(levels) for _ in lv]
         bb = np.array(boxes, dtype=np.float64).reshape(-1)
@@ -137,8 +139,8 @@ class NpsContext:
             raise RuntimeError("PAMR_build_mgh failed")
         return nl
 
-    def vcycle(self, giter=0):
-        r = self.L.amrd_vcycle(giter)
+    def vcycle(self, giter=0, save_tre=False):
+        r = self.L.amrd_vcycle_tre(giter, 1 if save_tre else 0)
         if r < 0:
             raise RuntimeError("amrd_vcycle failed")
         nl = self.L.PAMR_get_max_lev(PAMR_MGH)

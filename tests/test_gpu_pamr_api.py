@@ -201,3 +201,43 @@ def test_solve_elliptics_leaves_solution_in_amr_levels(torch_cuda):
         for g in range(m.ngrids(Lm)):
             assert np.array_equal(ctx.download(PAMR_AMRH, l, g, "V"), m.field(Lm, g, O.OA_U)), (l, g)
     ctx.close()
+
+
+@pytest.mark.parametrize("seed,op,tre_vars", [(1, 0, "V"), (4, 1, "V W")])
+def test_tre_matches_oracle(torch_cuda, seed, op, tre_vars):
+    """vcycle(giter, save_tre = 1): the truncation-error estimate written by
+    compute_MG_rhs (amrd/mg.c:288-295,306-313) into "V_tre" equals the
+    oracle's bit for bit on every grid of every MG level, with
+    num_f_tre_vars 1 and 2; the solution is unchanged by it."""
+    levels = layout(seed)
+    print(f"case tre seed={seed} op={op} tre_vars={tre_vars!r}", flush=True)
+    m, ctx = O.AmrMG(17, levels, op=op), NpsContext(17, levels, op=op, tre_vars=tre_vars)
+    ntre = len(tre_vars.split())
+    load_sources(m, ctx, op, 900 + seed)
+    m.set_params()
+    m.set_tre(True, ntre)
+    m.init()
+    ctx.set_params(pre=2, pst=2)
+    nl = ctx.build_mgh()
+    for c in range(3):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = ctx.vcycle(c, save_tre=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert ca == cm
+    for Lm in range(1, nl + 1):
+        for g in range(m.ngrids(Lm)):
+            assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, "V"), m.field(Lm, g, O.OA_U)), ("u", Lm, g)
+            got, want = ctx.download(PAMR_MGH, Lm, g, "V_tre"), m.field(Lm, g, O.OA_TRE)
+            assert np.array_equal(got, want), ("tre", Lm, g, np.max(np.abs(got - want)))
+    assert any(np.any(m.field(L, g, O.OA_TRE) > 0) for L in range(1, nl + 1) for g in range(m.ngrids(L)))
+    ctx.close()
+
+
+def test_tre_needs_tre_vars(torch_cuda):
+    m, ctx = pair(2)
+    load_sources(m, ctx, 0, 5)
+    ctx.set_params()
+    ctx.build_mgh()
+    assert ctx.L.amrd_vcycle_tre(0, 1) < 0
+    assert ctx.L.amrd_vcycle_tre(0, 0) > 0
+    ctx.close()

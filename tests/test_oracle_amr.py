@@ -173,3 +173,89 @@ def test_owned_segments_partition():
         union |= {(i, j, k) for k in range(b[4], b[5] + 1) for j in range(b[2], b[3] + 1)
                   for i in range(b[0], b[1] + 1)}
     assert set(cover) == union
+
+
+def _lop0(u, rho, h):
+    """interior of Delta_h u - rho (op 0), zero on the boundary"""
+    out = np.zeros_like(u)
+    c = u[1:-1, 1:-1, 1:-1]
+    lap = ((u[2:, 1:-1, 1:-1] - 2 * c + u[:-2, 1:-1, 1:-1]) + (u[1:-1, 2:, 1:-1] - 2 * c + u[1:-1, :-2, 1:-1])
+           + (u[1:-1, 1:-1, 2:] - 2 * c + u[1:-1, 1:-1, :-2])) / (h * h)
+    out[1:-1, 1:-1, 1:-1] = lap - rho[1:-1, 1:-1, 1:-1]
+    return out
+
+
+def _prolong(c):
+    """vertex-centred 2:1 trilinear prolongation"""
+    n = 2 * (c.shape[0] - 1) + 1
+    f = np.zeros((n, n, n))
+    f[::2, ::2, ::2] = c
+    f[1::2, :, :][:, ::2, ::2] = 0.5 * (f[:-1:2, ::2, ::2] + f[2::2, ::2, ::2])
+    f[:, 1::2, :][:, :, ::2] = 0.5 * (f[:, :-1:2, ::2] + f[:, 2::2, ::2])
+    f[:, :, 1::2] = 0.5 * (f[:, :, :-1:2] + f[:, :, 2::2])
+    return f
+
+
+@pytest.mark.parametrize("ntre", [1, 4])
+def test_tre_closed_form(ntre):
+    """compute_MG_rhs's truncation-error estimate (amrd/mg.c:288-295,306-313)
+    on a uniform MG hierarchy 33 -> 17 -> 9 -> 5 with no sweeps (pre = pst =
+    max_citer = 0), so every level keeps u_L = R u_{L+1} after the cycle:
+    level 1 holds |L_1 u_1| sqrt(ntre) (Laplacian minus source, zero on the
+    boundary); every finer level L holds the trilinear interpolation of level
+    L-1's own estimate, which overwrote what compute_MG_rhs(L) wrote there."""
+    m = O.AmrMG(33, [])
+    assert m.nlev == 4
+    rng = np.random.default_rng(3)
+    n = 33
+    x = np.linspace(0, 1, n)
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u0 = np.sin(np.pi * X) * np.sin(2 * np.pi * Y) * np.sin(np.pi * Z) + 0.01 * rng.standard_normal((n, n, n))
+    u0[0], u0[-1], u0[:, 0], u0[:, -1], u0[:, :, 0], u0[:, :, -1] = 0, 0, 0, 0, 0, 0
+    m.field(4, 0, O.OA_U)[:] = u0
+    m.field(4, 0, O.OA_RHO)[:] = -H.gaussian_on((n, n, n), (0, 1, 0, 1, 0, 1), seed=9)
+    m.set_params(pre=0, pst=0, max_citer=0)
+    m.set_tre(True, ntre)
+    m.init()
+    m.vcycle()
+    assert np.array_equal(m.field(4, 0, O.OA_U), u0)
+    own = {}
+    for L in range(1, 4):
+        nL = m.grid(L, 0)[0][0]
+        own[L] = np.abs(_lop0(m.field(L, 0, O.OA_U), m.field(L, 0, O.OA_RHO), 1.0 / (nL - 1))) * np.sqrt(ntre)
+    t1 = m.field(1, 0, O.OA_TRE)
+    assert np.allclose(t1, own[1], rtol=1e-13, atol=0), np.max(np.abs(t1 - own[1]))
+    assert np.max(t1) > 0
+    for L in range(2, 5):
+        want = _prolong(own[L - 1])
+        got = m.field(L, 0, O.OA_TRE)
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-300), (L, np.max(np.abs(got - want)))
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_tre_is_diagnostic(seed):
+    """save_tre changes neither the solution nor the norms; the estimate is
+    non-negative and scales with sqrt(ntre) (x2 for ntre = 4, exactly)."""
+    runs = {}
+    for key, (on, ntre) in {"off": (False, 1), "one": (True, 1), "four": (True, 4)}.items():
+        levels = small_levels(seed)
+        m = O.AmrMG(17, levels)
+        for L in range(m.base, m.nlev + 1):
+            for g in range(m.ngrids(L)):
+                shape, bb = m.grid(L, g)
+                m.field(L, g, O.OA_RHO)[:] = -H.gaussian_on(shape, bb, seed=seed)
+        m.set_params()
+        m.set_tre(on, ntre)
+        m.init()
+        norms = [m.vcycle()[1] for _ in range(3)]
+        u = {(L, g): m.field(L, g, O.OA_U).copy() for L in range(1, m.nlev + 1) for g in range(m.ngrids(L))}
+        t = {(L, g): m.field(L, g, O.OA_TRE).copy() for L in range(1, m.nlev + 1) for g in range(m.ngrids(L))}
+        runs[key] = (norms, u, t)
+    for a, b in zip(runs["off"][0], runs["one"][0]):
+        assert np.array_equal(a, b)
+    for k in runs["off"][1]:
+        assert np.array_equal(runs["off"][1][k], runs["one"][1][k])
+        assert np.all(runs["one"][2][k] >= 0)
+        assert np.array_equal(runs["four"][2][k], 2.0 * runs["one"][2][k])
+        assert not np.any(runs["off"][2][k])
+    assert any(np.any(v > 0) for v in runs["one"][2].values())

@@ -106,6 +106,22 @@ def test_amrd_mg_wiring(L):
     assert L.amrd_mg_def_vars(b"undefined", b"", b"", 0) == 0
 
 
+def test_tre_vars(L):
+    """amrd/io.c:810-838: <v>_tre in the AMR hierarchy with one time level and
+    a second-order regrid transfer; only the first also in the MG hierarchy
+    (where the MG estimate goes)."""
+    assert L.amrd_def_tre_vars(b"phi, psi") == 1
+    _, a = attribs(L, "phi_tre")
+    assert (a["in_amrh"], a["in_mgh"], a["num_tl"], a["mg_noinj_to_amr"], a["regrid_transfer"]) == (1, 1, 1, 1, 2)
+    assert (a["amr_inject"], a["amr_interp"], a["amr_sync"], a["mg_inject"], a["mg_interp"], a["mg_sync"]) == \
+        (0, 0, 0, 0, 0, 0)
+    _, a = attribs(L, "psi_tre")
+    assert (a["in_amrh"], a["in_mgh"], a["num_tl"], a["regrid_transfer"]) == (1, 0, 1, 2)
+    assert L.PAMR_get_gfn(b"phi_tre", MGH, 0) == L.PAMR_get_gfn(b"phi_tre", AMRH, 1) + 1
+    assert L.PAMR_get_gfn(b"psi_tre", MGH, 0) == 0
+    assert L.amrd_def_tre_vars(b"phi") == 0       # phi_tre exists
+
+
 def test_level_spacing(L):
     lam = 0.25
     L.PAMR_set_lambda(lam)
