@@ -51,6 +51,9 @@ double *pmg_amr_field(pmg_amr *A, int L, int g, int which);
 int pmg_amr_commit_u(pmg_amr *A);
 int pmg_amr_upload(pmg_amr *A, int L, int g, int which, const double *host);
 int pmg_amr_download(pmg_amr *A, int L, int g, int which, double *host);
+/* V(pre,pst) cycle parameters; rop_*: restriction ops (1 straight, 2
+ * half-weight, 3 full-weight) of u, u_res and the sources; interp_ord: the
+ * coarse-grid correction's interpolation order (u_res's mg_interp), 2 or 4 */
 int pmg_amr_set_params(pmg_amr *A, int pre, int pst, int max_citer, double crtol, double w0_r,
                        double w0_i, int rop_u, int rop_res, int rop_src, int interp_ord);
 /* TF_MGH_INIT injection into the levels below the base (src/gh.c:1334) */
@@ -64,10 +67,13 @@ int pmg_amr_solve(pmg_amr *A, int min_iter, int max_iter, double tol, double *no
 int pmg_amr_residual(pmg_amr *A, double *norm_out);
 /* single transfers of one field: PAMR_sync(L, AMR_bdy_width = bdy),
  * PAMR_inject(Lf -> Lf-1) with restriction op 1/2/3, PAMR_interp(Lc ->
- * Lc+1) second order */
+ * Lc+1) second order; pmg_amr_interp_ord at order 2 (PAMR_SECOND_ORDER) or
+ * 4 (PAMR_FOURTH_ORDER: dvi4q1d on the transfer's buffered temporary,
+ * src/transfer.c:1007-1051) */
 int pmg_amr_sync(pmg_amr *A, int L, int bdy, int which);
 int pmg_amr_inject(pmg_amr *A, int Lf, int which, int op);
 int pmg_amr_interp(pmg_amr *A, int Lc, int which);
+int pmg_amr_interp_ord(pmg_amr *A, int Lc, int which, int ord);
 /* set_cmask_bdy (which 0) / set_cmask_child (which 1) of grid g as the
  * reference's double mask (1.0 ON, 0.0 OFF), host output */
 int pmg_amr_cmask(pmg_amr *A, int L, int g, int which, double *host_out);
@@ -98,6 +104,8 @@ long long pmg_amr_pool_points(pmg_amr *A, int L);
 int pmg_amr_sync_fields(pmg_amr *A, int L, int bdy, int nf, const int *which);
 int pmg_amr_inject_fields(pmg_amr *A, int Lf, int nf, const int *which, const int *ops);
 int pmg_amr_interp_fields(pmg_amr *A, int Lc, int nf, const int *which);
+/* the same at order ord (2 or 4, otherwise -2) */
+int pmg_amr_interp_fields_ord(pmg_amr *A, int Lc, int ord, int nf, const int *which);
 /* set_cmask_bdy (which 0) / set_cmask_child (which 1) of every grid of L
  * (amrd/mg.c:362-488) written into a field as 1.0 / 0.0; enqueued on
  * pmg_amr_stream */

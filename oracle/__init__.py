@@ -114,6 +114,7 @@ def _declare(L):
     L.or_amr_set_params.argtypes = [vp, i, i, i, d, d, d, i, i, i, i]
     L.or_amr_init.argtypes = [vp]
     L.or_amr_set_tre.argtypes = [vp, i, i]
+    L.or_amr_interp_field_ord.argtypes = [vp, i, i, i]
     L.or_amr_vcycle.argtypes = [vp, _dp, _ip]
     L.or_amr_vcycle.restype = d
     L.or_amr_residual_fine.argtypes = [vp]
@@ -319,8 +320,8 @@ class AmrMG:
     def inject_field(self, lev_fine, which, op=3):
         self.L.or_amr_inject_field(self.h, lev_fine, which, op)
 
-    def interp_field(self, lev_coarse, which):
-        self.L.or_amr_interp_field(self.h, lev_coarse, which)
+    def interp_field(self, lev_coarse, which, ord=2):
+        self.L.or_amr_interp_field_ord(self.h, lev_coarse, which, ord)
 
 
 def net_forward(w1, b1, w2, b2, w3, b3, x):

@@ -802,9 +802,11 @@ void or_amr_inject_field(or_amr *m, int Lf, int which, int op)
   m->tf[which] = op;
   communicate(m, Lf, Lf - 1, 1, 0);
 }
-void or_amr_interp_field(or_amr *m, int Lc, int which)
+void or_amr_interp_field(or_amr *m, int Lc, int which) { or_amr_interp_field_ord(m, Lc, which, 2); }
+
+void or_amr_interp_field_ord(or_amr *m, int Lc, int which, int ord)
 {
   tf_clear(m);
-  m->tf[which] = 2;
+  m->tf[which] = ord;
   communicate(m, Lc, Lc + 1, 0, 1);
 }

@@ -75,6 +75,8 @@ void or_amr_set_cmask(or_amr *m, int L, int which);
 void or_amr_sync_field(or_amr *m, int L, int bdy, int which);
 void or_amr_inject_field(or_amr *m, int Lf, int which, int op);
 void or_amr_interp_field(or_amr *m, int Lc, int which);
+/* PAMR_interp at order ord: 2 (PAMR_SECOND_ORDER) or 4 (PAMR_FOURTH_ORDER) */
+void or_amr_interp_field_ord(or_amr *m, int Lc, int which, int ord);
 
 #ifdef __cplusplus
 }

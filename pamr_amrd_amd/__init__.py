@@ -125,6 +125,8 @@ def _declare(L):
     L.pmg_amr_sync.argtypes = [_vp, i, i, i]
     L.pmg_amr_inject.argtypes = [_vp, i, i, i]
     L.pmg_amr_interp.argtypes = [_vp, i, i]
+    L.pmg_amr_interp_ord.argtypes = [_vp, i, i, i]
+    L.pmg_amr_interp_fields_ord.argtypes = [_vp, i, i, i, _ip]
     L.pmg_amr_cmask.argtypes = [_vp, i, i, i, _dp]
     L.pmg_amr_plan_stats.argtypes = [_vp, i, _ip]
     L.pmg_amr_time_sweep.argtypes = [_vp, i, i, _dp]
@@ -467,8 +469,8 @@ class Amr:
     def inject(self, lev_fine, which, op=3):
         _check(self._L.pmg_amr_inject(self.h, lev_fine, which, op), "pmg_amr_inject")
 
-    def interp(self, lev_coarse, which):
-        _check(self._L.pmg_amr_interp(self.h, lev_coarse, which), "pmg_amr_interp")
+    def interp(self, lev_coarse, which, ord=2):
+        _check(self._L.pmg_amr_interp_ord(self.h, lev_coarse, which, ord), "pmg_amr_interp_ord")
 
     def cmask(self, lev, g, which):
         out = np.zeros(self.grid(lev, g)[0])

@@ -953,13 +953,18 @@ int PAMR_interp(int lc, int tl, int hierarchy) {
   std::vector<int> f, ops;
   selected(f, ops);
   for (int op : ops)
-    if (op != PAMR_SECOND_ORDER) {
+    if (op != PAMR_SECOND_ORDER && op != PAMR_FOURTH_ORDER) {
       std::printf("PAMR_interp: error ... interpolation order %i is not supported\n", op);
       return 0;
     }
-  if (pmg_amr_interp_fields(C->A, Lp, (int)f.size(), f.data()) != 0) {
-    std::printf("PAMR_interp: error ... device transfer failed\n");
-    return 0;
+  for (int ord : {PAMR_SECOND_ORDER, PAMR_FOURTH_ORDER}) {
+    std::vector<int> sel;
+    for (size_t q = 0; q < f.size(); q++)
+      if (ops[q] == ord) sel.push_back(f[q]);
+    if (pmg_amr_interp_fields_ord(C->A, Lp, ord, (int)sel.size(), sel.data()) != 0) {
+      std::printf("PAMR_interp: error ... device transfer failed\n");
+      return 0;
+    }
   }
   return 1;
 }

@@ -49,6 +49,7 @@ struct XJob {
   int sg, dg;          // source grid (source level), destination grid
   int lo[3], hi[3];    // destination index box, 0-based inclusive
   int sh[3];           // copy: src = dst + sh; restrict: fine = 2 dst + sh; interp: F = dst + sh
+  int t0[3], tn[3];    // interp: the buffered temporary's first coarse point (0-based) and point count
 };
 struct XBlk {
   int job, first;
@@ -139,6 +140,41 @@ __global__ __launch_bounds__(kXT) void k_xfer_interp(XFields<T> F, const GridGeo
     auto X = [&](int bb, int c) -> T { return ri == 0 ? R(ai, bb, c) : c0 * R(ai, bb, c) + c1 * R(ai + 1, bb, c); };
     auto Y = [&](int c) -> T { return rj == 0 ? X(aj, c) : c0 * X(aj, c) + c1 * X(aj + 1, c); };
     const T v = rk == 0 ? Y(ak) : c0 * Y(ak) + c1 * Y(ak + 1);
+    F.dst[f][qd] = v;
+    if (F.alt[f]) F.alt[f][qd] = v;
+  }
+}
+
+// interp at fourth order: dminterp3d ord 4 (src/m_util_r8.f:473-637, dvi4q1d
+// :65-137) on the transfer's buffered temporary (coarse points t0 .. t0+tn-1
+// per axis, src/transfer.c:1007-1051), whose ends take the one-sided cells,
+// evaluated per fine point in the reference's phase order: x on
+// coarse-anchored lines, y for every x, then z; anchors are copied, never
+// re-evaluated
+template <typename T>
+__global__ __launch_bounds__(kXT) void k_xfer_interp4(XFields<T> F, const GridGeo *gs, const GridGeo *gd,
+                                                      const XJob *jobs, const XBlk *blks) {
+  const XBlk b = blks[blockIdx.x];
+  const XJob J = jobs[b.job];
+  int i, j, k;
+  if (!job_point(J, b.first + threadIdx.x, i, j, k)) return;
+  const GridGeo S = gs[J.sg], D = gd[J.dg];
+  const size_t qd = D.off + ix3(i, j, k, D.nx, D.ny);
+  const int fx = i + J.sh[0] - 2 * J.t0[0], fy = j + J.sh[1] - 2 * J.t0[1], fz = k + J.sh[2] - 2 * J.t0[2];
+  for (int f = 0; f < F.nf; f++) {
+    const T *uc = F.src[f] + S.off;
+    auto line = [&](int d, int fd, const auto &A) -> T {
+      if ((fd & 1) == 0) return A(fd >> 1);
+      return vi4_point(J.tn[d], 2, 1, 2 * J.tn[d] - 1, fd, A);
+    };
+    auto Y = [&](int c) -> T {
+      auto X = [&](int bb) -> T {
+        auto R = [&](int a) -> T { return uc[ix3(J.t0[0] + a, J.t0[1] + bb, J.t0[2] + c, S.nx, S.ny)]; };
+        return line(0, fx, R);
+      };
+      return line(1, fy, X);
+    };
+    const T v = line(2, fz, Y);
     F.dst[f][qd] = v;
     if (F.alt[f]) F.alt[f][qd] = v;
   }
@@ -554,6 +590,10 @@ int make_jobs(const pmg_amr *A, int l1, int l2, int interior, int extend, std::v
           // b2 - 1 + (q - (ibd - 1)) from the coarse grid's first point
           const int b2 = (int)((ss.bb[2 * i] - ga.bbox[2 * i]) / dxd[i] + 1.5);
           J.sh[i] = b2 - ibd[2 * i];
+          // the temporary: interp_buffer = 2 coarse points either side, clipped
+          const int b2i = std::max(1, (((b2 - 1) / irho) - 2) * irho + 1), b1i = (b2i - 1) / irho + 1;
+          J.t0[i] = b1i - 1;
+          J.tn[i] = std::min(ga.n[i], (b2 + cnt - 1) / irho + 1 + 2 + 1) - b1i + 1;
         }
       }
       if (empty) continue;  // the Fortran loops run zero times
@@ -566,6 +606,9 @@ int make_jobs(const pmg_amr *A, int l1, int l2, int interior, int extend, std::v
         if (delta == -1) {
           const int f0 = J.lo[i] + J.sh[i], f1 = J.hi[i] + J.sh[i];
           if (f0 < 0 || (f1 >> 1) + ((f1 & 1) ? 1 : 0) >= ns) return -7;
+          if (J.t0[i] < 0 || J.tn[i] < 1 || J.t0[i] + J.tn[i] > ns || f0 - 2 * J.t0[i] < 0 ||
+              f1 - 2 * J.t0[i] > 2 * (J.tn[i] - 1))
+            return -7;
         }
       }
       jobs.push_back(J);
@@ -799,7 +842,8 @@ bool valid_field(const pmg_amr *A, int which) {
   return (which >= 0 && which <= PMG_F_C) || (which >= PMG_F_EXTRA && which < PMG_F_EXTRA + A->nextra);
 }
 
-// one transfer plan with up to kMaxFields fields; kind 0 copy, 1-3 restrict, 4 interp
+// one transfer plan with up to kMaxFields fields; kind 0 copy, 1-3 restrict,
+// 4 interp (second order), 5 interp (fourth order)
 int run_plan(pmg_amr *A, const XPlan &P, int Ls, int Ld, int kind, const int *which, int nf) {
   if (P.nblocks == 0 || nf == 0) return 0;
   ALevel &S = A->lev[Ls - 1], &D = A->lev[Ld - 1];
@@ -821,6 +865,7 @@ int run_plan(pmg_amr *A, const XPlan &P, int Ls, int Ld, int kind, const int *wh
     case 2: hipLaunchKernelGGL((k_xfer_restrict<double, 2>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
     case 3: hipLaunchKernelGGL((k_xfer_restrict<double, 3>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
     case 4: hipLaunchKernelGGL(k_xfer_interp<double>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
+    case 5: hipLaunchKernelGGL(k_xfer_interp4<double>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
     default: return -1;
   }
   PMG_HOST_CHECK(hipGetLastError());
@@ -978,12 +1023,12 @@ int lop_rhs(pmg_amr *A, int L) {
 // u += res on L (whole grids, both ping-pong buffers)
 int cgc(pmg_amr *A, int L) {
   ALevel &C = A->lev[L - 2], &F = A->lev[L - 1];
-  if (A->iord != 2) return -4;
+  if (A->iord != 2 && A->iord != 4) return -4;
   hipLaunchKernelGGL(k_cgc_diff<double>, dim3(nblk(C.pool)), dim3(256), 0, A->stream, C.res, C.u, C.lop, A->w0_i,
                      C.pool);
   PMG_HOST_CHECK(hipGetLastError());
   const int w = PMG_F_RES;
-  int rc = run_plan(A, F.interp, L - 1, L, 4, &w, 1);
+  int rc = run_plan(A, F.interp, L - 1, L, A->iord == 4 ? 5 : 4, &w, 1);
   if (rc) return rc;
   hipLaunchKernelGGL(k_add_alt<double>, dim3(nblk(F.pool)), dim3(256), 0, A->stream, F.u, F.ua, F.res, F.pool);
   PMG_HOST_CHECK(hipGetLastError());
@@ -1308,7 +1353,7 @@ int pmg_amr_set_params(pmg_amr *A, int pre, int pst, int max_citer, double crtol
   if (!A || pre < 0 || pst < 0 || max_citer < 0) return -1;
   for (int r : {rop_u, rop_res, rop_src})
     if (r < 0 || r > 3) return -1;
-  if (interp_ord != 2) return -2;
+  if (interp_ord != 2 && interp_ord != 4) return -2;
   A->pre = pre; A->pst = pst; A->max_citer = max_citer; A->crtol = crtol;
   A->w0_r = w0_r; A->w0_i = w0_i; A->rop_u = rop_u; A->rop_res = rop_res; A->rop_src = rop_src;
   A->iord = interp_ord;
@@ -1394,9 +1439,12 @@ int pmg_amr_inject(pmg_amr *A, int Lf, int which, int op) {
   return 0;
 }
 
-int pmg_amr_interp(pmg_amr *A, int Lc, int which) {
+int pmg_amr_interp(pmg_amr *A, int Lc, int which) { return pmg_amr_interp_ord(A, Lc, which, 2); }
+
+int pmg_amr_interp_ord(pmg_amr *A, int Lc, int which, int ord) {
   if (!A || Lc < 1 || Lc >= A->nlev || !valid_field(A, which)) return -1;
-  int rc = run_plan(A, A->lev[Lc].interp, Lc, Lc + 1, 4, &which, 1);
+  if (ord != 2 && ord != 4) return -2;
+  int rc = run_plan(A, A->lev[Lc].interp, Lc, Lc + 1, ord == 4 ? 5 : 4, &which, 1);
   if (rc) return rc;
   PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
   return 0;
@@ -1557,9 +1605,14 @@ int pmg_amr_fill_cmask(pmg_amr *A, int L, int which, int field) {
 }
 
 int pmg_amr_interp_fields(pmg_amr *A, int Lc, int nf, const int *which) {
+  return pmg_amr_interp_fields_ord(A, Lc, 2, nf, which);
+}
+
+int pmg_amr_interp_fields_ord(pmg_amr *A, int Lc, int ord, int nf, const int *which) {
   if (!A || Lc < 1 || Lc >= A->nlev || nf < 0 || (nf > 0 && !which)) return -1;
+  if (ord != 2 && ord != 4) return -2;
   for (int f = 0; f < nf; f++) if (!valid_field(A, which[f])) return -1;
-  return run_grouped(A, A->lev[Lc].interp, Lc, Lc + 1, 4, nf, which);
+  return run_grouped(A, A->lev[Lc].interp, Lc, Lc + 1, ord == 4 ? 5 : 4, nf, which);
 }
 
 }  // extern "C"

@@ -91,41 +91,7 @@ __global__ __launch_bounds__(NT) void k_interp3d(const T *u1, T *u2, int nx1, in
 }
 
 // ---- fourth-order interpolation (dminterp3d, ord = 4, no excision) ------
-// One fine point of dvi4q1d (src/m_util_r8.f:65-137) on a line whose anchors
-// are A(a), a = -1 .. n, at the 1-based line positions p0 + a*rho of a line
-// of N points; f = a*rho + r (0 < r < rho) is the point's offset from the
-// first anchor.  The first cell uses anchors 0..3 when p0 <= rho (no anchor
-// to its left), the last cell anchors n-4..n-1 when p0 + (n+2)*rho > N,
-// every other cell a-1..a+2; n < 4 falls back to dvi2q1d (:332-360).  The
-// literals are the reference's 0.1666 6666 6666 6667 d0 (16 digits: fixed-
-// form blanks are not significant) and the operations keep its order.
-template <typename F>
-__device__ __forceinline__ double vi4_point(int n, int rho, int p0, int N, int f, const F &A) {
-  const int a = f / rho, r = f % rho;
-  if (n < 4) {
-    const double sg = (r * 1.0) / rho;
-    const double c0 = 1 - sg, c1 = sg;
-    return c0 * A(a) + c1 * A(a + 1);
-  }
-  const double half = 0.5, mhalf = -0.5, sixth = 0.1666666666666667, msixth = -0.1666666666666667;
-  double sg;
-  int b;
-  if (p0 <= rho && a == 0) {
-    sg = (r * 1.0) / rho;
-    b = 0;
-  } else if (p0 + (n + 2) * rho > N && a == n - 2) {
-    sg = 2.0 + (r * 1.0) / rho;
-    b = n - 4;
-  } else {
-    sg = 1.0 + (r * 1.0) / rho;
-    b = a - 1;
-  }
-  const double c0 = msixth * (sg - 1.0) * (sg - 2.0) * (sg - 3.0);
-  const double c1 = half * sg * (sg - 2.0) * (sg - 3.0);
-  const double c2 = mhalf * sg * (sg - 1.0) * (sg - 3.0);
-  const double c3 = sixth * sg * (sg - 1.0) * (sg - 2.0);
-  return c0 * A(b) + c1 * A(b + 1) + c2 * A(b + 2) + c3 * A(b + 3);
-}
+// (vi4_point, one point of dvi4q1d, is in pmg_ops.h)
 
 // The three line phases of dminterp3d (src/m_util_r8.f:518-634), one launch
 // each, one thread per written point.  A phase writes only points off the

@@ -84,11 +84,33 @@ def test_transfers_bit_exact(torch_cuda, seed):
             m.inject_field(lf, O.OA_RES, op)
             a.inject(lf, pm.PMG_F_RES, op)
             compare(m, a, [lf - 1], O.OA_RES)
-    for lc in (m.base, m.base + 1):
-        fill_random(m, a, [lc, lc + 1], O.OA_RES, rng)
-        m.interp_field(lc, O.OA_RES)
-        a.interp(lc, pm.PMG_F_RES)
-        compare(m, a, [lc + 1], O.OA_RES)
+    for ordr in (2, 4):
+        for lc in (m.base - 1, m.base, m.base + 1):
+            fill_random(m, a, [lc, lc + 1], O.OA_RES, rng)
+            m.interp_field(lc, O.OA_RES, ord=ordr)
+            a.interp(lc, pm.PMG_F_RES, ord=ordr)
+            compare(m, a, [lc + 1], O.OA_RES)
+    a.close()
+
+
+@pytest.mark.parametrize("seed,op", [(6, 0), (7, 1)])
+def test_vcycles_fourth_order_cgc_bit_exact(torch_cuda, seed, op):
+    """the coarse-grid correction interpolated at fourth order (u_res's
+    mg_interp = PAMR_FOURTH_ORDER): V-cycles bit for bit"""
+    m, a = pair(seed, op=op)
+    set_sources(m, a, op, 4321 + seed)
+    for x in (m, a):
+        x.set_params(interp_ord=4)
+        x.init()
+    for c in range(4):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = a.vcycle()
+        print(f"cycle {c}: oracle {rm:.15e} gpu {ra:.15e}", flush=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert cm == ca
+        assert np.allclose(na, nm, rtol=1e-12, atol=0), (c, na, nm)
+    for L, g in each_grid(m):
+        assert np.array_equal(a.download(L, g, pm.PMG_F_U), m.field(L, g, O.OA_U)), ("u", L, g)
     a.close()
 
 

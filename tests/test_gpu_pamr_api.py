@@ -144,14 +144,15 @@ def test_transfer_entry_points(torch_cuda, seed):
             bit(O.OA_RES, op)
             assert L.PAMR_inject(lf, 0, PAMR_MGH) == 1
             compare([lf - 1], O.OA_RES)
-    for lc in (m.base, m.base + 1):
-        fill([lc, lc + 1], O.OA_RES)
-        m.interp_field(lc, O.OA_RES)
-        bit(O.OA_RES, 2)
-        assert L.PAMR_interp(lc, 0, PAMR_MGH) == 1
-        compare([lc + 1], O.OA_RES)
-    bit(O.OA_RES, 4)
-    assert L.PAMR_interp(m.base, 0, PAMR_MGH) == 0              # fourth order: not supported
+    for ordr in (2, 4):                                         # PAMR_SECOND_ORDER, PAMR_FOURTH_ORDER
+        for lc in (m.base, m.base + 1):
+            fill([lc, lc + 1], O.OA_RES)
+            m.interp_field(lc, O.OA_RES, ord=ordr)
+            bit(O.OA_RES, ordr)
+            assert L.PAMR_interp(lc, 0, PAMR_MGH) == 1
+            compare([lc + 1], O.OA_RES)
+    bit(O.OA_RES, 3)
+    assert L.PAMR_interp(m.base, 0, PAMR_MGH) == 0              # no third-order interpolation
     L.PAMR_thaw_tf_bits()
     ctx.close()
 

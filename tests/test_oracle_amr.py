@@ -119,6 +119,31 @@ def test_interp_exact_for_trilinear(seed):
             assert np.array_equal(got, poly_trilinear(*H.grid_points(shape, bb)))
 
 
+def poly_tricubic(X, Y, Z):
+    return (1.0 + X - 2.0 * X ** 2 + X ** 3) * (2.0 - Y + 0.5 * Y ** 3) * (1.0 + Z ** 2 - Z ** 3)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_interp4_exact_for_tricubic(seed):
+    """PAMR_interp at PAMR_FOURTH_ORDER (dvi4q1d on the buffered temporary
+    of every transfer): a tensor-product cubic on the coarse grids is
+    reproduced at every fine point to rounding (the coefficients hold 1/6)."""
+    m = O.AmrMG(17, small_levels(seed))
+    for Lc in (m.base, m.base + 1):
+        for g in range(m.ngrids(Lc)):
+            shape, bb = m.grid(Lc, g)
+            m.field(Lc, g, O.OA_RES)[:] = poly_tricubic(*H.grid_points(shape, bb))
+        for g in range(m.ngrids(Lc + 1)):
+            m.field(Lc + 1, g, O.OA_RES)[:] = np.nan
+        m.interp_field(Lc, O.OA_RES, ord=4)
+        for g in range(m.ngrids(Lc + 1)):
+            shape, bb = m.grid(Lc + 1, g)
+            got = m.field(Lc + 1, g, O.OA_RES)
+            want = poly_tricubic(*H.grid_points(shape, bb))
+            assert not np.isnan(got).any(), "fine point not reached by interp"
+            assert np.allclose(got, want, rtol=0, atol=1e-13), np.max(np.abs(got - want))
+
+
 @pytest.mark.parametrize("seed", [1, 2, 3])
 @pytest.mark.parametrize("op", [1, 2, 3])
 def test_inject_exact_for_affine(seed, op):
