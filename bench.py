@@ -40,8 +40,10 @@ METRIC = "time to 1e-10 rel. residual per 257^3 solve; learned+FP32 vs plain FP6
 HBM_PEAK_GBS = 8000.0
 
 
-SWEEP_SHAPES = {0: (64, 8, 256), 1: (64, 16, 512), 2: (32, 16, 512), 3: (128, 8, 512), 4: (64, 32, 512),
-                5: (128, 16, 1024)}
+# tile variants of the fused sweep: (TX, TY, threads, u planes in flight)
+SWEEP_SHAPES = {0: (64, 8, 256, 1), 1: (64, 16, 512, 1), 2: (32, 16, 512, 1), 3: (128, 8, 512, 1),
+                4: (64, 32, 512, 1), 5: (128, 16, 1024, 1), 6: (128, 16, 1024, 2), 7: (64, 16, 512, 2),
+                8: (128, 8, 512, 2)}
 
 
 def measured_traffic(n, variant, prec="double"):
@@ -51,8 +53,8 @@ def measured_traffic(n, variant, prec="double"):
     path = os.path.join(ROOT, "profiles", f"traffic_finest_{n}.json")
     if not os.path.exists(path) or variant not in SWEEP_SHAPES:
         return None
-    tx, ty, nt = SWEEP_SHAPES[variant]
-    name = f"pmg::k_sweep<{prec}, 0, false, true, true, {tx}, {ty}, {nt}>"
+    tx, ty, nt, pf = SWEEP_SHAPES[variant]
+    name = f"pmg::k_sweep<{prec}, 0, false, true, true, {tx}, {ty}, {nt}, {pf}>"
     with open(path) as f:
         entry = json.load(f).get("kernels", {}).get(name)
     return None if entry is None else entry["traffic_bytes"]

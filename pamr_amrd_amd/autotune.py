@@ -15,7 +15,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 TABLE = os.path.join(HERE, "tuning.json")
 DEFAULT = (1, 32)
 VARIANTS = {0: (64, 8, 256), 1: (64, 16, 512), 2: (32, 16, 512), 3: (128, 8, 512), 4: (64, 32, 512),
-            5: (128, 16, 1024)}
+            5: (128, 16, 1024), 6: (128, 16, 1024, "2 planes in flight"), 7: (64, 16, 512, "2 planes in flight"),
+            8: (128, 8, 512, "2 planes in flight")}
 
 
 def load_table():

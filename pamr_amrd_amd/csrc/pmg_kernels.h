@@ -196,8 +196,12 @@ __device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX
 // interior, planes [z0, z1).  `bid` is the tile's slot in the norm
 // workspace.  Shared by the single-grid and the batched (all grids of a
 // level) kernels.
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
+// PF: u planes in flight ahead of the march (1: one plane issued per step and
+// committed the next; 2: two, for latency-bound tiles such as FP32).  Planes
+// past z1+1, which no pass of the chunk reads, are not fetched.
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, int PF = 1>
 __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, int z0, int z1, int bid) {
+  static_assert(PF == 1 || PF == 2, "prefetch depth");
   constexpr int LX = TX + 4, LY = TY + 4;
   constexpr int NLD = (LX * LY + NT - 1) / NT;
   constexpr int N0 = (((TX + 3) / 2) * (TY + 2) + NT - 1) / NT;
@@ -217,19 +221,22 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       if (i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = uin[ix3(i, j, k, nx, ny)];
     }
   };
-  // u planes are prefetched two steps ahead into alternating register sets
-  T ldA[NLD], ldB[NLD];
+  // u planes are prefetched into rotating register sets
+  T ldA[NLD], ldB[NLD], ldC[NLD];
+  // the last plane the chunk reads: pass 0 also runs on plane z1 (pass 1 on
+  // z1-1 needs its updated points) and reads z1+1
+  const int kmax = min(z1 + 1, nz - 1);
   auto load_issue = [&](T(&ld)[NLD], int k) {
 #pragma unroll
     for (int m = 0; m < NLD; m++) {
       const int q = threadIdx.x + m * NT;
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      if (k < nz && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) ld[m] = uin[ix3(i, j, k, nx, ny)];
+      if (k <= kmax && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) ld[m] = uin[ix3(i, j, k, nx, ny)];
     }
   };
   auto load_commit = [&](const T(&ld)[NLD], int k) {
-    if (k >= nz) return;
+    if (k > kmax) return;
     T(*S)[LX] = sU[k & 3];
 #pragma unroll
     for (int m = 0; m < NLD; m++) {
@@ -261,6 +268,7 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   __syncthreads();
   load_direct(z0 + 2);
   load_issue(ldA, z0 + 3);
+  if constexpr (PF == 2) load_issue(ldB, z0 + 4);
   SlotTab<N0> tab0;
   SlotTab<N1> tab1;
   slot_table<T, TX, TY, NT, true, N0>(P, x0, y0, 0, tab0);
@@ -270,10 +278,10 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   phase_prefetch_t<T, EX, RHS0, N0>(P, z0 + 1, tab0, cur0);
   phase_prefetch_t<T, EX, RHS0, N1>(P, z0, tab1, cur1);
   __syncthreads();
-  // step k: issue plane k+4 and the next step's rho/rhs; pass 0 on k+1;
-  // pass 1 on k; store k; commit plane k+3 (issued one step earlier)
+  // step k: issue plane k+3+PF and the next step's rho/rhs; pass 0 on k+1;
+  // pass 1 on k; store k; commit plane k+3 (issued PF steps earlier)
   auto step = [&](int k, const T(&ldc)[NLD], T(&ldi)[NLD]) {
-    load_issue(ldi, k + 4);
+    load_issue(ldi, k + 3 + PF);
     phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0);
     phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1);
     phase_run<T, OP, P2, RHS0, TX, TY, N0>(P, sU, k + 1, cur0, k + 1 < z1, s, cnt);
@@ -286,21 +294,31 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
     cur1 = nxt1;
     __syncthreads();
   };
-  for (int k = z0; k < z1; k += 2) {
-    step(k, ldA, ldB);
-    if (k + 1 < z1) step(k + 1, ldB, ldA);
+  if constexpr (PF == 1) {
+    for (int k = z0; k < z1; k += 2) {
+      step(k, ldA, ldB);
+      if (k + 1 < z1) step(k + 1, ldB, ldA);
+    }
+  } else {
+    // set (k - z0) % 3 holds plane k+3; the set committed one step earlier
+    // takes plane k+5
+    for (int k = z0; k < z1; k += 3) {
+      step(k, ldA, ldC);
+      if (k + 1 < z1) step(k + 1, ldB, ldA);
+      if (k + 2 < z1) step(k + 2, ldC, ldB);
+    }
   }
   norm_partial<NT>(s, cnt, P.ws, P.slot, bid, P.nb);
 }
 
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT>
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, int PF = 1>
 __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int nty = (P.ny - 2 + TY - 1) / TY, ntx = (P.nx - 2 + TX - 1) / TX;
   const int ty = tile % nty, tx = (tile / nty) % ntx, tz = tile / (nty * ntx);
   const int z0 = 1 + tz * P.zchunk;
-  sweep_tile<T, OP, EX, P2, RHS0, TX, TY, NT>(P, 1 + tx * TX, 1 + ty * TY, z0, min(z0 + P.zchunk, P.nz - 1),
-                                              blockIdx.x);
+  sweep_tile<T, OP, EX, P2, RHS0, TX, TY, NT, PF>(P, 1 + tx * TX, 1 + ty * TY, z0, min(z0 + P.zchunk, P.nz - 1),
+                                                  blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

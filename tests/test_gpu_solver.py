@@ -36,13 +36,15 @@ def _pair(pm, n, rho, op=0, a=None, c=None, pre=2, pst=2, variant=1, zchunk=32, 
     return S, m
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
-def test_c1_ten_cycles_65(pm, variant):
-    """65^3 Poisson, seeded Gaussian source, V(2,2), FW, 10 cycles."""
-    print(f"case c1 65^3 variant={variant}", flush=True)
+@pytest.mark.parametrize("variant,zchunk", [(v, 32) for v in range(9)] + [(6, 5), (7, 4), (8, 7)])
+def test_c1_ten_cycles_65(pm, variant, zchunk):
+    """65^3 Poisson, seeded Gaussian source, V(2,2), FW, 10 cycles, every
+    sweep tile variant (z-chunks that leave every remainder of the
+    three-step march of the two-planes-in-flight variants)."""
+    print(f"case c1 65^3 variant={variant} zchunk={zchunk}", flush=True)
     n = 65
     rho = -pm.gaussian_source(n, seed=1234)
-    S, m = _pair(pm, n, rho, variant=variant)
+    S, m = _pair(pm, n, rho, variant=variant, zchunk=zchunk)
     assert S.nlev == m.nlev == 5
     for cyc in range(10):
         r_gpu, ln_gpu, ci_gpu = S.vcycle()
