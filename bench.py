@@ -390,8 +390,10 @@ def main():
     variant, zchunk = args.variant, args.zchunk
     if variant >= 0 and zchunk >= 0:
         S.set_level_sweep_config(S.nlev, variant, zchunk)
+        variant32, zchunk32 = variant, zchunk
     else:
         variant, zchunk = pm.tuned_sweep_config(n)
+        variant32, zchunk32 = pm.tuned_sweep_config(n, "f32")
     S.upload(S.nlev, pm.PMG_F_RHO, rho)
     S.init()
     Lf = S.nlev
@@ -439,7 +441,8 @@ def main():
             "kernel": "k_sweep<double> finest level (fused red-black sweep, 255^3 interior)",
             "algorithmic_bytes": bytes64, "avg_ms": round(t64, 5)}
     roof32 = {"bound": "hbm", "achieved": round(ach32, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-              "frac": round(ach32 / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n, variant, "float"),
+              "frac": round(ach32 / HBM_PEAK_GBS, 4),
+              "traffic": measured_traffic(n, variant32, "float"), "sweep_variant": variant32, "zchunk": zchunk32,
               "algorithmic_bytes": bytes32,
               "avg_ms": round(t32, 5)}
     cpu = None

@@ -159,6 +159,9 @@ int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double 
 /* execution tuning: sweep tile variant and z chunk; graph capture on/off */
 int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk);
 int pmg_set_level_sweep_config(pmg_solver *S, int L, int variant, int zchunk);
+/* the same for one precision only: prec 0 the FP64 sweeps, 1 the FP32 sweeps
+ * of the mixed-precision inner cycles */
+int pmg_set_level_sweep_config_prec(pmg_solver *S, int L, int prec, int variant, int zchunk);
 int pmg_num_sweep_variants(void);
 int pmg_set_graph(pmg_solver *S, int on);
 /* run the small bottom levels (all dimensions <= 17 points, below the

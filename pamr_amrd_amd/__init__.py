@@ -78,6 +78,7 @@ def _declare(L):
     L.pmg_time_kernel.argtypes = [_vp, i, i, i, i, _dp]
     L.pmg_set_sweep_config.argtypes = [_vp, i, i]
     L.pmg_set_level_sweep_config.argtypes = [_vp, i, i, i]
+    L.pmg_set_level_sweep_config_prec.argtypes = [_vp, i, i, i, i]
     L.pmg_num_sweep_variants.argtypes = []
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_set_bottom_fusion.argtypes = [_vp, i]
@@ -251,13 +252,16 @@ class Solver:
         self.nlev = L.pmg_nlev(self.h)
         self.apply_tuning()
 
-    def apply_tuning(self, prec="f64"):
-        """Per-level sweep tile/z-chunk from the autotuner table (tuning.json)."""
+    def apply_tuning(self):
+        """Per-level sweep tile/z-chunk of the FP64 and the FP32 sweeps from
+        the autotuner table (tuning.json)."""
         from .autotune import tuned_level_config
         for lev in range(1, self.nlev + 1):
-            cfg = tuned_level_config(self.shape(lev)[2], prec)
-            if cfg is not None:
-                self.set_level_sweep_config(lev, *cfg)
+            for pi, prec in enumerate(("f64", "f32")):
+                cfg = tuned_level_config(self.shape(lev)[2], prec)
+                if cfg is not None:
+                    _check(self._L.pmg_set_level_sweep_config_prec(self.h, lev, pi, *cfg),
+                           "pmg_set_level_sweep_config_prec")
 
     def set_level_sweep_config(self, lev, variant, zchunk):
         _check(self._L.pmg_set_level_sweep_config(self.h, lev, variant, zchunk),

@@ -129,7 +129,7 @@ struct pmg_solver {
   int *d_citer = nullptr;     // 2 ints (FP64 cycle, FP32 cycle)
   double *h_pinned = nullptr; // pinned readback
   int sweep_variant = 1, zchunk = 32;
-  int lv_variant[kMaxLev], lv_zchunk[kMaxLev];  // per-level sweep configuration
+  int lv_variant[2][kMaxLev], lv_zchunk[2][kMaxLev];  // per-level sweep configuration, [0] FP64, [1] FP32
   int use_graph = 1;
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   int use_fused_restrict = 1;  // finest residual + injection in one pass
@@ -319,8 +319,9 @@ int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
   P.bx0 = Lv.rbox.lo[0]; P.bx1 = Lv.rbox.hi[0];
   P.by0 = Lv.rbox.lo[1]; P.by1 = Lv.rbox.hi[1];
   P.bz0 = Lv.rbox.lo[2]; P.bz1 = Lv.rbox.hi[2];
-  P.zchunk = S->lv_zchunk[L - 1];
-  const int variant = S->lv_variant[L - 1];
+  constexpr int pi = std::is_same<T, float>::value ? 1 : 0;
+  P.zchunk = S->lv_zchunk[pi][L - 1];
+  const int variant = S->lv_variant[pi][L - 1];
   P.sp = spacing<T>(Lv);
   P.ws = S->ws;
   P.slot = slot;
@@ -860,8 +861,10 @@ pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidt
   }
   for (int L = 1; L <= nl; L++) {
     const int n = S->lev[L - 1].n[2];
-    S->lv_variant[L - 1] = 1;
-    S->lv_zchunk[L - 1] = n >= 257 ? 32 : (n >= 129 ? 16 : (n >= 65 ? 8 : 4));
+    for (int pi = 0; pi < 2; pi++) {
+      S->lv_variant[pi][L - 1] = 1;
+      S->lv_zchunk[pi][L - 1] = n >= 257 ? 32 : (n >= 129 ? 16 : (n >= 65 ? 8 : 4));
+    }
   }
   // norm workspace sized for the largest launch
   // partial-norm capacity: the largest launch over the finest level
@@ -942,18 +945,26 @@ int pmg_set_params(pmg_solver *S, int pre, int pst, int max_citer, double crtol,
 int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk) {
   // the same tile variant and z chunk for every level
   if (!S || variant < 0 || variant >= kNumSweepVariants || zchunk < 1) return -1;
-  for (int L = 0; L < S->nlev; L++) {
-    S->lv_variant[L] = variant;
-    S->lv_zchunk[L] = zchunk;
-  }
+  for (int L = 0; L < S->nlev; L++)
+    for (int pi = 0; pi < 2; pi++) {
+      S->lv_variant[pi][L] = variant;
+      S->lv_zchunk[pi][L] = zchunk;
+    }
   invalidate_graphs(S);
   return 0;
 }
 
 int pmg_set_level_sweep_config(pmg_solver *S, int L, int variant, int zchunk) {
-  if (!S || L < 1 || L > S->nlev || variant < 0 || variant >= kNumSweepVariants || zchunk < 1) return -1;
-  S->lv_variant[L - 1] = variant;
-  S->lv_zchunk[L - 1] = zchunk;
+  int rc = pmg_set_level_sweep_config_prec(S, L, 0, variant, zchunk);
+  return rc ? rc : pmg_set_level_sweep_config_prec(S, L, 1, variant, zchunk);
+}
+
+int pmg_set_level_sweep_config_prec(pmg_solver *S, int L, int prec, int variant, int zchunk) {
+  if (!S || L < 1 || L > S->nlev || prec < 0 || prec > 1 || variant < 0 || variant >= kNumSweepVariants ||
+      zchunk < 1)
+    return -1;
+  S->lv_variant[prec][L - 1] = variant;
+  S->lv_zchunk[prec][L - 1] = zchunk;
   invalidate_graphs(S);
   return 0;
 }
