@@ -49,6 +49,35 @@ void amrd_set_app_MG_level_hooks(amrd_level_relax_hook relax, amrd_level_real_ho
 int amrd_mg_def_vars(const char *elliptic_vars, const char *mg_fw_restr, const char *mg_interp2,
                      int do_ex);
 
+/* MG extrapolation in time for constrained evolution.
+ *
+ * amrd_mg_def_extrap_vars (amrd/io.c:898-934): for every elliptic variable v
+ * of amrd_mg_def_vars, "<v>_brs", "<v>_extrap_tm1" and "<v>_extrap_tm2" in
+ * the AMR hierarchy with one time level (the extrapolation variables
+ * straight-injected, fourth-order interpolated and regridded, second order
+ * when listed in amr_interp2 / amr_transfer2).  v itself needs two AMR time
+ * levels.  Call after amrd_mg_def_vars, before PAMR_compose_hierarchy.
+ *
+ * amrd_mg_extrap (amrd/evolve.c:346-368): on AMR level lev > 1 (level 1 is
+ * left alone), v(tl 1) = f1 tm1 + f2 tm2 with f1 = (tnp1 - (tnp1_lm1 -
+ * 2 dt_lm1)) / dt_lm1, f2 = -(tnp1 - (tnp1_lm1 - dt_lm1)) / dt_lm1 (method
+ * 0, AMRD_MG_EXTRAP_2ND_FROM_ETM1), or v(tl 1) = v(tl 2) + dt/dt_lm1 (tm2 -
+ * tm1) (method 1, AMRD_MG_EXTRAP_2ND_FROM_TN).
+ *
+ * amrd_set_mg_brs_vars (amrd/evolve.c:1624-1639): brs = v(tl 2) on levels
+ * L1..L2.
+ *
+ * amrd_update_mg_extrap_vars (amrd/evolve.c:1661-1702): on levels L1..Lf
+ * (L1+1..Lf when L1 > 1), tm2 = tm1, tm1 = v(tl 2), and on levels > 1 with
+ * eps_c != 0 (AMRD_MG_eps_c) tm2 += (tm1 - brs) (1 - eps_c / net_rho), net_rho
+ * the time refinement product the reference accumulates.
+ *
+ * All return 1 / 0 and enqueue on amrd_gpu_stream(). */
+int amrd_mg_def_extrap_vars(const char *amr_interp2, const char *amr_transfer2);
+int amrd_mg_extrap(int lev, int method, double tnp1, double dt, double tnp1_lm1, double dt_lm1);
+int amrd_set_mg_brs_vars(int L1, int L2);
+int amrd_update_mg_extrap_vars(int L1, int Lf, double eps_c);
+
 /* AMRD's MG parameters (amrd/io.c:97-106,269-285); defaults 3, 3, 1, 50,
  * 0.0, 1e-3, 5000, 1.0, 1.0 */
 void amrd_mg_set_params(int pre_swp, int pst_swp, int min_iter, int max_iter, double tol,

@@ -324,6 +324,40 @@ class AmrMG:
         self.L.or_amr_interp_field_ord(self.h, lev_coarse, which, ord)
 
 
+# --------------------------------------------------------------------------
+# MG extrapolation in time (numpy restatement, elementwise FP64)
+# --------------------------------------------------------------------------
+def mg_extrap(v2, tm1, tm2, method, tnp1, dt, tnp1_lm1, dt_lm1):
+    """v(tl 1) of an elliptic variable on a level > 1 (amrd/evolve.c:346-368):
+    method 0 (AMRD_MG_EXTRAP_2ND_FROM_ETM1) the line through tm1 at
+    tnp1_lm1 - dt_lm1 and tm2 at tnp1_lm1 - 2 dt_lm1, evaluated at tnp1;
+    method 1 (2ND_FROM_TN) v2 + dt/dt_lm1 (tm2 - tm1)."""
+    if method == 0:
+        f1 = (tnp1 - (tnp1_lm1 - 2 * dt_lm1)) / dt_lm1
+        f2 = -(tnp1 - (tnp1_lm1 - dt_lm1)) / dt_lm1
+        return f1 * tm1 + f2 * tm2
+    return v2 + (dt / dt_lm1) * (tm2 - tm1)
+
+
+def update_mg_extrap_vars(levels, L1, Lf, eps_c, rho_tm):
+    """amrd/evolve.c:1661-1702 over ``levels[lev] = {"tm1", "tm2", "v2",
+    "brs"}`` (numpy arrays, updated in place); rho_tm[l-1] is the time
+    refinement ratio below level l."""
+    net_rho = 1
+    if L1 > 1:
+        net_rho = rho_tm[L1 - 1]
+        L1 += 1
+    for lev in range(L1, Lf + 1):
+        d = levels.get(lev)
+        if d is not None:
+            d["tm2"][...] = d["tm1"]
+            d["tm1"][...] = d["v2"]
+            if lev > 1 and eps_c != 0:
+                f = 1.0 - eps_c / float(net_rho)
+                d["tm2"][...] = d["tm2"] + (d["tm1"] - d["brs"]) * f
+        net_rho *= rho_tm[L1 - 1]
+
+
 def net_forward(w1, b1, w2, b2, w3, b3, x):
     H = b1.shape[0]
     n2, n1, n0 = x.shape

@@ -217,6 +217,10 @@ def _declare_pamr(L):
     L.amrd_vcycle_tre.restype = d
     L.amrd_solve_elliptics_tre.argtypes = [i, i, i, _dp]
     L.amrd_def_tre_vars.argtypes = [s]
+    L.amrd_mg_def_extrap_vars.argtypes = [s, s]
+    L.amrd_mg_extrap.argtypes = [i, i, d, d, d, d]
+    L.amrd_set_mg_brs_vars.argtypes = [i, i]
+    L.amrd_update_mg_extrap_vars.argtypes = [i, i, d]
     L.amrd_mg_last_norms.argtypes = [_dp, _ip]
     L.amrd_gpu_stream.argtypes = []
     L.amrd_gpu_stream.restype = _vp

@@ -152,11 +152,21 @@ static void level_lop(int L)
  * and restricted with rop_u, the sources restricted with rop_src into
  * MG-only levels; then the MG variables via amrd_mg_def_vars.  Call after
  * PAMR_init_context, before PAMR_compose_hierarchy.  Returns 1 / 0. */
+int nps_define_vars_tl(int op_, int rop_u, int rop_res, int rop_src, int num_tl);
+
 int nps_define_vars(int op_, int rop_u, int rop_res, int rop_src)
+{
+  return nps_define_vars_tl(op_, rop_u, rop_res, rop_src, 1);
+}
+
+/* the same with num_tl AMR time levels of V (2 or more for the MG
+ * extrapolation of a constrained evolution, amrd/evolve.c:346-368); the MG
+ * hierarchy then works on the time level given to PAMR_build_mgh */
+int nps_define_vars_tl(int op_, int rop_u, int rop_res, int rop_src, int num_tl)
 {
   int pbt[6] = {0, 0, 0, 0, 0, 0};
   op = op_;
-  if (!PAMR_def_var_full("V", 1, 1, 1, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP, PAMR_NO_SYNC, rop_u,
+  if (!PAMR_def_var_full("V", 1, 1, num_tl, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP, PAMR_NO_SYNC, rop_u,
                          PAMR_NO_INTERP, PAMR_SYNC, 0, PAMR_NO_TRANSFER, pbt))
     return 0;
   if (!PAMR_def_var_full("rho", 1, 1, 1, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP, PAMR_NO_SYNC, rop_src,

@@ -243,6 +243,33 @@ __global__ __launch_bounds__(kNT) void k_mg_rhs_tre(double *rhs, double *lop, co
   lop[i] = f[i];
 }
 
+// MG extrapolation in time (amrd/evolve.c:346-368): the tl = 1 value of an
+// elliptic variable from its extrapolation variables
+__global__ __launch_bounds__(kNT) void k_extrap_etm1(double *v1, const double *etm1, const double *etm2, double f1,
+                                                     double f2, long long n) {
+  const long long i = (long long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= n) return;
+  v1[i] = f1 * etm1[i] + f2 * etm2[i];
+}
+__global__ __launch_bounds__(kNT) void k_extrap_tn(double *v1, const double *v2, const double *etm1,
+                                                   const double *etm2, double r, long long n) {
+  const long long i = (long long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= n) return;
+  v1[i] = v2[i] + r * (etm2[i] - etm1[i]);
+}
+// update_mg_extrap_vars (amrd/evolve.c:1684-1697): tm2 <- tm1, tm1 <- v(tl 2),
+// then (eps) tm2 += (tm1 - brs) * f
+__global__ __launch_bounds__(kNT) void k_update_extrap(double *tm1, double *tm2, const double *v2,
+                                                       const double *brs, int eps, double f, long long n) {
+  const long long i = (long long)blockIdx.x * kNT + threadIdx.x;
+  if (i >= n) return;
+  double t2 = tm1[i];
+  const double t1 = v2[i];
+  if (eps) t2 += (t1 - brs[i]) * f;
+  tm2[i] = t2;
+  tm1[i] = t1;
+}
+
 // compute_cgc, coarse part (amrd/mg.c:336-339)
 __global__ __launch_bounds__(kNT) void k_cgc_res(double *res, const double *f, const double *lop, double w0_i,
                                                  long long n) {
@@ -448,6 +475,28 @@ void set_res_sync(const std::vector<EllGfns> &E) {
   PAMR_freeze_tf_bits();
   PAMR_clear_tf_bits();
   for (const EllGfns &e : E) PAMR_set_tf_bit(e.res, PAMR_SYNC);
+}
+
+struct ExtrapGfns {
+  int v1, v2, tm1, tm2, brs;
+};
+bool extrap_gfns(std::vector<ExtrapGfns> &out, const char *who) {
+  out.clear();
+  for (const std::string &v : D.ell) {
+    ExtrapGfns e;
+    e.v1 = PAMR_get_gfn((char *)v.c_str(), PAMR_AMRH, 1);
+    e.v2 = PAMR_get_gfn((char *)v.c_str(), PAMR_AMRH, 2);
+    e.tm1 = PAMR_get_gfn((char *)(v + "_extrap_tm1").c_str(), PAMR_AMRH, 1);
+    e.tm2 = PAMR_get_gfn((char *)(v + "_extrap_tm2").c_str(), PAMR_AMRH, 1);
+    e.brs = PAMR_get_gfn((char *)(v + "_brs").c_str(), PAMR_AMRH, 1);
+    if (e.v1 <= 0 || e.v2 <= 0 || e.tm1 <= 0 || e.tm2 <= 0 || e.brs <= 0) {
+      std::printf("%s: error ... <%s> needs two AMR time levels and its extrapolation variables "
+                  "(amrd_mg_def_extrap_vars)\n", who, v.c_str());
+      return false;
+    }
+    out.push_back(e);
+  }
+  return true;
 }
 
 }  // namespace
@@ -1186,6 +1235,120 @@ int amrd_def_tre_vars(const char *tre_vars) {
       return 0;
     if (in_mgh) D.tre = name;
     D.ntre++;
+  }
+  return 1;
+}
+
+int amrd_mg_def_extrap_vars(const char *amr_interp2, const char *amr_transfer2) {
+  if (!C) {
+    std::printf("amrd_mg_def_extrap_vars: error ... no current context\n");
+    return 0;
+  }
+  if (D.ell.empty()) {
+    std::printf("amrd_mg_def_extrap_vars: error ... no elliptic variables (amrd_mg_def_vars)\n");
+    return 0;
+  }
+  const std::vector<std::string> i2 = split_list(amr_interp2), t2 = split_list(amr_transfer2);
+  int pbt[6] = {PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN};
+  for (const std::string &v : D.ell) {
+    // amrd/io.c:898-934: the attributes left by the preceding <v>_rhs, then
+    // AMR-only with one time level
+    if (!PAMR_def_var_full((char *)(v + "_brs").c_str(), 1, 0, 1, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP,
+                           PAMR_NO_SYNC, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_SYNC, 0, PAMR_NO_TRANSFER, pbt))
+      return 0;
+    for (const char *suf : {"_extrap_tm1", "_extrap_tm2"}) {
+      const std::string name = v + suf;
+      int amr_interp = PAMR_FOURTH_ORDER, amr_bdy_interp = PAMR_NO_INTERP, regrid = PAMR_FOURTH_ORDER;
+      if (in_list(name, i2)) amr_interp = amr_bdy_interp = PAMR_SECOND_ORDER;
+      if (in_list(name, t2)) regrid = PAMR_SECOND_ORDER;
+      if (!PAMR_def_var_full((char *)name.c_str(), 1, 0, 1, PAMR_STRAIGHT_INJECT, amr_interp, amr_bdy_interp,
+                             PAMR_NO_SYNC, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_SYNC, 1, regrid, pbt))
+        return 0;
+    }
+  }
+  return 1;
+}
+
+
+int amrd_mg_extrap(int lev, int method, double tnp1, double dt, double tnp1_lm1, double dt_lm1) {
+  if (!C || !C->A) {
+    std::printf("amrd_mg_extrap: error ... no AMR hierarchy\n");
+    return 0;
+  }
+  const int Lp = plevel(lev, PAMR_AMRH);
+  if (!Lp) {
+    std::printf("amrd_mg_extrap: error ... level %i out of range\n", lev);
+    return 0;
+  }
+  if (method != 0 && method != 1) {
+    std::printf("evolve: invalid MG_extrap_method\n");
+    return 0;
+  }
+  std::vector<ExtrapGfns> E;
+  if (!extrap_gfns(E, "amrd_mg_extrap")) return 0;
+  if (lev <= 1) return 1;  // amrd/evolve.c:349
+  const double f1 = (tnp1 - (tnp1_lm1 - 2 * dt_lm1)) / dt_lm1;
+  const double f2 = -(tnp1 - (tnp1_lm1 - dt_lm1)) / dt_lm1;
+  for (const ExtrapGfns &e : E) {
+    long long n;
+    double *v1 = pool(Lp, e.v1, n), *v2 = pool(Lp, e.v2, n), *tm1 = pool(Lp, e.tm1, n), *tm2 = pool(Lp, e.tm2, n);
+    if (!v1 || !v2 || !tm1 || !tm2) return 0;
+    if (method == 0)
+      hipLaunchKernelGGL(k_extrap_etm1, dim3(nblocks(n)), dim3(kNT), 0, C->stream, v1, tm1, tm2, f1, f2, n);
+    else
+      hipLaunchKernelGGL(k_extrap_tn, dim3(nblocks(n)), dim3(kNT), 0, C->stream, v1, v2, tm1, tm2, dt / dt_lm1, n);
+    if (hipGetLastError() != hipSuccess) return 0;
+  }
+  return 1;
+}
+
+int amrd_set_mg_brs_vars(int L1, int L2) {
+  if (!C || !C->A) {
+    std::printf("amrd_set_mg_brs_vars: error ... no AMR hierarchy\n");
+    return 0;
+  }
+  std::vector<ExtrapGfns> E;
+  if (!extrap_gfns(E, "amrd_set_mg_brs_vars")) return 0;
+  for (int lev = L1; lev <= L2; lev++) {
+    const int Lp = plevel(lev, PAMR_AMRH);
+    if (!Lp) continue;
+    for (const ExtrapGfns &e : E) {
+      long long n;
+      double *brs = pool(Lp, e.brs, n), *v2 = pool(Lp, e.v2, n);
+      if (!brs || !v2 || hipMemcpyAsync(brs, v2, n * sizeof(double), hipMemcpyDeviceToDevice, C->stream) != hipSuccess)
+        return 0;
+    }
+  }
+  return 1;
+}
+
+int amrd_update_mg_extrap_vars(int L1, int Lf, double eps_c) {
+  if (!C || !C->A) {
+    std::printf("amrd_update_mg_extrap_vars: error ... no AMR hierarchy\n");
+    return 0;
+  }
+  std::vector<ExtrapGfns> E;
+  if (!extrap_gfns(E, "amrd_update_mg_extrap_vars")) return 0;
+  int net_rho = 1;
+  if (L1 > 1) {  // amrd/evolve.c:1671-1676
+    net_rho = C->rho_tm[L1 - 1];
+    L1++;
+  }
+  for (int lev = L1; lev <= Lf; lev++) {
+    const int Lp = plevel(lev, PAMR_AMRH);
+    if (Lp) {
+      const int eps = lev > 1 && eps_c != 0;
+      const double f = eps ? 1.0 - eps_c / ((double)net_rho) : 0.0;
+      for (const ExtrapGfns &e : E) {
+        long long n;
+        double *tm1 = pool(Lp, e.tm1, n), *tm2 = pool(Lp, e.tm2, n), *v2 = pool(Lp, e.v2, n),
+               *brs = pool(Lp, e.brs, n);
+        if (!tm1 || !tm2 || !v2 || !brs) return 0;
+        hipLaunchKernelGGL(k_update_extrap, dim3(nblocks(n)), dim3(kNT), 0, C->stream, tm1, tm2, v2, brs, eps, f, n);
+        if (hipGetLastError() != hipSuccess) return 0;
+      }
+    }
+    net_rho *= C->rho_tm[L1 - 1];  // amrd/evolve.c:1700: the factor of level L1, every level
   }
   return 1;
 }

@@ -33,6 +33,7 @@ def nps_lib():
         A = C.CDLL(APP_PATH)
         i, vp, dp = C.c_int, C.c_void_p, C.POINTER(C.c_double)
         A.nps_define_vars.argtypes = [i, i, i, i]
+        A.nps_define_vars_tl.argtypes = [i, i, i, i, i]
         A.nps_init.argtypes = []
         A.nps_set_level_hooks.argtypes = [i]
         A.nps_set_level_hooks.restype = None
@@ -49,7 +50,7 @@ class NpsContext:
     refined levels' boxes (``levels[l]``: boxes of AMR level l + 2)."""
 
     def __init__(self, base_n, levels, op=0, rop_u=3, rop_res=3, rop_src=3, mcw=5,
-                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tre_vars=None):
+                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tre_vars=None, num_tl=1, extrap=False):
         self.L, self.app = lib(), nps_lib()
         L = self.L
         shape = (C.c_int * 3)(base_n, base_n, base_n)
@@ -59,8 +60,10 @@ class NpsContext:
         self.open = True
         w = (C.c_int * 3)(mcw, mcw, mcw)
         L.PAMR_set_MG_coarse_width(w)
-        if not self.app.nps_define_vars(op, rop_u, rop_res, rop_src):
+        if not self.app.nps_define_vars_tl(op, rop_u, rop_res, rop_src, num_tl):
             raise RuntimeError("nps_define_vars failed")
+        if extrap and not L.amrd_mg_def_extrap_vars(b"", b""):
+            raise RuntimeError("amrd_mg_def_extrap_vars failed")
         if tre_vars and not L.amrd_def_tre_vars(tre_vars.encode()):
             raise RuntimeError("amrd_def_tre_vars failed")
         boxes = [list(self.bbox)] + [list(b) for lv in levels for b in lv]

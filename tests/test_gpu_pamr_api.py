@@ -242,3 +242,78 @@ def test_tre_needs_tre_vars(torch_cuda):
     assert ctx.L.amrd_vcycle_tre(0, 1) < 0
     assert ctx.L.amrd_vcycle_tre(0, 0) > 0
     ctx.close()
+
+
+def test_mg_extrapolation_matches_oracle(torch_cuda):
+    """The MG extrapolation of constrained evolution (amrd/evolve.c:346-368,
+    1624-1639, 1661-1702) on every grid of every AMR level against the
+    oracle's restatement (pinned in test_oracle_closed_forms), bit for bit:
+    both methods, brs, and the extrapolation-variable update with and
+    without the eps_c correction."""
+    levels = layout(8)
+    print(f"case mg extrapolation boxes={[len(l) for l in levels]}", flush=True)
+    ctx = NpsContext(17, levels, num_tl=2, extrap=True)
+    L = ctx.L
+    nA = ctx.max_lev
+    rng = np.random.default_rng(8)
+    names = ("V1", "V2", "V_extrap_tm1", "V_extrap_tm2", "V_brs")
+
+    def put(lev, g, name, arr):
+        if name in ("V1", "V2"):
+            ctx.upload(PAMR_AMRH, lev, g, "V", arr, tl=int(name[1]))
+        else:
+            ctx.upload(PAMR_AMRH, lev, g, name, arr)
+
+    def get(lev, g, name):
+        if name in ("V1", "V2"):
+            return ctx.download(PAMR_AMRH, lev, g, "V", tl=int(name[1]))
+        return ctx.download(PAMR_AMRH, lev, g, name)
+
+    def fill():
+        data = {}
+        for lev in range(1, nA + 1):
+            for g, (shape, _, _) in enumerate(ctx.grids(PAMR_AMRH, lev)):
+                for nm in names:
+                    v = rng.integers(-512, 512, size=shape).astype(np.float64) / 64.0
+                    put(lev, g, nm, v)
+                    data[lev, g, nm] = v
+        return data
+
+    for method in (0, 1):
+        data = fill()
+        tnp1_lm1, dt_lm1, dt = 0.75, 0.25, 0.125
+        tnp1 = tnp1_lm1 - dt_lm1 + dt
+        for lev in range(1, nA + 1):
+            assert L.amrd_mg_extrap(lev, method, tnp1, dt, tnp1_lm1, dt_lm1) == 1
+        for (lev, g, nm), v in data.items():
+            if nm != "V1":
+                continue
+            want = v if lev == 1 else O.mg_extrap(data[lev, g, "V2"], data[lev, g, "V_extrap_tm1"],
+                                                   data[lev, g, "V_extrap_tm2"], method, tnp1, dt, tnp1_lm1, dt_lm1)
+            assert np.array_equal(get(lev, g, "V1"), want), (method, lev, g)
+    assert L.amrd_mg_extrap(2, 5, 1.0, 0.1, 1.0, 0.1) == 0                 # invalid method
+    data = fill()
+    assert L.amrd_set_mg_brs_vars(2, nA) == 1
+    for (lev, g, nm), v in data.items():
+        if nm == "V_brs":
+            assert np.array_equal(get(lev, g, nm), data[lev, g, "V2"] if lev >= 2 else v), (lev, g)
+    for L1, eps in ((1, 1.0), (2, 1.0), (1, 0.0)):
+        data = fill()
+        assert L.amrd_update_mg_extrap_vars(L1, nA, eps) == 1
+        ng = max(len(ctx.grids(PAMR_AMRH, lev)) for lev in range(1, nA + 1))
+        for g in range(ng):
+            lv = {lev: {"tm1": data[lev, g, "V_extrap_tm1"].copy(), "tm2": data[lev, g, "V_extrap_tm2"].copy(),
+                        "v2": data[lev, g, "V2"], "brs": data[lev, g, "V_brs"]}
+                  for lev in range(1, nA + 1) if (lev, g, "V2") in data}
+            O.update_mg_extrap_vars(lv, L1, nA, eps, [2] * 50)
+            for lev, d in lv.items():
+                assert np.array_equal(get(lev, g, "V_extrap_tm1"), d["tm1"]), (L1, eps, lev, g)
+                assert np.array_equal(get(lev, g, "V_extrap_tm2"), d["tm2"]), (L1, eps, lev, g)
+    ctx.close()
+
+
+def test_extrapolation_needs_its_variables(torch_cuda):
+    m, ctx = pair(2)
+    assert ctx.L.amrd_mg_extrap(2, 0, 1.0, 0.1, 1.0, 0.1) == 0             # V has one time level
+    assert ctx.L.amrd_update_mg_extrap_vars(1, ctx.max_lev, 1.0) == 0
+    ctx.close()

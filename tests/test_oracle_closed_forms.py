@@ -222,3 +222,62 @@ def test_fas_equals_correction_scheme():
         # differ only in operation order: FAS carries R u, CS carries e)
         assert abs(r_fas - r_cs) <= 1e-13 * r_first, (cyc, r_fas, r_cs)
     assert np.allclose(m.field(m.nlev, O.F_U), u_cs, rtol=0, atol=1e-12 * np.abs(u_cs).max())
+
+
+# ---------------------------------------------------------------------------
+# MG extrapolation in time (amrd/evolve.c:346-368, 1661-1702)
+# ---------------------------------------------------------------------------
+def _dyadic(rng, shape):
+    return rng.integers(-64, 64, size=shape).astype(np.float64) / 8.0
+
+
+def test_extrap_etm1_exact_for_linear_in_time():
+    """Method 0 is the line through tm1 (at tnp1_lm1 - dt_lm1) and tm2 (at
+    tnp1_lm1 - 2 dt_lm1): data linear in time are reproduced exactly at
+    tnp1 (dyadic values and times: no rounding)."""
+    rng = np.random.default_rng(1)
+    a, b = _dyadic(rng, (5, 6, 7)), _dyadic(rng, (5, 6, 7))
+    tnp1_lm1, dt_lm1 = 1.5, 0.25
+    for dt in (0.125, 0.0625, 0.25):
+        tnp1 = tnp1_lm1 - dt_lm1 + dt
+        tm1, tm2 = a + b * (tnp1_lm1 - dt_lm1), a + b * (tnp1_lm1 - 2 * dt_lm1)
+        got = O.mg_extrap(None, tm1, tm2, 0, tnp1, dt, tnp1_lm1, dt_lm1)
+        assert np.array_equal(got, a + b * tnp1)
+        # a quadratic is not reproduced (the method is second order)
+        q1, q2 = tm1 + (tnp1_lm1 - dt_lm1) ** 2, tm2 + (tnp1_lm1 - 2 * dt_lm1) ** 2
+        assert not np.allclose(O.mg_extrap(None, q1, q2, 0, tnp1, dt, tnp1_lm1, dt_lm1), a + b * tnp1 + tnp1 ** 2)
+
+
+def test_extrap_tn():
+    """Method 1 adds dt/dt_lm1 times the difference of the extrapolation
+    variables to tl 2: nothing when they agree, half the difference for
+    dt = dt_lm1 / 2 (the reference's sign: tm2 - tm1)."""
+    rng = np.random.default_rng(2)
+    v2, tm1, tm2 = (_dyadic(rng, (4, 5, 6)) for _ in range(3))
+    assert np.array_equal(O.mg_extrap(v2, tm1, tm1.copy(), 1, 2.0, 0.125, 1.9, 0.25), v2)
+    assert np.array_equal(O.mg_extrap(v2, tm1, tm2, 1, 2.0, 0.125, 1.9, 0.25), v2 + 0.5 * (tm2 - tm1))
+
+
+def test_update_extrap_rotation_and_eps_correction():
+    """eps_c = 0: tm2 <- tm1, tm1 <- v(tl 2) on every level; eps_c = 1 with
+    ratio 2: on level L > L1 the correction (tm1 - brs)(1 - eps_c / 2^(L - L1))
+    of the reference's comment (amrd/evolve.c:1647-1659), level L1 itself
+    skipped when L1 > 1."""
+    rng = np.random.default_rng(3)
+    rho_tm = [2] * 10
+    for L1, eps in ((1, 0.0), (1, 1.0), (2, 1.0), (2, 0.0)):
+        levels = {lev: {k: _dyadic(rng, (3, 4, 5)) for k in ("tm1", "tm2", "v2", "brs")} for lev in range(1, 5)}
+        before = {lev: {k: v.copy() for k, v in d.items()} for lev, d in levels.items()}
+        O.update_mg_extrap_vars(levels, L1, 4, eps, rho_tm)
+        first = L1 + 1 if L1 > 1 else L1
+        for lev in range(1, 5):
+            d, b = levels[lev], before[lev]
+            if lev < first:
+                for k in d:
+                    assert np.array_equal(d[k], b[k])
+                continue
+            assert np.array_equal(d["tm1"], b["v2"])
+            want = b["tm1"]
+            if eps and lev > 1:
+                want = want + (b["v2"] - b["brs"]) * (1.0 - eps / 2.0 ** (lev - L1))
+            assert np.array_equal(d["tm2"], want), (L1, eps, lev)
