@@ -306,21 +306,49 @@ struct FoldSpec {
   int ng;
   const int *gstart;  // ng+1 tile starts
 };
+// one block per norm slot: each wave sums the tile partials of every fourth
+// grid (fixed lane order, then a fixed shuffle tree), the per-grid RMS values
+// go to LDS, and one thread folds them with call_rr_app's max in grid order
+constexpr int kFoldGrids = 512;  // grids folded through LDS; more are folded in chunks
 __global__ __launch_bounds__(256) void k_fold_max(NormWs ws, const FoldSpec *specs, double *out) {
   const FoldSpec F = specs[blockIdx.x];
   const double *p = ws.partials + (size_t)F.slot * ws.cap * 2;
+  __shared__ double rg[kFoldGrids];
+  const int lane = (int)threadIdx.x & (kWave - 1), w = (int)threadIdx.x / kWave;
   double best = 0.0;
-  for (int g = 0; g < F.ng; g++) {
-    double s = 0.0, c = 0.0;
-    for (int b = F.gstart[g] + (int)threadIdx.x; b < F.gstart[g + 1]; b += 256) {
-      s += p[2 * b];
-      c += p[2 * b + 1];
+  for (int g0 = 0; g0 < F.ng; g0 += kFoldGrids) {
+    const int ng = min(kFoldGrids, F.ng - g0);
+    for (int q = w; q < ng; q += 256 / kWave) {
+      const int g = g0 + q;
+      double s = 0.0, c = 0.0;
+      for (int b = F.gstart[g] + lane; b < F.gstart[g + 1]; b += kWave) {
+        s += p[2 * b];
+        c += p[2 * b + 1];
+      }
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        s += __shfl_down(s, off, kWave);
+        c += __shfl_down(c, off, kWave);
+      }
+      if (lane == 0) rg[q] = sqrt(s / c);
     }
-    block_reduce2<256>(s, c);
-    const double r = sqrt(s / c);
-    best = g == 0 ? r : (r > best ? r : best);
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int q = 0; q < ng; q++) {
+        const double r = rg[q];
+        best = g0 + q == 0 ? r : (r > best ? r : best);
+      }
+    __syncthreads();
   }
   if (threadIdx.x == 0) out[blockIdx.x] = best;
+}
+
+// clear_MG_vars on one level: res, rhs and lop in one launch
+__global__ __launch_bounds__(256) void k_clear3(double *a, double *b, double *c, size_t n) {
+  const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  a[i] = 0.0;
+  b[i] = 0.0;
+  c[i] = 0.0;
 }
 
 inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -897,19 +925,13 @@ int do_inject_cycle(pmg_amr *A, int Lf) {
   return 0;
 }
 
-int fill_pool(pmg_amr *A, double *p, size_t n) {
-  if (!p || !n) return 0;
-  hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(256), 0, A->stream, p, 0.0, n);
-  PMG_HOST_CHECK(hipGetLastError());
-  return 0;
-}
-
 int clear_mg_vars(pmg_amr *A, int L) {
   ALevel &Lv = A->lev[L - 1];
-  int rc;
-  if ((rc = fill_pool(A, Lv.res, Lv.pool))) return rc;
-  if ((rc = fill_pool(A, Lv.rhs, Lv.pool))) return rc;
-  return fill_pool(A, Lv.lop, Lv.pool);
+  if (!Lv.pool) return 0;
+  if (!Lv.res || !Lv.rhs || !Lv.lop) return -1;
+  hipLaunchKernelGGL(k_clear3, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.res, Lv.rhs, Lv.lop, Lv.pool);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
 }
 
 template <int OP, bool P2, bool RHS0>
