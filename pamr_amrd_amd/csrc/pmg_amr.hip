@@ -180,6 +180,53 @@ __global__ __launch_bounds__(kXT) void k_xfer_interp4(XFields<T> F, const GridGe
   }
 }
 
+// compute_cgc fused (amrd/mg.c:323-355), for levels whose interp plan writes
+// every fine point exactly once: the coarse f_res = (u - lop)/w0_i is formed
+// where the interpolation reads it and the interpolated value is added to u
+// (and its ping-pong twin) at once; the same operations in the same order as
+// the three-step form, without storing either f_res
+template <int ORD>
+__global__ __launch_bounds__(kXT) void k_xfer_cgc(const double *__restrict__ cu, const double *__restrict__ clop,
+                                                  double w0_i, double *fu, double *fua, const GridGeo *gs,
+                                                  const GridGeo *gd, const XJob *jobs, const XBlk *blks) {
+  const XBlk b = blks[blockIdx.x];
+  const XJob J = jobs[b.job];
+  int i, j, k;
+  if (!job_point(J, b.first + threadIdx.x, i, j, k)) return;
+  const GridGeo S = gs[J.sg], D = gd[J.dg];
+  const size_t qd = D.off + ix3(i, j, k, D.nx, D.ny);
+  auto C = [&](int a, int bb, int c) -> double {
+    const size_t q = S.off + ix3(a, bb, c, S.nx, S.ny);
+    return (cu[q] - clop[q]) / w0_i;
+  };
+  double v;
+  if constexpr (ORD == 2) {
+    const int Fx = i + J.sh[0], Fy = j + J.sh[1], Fz = k + J.sh[2];
+    const int ai = Fx >> 1, ri = Fx & 1, aj = Fy >> 1, rj = Fy & 1, ak = Fz >> 1, rk = Fz & 1;
+    const double c0 = 1 - (1 * 1.0) / 2, c1 = (1 * 1.0) / 2;
+    auto X = [&](int bb, int c) -> double { return ri == 0 ? C(ai, bb, c) : c0 * C(ai, bb, c) + c1 * C(ai + 1, bb, c); };
+    auto Y = [&](int c) -> double { return rj == 0 ? X(aj, c) : c0 * X(aj, c) + c1 * X(aj + 1, c); };
+    v = rk == 0 ? Y(ak) : c0 * Y(ak) + c1 * Y(ak + 1);
+  } else {
+    const int fx = i + J.sh[0] - 2 * J.t0[0], fy = j + J.sh[1] - 2 * J.t0[1], fz = k + J.sh[2] - 2 * J.t0[2];
+    auto line = [&](int d, int fd, const auto &A) -> double {
+      if ((fd & 1) == 0) return A(fd >> 1);
+      return vi4_point(J.tn[d], 2, 1, 2 * J.tn[d] - 1, fd, A);
+    };
+    auto Y = [&](int c) -> double {
+      auto X = [&](int bb) -> double {
+        auto R = [&](int a) -> double { return C(J.t0[0] + a, J.t0[1] + bb, J.t0[2] + c); };
+        return line(0, fx, R);
+      };
+      return line(1, fy, X);
+    };
+    v = line(2, fz, Y);
+  }
+  const double nu = fu[qd] + v;
+  fu[qd] = nu;
+  fua[qd] = nu;
+}
+
 // ---------------------------------------------------------------------------
 // batched stencil kernels over all grids of a level
 // ---------------------------------------------------------------------------
@@ -420,6 +467,7 @@ struct ALevel {
   int nptiles = 0;
   int *d_pstart = nullptr;
   XPlan sync0, sync1, inject, interp;  // inject: from L+1 into L; interp: from L-1 into L
+  bool interp_once = false;  // the interp plan writes every point of every grid exactly once
 };
 
 }  // namespace
@@ -1046,6 +1094,18 @@ int lop_rhs(pmg_amr *A, int L) {
 int cgc(pmg_amr *A, int L) {
   ALevel &C = A->lev[L - 2], &F = A->lev[L - 1];
   if (A->iord != 2 && A->iord != 4) return -4;
+  if (F.interp_once) {
+    const XPlan &P = F.interp;
+    if (P.nblocks == 0) return 0;
+    if (A->iord == 4)
+      hipLaunchKernelGGL(k_xfer_cgc<4>, dim3(P.nblocks), dim3(kXT), 0, A->stream, C.u, C.lop, A->w0_i, F.u, F.ua,
+                         C.d_geo, F.d_geo, P.d_jobs, P.d_blks);
+    else
+      hipLaunchKernelGGL(k_xfer_cgc<2>, dim3(P.nblocks), dim3(kXT), 0, A->stream, C.u, C.lop, A->w0_i, F.u, F.ua,
+                         C.d_geo, F.d_geo, P.d_jobs, P.d_blks);
+    PMG_HOST_CHECK(hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(k_cgc_diff<double>, dim3(nblk(C.pool)), dim3(256), 0, A->stream, C.res, C.u, C.lop, A->w0_i,
                      C.pool);
   PMG_HOST_CHECK(hipGetLastError());
@@ -1272,6 +1332,24 @@ pmg_amr *pmg_amr_create(const int *base_shape, const double *bbox, int namr, con
     if (!rc) rc = upload_plan(A, Lv.sync1, j1);
     if (!rc) rc = upload_plan(A, Lv.inject, ji);
     if (!rc) rc = upload_plan(A, Lv.interp, jp);
+    if (!rc && L > 1) {  // does the interp plan write every point of every grid of L exactly once?
+      std::vector<std::vector<uint8_t>> cnt(Lv.g.size());
+      for (size_t gi = 0; gi < Lv.g.size(); gi++) cnt[gi].assign(Lv.g[gi].size(), 0);
+      bool once = true;
+      for (const XJob &J : jp)
+        for (int k = J.lo[2]; k <= J.hi[2] && once; k++)
+          for (int j = J.lo[1]; j <= J.hi[1] && once; j++)
+            for (int i = J.lo[0]; i <= J.hi[0]; i++) {
+              const HGrid &gg = Lv.g[J.dg];
+              uint8_t &c = cnt[J.dg][((size_t)k * gg.n[1] + j) * gg.n[0] + i];
+              if (c) { once = false; break; }
+              c = 1;
+            }
+      for (size_t gi = 0; gi < cnt.size() && once; gi++)
+        for (uint8_t c : cnt[gi])
+          if (!c) { once = false; break; }
+      Lv.interp_once = once;
+    }
     if (rc) { std::fprintf(stderr, "pamr_amr: transfer plan of level %d failed (%d)\n", L, rc); pmg_amr_destroy(A); return nullptr; }
   }
   A->ws.cap = cap;
