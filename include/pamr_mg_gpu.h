@@ -207,6 +207,13 @@ void *pmg_net_stream(pmg_net *N);
 int pmg_net_correct(pmg_net *N, const float *x, float scale, double *u, double coef, int nx,
                     int ny, int nz, void *stream);
 int pmg_net_recorrect(pmg_net *N, double *u, double coef, int nx, int ny, int nz, void *stream);
+/* the inference path pmg_net_correct uses: y = N(scale * x) on nb boxes
+ * (device fp32 [nb][P] in and out), layer 1 read straight from x, no
+ * pre-activations stored; equal to pmg_net_forward to FP32 summation order */
+int pmg_net_infer(pmg_net *N, const float *x, float scale, float *y, int nx, int ny, int nz, int nb);
+/* allocates the inference buffers for `points` points and packs the weight
+ * fragments ahead of time (pmg_net_correct would do it on first use) */
+int pmg_net_reserve(pmg_net *N, long long points);
 /* training crops: out[c] = scale*(a - b) on side^3 cubes at host origins */
 int pmg_gather_crops(const double *a, const double *b, int nx, int ny, int nz, const int *origins_host,
                      int ncrop, int side, double scale, float *out, void *stream);

@@ -100,6 +100,8 @@ def _declare(L):
     L.pmg_net_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
     L.pmg_net_correct.argtypes = [_vp, _vp, C.c_float, _vp, d, i, i, i, _vp]
     L.pmg_net_recorrect.argtypes = [_vp, _vp, d, i, i, i, _vp]
+    L.pmg_net_infer.argtypes = [_vp, _vp, C.c_float, _vp, i, i, i, i]
+    L.pmg_net_reserve.argtypes = [_vp, C.c_longlong]
     L.pmg_gather_crops.argtypes = [_vp, _vp, i, i, i, _ip, i, i, d, _vp, _vp]
     L.pmg_set_net.argtypes = [_vp, _vp]
     L.pmg_net_stats.argtypes = [_vp, _ip, _ip, _dp, _dp]
@@ -546,6 +548,11 @@ class Net:
     def forward(self, x_dev, y_dev, shape, nb=1):
         nz, ny, nx = shape
         _check(self._L.pmg_net_forward(self.h, tptr(x_dev), tptr(y_dev), nx, ny, nz, nb), "pmg_net_forward")
+
+    def infer(self, x_dev, y_dev, shape, nb=1, scale=1.0):
+        """y = N(scale x) through the inference path of pmg_net_correct."""
+        nz, ny, nx = shape
+        _check(self._L.pmg_net_infer(self.h, tptr(x_dev), scale, tptr(y_dev), nx, ny, nz, nb), "pmg_net_infer")
 
     def loss_grad(self, x_dev, t_dev, shape, nb=1, margin=0):
         nz, ny, nx = shape

@@ -87,9 +87,11 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t *A, const uint
 //   EPI 0: z = conv + bias -> Z (fp32), A = bf16(gelu(z))
 //   EPI 1: z = conv + bias -> Z (fp32) only (last layer)
 //   EPI 2: backward data: v = conv * gelu'(aux) -> Z (fp32), A = bf16(v)
+//   EPI 3: inference hidden layer: A = bf16(gelu(conv + bias)), no Z
+//   EPI 4: inference last layer: channel 0 only, Z[p] (fp32, one value per point)
 // Wf: weight fragments [27][64 lanes][4] bf16 (B operand order).
 // ---------------------------------------------------------------------------
-template <int EPI>
+template <int EPI, int ZC = 1>
 __global__ __launch_bounds__(256) void k_conv16(const uint16_t *__restrict__ X,
                                                 const uint16_t *__restrict__ Wf,
                                                 const float *__restrict__ bias,
@@ -98,39 +100,235 @@ __global__ __launch_bounds__(256) void k_conv16(const uint16_t *__restrict__ X,
                                                 uint16_t *__restrict__ Aout, Geom g) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int x0 = blockIdx.x * 16, y = blockIdx.y * 4 + w;
-  const int z = blockIdx.z % g.nz, b = blockIdx.z / g.nz;
+  const int nzc = (g.nz + ZC - 1) / ZC;
+  const int zc = blockIdx.z % nzc, b = blockIdx.z / nzc;
   if (y >= g.ny) return;  // uniform per wave; no block barrier below
   const int m = lane & 15, grp = lane >> 4;
   const size_t box = (size_t)g.nx * g.ny * g.nz;
   const uint16_t *Xb = X + (size_t)b * box * NC;
-  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int z1 = min(g.nz, zc * ZC + ZC);
+  for (int z = zc * ZC; z < z1; z++) {
+    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 3
-  for (int tap = 0; tap < NTAP; tap++) {
-    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-    const int xx = x0 + m + dx, yy = y + dy, zz = z + dz;
-    bf16x4 a = {0, 0, 0, 0};
-    if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
-      a = *(const bf16x4 *)(Xb + (((size_t)zz * g.ny + yy) * g.nx + xx) * NC + 4 * grp);
-    const bf16x4 bw = *(const bf16x4 *)(Wf + ((size_t)tap * 64 + lane) * 4);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int xx = x0 + 4 * grp + i;
-    if (xx >= g.nx) continue;
-    const size_t p = ((size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx) * NC + m;
-    float v = acc[i];
-    if constexpr (EPI == 0) {
-      v += bias[m];
-      Zout[p] = v;
-      Aout[p] = f2bf(gelu_f(v));
-    } else if constexpr (EPI == 1) {
-      Zout[p] = v + bias[m];
-    } else {
-      v *= gelu_grad_f(aux[p]);
-      Zout[p] = v;
-      Aout[p] = f2bf(v);
+    for (int tap = 0; tap < NTAP; tap++) {
+      const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+      const int xx = x0 + m + dx, yy = y + dy, zz = z + dz;
+      bf16x4 a = {0, 0, 0, 0};
+      if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
+        a = *(const bf16x4 *)(Xb + (((size_t)zz * g.ny + yy) * g.nx + xx) * NC + 4 * grp);
+      const bf16x4 bw = *(const bf16x4 *)(Wf + ((size_t)tap * 64 + lane) * 4);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
     }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int xx = x0 + 4 * grp + i;
+      if (xx >= g.nx) continue;
+      const size_t pp = (size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx;
+      const size_t p = pp * NC + m;
+      float v = acc[i];
+      if constexpr (EPI == 0) {
+        v += bias[m];
+        Zout[p] = v;
+        Aout[p] = f2bf(gelu_f(v));
+      } else if constexpr (EPI == 1) {
+        Zout[p] = v + bias[m];
+      } else if constexpr (EPI == 2) {
+        v *= gelu_grad_f(aux[p]);
+        Zout[p] = v;
+        Aout[p] = f2bf(v);
+      } else if constexpr (EPI == 3) {
+        Aout[p] = f2bf(gelu_f(v + bias[m]));
+      } else {
+        if (m == 0) Zout[pp] = v + bias[0];
+      }
+    }
+  }
+}
+
+// Hidden and last layer of the inference path with the input staged in LDS:
+// a block owns a CTX x CTY tile of (x, y) columns and marches ZC planes in
+// z, keeping a ring of 4 input planes (tile plus a 1-point halo, 16 BF16
+// channels = 32 B per point, zero outside the box) in LDS.  Wave w computes
+// rows 2w, 2w+1 of the tile, two 16-point segments each: per tap one weight
+// fragment and four MFMAs whose A operands come from LDS.  One barrier per
+// plane: plane z+2 is stored into the slot plane z-2 left at the start of
+// step z, plane z+3 is fetched into registers, and step z reads z-1..z+1.
+//   EPI 3: A = bf16(gelu(conv + bias));  EPI 4: Z[p] = conv + bias, channel 0
+constexpr int CTX = 32, CTY = 8, CLX = CTX + 2, CLY = CTY + 2;
+constexpr int CCHUNK = CLY * CLX * 2;                  // 16-byte chunks per staged plane
+constexpr int CLOAD = (CCHUNK + 255) / 256;            // per thread
+template <int EPI, int ZC>
+__global__ __launch_bounds__(256) void k_conv16_tile(const uint16_t *__restrict__ X,
+                                                     const uint16_t *__restrict__ Wf,
+                                                     const float *__restrict__ bias, float *__restrict__ Zout,
+                                                     uint16_t *__restrict__ Aout, Geom g) {
+  __shared__ uint4 sX[4][CLY][CLX][2];
+  const int ntx = (g.nx + CTX - 1) / CTX, nty = (g.ny + CTY - 1) / CTY, nzc = (g.nz + ZC - 1) / ZC;
+  int t = blockIdx.x;
+  const int tx = t % ntx;
+  t /= ntx;
+  const int ty = t % nty;
+  t /= nty;
+  const int zc = t % nzc, b = t / nzc;
+  const int x0 = tx * CTX, y0 = ty * CTY, z0 = zc * ZC, z1 = min(g.nz, z0 + ZC);
+  const size_t box = (size_t)g.nx * g.ny * g.nz;
+  const uint16_t *Xb = X + (size_t)b * box * NC;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, m = lane & 15, grp = lane >> 4;
+  uint4 reg[CLOAD];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < CLOAD; q++) {
+      const int idx = (int)threadIdx.x + q * 256;
+      reg[q] = make_uint4(0u, 0u, 0u, 0u);
+      if (idx >= CCHUNK || k < 0 || k >= g.nz) continue;
+      const int ly = idx / (CLX * 2), rem = idx % (CLX * 2), lx = rem >> 1, h = rem & 1;
+      const int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+      if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny)
+        reg[q] = ((const uint4 *)(Xb + (((size_t)k * g.ny + gy) * g.nx + gx) * NC))[h];
+    }
+  };
+  auto store = [&](int k) {
+    uint4 *S = &sX[k & 3][0][0][0];
+#pragma unroll
+    for (int q = 0; q < CLOAD; q++) {
+      const int idx = (int)threadIdx.x + q * 256;
+      if (idx < CCHUNK) S[idx] = reg[q];
+    }
+  };
+  // the 27 weight fragments stay in registers for the whole march
+  bf16x4 wk[NTAP];
+#pragma unroll
+  for (int tap = 0; tap < NTAP; tap++) wk[tap] = *(const bf16x4 *)(Wf + ((size_t)tap * 64 + lane) * 4);
+  fetch(z0 - 1);
+  store(z0 - 1);
+  fetch(z0);
+  store(z0);
+  fetch(z0 + 1);
+  store(z0 + 1);
+  fetch(z0 + 2);
+  __syncthreads();
+  for (int z = z0; z < z1; z++) {
+    store(z + 2);  // the slot of plane z-2, free since the barrier ending step z-1
+    fetch(z + 3);
+    floatx4 acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int tap = 0; tap < NTAP; tap++) {
+      const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+      const bf16x4 bw = wk[tap];
+      const int slot = (z + dz) & 3;
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int r = 2 * w + (q >> 1), sg = q & 1;
+        const uint16_t *pt = (const uint16_t *)&sX[slot][r + 1 + dy][16 * sg + m + 1 + dx][0];
+        const bf16x4 a = *(const bf16x4 *)(pt + 4 * grp);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[q], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int y = y0 + 2 * w + (q >> 1);
+      if (y >= g.ny) continue;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int xx = x0 + 16 * (q & 1) + 4 * grp + i;
+        if (xx >= g.nx) continue;
+        const size_t pp = (size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx;
+        if constexpr (EPI == 3) {
+          Aout[pp * NC + m] = f2bf(gelu_f(acc[q][i] + bias[m]));
+        } else {
+          if (m == 0) Zout[pp] = acc[q][i] + bias[0];
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Layer 1 of the inference path on the same tiles: the plane tile of
+// bf16(scale * x) (one channel, 2 B per point, zero outside the box) is staged
+// in a 4-plane LDS ring; lane (m, grp) gathers its taps 16h + 4grp + j from
+// LDS for the two MFMAs with K = taps.
+template <int ZC>
+__global__ __launch_bounds__(256) void k_conv_in_tile(const float *__restrict__ x, float scale,
+                                                      const uint16_t *__restrict__ Wfi,
+                                                      const float *__restrict__ bias, uint16_t *__restrict__ Aout,
+                                                      Geom g) {
+  __shared__ uint16_t sx[4][CLY][CLX];
+  const int ntx = (g.nx + CTX - 1) / CTX, nty = (g.ny + CTY - 1) / CTY, nzc = (g.nz + ZC - 1) / ZC;
+  int t = blockIdx.x;
+  const int tx = t % ntx;
+  t /= ntx;
+  const int ty = t % nty;
+  t /= nty;
+  const int zc = t % nzc, b = t / nzc;
+  const int x0 = tx * CTX, y0 = ty * CTY, z0 = zc * ZC, z1 = min(g.nz, z0 + ZC);
+  const size_t box = (size_t)g.nx * g.ny * g.nz;
+  const float *xb = x + (size_t)b * box;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, m = lane & 15, grp = lane >> 4;
+  constexpr int NPT = CLY * CLX, NLD = (NPT + 255) / 256;
+  uint16_t reg[NLD];
+  auto fetch = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < NLD; q++) {
+      const int idx = (int)threadIdx.x + q * 256;
+      reg[q] = 0;
+      if (idx >= NPT || k < 0 || k >= g.nz) continue;
+      const int ly = idx / CLX, lx = idx % CLX, gx = x0 - 1 + lx, gy = y0 - 1 + ly;
+      if (gx >= 0 && gx < g.nx && gy >= 0 && gy < g.ny) reg[q] = f2bf(xb[((size_t)k * g.ny + gy) * g.nx + gx] * scale);
+    }
+  };
+  auto store = [&](int k) {
+    uint16_t *S = &sx[k & 3][0][0];
+#pragma unroll
+    for (int q = 0; q < NLD; q++) {
+      const int idx = (int)threadIdx.x + q * 256;
+      if (idx < NPT) S[idx] = reg[q];
+    }
+  };
+  const bf16x4 bw0 = *(const bf16x4 *)(Wfi + ((size_t)0 * 64 + lane) * 4);
+  const bf16x4 bw1 = *(const bf16x4 *)(Wfi + ((size_t)1 * 64 + lane) * 4);
+  const float bm = bias[m];
+  fetch(z0 - 1);
+  store(z0 - 1);
+  fetch(z0);
+  store(z0);
+  fetch(z0 + 1);
+  store(z0 + 1);
+  fetch(z0 + 2);
+  __syncthreads();
+  for (int z = z0; z < z1; z++) {
+    store(z + 2);
+    fetch(z + 3);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int r = 2 * w + (q >> 1), sg = q & 1;
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        bf16x4 a = {0, 0, 0, 0};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int tap = 16 * h + 4 * grp + j;
+          if (tap < NTAP) {
+            const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+            a[j] = (short)sx[(z + dz) & 3][r + 1 + dy][16 * sg + m + 1 + dx];
+          }
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h == 0 ? bw0 : bw1, acc, 0, 0, 0);
+      }
+      const int y = y0 + r;
+      if (y < g.ny) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int xx = x0 + 16 * sg + 4 * grp + i;
+          if (xx >= g.nx) continue;
+          Aout[((size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx) * NC + m] = f2bf(gelu_f(acc[i] + bm));
+        }
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -287,18 +485,26 @@ __global__ __launch_bounds__(256) void k_pack_frags(const float *__restrict__ W,
   Wb[idx] = f2bf(W[(26 - tap) * 256 + m * 16 + (4 * grp + j)]);
 }
 
-// u += coef * Z[p][0] at interior points of the grid (box = the grid)
-template <typename TU>
+// layer-1 fragments of the inference path (K = taps of input channel 0):
+// Wfi[h][lane][j] = W[tap = 16h + 4(lane/16) + j][ci 0][co lane%16], 0 past tap 26
+__global__ __launch_bounds__(256) void k_pack_frags_in(const float *__restrict__ W, uint16_t *__restrict__ Wfi) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;  // over 2*64*4
+  if (idx >= 2 * 64 * 4) return;
+  const int h = idx / 256, lane = (idx / 4) % 64, j = idx % 4;
+  const int tap = 16 * h + 4 * (lane >> 4) + j;
+  Wfi[idx] = tap < NTAP ? f2bf(W[tap * 256 + (lane & 15)]) : (uint16_t)0;
+}
+
+// u += coef * Z[p * STRIDE] at interior points of the grid (box = the grid)
+template <typename TU, int STRIDE>
 __global__ __launch_bounds__(256) void k_apply_correction(TU *__restrict__ u, const float *__restrict__ Z,
-                                                          int nx, int ny, int nz, double coef,
-                                                          TU *__restrict__ alt) {
+                                                          int nx, int ny, int nz, double coef) {
   const int i = blockIdx.x * 64 + (int)(threadIdx.x % 64);
   const int j = blockIdx.y * 4 + (int)(threadIdx.x / 64);
   const int k = blockIdx.z;
   if (i < 1 || j < 1 || k < 1 || i >= nx - 1 || j >= ny - 1 || k >= nz - 1) return;
   const size_t p = ix3(i, j, k, nx, ny);
-  u[p] = u[p] + (TU)(coef * (double)Z[p * NC]);
-  (void)alt;
+  u[p] = u[p] + (TU)(coef * (double)Z[p * STRIDE]);
 }
 
 }  // namespace
@@ -312,11 +518,16 @@ struct pmg_net {
   float *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
   uint8_t *mask = nullptr;  // trainable entries
   uint16_t *wf = nullptr, *wb = nullptr;  // 3 layers * 27*64*4 fragments each
+  uint16_t *wfi = nullptr;                // layer 1 with K = taps (inference path), 2*64*4
   int adam_t = 0;
   // activation buffers (grown on demand): X0, A1, A2, dA (bf16 [P][16]); Z1, Z2, Z3, dZ (fp32)
   size_t cap = 0;
   uint16_t *x0 = nullptr, *a1 = nullptr, *a2 = nullptr, *d16 = nullptr, *d16b = nullptr;
   float *z1 = nullptr, *z2 = nullptr, *z3 = nullptr, *dz = nullptr;
+  // inference buffers (their own capacity: inference allocates no training buffers)
+  size_t icap = 0;
+  uint16_t *ia1 = nullptr, *ia2 = nullptr;  // bf16 [P][16]
+  float *y = nullptr;                       // output, one FP32 per point
   float *partial = nullptr;
   size_t partial_cap = 0;
   double *lpart = nullptr, *lsum = nullptr;
@@ -347,6 +558,18 @@ int net_ensure(pmg_net *N, size_t P) {
   return 0;
 }
 
+int net_ensure_infer(pmg_net *N, size_t P) {
+  if (P <= N->icap) return 0;
+  PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+  for (void *p : {(void *)N->ia1, (void *)N->ia2, (void *)N->y})
+    if (p) PMG_HOST_CHECK(hipFree(p));
+  PMG_HOST_CHECK(hipMalloc(&N->ia1, P * NC * sizeof(uint16_t)));
+  PMG_HOST_CHECK(hipMalloc(&N->ia2, P * NC * sizeof(uint16_t)));
+  PMG_HOST_CHECK(hipMalloc(&N->y, P * sizeof(float)));
+  N->icap = P;
+  return 0;
+}
+
 int net_frags(pmg_net *N) {
   if (N->frags_valid) return 0;
   for (int l = 0; l < 3; l++) {
@@ -354,12 +577,36 @@ int net_frags(pmg_net *N) {
                        N->w + (size_t)l * LAYER_FLOATS, N->wf + (size_t)l * NTAP * 256,
                        N->wb + (size_t)l * NTAP * 256);
   }
+  hipLaunchKernelGGL(k_pack_frags_in, dim3(2), dim3(256), 0, N->stream, N->w, N->wfi);
   PMG_HOST_CHECK(hipGetLastError());
   N->frags_valid = true;
   return 0;
 }
 
 dim3 conv_grid(const Geom &g) { return dim3((g.nx + 15) / 16, (g.ny + 3) / 4, g.nz * g.nb); }
+
+// z planes one block of the LDS-tiled inference kernels marches
+constexpr int kTilePlanes = 16;
+
+// inference: N(scale * x) into N->y (one FP32 per point).  Layer 1 reads x
+// directly (K = taps), the hidden layer keeps only the BF16 activations, the
+// last layer writes channel 0 only -- no pre-activations are stored
+int net_infer(pmg_net *N, const float *x, float scale, const Geom &g) {
+  int rc;
+  if ((rc = net_frags(N))) return rc;
+  const float *b1 = N->w + 0 * LAYER_FLOATS + NTAP * 256;
+  const float *b2 = N->w + 1 * LAYER_FLOATS + NTAP * 256;
+  const float *b3 = N->w + 2 * LAYER_FLOATS + NTAP * 256;
+  constexpr int TZ = kTilePlanes;
+  const dim3 gt((unsigned)(((g.nx + CTX - 1) / CTX) * ((g.ny + CTY - 1) / CTY) * ((g.nz + TZ - 1) / TZ) * g.nb));
+  hipLaunchKernelGGL(k_conv_in_tile<TZ>, gt, dim3(256), 0, N->stream, x, scale, N->wfi, b1, N->ia1, g);
+  hipLaunchKernelGGL((k_conv16_tile<3, TZ>), gt, dim3(256), 0, N->stream, N->ia1, N->wf + NTAP * 256, b2,
+                     (float *)nullptr, N->ia2, g);
+  hipLaunchKernelGGL((k_conv16_tile<4, TZ>), gt, dim3(256), 0, N->stream, N->ia2, N->wf + 2 * NTAP * 256, b3,
+                     N->y, (uint16_t *)nullptr, g);
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
 
 // forward of the 3 layers on X0 (already packed); Z3[p][0] is the output
 int net_forward(pmg_net *N, const Geom &g) {
@@ -494,6 +741,7 @@ pmg_net *pmg_net_create(int hidden, unsigned long long seed) {
             hipMalloc(&N->mask, nf) == hipSuccess &&
             hipMalloc(&N->wf, 3 * NTAP * 256 * sizeof(uint16_t)) == hipSuccess &&
             hipMalloc(&N->wb, 3 * NTAP * 256 * sizeof(uint16_t)) == hipSuccess &&
+            hipMalloc(&N->wfi, 2 * 64 * 4 * sizeof(uint16_t)) == hipSuccess &&
             hipMalloc(&N->lsum, sizeof(double)) == hipSuccess;
   if (!ok) { pmg_net_destroy(N); return nullptr; }
   (void)hipMemcpy(N->w, w.data(), nf * 4, hipMemcpyHostToDevice);
@@ -510,7 +758,8 @@ void pmg_net_destroy(pmg_net *N) {
   for (void *p : {(void *)N->w, (void *)N->m1, (void *)N->m2, (void *)N->grad, (void *)N->mask,
                   (void *)N->wf, (void *)N->wb, (void *)N->x0, (void *)N->a1, (void *)N->a2,
                   (void *)N->d16, (void *)N->d16b, (void *)N->z1, (void *)N->z2, (void *)N->z3,
-                  (void *)N->dz, (void *)N->partial, (void *)N->lpart, (void *)N->lsum})
+                  (void *)N->dz, (void *)N->ia1, (void *)N->ia2, (void *)N->y, (void *)N->wfi,
+                  (void *)N->partial, (void *)N->lpart, (void *)N->lsum})
     if (p) (void)hipFree(p);
   if (N->stream) (void)hipStreamDestroy(N->stream);
   delete N;
@@ -634,26 +883,42 @@ int pmg_net_correct(pmg_net *N, const float *x, float scale, double *u, double c
   if (!N || !x || !u) return -1;
   hipStream_t st = (hipStream_t)stream;
   const size_t P = (size_t)nx * ny * nz;
-  if (net_ensure(N, P)) return -1;
+  if (net_ensure_infer(N, P)) return -1;
   hipStream_t keep = N->stream;
   N->stream = st;
-  Geom g{nx, ny, nz, 1};
-  hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, st, x, N->x0, P,
-                     scale);
-  int rc = net_forward(N, g);
+  int rc = net_infer(N, x, scale, Geom{nx, ny, nz, 1});
   N->stream = keep;
   if (rc) return rc;
-  hipLaunchKernelGGL(k_apply_correction<double>, dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0, st, u,
-                     N->z3, nx, ny, nz, coef, (double *)nullptr);
+  hipLaunchKernelGGL((k_apply_correction<double, 1>), dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0, st, u,
+                     N->y, nx, ny, nz, coef);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
 
 int pmg_net_recorrect(pmg_net *N, double *u, double coef, int nx, int ny, int nz, void *stream) {
-  if (!N || !u || (size_t)nx * ny * nz > N->cap) return -1;
-  hipLaunchKernelGGL(k_apply_correction<double>, dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0,
-                     (hipStream_t)stream, u, N->z3, nx, ny, nz, coef, (double *)nullptr);
+  if (!N || !u || (size_t)nx * ny * nz > N->icap) return -1;
+  hipLaunchKernelGGL((k_apply_correction<double, 1>), dim3((nx + 63) / 64, (ny + 3) / 4, nz), dim3(256), 0,
+                     (hipStream_t)stream, u, N->y, nx, ny, nz, coef);
   PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int pmg_net_reserve(pmg_net *N, long long points) {
+  if (!N || points < 1) return -1;
+  if (net_ensure_infer(N, (size_t)points)) return -1;
+  if (net_frags(N)) return -1;
+  PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
+  return 0;
+}
+
+int pmg_net_infer(pmg_net *N, const float *x, float scale, float *y, int nx, int ny, int nz, int nb) {
+  if (!N || !x || !y || nx < 1 || ny < 1 || nz < 1 || nb < 1) return -1;
+  const size_t P = (size_t)nx * ny * nz * nb;
+  if (net_ensure_infer(N, P)) return -1;
+  int rc = net_infer(N, x, scale, Geom{nx, ny, nz, nb});
+  if (rc) return rc;
+  PMG_HOST_CHECK(hipMemcpyAsync(y, N->y, P * sizeof(float), hipMemcpyDeviceToDevice, N->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
   return 0;
 }
 

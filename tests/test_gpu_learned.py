@@ -76,6 +76,37 @@ def test_forward_and_gradients_match_oracle(pm, torch_cuda):
         assert e < 5e-2, name
 
 
+@pytest.mark.parametrize("shape,nb", [((7, 9, 11), 2), ((20, 17, 35), 1)])
+def test_inference_path_matches_oracle_and_forward(pm, torch_cuda, shape, nb):
+    """The inference path of pmg_net_correct (layer 1 with the 27 taps as
+    the MFMA's K, no stored pre-activations, channel 0 only at the end)
+    against the FP64 oracle network and against the training forward
+    (the same BF16 inputs and weights; FP32 summation order differs, and a
+    BF16 activation may round the other way)."""
+    torch = torch_cuda
+    rng = np.random.default_rng(7)
+    ws = _rand_weights(rng)
+    wsb = {k: (_bf16(v) if k.startswith("w") else v) for k, v in ws.items()}
+    scale = 0.5
+    x = rng.standard_normal((nb,) + shape)
+    xs = _bf16(x * scale)
+    net = pm.Net(seed=3)
+    net.set_weights(ws)
+    xd = torch.from_numpy(x.astype(np.float32)).cuda()
+    yi = torch.zeros_like(xd)
+    net.infer(xd, yi, shape, nb, scale=scale)
+    yf = torch.zeros_like(xd)
+    net.forward(torch.from_numpy((x * scale).astype(np.float32)).cuda(), yf, shape, nb)
+    yi, yf = yi.cpu().numpy().astype(np.float64), yf.cpu().numpy().astype(np.float64)
+    yref = np.stack([O.net_forward(*[wsb[k] for k in ("w1", "b1", "w2", "b2", "w3", "b3")],
+                                   np.ascontiguousarray(xs[b])) for b in range(nb)])
+    e_ref = np.linalg.norm(yi - yref) / np.linalg.norm(yref)
+    e_fwd = np.linalg.norm(yi - yf) / np.linalg.norm(yf)
+    print(f"inference vs oracle {e_ref:.3e}, vs training forward {e_fwd:.3e}", flush=True)
+    assert e_ref < 2e-2
+    assert e_fwd < 5e-3
+
+
 def test_training_reduces_loss(pm, torch_cuda):
     torch = torch_cuda
     rng = np.random.default_rng(7)
