@@ -337,6 +337,23 @@ def probe_kernels(S, n, pm, torch):
     print(f"probe copy: {8 * n ** 3} bytes read and written per launch", flush=True)
 
 
+def probe_net(n, pm, torch):
+    """Eight inference passes of the correction network over n^3 points
+    (random weights and input; the cost does not depend on them), for a
+    kernel trace of the inference kernels."""
+    net = pm.Net(seed=1)
+    x = torch.randn((n, n, n), dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    net.infer(x, y, (n, n, n), 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(8):
+        net.infer(x, y, (n, n, n), 1)
+    torch.cuda.synchronize()
+    print(f"probe net: {1e3 * (time.perf_counter() - t0) / 8:.3f} ms per {n}^3 inference", flush=True)
+    net.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,6 +381,8 @@ def main():
     ap.add_argument("--c5-max-iter", type=int, default=300)
     ap.add_argument("--c5-restart", type=int, default=8, help="FGMRES restart length of the excised leg")
     ap.add_argument("--c5-mixed-rhs", type=int, default=2, help="right-hand sides also solved by defect correction")
+    ap.add_argument("--probe-net", action="store_true",
+                    help="only time the correction network's inference on the n^3 grid (for a kernel trace)")
     ap.add_argument("--probe-kernels", action="store_true",
                     help="only launch the finest-level kernels (for rocprofv3 --pmc runs): "
                          "FP64 sweep, FP64 residual, FP32 sweep and a plain 8 B/point copy "
@@ -383,6 +402,9 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     n = args.n
+    if args.probe_net:
+        probe_net(n, pm, torch)
+        return
     print(f"bench: generating {n}^3 source", file=sys.stderr, flush=True)
     rho = -pm.gaussian_source(n, seed=1234)
     S = pm.Solver(n)
