@@ -166,7 +166,8 @@ int pmg_num_sweep_variants(void);
 int pmg_set_graph(pmg_solver *S, int on);
 /* run the small bottom levels (all dimensions <= 17 points, below the
  * finest level, no excision) as one single-workgroup kernel (default on);
- * results are identical either way */
+ * on = 0 off, 1 the default, 2..17 the largest side fused (9: only the
+ * levels of 9 points and below); results are identical either way */
 int pmg_set_bottom_fusion(pmg_solver *S, int on);
 /* compute the finest-level FP64 residual inside the full-weight injection of u
  * and f_res instead of storing it (default on; no excision, full weighting

@@ -380,8 +380,10 @@ class Solver:
     def set_graph(self, on):
         _check(self._L.pmg_set_graph(self.h, 1 if on else 0), "pmg_set_graph")
 
-    def set_bottom_fusion(self, on):
-        _check(self._L.pmg_set_bottom_fusion(self.h, 1 if on else 0), "pmg_set_bottom_fusion")
+    def set_bottom_fusion(self, on, max_n=None):
+        """fuse the bottom levels into one workgroup (max_n: largest side fused)"""
+        v = (max_n if max_n is not None else 1) if on else 0
+        _check(self._L.pmg_set_bottom_fusion(self.h, v), "pmg_set_bottom_fusion")
 
     def set_restrict_fusion(self, on):
         _check(self._L.pmg_set_restrict_fusion(self.h, 1 if on else 0), "pmg_set_restrict_fusion")

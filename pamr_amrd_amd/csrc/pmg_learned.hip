@@ -87,11 +87,9 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t *A, const uint
 //   EPI 0: z = conv + bias -> Z (fp32), A = bf16(gelu(z))
 //   EPI 1: z = conv + bias -> Z (fp32) only (last layer)
 //   EPI 2: backward data: v = conv * gelu'(aux) -> Z (fp32), A = bf16(v)
-//   EPI 3: inference hidden layer: A = bf16(gelu(conv + bias)), no Z
-//   EPI 4: inference last layer: channel 0 only, Z[p] (fp32, one value per point)
 // Wf: weight fragments [27][64 lanes][4] bf16 (B operand order).
 // ---------------------------------------------------------------------------
-template <int EPI, int ZC = 1>
+template <int EPI>
 __global__ __launch_bounds__(256) void k_conv16(const uint16_t *__restrict__ X,
                                                 const uint16_t *__restrict__ Wf,
                                                 const float *__restrict__ bias,
@@ -100,47 +98,39 @@ __global__ __launch_bounds__(256) void k_conv16(const uint16_t *__restrict__ X,
                                                 uint16_t *__restrict__ Aout, Geom g) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int x0 = blockIdx.x * 16, y = blockIdx.y * 4 + w;
-  const int nzc = (g.nz + ZC - 1) / ZC;
-  const int zc = blockIdx.z % nzc, b = blockIdx.z / nzc;
+  const int z = blockIdx.z % g.nz, b = blockIdx.z / g.nz;
   if (y >= g.ny) return;  // uniform per wave; no block barrier below
   const int m = lane & 15, grp = lane >> 4;
   const size_t box = (size_t)g.nx * g.ny * g.nz;
   const uint16_t *Xb = X + (size_t)b * box * NC;
-  const int z1 = min(g.nz, zc * ZC + ZC);
-  for (int z = zc * ZC; z < z1; z++) {
-    floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 3
-    for (int tap = 0; tap < NTAP; tap++) {
-      const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-      const int xx = x0 + m + dx, yy = y + dy, zz = z + dz;
-      bf16x4 a = {0, 0, 0, 0};
-      if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
-        a = *(const bf16x4 *)(Xb + (((size_t)zz * g.ny + yy) * g.nx + xx) * NC + 4 * grp);
-      const bf16x4 bw = *(const bf16x4 *)(Wf + ((size_t)tap * 64 + lane) * 4);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
-    }
+  for (int tap = 0; tap < NTAP; tap++) {
+    const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+    const int xx = x0 + m + dx, yy = y + dy, zz = z + dz;
+    bf16x4 a = {0, 0, 0, 0};
+    if (xx >= 0 && xx < g.nx && yy >= 0 && yy < g.ny && zz >= 0 && zz < g.nz)
+      a = *(const bf16x4 *)(Xb + (((size_t)zz * g.ny + yy) * g.nx + xx) * NC + 4 * grp);
+    const bf16x4 bw = *(const bf16x4 *)(Wf + ((size_t)tap * 64 + lane) * 4);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc, 0, 0, 0);
+  }
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int xx = x0 + 4 * grp + i;
-      if (xx >= g.nx) continue;
-      const size_t pp = (size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx;
-      const size_t p = pp * NC + m;
-      float v = acc[i];
-      if constexpr (EPI == 0) {
-        v += bias[m];
-        Zout[p] = v;
-        Aout[p] = f2bf(gelu_f(v));
-      } else if constexpr (EPI == 1) {
-        Zout[p] = v + bias[m];
-      } else if constexpr (EPI == 2) {
-        v *= gelu_grad_f(aux[p]);
-        Zout[p] = v;
-        Aout[p] = f2bf(v);
-      } else if constexpr (EPI == 3) {
-        Aout[p] = f2bf(gelu_f(v + bias[m]));
-      } else {
-        if (m == 0) Zout[pp] = v + bias[0];
-      }
+  for (int i = 0; i < 4; i++) {
+    const int xx = x0 + 4 * grp + i;
+    if (xx >= g.nx) continue;
+    const size_t pp = (size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx;
+    const size_t p = pp * NC + m;
+    float v = acc[i];
+    if constexpr (EPI == 0) {
+      v += bias[m];
+      Zout[p] = v;
+      Aout[p] = f2bf(gelu_f(v));
+    } else if constexpr (EPI == 1) {
+      Zout[p] = v + bias[m];
+    } else {
+      v *= gelu_grad_f(aux[p]);
+      Zout[p] = v;
+      Aout[p] = f2bf(v);
     }
   }
 }

@@ -133,6 +133,7 @@ struct pmg_solver {
   int lv_variant[2][kMaxLev], lv_zchunk[2][kMaxLev];  // per-level sweep configuration, [0] FP64, [1] FP32
   int use_graph = 1;
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
+  int bottom_max_n = 17;  // ... those with at most this many points per side
   int use_fused_restrict = 1;  // finest residual + injection in one pass
   int use_sweep2 = 0;          // pairs of sweeps temporally blocked on large levels
                                // (measured slower than single sweeps; off by default)
@@ -593,7 +594,8 @@ int bottom_top(const pmg_solver *S) {
   int Lb = 0;
   for (int L = 1; L < S->nlev && L <= kBotMax; L++) {
     const Level &Lv = S->lev[L - 1];
-    if (Lv.n[0] <= kBottomMaxN && Lv.n[1] <= kBottomMaxN && Lv.n[2] <= kBottomMaxN && Lv.rbox.ok) Lb = L;
+    const int mx = std::min(kBottomMaxN, S->bottom_max_n);
+    if (Lv.n[0] <= mx && Lv.n[1] <= mx && Lv.n[2] <= mx && Lv.rbox.ok) Lb = L;
     else break;
   }
   return Lb >= 2 ? Lb : 0;
@@ -980,8 +982,9 @@ int pmg_set_graph(pmg_solver *S, int on) {
 }
 
 int pmg_set_bottom_fusion(pmg_solver *S, int on) {
-  if (!S) return -1;
+  if (!S || on < 0 || on > kBottomMaxN) return -1;
   S->use_bottom = on ? 1 : 0;
+  S->bottom_max_n = on > 1 ? on : kBottomMaxN;
   invalidate_graphs(S);
   return 0;
 }
