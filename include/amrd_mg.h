@@ -78,6 +78,28 @@ int amrd_mg_extrap(int lev, int method, double tnp1, double dt, double tnp1_lm1,
 int amrd_set_mg_brs_vars(int L1, int L2);
 int amrd_update_mg_extrap_vars(int L1, int Lf, double eps_c);
 
+/* The t = 0 branches of solve_elliptics (amrd/mg.c:28-29,45-68).
+ *
+ * amrd_mg_def_vars_t0: AMRD's elliptic_vars_t0 -- MG variables of the
+ * application solved alongside the elliptic variables while is_t0 is set
+ * (every MG routine then runs over both lists, amrd/mg.c:248,279,327,612);
+ * their <v>_res, <v>_lop, <v>_rhs as amrd_mg_def_vars defines them.
+ * amrd_mg_def_cnst_data_vars: MG_cnst_data_vars, MG variables the
+ * application fills in its app_t0_cnst_data hook.
+ * amrd_set_is_t0: AMRD_is_t0 (default 1).
+ * amrd_set_elliptic_vars_t0_init (amrd/amrd.c:109-112) and
+ * amrd_set_app_t0_cnst_data (amrd()'s argument of that name): per-grid hooks
+ * over the MG hierarchy.  With is_t0, solve_elliptics calls the t0 init hook
+ * on every MG level from the coarsest up, syncing its variables on the level
+ * and interpolating them (fourth order) to the next, then injects them
+ * (straight) back down; after every V-cycle it does the same with the
+ * constraint-data hook and variables.  Definitions return 1 / 0. */
+int amrd_mg_def_vars_t0(const char *elliptic_vars_t0, const char *mg_fw_restr, const char *mg_interp2);
+int amrd_mg_def_cnst_data_vars(const char *vars);
+void amrd_set_is_t0(int is_t0);
+void amrd_set_elliptic_vars_t0_init(amrd_void_hook app_elliptic_vars_t0_init);
+void amrd_set_app_t0_cnst_data(amrd_void_hook app_t0_cnst_data);
+
 /* AMRD's MG parameters (amrd/io.c:97-106,269-285); defaults 3, 3, 1, 50,
  * 0.0, 1e-3, 5000, 1.0, 1.0 */
 void amrd_mg_set_params(int pre_swp, int pst_swp, int min_iter, int max_iter, double tol,

@@ -220,6 +220,14 @@ def _declare_pamr(L):
     L.amrd_solve_elliptics_tre.argtypes = [i, i, i, _dp]
     L.amrd_def_tre_vars.argtypes = [s]
     L.amrd_mg_def_extrap_vars.argtypes = [s, s]
+    L.amrd_mg_def_vars_t0.argtypes = [s, s, s]
+    L.amrd_mg_def_cnst_data_vars.argtypes = [s]
+    L.amrd_set_is_t0.argtypes = [i]
+    L.amrd_set_is_t0.restype = None
+    L.amrd_set_elliptic_vars_t0_init.argtypes = [C.c_void_p]
+    L.amrd_set_elliptic_vars_t0_init.restype = None
+    L.amrd_set_app_t0_cnst_data.argtypes = [C.c_void_p]
+    L.amrd_set_app_t0_cnst_data.restype = None
     L.amrd_mg_extrap.argtypes = [i, i, d, d, d, d]
     L.amrd_set_mg_brs_vars.argtypes = [i, i]
     L.amrd_update_mg_extrap_vars.argtypes = [i, i, d]

@@ -295,6 +295,10 @@ struct Amrd {
   amrd_level_real_hook lresidual = nullptr;
   amrd_level_void_hook llop = nullptr;
   std::vector<std::string> ell;
+  std::vector<std::string> ell_t0;  // elliptic_vars_t0: solved alongside while is_t0
+  std::vector<std::string> cnst;    // MG_cnst_data_vars
+  int is_t0 = 1;                    // AMRD_is_t0 (amrd/globals.c:248)
+  amrd_void_hook t0_init = nullptr, t0_cnst = nullptr;
   std::string tre;  // "<v>_tre" of the first truncation-error variable (also in the MG hierarchy)
   int ntre = 0;     // AMRD_num_f_tre_vars
   int do_ex = 0;
@@ -332,7 +336,11 @@ struct EllGfns {
 
 bool ell_gfns(std::vector<EllGfns> &out, int &cmask) {
   out.clear();
-  for (const std::string &v : D.ell) {
+  // num = AMRD_num_elliptic_vars (+ AMRD_num_elliptic_vars_t0 while is_t0),
+  // amrd/mg.c:248,279,327,612
+  std::vector<std::string> all = D.ell;
+  if (D.is_t0) all.insert(all.end(), D.ell_t0.begin(), D.ell_t0.end());
+  for (const std::string &v : all) {
     EllGfns e;
     e.f = PAMR_get_gfn((char *)v.c_str(), PAMR_MGH, 0);
     e.res = PAMR_get_gfn((char *)(v + "_res").c_str(), PAMR_MGH, 0);
@@ -499,6 +507,56 @@ bool extrap_gfns(std::vector<ExtrapGfns> &out, const char *who) {
   return true;
 }
 
+// <v>_res, <v>_lop, <v>_rhs of each elliptic variable (amrd/io.c:850-897)
+bool def_mg_work_vars(const char *who, const std::vector<std::string> &ell, const std::vector<std::string> &fw,
+                      const std::vector<std::string> &i2) {
+  int pbt[6] = {PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN};
+  for (const std::string &v : ell) {
+    Var *f = find_var(v.c_str());
+    if (!f || !f->a.in_mgh) {
+      std::printf("%s: error ... elliptic variable <%s> is not defined in the MG hierarchy\n", who, v.c_str());
+      return false;
+    }
+    const int mg_inject = in_list(v + "_res", fw) ? PAMR_FW_RESTR : PAMR_HW_RESTR;
+    const int mg_interp = in_list(v, i2) ? PAMR_SECOND_ORDER : PAMR_NO_INTERP;
+    if (!PAMR_def_var_full((char *)(v + "_res").c_str(), 0, 1, 0, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP,
+                           PAMR_NO_SYNC, mg_inject, mg_interp, PAMR_NO_SYNC, 0, PAMR_NO_TRANSFER, pbt))
+      return false;
+    for (const char *suf : {"_lop", "_rhs"})
+      if (!PAMR_def_var_full((char *)(v + suf).c_str(), 0, 1, 0, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP,
+                             PAMR_NO_SYNC, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_SYNC, 0, PAMR_NO_TRANSFER, pbt))
+        return false;
+  }
+  return true;
+}
+
+// sync / fourth-order interp / straight inject of a list of MG variables
+// (sync_cnst_data ... inject_elliptic_vars_t0_data, amrd/mg.c:622-698)
+bool mg_list_transfer(const std::vector<std::string> &vars, int what, int L) {
+  PAMR_freeze_tf_bits();
+  PAMR_clear_tf_bits();
+  const int bit = what == 0 ? PAMR_SYNC : what == 1 ? PAMR_FOURTH_ORDER : PAMR_STRAIGHT_INJECT;
+  for (const std::string &v : vars) PAMR_set_tf_bit(PAMR_get_gfn((char *)v.c_str(), PAMR_MGH, 0), bit);
+  const int ok = what == 0 ? PAMR_sync(L, 0, PAMR_MGH, 0) : what == 1 ? PAMR_interp(L, 0, PAMR_MGH)
+                                                                    : PAMR_inject(L, 0, PAMR_MGH);
+  PAMR_thaw_tf_bits();
+  return ok != 0;
+}
+
+// the t = 0 hook pass of solve_elliptics (amrd/mg.c:45-52, 60-68): the hook on
+// every MG level from the coarsest up, each level synchronised and
+// interpolated to the next, then injected back down
+bool t0_pass(amrd_void_hook h, const std::vector<std::string> &vars, int Lf) {
+  for (int l = 1; l <= Lf; l++) {
+    call_app(h, l);
+    if (!mg_list_transfer(vars, 0, l)) return false;
+    if (l < Lf && !mg_list_transfer(vars, 1, l)) return false;
+  }
+  for (int l = Lf; l > 1; l--)
+    if (!mg_list_transfer(vars, 2, l)) return false;
+  return true;
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -545,6 +603,8 @@ void PAMR_free_context(int num) {
   delete C;
   C = nullptr;
   D.ell.clear();  // AMRD's variable tables belong to the context's variables
+  D.ell_t0.clear();
+  D.cnst.clear();
   D.tre.clear();
   D.ntre = 0;
 }
@@ -1169,6 +1229,47 @@ void amrd_set_app_MG_level_hooks(amrd_level_relax_hook relax, amrd_level_real_ho
   D.llop = lop;
 }
 
+
+// amrd/io.c:843-937 (stage 4), AMRD's elliptic_vars_t0
+int amrd_mg_def_vars_t0(const char *elliptic_vars_t0, const char *mg_fw_restr, const char *mg_interp2) {
+  if (!C) {
+    std::printf("amrd_mg_def_vars_t0: error ... no current context\n");
+    return 0;
+  }
+  const std::vector<std::string> t0 = split_list(elliptic_vars_t0);
+  if (!def_mg_work_vars("amrd_mg_def_vars_t0", t0, split_list(mg_fw_restr), split_list(mg_interp2))) return 0;
+  D.ell_t0 = t0;
+  return 1;
+}
+
+// AMRD's MG_cnst_data_vars (amrd/io.c:354-358, 1102): MG variables of the application
+int amrd_mg_def_cnst_data_vars(const char *vars) {
+  if (!C) {
+    std::printf("amrd_mg_def_cnst_data_vars: error ... no current context\n");
+    return 0;
+  }
+  const std::vector<std::string> v = split_list(vars);
+  for (const std::string &n : v) {
+    Var *f = find_var(n.c_str());
+    if (!f || !f->a.in_mgh) {
+      std::printf("amrd_mg_def_cnst_data_vars: error ... <%s> is not defined in the MG hierarchy\n", n.c_str());
+      return 0;
+    }
+  }
+  D.cnst = v;
+  return 1;
+}
+
+void amrd_set_is_t0(int is_t0) { D.is_t0 = is_t0 ? 1 : 0; }
+
+// amrd/amrd.c:109-112
+void amrd_set_elliptic_vars_t0_init(amrd_void_hook app_elliptic_vars_t0_init) {
+  D.t0_init = app_elliptic_vars_t0_init;
+}
+
+// the app_t0_cnst_data argument of amrd() (amrd/amrd.c:21,51)
+void amrd_set_app_t0_cnst_data(amrd_void_hook app_t0_cnst_data) { D.t0_cnst = app_t0_cnst_data; }
+
 // amrd/io.c:843-937 (stage 1) and 988-1005
 int amrd_mg_def_vars(const char *elliptic_vars, const char *mg_fw_restr, const char *mg_interp2, int do_ex) {
   if (!C) {
@@ -1178,23 +1279,7 @@ int amrd_mg_def_vars(const char *elliptic_vars, const char *mg_fw_restr, const c
   const std::vector<std::string> ell = split_list(elliptic_vars), fw = split_list(mg_fw_restr),
                                  i2 = split_list(mg_interp2);
   int pbt[6] = {PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN, PAMR_UNKNOWN};
-  for (const std::string &v : ell) {
-    Var *f = find_var(v.c_str());
-    if (!f || !f->a.in_mgh) {
-      std::printf("amrd_mg_def_vars: error ... elliptic variable <%s> is not defined in the MG hierarchy\n",
-                  v.c_str());
-      return 0;
-    }
-    const int mg_inject = in_list(v + "_res", fw) ? PAMR_FW_RESTR : PAMR_HW_RESTR;
-    const int mg_interp = in_list(v, i2) ? PAMR_SECOND_ORDER : PAMR_NO_INTERP;
-    if (!PAMR_def_var_full((char *)(v + "_res").c_str(), 0, 1, 0, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP,
-                           PAMR_NO_SYNC, mg_inject, mg_interp, PAMR_NO_SYNC, 0, PAMR_NO_TRANSFER, pbt))
-      return 0;
-    for (const char *suf : {"_lop", "_rhs"})
-      if (!PAMR_def_var_full((char *)(v + suf).c_str(), 0, 1, 0, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP,
-                             PAMR_NO_SYNC, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_SYNC, 0, PAMR_NO_TRANSFER, pbt))
-        return 0;
-  }
+  if (!def_mg_work_vars("amrd_mg_def_vars", ell, fw, i2)) return 0;
   if (!PAMR_def_var_full((char *)"cmask", 1, 1, 1, PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_INTERP, PAMR_NO_SYNC,
                          PAMR_NO_INJECT, PAMR_NO_INTERP, PAMR_NO_SYNC, 1, PAMR_NO_TRANSFER, pbt))
     return 0;
@@ -1434,9 +1519,14 @@ int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out) {
     return -1;
   }
   const int Lf_amr = PAMR_get_max_lev(PAMR_AMRH);
-  if (D.ell.empty()) return 0;
+  if ((D.ell.empty() && !D.is_t0) || (D.ell.empty() && D.ell_t0.empty() && D.is_t0)) return 0;  // mg.c:28-29
   if (!PAMR_build_mgh(L, Lf_amr, tl)) {
     std::printf("solve_elliptics: PAMR_build_mgh() failed\n");
+    return -1;
+  }
+  const int Lf = PAMR_get_max_lev(PAMR_MGH);
+  if (D.t0_init && D.is_t0 && !t0_pass(D.t0_init, D.ell_t0, Lf)) {  // mg.c:45-52
+    PAMR_destroy_mgh();
     return -1;
   }
   double res = D.tol + 1;
@@ -1444,6 +1534,10 @@ int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out) {
   while ((it < D.min_iter || res > D.tol) && it < D.max_iter) {
     res = amrd_vcycle_tre(it, save_tre);
     if (res < 0) {
+      PAMR_destroy_mgh();
+      return -1;
+    }
+    if (!D.cnst.empty() && D.is_t0 && (!D.t0_cnst || !t0_pass(D.t0_cnst, D.cnst, Lf))) {  // mg.c:60-68
       PAMR_destroy_mgh();
       return -1;
     }

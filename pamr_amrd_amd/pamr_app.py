@@ -50,7 +50,7 @@ class NpsContext:
     refined levels' boxes (``levels[l]``: boxes of AMR level l + 2)."""
 
     def __init__(self, base_n, levels, op=0, rop_u=3, rop_res=3, rop_src=3, mcw=5,
-                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tre_vars=None, num_tl=1, extrap=False):
+                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tre_vars=None, num_tl=1, extrap=False, extra_defs=None):
         self.L, self.app = lib(), nps_lib()
         L = self.L
         shape = (C.c_int * 3)(base_n, base_n, base_n)
@@ -64,6 +64,8 @@ class NpsContext:
             raise RuntimeError("nps_define_vars failed")
         if extrap and not L.amrd_mg_def_extrap_vars(b"", b""):
             raise RuntimeError("amrd_mg_def_extrap_vars failed")
+        if extra_defs is not None:
+            extra_defs(L)  # more variables, before the hierarchy is composed
         if tre_vars and not L.amrd_def_tre_vars(tre_vars.encode()):
             raise RuntimeError("amrd_def_tre_vars failed")
         boxes = [list(self.bbox)] + [list(b) for lv in levels for b in lv]

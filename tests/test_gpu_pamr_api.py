@@ -317,3 +317,131 @@ def test_extrapolation_needs_its_variables(torch_cuda):
     assert ctx.L.amrd_mg_extrap(2, 0, 1.0, 0.1, 1.0, 0.1) == 0             # V has one time level
     assert ctx.L.amrd_update_mg_extrap_vars(1, ctx.max_lev, 1.0) == 0
     ctx.close()
+
+
+HOOK = C.CFUNCTYPE(None)
+
+
+def _coords(shape, bb):
+    nz, ny, nx = shape
+    x = bb[0] + np.arange(nx) * ((bb[1] - bb[0]) / (nx - 1))
+    y = bb[2] + np.arange(ny) * ((bb[3] - bb[2]) / (ny - 1))
+    z = bb[4] + np.arange(nz) * ((bb[5] - bb[4]) / (nz - 1))
+    return np.meshgrid(z, y, x, indexing="ij")
+
+
+def _t0_term(shape, bb):
+    """what the test hooks add on a grid: a cubic in the coordinates plus the
+    grid spacing (dyadic values: the same bits wherever computed)"""
+    Z, Y, X = _coords(shape, bb)
+    return X * X * X - 2.0 * Y * Y + Z + (bb[1] - bb[0]) / (shape[2] - 1)
+
+
+def _cur_grid(ctx):
+    L = ctx.L
+    rank, dim, ngfs = C.c_int(), C.c_int(), C.c_int()
+    shp = (C.c_int * 3)()
+    gw = (C.c_int * 6)()
+    bb = np.zeros(6)
+    t = C.c_double()
+    x = (C.POINTER(C.c_double) * 3)()
+    gfs = (C.c_void_p * 512)()
+    assert L.PAMR_get_g_attribs(C.byref(rank), C.byref(dim), shp, pm.P(bb), gw, C.byref(t), C.byref(ngfs), x, gfs)
+    return (shp[2], shp[1], shp[0]), bb, gfs
+
+
+def _rmw(ctx, name, f):
+    """read-modify-write grid function `name` of the iterator's current grid"""
+    shape, bb, gfs = _cur_grid(ctx)
+    ptr = gfs[ctx.gfn(name, PAMR_MGH) - 1]
+    a = np.empty(shape)
+    assert ctx.app.nps_download(pm.P(a), ptr, a.size)
+    a = np.ascontiguousarray(f(a, shape, bb))
+    assert ctx.app.nps_upload(ptr, pm.P(a), a.size)
+
+
+def _replay_pass(m, which, add):
+    """the oracle side of one t = 0 hook pass over the MG levels"""
+    Lf = m.nlev
+    for L in range(1, Lf + 1):
+        for g in range(m.ngrids(L)):
+            shape, bb = m.grid(L, g)
+            f = m.field(L, g, which)
+            f[:] = f + add(L, g, shape, bb)
+        m.sync_field(L, 0, which)
+        if L < Lf:
+            m.interp_field(L, which, ord=4)
+    for L in range(Lf, 1, -1):
+        m.inject_field(L, which, 1)
+
+
+def test_t0_branches_match_oracle(torch_cuda):
+    """solve_elliptics at t = 0 (amrd/mg.c:45-68): the elliptic_vars_t0 init
+    pass before the cycles and the constraint-data pass after each cycle --
+    hook on every MG level from the coarsest up, sync, fourth-order interp to
+    the next level, straight injection back down -- against the oracle's
+    replay of the same transfers, bit for bit on every AMR grid.  The hooks
+    add to what the interpolation left (W += cubic + h, Cd += V + h)."""
+    levels = layout(5)
+    print(f"case t0 branches boxes={[len(l) for l in levels]}", flush=True)
+    pbt = (C.c_int * 6)()
+
+    def defs(L):
+        for nm in (b"W", b"Cd"):   # AMR and MG, one time level, no automatic transfers
+            assert L.PAMR_def_var_full(nm, 1, 1, 1, 0, 0, 0, 0, 0, 0, 0, 1, 0, pbt) > 0
+        assert L.amrd_mg_def_vars_t0(b"W", b"", b"") == 1
+        assert L.amrd_mg_def_cnst_data_vars(b"Cd") == 1
+
+    m = O.AmrMG(17, levels)
+    ctx = NpsContext(17, levels, extra_defs=defs)
+    load_sources(m, ctx, 0, 31)
+    m.set_params()
+    m.init()
+    for l in range(1, ctx.max_lev + 1):
+        for g, (shape, _, _) in enumerate(ctx.grids(PAMR_AMRH, l)):
+            for nm in ("W", "Cd"):
+                ctx.upload(PAMR_AMRH, l, g, nm, np.zeros(shape))
+    t0_init = HOOK(lambda: _rmw(ctx, "W", lambda a, shape, bb: a + _t0_term(shape, bb)))
+
+    def cnst():
+        shape, bb, gfs = _cur_grid(ctx)
+        v = np.empty(shape)
+        assert ctx.app.nps_download(pm.P(v), gfs[ctx.gfn("V", PAMR_MGH) - 1], v.size)
+        _rmw(ctx, "Cd", lambda a, sh, b: a + (v + (b[1] - b[0]) / (sh[2] - 1)))
+    t0_cnst = HOOK(cnst)
+    L = ctx.L
+    L.amrd_set_elliptic_vars_t0_init(C.cast(t0_init, C.c_void_p))
+    L.amrd_set_app_t0_cnst_data(C.cast(t0_cnst, C.c_void_p))
+    # W is a t = 0 elliptic variable: switch the t = 0 solve off for the
+    # cycles (the oracle has one elliptic variable), on for the init pass
+    # alone (no cycles), then the constraint pass around two cycles
+    L.amrd_set_is_t0(1)
+    ctx.set_params(pre=2, pst=2, min_iter=0, max_iter=0, tol=1.0)
+    res = C.c_double()
+    assert L.amrd_solve_elliptics(1, 1, C.byref(res)) == 0
+    _replay_pass(m, O.OA_TRE, lambda Lm, g, shape, bb: _t0_term(shape, bb))
+    for l in range(1, ctx.max_lev + 1):
+        Lm = m.base + l - 1
+        for g in range(m.ngrids(Lm)):
+            got, want = ctx.download(PAMR_AMRH, l, g, "W"), m.field(Lm, g, O.OA_TRE)
+            assert np.array_equal(got, want), ("W", l, g, np.max(np.abs(got - want)))
+    # constraint data: the oracle's OA_TRE now plays Cd
+    for Lm in range(1, m.nlev + 1):
+        for g in range(m.ngrids(Lm)):
+            m.field(Lm, g, O.OA_TRE)[:] = 0.0
+    L.amrd_mg_def_vars_t0(b"", b"", b"")   # no t = 0 elliptic variable in the cycles
+    ctx.set_params(pre=2, pst=2, min_iter=2, max_iter=2, tol=0.0)
+    assert L.amrd_solve_elliptics(1, 1, C.byref(res)) == 2
+    for _ in range(2):
+        m.vcycle()
+        _replay_pass(m, O.OA_TRE,
+                     lambda Lm, g, shape, bb: m.field(Lm, g, O.OA_U) + (bb[1] - bb[0]) / (shape[2] - 1))
+    for l in range(1, ctx.max_lev + 1):
+        Lm = m.base + l - 1
+        for g in range(m.ngrids(Lm)):
+            assert np.array_equal(ctx.download(PAMR_AMRH, l, g, "V"), m.field(Lm, g, O.OA_U)), ("V", l, g)
+            got, want = ctx.download(PAMR_AMRH, l, g, "Cd"), m.field(Lm, g, O.OA_TRE)
+            assert np.array_equal(got, want), ("Cd", l, g, np.max(np.abs(got - want)))
+    L.amrd_set_elliptic_vars_t0_init(None)
+    L.amrd_set_app_t0_cnst_data(None)
+    ctx.close()
