@@ -22,6 +22,7 @@ extern "C" {
 
 typedef double (*amrd_real_hook)(void);
 typedef void (*amrd_void_hook)(void);
+typedef void (*amrd_evolve_hook)(int iter);
 typedef double (*amrd_level_relax_hook)(int L, int comm_only);
 typedef double (*amrd_level_real_hook)(int L);
 typedef void (*amrd_level_void_hook)(int L);
@@ -99,6 +100,19 @@ int amrd_mg_def_cnst_data_vars(const char *vars);
 void amrd_set_is_t0(int is_t0);
 void amrd_set_elliptic_vars_t0_init(amrd_void_hook app_elliptic_vars_t0_init);
 void amrd_set_app_t0_cnst_data(amrd_void_hook app_t0_cnst_data);
+
+/* tstep's iteration of the evolution and elliptic equations on AMR level
+ * lev (amrd/evolve.c:449-500; the dissipation and i-1 boundary steps around
+ * it are the application's): on the finest level with elliptic variables it
+ * builds the MG hierarchy PAMR_build_mgh(Lf, Lf, 1) -- here only when the
+ * finest level is level 1 --, then while (iter < evo_min_iter || evo_res >
+ * evo_tol || mg_res > MG tol) && iter < evo_max_iter: app_evolve(iter+1) on
+ * every grid of lev, PAMR_sync(lev, tl 1, AMR_bdy_width 0 first then 1),
+ * evo_res = max over the grids of app_evo_residual(), and (unless evo_ssc
+ * says it already converged) mg_res = one V-cycle.  Returns the iteration
+ * count (negative on error) and the last residuals. */
+int amrd_evo_iterate(int lev, amrd_evolve_hook app_evolve, amrd_real_hook app_evo_residual, int evo_min_iter,
+                     int evo_max_iter, double evo_tol, int evo_ssc, double *evo_res, double *mg_res);
 
 /* AMRD's MG parameters (amrd/io.c:97-106,269-285); defaults 3, 3, 1, 50,
  * 0.0, 1e-3, 5000, 1.0, 1.0 */

@@ -221,6 +221,7 @@ def _declare_pamr(L):
     L.amrd_def_tre_vars.argtypes = [s]
     L.amrd_mg_def_extrap_vars.argtypes = [s, s]
     L.amrd_mg_def_vars_t0.argtypes = [s, s, s]
+    L.amrd_evo_iterate.argtypes = [i, C.c_void_p, C.c_void_p, i, i, d, i, _dp, _dp]
     L.amrd_mg_def_cnst_data_vars.argtypes = [s]
     L.amrd_set_is_t0.argtypes = [i]
     L.amrd_set_is_t0.restype = None

@@ -1510,6 +1510,63 @@ double amrd_vcycle_tre(int giter, int save_tre) {
   return res[Lf];
 }
 
+// tstep's iteration of the evolution and the elliptic equations on level lev
+// (amrd/evolve.c:449-500), without the dissipation and i-1 boundary steps
+int amrd_evo_iterate(int lev, amrd_evolve_hook app_evolve, amrd_real_hook app_evo_residual, int evo_min_iter,
+                     int evo_max_iter, double evo_tol, int evo_ssc, double *evo_res_out, double *mg_res_out) {
+  if (!C || !C->A || !app_evolve || !app_evo_residual) {
+    std::printf("amrd_evo_iterate: error ... no AMR hierarchy or hooks\n");
+    return -1;
+  }
+  const int Lf = PAMR_get_max_lev(PAMR_AMRH);
+  const bool mg = lev == Lf && !D.ell.empty();
+  double mg_res = 0.0;
+  if (mg) {
+    mg_res = D.tol + 1;
+    if (!PAMR_build_mgh(Lf, Lf, 1)) {
+      std::printf("tstep: PAMR_build_mgh() failed\n");
+      return -1;
+    }
+  }
+  int iter = 0;
+  double evo_res = evo_tol + 1;
+  while ((iter < evo_min_iter || evo_res > evo_tol || mg_res > D.tol) && iter < evo_max_iter) {
+    if (evo_res > evo_tol || !evo_ssc || iter < evo_min_iter) {
+      int valid = PAMR_init_s_iter(lev, PAMR_AMRH, 0);  // call_app_1int (amrd/util.c)
+      while (valid) {
+        app_evolve(iter + 1);
+        valid = PAMR_next_g();
+      }
+      if (!PAMR_sync(lev, 1, PAMR_AMRH, iter == 0 ? 0 : 1)) return -1;
+      double r = 0.0;
+      bool first = true;
+      valid = PAMR_init_s_iter(lev, PAMR_AMRH, 0);  // call_rr_app, max over the grids
+      while (valid) {
+        const double g = app_evo_residual();
+        r = first ? g : std::max(g, r);
+        first = false;
+        valid = PAMR_next_g();
+      }
+      evo_res = r;
+    }
+    if (mg && (!evo_ssc || mg_res > D.tol || iter < evo_min_iter)) {
+      mg_res = amrd_vcycle_tre(iter, 0);
+      if (mg_res < 0) {
+        PAMR_destroy_mgh();
+        return -1;
+      }
+    }
+    iter++;
+  }
+  if (iter == evo_max_iter && (evo_res > evo_tol || mg_res > D.tol))
+    std::printf("\nWARNING ... failed to solve evolution (mg) equations to within %lf (%lf) in %i iterations "
+                "(lev=%i): res=%12.8e (%12.8e)\n", evo_tol, D.tol, evo_max_iter, lev, evo_res, mg_res);
+  if (mg) PAMR_destroy_mgh();
+  if (evo_res_out) *evo_res_out = evo_res;
+  if (mg_res_out) *mg_res_out = mg_res;
+  return iter;
+}
+
 int amrd_solve_elliptics(int L, int tl, double *res_out) { return amrd_solve_elliptics_tre(L, tl, 0, res_out); }
 
 // amrd/mg.c:20-79 (no t0 initial-data or constraint hooks)

@@ -445,3 +445,81 @@ def test_t0_branches_match_oracle(torch_cuda):
     L.amrd_set_elliptic_vars_t0_init(None)
     L.amrd_set_app_t0_cnst_data(None)
     ctx.close()
+
+
+EVOLVE = C.CFUNCTYPE(None, C.c_int)
+REAL = C.CFUNCTYPE(C.c_double)
+
+
+def test_evo_iterate_matches_oracle(torch_cuda):
+    """tstep's evolution / elliptic iteration (amrd/evolve.c:449-500) on a
+    one-level hierarchy: a toy evolution phi(tl 1) = phi(tl 2) + dt V whose
+    result is the elliptic source (rho = phi(tl 1)), one V-cycle per
+    iteration, stopped by the MG tolerance; against the same loop over the
+    oracle's V-cycles -- iteration count, last MG residual and V, phi bit for
+    bit."""
+    pbt = (C.c_int * 6)()
+    dt = 1.0 / 64.0
+
+    def defs(L):
+        assert L.PAMR_def_var_full(b"phi", 1, 0, 2, 0, 0, 0, 1, 0, 0, 0, 0, 0, pbt) > 0
+
+    m = O.AmrMG(17, [])
+    ctx = NpsContext(17, [], extra_defs=defs)
+    assert ctx.max_lev == 1
+    shape, bb, _ = ctx.grids(PAMR_AMRH, 1)[0]
+    Z, Y, X = _coords(shape, bb)
+    phi2 = np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+    zero = np.zeros(shape)
+    ctx.upload(PAMR_AMRH, 1, 0, "phi", phi2, tl=2)
+    ctx.upload(PAMR_AMRH, 1, 0, "phi", phi2, tl=1)
+    ctx.upload(PAMR_AMRH, 1, 0, "rho", phi2)
+    ctx.upload(PAMR_AMRH, 1, 0, "V", zero)
+    m.field(m.base, 0, O.OA_RHO)[:] = phi2
+    L = ctx.L
+
+    def grid_arrays(names):
+        sh, b, gfs = _cur_grid(ctx)
+        out = {}
+        for nm, tl in names:
+            a = np.empty(sh)
+            assert ctx.app.nps_download(pm.P(a), gfs[ctx.gfn(nm, PAMR_AMRH, tl) - 1], a.size)
+            out[nm, tl] = a
+        return gfs, out
+
+    def evolve(it):
+        gfs, a = grid_arrays([("phi", 2), ("V", 1)])
+        p1 = np.ascontiguousarray(a["phi", 2] + dt * a["V", 1])
+        assert ctx.app.nps_upload(gfs[ctx.gfn("phi", PAMR_AMRH, 1) - 1], pm.P(p1), p1.size)
+        assert ctx.app.nps_upload(gfs[ctx.gfn("rho", PAMR_AMRH, 1) - 1], pm.P(p1), p1.size)
+
+    def evo_res():
+        _, a = grid_arrays([("phi", 1), ("phi", 2), ("V", 1)])
+        d = a["phi", 1] - a["phi", 2] - dt * a["V", 1]
+        return float(np.sqrt(np.mean(d * d)))
+
+    ev, er = EVOLVE(evolve), REAL(evo_res)
+    mg_tol = 1e-9
+    ctx.set_params(pre=2, pst=2, tol=mg_tol)
+    r_evo, r_mg = C.c_double(), C.c_double()
+    n_it = L.amrd_evo_iterate(1, C.cast(ev, C.c_void_p), C.cast(er, C.c_void_p), 2, 30, 1e-12, 0,
+                              C.byref(r_evo), C.byref(r_mg))
+    # the oracle's side of the same loop
+    m.set_params()
+    m.init()
+    it, mg_res, evo = 0, mg_tol + 1, 1.0
+    while (it < 2 or evo > 1e-12 or mg_res > mg_tol) and it < 30:
+        U = m.field(m.base, 0, O.OA_U)
+        p1 = phi2 + dt * U
+        m.field(m.base, 0, O.OA_RHO)[:] = p1
+        d = p1 - phi2 - dt * U
+        evo = float(np.sqrt(np.mean(d * d)))
+        mg_res = m.vcycle()[0]
+        it += 1
+    print(f"evo iterate: gpu {n_it} iterations mg_res {r_mg.value:.6e}, oracle {it} {mg_res:.6e}", flush=True)
+    assert n_it == it and 2 < it < 30
+    assert abs(r_mg.value - mg_res) <= 1e-12 * mg_res
+    assert r_evo.value == evo        # the last evolution residual (rounding-level, the same bits)
+    assert np.array_equal(ctx.download(PAMR_AMRH, 1, 0, "V"), m.field(m.base, 0, O.OA_U))
+    assert np.array_equal(ctx.download(PAMR_AMRH, 1, 0, "phi", tl=1), p1)
+    ctx.close()
