@@ -1147,6 +1147,7 @@ int coarse_solve(pmg_amr *A) {
 }
 
 int fold(pmg_amr *A) {
+  if (!A->d_fold) return -1;  // a failed re-tiling left no fold table
   hipLaunchKernelGGL(k_fold_max, dim3(A->nfold), dim3(256), 0, A->stream, A->ws, A->d_fold, A->d_norms);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
