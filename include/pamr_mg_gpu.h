@@ -244,6 +244,9 @@ int pmg_history_warm_start(pmg_solver *S, int K, const double *const *F, const d
 /* lane-layout self-test of the BF16 16x16x16 MFMA used by the network:
  * D = bf16(A) * bf16(B) (host 16x16 row-major) */
 int pmg_net_mfma_selftest(const float *A, const float *B, float *D_out);
+/* the same for the K = 32 instruction the inference kernels use: A 16x32,
+ * B 32x16 (row-major host FP32, rounded to BF16), D = A*B 16x16 */
+int pmg_net_mfma32_selftest(const float *A, const float *B, float *D_out);
 
 #ifdef __cplusplus
 }

@@ -98,6 +98,7 @@ def _declare(L):
     L.pmg_net_stream.argtypes = [_vp]
     L.pmg_net_stream.restype = _vp
     L.pmg_net_mfma_selftest.argtypes = [C.POINTER(C.c_float)] * 3
+    L.pmg_net_mfma32_selftest.argtypes = [C.POINTER(C.c_float)] * 3
     L.pmg_net_correct.argtypes = [_vp, _vp, C.c_float, _vp, d, i, i, i, _vp]
     L.pmg_net_recorrect.argtypes = [_vp, _vp, d, i, i, i, _vp]
     L.pmg_net_infer.argtypes = [_vp, _vp, C.c_float, _vp, i, i, i, i]
@@ -584,12 +585,13 @@ class Net:
 
 
 def mfma_selftest(A, B):
+    """D = A B through one MFMA: A 16x16, B 16x16 (K = 16) or A 16x32, B 32x16 (K = 32)."""
     A = np.ascontiguousarray(A, dtype=np.float32)
     B = np.ascontiguousarray(B, dtype=np.float32)
     D = np.zeros((16, 16), dtype=np.float32)
     fp = C.POINTER(C.c_float)
-    _check(lib().pmg_net_mfma_selftest(A.ctypes.data_as(fp), B.ctypes.data_as(fp), D.ctypes.data_as(fp)),
-           "pmg_net_mfma_selftest")
+    f = lib().pmg_net_mfma32_selftest if A.shape[1] == 32 else lib().pmg_net_mfma_selftest
+    _check(f(A.ctypes.data_as(fp), B.ctypes.data_as(fp), D.ctypes.data_as(fp)), "pmg_net_mfma_selftest")
     return D
 
 

@@ -31,6 +31,7 @@ using namespace pmg;
 namespace {
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 constexpr int NC = 16;      // channels (padded)
@@ -78,6 +79,23 @@ __global__ __launch_bounds__(64) void k_mfma_probe(const uint16_t *A, const uint
   }
   floatx4 acc = {0.f, 0.f, 0.f, 0.f};
   acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+  for (int i = 0; i < 4; i++) D[(4 * g + i) * 16 + m] = acc[i];
+}
+
+// The K = 32 form (v_mfma_f32_16x16x32_bf16): lane l holds A[l%16][8*(l/16)+j]
+// and B[8*(l/16)+j][l%16], j = 0..7; D as above.
+__global__ __launch_bounds__(64) void k_mfma32_probe(const uint16_t *A, const uint16_t *B, float *D) {
+  const int l = threadIdx.x, m = l & 15, g = l >> 4;
+  uint16_t a[8], b[8];
+  for (int j = 0; j < 8; j++) {
+    a[j] = A[m * 32 + 8 * g + j];
+    b[j] = B[(8 * g + j) * 16 + m];
+  }
+  bf16x8 av, bv;
+  __builtin_memcpy(&av, a, 16);
+  __builtin_memcpy(&bv, b, 16);
+  floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
   for (int i = 0; i < 4; i++) D[(4 * g + i) * 16 + m] = acc[i];
 }
 
@@ -135,21 +153,57 @@ __global__ __launch_bounds__(256) void k_conv16(const uint16_t *__restrict__ X,
   }
 }
 
+// GELU (tanh form) through the logistic function: 0.5 x (1 + tanh u) =
+// x / (1 + e^(-2u)), u = k (x + 0.044715 x^3); the inference epilogues use it
+// (the training kernels keep tanhf; the two agree to FP32 rounding)
+__device__ inline float gelu_fast(float x) {
+  const float k = 0.7978845608028654f;
+  const float u = k * (x + 0.044715f * x * x * x);
+  return x / (1.0f + __expf(-2.0f * u));
+}
+
+// Stores one MFMA result tile of BF16 activations channel-last: lane (m, grp)
+// holds channel m of points 4grp + i (v[i], i = 0..3); after a shuffle lane L
+// holds channels 4(L%4) .. +3 of point L/4 and writes them as one 8-byte store,
+// so the wave writes 16 points x 32 B contiguously.  `row` is the element
+// offset of the group's point 0, `npts` how many of its 16 points exist.
+__device__ inline void store_act16(uint16_t *__restrict__ Aout, size_t row, int npts, const uint16_t (&v)[4]) {
+  const int L = threadIdx.x & 63;
+  const unsigned w01 = (unsigned)v[0] | ((unsigned)v[1] << 16), w23 = (unsigned)v[2] | ((unsigned)v[3] << 16);
+  const int p = L >> 2, c0 = (L & 3) * 4, i = p & 3, sg = p >> 2;
+  unsigned out[2];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int src = (c0 + k) + 16 * sg;
+    const unsigned a = (unsigned)__builtin_amdgcn_ds_bpermute(src * 4, (int)w01);
+    const unsigned b = (unsigned)__builtin_amdgcn_ds_bpermute(src * 4, (int)w23);
+    const unsigned word = i < 2 ? a : b;
+    const unsigned h = (i & 1) ? (word >> 16) : (word & 0xffffu);
+    if (k & 1) out[k >> 1] |= h << 16;
+    else out[k >> 1] = h;
+  }
+  if (p < npts) *(uint2 *)(Aout + (row + (size_t)p) * NC + c0) = make_uint2(out[0], out[1]);
+}
+
 // Hidden and last layer of the inference path with the input staged in LDS:
 // a block owns a CTX x CTY tile of (x, y) columns and marches ZC planes in
 // z, keeping a ring of 4 input planes (tile plus a 1-point halo, 16 BF16
 // channels = 32 B per point, zero outside the box) in LDS.  Wave w computes
-// rows 2w, 2w+1 of the tile, two 16-point segments each: per tap one weight
-// fragment and four MFMAs whose A operands come from LDS.  One barrier per
-// plane: plane z+2 is stored into the slot plane z-2 left at the start of
-// step z, plane z+3 is fetched into registers, and step z reads z-1..z+1.
+// rows 2w, 2w+1 of the tile, two 16-point segments each.  The 27 taps go in
+// pairs through the K = 32 MFMA (14 per segment and plane, the last pair
+// padded with zeros): lanes 0-31 read 16 channel bytes of the pair's first
+// tap, lanes 32-63 of its second, as one 16-byte LDS read each; the 14 weight
+// fragments stay in registers.  One barrier per plane: plane z+2 is stored
+// into the slot plane z-2 left at the start of step z, plane z+3 is fetched
+// into registers, and step z reads z-1..z+1.
 //   EPI 3: A = bf16(gelu(conv + bias));  EPI 4: Z[p] = conv + bias, channel 0
 constexpr int CTX = 32, CTY = 8, CLX = CTX + 2, CLY = CTY + 2;
 constexpr int CCHUNK = CLY * CLX * 2;                  // 16-byte chunks per staged plane
 constexpr int CLOAD = (CCHUNK + 255) / 256;            // per thread
+constexpr int NPAIR = (NTAP + 1) / 2;                  // tap pairs of the K = 32 form
 template <int EPI, int ZC>
 __global__ __launch_bounds__(256) void k_conv16_tile(const uint16_t *__restrict__ X,
-                                                     const uint16_t *__restrict__ Wf,
+                                                     const uint16_t *__restrict__ Wp,
                                                      const float *__restrict__ bias, float *__restrict__ Zout,
                                                      uint16_t *__restrict__ Aout, Geom g) {
   __shared__ uint4 sX[4][CLY][CLX][2];
@@ -185,10 +239,12 @@ __global__ __launch_bounds__(256) void k_conv16_tile(const uint16_t *__restrict_
       if (idx < CCHUNK) S[idx] = reg[q];
     }
   };
-  // the 27 weight fragments stay in registers for the whole march
-  bf16x4 wk[NTAP];
+  // the 14 weight fragments stay in registers for the whole march
+  bf16x8 wk[NPAIR];
 #pragma unroll
-  for (int tap = 0; tap < NTAP; tap++) wk[tap] = *(const bf16x4 *)(Wf + ((size_t)tap * 64 + lane) * 4);
+  for (int pr = 0; pr < NPAIR; pr++) wk[pr] = *(const bf16x8 *)(Wp + ((size_t)pr * 64 + lane) * 8);
+  const int half = grp & 1, second = grp >> 1;  // channel half; first or second tap of the pair
+  const float bm = bias[EPI == 3 ? m : 0];
   fetch(z0 - 1);
   store(z0 - 1);
   fetch(z0);
@@ -204,31 +260,36 @@ __global__ __launch_bounds__(256) void k_conv16_tile(const uint16_t *__restrict_
 #pragma unroll
     for (int q = 0; q < 4; q++) acc[q] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int tap = 0; tap < NTAP; tap++) {
-      const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-      const bf16x4 bw = wk[tap];
+    for (int pr = 0; pr < NPAIR; pr++) {
+      const int tap = 2 * pr + second;
+      const bool live = tap < NTAP;
+      const int tp = live ? tap : 0;
+      const int dx = tp % 3 - 1, dy = (tp / 3) % 3 - 1, dz = tp / 9 - 1;
       const int slot = (z + dz) & 3;
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int r = 2 * w + (q >> 1), sg = q & 1;
-        const uint16_t *pt = (const uint16_t *)&sX[slot][r + 1 + dy][16 * sg + m + 1 + dx][0];
-        const bf16x4 a = *(const bf16x4 *)(pt + 4 * grp);
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bw, acc[q], 0, 0, 0);
+        bf16x8 a = *(const bf16x8 *)&sX[slot][r + 1 + dy][16 * sg + m + 1 + dx][half];
+        if (!live) a = bf16x8{};
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, wk[pr], acc[q], 0, 0, 0);
       }
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int y = y0 + 2 * w + (q >> 1);
-      if (y >= g.ny) continue;
+      if (y >= g.ny) continue;  // uniform per wave
+      const int xs = x0 + 16 * (q & 1);
+      const size_t row = (size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xs;
+      if constexpr (EPI == 3) {
+        uint16_t v[4];
 #pragma unroll
-      for (int i = 0; i < 4; i++) {
-        const int xx = x0 + 16 * (q & 1) + 4 * grp + i;
-        if (xx >= g.nx) continue;
-        const size_t pp = (size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx;
-        if constexpr (EPI == 3) {
-          Aout[pp * NC + m] = f2bf(gelu_f(acc[q][i] + bias[m]));
-        } else {
-          if (m == 0) Zout[pp] = acc[q][i] + bias[0];
+        for (int i = 0; i < 4; i++) v[i] = f2bf(gelu_fast(acc[q][i] + bm));
+        store_act16(Aout, row, min(16, g.nx - xs), v);
+      } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int xx = xs + 4 * grp + i;
+          if (m == 0 && xx < g.nx) Zout[row + 4 * grp + i] = acc[q][i] + bm;
         }
       }
     }
@@ -238,8 +299,8 @@ __global__ __launch_bounds__(256) void k_conv16_tile(const uint16_t *__restrict_
 
 // Layer 1 of the inference path on the same tiles: the plane tile of
 // bf16(scale * x) (one channel, 2 B per point, zero outside the box) is staged
-// in a 4-plane LDS ring; lane (m, grp) gathers its taps 16h + 4grp + j from
-// LDS for the two MFMAs with K = taps.
+// in a 4-plane LDS ring; lane (m, grp) gathers its taps 8grp + j from LDS for
+// one K = 32 MFMA whose K is the 27 taps (5 zero).
 template <int ZC>
 __global__ __launch_bounds__(256) void k_conv_in_tile(const float *__restrict__ x, float scale,
                                                       const uint16_t *__restrict__ Wfi,
@@ -277,8 +338,7 @@ __global__ __launch_bounds__(256) void k_conv_in_tile(const float *__restrict__ 
       if (idx < NPT) S[idx] = reg[q];
     }
   };
-  const bf16x4 bw0 = *(const bf16x4 *)(Wfi + ((size_t)0 * 64 + lane) * 4);
-  const bf16x4 bw1 = *(const bf16x4 *)(Wfi + ((size_t)1 * 64 + lane) * 4);
+  const bf16x8 bw = *(const bf16x8 *)(Wfi + (size_t)lane * 8);
   const float bm = bias[m];
   fetch(z0 - 1);
   store(z0 - 1);
@@ -294,28 +354,27 @@ __global__ __launch_bounds__(256) void k_conv_in_tile(const float *__restrict__ 
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       const int r = 2 * w + (q >> 1), sg = q & 1;
-      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      uint16_t av[8];
 #pragma unroll
-      for (int h = 0; h < 2; h++) {
-        bf16x4 a = {0, 0, 0, 0};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int tap = 16 * h + 4 * grp + j;
-          if (tap < NTAP) {
-            const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
-            a[j] = (short)sx[(z + dz) & 3][r + 1 + dy][16 * sg + m + 1 + dx];
-          }
+      for (int j = 0; j < 8; j++) {
+        const int tap = 8 * grp + j;
+        av[j] = 0;
+        if (tap < NTAP) {
+          const int dx = tap % 3 - 1, dy = (tap / 3) % 3 - 1, dz = tap / 9 - 1;
+          av[j] = sx[(z + dz) & 3][r + 1 + dy][16 * sg + m + 1 + dx];
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, h == 0 ? bw0 : bw1, acc, 0, 0, 0);
       }
+      bf16x8 a;
+      __builtin_memcpy(&a, av, 16);
+      floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bw, acc, 0, 0, 0);
       const int y = y0 + r;
-      if (y < g.ny) {
+      if (y < g.ny) {  // uniform per wave
+        const int xs = x0 + 16 * sg;
+        uint16_t v[4];
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int xx = x0 + 16 * sg + 4 * grp + i;
-          if (xx >= g.nx) continue;
-          Aout[((size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xx) * NC + m] = f2bf(gelu_f(acc[i] + bm));
-        }
+        for (int i = 0; i < 4; i++) v[i] = f2bf(gelu_fast(acc[i] + bm));
+        store_act16(Aout, (size_t)b * box + ((size_t)z * g.ny + y) * g.nx + xs, min(16, g.nx - xs), v);
       }
     }
     __syncthreads();
@@ -475,14 +534,23 @@ __global__ __launch_bounds__(256) void k_pack_frags(const float *__restrict__ W,
   Wb[idx] = f2bf(W[(26 - tap) * 256 + m * 16 + (4 * grp + j)]);
 }
 
-// layer-1 fragments of the inference path (K = taps of input channel 0):
-// Wfi[h][lane][j] = W[tap = 16h + 4(lane/16) + j][ci 0][co lane%16], 0 past tap 26
+// inference fragments for the K = 32 MFMA.  Layer 1 (K = taps of input
+// channel 0): Wfi[lane][j] = W[tap = 8(lane/16) + j][ci 0][co lane%16], 0 past
+// tap 26.  Hidden / last layer (K = a tap pair x 16 channels):
+// Wp[pr][lane][j] = W[tap = 2 pr + (lane/16)/2][ci = 8((lane/16)%2) + j][co lane%16]
 __global__ __launch_bounds__(256) void k_pack_frags_in(const float *__restrict__ W, uint16_t *__restrict__ Wfi) {
-  const int idx = blockIdx.x * 256 + threadIdx.x;  // over 2*64*4
-  if (idx >= 2 * 64 * 4) return;
-  const int h = idx / 256, lane = (idx / 4) % 64, j = idx % 4;
-  const int tap = 16 * h + 4 * (lane >> 4) + j;
+  const int idx = blockIdx.x * 256 + threadIdx.x;  // over 64*8
+  if (idx >= 64 * 8) return;
+  const int lane = idx / 8, j = idx % 8;
+  const int tap = 8 * (lane >> 4) + j;
   Wfi[idx] = tap < NTAP ? f2bf(W[tap * 256 + (lane & 15)]) : (uint16_t)0;
+}
+__global__ __launch_bounds__(256) void k_pack_frags_pair(const float *__restrict__ W, uint16_t *__restrict__ Wp) {
+  const int idx = blockIdx.x * 256 + threadIdx.x;  // over NPAIR*64*8
+  if (idx >= NPAIR * 64 * 8) return;
+  const int pr = idx / 512, lane = (idx / 8) % 64, j = idx % 8, grp = lane >> 4;
+  const int tap = 2 * pr + (grp >> 1), ci = 8 * (grp & 1) + j;
+  Wp[idx] = tap < NTAP ? f2bf(W[tap * 256 + ci * 16 + (lane & 15)]) : (uint16_t)0;
 }
 
 // u += coef * Z[p * STRIDE] at interior points of the grid (box = the grid)
@@ -508,7 +576,8 @@ struct pmg_net {
   float *m1 = nullptr, *m2 = nullptr, *grad = nullptr;
   uint8_t *mask = nullptr;  // trainable entries
   uint16_t *wf = nullptr, *wb = nullptr;  // 3 layers * 27*64*4 fragments each
-  uint16_t *wfi = nullptr;                // layer 1 with K = taps (inference path), 2*64*4
+  uint16_t *wfi = nullptr;                // layer 1 with K = taps (inference path), 64*8
+  uint16_t *wp = nullptr;                 // hidden and last layer, tap pairs: 2 * NPAIR*64*8
   int adam_t = 0;
   // activation buffers (grown on demand): X0, A1, A2, dA (bf16 [P][16]); Z1, Z2, Z3, dZ (fp32)
   size_t cap = 0;
@@ -568,6 +637,9 @@ int net_frags(pmg_net *N) {
                        N->wb + (size_t)l * NTAP * 256);
   }
   hipLaunchKernelGGL(k_pack_frags_in, dim3(2), dim3(256), 0, N->stream, N->w, N->wfi);
+  for (int l = 1; l < 3; l++)
+    hipLaunchKernelGGL(k_pack_frags_pair, dim3((NPAIR * 512 + 255) / 256), dim3(256), 0, N->stream,
+                       N->w + (size_t)l * LAYER_FLOATS, N->wp + (size_t)(l - 1) * NPAIR * 512);
   PMG_HOST_CHECK(hipGetLastError());
   N->frags_valid = true;
   return 0;
@@ -590,10 +662,10 @@ int net_infer(pmg_net *N, const float *x, float scale, const Geom &g) {
   constexpr int TZ = kTilePlanes;
   const dim3 gt((unsigned)(((g.nx + CTX - 1) / CTX) * ((g.ny + CTY - 1) / CTY) * ((g.nz + TZ - 1) / TZ) * g.nb));
   hipLaunchKernelGGL(k_conv_in_tile<TZ>, gt, dim3(256), 0, N->stream, x, scale, N->wfi, b1, N->ia1, g);
-  hipLaunchKernelGGL((k_conv16_tile<3, TZ>), gt, dim3(256), 0, N->stream, N->ia1, N->wf + NTAP * 256, b2,
-                     (float *)nullptr, N->ia2, g);
-  hipLaunchKernelGGL((k_conv16_tile<4, TZ>), gt, dim3(256), 0, N->stream, N->ia2, N->wf + 2 * NTAP * 256, b3,
-                     N->y, (uint16_t *)nullptr, g);
+  hipLaunchKernelGGL((k_conv16_tile<3, TZ>), gt, dim3(256), 0, N->stream, N->ia1, N->wp, b2, (float *)nullptr,
+                     N->ia2, g);
+  hipLaunchKernelGGL((k_conv16_tile<4, TZ>), gt, dim3(256), 0, N->stream, N->ia2, N->wp + NPAIR * 512, b3, N->y,
+                     (uint16_t *)nullptr, g);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
@@ -677,6 +749,26 @@ int net_loss_backward(pmg_net *N, const Geom &g, const float *T, int margin, dou
 
 extern "C" {
 
+int pmg_net_mfma32_selftest(const float *A, const float *B, float *D_out) {
+  // host A (16x32), B (32x16) rounded to bf16; D_out = A*B from one K = 32 MFMA
+  std::vector<uint16_t> ha(512), hb(512);
+  for (int i = 0; i < 512; i++) { ha[i] = f2bf(A[i]); hb[i] = f2bf(B[i]); }
+  uint16_t *dA, *dB;
+  float *dD;
+  PMG_HOST_CHECK(hipMalloc(&dA, 1024));
+  PMG_HOST_CHECK(hipMalloc(&dB, 1024));
+  PMG_HOST_CHECK(hipMalloc(&dD, 1024));
+  PMG_HOST_CHECK(hipMemcpy(dA, ha.data(), 1024, hipMemcpyHostToDevice));
+  PMG_HOST_CHECK(hipMemcpy(dB, hb.data(), 1024, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_mfma32_probe, dim3(1), dim3(64), 0, 0, dA, dB, dD);
+  PMG_HOST_CHECK(hipGetLastError());
+  PMG_HOST_CHECK(hipMemcpy(D_out, dD, 1024, hipMemcpyDeviceToHost));
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dD);
+  return 0;
+}
+
 int pmg_net_mfma_selftest(const float *A, const float *B, float *D_out) {
   // host A (16x16), B (16x16) rounded to bf16; D_out = A*B from one MFMA
   std::vector<uint16_t> ha(256), hb(256);
@@ -731,7 +823,8 @@ pmg_net *pmg_net_create(int hidden, unsigned long long seed) {
             hipMalloc(&N->mask, nf) == hipSuccess &&
             hipMalloc(&N->wf, 3 * NTAP * 256 * sizeof(uint16_t)) == hipSuccess &&
             hipMalloc(&N->wb, 3 * NTAP * 256 * sizeof(uint16_t)) == hipSuccess &&
-            hipMalloc(&N->wfi, 2 * 64 * 4 * sizeof(uint16_t)) == hipSuccess &&
+            hipMalloc(&N->wfi, 64 * 8 * sizeof(uint16_t)) == hipSuccess &&
+            hipMalloc(&N->wp, 2 * NPAIR * 512 * sizeof(uint16_t)) == hipSuccess &&
             hipMalloc(&N->lsum, sizeof(double)) == hipSuccess;
   if (!ok) { pmg_net_destroy(N); return nullptr; }
   (void)hipMemcpy(N->w, w.data(), nf * 4, hipMemcpyHostToDevice);
@@ -748,7 +841,7 @@ void pmg_net_destroy(pmg_net *N) {
   for (void *p : {(void *)N->w, (void *)N->m1, (void *)N->m2, (void *)N->grad, (void *)N->mask,
                   (void *)N->wf, (void *)N->wb, (void *)N->x0, (void *)N->a1, (void *)N->a2,
                   (void *)N->d16, (void *)N->d16b, (void *)N->z1, (void *)N->z2, (void *)N->z3,
-                  (void *)N->dz, (void *)N->ia1, (void *)N->ia2, (void *)N->y, (void *)N->wfi,
+                  (void *)N->dz, (void *)N->ia1, (void *)N->ia2, (void *)N->y, (void *)N->wfi, (void *)N->wp,
                   (void *)N->partial, (void *)N->lpart, (void *)N->lsum})
     if (p) (void)hipFree(p);
   if (N->stream) (void)hipStreamDestroy(N->stream);

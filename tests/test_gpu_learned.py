@@ -24,6 +24,10 @@ def test_mfma_lane_layout(pm):
     B = rng.integers(-8, 9, (16, 16)).astype(np.float32) / 4
     D = pm.mfma_selftest(A, B)
     assert np.array_equal(D, A @ B)
+    # the K = 32 instruction of the inference kernels
+    A2 = rng.integers(-8, 9, (16, 32)).astype(np.float32) / 8
+    B2 = rng.integers(-8, 9, (32, 16)).astype(np.float32) / 4
+    assert np.array_equal(pm.mfma_selftest(A2, B2), A2 @ B2)
 
 
 def _rand_weights(rng, scale=1.0):
