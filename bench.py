@@ -122,10 +122,22 @@ def sequence_leg(args, pm, torch):
     t_collect = time.perf_counter() - t0
     log(f"sequence: {X.shape[0]} training pairs from {T} solves in {t_collect:.1f} s")
     net = pm.Net(seed=3)
+
+    def fixed_crops_loss():
+        # the loss on the same 256 training crops before and after training
+        # (single minibatch losses are too noisy to compare)
+        tot = 0.0
+        for q in range(8):
+            xb, yb = X[32 * q:32 * (q + 1)].contiguous(), Y[32 * q:32 * (q + 1)].contiguous()
+            tot += net.loss_grad(xb, yb, (16, 16, 16), 32, margin=3)[0] / 8
+        return tot
+
+    loss_before = fixed_crops_loss()
     t0 = time.perf_counter()
     losses = SQ.train(net, X, Y, side=16, steps=args.seq_train_steps, batch=32, lr=2e-3, margin=3)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
+    loss_after = fixed_crops_loss()
     log(f"sequence: trained {args.seq_train_steps} Adam steps in {t_train:.1f} s, "
         f"loss {sum(losses[:20]) / 20:.3e} -> {sum(losses[-20:]) / 20:.3e}")
     S = seq.S
@@ -184,7 +196,8 @@ def sequence_leg(args, pm, torch):
         "network": S.net_stats(),
         "training": {"solves": T, "pairs": int(X.shape[0]), "crop": 16, "adam_steps": args.seq_train_steps,
                      "collect_s": round(t_collect, 2), "train_s": round(t_train, 2),
-                     "loss_first": sum(losses[:20]) / 20, "loss_last": sum(losses[-20:]) / 20},
+                     "loss_first": sum(losses[:20]) / 20, "loss_last": sum(losses[-20:]) / 20,
+                     "loss_256_crops_before": loss_before, "loss_256_crops_after": loss_after},
         "solves_timed": [T, T + K - 1],
         "ms_per_solve_first_10": round(sum(times["learned_fp32"][:10]) / min(10, K), 4),
         "ms_per_solve_max": round(max(times["learned_fp32"]), 4),
