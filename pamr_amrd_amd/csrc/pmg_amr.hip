@@ -1333,11 +1333,11 @@ pmg_amr *pmg_amr_create(const int *base_shape, const double *bbox, int namr, con
     if (!rc) rc = upload_plan(A, Lv.sync1, j1);
     if (!rc) rc = upload_plan(A, Lv.inject, ji);
     if (!rc) rc = upload_plan(A, Lv.interp, jp);
-    if (!rc && L > 1) {  // does the interp plan write every point of every grid of L exactly once?
+    if (!rc && L > 1) {  // does the (resolved) interp plan write every point of every grid of L exactly once?
       std::vector<std::vector<uint8_t>> cnt(Lv.g.size());
       for (size_t gi = 0; gi < Lv.g.size(); gi++) cnt[gi].assign(Lv.g[gi].size(), 0);
       bool once = true;
-      for (const XJob &J : jp)
+      for (const XJob &J : Lv.interp.jobs)
         for (int k = J.lo[2]; k <= J.hi[2] && once; k++)
           for (int j = J.lo[1]; j <= J.hi[1] && once; j++)
             for (int i = J.lo[0]; i <= J.hi[0]; i++) {
