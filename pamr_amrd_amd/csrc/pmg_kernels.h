@@ -250,44 +250,23 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = ld[m];
     }
   };
-  // acc64 mode: the FP64 values of plane k are loaded at the start of step k
-  // (acc_issue) and updated in store(k); every interior point is stored by
-  // exactly one block
-  constexpr int NST = (TX * TY + NT - 1) / NT;
-  constexpr bool ACC = std::is_same<T, float>::value;
-  double accv[ACC ? NST : 1];
-  auto acc_issue = [&](int k) {
-    if constexpr (ACC) {
-#pragma unroll
-      for (int m = 0; m < NST; m++) {
-        const int q = threadIdx.x + m * NT;
-        const int i = x0 + q % TX, j = y0 + q / TX;
-        if (q < TX * TY && i <= nx - 2 && j <= ny - 2) accv[m] = P.acc64[ix3(i, j, k, nx, ny)];
-      }
-    }
-  };
   auto store = [&](int k) {
     T(*S)[LX] = sU[k & 3];
-    if constexpr (ACC) {
-      if (P.acc64) {
-#pragma unroll
-        for (int m = 0; m < NST; m++) {
-          const int q = threadIdx.x + m * NT;
-          const int tx = q % TX, ty = q / TX;
-          const int i = x0 + tx, j = y0 + ty;
-          if (q < TX * TY && i <= nx - 2 && j <= ny - 2) {
-            const size_t p = ix3(i, j, k, nx, ny);
-            P.acc64[p] = accv[m] + (double)S[ty + 2][tx + 2];
-            uout[p] = T(0);
-          }
-        }
-        return;
-      }
-    }
     for (int q = threadIdx.x; q < TX * TY; q += NT) {
       const int tx = q % TX, ty = q / TX;
       const int i = x0 + tx, j = y0 + ty;
-      if (i <= nx - 2 && j <= ny - 2) uout[ix3(i, j, k, nx, ny)] = S[ty + 2][tx + 2];
+      if (i <= nx - 2 && j <= ny - 2) {
+        const size_t p = ix3(i, j, k, nx, ny);
+        if constexpr (std::is_same<T, float>::value) {
+          // every interior point is stored by exactly one block
+          if (P.acc64) {
+            P.acc64[p] = P.acc64[p] + (double)S[ty + 2][tx + 2];
+            uout[p] = T(0);
+            continue;
+          }
+        }
+        uout[p] = S[ty + 2][tx + 2];
+      }
     }
   };
 
@@ -317,7 +296,6 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   // step k: issue plane k+3+PF and the next step's rho/rhs; pass 0 on k+1;
   // pass 1 on k; store k; commit plane k+3 (issued PF steps earlier)
   auto step = [&](int k, const T(&ldc)[NLD], T(&ldi)[NLD]) {
-    if constexpr (ACC) { if (P.acc64) acc_issue(k); }
     load_issue(ldi, k + 3 + PF);
     phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0);
     phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1);
