@@ -173,10 +173,6 @@ int pmg_set_bottom_fusion(pmg_solver *S, int on);
  * and f_res instead of storing it (default on; no excision, full weighting
  * for both); results are identical either way */
 int pmg_set_restrict_fusion(pmg_solver *S, int on);
-/* mixed precision (pmg_solve_rel mode 1): add the FP32 correction e to the
- * FP64 u inside the last finest-level sweep of the FP32 cycle instead of in
- * a separate pass (default on); results are identical either way */
-int pmg_set_add_fusion(pmg_solver *S, int on);
 /* run pairs of red-black sweeps on levels of at least 65 points per side as
  * one temporally blocked pass (default off: at 257^3 a pair takes 0.26 ms
  * against 2 x 0.11 ms for single sweeps; zchunk > 0 sets its z depth);

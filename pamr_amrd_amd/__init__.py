@@ -83,7 +83,6 @@ def _declare(L):
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_set_bottom_fusion.argtypes = [_vp, i]
     L.pmg_set_restrict_fusion.argtypes = [_vp, i]
-    L.pmg_set_add_fusion.argtypes = [_vp, i]
     L.pmg_set_sweep_blocking.argtypes = [_vp, i, i]
     L.pmg_stream.argtypes = [_vp]
     L.pmg_stream.restype = _vp
@@ -398,10 +397,6 @@ class Solver:
 
     def set_restrict_fusion(self, on):
         _check(self._L.pmg_set_restrict_fusion(self.h, 1 if on else 0), "pmg_set_restrict_fusion")
-
-    def set_add_fusion(self, on):
-        """mixed precision: u += e inside the last finest FP32 sweep"""
-        _check(self._L.pmg_set_add_fusion(self.h, 1 if on else 0), "pmg_set_add_fusion")
 
     def set_sweep_blocking(self, on, zchunk=0):
         _check(self._L.pmg_set_sweep_blocking(self.h, 1 if on else 0, zchunk), "pmg_set_sweep_blocking")

@@ -37,10 +37,6 @@ struct SweepP {
   NormWs ws;
   int slot;  // norm workspace slot
   int nb;
-  // FP32 only, optional: the stored result is added to this FP64 field
-  // (acc64 = acc64 + (double)u) and u_out receives zero -- the defect
-  // correction's update u64 += e fused into the last sweep of the cycle
-  double *acc64 = nullptr;
 };
 
 // Position of colour-slot q of a sweep phase on plane k.  HALO (pass 0):
@@ -255,18 +251,7 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
     for (int q = threadIdx.x; q < TX * TY; q += NT) {
       const int tx = q % TX, ty = q / TX;
       const int i = x0 + tx, j = y0 + ty;
-      if (i <= nx - 2 && j <= ny - 2) {
-        const size_t p = ix3(i, j, k, nx, ny);
-        if constexpr (std::is_same<T, float>::value) {
-          // every interior point is stored by exactly one block
-          if (P.acc64) {
-            P.acc64[p] = P.acc64[p] + (double)S[ty + 2][tx + 2];
-            uout[p] = T(0);
-            continue;
-          }
-        }
-        uout[p] = S[ty + 2][tx + 2];
-      }
+      if (i <= nx - 2 && j <= ny - 2) uout[ix3(i, j, k, nx, ny)] = S[ty + 2][tx + 2];
     }
   };
 

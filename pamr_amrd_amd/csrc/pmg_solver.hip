@@ -135,7 +135,6 @@ struct pmg_solver {
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   int bottom_max_n = 17;  // ... those with at most this many points per side
   int use_fused_restrict = 1;  // finest residual + injection in one pass
-  int fuse_add = 1;            // mixed precision: u += e inside the last finest FP32 sweep
   int use_sweep2 = 0;          // pairs of sweeps temporally blocked on large levels
                                // (measured slower than single sweeps; off by default)
   int sweep2_zchunk = 32;
@@ -309,15 +308,11 @@ bool use_sweep2(const pmg_solver *S, const Level &Lv) {
   return S->use_sweep2 && Lv.n[0] >= kSweep2MinN && Lv.n[1] >= kSweep2MinN && Lv.n[2] >= kSweep2MinN;
 }
 
-// n red-black sweeps on level L (ping-pong u <-> ua; odd counts copy back).
-// acc64 (FP32 only): the last sweep adds its result to that FP64 field and
-// leaves zero behind (*fused reports whether it did; the two-sweep kernel
-// does not fuse)
+// n red-black sweeps on level L (ping-pong u <-> ua; odd counts copy back)
 template <typename T>
-int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0, double *acc64 = nullptr, bool *fused = nullptr) {
+int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
   Level &Lv = S->lev[L - 1];
   LevelArrays<T> &A = arrays<T>(Lv);
-  if (fused) *fused = false;
   if (n <= 0) return 0;
   if (!Lv.rbox.ok) return 0;
   SweepP<T> P;
@@ -347,10 +342,6 @@ int sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0, double *acc64 = nul
       else rc = S->do_ex ? launch_sweep2<T, 1, true>(S, Q, p2, rhs0) : launch_sweep2<T, 1, false>(S, Q, p2, rhs0);
       done += 2;
     } else {
-      if (std::is_same<T, float>::value && acc64 && done + 1 == n) {
-        P.acc64 = acc64;
-        if (fused) *fused = true;
-      }
       if (S->op == 0) rc = S->do_ex ? launch_sweep<T, 0, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 0, false>(S, variant, P, p2, rhs0);
       else rc = S->do_ex ? launch_sweep<T, 1, true>(S, variant, P, p2, rhs0) : launch_sweep<T, 1, false>(S, variant, P, p2, rhs0);
       done += 1;
@@ -697,12 +688,9 @@ int resid_restrict(pmg_solver *S, int slot) {
   return p2 ? launch_resid_restrict_t<T, 1, true>(S, P) : launch_resid_restrict_t<T, 1, false>(S, P);
 }
 
-// acc64 / fused: passed to the finest level's post-smoothing (see sweeps)
 template <typename T>
-int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cnorm, int *citer,
-                   double *acc64 = nullptr, bool *fused = nullptr) {
+int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cnorm, int *citer) {
   const int Lf = S->nlev;
-  if (fused) *fused = false;
   const int Lb = S->use_bottom ? bottom_top(S) : 0;
   int rc;
   for (int L = Lf; L >= 1; L--) {
@@ -732,9 +720,7 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
   }
   for (int L = Lb ? Lb + 1 : 2; L <= Lf; L++) {
     if ((rc = cgc<T>(S, L))) return rc;
-    if ((rc = sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf, L == Lf ? acc64 : nullptr,
-                        L == Lf ? fused : nullptr)))
-      return rc;
+    if ((rc = sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
     if ((rc = settle<T>(S, L))) return rc;
   }
   if (final_residual) {
@@ -785,17 +771,9 @@ int enq_mixed(pmg_solver *S) {
   const int Lf = S->nlev;
   Level &F = S->lev[Lf - 1];
   int rc;
-  // the update u += e rides on the last finest-level sweep when it can (the
-  // boundary shell of e stays zero: no kernel of the cycle writes a nonzero
-  // value there); otherwise k_add_f32 does it
-  bool fused = false;
-  if ((rc = enqueue_vcycle<float>(S, false, NS_F32, S->d_norms + NS_COARSE + 2, S->d_citer + 1,
-                                  S->fuse_add ? F.d.u : nullptr, &fused)))
-    return rc;
-  if (!fused) {
-    hipLaunchKernelGGL(k_add_f32, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.u, F.f.u, F.size);
-    PMG_HOST_CHECK(hipGetLastError());
-  }
+  if ((rc = enqueue_vcycle<float>(S, false, NS_F32, S->d_norms + NS_COARSE + 2, S->d_citer + 1))) return rc;
+  hipLaunchKernelGGL(k_add_f32, dim3(nblk256(F.size)), dim3(256), 0, S->stream, F.d.u, F.f.u, F.size);
+  PMG_HOST_CHECK(hipGetLastError());
   if ((rc = residual<double>(S, Lf, NS_OUTER, true, true, F.f.rho))) return rc;
   return fold_norms(S, 0, 2 * kNormSlots);
 }
@@ -999,13 +977,6 @@ int pmg_num_sweep_variants(void) { return kNumSweepVariants; }
 int pmg_set_graph(pmg_solver *S, int on) {
   if (!S) return -1;
   S->use_graph = on ? 1 : 0;
-  invalidate_graphs(S);
-  return 0;
-}
-
-int pmg_set_add_fusion(pmg_solver *S, int on) {
-  if (!S) return -1;
-  S->fuse_add = on ? 1 : 0;
   invalidate_graphs(S);
   return 0;
 }

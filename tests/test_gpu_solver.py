@@ -207,37 +207,6 @@ def test_bottom_fusion_matches_unfused(pm, n):
     Su.close()
 
 
-@pytest.mark.parametrize("n,op,pst", [(65, 0, 2), (129, 0, 2), (65, 1, 2), (65, 0, 1), (33, 0, 0)])
-def test_add_fusion_matches_separate_update(pm, n, op, pst):
-    """u += e inside the last finest FP32 sweep leaves u, the outer residual
-    history and the next cycles' inputs identical to the separate update
-    pass (pst = 0: no post-sweep, the separate pass runs either way)"""
-    print(f"case add fusion n={n} op={op} pst={pst}", flush=True)
-    rng = np.random.default_rng(7)
-    rho = -pm.gaussian_source(n, seed=7)
-    a = np.exp(0.5 * rng.standard_normal((n, n, n))) if op == 1 else None
-    c = rng.random((n, n, n)) if op == 1 else None
-    Sf, _ = _pair(pm, n, rho, op=op, a=a, c=c, pst=pst)
-    Su, _ = _pair(pm, n, rho, op=op, a=a, c=c, pst=pst)
-    Su.set_add_fusion(False)
-    it = 12 if pst > 0 else 30
-    r0f, hf = Sf.solve_rel(1, 1e-10, it)
-    r0u, hu = Su.solve_rel(1, 1e-10, it)
-    assert r0f == r0u
-    assert np.array_equal(hf, hu)
-    L = Sf.nlev
-    assert np.array_equal(Sf.download(L, pm.PMG_F_U), Su.download(L, pm.PMG_F_U))
-    # a second solve on the same hierarchies starts from the e each one left
-    Sf.upload(L, pm.PMG_F_RHO, 0.5 * rho)
-    Su.upload(L, pm.PMG_F_RHO, 0.5 * rho)
-    _, hf2 = Sf.solve_rel(1, 1e-10, 4)
-    _, hu2 = Su.solve_rel(1, 1e-10, 4)
-    assert np.array_equal(hf2, hu2)
-    assert np.array_equal(Sf.download(L, pm.PMG_F_U), Su.download(L, pm.PMG_F_U))
-    Sf.close()
-    Su.close()
-
-
 def test_mixed_precision_reaches_1e10(pm):
     """FP32 inner V-cycles in FP64 defect correction reach ||r||/||r0|| <= 1e-10;
     the error against a 40-cycle FP64 solution is reported and bounded."""
