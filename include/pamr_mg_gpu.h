@@ -169,9 +169,10 @@ int pmg_set_graph(pmg_solver *S, int on);
  * on = 0 off, 1 the default, 2..17 the largest side fused (9: only the
  * levels of 9 points and below); results are identical either way */
 int pmg_set_bottom_fusion(pmg_solver *S, int on);
-/* compute the finest-level FP64 residual inside the full-weight injection of u
- * and f_res instead of storing it (default on; no excision, full weighting
- * for both); results are identical either way */
+/* compute the finest-level residual (of the FP64 cycle and of the FP32 inner
+ * cycle) inside the full-weight injection of u and f_res instead of storing
+ * it (default on; no excision, full weighting for both); results are
+ * identical either way */
 int pmg_set_restrict_fusion(pmg_solver *S, int on);
 /* run pairs of red-black sweeps on levels of at least 65 points per side as
  * one temporally blocked pass (default off: at 257^3 a pair takes 0.26 ms

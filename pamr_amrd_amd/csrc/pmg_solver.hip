@@ -647,7 +647,6 @@ int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
 // (FP64 only: in FP32 the separate residual and restriction are as fast)
 template <typename T>
 bool can_fuse_resid_restrict(const pmg_solver *S) {
-  if (!std::is_same<T, double>::value) return false;
   if (!S->use_fused_restrict || S->do_ex || S->rop_u != 3 || S->rop_res != 3 || S->nlev < 2) return false;
   const Level &F = S->lev[S->nlev - 1], &C = S->lev[S->nlev - 2];
   for (int d = 0; d < 3; d++)
