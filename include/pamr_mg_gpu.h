@@ -174,6 +174,10 @@ int pmg_set_bottom_fusion(pmg_solver *S, int on);
  * it (default on; no excision, full weighting for both); results are
  * identical either way */
 int pmg_set_restrict_fusion(pmg_solver *S, int on);
+/* coarse planes per workgroup of that fused kernel (prec 0 FP64 cycle, 1 FP32
+ * inner cycle; 1..64; default 5 / 12 when the coarse level has at least 129
+ * planes, else 8); results are identical for every value */
+int pmg_set_resid_restrict_zchunk(pmg_solver *S, int prec, int czchunk);
 /* run pairs of red-black sweeps on levels of at least 65 points per side as
  * one temporally blocked pass (default off: at 257^3 a pair takes 0.26 ms
  * against 2 x 0.11 ms for single sweeps; zchunk > 0 sets its z depth);

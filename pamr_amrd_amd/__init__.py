@@ -83,6 +83,7 @@ def _declare(L):
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_set_bottom_fusion.argtypes = [_vp, i]
     L.pmg_set_restrict_fusion.argtypes = [_vp, i]
+    L.pmg_set_resid_restrict_zchunk.argtypes = [_vp, i, i]
     L.pmg_set_sweep_blocking.argtypes = [_vp, i, i]
     L.pmg_stream.argtypes = [_vp]
     L.pmg_stream.restype = _vp
@@ -397,6 +398,10 @@ class Solver:
 
     def set_restrict_fusion(self, on):
         _check(self._L.pmg_set_restrict_fusion(self.h, 1 if on else 0), "pmg_set_restrict_fusion")
+
+    def set_resid_restrict_zchunk(self, prec, czchunk):
+        """coarse planes per workgroup of the fused finest residual + injection"""
+        _check(self._L.pmg_set_resid_restrict_zchunk(self.h, prec, czchunk), "pmg_set_resid_restrict_zchunk")
 
     def set_sweep_blocking(self, on, zchunk=0):
         _check(self._L.pmg_set_sweep_blocking(self.h, 1 if on else 0, zchunk), "pmg_set_sweep_blocking")

@@ -135,6 +135,7 @@ struct pmg_solver {
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   int bottom_max_n = 17;  // ... those with at most this many points per side
   int use_fused_restrict = 1;  // finest residual + injection in one pass
+  int rr_czchunk[2] = {8, 8};   // its coarse planes per block, [0] FP64, [1] FP32
   int use_sweep2 = 0;          // pairs of sweeps temporally blocked on large levels
                                // (measured slower than single sweeps; off by default)
   int sweep2_zchunk = 32;
@@ -678,7 +679,7 @@ int resid_restrict(pmg_solver *S, int slot) {
   P.cu = AC.u; P.cua = AC.ua; P.cres = AC.res;
   P.nx = F.n[0]; P.ny = F.n[1]; P.nz = F.n[2];
   P.cnx = C.n[0]; P.cny = C.n[1]; P.cnz = C.n[2];
-  P.czchunk = std::max(1, std::min(8, C.n[2]));
+  P.czchunk = std::max(1, std::min(S->rr_czchunk[std::is_same<T, float>::value ? 1 : 0], C.n[2]));
   P.sp = spacing<T>(F);
   P.ws = S->ws;
   P.slot = slot;
@@ -868,6 +869,13 @@ pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidt
       S->lv_zchunk[pi][L - 1] = n >= 257 ? 32 : (n >= 129 ? 16 : (n >= 65 ? 8 : 4));
     }
   }
+  // coarse planes per block of the fused finest residual + injection: on
+  // coarse levels of 129 planes and more, 5 (FP64) and 12 (FP32) -- fastest at
+  // 257^3 (12: one round of workgroups on the 256 CUs); smaller grids keep 8
+  if (nl >= 2 && S->lev[nl - 2].n[2] >= 129) {
+    S->rr_czchunk[0] = 5;
+    S->rr_czchunk[1] = 12;
+  }
   // norm workspace sized for the largest launch
   // partial-norm capacity: the largest launch over the finest level
   const Level &F = S->lev[nl - 1];
@@ -976,6 +984,13 @@ int pmg_num_sweep_variants(void) { return kNumSweepVariants; }
 int pmg_set_graph(pmg_solver *S, int on) {
   if (!S) return -1;
   S->use_graph = on ? 1 : 0;
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_set_resid_restrict_zchunk(pmg_solver *S, int prec, int czchunk) {
+  if (!S || (prec != 0 && prec != 1) || czchunk < 1 || czchunk > 64) return -1;
+  S->rr_czchunk[prec] = czchunk;
   invalidate_graphs(S);
   return 0;
 }

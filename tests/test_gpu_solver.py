@@ -182,6 +182,33 @@ def test_restrict_fusion_matches_unfused(pm, n):
     Su.close()
 
 
+@pytest.mark.parametrize("cz", [1, 3, 5, 16])
+def test_resid_restrict_zchunk_bit_exact(pm, cz):
+    """every z-chunk of the fused finest residual + injection gives the same
+    cycles (FP64) and the same mixed-precision history (FP32) as the default"""
+    print(f"case resid_restrict zchunk {cz}", flush=True)
+    n = 65
+    rho = -pm.gaussian_source(n, seed=11)
+    Sd, _ = _pair(pm, n, rho)
+    Sz, _ = _pair(pm, n, rho)
+    Sz.set_resid_restrict_zchunk(0, cz)
+    Sz.set_resid_restrict_zchunk(1, cz)
+    for _ in range(3):
+        rd, nd, cd = Sd.vcycle()
+        rz, nz_, cz_ = Sz.vcycle()
+        assert cd == cz_
+        assert abs(rd - rz) <= 1e-13 * rd
+    assert np.array_equal(Sd.download(Sd.nlev, pm.PMG_F_U), Sz.download(Sz.nlev, pm.PMG_F_U))
+    Sd.zero_solution()
+    Sz.zero_solution()
+    _, hd = Sd.solve_rel(1, 1e-10, 40)
+    _, hz = Sz.solve_rel(1, 1e-10, 40)
+    assert np.array_equal(hd, hz)
+    assert np.array_equal(Sd.download(Sd.nlev, pm.PMG_F_U), Sz.download(Sz.nlev, pm.PMG_F_U))
+    Sd.close()
+    Sz.close()
+
+
 @pytest.mark.parametrize("n", [65, 129])
 def test_bottom_fusion_matches_unfused(pm, n):
     """the single-workgroup bottom V (levels <= 17^3) gives the same u, the
