@@ -134,7 +134,7 @@ int pmg_residual(pmg_solver *S, double *norm_out);
 int pmg_solve(pmg_solver *S, int min_iter, int max_iter, double tol, double *norms, int *iters);
 /* relative solve ||r_k|| <= rtol ||r_0||; mode 0 FP64 V-cycles, 1 FP32
  * inner V-cycles in FP64 defect correction, 2 as 1 with the learned
- * correction (pmg_set_net); hist[k] = ||r_{k+1}||.  r0_out: on entry, a
+ * correction (pmg_set_net; -1 on a solver with excision); hist[k] = ||r_{k+1}||.  r0_out: on entry, a
  * value > 0 is taken as the reference norm instead of ||r_0|| of the
  * initial u (warm starts measure against the residual of u = 0); on exit
  * the reference norm used. */
@@ -171,12 +171,15 @@ int pmg_set_graph(pmg_solver *S, int on);
 int pmg_set_bottom_fusion(pmg_solver *S, int on);
 /* compute the finest-level residual (of the FP64 cycle and of the FP32 inner
  * cycle) inside the full-weight injection of u and f_res instead of storing
- * it (default on; no excision, full weighting for both); results are
- * identical either way */
+ * it (default on; no excision, full weighting for both); u, the norms and
+ * every coarse-level field are identical either way, but with the fusion on
+ * the finest level's f_res (PMG_F_RES) is not stored by the V-cycle */
 int pmg_set_restrict_fusion(pmg_solver *S, int on);
 /* coarse planes per workgroup of that fused kernel (prec 0 FP64 cycle, 1 FP32
  * inner cycle; 1..64; default 5 / 12 when the coarse level has at least 129
- * planes, else 8); results are identical for every value */
+ * planes, else 8); results are identical for every value.  -3 (value kept)
+ * when the launch would need more workgroups than the solver's partial-norm
+ * workspace holds (small values on large grids) */
 int pmg_set_resid_restrict_zchunk(pmg_solver *S, int prec, int czchunk);
 /* run pairs of red-black sweeps on levels of at least 65 points per side as
  * one temporally blocked pass (default off: at 257^3 a pair takes 0.26 ms

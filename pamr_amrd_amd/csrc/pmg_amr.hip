@@ -1571,9 +1571,9 @@ int pmg_amr_plan_stats(pmg_amr *A, int L, int *out) {
 
 int pmg_amr_time_sweep(pmg_amr *A, int L, int nrep, double *ms_avg) {
   if (!A || L < 2 || L > A->nlev || nrep < 1 || !ms_avg) return -1;
-  hipEvent_t e0, e1;
-  PMG_HOST_CHECK(hipEventCreate(&e0));
-  PMG_HOST_CHECK(hipEventCreate(&e1));
+  EventPair ev;
+  PMG_HOST_CHECK(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   int rc = sweeps(A, L, 2, false);  // warm up (even count: u returns to its buffer)
   if (rc) return rc;
   PMG_HOST_CHECK(hipEventRecord(e0, A->stream));
@@ -1584,8 +1584,6 @@ int pmg_amr_time_sweep(pmg_amr *A, int L, int nrep, double *ms_avg) {
   float ms = 0.f;
   PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));
   *ms_avg = ms / (2.0 * nrep);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return 0;
 }
 

@@ -25,6 +25,23 @@
 
 namespace pmg {
 
+// Two HIP events for host-side timing, destroyed on every exit path
+// (early returns of PMG_HOST_CHECK included).
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t create() {
+    hipError_t r = hipEventCreate(&e0);
+    return r != hipSuccess ? r : hipEventCreate(&e1);
+  }
+  EventPair() = default;
+  EventPair(const EventPair &) = delete;
+  EventPair &operator=(const EventPair &) = delete;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
 constexpr int kWave = 64;        // CDNA wavefront width
 constexpr int kXcd = 8;          // MI355X: 8 XCDs, each with its own L2
 

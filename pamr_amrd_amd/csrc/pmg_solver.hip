@@ -15,7 +15,7 @@
 #include <vector>
 
 #include "pmg_kernels.h"
-#include "pmg_solver.h"
+#include "pamr_mg_gpu.h"
 
 extern "C" int pmg_net_reserve(pmg_net *N, long long points);
 extern "C" int pmg_net_correct(pmg_net *N, const float *x, float scale, double *u, double coef, int nx,
@@ -644,8 +644,8 @@ int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
 }
 
 // finest-level residual + full-weight injection of u and f_res in one pass
-// (no excision, full weighting for both, fine = 2 coarse - 1)
-// (FP64 only: in FP32 the separate residual and restriction are as fast)
+// (no excision, full weighting for both, fine = 2 coarse - 1), in the FP64
+// cycle and in the FP32 inner cycle; the finest f_res is then never stored
 template <typename T>
 bool can_fuse_resid_restrict(const pmg_solver *S) {
   if (!S->use_fused_restrict || S->do_ex || S->rop_u != 3 || S->rop_res != 3 || S->nlev < 2) return false;
@@ -657,11 +657,19 @@ bool can_fuse_resid_restrict(const pmg_solver *S) {
          F.rbox.hi[1] == F.n[1] - 2 && F.rbox.hi[2] == F.n[2] - 2;
 }
 
+// workgroups of the fused kernel for a coarse level of cn points and a
+// given number of coarse planes per workgroup
+template <typename T>
+int resid_restrict_blocks(const int *cn, int czchunk) {
+  constexpr int CX = std::is_same<T, double>::value ? 16 : 32, CY = 8;
+  return ((cn[0] + CX - 1) / CX) * ((cn[1] + CY - 1) / CY) * ((cn[2] + czchunk - 1) / czchunk);
+}
+
 template <typename T, int OP, bool P2>
 int launch_resid_restrict_t(pmg_solver *S, ResRestrP<T> P) {
   constexpr int CX = std::is_same<T, double>::value ? 16 : 32, CY = 8, NT = 256;
-  const int ncz = (P.cnz + P.czchunk - 1) / P.czchunk;
-  const dim3 grid(((P.cnx + CX - 1) / CX) * ((P.cny + CY - 1) / CY) * ncz);
+  const int cn[3] = {P.cnx, P.cny, P.cnz};
+  const dim3 grid(resid_restrict_blocks<T>(cn, P.czchunk));
   P.nb = (int)grid.x;
   if (P.nb > S->ws.cap) return -3;
   hipLaunchKernelGGL((k_resid_restrict<T, OP, P2, CX, CY, NT>), grid, dim3(NT), 0, S->stream, P);
@@ -990,6 +998,17 @@ int pmg_set_graph(pmg_solver *S, int on) {
 
 int pmg_set_resid_restrict_zchunk(pmg_solver *S, int prec, int czchunk) {
   if (!S || (prec != 0 && prec != 1) || czchunk < 1 || czchunk > 64) return -1;
+  if (S->nlev >= 2) {
+    // the launch must fit the partial-norm workspace
+    const int *cn = S->lev[S->nlev - 2].n;
+    const int cz = std::min(czchunk, cn[2]);
+    const int nb = prec ? resid_restrict_blocks<float>(cn, cz) : resid_restrict_blocks<double>(cn, cz);
+    if (nb > S->ws.cap) {
+      std::fprintf(stderr, "pamr_mg: %d coarse planes per workgroup need %d workgroups, more than the %d the norm "
+                   "workspace holds\n", czchunk, nb, S->ws.cap);
+      return -3;
+    }
+  }
   S->rr_czchunk[prec] = czchunk;
   invalidate_graphs(S);
   return 0;
@@ -1220,11 +1239,16 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
     // V-cycle, kept only when its contraction per unit time beats the
     // plain cycle's (measured); otherwise undone and retried later
     if (!S->net) return -1;
+    if (S->do_ex) {
+      // the correction kernel writes every interior point, excised ones too
+      std::fprintf(stderr, "pamr_mg: pmg_solve_rel mode 2 does not support excision\n");
+      return -1;
+    }
     Level &F = S->lev[S->nlev - 1];
     const double h2 = F.hdx[0] * F.hdx[0];
-    hipEvent_t e0, e1;
-    PMG_HOST_CHECK(hipEventCreate(&e0));
-    PMG_HOST_CHECK(hipEventCreate(&e1));
+    EventPair ev;
+    PMG_HOST_CHECK(ev.create());
+    hipEvent_t e0 = ev.e0, e1 = ev.e1;
     auto read_outer = [&](double *out) -> int {
       PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost, S->stream));
       PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
@@ -1281,8 +1305,6 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
       if (hist) hist[it] = r;
       it++;
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
   } else {
     return -1;
   }
@@ -1940,9 +1962,9 @@ int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double 
   // to its buffer), 1 = residual.  prec: 0 FP64, 1 FP32.
   if (!S || L < 1 || L > S->nlev || nrep < 2 || !ms_avg) return -1;
   if (prec == 1 && ensure_fp32(S)) return -1;
-  hipEvent_t e0, e1;
-  PMG_HOST_CHECK(hipEventCreate(&e0));
-  PMG_HOST_CHECK(hipEventCreate(&e1));
+  EventPair ev;
+  PMG_HOST_CHECK(ev.create());
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   nrep -= nrep % 2;
   const bool fine = (L == S->nlev);
   int rc = 0;
@@ -1960,8 +1982,6 @@ int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double 
   PMG_HOST_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
   PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   *ms_avg = (double)ms / nrep;
   return rc;
 }

@@ -5,7 +5,9 @@
 // by value, with u1/u2/chr1 being device pointers, plus a HIP stream.  The
 // silent early returns of the Fortran bounds checks are kept (return 0,
 // nothing written); launch errors return -1.
-#include "pmg_transfer.h"
+#include <hip/hip_runtime.h>
+
+#include "pamr_mg_gpu.h"
 
 #include "pmg_ops.h"
 

@@ -2,15 +2,35 @@
 oracle/_ref (present in the build container only), on more sizes and
 random masks than the committed golden vectors."""
 import ctypes as C
+import os
 
 import numpy as np
 import pytest
 
 import oracle as O
 
-R = O.ref_lib("mutil")
-NPS = O.ref_lib("nps")
-pytestmark = pytest.mark.skipif(R is None or NPS is None, reason="oracle/_ref not built")
+REF_TREE = "/root/reference"
+
+
+def _ref(which):
+    # loaded only where the reference source tree is (the build container):
+    # the reference, compiled or not, is never loaded anywhere else
+    if not os.path.isdir(REF_TREE):
+        pytest.skip("reference source tree absent")
+    lib = O.ref_lib(which)
+    if lib is None:
+        pytest.skip("oracle/_ref not built")
+    return lib
+
+
+@pytest.fixture(scope="module")
+def R():
+    return _ref("mutil")
+
+
+@pytest.fixture(scope="module")
+def NPS():
+    return _ref("nps")
 
 
 def ci(v):
@@ -24,7 +44,7 @@ def cd(v):
 @pytest.mark.parametrize("shape", [(5, 5, 5), (9, 9, 9), (17, 9, 5), (17, 13, 11), (33, 17, 9)])
 @pytest.mark.parametrize("op", [2, 3])
 @pytest.mark.parametrize("ex", [0, 1])
-def test_restrict_live(shape, op, ex):
+def test_restrict_live(R, shape, op, ex):
     nx, ny, nz = shape
     rng = np.random.default_rng(nx * 100 + ny + op + 7 * ex)
     u = rng.standard_normal((nz, ny, nx))
@@ -44,7 +64,7 @@ def test_restrict_live(shape, op, ex):
 
 @pytest.mark.parametrize("cshape", [(3, 3, 3), (5, 5, 5), (9, 5, 3), (17, 9, 9)])
 @pytest.mark.parametrize("ex", [0, 1])
-def test_interp_live(cshape, ex):
+def test_interp_live(R, cshape, ex):
     cx, cy, cz = cshape
     rng = np.random.default_rng(cx * 31 + cy + ex)
     u = rng.standard_normal((cz, cy, cx))
@@ -67,7 +87,7 @@ def test_interp_live(cshape, ex):
     ((4, 7, 3), (2, 1, 3), 2, 1),       # nk = 3 < 4: that axis falls back to dvi2q1d
     ((5, 4, 4), (5, 9, 1), 4, 8),       # rho 4
     ((9, 5, 6), (3, 1, 3), 2, 2)])
-def test_interp4_live(seg):
+def test_interp4_live(R, seg):
     """ord = 4 (dvi4q1d), no excision, with u2 pre-filled: the interior
     cubic stencils read u2's anchors outside the interpolated segment"""
     (ni, nj, nk), (i2, j2, k2), rho, extra = seg
@@ -93,7 +113,7 @@ def test_interp4_live(seg):
     ((9, 9, 9), (1, 1, 1), (9, 9, 9), (1, 1, 1), 2, 0.05),
     ((14, 13, 15), (5, 5, 5), (6, 5, 7), (9, 9, 9), 2, 0.2),
     ((13, 13, 13), (5, 5, 5), (5, 5, 5), (17, 17, 17), 4, 0.25)])
-def test_interp4_ex_live(case):
+def test_interp4_ex_live(R, case):
     """ord = 4 with excision (dvi4q1d_ex): extrapolation into excised
     anchors, run starts / ends, the dvi2q1d_ex fallback for short runs"""
     (cx, cy, cz), (i1, j1, k1), (ni, nj, nk), (i2, j2, k2), rho, dens = case
@@ -115,7 +135,7 @@ def test_interp4_ex_live(case):
 
 
 @pytest.mark.parametrize("shape", [(5, 5, 5), (12, 9, 7), (17, 17, 17)])
-def test_hooks_live(shape):
+def test_hooks_live(NPS, shape):
     nx, ny, nz = shape
     rng = np.random.default_rng(nx + 1000 * nz)
     V = rng.standard_normal((nz, ny, nx))
