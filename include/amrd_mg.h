@@ -148,6 +148,18 @@ int amrd_solve_elliptics(int L, int tl, double *res_out);
  * V-cycle run as amrd_vcycle_tre(iter, save_tre) */
 int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out);
 
+/* The reference's MG driver under its own names and signatures
+ * (amrd/mg.h:32,37), for AMRD code that calls them unchanged
+ * (amrd/evolve.c:285,483,980,1156,1192,1257,1294).
+ * solve_elliptics(L, tl, save_tre) replaces amrd/mg.c:20-79: it is
+ * amrd_solve_elliptics_tre, and like the reference (AMRD_stop,
+ * amrd/util.c:27-34) it prints a message, frees the context and exits the
+ * program with status 1 when the MG hierarchy cannot be built or a device
+ * step fails.  vcycle(giter, save_tre) replaces amrd/mg.c:84-201: it is
+ * amrd_vcycle_tre (the residual of the finest level; negative on error). */
+void solve_elliptics(int L, int tl, int save_tre);
+double vcycle(int giter, int save_tre);
+
 /* last norms of amrd_vcycle: lev_norms[L-1] for MG levels 1..Lf (the
  * reference's res[] array) and the coarsest-level sweep count */
 int amrd_mg_last_norms(double *lev_norms, int *citer);

@@ -217,6 +217,10 @@ def _declare_pamr(L):
     L.amrd_vcycle.argtypes = [i]
     L.amrd_vcycle.restype = d
     L.amrd_solve_elliptics.argtypes = [i, i, _dp]
+    L.solve_elliptics.argtypes = [i, i, i]
+    L.solve_elliptics.restype = None
+    L.vcycle.argtypes = [i, i]
+    L.vcycle.restype = d
     L.amrd_vcycle_tre.argtypes = [i, i]
     L.amrd_vcycle_tre.restype = d
     L.amrd_solve_elliptics_tre.argtypes = [i, i, i, _dp]

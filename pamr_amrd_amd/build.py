@@ -63,10 +63,27 @@ def build_apps() -> list:
     return out
 
 
-def build(jobs: int = 8) -> str:
+def _stale(src) -> bool:
+    """True when src's object is missing or older than src or any header."""
+    out = os.path.join(OBJ, src + ".o")
+    if not os.path.exists(out):
+        return True
+    t = os.path.getmtime(out)
+    deps = [os.path.join(CSRC, src)]
+    deps += [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    deps += [os.path.join(INC, f) for f in os.listdir(INC) if f.endswith(".h")]
+    return any(os.path.getmtime(d) > t for d in deps)
+
+
+def build(jobs: int = 8, incremental: bool = False) -> str:
+    """Compile every source (incremental: only those whose object is stale)
+    and link libpamr_mg.so and the example applications."""
     srcs = sources()
-    with ThreadPoolExecutor(max_workers=min(jobs, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
+    todo = [s for s in srcs if _stale(s)] if incremental else srcs
+    if todo:
+        with ThreadPoolExecutor(max_workers=min(jobs, len(todo))) as ex:
+            list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJ, s + ".o") for s in srcs]
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp", *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -77,4 +94,4 @@ def build(jobs: int = 8) -> str:
 
 
 if __name__ == "__main__":
-    print(build(int(sys.argv[1]) if len(sys.argv) > 1 else 8))
+    print(build(8, incremental="-i" in sys.argv[1:]))

@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1607,6 +1608,20 @@ int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out) {
   if (res_out) *res_out = res;
   return it;
 }
+
+// The reference's own names and signatures (amrd/mg.h:32,37), so that AMRD
+// code calling them (amrd/evolve.c:285,483,980,1156,1192,1257,1294) links
+// unchanged.  A failed solve stops the program as AMRD_stop does
+// (amrd/util.c:27-34, amrd/mg.c:31): message, context freed, exit(1).
+void solve_elliptics(int L, int tl, int save_tre) {
+  if (amrd_solve_elliptics_tre(L, tl, save_tre, nullptr) >= 0) return;
+  std::printf("\nsolve_elliptics: PAMR_build_mgh() failed\n\nStopping.\n");
+  std::fflush(stdout);
+  if (C) PAMR_free_context(1);
+  std::exit(1);
+}
+
+double vcycle(int giter, int save_tre) { return amrd_vcycle_tre(giter, save_tre); }
 
 int amrd_mg_last_norms(double *lev_norms, int *citer) {
   if (!D.last_nlev) return 0;

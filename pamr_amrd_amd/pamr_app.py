@@ -144,8 +144,11 @@ class NpsContext:
             raise RuntimeError("PAMR_build_mgh failed")
         return nl
 
-    def vcycle(self, giter=0, save_tre=False):
-        r = self.L.amrd_vcycle_tre(giter, 1 if save_tre else 0)
+    def vcycle(self, giter=0, save_tre=False, reference_name=False):
+        """One V-cycle through amrd_vcycle_tre, or through the reference's
+        own entry point vcycle(giter, save_tre) when reference_name."""
+        fn = self.L.vcycle if reference_name else self.L.amrd_vcycle_tre
+        r = fn(giter, 1 if save_tre else 0)
         if r < 0:
             raise RuntimeError("amrd_vcycle failed")
         nl = self.L.PAMR_get_max_lev(PAMR_MGH)

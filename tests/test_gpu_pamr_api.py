@@ -159,6 +159,8 @@ def test_transfer_entry_points(torch_cuda, seed):
 
 @pytest.mark.parametrize("seed,op,level_hooks", [(1, 0, False), (2, 0, True), (4, 1, False), (5, 1, True)])
 def test_vcycles_match_oracle(torch_cuda, seed, op, level_hooks):
+    """V-cycles of the drop-in driver with the application's hooks; odd
+    seeds call the reference-named entry point vcycle(giter, 0)."""
     m, ctx = pair(seed, op=op)
     load_sources(m, ctx, op, 1234 + seed)
     m.set_params()
@@ -173,7 +175,7 @@ def test_vcycles_match_oracle(torch_cuda, seed, op, level_hooks):
                 assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, NAME[w]), m.field(Lm, g, w)), ("init", Lm, g, w)
     for c in range(4):
         rm, nm, cm = m.vcycle()
-        ra, na, ca = ctx.vcycle(c)
+        ra, na, ca = ctx.vcycle(c, reference_name=seed % 2 == 1)
         print(f"cycle {c}: oracle {rm:.15e} hooks {ra:.15e} citer {cm} {ca}", flush=True)
         assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
         assert ca == cm
@@ -184,15 +186,24 @@ def test_vcycles_match_oracle(torch_cuda, seed, op, level_hooks):
     ctx.close()
 
 
-def test_solve_elliptics_leaves_solution_in_amr_levels(torch_cuda):
+@pytest.mark.parametrize("reference_name", [False, True])
+def test_solve_elliptics_leaves_solution_in_amr_levels(torch_cuda, reference_name):
+    """amrd_solve_elliptics, or the reference-named solve_elliptics(1, 1, 0)
+    (amrd/mg.h:32), leaves the oracle's solution in the AMR levels."""
     m, ctx = pair(3)
     load_sources(m, ctx, 0, 77)
     m.set_params()
     m.init()
     ctx.set_params(pre=2, pst=2, min_iter=1, max_iter=6, tol=0.0)
-    res = C.c_double()
-    it = ctx.L.amrd_solve_elliptics(1, 1, C.byref(res))
-    assert it == 6
+    if reference_name:
+        ctx.L.solve_elliptics(1, 1, 0)
+        norms = np.zeros(16)
+        assert ctx.L.amrd_mg_last_norms(pm.P(norms), None) == 1
+        res = C.c_double(norms[m.nlev - 1])
+    else:
+        res = C.c_double()
+        it = ctx.L.amrd_solve_elliptics(1, 1, C.byref(res))
+        assert it == 6
     for _ in range(6):
         rm, _, _ = m.vcycle()
     assert abs(res.value - rm) <= 1e-12 * rm
