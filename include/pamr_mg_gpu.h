@@ -68,6 +68,21 @@ int pmg_hook_relax(int op, double *V, const double *rhs, const double *rho, cons
                    int ny, int nz, double dx, double dy, double dz, double *norm_dev,
                    hipStream_t stream);
 
+/* No counterpart in the reference: pmg_hook_relax for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, same results bit for
+ * bit; it calls no allocating routine.  Its partial-norm scratch is a fixed
+ * 16 MiB device array of the library (one per GPU, held until the process
+ * ends, located on the first call on each device: call it with the stream's
+ * device current), which limits the grid to (nx-2)(ny-2)(nz-2) <= 2^27
+ * interior points (-2 beyond, nothing written).  Every call on a GPU uses
+ * that GPU's one array: calls that overlap on one GPU (other streams or
+ * threads) race. */
+int pmg_hook_relax_noalloc(int op, double *V, const double *rhs, const double *rho, const double *a,
+                           const double *c, const double *cmask, const double *chr, double ex, int nx,
+                           int ny, int nz, double dx, double dy, double dz, double *norm_dev,
+                           hipStream_t stream);
+
 /* res = L[V] - rhs where cmask ON; replaces app_MG_residual
  * (include/amrd.h:137-141; w/examples/nps/num_w.f:59-90) */
 int pmg_hook_residual(int op, double *res, const double *rhs, const double *V, const double *rho,

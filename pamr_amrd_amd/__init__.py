@@ -51,6 +51,7 @@ def _declare(L):
     L.pmg_dmhwr3d.argtypes = m_args + [_vp, d, i, _vp]
     L.pmg_dminterp3d.argtypes = [_vp, _vp] + [i] * 21 + [i, _vp, d, i, _vp]
     L.pmg_hook_relax.argtypes = [i] + [_vp] * 7 + [d, i, i, i, d, d, d, _vp, _vp]
+    L.pmg_hook_relax_noalloc.argtypes = [i] + [_vp] * 7 + [d, i, i, i, d, d, d, _vp, _vp]
     L.pmg_hook_residual.argtypes = [i] + [_vp] * 8 + [d, i, i, i, d, d, d, _vp, _vp]
     L.pmg_hook_lop.argtypes = [i] + [_vp] * 7 + [d, i, i, i, d, d, d, _vp]
     L.pmg_create.argtypes = [i, i, i, _dp, i, i, i, d]

@@ -9,6 +9,8 @@
 // Arrays are device pointers; cmask and chr are double grid functions as in
 // the reference.  The norm (RMS of the residuals the hook evaluated) is
 // written to a device double, so the caller decides when to synchronise.
+#include <mutex>
+
 #include "pmg_ops.h"
 #include "../../include/pamr_mg_gpu.h"
 
@@ -17,6 +19,38 @@ using namespace pmg;
 namespace {
 
 constexpr int NT = 256;
+
+// Partial-norm scratch of pmg_hook_relax_noalloc: a fixed device array of
+// the library (one copy per GPU, held until the process ends) holding the
+// two colour slots of (sum, count) partials, kScratchBlocks blocks each,
+// then the two per-slot block counts.  Nothing in it needs clearing: every
+// launch writes all partials it folds and block 0 writes its slot's count.
+constexpr int kScratchBlocks = 1 << 19;
+constexpr size_t kScratchPartials = 2 * 2 * (size_t)kScratchBlocks;
+static __device__ __align__(256) double scratch[kScratchPartials + 2];
+
+constexpr int kMaxDevices = 64;
+double *g_scratch_dev[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// the device address of this GPU's scratch (looked up once per device, with
+// that device current)
+int scratch_address(double **out) {
+  int dev = -1;
+  PMG_HOST_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) {
+    std::fprintf(stderr, "pamr_mg: device %d beyond the %d the no-alloc hook supports\n", dev, kMaxDevices);
+    return -1;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void *p = nullptr;
+    PMG_HOST_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return 0;
+}
 
 struct HookGeom {
   int nx, ny, nz;
@@ -135,6 +169,27 @@ int ensure_ws(int nb) {
   return 0;
 }
 
+// the two colour passes and the norm fold of a relax sweep over nb blocks,
+// with the partial norms in ws
+int relax_launch(int op, double *V, const double *rhs, const double *rho, const double *a, const double *c,
+                 const double *cmask, const double *chr, double ex, const HookGeom &g, const Spacing<double> &sp,
+                 int nb, NormWs ws, double *norm_dev, hipStream_t stream) {
+  for (int pass = 0; pass <= 1; pass++) {
+#define PMG_HR(OPV, P2V)                                                                         \
+  hipLaunchKernelGGL((k_hook_relax_pass<OPV, P2V>), dim3(nb), dim3(NT), 0, stream, V, rhs, rho, \
+                     a, c, cmask, chr, ex, g, sp, pass, ws, nb)
+    if (op == 0) { if (sp.pow2) PMG_HR(0, true); else PMG_HR(0, false); }
+    else { if (sp.pow2) PMG_HR(1, true); else PMG_HR(1, false); }
+#undef PMG_HR
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  if (norm_dev) {
+    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(kFoldThreads), 0, stream, ws, 0, 2, norm_dev);
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -150,20 +205,28 @@ int pmg_hook_relax(int op, double *V, const double *rhs, const double *rho, cons
   size_t n = (size_t)(nx - 2) * (ny - 2) * (nz - 2);
   int nb = (int)((n + NT - 1) / NT);
   if (ensure_ws(nb)) return -1;
-  for (int pass = 0; pass <= 1; pass++) {
-#define PMG_HR(OPV, P2V)                                                                         \
-  hipLaunchKernelGGL((k_hook_relax_pass<OPV, P2V>), dim3(nb), dim3(NT), 0, stream, V, rhs, rho, \
-                     a, c, cmask, chr, ex, g, sp, pass, g_ws, nb)
-    if (op == 0) { if (sp.pow2) PMG_HR(0, true); else PMG_HR(0, false); }
-    else { if (sp.pow2) PMG_HR(1, true); else PMG_HR(1, false); }
-#undef PMG_HR
-    PMG_HOST_CHECK(hipGetLastError());
+  return relax_launch(op, V, rhs, rho, a, c, cmask, chr, ex, g, sp, nb, g_ws, norm_dev, stream);
+}
+
+int pmg_hook_relax_noalloc(int op, double *V, const double *rhs, const double *rho, const double *a,
+                           const double *c, const double *cmask, const double *chr, double ex, int nx,
+                           int ny, int nz, double dx, double dy, double dz, double *norm_dev,
+                           hipStream_t stream) {
+  if (nx < 3 || ny < 3 || nz < 3 || !V || !rhs || !rho || !cmask) return -1;
+  if (op == 1 && (!a || !c)) return -1;
+  const size_t n = (size_t)(nx - 2) * (ny - 2) * (nz - 2);
+  if ((n + NT - 1) / NT > (size_t)kScratchBlocks) {
+    std::fprintf(stderr, "pmg_hook_relax_noalloc: %zu interior points exceed the %zu its scratch holds\n", n,
+                 (size_t)kScratchBlocks * NT);
+    return -2;
   }
-  if (norm_dev) {
-    hipLaunchKernelGGL(k_norm_fold, dim3(1), dim3(kFoldThreads), 0, stream, g_ws, 0, 2, norm_dev);
-    PMG_HOST_CHECK(hipGetLastError());
-  }
-  return 0;
+  double *base = nullptr;
+  if (scratch_address(&base)) return -1;
+  NormWs ws{base, reinterpret_cast<int *>(base + kScratchPartials), kScratchBlocks, 2};
+  HookGeom g{nx, ny, nz, (size_t)nx, (size_t)nx * ny};
+  Spacing<double> sp = make_spacing(dx, dy, dz);
+  const int nb = (int)((n + NT - 1) / NT);
+  return relax_launch(op, V, rhs, rho, a, c, cmask, chr, ex, g, sp, nb, ws, norm_dev, stream);
 }
 
 int pmg_hook_residual(int op, double *res, const double *rhs, const double *V, const double *rho,
