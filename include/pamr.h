@@ -16,7 +16,8 @@
  * Supported scope: one context, dim 3, one rank (ghost width 0), refinement
  * ratio 2 on every level, AMR level 1 a single grid over the whole domain,
  * MG hierarchies built from AMR level 1 upwards (min_lev 1), restriction
- * ops 1-3, second-order interpolation, AMR_bdy_width 0 or 1 in PAMR_sync.
+ * ops 1-3, second- and fourth-order interpolation, AMR_bdy_width 0 or 1 in
+ * PAMR_sync, excision (PAMR_excision_on).
  * Anything else fails with a message (return 0).  Not reentrant, as the
  * reference (one current context, src/gh.c:23-24).
  */
@@ -111,6 +112,19 @@ void PAMR_set_tf_bits(int tl, int hierarchy, int stage);
 int PAMR_sync(int l, int tl, int hierarchy, int AMR_bdy_width);
 int PAMR_inject(int lf, int tl, int hierarchy);
 int PAMR_interp(int lc, int tl, int hierarchy);
+
+/* ---- excision (include/pamr.h:160-163; src/pamr.c:1983-2037) ------------
+ * ex_mask_var: a variable in both hierarchies with one time level; the
+ * application's fill function writes `excised` into a HOST array of the
+ * grid's shape at excised points (it is called per grid, by
+ * PAMR_compose_hierarchy, by PAMR_build_mgh for the new MG levels, and now
+ * when initialize_now; the library uploads the result).  While excision is
+ * on, PAMR_inject and PAMR_interp transfer as local_transfer does with
+ * do_ex (src/transfer.c:946-1163). */
+int PAMR_excision_on(char *ex_mask_var,
+                     void (*app_fill_ex_mask_fnc)(double *mask, int dim, int *shape, double *bbox, double excised),
+                     double excised, int initialize_now);
+void PAMR_excision_off(void);
 
 /* ---- time (include/pamr.h:153-154) -------------------------------------- */
 void PAMR_set_time(int lev, double t);

@@ -18,7 +18,7 @@
  * launch.  A whole V-cycle is captured in one hipGraph.
  *
  * Scope: single MPI rank (ghost_width 0), refinement ratio 2, one base grid,
- * no excision, FP64.  Return codes: 0 ok, < 0 error (as pamr_mg_gpu.h).
+ * FP64; excision in the transfers of pmg_amr_*_fields_ex only.  Return codes: 0 ok, < 0 error (as pamr_mg_gpu.h).
  */
 #ifndef PAMR_AMR_GPU_H
 #define PAMR_AMR_GPU_H
@@ -38,6 +38,21 @@ typedef struct pmg_amr pmg_amr;
 pmg_amr *pmg_amr_create(const int *base_shape, const double *bbox, int namr, const int *nbox,
                         const double *boxes, int min_mg_cwidth, int op);
 void pmg_amr_destroy(pmg_amr *A);
+/* PAMR_build_mgh(amr_lev, finest, tl) for amr_lev >= 1 (src/gh.c:1222-1365;
+ * AMRD's tstep builds PAMR_build_mgh(Lf, Lf, 1), amrd/evolve.c:454): an MG
+ * hierarchy whose finest levels are A's AMR levels amr_lev .. finest -- the
+ * same grids, sharing A's grid functions and stream -- and whose coarser
+ * levels are coarsened from the boxes given for AMR level amr_lev (in the
+ * order given to pmg_amr_create): for cf = 2, 4, ... a level of the boxes
+ * whose calc_island_no (src/gh.c:1427-1467) is nonzero, grids flagged
+ * coarsest where it is negative, until no box survives; the grids of level
+ * 1 are all coarsest.  New levels have every field A has (zero).  The
+ * transfers and pmg_amr_field work on it; the built-in cycle
+ * (pmg_amr_vcycle / _solve / _residual) does not (-1).  Destroy it before A;
+ * NULL on failure. */
+pmg_amr *pmg_amr_create_mgh(pmg_amr *A, int amr_lev);
+/* 1 when grid g of level L is flagged coarsest, 0 when not, -1 on error */
+int pmg_amr_grid_coarsest(pmg_amr *A, int L, int g);
 /* PAMR_get_max_lev(PAMR_MGH) */
 int pmg_amr_nlev(pmg_amr *A);
 /* MG level that holds AMR level 1 */
@@ -102,10 +117,23 @@ long long pmg_amr_pool_points(pmg_amr *A, int L);
  * Enqueued on pmg_amr_stream without a host synchronisation.  sync: bdy
  * (AMR_bdy_width) 0 or 1, otherwise -2. */
 int pmg_amr_sync_fields(pmg_amr *A, int L, int bdy, int nf, const int *which);
+/* PAMR_sync(L, AMR_bdy_width 1) while only the coarsest grids communicate
+ * (set_comm_coarsest, amrd/mg.c:571-586; the transfer lists then skip grids
+ * with comm = 0, src/transfer.c:438,482) */
+int pmg_amr_sync_fields_coarsest(pmg_amr *A, int L, int nf, const int *which);
 int pmg_amr_inject_fields(pmg_amr *A, int Lf, int nf, const int *which, const int *ops);
 int pmg_amr_interp_fields(pmg_amr *A, int Lc, int nf, const int *which);
 /* the same at order ord (2 or 4, otherwise -2) */
 int pmg_amr_interp_fields_ord(pmg_amr *A, int Lc, int ord, int nf, const int *which);
+/* The same transfers with excision, as local_transfer does them when
+ * PAMR's excision is on (src/transfer.c:946-1163): one transfer of the
+ * level's list at a time, in the reference's order, with the device
+ * m_util_r8 routines (restriction renormalised over the fine grid's
+ * non-excised points; interpolation dminterp3d with do_ex on the buffered
+ * temporary, then clear_ex of the whole source and destination grids).
+ * mask: the field holding the excision mask (value ex = excised). */
+int pmg_amr_inject_fields_ex(pmg_amr *A, int Lf, int nf, const int *which, const int *ops, int mask, double ex);
+int pmg_amr_interp_fields_ex(pmg_amr *A, int Lc, int ord, int nf, const int *which, int mask, double ex);
 /* set_cmask_bdy (which 0) / set_cmask_child (which 1) of every grid of L
  * (amrd/mg.c:362-488) written into a field as 1.0 / 0.0; enqueued on
  * pmg_amr_stream */

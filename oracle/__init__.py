@@ -105,6 +105,9 @@ def _declare(L):
     L.or_amr_create.argtypes = [_ip, _dp, i, _ip, _dp, i, i]
     L.or_amr_create.restype = vp
     L.or_amr_destroy.argtypes = [vp]
+    L.or_amr_create_mgh.argtypes = [vp, i]
+    L.or_amr_create_mgh.restype = vp
+    L.or_amr_grid_coarsest.argtypes = [vp, i, i]
     for f in ("or_amr_nlev", "or_amr_base_level"):
         getattr(L, f).argtypes = [vp]
     L.or_amr_ngrids.argtypes = [vp, i]
@@ -114,6 +117,7 @@ def _declare(L):
     L.or_amr_set_params.argtypes = [vp, i, i, i, d, d, d, i, i, i, i]
     L.or_amr_init.argtypes = [vp]
     L.or_amr_set_tre.argtypes = [vp, i, i]
+    L.or_amr_set_excision.argtypes = [vp, i, d]
     L.or_amr_interp_field_ord.argtypes = [vp, i, i, i]
     L.or_amr_vcycle.argtypes = [vp, _dp, _ip]
     L.or_amr_vcycle.restype = d
@@ -231,7 +235,7 @@ class UniformMG:
         return self.L.or_mg_residual_fine(self.h)
 
 
-OA_U, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_TRE = range(9)
+OA_U, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_TRE, OA_CHR = range(10)
 
 
 class AmrMG:
@@ -258,6 +262,23 @@ class AmrMG:
         except Exception:
             pass
 
+    def sub_mgh(self, amr_lev):
+        """the MG hierarchy PAMR_build_mgh(amr_lev, finest, tl) builds: island
+        levels below AMR level amr_lev, which and the finer AMR levels share
+        this hierarchy's grid functions (keep this object alive meanwhile)"""
+        sub = object.__new__(AmrMG)
+        sub.L = self.L
+        sub.h = self.L.or_amr_create_mgh(self.h, amr_lev)
+        if not sub.h:
+            raise ValueError(f"no MG hierarchy from AMR level {amr_lev}")
+        sub.nlev = self.L.or_amr_nlev(sub.h)
+        sub.base = self.L.or_amr_base_level(sub.h)
+        sub.parent = self
+        return sub
+
+    def coarsest(self, lev, g):
+        return bool(self.L.or_amr_grid_coarsest(self.h, lev, g))
+
     def ngrids(self, lev):
         return self.L.or_amr_ngrids(self.h, lev)
 
@@ -282,6 +303,11 @@ class AmrMG:
 
     def init(self):
         self.L.or_amr_init(self.h)
+
+    def set_excision(self, ex, do_ex=True):
+        """PAMR's excision: OA_CHR of every grid is the mask (value ex =
+        excised); fill it on every level before init()"""
+        self.L.or_amr_set_excision(self.h, 1 if do_ex else 0, ex)
 
     def set_tre(self, save_tre, ntre=1):
         self.L.or_amr_set_tre(self.h, 1 if save_tre else 0, ntre)

@@ -37,6 +37,7 @@ typedef struct {
   int ng;
   ag_grid *g;
   int in_amrh;
+  int shared; /* grid functions belong to another hierarchy (MG over AMR levels) */
 } ag_level;
 
 struct or_amr {
@@ -47,7 +48,16 @@ struct or_amr {
   int pre, pst, max_citer, rop_u, rop_res, rop_src, iord;
   double crtol, w0_r, w0_i;
   int save_tre, ntre;
+  int do_ex;
+  double ex;
   int tf[OA_NUM]; /* transfer bits per field for the current communicate() */
+  /* the boxes given for each AMR level l (index l-1) in the caller's order
+   * (save_sgh_bboxes, src/gh.c:273-298); level 1 holds the base box */
+  int nsgh[OR_MAXLEV];
+  double *sgh[OR_MAXLEV];
+  int namr_lev; /* AMR levels (base included) */
+  double dx0[3];
+  int mcw;
 };
 
 /* ------------------------------------------------------------------------ */
@@ -309,6 +319,8 @@ static void local_transfer(or_amr *m, int Ls, int Ld, tpair *pr, int np)
     double dx[3], dxd[3];
     int nx1 = g1->shape[0], ny1 = g1->shape[1], nz1 = g1->shape[2];
     int nx2 = g2->shape[0], ny2 = g2->shape[1], nz2 = g2->shape[2];
+    /* excision masks of both grids (src/transfer.c:981-991) */
+    const double *chr1 = m->do_ex ? g1->f[OA_CHR] : 0, *chr2 = m->do_ex ? g2->f[OA_CHR] : 0;
     if (delta == 1) align_src(g1, g2, &pr[q].src, &pr[q].dst);
     or_calc_gs_ibbox(pr[q].src.bb, g1->bbox, g1->shape, 3, ib, dx);
     or_calc_gs_ibbox(pr[q].dst.bb, g2->bbox, g2->shape, 3, ibd, dxd);
@@ -335,20 +347,27 @@ static void local_transfer(or_amr *m, int Ls, int Ld, tpair *pr, int np)
         }
         data = (double *)malloc(sizeof(double) * (size_t)nb1b[0] * nb1b[1] * nb1b[2]);
         or_dminterp3d(g1->f[w], data, nx1, ny1, nz1, nb1b[0], nb1b[1], nb1b[2], b1i[0], b1i[1], b1i[2], 1, 1, 1,
-                      nbi[0], nbi[1], nbi[2], 1, 1, 1, irho, irho, irho, tb, g1->f[w], 1.0, 0);
+                      nbi[0], nbi[1], nbi[2], 1, 1, 1, irho, irho, irho, tb, m->do_ex ? chr1 : g1->f[w], m->ex,
+                      m->do_ex);
         or_dmcopy3d(data, g2->f[w], nb1b[0], nb1b[1], nb1b[2], nx2, ny2, nz2, b2[0], b2[1], b2[2], ibd[0], ibd[2],
                     ibd[4], nib, njb, nkb, 1, 1, 1, 1, 1, 1);
         free(data);
+        if (m->do_ex) {
+          /* clear_ex of both whole grids (src/transfer.c:930-935,1110-1114) */
+          size_t j, n1 = (size_t)nx1 * ny1 * nz1, n2 = (size_t)nx2 * ny2 * nz2;
+          for (j = 0; j < n1; j++) if (chr1[j] == m->ex) g1->f[w][j] = 0;
+          for (j = 0; j < n2; j++) if (chr2[j] == m->ex) g2->f[w][j] = 0;
+        }
       } else {
         if (tb == 1)
           or_dmcopy3d(g1->f[w], g2->f[w], nx1, ny1, nz1, nx2, ny2, nz2, ib[0], ib[2], ib[4], ibd[0], ibd[2], ibd[4],
                       nib, njb, nkb, irho, irho, irho, 1, 1, 1);
         else if (tb == 2)
           or_dmhwr3d(g1->f[w], g2->f[w], nx1, ny1, nz1, nx2, ny2, nz2, ib[0], ib[2], ib[4], ibd[0], ibd[2], ibd[4],
-                     nib, njb, nkb, irho, irho, irho, 1, 1, 1, g1->f[w], 1.0, 0);
+                     nib, njb, nkb, irho, irho, irho, 1, 1, 1, m->do_ex ? chr1 : g1->f[w], m->ex, m->do_ex);
         else
           or_dmfwr3d(g1->f[w], g2->f[w], nx1, ny1, nz1, nx2, ny2, nz2, ib[0], ib[2], ib[4], ibd[0], ibd[2], ibd[4],
-                     nib, njb, nkb, irho, irho, irho, 1, 1, 1, g1->f[w], 1.0, 0);
+                     nib, njb, nkb, irho, irho, irho, 1, 1, 1, m->do_ex ? chr1 : g1->f[w], m->ex, m->do_ex);
       }
     }
   }
@@ -440,7 +459,15 @@ static void grid_setup(or_amr *m, ag_grid *g, const double *bb, const double *ld
 
 /* a level's grids from a bbox list: build_sgh then compose_cgh each apply
  * insert_grid (src/gh.c:331-378, 602-928) */
+static void level_from_boxes_perm(or_amr *m, ag_level *lv, int nb, const double *boxes, const double *ldx,
+                                 int *perm);
 static void level_from_boxes(or_amr *m, ag_level *lv, int nb, const double *boxes, const double *ldx)
+{
+  level_from_boxes_perm(m, lv, nb, boxes, ldx, 0);
+}
+/* perm (may be NULL): perm[k] = index in `boxes` of grid k */
+static void level_from_boxes_perm(or_amr *m, ag_level *lv, int nb, const double *boxes, const double *ldx,
+                                 int *perm)
 {
   double(*l1)[6] = malloc(sizeof(double[6]) * (nb + 1)), (*l2)[6] = malloc(sizeof(double[6]) * (nb + 1));
   int *i1 = malloc(sizeof(int) * (nb + 1)), *i2 = malloc(sizeof(int) * (nb + 1));
@@ -450,6 +477,8 @@ static void level_from_boxes(or_amr *m, ag_level *lv, int nb, const double *boxe
   lv->ng = nb;
   lv->g = (ag_grid *)calloc(nb > 0 ? (size_t)nb : 1, sizeof(ag_grid));
   for (k = 0; k < nb; k++) grid_setup(m, &lv->g[k], l2[k], ldx);
+  if (perm)
+    for (k = 0; k < nb; k++) perm[k] = i2[k];
   free(l1);
   free(l2);
   free(i1);
@@ -474,6 +503,19 @@ or_amr *or_amr_create(const int *base_shape, const double *bbox, int namr, const
   }
   m->nbase = ncoarse + 1;
   m->nlev = ncoarse + 1 + namr;
+  m->namr_lev = namr + 1;
+  m->mcw = mcw;
+  memcpy(m->dx0, dx0, sizeof dx0);
+  m->nsgh[0] = 1;
+  m->sgh[0] = (double *)malloc(6 * sizeof(double));
+  memcpy(m->sgh[0], bbox, 6 * sizeof(double));
+  for (l = 0, off = 0; l < namr; l++) {
+    m->nsgh[l + 1] = nbox[l];
+    m->sgh[l + 1] = (double *)malloc(6 * sizeof(double) * (nbox[l] > 0 ? nbox[l] : 1));
+    memcpy(m->sgh[l + 1], boxes + 6 * off, 6 * sizeof(double) * nbox[l]);
+    off += nbox[l];
+  }
+  off = 0;
   for (L = 1; L <= ncoarse; L++) {
     int c = 1 << (ncoarse + 1 - L);
     for (d = 0; d < 3; d++) ldx[d] = dx0[d] * c;
@@ -504,12 +546,97 @@ void or_amr_destroy(or_amr *m)
   int L, g, w;
   if (!m) return;
   for (L = 0; L < m->nlev; L++) {
-    for (g = 0; g < m->lev[L].ng; g++)
-      for (w = 0; w < OA_NUM; w++) free(m->lev[L].g[g].f[w]);
+    if (!m->lev[L].shared)
+      for (g = 0; g < m->lev[L].ng; g++)
+        for (w = 0; w < OA_NUM; w++) free(m->lev[L].g[g].f[w]);
     free(m->lev[L].g);
   }
+  for (L = 0; L < OR_MAXLEV; L++) free(m->sgh[L]);
   free(m);
 }
+
+/* PAMR_build_mgh(min_lev, max_lev, tl) for min_lev = amr_lev (src/gh.c:
+ * 1222-1365): AMR levels amr_lev .. finest replicated -- their grid
+ * functions are m's; the MG-only ones are zeroed, as amr_to_mgh allocates
+ * them fresh (src/gh.c:1150-1157) -- and below them, for cf = 2, 4, ..., a
+ * level of the boxes given for amr_lev whose calc_island_no(cf) is nonzero,
+ * spacing dx * cf, coarsest where the island number is negative, until no box
+ * survives (the n < 0 branch, src/gh.c:1281-1340); the grids of level 1 are
+ * all coarsest (:1352-1354).  The excision mask of a new level is left to
+ * the caller, as init_ex_mask calls the application (:1322). */
+or_amr *or_amr_create_mgh(or_amr *m, int amr_lev)
+{
+  or_amr *q;
+  int L, l, k, cf, j, nc = 0, nsh, mcwv[3];
+  int num = m->nsgh[amr_lev - 1];
+  const double *bb = m->sgh[amr_lev - 1];
+  double dx[3], *isl, *keep;
+  int *kisl, *perm;
+  ag_level coarse[OR_MAXLEV];
+  if (amr_lev < 1 || amr_lev > m->namr_lev) return 0;
+  q = (or_amr *)calloc(1, sizeof(or_amr));
+  memcpy(q->cbbox, m->cbbox, sizeof q->cbbox);
+  memcpy(q->periodic, m->periodic, sizeof q->periodic);
+  q->op = m->op; q->pre = m->pre; q->pst = m->pst; q->max_citer = m->max_citer;
+  q->crtol = m->crtol; q->w0_r = m->w0_r; q->w0_i = m->w0_i;
+  q->rop_u = m->rop_u; q->rop_res = m->rop_res; q->rop_src = m->rop_src; q->iord = m->iord;
+  q->do_ex = m->do_ex; q->ex = m->ex; q->mcw = m->mcw;
+  for (j = 0; j < 3; j++) {
+    dx[j] = m->dx0[j] / (double)(1 << (amr_lev - 1));
+    mcwv[j] = m->mcw;
+  }
+  nsh = m->nlev - (m->nbase + amr_lev - 1) + 1;
+  isl = (double *)malloc(sizeof(double) * num);
+  keep = (double *)malloc(sizeof(double) * 6 * num);
+  kisl = (int *)malloc(sizeof(int) * num);
+  perm = (int *)malloc(sizeof(int) * num);
+  for (cf = 2; nc + nsh < OR_MAXLEV; cf *= 2) {
+    int n = 0;
+    double ldx[3];
+    ag_level *lv = &coarse[nc];
+    or_calc_island_no(bb, num, isl, dx, cf, 3, mcwv);
+    for (k = 0; k < num; k++)
+      if (isl[k] != 0) {
+        memcpy(keep + 6 * n, bb + 6 * k, 6 * sizeof(double));
+        kisl[n++] = isl[k] < 0;
+      }
+    if (!n) break;
+    for (j = 0; j < 3; j++) ldx[j] = dx[j] * cf;
+    memset(lv, 0, sizeof *lv);
+    level_from_boxes_perm(q, lv, n, keep, ldx, perm);
+    for (k = 0; k < n; k++) lv->g[k].coarsest = kisl[perm[k]];
+    nc++;
+  }
+  free(isl);
+  free(keep);
+  free(kisl);
+  free(perm);
+  q->nbase = nc + 1;
+  q->nlev = nc + nsh;
+  for (L = 1; L <= nc; L++) q->lev[L - 1] = coarse[nc - L];
+  for (l = 0; l < nsh; l++) {
+    const ag_level *src = &m->lev[m->nbase + amr_lev - 2 + l];
+    ag_level *dst = &q->lev[nc + l];
+    int g, w;
+    dst->ng = src->ng;
+    dst->in_amrh = 1;
+    dst->shared = 1;
+    dst->g = (ag_grid *)malloc(sizeof(ag_grid) * (src->ng > 0 ? src->ng : 1));
+    memcpy(dst->g, src->g, sizeof(ag_grid) * src->ng);
+    for (g = 0; g < dst->ng; g++) {
+      dst->g[g].coarsest = 0;
+      dst->g[g].comm = 1;
+      for (w = 0; w < OA_NUM; w++) /* MG-only grid functions start at zero */
+        if (dst->g[g].f[w] && (w == OA_RHS || w == OA_RES || w == OA_LOP || w == OA_CMASK))
+          memset(dst->g[g].f[w], 0, sizeof(double) * (size_t)dst->g[g].shape[0] * dst->g[g].shape[1] *
+                                         dst->g[g].shape[2]);
+    }
+  }
+  for (k = 0; k < q->lev[0].ng; k++) q->lev[0].g[k].coarsest = 1;
+  return q;
+}
+
+int or_amr_grid_coarsest(const or_amr *m, int L, int g) { return m->lev[L - 1].g[g].coarsest; }
 
 int or_amr_nlev(const or_amr *m) { return m->nlev; }
 int or_amr_base_level(const or_amr *m) { return m->nbase; }
@@ -531,6 +658,12 @@ void or_amr_set_params(or_amr *m, int pre, int pst, int max_citer, double crtol,
   m->pre = pre; m->pst = pst; m->max_citer = max_citer; m->crtol = crtol;
   m->w0_r = w0_r; m->w0_i = w0_i; m->rop_u = rop_u; m->rop_res = rop_res;
   m->rop_src = rop_src; m->iord = interp_ord;
+}
+
+void or_amr_set_excision(or_amr *m, int do_ex, double ex)
+{
+  m->do_ex = do_ex;
+  m->ex = ex;
 }
 
 void or_amr_set_tre(or_amr *m, int save_tre, int ntre)
@@ -595,7 +728,8 @@ static double hook_relax_level(or_amr *m, int L, int comm_only)
   for (gi = 0; gi < m->lev[L - 1].ng; gi++) {
     ag_grid *g = &m->lev[L - 1].g[gi];
     if (!(g->comm || !comm_only)) continue;
-    r = or_relax(m->op, g->f[OA_U], g->f[OA_RHS], g->f[OA_RHO], g->f[OA_A], g->f[OA_C], g->f[OA_CMASK], 0, 1.0,
+    r = or_relax(m->op, g->f[OA_U], g->f[OA_RHS], g->f[OA_RHO], g->f[OA_A], g->f[OA_C], g->f[OA_CMASK],
+                 m->do_ex ? g->f[OA_CHR] : 0, m->ex,
                  g->shape[0], g->shape[1], g->shape[2], g->hdx[0], g->hdx[1], g->hdx[2]);
     if (first) lres = r;
     lres = MAX(r, lres);
@@ -610,7 +744,8 @@ static double hook_residual_level(or_amr *m, int L)
   for (gi = 0; gi < m->lev[L - 1].ng; gi++) {
     ag_grid *g = &m->lev[L - 1].g[gi];
     r = or_residual(m->op, g->f[OA_RES], g->f[OA_RHS], g->f[OA_U], g->f[OA_RHO], g->f[OA_A], g->f[OA_C],
-                    g->f[OA_CMASK], 0, 1.0, g->shape[0], g->shape[1], g->shape[2], g->hdx[0], g->hdx[1], g->hdx[2]);
+                    g->f[OA_CMASK], m->do_ex ? g->f[OA_CHR] : 0, m->ex, g->shape[0], g->shape[1], g->shape[2],
+                    g->hdx[0], g->hdx[1], g->hdx[2]);
     if (first) lres = r;
     lres = MAX(r, lres);
     first = 0;
@@ -622,7 +757,8 @@ static void hook_lop_level(or_amr *m, int L)
   int gi;
   for (gi = 0; gi < m->lev[L - 1].ng; gi++) {
     ag_grid *g = &m->lev[L - 1].g[gi];
-    or_lop(m->op, g->f[OA_LOP], g->f[OA_U], g->f[OA_RHO], g->f[OA_A], g->f[OA_C], g->f[OA_CMASK], 0, 1.0,
+    or_lop(m->op, g->f[OA_LOP], g->f[OA_U], g->f[OA_RHO], g->f[OA_A], g->f[OA_C], g->f[OA_CMASK],
+           m->do_ex ? g->f[OA_CHR] : 0, m->ex,
            g->shape[0], g->shape[1], g->shape[2], g->hdx[0], g->hdx[1], g->hdx[2]);
   }
 }

@@ -26,7 +26,7 @@ extern "C" {
 typedef struct or_amr or_amr;
 
 /* field ids of or_amr_field */
-enum { OA_U = 0, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_TRE, OA_NUM };
+enum { OA_U = 0, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_TRE, OA_CHR, OA_NUM };
 
 /* base_shape: points of the base grid over bbox; namr: number of refined
  * AMR levels; nbox[l]: boxes of refined level l+2; boxes: 6 doubles per box
@@ -34,6 +34,13 @@ enum { OA_U = 0, OA_RHO, OA_RHS, OA_RES, OA_LOP, OA_CMASK, OA_A, OA_C, OA_TRE, O
 or_amr *or_amr_create(const int *base_shape, const double *bbox, int namr,
                       const int *nbox, const double *boxes, int mcw, int op);
 void or_amr_destroy(or_amr *m);
+/* the MG hierarchy PAMR_build_mgh(amr_lev, finest, tl) builds from AMR level
+ * amr_lev > 1 upward (src/gh.c:1222-1365): islands of that level's boxes
+ * coarsened below it, its own levels sharing m's grid functions (destroy it
+ * before m) */
+or_amr *or_amr_create_mgh(or_amr *m, int amr_lev);
+/* 1 when grid g of level L is flagged coarsest */
+int or_amr_grid_coarsest(const or_amr *m, int L, int g);
 int or_amr_nlev(const or_amr *m);
 /* MG level of AMR level 1 (the base grid) */
 int or_amr_base_level(const or_amr *m);
@@ -51,6 +58,12 @@ void or_amr_init(or_amr *m);
  * on each coarse level, syncs it and interpolates it to the next finer
  * level.  ntre is AMRD_num_f_tre_vars. */
 void or_amr_set_tre(or_amr *m, int save_tre, int ntre);
+/* PAMR's excision (src/pamr.c:1983-2021): with do_ex set, OA_CHR of every
+ * grid is the mask (ex = excised; the caller fills it on every level), the
+ * transfers take local_transfer's do_ex paths (renormalised restriction,
+ * dminterp3d with the source mask, clear_ex of source and destination,
+ * src/transfer.c:946-1163) and the hooks skip excised points */
+void or_amr_set_excision(or_amr *m, int do_ex, double ex);
 /* one FAS V-cycle over the hierarchy; returns res[Lf] (max over the grids
  * of Lf); lev_norms (nlev doubles) gets the last norm of every level */
 double or_amr_vcycle(or_amr *m, double *lev_norms, int *citer);

@@ -161,6 +161,9 @@ def _declare_pamr(L):
     L.PAMR_compose_hierarchy.argtypes = [i, i, i, _ip, _dp, d]
     L.PAMR_build_mgh.argtypes = [i, i, i]
     L.PAMR_destroy_mgh.argtypes = []
+    L.PAMR_excision_on.argtypes = [s, C.c_void_p, d, i]
+    L.PAMR_excision_off.argtypes = []
+    L.PAMR_excision_off.restype = None
     for f in ("PAMR_set_rho", "PAMR_get_rho"):
         getattr(L, f).argtypes = [_ip, _ip, i]
         getattr(L, f).restype = None

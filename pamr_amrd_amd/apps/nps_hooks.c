@@ -14,7 +14,7 @@
  * source; L[V] = lap V + rho, the "rhs on the lhs" convention of
  * include/amrd.h:201-203), and for op 1 the coefficients a, c
  * (L[V] = div(a grad V) + c V + rho).  amrd_mg_def_vars adds V_res, V_lop,
- * V_rhs and cmask.
+ * V_rhs and cmask (and chr with excision, nps_define_vars_ex).
  *
  * The level hooks (nps_set_level_hooks) launch the same kernels for every
  * grid of a level and read the norms back once per level.
@@ -31,8 +31,10 @@
 #define NPS_MAX_GRIDS 4096
 
 static int op;
-static int V_gfn, rho_gfn, a_gfn, c_gfn, V_res_gfn, V_rhs_gfn, V_lop_gfn, cmask_gfn;
-static double *V, *rho, *a, *c, *V_res, *V_rhs, *V_lop, *cmask;
+static int V_gfn, rho_gfn, a_gfn, c_gfn, V_res_gfn, V_rhs_gfn, V_lop_gfn, cmask_gfn, chr_gfn;
+static double *V, *rho, *a, *c, *V_res, *V_rhs, *V_lop, *cmask, *chr;
+static int do_ex;      /* excision: the hooks skip points where chr == ex_val */
+static double ex_val;
 static int shape[3];
 static double dx, dy, dz;
 static double *d_norm; /* device norms, one per grid of a level */
@@ -57,6 +59,7 @@ static int ldptr(void)
   V_rhs = gfs[V_rhs_gfn - 1];
   V_lop = gfs[V_lop_gfn - 1];
   cmask = gfs[cmask_gfn - 1];
+  chr = do_ex ? gfs[chr_gfn - 1] : NULL;
   dx = x[0][1] - x[0][0];
   dy = x[1][1] - x[1][0];
   dz = x[2][1] - x[2][0];
@@ -79,14 +82,14 @@ static double read_norm(int n)
 static int relax_grid(double *norm)
 {
   if (!ldptr()) return -1;
-  return pmg_hook_relax(op, V, V_rhs, rho, a, c, cmask, NULL, 0.0, shape[0], shape[1], shape[2], dx, dy, dz,
+  return pmg_hook_relax(op, V, V_rhs, rho, a, c, cmask, chr, ex_val, shape[0], shape[1], shape[2], dx, dy, dz,
                         norm, stream());
 }
 
 static int residual_grid(double *norm)
 {
   if (!ldptr()) return -1;
-  return pmg_hook_residual(op, V_res, V_rhs, V, rho, a, c, cmask, NULL, 0.0, shape[0], shape[1], shape[2], dx,
+  return pmg_hook_residual(op, V_res, V_rhs, V, rho, a, c, cmask, chr, ex_val, shape[0], shape[1], shape[2], dx,
                            dy, dz, norm, stream());
 }
 
@@ -106,7 +109,7 @@ double app_MG_residual(void)
 void app_L_op(void)
 {
   if (!ldptr()) return;
-  if (pmg_hook_lop(op, V_lop, V, rho, a, c, cmask, NULL, 0.0, shape[0], shape[1], shape[2], dx, dy, dz, stream()))
+  if (pmg_hook_lop(op, V_lop, V, rho, a, c, cmask, chr, ex_val, shape[0], shape[1], shape[2], dx, dy, dz, stream()))
     printf("app_L_op: hook kernel failed\n");
 }
 
@@ -180,7 +183,17 @@ int nps_define_vars_tl(int op_, int rop_u, int rop_res, int rop_src, int num_tl)
                            PAMR_NO_INTERP, PAMR_NO_SYNC, 1, PAMR_NO_TRANSFER, pbt))
       return 0;
   }
-  return amrd_mg_def_vars("V", rop_res == PAMR_FW_RESTR ? "V_res" : "", "V", 0);
+  return amrd_mg_def_vars("V", rop_res == PAMR_FW_RESTR ? "V_res" : "", "V", do_ex);
+}
+
+/* the same with excision: AMRD's chr variable is defined too (amrd/io.c:
+ * 988-1005) and the hooks skip the points where chr == ex; the application
+ * then turns PAMR's excision on with PAMR_excision_on("chr", fill, ex, ...) */
+int nps_define_vars_ex(int op_, int rop_u, int rop_res, int rop_src, double ex)
+{
+  do_ex = 1;
+  ex_val = ex;
+  return nps_define_vars_tl(op_, rop_u, rop_res, rop_src, 1);
 }
 
 /* looks up the gfns, allocates the norm buffers and registers the per-grid
@@ -197,6 +210,7 @@ int nps_init(void)
   if ((V_rhs_gfn = PAMR_get_gfn("V_rhs", PAMR_MGH, 0)) <= 0) return 0;
   if ((V_lop_gfn = PAMR_get_gfn("V_lop", PAMR_MGH, 0)) <= 0) return 0;
   if ((cmask_gfn = PAMR_get_gfn("cmask", PAMR_MGH, 0)) <= 0) return 0;
+  if (do_ex && (chr_gfn = PAMR_get_gfn("chr", PAMR_MGH, 0)) <= 0) return 0;
   if (!d_norm && hipMalloc((void **)&d_norm, sizeof(double) * NPS_MAX_GRIDS) != hipSuccess) return 0;
   if (!h_norm && hipHostMalloc((void **)&h_norm, sizeof(double) * NPS_MAX_GRIDS, hipHostMallocDefault) != hipSuccess)
     return 0;
@@ -216,6 +230,8 @@ void nps_release(void)
   if (d_norm) hipFree(d_norm);
   if (h_norm) hipHostFree(h_norm);
   d_norm = h_norm = NULL;
+  do_ex = 0;
+  ex_val = 0.0;
 }
 
 /* host <-> device copies of n doubles on amrd_gpu_stream(), synchronous */

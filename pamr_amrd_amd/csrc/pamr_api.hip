@@ -92,6 +92,11 @@ struct Ctx {
   Iter stack[kIterStack];
   int sp = 0;
   hipStream_t stream = nullptr;
+  // excision (src/pamr.c:1983-2037): the mask variable and the application's fill
+  bool ex_on = false;
+  double excised = 0.0;
+  std::string ex_var;
+  void (*ex_fill)(double *, int, int *, double *, double) = nullptr;
 };
 
 Ctx *C = nullptr;
@@ -202,6 +207,33 @@ View *cur_view(const char *who) {
     return nullptr;
   }
   return &C->views[Lp - 1][C->it.g];
+}
+
+// the pmg field of the excision mask (-1: excision off or no hierarchy)
+int mask_field() {
+  if (!C->ex_on || !C->A) return -1;
+  Var *v = find_var(C->ex_var.c_str());
+  return (v && !v->amr_field.empty()) ? v->amr_field[0] : -1;
+}
+
+// init_ex_mask (src/gh.c:1555-1577) on pmg levels L0..L1: the application's
+// fill function on a host array of each grid, uploaded into the mask field
+bool init_ex_mask(int L0, int L1) {
+  const int f = mask_field();
+  if (f < 0 || !C->ex_fill) return false;
+  if (hipStreamSynchronize(C->stream) != hipSuccess) return false;
+  for (int L = L0; L <= L1; L++)
+    for (int g = 0; g < pmg_amr_ngrids(C->A, L); g++) {
+      int shp[3];
+      double bb[6];
+      pmg_amr_grid_info(C->A, L, g, shp, bb);
+      std::vector<double> h((size_t)shp[0] * shp[1] * shp[2], 0.0);
+      C->ex_fill(h.data(), 3, shp, bb, C->excised);
+      if (hipMemcpy(pmg_amr_field(C->A, L, g, f), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        return false;
+    }
+  return true;
 }
 
 // fields of the gfns whose transfer bit is set, with their operator
@@ -865,6 +897,11 @@ int PAMR_compose_hierarchy(int min_lev, int max_lev, int num, int *lev, double *
   for (int l = 0; l < max_lev; l++) C->t[l] = t;
   C->it = Iter();
   build_views();
+  // src/pamr.c:309
+  if (C->ex_on && !init_ex_mask(C->nbase, C->nlev)) {
+    std::printf("PAMR_compose_hierarchy: error ... filling the excision mask failed\n");
+    return 0;
+  }
   return 1;
 oom:
   std::printf("PAMR_compose_hierarchy: error ... out of device memory (at most %d grid functions)\n",
@@ -906,6 +943,13 @@ int PAMR_build_mgh(int min_lev, int max_lev, int tl) {
     }
   }
   build_views();
+  // the excision mask of the new MG levels (src/gh.c:1322)
+  if (C->ex_on && C->nbase > 1 && !init_ex_mask(1, C->nbase - 1)) {
+    std::printf("PAMR_build_mgh: error ... filling the excision mask failed\n");
+    C->mgh = false;
+    build_views();
+    return 0;
+  }
   // initial injection into each new MG level (src/gh.c:1237-1239,1334)
   PAMR_thaw_tf_bits();
   PAMR_set_tf_bits(1, PAMR_MGH, PAMR_TF_MGH_INIT);
@@ -951,6 +995,49 @@ int PAMR_get_min_lev(int hier) {
     return 0;
   }
   return 1;
+}
+
+// ---- excision (src/pamr.c:1983-2037) ----------------------------------------------
+int PAMR_excision_on(char *ex_mask_var, void (*app_fill_ex_mask_fnc)(double *, int, int *, double *, double),
+                     double excised, int initialize_now) {
+  if (!C) {
+    std::printf("PAMR_excision_on: error ... no current context\n");
+    return 0;
+  }
+  Var *f = find_var(ex_mask_var);
+  if (!f) {
+    std::printf("PAMR_excision_on: error ... no variable %s\n", ex_mask_var ? ex_mask_var : "(null)");
+    return 0;
+  }
+  if (!f->a.in_amrh || !f->a.in_mgh) {
+    std::printf("PAMR_excision_on: error ... variable %s must be defined in both the MG and AMR hierarchies\n",
+                ex_mask_var);
+    return 0;
+  }
+  if (f->a.num_tl != 1) {
+    std::printf("PAMR_excision_on: error ... variable %s can only have 1 time level\n", ex_mask_var);
+    return 0;
+  }
+  C->ex_on = true;
+  C->excised = excised;
+  C->ex_var = f->name;
+  C->ex_fill = app_fill_ex_mask_fnc;
+  if (initialize_now && C->A) {
+    const int Lmg0 = C->mgh ? 1 : C->nbase;
+    if (!init_ex_mask(Lmg0, C->nlev)) {
+      std::printf("PAMR_excision_on: error ... filling the excision mask failed\n");
+      return 0;
+    }
+  }
+  return 1;
+}
+
+void PAMR_excision_off(void) {
+  if (!C) {
+    std::printf("PAMR_excision_off: error ... no current context\n");
+    return;
+  }
+  C->ex_on = false;
 }
 
 void PAMR_set_time(int lev, double t) {
@@ -1048,7 +1135,10 @@ int PAMR_inject(int lf, int tl, int hierarchy) {
       std::printf("PAMR_inject: error ... injection operator %i is not supported\n", op);
       return 0;
     }
-  if (pmg_amr_inject_fields(C->A, Lp, (int)f.size(), f.data(), ops.data()) != 0) {
+  const int mf = mask_field();
+  const int rc = mf >= 0 ? pmg_amr_inject_fields_ex(C->A, Lp, (int)f.size(), f.data(), ops.data(), mf, C->excised)
+                         : pmg_amr_inject_fields(C->A, Lp, (int)f.size(), f.data(), ops.data());
+  if (rc != 0) {
     std::printf("PAMR_inject: error ... device transfer failed\n");
     return 0;
   }
@@ -1071,7 +1161,10 @@ int PAMR_interp(int lc, int tl, int hierarchy) {
     std::vector<int> sel;
     for (size_t q = 0; q < f.size(); q++)
       if (ops[q] == ord) sel.push_back(f[q]);
-    if (pmg_amr_interp_fields_ord(C->A, Lp, ord, (int)sel.size(), sel.data()) != 0) {
+    const int mf = mask_field();
+    const int rc = mf >= 0 ? pmg_amr_interp_fields_ex(C->A, Lp, ord, (int)sel.size(), sel.data(), mf, C->excised)
+                           : pmg_amr_interp_fields_ord(C->A, Lp, ord, (int)sel.size(), sel.data());
+    if (rc != 0) {
       std::printf("PAMR_interp: error ... device transfer failed\n");
       return 0;
     }

@@ -390,6 +390,12 @@ __global__ __launch_bounds__(256) void k_fold_max(NormWs ws, const FoldSpec *spe
 }
 
 // clear_MG_vars on one level: res, rhs and lop in one launch
+// clear_ex (src/transfer.c:930-935): f = 0 where chr == ex
+__global__ __launch_bounds__(256) void k_clear_ex(double *f, const double *chr, double ex, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n && chr[q] == ex) f[q] = 0.0;
+}
+
 __global__ __launch_bounds__(256) void k_clear3(double *a, double *b, double *c, size_t n) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
@@ -442,7 +448,8 @@ struct Hier;
 // level state
 // ---------------------------------------------------------------------------
 struct XPlan {
-  std::vector<XJob> jobs;
+  std::vector<XJob> raw;   // the transfer list in the reference's order (excision path)
+  std::vector<XJob> jobs;  // resolved into disjoint boxes (batched path)
   XJob *d_jobs = nullptr;
   XBlk *d_blks = nullptr;
   int nblocks = 0;
@@ -467,7 +474,10 @@ struct ALevel {
   int nptiles = 0;
   int *d_pstart = nullptr;
   XPlan sync0, sync1, inject, interp;  // inject: from L+1 into L; interp: from L-1 into L
+  XPlan sync1c;  // sync (AMR_bdy_width 1) among the coarsest grids only (comm = coarsest)
   bool interp_once = false;  // the interp plan writes every point of every grid exactly once
+  bool shared = false;       // grid functions belong to the parent hierarchy, its level plev
+  int plev = 0;
 };
 
 }  // namespace
@@ -491,6 +501,15 @@ struct pmg_amr {
   int use_graph = 1;
   hipGraphExec_t graph = nullptr;
   hipGraphExec_t graph_res = nullptr;
+  double *xtmp = nullptr;  // interpolation temporary of the excision path
+  size_t xtmp_cap = 0;
+  // the boxes given for each AMR level (index l-1) in the caller's order
+  // (save_sgh_bboxes, src/gh.c:273-298); level 1 holds the base box
+  std::vector<std::vector<double>> sgh;
+  double dx0[3] = {0, 0, 0};
+  int mcw = 5;
+  const pmg_amr *parent = nullptr;  // an MG hierarchy over AMR levels of parent
+  bool own_stream = true;
 };
 
 namespace {
@@ -567,12 +586,13 @@ void seg_subtract(const std::vector<HGrid> &grids, const Seg &A, const Seg &B, s
 }
 
 // build_owned_gsl (src/transfer.c:415-458) for one rank, with extend_gsl
-std::vector<Seg> owned_segs(const pmg_amr *A, int L, int interior, int extend) {
+std::vector<Seg> owned_segs(const pmg_amr *A, int L, int interior, int extend, bool coarsest_only = false) {
   const std::vector<HGrid> &grids = A->lev[L - 1].g;
   const int cint = interior ? 1 : 0, cred = interior > 1 ? interior - 1 : 0;
   std::vector<Seg> owned, prior;
   for (int gi = 0; gi < (int)grids.size(); gi++) {
     Seg one;
+    if (coarsest_only && !grids[gi].coarsest) continue;  // comm == 0 (src/transfer.c:438)
     if (!seg_alloc(A, grids[gi], gi, cint, cred, one)) continue;
     std::vector<Seg> C{one};
     for (const Seg &B : prior) {  // gsl_subtract (src/transfer.c:285-314)
@@ -597,11 +617,13 @@ std::vector<Seg> owned_segs(const pmg_amr *A, int L, int interior, int extend) {
 }
 
 // communicate() transfer list -> XJobs (before conflict resolution)
-int make_jobs(const pmg_amr *A, int l1, int l2, int interior, int extend, std::vector<XJob> &jobs) {
+int make_jobs(const pmg_amr *A, int l1, int l2, int interior, int extend, std::vector<XJob> &jobs,
+              bool coarsest_only = false) {
   const std::vector<HGrid> &G1 = A->lev[l1 - 1].g, &G2 = A->lev[l2 - 1].g;
-  std::vector<Seg> src = owned_segs(A, l1, interior, extend), dst;
+  std::vector<Seg> src = owned_segs(A, l1, interior, extend, coarsest_only), dst;
   for (int gi = 0; gi < (int)G2.size(); gi++) {  // build_complete_gsl, AMR_bdy_only = 0
     Seg one;
+    if (coarsest_only && !G2[gi].coarsest) continue;  // comm == 0 (src/transfer.c:482)
     if (seg_alloc(A, G2[gi], gi, 0, 0, one)) dst.push_back(one);
   }
   for (const Seg &s : src)
@@ -737,6 +759,7 @@ std::vector<XJob> resolve(const std::vector<XJob> &jobs) {
 }
 
 int upload_plan(pmg_amr *A, XPlan &P, const std::vector<XJob> &jobs) {
+  P.raw = jobs;
   P.jobs = resolve(jobs);
   std::vector<XBlk> blks;
   for (int q = 0; q < (int)P.jobs.size(); q++) {
@@ -781,6 +804,61 @@ std::vector<HGrid> level_grids(int nb, const double *boxes, const double *ldx) {
   std::vector<HGrid> out(nb);
   for (int k = 0; k < nb; k++) grid_geometry(out[k], l2[k].data(), ldx);
   return out;
+}
+
+// build_sgh + compose_cgh order of a box list, with perm[k] = the index in
+// `boxes` of grid k
+std::vector<HGrid> level_grids_perm(int nb, const double *boxes, const double *ldx, std::vector<int> &perm) {
+  std::vector<std::vector<double>> l1, l2;
+  std::vector<int> i1, i2;
+  auto insert = [](std::vector<std::vector<double>> &list, std::vector<int> &idx, const double *bb, int id) {
+    size_t p = 0;
+    while (p < list.size() && list[p][0] < bb[0] && list[p][2] < bb[2] && list[p][4] < bb[4] && list[p][1] < bb[1] &&
+           list[p][3] < bb[3] && list[p][5] < bb[5])
+      p++;
+    list.insert(list.begin() + p, std::vector<double>(bb, bb + 6));
+    idx.insert(idx.begin() + p, id);
+  };
+  for (int k = 0; k < nb; k++) insert(l1, i1, boxes + 6 * k, k);
+  for (int k = 0; k < nb; k++) insert(l2, i2, l1[k].data(), i1[k]);
+  std::vector<HGrid> out(nb);
+  for (int k = 0; k < nb; k++) grid_geometry(out[k], l2[k].data(), ldx);
+  perm = i2;
+  return out;
+}
+
+// calc_island_no (src/gh.c:1427-1467) of a box list: per box 0 when it (or a
+// box it is joined with) cannot be coarsened by cf, negative when it becomes
+// coarsest there.  Boxes are joined by the reference's own test (each
+// dimension: not lo_i > hi_j and not hi_i > lo_j, with fuzz dx/2).
+std::vector<int> island_numbers(const double *bb, int num, const double *dx, int cf, int mcw) {
+  std::vector<int> isl(num);
+  for (int i = 0; i < num; i++) {
+    isl[i] = i + 1;
+    for (int j = 0; j < 3; j++) {
+      const int shape = (int)((bb[6 * i + 2 * j + 1] - bb[6 * i + 2 * j]) / dx[j] + 1.5);
+      if ((((shape - 1) % cf) && ((shape - 1) / cf + 1) >= mcw) || (((shape - 1) / cf + 1) < mcw)) isl[i] = 0;
+      else if (((shape - 1) / cf / 2 + 1) < mcw) isl[i] = -std::abs(isl[i]);
+    }
+  }
+  for (int delta = 1; delta;) {
+    delta = 0;
+    for (int i = 0; i < num; i++)
+      for (int j = i + 1; j < num; j++) {
+        if (isl[i] == isl[j]) continue;
+        bool overlap = true;
+        for (int k = 0; k < 3; k++)
+          if (fz_gt(bb[6 * i + 2 * k], bb[6 * j + 2 * k + 1], dx[k] / 2) ||
+              fz_gt(bb[6 * i + 2 * k + 1], bb[6 * j + 2 * k], dx[k] / 2))
+            overlap = false;
+        if (overlap) {
+          delta++;
+          if (isl[i] == 0 || isl[j] == 0) isl[i] = isl[j] = 0;
+          else isl[i] = isl[j] = std::min(isl[i], isl[j]);
+        }
+      }
+  }
+  return isl;
 }
 
 // calc_island_no (src/gh.c:1427-1467) of a single box
@@ -1232,6 +1310,130 @@ int build_folds(pmg_amr *A) {
   return 0;
 }
 
+// per-level device state of a hierarchy whose level grids are set: pools
+// (aliased from the parent on shared levels), grid tables, child masks, sweep
+// tiles, transfer plans, norm workspace and fold table.  Returns 0 / -1.
+int setup_levels(pmg_amr *A, int op) {
+  if (A->own_stream && hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking) != hipSuccess) return -1;
+  int cap = 1 << 14;
+  for (int L = 1; L <= A->nlev; L++) {
+    ALevel &Lv = A->lev[L - 1];
+    // all grids of a level share one spacing (dyadic boxes); check it
+    for (const HGrid &g : Lv.g)
+      for (int d = 0; d < 3; d++)
+        if (g.hdx[d] != Lv.g[0].hdx[d] || g.n[d] < 3) return -1;
+    Lv.sp = spacing_of(Lv.g[0].hdx);
+    size_t pool = 0;
+    std::vector<GridGeo> geo;
+    for (HGrid &g : Lv.g) {
+      g.off = pool;
+      geo.push_back(GridGeo{pool, g.n[0], g.n[1], g.n[2]});
+      pool += (g.size() + 63) / 64 * 64;
+    }
+    Lv.pool = pool;
+    if (Lv.shared) {
+      const ALevel &P = A->parent->lev[Lv.plev - 1];
+      if (P.pool != pool) return -1;
+      Lv.u = P.u; Lv.ua = P.ua; Lv.rho = P.rho; Lv.rhs = P.rhs; Lv.res = P.res; Lv.lop = P.lop;
+      Lv.a = P.a; Lv.c = P.c;
+      Lv.extra = P.extra;
+    } else {
+      for (double **p : {&Lv.u, &Lv.ua, &Lv.rho, &Lv.rhs, &Lv.res, &Lv.lop}) {
+        if (hipMalloc(p, pool * sizeof(double)) != hipSuccess) return -1;
+        (void)hipMemset(*p, 0, pool * sizeof(double));
+      }
+      if (op == 1)
+        for (double **p : {&Lv.a, &Lv.c}) {
+          if (hipMalloc(p, pool * sizeof(double)) != hipSuccess) return -1;
+          (void)hipMemset(*p, 0, pool * sizeof(double));
+        }
+      for (int f = 0; f < A->nextra; f++) {  // the parent's extra fields, zero on new levels
+        double *p = nullptr;
+        if (hipMalloc(&p, pool * sizeof(double)) != hipSuccess) return -1;
+        (void)hipMemset(p, 0, pool * sizeof(double));
+        Lv.extra.push_back(p);
+      }
+    }
+    if (hipMalloc(&Lv.d_geo, sizeof(GridGeo) * geo.size()) != hipSuccess) return -1;
+    (void)hipMemcpy(Lv.d_geo, geo.data(), sizeof(GridGeo) * geo.size(), hipMemcpyHostToDevice);
+    // child mask (set_cmask_child) as bytes, fixed for the hierarchy
+    std::vector<uint8_t> child(pool, 0), m;
+    for (int gi = 0; gi < (int)Lv.g.size(); gi++) {
+      cmask_bytes(A, L, gi, 1, m);
+      std::copy(m.begin(), m.end(), child.begin() + Lv.g[gi].off);
+    }
+    if (hipMalloc(&Lv.child, pool) != hipSuccess) return -1;
+    (void)hipMemcpy(Lv.child, child.data(), pool, hipMemcpyHostToDevice);
+    // sweep tiles: single-grid levels use the wide tile, patch levels 32 x 16
+    const HGrid &g0 = Lv.g[0];
+    if (Lv.g.size() == 1) {
+      Lv.variant = 1;
+      Lv.zchunk = g0.n[2] >= 257 ? 32 : g0.n[2] >= 129 ? 16 : g0.n[2] >= 65 ? 8 : 4;
+    } else {
+      Lv.variant = 2;
+      Lv.zchunk = 8;
+    }
+    if (build_tiles(A, Lv)) return -1;
+    cap = std::max(cap, std::max(Lv.nstiles, Lv.nptiles) + 64);
+  }
+  // transfer plans
+  for (int L = 1; L <= A->nlev; L++) {
+    ALevel &Lv = A->lev[L - 1];
+    std::vector<XJob> j0, j1, ji, jp;
+    int rc = make_jobs(A, L, L, 1, 0, j0);
+    if (!rc) rc = make_jobs(A, L, L, 2, 0, j1);
+    if (!rc && L < A->nlev) rc = make_jobs(A, L + 1, L, 1, 0, ji);
+    if (!rc && L > 1) rc = make_jobs(A, L - 1, L, 0, 1, jp);
+    if (!rc) rc = upload_plan(A, Lv.sync0, j0);
+    if (!rc) rc = upload_plan(A, Lv.sync1, j1);
+    if (!rc) rc = upload_plan(A, Lv.inject, ji);
+    if (!rc) rc = upload_plan(A, Lv.interp, jp);
+    {  // a level with coarsest and other grids: the coarsest loop's sync among the coarsest
+      int nco = 0;
+      for (const HGrid &g : Lv.g) nco += g.coarsest ? 1 : 0;
+      if (!rc && nco > 0 && nco < (int)Lv.g.size()) {
+        std::vector<XJob> jc;
+        rc = make_jobs(A, L, L, 2, 0, jc, true);
+        if (!rc) rc = upload_plan(A, Lv.sync1c, jc);
+      }
+    }
+    if (!rc && L > 1) {  // does the (resolved) interp plan write every point of every grid of L exactly once?
+      std::vector<std::vector<uint8_t>> cnt(Lv.g.size());
+      for (size_t gi = 0; gi < Lv.g.size(); gi++) cnt[gi].assign(Lv.g[gi].size(), 0);
+      bool once = true;
+      for (const XJob &J : Lv.interp.jobs)
+        for (int k = J.lo[2]; k <= J.hi[2] && once; k++)
+          for (int j = J.lo[1]; j <= J.hi[1] && once; j++)
+            for (int i = J.lo[0]; i <= J.hi[0]; i++) {
+              const HGrid &gg = Lv.g[J.dg];
+              uint8_t &c = cnt[J.dg][((size_t)k * gg.n[1] + j) * gg.n[0] + i];
+              if (c) { once = false; break; }
+              c = 1;
+            }
+      for (size_t gi = 0; gi < cnt.size() && once; gi++)
+        for (uint8_t c : cnt[gi])
+          if (!c) { once = false; break; }
+      Lv.interp_once = once;
+    }
+    if (rc) { std::fprintf(stderr, "pamr_amr: transfer plan of level %d failed (%d)\n", L, rc); return -1; }
+  }
+  A->ws.cap = cap;
+  A->ws.nslots = kAmrSlots;
+  if (hipMalloc(&A->ws.partials, sizeof(double) * 2 * (size_t)cap * kAmrSlots) != hipSuccess ||
+      hipMalloc(&A->ws.slot_nb, sizeof(int) * kAmrSlots) != hipSuccess ||
+      hipMalloc(&A->d_norms, sizeof(double) * kAmrSlots) != hipSuccess ||
+      hipMalloc(&A->d_cnorm, sizeof(double) * 2) != hipSuccess || hipMalloc(&A->d_citer, sizeof(int) * 2) != hipSuccess ||
+      hipHostMalloc(&A->h_pinned, sizeof(double) * (kAmrSlots + 4)) != hipSuccess)
+    return -1;
+  (void)hipMemset(A->ws.partials, 0, sizeof(double) * 2 * (size_t)cap * kAmrSlots);
+  (void)hipMemset(A->d_norms, 0, sizeof(double) * kAmrSlots);
+  (void)hipMemset(A->d_cnorm, 0, sizeof(double) * 2);
+  (void)hipMemset(A->d_citer, 0, sizeof(int) * 2);
+  if (build_folds(A)) return -1;
+  if (hipDeviceSynchronize() != hipSuccess) return -1;
+  return 0;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1255,6 +1457,13 @@ pmg_amr *pmg_amr_create(const int *base_shape, const double *bbox, int namr, con
   A->nbase = ncoarse + 1;
   A->nlev = ncoarse + 1 + namr;
   A->lev.resize(A->nlev);
+  std::memcpy(A->dx0, dx0, sizeof dx0);
+  A->mcw = min_mg_cwidth;
+  A->sgh.assign(1, std::vector<double>(bbox, bbox + 6));
+  for (int l = 0, o = 0; l < namr; o += nbox[l], l++) {
+    if (nbox[l] < 1) { delete A; return nullptr; }
+    A->sgh.emplace_back(boxes + 6 * o, boxes + 6 * (o + nbox[l]));
+  }
   for (int L = 1; L <= ncoarse; L++) {
     const int c = 1 << (ncoarse + 1 - L);
     const double ldx[3] = {dx0[0] * c, dx0[1] * c, dx0[2] * c};
@@ -1273,102 +1482,7 @@ pmg_amr *pmg_amr_create(const int *base_shape, const double *bbox, int namr, con
   }
   for (HGrid &g : A->lev[0].g) g.coarsest = true;
   if (A->lev[0].g.size() != 1) { delete A; return nullptr; }
-  if (hipStreamCreateWithFlags(&A->stream, hipStreamNonBlocking) != hipSuccess) { delete A; return nullptr; }
-  int cap = 1 << 14;
-  for (int L = 1; L <= A->nlev; L++) {
-    ALevel &Lv = A->lev[L - 1];
-    // all grids of a level share one spacing (dyadic boxes); check it
-    for (const HGrid &g : Lv.g)
-      for (int d = 0; d < 3; d++)
-        if (g.hdx[d] != Lv.g[0].hdx[d] || g.n[d] < 3) { pmg_amr_destroy(A); return nullptr; }
-    Lv.sp = spacing_of(Lv.g[0].hdx);
-    size_t pool = 0;
-    std::vector<GridGeo> geo;
-    for (HGrid &g : Lv.g) {
-      g.off = pool;
-      geo.push_back(GridGeo{pool, g.n[0], g.n[1], g.n[2]});
-      pool += (g.size() + 63) / 64 * 64;
-    }
-    Lv.pool = pool;
-    for (double **p : {&Lv.u, &Lv.ua, &Lv.rho, &Lv.rhs, &Lv.res, &Lv.lop}) {
-      if (hipMalloc(p, pool * sizeof(double)) != hipSuccess) { pmg_amr_destroy(A); return nullptr; }
-      (void)hipMemset(*p, 0, pool * sizeof(double));
-    }
-    if (op == 1)
-      for (double **p : {&Lv.a, &Lv.c}) {
-        if (hipMalloc(p, pool * sizeof(double)) != hipSuccess) { pmg_amr_destroy(A); return nullptr; }
-        (void)hipMemset(*p, 0, pool * sizeof(double));
-      }
-    if (hipMalloc(&Lv.d_geo, sizeof(GridGeo) * geo.size()) != hipSuccess) { pmg_amr_destroy(A); return nullptr; }
-    (void)hipMemcpy(Lv.d_geo, geo.data(), sizeof(GridGeo) * geo.size(), hipMemcpyHostToDevice);
-    // child mask (set_cmask_child) as bytes, fixed for the hierarchy
-    std::vector<uint8_t> child(pool, 0), m;
-    for (int gi = 0; gi < (int)Lv.g.size(); gi++) {
-      cmask_bytes(A, L, gi, 1, m);
-      std::copy(m.begin(), m.end(), child.begin() + Lv.g[gi].off);
-    }
-    if (hipMalloc(&Lv.child, pool) != hipSuccess) { pmg_amr_destroy(A); return nullptr; }
-    (void)hipMemcpy(Lv.child, child.data(), pool, hipMemcpyHostToDevice);
-    // sweep tiles: single-grid levels use the wide tile, patch levels 32 x 16
-    const HGrid &g0 = Lv.g[0];
-    if (Lv.g.size() == 1) {
-      Lv.variant = 1;
-      Lv.zchunk = g0.n[2] >= 257 ? 32 : g0.n[2] >= 129 ? 16 : g0.n[2] >= 65 ? 8 : 4;
-    } else {
-      Lv.variant = 2;
-      Lv.zchunk = 8;
-    }
-    if (build_tiles(A, Lv)) { pmg_amr_destroy(A); return nullptr; }
-    cap = std::max(cap, std::max(Lv.nstiles, Lv.nptiles) + 64);
-  }
-  // transfer plans
-  for (int L = 1; L <= A->nlev; L++) {
-    ALevel &Lv = A->lev[L - 1];
-    std::vector<XJob> j0, j1, ji, jp;
-    int rc = make_jobs(A, L, L, 1, 0, j0);
-    if (!rc) rc = make_jobs(A, L, L, 2, 0, j1);
-    if (!rc && L < A->nlev) rc = make_jobs(A, L + 1, L, 1, 0, ji);
-    if (!rc && L > 1) rc = make_jobs(A, L - 1, L, 0, 1, jp);
-    if (!rc) rc = upload_plan(A, Lv.sync0, j0);
-    if (!rc) rc = upload_plan(A, Lv.sync1, j1);
-    if (!rc) rc = upload_plan(A, Lv.inject, ji);
-    if (!rc) rc = upload_plan(A, Lv.interp, jp);
-    if (!rc && L > 1) {  // does the (resolved) interp plan write every point of every grid of L exactly once?
-      std::vector<std::vector<uint8_t>> cnt(Lv.g.size());
-      for (size_t gi = 0; gi < Lv.g.size(); gi++) cnt[gi].assign(Lv.g[gi].size(), 0);
-      bool once = true;
-      for (const XJob &J : Lv.interp.jobs)
-        for (int k = J.lo[2]; k <= J.hi[2] && once; k++)
-          for (int j = J.lo[1]; j <= J.hi[1] && once; j++)
-            for (int i = J.lo[0]; i <= J.hi[0]; i++) {
-              const HGrid &gg = Lv.g[J.dg];
-              uint8_t &c = cnt[J.dg][((size_t)k * gg.n[1] + j) * gg.n[0] + i];
-              if (c) { once = false; break; }
-              c = 1;
-            }
-      for (size_t gi = 0; gi < cnt.size() && once; gi++)
-        for (uint8_t c : cnt[gi])
-          if (!c) { once = false; break; }
-      Lv.interp_once = once;
-    }
-    if (rc) { std::fprintf(stderr, "pamr_amr: transfer plan of level %d failed (%d)\n", L, rc); pmg_amr_destroy(A); return nullptr; }
-  }
-  A->ws.cap = cap;
-  A->ws.nslots = kAmrSlots;
-  if (hipMalloc(&A->ws.partials, sizeof(double) * 2 * (size_t)cap * kAmrSlots) != hipSuccess ||
-      hipMalloc(&A->ws.slot_nb, sizeof(int) * kAmrSlots) != hipSuccess ||
-      hipMalloc(&A->d_norms, sizeof(double) * kAmrSlots) != hipSuccess ||
-      hipMalloc(&A->d_cnorm, sizeof(double) * 2) != hipSuccess || hipMalloc(&A->d_citer, sizeof(int) * 2) != hipSuccess ||
-      hipHostMalloc(&A->h_pinned, sizeof(double) * (kAmrSlots + 4)) != hipSuccess) {
-    pmg_amr_destroy(A);
-    return nullptr;
-  }
-  (void)hipMemset(A->ws.partials, 0, sizeof(double) * 2 * (size_t)cap * kAmrSlots);
-  (void)hipMemset(A->d_norms, 0, sizeof(double) * kAmrSlots);
-  (void)hipMemset(A->d_cnorm, 0, sizeof(double) * 2);
-  (void)hipMemset(A->d_citer, 0, sizeof(int) * 2);
-  if (build_folds(A)) { pmg_amr_destroy(A); return nullptr; }
-  if (hipDeviceSynchronize() != hipSuccess) { pmg_amr_destroy(A); return nullptr; }
+  if (setup_levels(A, op)) { pmg_amr_destroy(A); return nullptr; }
   return A;
 }
 
@@ -1377,24 +1491,92 @@ void pmg_amr_destroy(pmg_amr *A) {
   if (A->stream) (void)hipStreamSynchronize(A->stream);
   drop_graphs(A);
   for (ALevel &Lv : A->lev) {
-    for (double **p : {&Lv.u, &Lv.ua, &Lv.rho, &Lv.rhs, &Lv.res, &Lv.lop, &Lv.a, &Lv.c})
-      if (*p) (void)hipFree(*p);
-    for (double *p : Lv.extra) if (p) (void)hipFree(p);
+    if (!Lv.shared) {
+      for (double **p : {&Lv.u, &Lv.ua, &Lv.rho, &Lv.rhs, &Lv.res, &Lv.lop, &Lv.a, &Lv.c})
+        if (*p) (void)hipFree(*p);
+      for (double *p : Lv.extra) if (p) (void)hipFree(p);
+    }
     for (double *p : Lv.cm) if (p) (void)hipFree(p);
     if (Lv.child) (void)hipFree(Lv.child);
     if (Lv.d_geo) (void)hipFree(Lv.d_geo);
     for (BTile *p : {Lv.d_stiles, Lv.d_ptiles}) if (p) (void)hipFree(p);
     for (int *p : {Lv.d_sstart, Lv.d_pstart}) if (p) (void)hipFree(p);
-    for (XPlan *P : {&Lv.sync0, &Lv.sync1, &Lv.inject, &Lv.interp}) free_plan(*P);
+    for (XPlan *P : {&Lv.sync0, &Lv.sync1, &Lv.sync1c, &Lv.inject, &Lv.interp}) free_plan(*P);
   }
+  if (A->xtmp) (void)hipFree(A->xtmp);
   if (A->ws.partials) (void)hipFree(A->ws.partials);
   if (A->ws.slot_nb) (void)hipFree(A->ws.slot_nb);
   for (double *p : {A->d_norms, A->d_cnorm}) if (p) (void)hipFree(p);
   if (A->d_citer) (void)hipFree(A->d_citer);
   if (A->d_fold) (void)hipFree(A->d_fold);
   if (A->h_pinned) (void)hipHostFree(A->h_pinned);
-  if (A->stream) (void)hipStreamDestroy(A->stream);
+  if (A->stream && A->own_stream) (void)hipStreamDestroy(A->stream);
   delete A;
+}
+
+pmg_amr *pmg_amr_create_mgh(pmg_amr *P, int amr_lev) {
+  if (!P || P->parent || amr_lev < 1 || amr_lev > (int)P->sgh.size()) return nullptr;
+  pmg_amr *A = new pmg_amr();
+  std::memcpy(A->cbbox, P->cbbox, sizeof A->cbbox);
+  std::memcpy(A->periodic, P->periodic, sizeof A->periodic);
+  A->op = P->op;
+  A->nextra = P->nextra;
+  A->pre = P->pre; A->pst = P->pst; A->max_citer = P->max_citer;
+  A->rop_u = P->rop_u; A->rop_res = P->rop_res; A->rop_src = P->rop_src; A->iord = P->iord;
+  A->crtol = P->crtol; A->w0_r = P->w0_r; A->w0_i = P->w0_i;
+  A->mcw = P->mcw;
+  std::memcpy(A->dx0, P->dx0, sizeof A->dx0);
+  A->parent = P;
+  A->stream = P->stream;  // one stream for both hierarchies: transfers and hooks stay ordered
+  A->own_stream = false;
+  const std::vector<double> &bb = P->sgh[amr_lev - 1];
+  const int num = (int)bb.size() / 6;
+  double dx[3];
+  for (int d = 0; d < 3; d++) dx[d] = P->dx0[d] / (double)(1 << (amr_lev - 1));
+  const int Ls = P->nbase + amr_lev - 1;  // the parent's level holding AMR level amr_lev
+  const int nsh = P->nlev - Ls + 1;
+  // build_mgh below min_lev (src/gh.c:1281-1340): islands of the level's boxes
+  std::vector<std::vector<HGrid>> coarse;  // finest first
+  for (int cf = 2; (int)coarse.size() + nsh < kMaxLev; cf *= 2) {
+    const std::vector<int> isl = island_numbers(bb.data(), num, dx, cf, P->mcw);
+    std::vector<double> keep;
+    std::vector<char> kc;
+    for (int k = 0; k < num; k++)
+      if (isl[k] != 0) {
+        keep.insert(keep.end(), bb.begin() + 6 * k, bb.begin() + 6 * k + 6);
+        kc.push_back(isl[k] < 0);
+      }
+    if (kc.empty()) break;
+    const double ldx[3] = {dx[0] * cf, dx[1] * cf, dx[2] * cf};
+    std::vector<int> perm;
+    std::vector<HGrid> g = level_grids_perm((int)kc.size(), keep.data(), ldx, perm);
+    for (size_t k = 0; k < g.size(); k++) g[k].coarsest = kc[perm[k]] != 0;  // fill_mgh_level (src/gh.c:1192)
+    coarse.push_back(std::move(g));
+  }
+  const int nc = (int)coarse.size();
+  A->nbase = nc + 1;
+  A->nlev = nc + nsh;
+  A->lev.resize(A->nlev);
+  for (int L = 1; L <= nc; L++) A->lev[L - 1].g = coarse[nc - L];
+  for (int j = 0; j < nsh; j++) {  // amr_to_mgh (src/gh.c:1097-1165)
+    ALevel &Lv = A->lev[nc + j];
+    Lv.g = P->lev[Ls - 1 + j].g;
+    for (HGrid &g : Lv.g) g.coarsest = false;
+    Lv.in_amrh = true;
+    Lv.shared = true;
+    Lv.plev = Ls + j;
+  }
+  for (HGrid &g : A->lev[0].g) g.coarsest = true;  // src/gh.c:1352-1354
+  if (setup_levels(A, A->op)) {
+    pmg_amr_destroy(A);
+    return nullptr;
+  }
+  return A;
+}
+
+int pmg_amr_grid_coarsest(pmg_amr *A, int L, int g) {
+  if (!A || L < 1 || L > A->nlev || g < 0 || g >= (int)A->lev[L - 1].g.size()) return -1;
+  return A->lev[L - 1].g[g].coarsest ? 1 : 0;
 }
 
 int pmg_amr_nlev(pmg_amr *A) { return A ? A->nlev : -1; }
@@ -1498,6 +1680,10 @@ static int read_norms(pmg_amr *A, double *res_out, double *lev_norms, int *citer
 
 int pmg_amr_vcycle(pmg_amr *A, double *res_out, double *lev_norms, int *citer) {
   if (!A) return -1;
+  if (A->parent) {  // the built-in cycle needs one coarsest grid; MG hierarchies from a refined level run the hooks
+    std::fprintf(stderr, "pamr_amr: the built-in V-cycle runs on hierarchies from level 1 only\n");
+    return -1;
+  }
   int rc = capture_or_run(A, &A->graph, enqueue_vcycle);
   if (rc) return rc;
   return read_norms(A, res_out, lev_norms, citer);
@@ -1505,6 +1691,10 @@ int pmg_amr_vcycle(pmg_amr *A, double *res_out, double *lev_norms, int *citer) {
 
 int pmg_amr_solve(pmg_amr *A, int min_iter, int max_iter, double tol, double *norms, int *iters) {
   if (!A) return -1;
+  if (A->parent) {  // the built-in cycle needs one coarsest grid; MG hierarchies from a refined level run the hooks
+    std::fprintf(stderr, "pamr_amr: the built-in V-cycle runs on hierarchies from level 1 only\n");
+    return -1;
+  }
   int it = 0;
   double res = tol + 1;
   while ((it < min_iter || res > tol) && it < max_iter) {
@@ -1519,6 +1709,10 @@ int pmg_amr_solve(pmg_amr *A, int min_iter, int max_iter, double tol, double *no
 
 int pmg_amr_residual(pmg_amr *A, double *norm_out) {
   if (!A) return -1;
+  if (A->parent) {  // the built-in cycle needs one coarsest grid; MG hierarchies from a refined level run the hooks
+    std::fprintf(stderr, "pamr_amr: the built-in V-cycle runs on hierarchies from level 1 only\n");
+    return -1;
+  }
   int rc = capture_or_run(A, &A->graph_res, enqueue_residual);
   if (rc) return rc;
   return read_norms(A, norm_out, nullptr, nullptr);
@@ -1635,6 +1829,16 @@ int pmg_amr_sync_fields(pmg_amr *A, int L, int bdy, int nf, const int *which) {
   return run_grouped(A, bdy ? Lv.sync1 : Lv.sync0, L, L, 0, nf, which);
 }
 
+int pmg_amr_sync_fields_coarsest(pmg_amr *A, int L, int nf, const int *which) {
+  if (!A || L < 1 || L > A->nlev || nf < 0 || (nf > 0 && !which)) return -1;
+  for (int f = 0; f < nf; f++) if (!valid_field(A, which[f])) return -1;
+  ALevel &Lv = A->lev[L - 1];
+  int nco = 0;
+  for (const HGrid &g : Lv.g) nco += g.coarsest ? 1 : 0;
+  if (nco == 0) return 0;
+  return run_grouped(A, nco == (int)Lv.g.size() ? Lv.sync1 : Lv.sync1c, L, L, 0, nf, which);
+}
+
 int pmg_amr_inject_fields(pmg_amr *A, int Lf, int nf, const int *which, const int *ops) {
   if (!A || Lf < 2 || Lf > A->nlev || nf < 0 || (nf > 0 && (!which || !ops))) return -1;
   for (int f = 0; f < nf; f++) if (!valid_field(A, which[f]) || ops[f] < 1 || ops[f] > 3) return -1;
@@ -1705,6 +1909,92 @@ int pmg_amr_fill_cmask(pmg_amr *A, int L, int which, int field) {
 
 int pmg_amr_interp_fields(pmg_amr *A, int Lc, int nf, const int *which) {
   return pmg_amr_interp_fields_ord(A, Lc, 2, nf, which);
+}
+
+// PAMR_inject with excision: local_transfer's coarsen branch (src/transfer.c:
+// 1126-1149) one transfer at a time in list order, each the device
+// m_util_r8 routine over the transfer's index boxes with the fine grid's
+// mask; fields inner, as the reference loops over gfns per transfer
+int pmg_amr_inject_fields_ex(pmg_amr *A, int Lf, int nf, const int *which, const int *ops, int mask, double ex) {
+  if (!A || Lf < 2 || Lf > A->nlev || nf < 0 || (nf > 0 && (!which || !ops)) || !valid_field(A, mask)) return -1;
+  for (int f = 0; f < nf; f++) if (!valid_field(A, which[f]) || ops[f] < 1 || ops[f] > 3) return -1;
+  ALevel &S = A->lev[Lf - 1], &D = A->lev[Lf - 2];
+  const double *chr = field_pool(S, mask);
+  for (const XJob &J : D.inject.raw) {
+    const HGrid &gs = S.g[J.sg], &gd = D.g[J.dg];
+    int i1[3], i2[3], cnt[3];
+    for (int d = 0; d < 3; d++) {
+      i2[d] = J.lo[d] + 1;
+      cnt[d] = J.hi[d] - J.lo[d] + 1;
+      i1[d] = J.sh[d] + 2 * J.lo[d] + 1;
+    }
+    for (int f = 0; f < nf; f++) {
+      const double *src = field_pool(S, which[f]) + gs.off;
+      double *dst = field_pool(D, which[f]) + gd.off;
+      int rc;
+      if (ops[f] == 1)
+        rc = pmg_dmcopy3d(src, dst, gs.n[0], gs.n[1], gs.n[2], gd.n[0], gd.n[1], gd.n[2], i1[0], i1[1], i1[2], i2[0],
+                          i2[1], i2[2], cnt[0], cnt[1], cnt[2], 2, 2, 2, 1, 1, 1, A->stream);
+      else
+        rc = (ops[f] == 2 ? pmg_dmhwr3d : pmg_dmfwr3d)(src, dst, gs.n[0], gs.n[1], gs.n[2], gd.n[0], gd.n[1], gd.n[2],
+                                                      i1[0], i1[1], i1[2], i2[0], i2[1], i2[2], cnt[0], cnt[1],
+                                                      cnt[2], 2, 2, 2, 1, 1, 1, chr + gs.off, ex, 1, A->stream);
+      if (rc) return rc;
+    }
+  }
+  return 0;
+}
+
+// PAMR_interp with excision: local_transfer's interpolation branch
+// (src/transfer.c:1089-1122) one transfer at a time in list order: the
+// buffered temporary interpolated with the source grid's mask, its part
+// copied into the destination, then clear_ex of the whole source grid and
+// the whole destination grid (later transfers from the same source read the
+// cleared values, as in the reference)
+int pmg_amr_interp_fields_ex(pmg_amr *A, int Lc, int ord, int nf, const int *which, int mask, double ex) {
+  if (!A || Lc < 1 || Lc >= A->nlev || nf < 0 || (nf > 0 && !which) || !valid_field(A, mask)) return -1;
+  if (ord != 2 && ord != 4) return -2;
+  for (int f = 0; f < nf; f++) if (!valid_field(A, which[f])) return -1;
+  ALevel &S = A->lev[Lc - 1], &D = A->lev[Lc];
+  const double *chrs = field_pool(S, mask), *chrd = field_pool(D, mask);
+  for (const XJob &J : D.interp.raw) {
+    const HGrid &gs = S.g[J.sg], &gd = D.g[J.dg];
+    int i1i[3], nii[3], nb[3], i2b[3], i2[3], cnt[3];
+    size_t tsize = 1;
+    for (int d = 0; d < 3; d++) {
+      i1i[d] = J.t0[d] + 1;
+      nii[d] = J.tn[d];
+      nb[d] = (nii[d] - 1) * 2 + 1;
+      i2[d] = J.lo[d] + 1;
+      cnt[d] = J.hi[d] - J.lo[d] + 1;
+      i2b[d] = J.sh[d] + J.lo[d] - 2 * J.t0[d] + 1;
+      tsize *= (size_t)nb[d];
+    }
+    if (tsize > A->xtmp_cap) {
+      PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
+      if (A->xtmp) PMG_HOST_CHECK(hipFree(A->xtmp));
+      A->xtmp = nullptr;
+      A->xtmp_cap = 0;
+      PMG_HOST_CHECK(hipMalloc(&A->xtmp, tsize * sizeof(double)));
+      A->xtmp_cap = tsize;
+    }
+    for (int f = 0; f < nf; f++) {
+      double *src = field_pool(S, which[f]) + gs.off, *dst = field_pool(D, which[f]) + gd.off;
+      int rc = pmg_dminterp3d(src, A->xtmp, gs.n[0], gs.n[1], gs.n[2], nb[0], nb[1], nb[2], i1i[0], i1i[1], i1i[2],
+                              1, 1, 1, nii[0], nii[1], nii[2], 1, 1, 1, 2, 2, 2, ord, chrs + gs.off, ex, 1, A->stream);
+      if (!rc)
+        rc = pmg_dmcopy3d(A->xtmp, dst, nb[0], nb[1], nb[2], gd.n[0], gd.n[1], gd.n[2], i2b[0], i2b[1], i2b[2], i2[0],
+                          i2[1], i2[2], cnt[0], cnt[1], cnt[2], 1, 1, 1, 1, 1, 1, A->stream);
+      if (rc) return rc;
+      const size_t ns = gs.size(), nd = gd.size();
+      hipLaunchKernelGGL(k_clear_ex, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, A->stream, src,
+                         chrs + gs.off, ex, ns);
+      hipLaunchKernelGGL(k_clear_ex, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, A->stream, dst,
+                         chrd + gd.off, ex, nd);
+      PMG_HOST_CHECK(hipGetLastError());
+    }
+  }
+  return 0;
 }
 
 int pmg_amr_interp_fields_ord(pmg_amr *A, int Lc, int ord, int nf, const int *which) {

@@ -21,6 +21,9 @@ PAMR_AMRH, PAMR_MGH = 1, 2
 MAX_GFNS = 512
 
 _app = None
+# void (*app_fill_ex_mask_fnc)(real *mask, int dim, int *shape, real *bbox, real excised)
+FILL_EX_MASK = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double),
+                           C.c_double)
 
 
 def nps_lib():
@@ -34,6 +37,7 @@ def nps_lib():
         i, vp, dp = C.c_int, C.c_void_p, C.POINTER(C.c_double)
         A.nps_define_vars.argtypes = [i, i, i, i]
         A.nps_define_vars_tl.argtypes = [i, i, i, i, i]
+        A.nps_define_vars_ex.argtypes = [i, i, i, i, C.c_double]
         A.nps_init.argtypes = []
         A.nps_set_level_hooks.argtypes = [i]
         A.nps_set_level_hooks.restype = None
@@ -50,7 +54,12 @@ class NpsContext:
     refined levels' boxes (``levels[l]``: boxes of AMR level l + 2)."""
 
     def __init__(self, base_n, levels, op=0, rop_u=3, rop_res=3, rop_src=3, mcw=5,
-                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tre_vars=None, num_tl=1, extrap=False, extra_defs=None):
+                 bbox=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), tre_vars=None, num_tl=1, extrap=False, extra_defs=None,
+                 excision=None):
+        """excision: (ex, fill) -- AMRD's chr variable is defined, the hooks
+        skip chr == ex, and PAMR_excision_on("chr", ...) is called before the
+        hierarchy is composed with a fill function that writes
+        fill((nz, ny, nx), bbox) (numpy, ex = excised) into each grid's mask."""
         self.L, self.app = lib(), nps_lib()
         L = self.L
         shape = (C.c_int * 3)(base_n, base_n, base_n)
@@ -60,7 +69,21 @@ class NpsContext:
         self.open = True
         w = (C.c_int * 3)(mcw, mcw, mcw)
         L.PAMR_set_MG_coarse_width(w)
-        if not self.app.nps_define_vars_tl(op, rop_u, rop_res, rop_src, num_tl):
+        if excision is not None:
+            ex, fill = excision
+            if num_tl != 1 or not self.app.nps_define_vars_ex(op, rop_u, rop_res, rop_src, ex):
+                raise RuntimeError("nps_define_vars_ex failed")
+
+            def _fill(mask, dim, shape, bb, excised):
+                shp = (shape[2], shape[1], shape[0])
+                box = np.array([bb[k] for k in range(6)])
+                vals = np.ascontiguousarray(fill(shp, box), dtype=np.float64)
+                C.memmove(mask, vals.ctypes.data, vals.nbytes)
+
+            self._fill_cb = FILL_EX_MASK(_fill)  # kept alive while the context is
+            if not L.PAMR_excision_on(b"chr", C.cast(self._fill_cb, C.c_void_p), ex, 0):
+                raise RuntimeError("PAMR_excision_on failed")
+        elif not self.app.nps_define_vars_tl(op, rop_u, rop_res, rop_src, num_tl):
             raise RuntimeError("nps_define_vars failed")
         if extrap and not L.amrd_mg_def_extrap_vars(b"", b""):
             raise RuntimeError("amrd_mg_def_extrap_vars failed")

@@ -534,3 +534,112 @@ def test_evo_iterate_matches_oracle(torch_cuda):
     assert np.array_equal(ctx.download(PAMR_AMRH, 1, 0, "V"), m.field(m.base, 0, O.OA_U))
     assert np.array_equal(ctx.download(PAMR_AMRH, 1, 0, "phi", tl=1), p1)
     ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# excision through the PAMR surface (PAMR_excision_on, src/pamr.c:1983-2021)
+# ---------------------------------------------------------------------------
+EX = 1.0
+
+
+def sphere_mask(spheres):
+    """fill((nz, ny, nx), bbox): EX inside any sphere (cx, cy, cz, r), EX + 1
+    elsewhere, on the grid's vertex coordinates"""
+    def fill(shape, bb):
+        nz, ny, nx = shape
+        x = bb[0] + np.arange(nx) * ((bb[1] - bb[0]) / (nx - 1))
+        y = bb[2] + np.arange(ny) * ((bb[3] - bb[2]) / (ny - 1))
+        z = bb[4] + np.arange(nz) * ((bb[5] - bb[4]) / (nz - 1))
+        inside = np.zeros(shape, dtype=bool)
+        for cx, cy, cz, r in spheres:
+            inside |= ((x[None, None, :] - cx) ** 2 + (y[None, :, None] - cy) ** 2
+                       + (z[:, None, None] - cz) ** 2) <= r * r
+        return np.where(inside, EX, EX + 1.0)
+    return fill
+
+
+SPHERES = [(0.37, 0.52, 0.46, 0.17), (0.71, 0.3, 0.62, 0.12)]
+
+
+@pytest.mark.parametrize("n", [33, 65])
+def test_excised_unigrid_matches_oracle(torch_cuda, n):
+    """PAMR_excision_on with AMRD's chr on one 33^3 / 65^3 grid: the drop-in
+    driver's V-cycles (excised transfers, hooks skipping chr == ex) equal
+    the oracle's excised uniform solve bit for bit; solve_elliptics by its
+    reference name leaves the same solution"""
+    print(f"case excised unigrid n={n}", flush=True)
+    fill = sphere_mask(SPHERES)
+    rho = -pm.gaussian_source(n, seed=91)
+    m = O.UniformMG(n, do_ex=1, ex=EX)
+    m.set_params(pre=2, pst=2)
+    m.field(m.nlev, O.F_RHO)[:] = rho
+    for L in range(1, m.nlev + 1):
+        m.field(L, O.F_CHR)[:] = fill(m.shape(L), np.array([0.0, 1.0, 0.0, 1.0, 0.0, 1.0]))
+    m.init()
+    ctx = NpsContext(n, [], excision=(EX, fill))
+    ctx.upload(PAMR_AMRH, 1, 0, "rho", rho)
+    ctx.set_params(pre=2, pst=2)
+    nl = ctx.build_mgh()
+    assert nl == m.nlev
+    for L in range(1, nl + 1):
+        assert np.array_equal(ctx.download(PAMR_MGH, L, 0, "chr"), m.field(L, O.F_CHR)), L
+        assert np.array_equal(ctx.download(PAMR_MGH, L, 0, "rho"), m.field(L, O.F_RHO)), L
+    for c in range(4):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = ctx.vcycle(c, reference_name=True)
+        print(f"cycle {c}: oracle {rm:.15e} drop-in {ra:.15e}", flush=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert ca == cm
+        assert np.allclose(na, nm, rtol=1e-12, atol=0)
+    for L in range(1, nl + 1):
+        assert np.array_equal(ctx.download(PAMR_MGH, L, 0, "V"), m.field(L, O.F_U)), L
+    ctx.L.PAMR_destroy_mgh()
+    # a whole solve through solve_elliptics(1, 1, 0): 3 more cycles from the same u
+    ctx.set_params(pre=2, pst=2, min_iter=3, max_iter=3, tol=0.0)
+    ctx.L.solve_elliptics(1, 1, 0)
+    for _ in range(3):
+        m.vcycle()
+    assert np.array_equal(ctx.download(PAMR_AMRH, 1, 0, "V"), m.field(m.nlev, O.F_U))
+    ctx.close()
+
+
+@pytest.mark.parametrize("seed", [3, 8])
+def test_excised_amr_matches_oracle(torch_cuda, seed):
+    """excision on a refined hierarchy (17^3 base, two levels of patches):
+    every transfer runs local_transfer's do_ex path one transfer at a time;
+    masks, V-cycle norms and u on every grid equal the oracle's"""
+    levels = layout(seed)
+    print(f"case excised amr seed={seed} boxes={[len(l) for l in levels]}", flush=True)
+    fill = sphere_mask(SPHERES)
+    m = O.AmrMG(17, levels)
+    m.set_excision(EX)
+    for Lm in range(1, m.nlev + 1):
+        for g in range(m.ngrids(Lm)):
+            shape, bb = m.grid(Lm, g)
+            m.field(Lm, g, O.OA_CHR)[:] = fill(shape, bb)
+    ctx = NpsContext(17, levels, excision=(EX, fill))
+    load_sources(m, ctx, 0, 300 + seed)
+    m.set_params()
+    m.init()
+    ctx.set_params(pre=2, pst=2)
+    nl = ctx.build_mgh()
+    assert nl == m.nlev
+    nex = 0
+    for Lm in range(1, nl + 1):
+        for g in range(m.ngrids(Lm)):
+            ch = ctx.download(PAMR_MGH, Lm, g, "chr")
+            assert np.array_equal(ch, m.field(Lm, g, O.OA_CHR)), ("chr", Lm, g)
+            nex += int((ch == EX).sum())
+            assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, "rho"), m.field(Lm, g, O.OA_RHO)), ("rho", Lm, g)
+    assert nex > 0
+    for c in range(3):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = ctx.vcycle(c)
+        print(f"cycle {c}: oracle {rm:.15e} drop-in {ra:.15e}", flush=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert ca == cm
+        assert np.allclose(na, nm, rtol=1e-12, atol=0)
+    for Lm in range(1, nl + 1):
+        for g in range(m.ngrids(Lm)):
+            assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, "V"), m.field(Lm, g, O.OA_U)), ("u", Lm, g)
+    ctx.close()
