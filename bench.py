@@ -348,7 +348,8 @@ def probe_kernels(S, n, pm, torch):
     """Launch each finest-level kernel of interest 4 times, nothing else
     large: the counter runs attribute FETCH_SIZE / WRITE_SIZE per dispatch."""
     Lf = S.nlev
-    for name, which, prec in (("sweep f64", 0, 0), ("residual f64", 1, 0), ("sweep f32", 0, 1)):
+    for name, which, prec in (("sweep f64", 0, 0), ("residual f64", 1, 0), ("sweep f32", 0, 1),
+                              ("resid_restrict f64", 2, 0), ("resid_restrict f32", 2, 1)):
         print(f"probe {name}: {S.time_kernel(which, Lf, prec, 4):.5f} ms", flush=True)
     # calibration: pmg_dmcopy3d over the whole grid = plain copy, 8 B read + 8 B written per point
     src = S.field_ptr(Lf, pm.PMG_F_RHO)

@@ -1969,9 +1969,16 @@ int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double 
   const bool fine = (L == S->nlev);
   int rc = 0;
   PMG_HOST_CHECK(hipEventRecord(e0, S->stream));
+  if (which == 2) {
+    // the finest residual fused with the full-weight injection into L-1
+    if (!fine || !(prec ? can_fuse_resid_restrict<float>(S) : can_fuse_resid_restrict<double>(S))) return -1;
+  }
   if (which == 0) {
     rc = prec ? sweeps<float>(S, L, nrep, NS_F32 + NS_RELAX + L - 1, fine)
               : sweeps<double>(S, L, nrep, NS_RELAX + L - 1, fine);
+  } else if (which == 2) {
+    for (int r = 0; r < nrep && !rc; r++)
+      rc = prec ? resid_restrict<float>(S, NS_F32 + NS_RESID + L - 1) : resid_restrict<double>(S, NS_RESID + L - 1);
   } else {
     for (int r = 0; r < nrep && !rc; r++)
       rc = prec ? residual<float>(S, L, NS_F32 + NS_RESID + L - 1, fine, fine)

@@ -150,13 +150,19 @@ def _solver_only(pm, n, rho, pre, pst):
     return S
 
 
-@pytest.mark.parametrize("n", [33, 65, 129])
-def test_restrict_fusion_matches_unfused(pm, n):
+@pytest.mark.parametrize("n,op", [(33, 0), (65, 0), (129, 0), (65, 1), (97, 1)])
+def test_restrict_fusion_matches_unfused(pm, n, op):
     """the finest residual computed inside the injection gives the same
     coarse u and res, the same cycles and the same u as residual + dmfwr3d"""
+    print(f"case restrict fusion n={n} op={op}", flush=True)
     rho = -pm.gaussian_source(n, seed=6)
-    Sf, _ = _pair(pm, n, rho)
-    Su, _ = _pair(pm, n, rho)
+    a = c = None
+    if op == 1:
+        rng = np.random.default_rng(n)
+        a = np.exp(0.5 * rng.standard_normal((n, n, n)))
+        c = rng.random((n, n, n))
+    Sf, _ = _pair(pm, n, rho, op=op, a=a, c=c)
+    Su, _ = _pair(pm, n, rho, op=op, a=a, c=c)
     Su.set_restrict_fusion(False)
     for _ in range(3):
         rf, nf, cf = Sf.vcycle()
@@ -182,12 +188,11 @@ def test_restrict_fusion_matches_unfused(pm, n):
     Su.close()
 
 
-@pytest.mark.parametrize("cz", [1, 3, 5, 16])
-def test_resid_restrict_zchunk_bit_exact(pm, cz):
+@pytest.mark.parametrize("cz,n", [(1, 65), (3, 65), (5, 65), (16, 65), (2, 257), (13, 257)])
+def test_resid_restrict_zchunk_bit_exact(pm, cz, n):
     """every z-chunk of the fused finest residual + injection gives the same
     cycles (FP64) and the same mixed-precision history (FP32) as the default"""
-    print(f"case resid_restrict zchunk {cz}", flush=True)
-    n = 65
+    print(f"case resid_restrict zchunk {cz} n={n}", flush=True)
     rho = -pm.gaussian_source(n, seed=11)
     Sd, _ = _pair(pm, n, rho)
     Sz, _ = _pair(pm, n, rho)
