@@ -17,8 +17,9 @@
  * every grid of the level, a sync, an inject, an interp -- is one kernel
  * launch.  A whole V-cycle is captured in one hipGraph.
  *
- * Scope: single MPI rank (ghost_width 0), refinement ratio 2, one base grid,
- * FP64; excision in the transfers of pmg_amr_*_fields_ex only.  Return codes: 0 ok, < 0 error (as pamr_mg_gpu.h).
+ * Scope: single MPI rank (ghost_width 0), refinement ratio 2, one base grid;
+ * FP64, with FP32 inner cycles in pmg_amr_solve_rel mode 1; excision in the
+ * transfers of pmg_amr_*_fields_ex only.  Return codes: 0 ok, < 0 error (as pamr_mg_gpu.h).
  */
 #ifndef PAMR_AMR_GPU_H
 #define PAMR_AMR_GPU_H
@@ -78,6 +79,26 @@ int pmg_amr_init(pmg_amr *A);
 int pmg_amr_vcycle(pmg_amr *A, double *res_out, double *lev_norms, int *citer);
 /* solve_elliptics loop (amrd/mg.c:56-72) */
 int pmg_amr_solve(pmg_amr *A, int min_iter, int max_iter, double tol, double *norms, int *iters);
+/* relative solve ||r_k|| <= rtol ||r_0|| over the whole hierarchy (r: the
+ * finest level's residual, max over its grids; no counterpart in the
+ * reference, whose solve_elliptics loop has an absolute tolerance,
+ * amrd/mg.c:56-72).  mode 0: FP64 V-cycles.  mode 1: mixed precision --
+ * FP32 V-cycles of the same hierarchy on A e = -r from e = 0 (the FP64
+ * defect r = L[u] of every AMR level after u is injected down the levels as
+ * the FAS cycle does, written negated as the FP32 source), u += e in FP64 on
+ * every AMR level, the FP64 defect as the test; while ||r|| > frac ||r0||
+ * (pmg_amr_set_mixed_switch, default 1e-2) FP64 V-cycles are run instead.
+ * hist[k] = ||r_{k+1}||; r0_out as pmg_solve_rel (a value > 0 on entry
+ * replaces ||r_0||). */
+int pmg_amr_solve_rel(pmg_amr *A, int mode, double rtol, int max_iter, double *hist, int *iters, double *r0_out);
+/* pmg_amr_solve_rel mode 1: run FP64 V-cycles while the residual is above
+ * frac (0..1, default 1e-2; 1 = FP32 from the start) times the reference
+ * norm, FP32 inner cycles below.  A fine grid's AMR-boundary values are set
+ * only through coarse-grid corrections, which no later defect measures, so
+ * the FP32 rounding of the large early corrections stays in them; starting
+ * FP32 late bounds that by frac.  No counterpart in the reference (a
+ * precision policy of this implementation). */
+int pmg_amr_set_mixed_switch(pmg_amr *A, double frac);
 /* residual of the finest level (max over grids) */
 int pmg_amr_residual(pmg_amr *A, double *norm_out);
 /* single transfers of one field: PAMR_sync(L, AMR_bdy_width = bdy),

@@ -186,6 +186,11 @@ int pmg_set_graph(pmg_solver *S, int on);
  * on = 0 off, 1 the default, 2..17 the largest side fused (9: only the
  * levels of 9 points and below); results are identical either way */
 int pmg_set_bottom_fusion(pmg_solver *S, int on);
+/* keep every array of the fused bottom levels in LDS for the whole bottom V
+ * when they fit in one workgroup's LDS (on = 1, the default) or run it from
+ * global memory (0); results are identical either way.  No counterpart in
+ * the reference (a tuning switch of this implementation). */
+int pmg_set_bottom_lds(pmg_solver *S, int on);
 /* compute the finest-level residual (of the FP64 cycle and of the FP32 inner
  * cycle) inside the full-weight injection of u and f_res instead of storing
  * it (default on; no excision, full weighting for both); u, the norms and

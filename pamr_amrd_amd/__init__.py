@@ -83,6 +83,7 @@ def _declare(L):
     L.pmg_num_sweep_variants.argtypes = []
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_set_bottom_fusion.argtypes = [_vp, i]
+    L.pmg_set_bottom_lds.argtypes = [_vp, i]
     L.pmg_set_restrict_fusion.argtypes = [_vp, i]
     L.pmg_set_resid_restrict_zchunk.argtypes = [_vp, i, i]
     L.pmg_set_sweep_blocking.argtypes = [_vp, i, i]
@@ -127,6 +128,12 @@ def _declare(L):
     L.pmg_amr_set_params.argtypes = [_vp, i, i, i, d, d, d, i, i, i, i]
     L.pmg_amr_vcycle.argtypes = [_vp, _dp, _dp, _ip]
     L.pmg_amr_solve.argtypes = [_vp, i, i, d, _dp, _ip]
+    L.pmg_amr_solve_rel.argtypes = [_vp, i, d, i, _dp, _ip, _dp]
+    L.pmg_amr_set_mixed_switch.argtypes = [_vp, d]
+    L.pmg_amr_create_mgh.argtypes = [_vp, i]
+    L.pmg_amr_create_mgh.restype = _vp
+    L.pmg_amr_grid_coarsest.argtypes = [_vp, i, i]
+    L.pmg_amr_sync_fields_coarsest.argtypes = [_vp, i, i, _ip]
     L.pmg_amr_residual.argtypes = [_vp, _dp]
     L.pmg_amr_sync.argtypes = [_vp, i, i, i]
     L.pmg_amr_inject.argtypes = [_vp, i, i, i]
@@ -404,6 +411,10 @@ class Solver:
         v = (max_n if max_n is not None else 1) if on else 0
         _check(self._L.pmg_set_bottom_fusion(self.h, v), "pmg_set_bottom_fusion")
 
+    def set_bottom_lds(self, on):
+        """keep the fused bottom levels' arrays in LDS when they fit"""
+        _check(self._L.pmg_set_bottom_lds(self.h, 1 if on else 0), "pmg_set_bottom_lds")
+
     def set_restrict_fusion(self, on):
         _check(self._L.pmg_set_restrict_fusion(self.h, 1 if on else 0), "pmg_set_restrict_fusion")
 
@@ -497,6 +508,20 @@ class Amr:
         r = C.c_double(0)
         _check(self._L.pmg_amr_residual(self.h, C.byref(r)), "pmg_amr_residual")
         return r.value
+
+    def solve_rel(self, mode, rtol, max_iter, r_ref=0.0):
+        """(reference norm, residual history) of pmg_amr_solve_rel: mode 0
+        FP64 V-cycles, 1 FP32 V-cycles in FP64 defect correction"""
+        hist = np.zeros(max(max_iter, 1))
+        it = C.c_int(0)
+        r0 = C.c_double(r_ref)
+        _check(self._L.pmg_amr_solve_rel(self.h, mode, rtol, max_iter, P(hist), C.byref(it), C.byref(r0)),
+               "pmg_amr_solve_rel")
+        return r0.value, hist[: it.value]
+
+    def set_mixed_switch(self, frac):
+        """mode 1 runs FP64 V-cycles until ||r|| <= frac ||r0||, then FP32 inner cycles"""
+        _check(self._L.pmg_amr_set_mixed_switch(self.h, frac), "pmg_amr_set_mixed_switch")
 
     def sync(self, lev, bdy, which):
         _check(self._L.pmg_amr_sync(self.h, lev, bdy, which), "pmg_amr_sync")

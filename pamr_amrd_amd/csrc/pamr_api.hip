@@ -97,6 +97,13 @@ struct Ctx {
   double excised = 0.0;
   std::string ex_var;
   void (*ex_fill)(double *, int, int *, double *, double) = nullptr;
+  // the MG hierarchy's levels: A's when it is built from AMR level 1, else an
+  // island hierarchy over AMR levels mg_min .. finest (kept in subs for reuse)
+  pmg_amr *M = nullptr;
+  int mg_min = 1;
+  std::vector<std::pair<int, pmg_amr *>> subs;
+  std::vector<std::vector<View>> mviews;  // views of M's levels when M != A
+  int comm_coarsest = 0;  // MG level where only the coarsest grids communicate (0: none)
 };
 
 Ctx *C = nullptr;
@@ -153,19 +160,32 @@ int field_of_gfn(int gfn) {
   return v->mg_field;
 }
 
+// the device hierarchy behind a PAMR hierarchy
+pmg_amr *hobj(int hier) { return (hier == PAMR_MGH && C->M) ? C->M : C->A; }
+int mg_nlev() { return (C && C->mgh && C->M) ? pmg_amr_nlev(C->M) : 0; }
+// MG level holding AMR level mg_min
+int mg_nbase() { return (C && C->mgh && C->M) ? (C->M == C->A ? C->nbase : pmg_amr_base_level(C->M)) : 0; }
+
 // pmg level of level l of a hierarchy (0: none)
 int plevel(int l, int hier) {
   if (!C || !C->A) return 0;
   if (hier == PAMR_AMRH) return (l >= 1 && l <= C->namr + 1) ? C->nbase + l - 1 : 0;
-  if (hier == PAMR_MGH) return (C->mgh && l >= 1 && l <= C->nlev) ? l : 0;
+  if (hier == PAMR_MGH) return (C->mgh && l >= 1 && l <= mg_nlev()) ? l : 0;
   return 0;
 }
 
-void build_views() {
-  C->views.assign(C->nlev, {});
-  for (int L = 1; L <= C->nlev; L++) {
-    const int ng = pmg_amr_ngrids(C->A, L);
-    std::vector<View> &vs = C->views[L - 1];
+std::vector<std::vector<View>> &views_of(int hier) {
+  return (hier == PAMR_MGH && C->M && C->M != C->A) ? C->mviews : C->views;
+}
+
+// grid views of the levels of P: AMR gfns on levels >= Lamr0, MG gfns on
+// levels >= Lmg0 (amr_to_mgh shares the AMR grid's gfs[], src/gh.c:1128-1146)
+void fill_views(pmg_amr *P, int Lamr0, int Lmg0, std::vector<std::vector<View>> &views) {
+  const int nl = pmg_amr_nlev(P);
+  views.assign(nl, {});
+  for (int L = 1; L <= nl; L++) {
+    const int ng = pmg_amr_ngrids(P, L);
+    std::vector<View> &vs = views[L - 1];
     vs.resize(ng);
     for (int g = 0; g < ng; g++) {
       View &w = vs[g];
@@ -174,17 +194,14 @@ void build_views() {
         int tl = 0;
         Var *v = var_of_gfn(gfn, &tl);
         if (!v) continue;
-        const bool amr_level = L >= C->nbase;
-        bool present;
-        if (tl > 0) present = amr_level;
-        else present = C->mgh;
+        const bool present = tl > 0 ? L >= Lamr0 : L >= Lmg0;
         if (!present) continue;
         const int f = field_of_gfn(gfn);
-        if (f >= 0) w.gfs[gfn - 1] = pmg_amr_field(C->A, L, g, f);
+        if (f >= 0) w.gfs[gfn - 1] = pmg_amr_field(P, L, g, f);
       }
       int shp[3];
       double bb[6];
-      pmg_amr_grid_info(C->A, L, g, shp, bb);
+      pmg_amr_grid_info(P, L, g, shp, bb);
       for (int d = 0; d < 3; d++) {  // compose_cgh's accumulation (src/gh.c:902-906)
         w.x[d].resize(shp[d]);
         const double h = (bb[2 * d + 1] - bb[2 * d]) / (shp[d] - 1);
@@ -195,6 +212,13 @@ void build_views() {
   }
 }
 
+void build_views() {
+  const int big = 1 << 30;
+  fill_views(C->A, C->nbase, !C->mgh ? big : (C->M == C->A ? 1 : C->nbase + C->mg_min - 1), C->views);
+  if (C->mgh && C->M && C->M != C->A) fill_views(C->M, mg_nbase(), 1, C->mviews);
+  else C->mviews.clear();
+}
+
 View *cur_view(const char *who) {
   if (!need_ctx(who)) return nullptr;
   if (C->it.g < 0) {
@@ -202,11 +226,12 @@ View *cur_view(const char *who) {
     return nullptr;
   }
   const int Lp = plevel(C->it.L, C->it.hier);
-  if (!Lp || C->it.g >= (int)C->views[Lp - 1].size()) {
+  std::vector<std::vector<View>> &vw = views_of(C->it.hier);
+  if (!Lp || Lp > (int)vw.size() || C->it.g >= (int)vw[Lp - 1].size()) {
     std::printf("%s: error ... iterator not initialized\n", who);
     return nullptr;
   }
-  return &C->views[Lp - 1][C->it.g];
+  return &vw[Lp - 1][C->it.g];
 }
 
 // the pmg field of the excision mask (-1: excision off or no hierarchy)
@@ -216,20 +241,20 @@ int mask_field() {
   return (v && !v->amr_field.empty()) ? v->amr_field[0] : -1;
 }
 
-// init_ex_mask (src/gh.c:1555-1577) on pmg levels L0..L1: the application's
+// init_ex_mask (src/gh.c:1555-1577) on levels L0..L1 of P: the application's
 // fill function on a host array of each grid, uploaded into the mask field
-bool init_ex_mask(int L0, int L1) {
+bool init_ex_mask(pmg_amr *P, int L0, int L1) {
   const int f = mask_field();
   if (f < 0 || !C->ex_fill) return false;
   if (hipStreamSynchronize(C->stream) != hipSuccess) return false;
   for (int L = L0; L <= L1; L++)
-    for (int g = 0; g < pmg_amr_ngrids(C->A, L); g++) {
+    for (int g = 0; g < pmg_amr_ngrids(P, L); g++) {
       int shp[3];
       double bb[6];
-      pmg_amr_grid_info(C->A, L, g, shp, bb);
+      pmg_amr_grid_info(P, L, g, shp, bb);
       std::vector<double> h((size_t)shp[0] * shp[1] * shp[2], 0.0);
       C->ex_fill(h.data(), 3, shp, bb, C->excised);
-      if (hipMemcpy(pmg_amr_field(C->A, L, g, f), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) !=
+      if (hipMemcpy(pmg_amr_field(P, L, g, f), h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice) !=
           hipSuccess)
         return false;
     }
@@ -386,10 +411,12 @@ bool ell_gfns(std::vector<EllGfns> &out, int &cmask) {
   return cmask > 0;
 }
 
-double *pool(int L, int gfn, long long &n) {
-  n = pmg_amr_pool_points(C->A, L);
+// a level's field pool; L an MG level, or a pmg level of A for hier AMRH
+double *pool(int L, int gfn, long long &n, int hier = PAMR_MGH) {
+  pmg_amr *P = hobj(hier);
+  n = pmg_amr_pool_points(P, L);
   const int f = field_of_gfn(gfn);
-  return f < 0 ? nullptr : pmg_amr_field(C->A, L, 0, f);
+  return f < 0 ? nullptr : pmg_amr_field(P, L, 0, f);
 }
 
 int nblocks(long long n) { return (int)((n + kNT - 1) / kNT); }
@@ -467,7 +494,7 @@ bool compute_cgc(int L, const std::vector<EllGfns> &E) {
 // amrd/mg.c:362-426) on every grid of MG level L
 bool set_cmask(int L, int which, int cmask) {
   const int f = field_of_gfn(cmask);
-  return f >= 0 && pmg_amr_fill_cmask(C->A, L, which, f) == 0;
+  return f >= 0 && pmg_amr_fill_cmask(hobj(PAMR_MGH), L, which, f) == 0;
 }
 
 // call_app / call_rr_app (amrd/util.c:39-103) over the MG grids of L
@@ -506,9 +533,21 @@ void lop_level(int L) {
   else call_app(D.lop, L);
 }
 
+// is_coarsest (amrd/mg.c:553-565): a grid of MG level L is flagged coarsest
+bool is_coarsest(int L) {
+  for (int g = 0; g < pmg_amr_ngrids(hobj(PAMR_MGH), L); g++)
+    if (pmg_amr_grid_coarsest(hobj(PAMR_MGH), L, g) == 1) return true;
+  return false;
+}
+
+// set_comm_coarsest / reset_comm (amrd/mg.c:571-600): only the coarsest grids
+// of L communicate, or all of them again
 void set_comm(int L, bool coarsest_only) {
-  if (L < 1 || L > C->nlev) return;
-  for (View &w : C->views[L - 1]) w.comm = coarsest_only ? (L == 1 ? 1 : 0) : 1;
+  std::vector<std::vector<View>> &vw = views_of(PAMR_MGH);
+  if (L < 1 || L > (int)vw.size()) return;
+  for (int g = 0; g < (int)vw[L - 1].size(); g++)
+    vw[L - 1][g].comm = coarsest_only ? pmg_amr_grid_coarsest(hobj(PAMR_MGH), L, g) == 1 : 1;
+  C->comm_coarsest = coarsest_only ? L : 0;
 }
 
 // set_res_sync (amrd/mg.c:606-616)
@@ -632,6 +671,7 @@ int PAMR_init_context(double *q, int q_size, int dim, int *shape, double *bbox, 
 
 void PAMR_free_context(int num) {
   if (!C || num != 1) return;
+  for (auto &sub : C->subs) pmg_amr_destroy(sub.second);
   if (C->A) pmg_amr_destroy(C->A);
   delete C;
   C = nullptr;
@@ -898,7 +938,7 @@ int PAMR_compose_hierarchy(int min_lev, int max_lev, int num, int *lev, double *
   C->it = Iter();
   build_views();
   // src/pamr.c:309
-  if (C->ex_on && !init_ex_mask(C->nbase, C->nlev)) {
+  if (C->ex_on && !init_ex_mask(C->A, C->nbase, C->nlev)) {
     std::printf("PAMR_compose_hierarchy: error ... filling the excision mask failed\n");
     return 0;
   }
@@ -921,32 +961,52 @@ int PAMR_build_mgh(int min_lev, int max_lev, int tl) {
     std::printf("PAMR_build_mgh: error ... AMR hierarchy does not exist\n");
     return 0;
   }
-  if (min_lev != 1 || max_lev != C->namr + 1) {
-    std::printf("PAMR_build_mgh: error ... only min_lev = 1, max_lev = %i (the finest AMR level) are supported\n",
-                C->namr + 1);
+  if (min_lev < 1 || min_lev > C->namr + 1 || max_lev != C->namr + 1) {
+    std::printf("PAMR_build_mgh: error ... only min_lev in 1..%i and max_lev = %i (the finest AMR level) are "
+                "supported\n", C->namr + 1, C->namr + 1);
     return 0;
   }
   if (tl < 1) {
     std::printf("PAMR_build_mgh: error ... invalid time level %i\n", tl);
     return 0;
   }
+  // from AMR level 1 the MG levels are A's own; from a refined level an
+  // island hierarchy over A's levels, built once and kept (src/gh.c:1222-1365)
+  pmg_amr *M = C->A;
+  if (min_lev > 1) {
+    M = nullptr;
+    for (auto &sub : C->subs)
+      if (sub.first == min_lev) M = sub.second;
+    if (!M) {
+      M = pmg_amr_create_mgh(C->A, min_lev);
+      if (!M) {
+        std::printf("PAMR_build_mgh: error ... building the MG hierarchy from level %i failed\n", min_lev);
+        return 0;
+      }
+      C->subs.emplace_back(min_lev, M);
+    }
+  }
+  C->M = M;
+  C->mg_min = min_lev;
   C->mgh = true;
   C->mgh_tl = tl;
-  // freshly allocated MG grid functions are zero (src/gh.c:895-898)
+  const int mb = mg_nbase(), ml = mg_nlev();
+  // freshly allocated MG grid functions are zero (src/gh.c:895-898,1150-1157)
   for (const Var &v : C->vars) {
     if (!v.a.in_mgh) continue;
     const int f = v.a.in_amrh ? v.amr_field[std::min<size_t>(tl, v.amr_field.size()) - 1] : v.mg_field;
-    const int Ltop = v.a.in_amrh ? C->nbase - 1 : C->nlev;
+    const int Ltop = v.a.in_amrh ? mb - 1 : ml;
     for (int L = 1; L <= Ltop; L++) {
-      double *p = pmg_amr_field(C->A, L, 0, f);
-      if (hipMemsetAsync(p, 0, pmg_amr_pool_points(C->A, L) * sizeof(double), C->stream) != hipSuccess) return 0;
+      double *p = pmg_amr_field(M, L, 0, f);
+      if (hipMemsetAsync(p, 0, pmg_amr_pool_points(M, L) * sizeof(double), C->stream) != hipSuccess) return 0;
     }
   }
   build_views();
   // the excision mask of the new MG levels (src/gh.c:1322)
-  if (C->ex_on && C->nbase > 1 && !init_ex_mask(1, C->nbase - 1)) {
+  if (C->ex_on && mb > 1 && !init_ex_mask(M, 1, mb - 1)) {
     std::printf("PAMR_build_mgh: error ... filling the excision mask failed\n");
     C->mgh = false;
+    C->M = nullptr;
     build_views();
     return 0;
   }
@@ -954,14 +1014,14 @@ int PAMR_build_mgh(int min_lev, int max_lev, int tl) {
   PAMR_thaw_tf_bits();
   PAMR_set_tf_bits(1, PAMR_MGH, PAMR_TF_MGH_INIT);
   PAMR_freeze_tf_bits();
-  for (int L = C->nbase; L > 1; L--)
+  for (int L = mb; L > 1; L--)
     if (!PAMR_inject(L, 1, PAMR_MGH)) {
       PAMR_thaw_tf_bits();
       return 0;
     }
   PAMR_thaw_tf_bits();
   C->it = Iter();
-  return C->nlev;
+  return ml;
 }
 
 int PAMR_destroy_mgh(void) {
@@ -969,6 +1029,8 @@ int PAMR_destroy_mgh(void) {
   if (!C->mgh) return 1;
   if (C->stream) (void)hipStreamSynchronize(C->stream);
   C->mgh = false;
+  C->M = nullptr;
+  C->comm_coarsest = 0;
   C->it = Iter();
   build_views();
   return 1;
@@ -982,7 +1044,7 @@ int PAMR_get_max_lev(int hier) {
   }
   if (hier == PAMR_MGH) {
     if (!C->mgh) { std::printf("PAMR_get_max_lev: error ... no MG hierarchy\n"); return 0; }
-    return C->nlev;
+    return mg_nlev();
   }
   std::printf("PAMR_get_max_lev: error ... invalid hierarchy\n");
   return 0;
@@ -1023,8 +1085,7 @@ int PAMR_excision_on(char *ex_mask_var, void (*app_fill_ex_mask_fnc)(double *, i
   C->ex_var = f->name;
   C->ex_fill = app_fill_ex_mask_fnc;
   if (initialize_now && C->A) {
-    const int Lmg0 = C->mgh ? 1 : C->nbase;
-    if (!init_ex_mask(Lmg0, C->nlev)) {
+    if (!init_ex_mask(C->A, C->nbase, C->nlev) || (C->mgh && mg_nbase() > 1 && !init_ex_mask(hobj(PAMR_MGH), 1, mg_nbase() - 1))) {
       std::printf("PAMR_excision_on: error ... filling the excision mask failed\n");
       return 0;
     }
@@ -1113,7 +1174,14 @@ int PAMR_sync(int l, int tl, int hierarchy, int AMR_bdy_width) {
   PAMR_set_tf_bits(tl, hierarchy, PAMR_TF_SYNC);
   std::vector<int> f, ops;
   selected(f, ops);
-  if (pmg_amr_sync_fields(C->A, Lp, AMR_bdy_width, (int)f.size(), f.data()) != 0) {
+  int rc;
+  if (hierarchy == PAMR_MGH && C->comm_coarsest == Lp) {  // only the coarsest grids communicate
+    if (AMR_bdy_width != 1) { std::printf("PAMR_sync: error ... coarsest-only sync needs AMR_bdy_width 1\n"); return 0; }
+    rc = pmg_amr_sync_fields_coarsest(hobj(hierarchy), Lp, (int)f.size(), f.data());
+  } else {
+    rc = pmg_amr_sync_fields(hobj(hierarchy), Lp, AMR_bdy_width, (int)f.size(), f.data());
+  }
+  if (rc != 0) {
     std::printf("PAMR_sync: error ... device transfer failed\n");
     return 0;
   }
@@ -1124,7 +1192,7 @@ int PAMR_inject(int lf, int tl, int hierarchy) {
   if (!need_ctx("PAMR_inject")) return 0;
   const int Lp = plevel(lf, hierarchy), Lc = plevel(lf - 1, hierarchy);
   if (lf < 2 || !Lp || !Lc) { std::printf("PAMR_inject: error ... level %i out of range\n", lf); return 0; }
-  if (hierarchy == PAMR_MGH && Lc < C->nbase)  // the coarse level is not shared with the AMR hierarchy
+  if (hierarchy == PAMR_MGH && Lc < mg_nbase())  // the coarse level is not shared with the AMR hierarchy
     PAMR_set_tf_bits(tl, hierarchy, PAMR_TF_INJECT_TO_MG_LEV);
   else
     PAMR_set_tf_bits(tl, hierarchy, PAMR_TF_INJECT);
@@ -1136,8 +1204,9 @@ int PAMR_inject(int lf, int tl, int hierarchy) {
       return 0;
     }
   const int mf = mask_field();
-  const int rc = mf >= 0 ? pmg_amr_inject_fields_ex(C->A, Lp, (int)f.size(), f.data(), ops.data(), mf, C->excised)
-                         : pmg_amr_inject_fields(C->A, Lp, (int)f.size(), f.data(), ops.data());
+  pmg_amr *P = hobj(hierarchy);
+  const int rc = mf >= 0 ? pmg_amr_inject_fields_ex(P, Lp, (int)f.size(), f.data(), ops.data(), mf, C->excised)
+                         : pmg_amr_inject_fields(P, Lp, (int)f.size(), f.data(), ops.data());
   if (rc != 0) {
     std::printf("PAMR_inject: error ... device transfer failed\n");
     return 0;
@@ -1162,8 +1231,9 @@ int PAMR_interp(int lc, int tl, int hierarchy) {
     for (size_t q = 0; q < f.size(); q++)
       if (ops[q] == ord) sel.push_back(f[q]);
     const int mf = mask_field();
-    const int rc = mf >= 0 ? pmg_amr_interp_fields_ex(C->A, Lp, ord, (int)sel.size(), sel.data(), mf, C->excised)
-                           : pmg_amr_interp_fields_ord(C->A, Lp, ord, (int)sel.size(), sel.data());
+    pmg_amr *P = hobj(hierarchy);
+    const int rc = mf >= 0 ? pmg_amr_interp_fields_ex(P, Lp, ord, (int)sel.size(), sel.data(), mf, C->excised)
+                           : pmg_amr_interp_fields_ord(P, Lp, ord, (int)sel.size(), sel.data());
     if (rc != 0) {
       std::printf("PAMR_interp: error ... device transfer failed\n");
       return 0;
@@ -1193,7 +1263,7 @@ int PAMR_init_s_iter(int l, int hier, int all) {
   }
   C->it.hier = hier;
   C->it.L = l;
-  C->it.g = C->views[Lp - 1].empty() ? -1 : 0;
+  C->it.g = views_of(hier)[Lp - 1].empty() ? -1 : 0;
   return C->it.g >= 0 ? 1 : 0;
 }
 
@@ -1202,7 +1272,7 @@ int PAMR_next_g(void) {
   if (C->it.g < 0) return 0;
   const int Lp = plevel(C->it.L, C->it.hier);
   C->it.g++;
-  if (!Lp || C->it.g >= (int)C->views[Lp - 1].size()) C->it.g = -1;
+  if (!Lp || C->it.g >= (int)views_of(C->it.hier)[Lp - 1].size()) C->it.g = -1;
   return C->it.g >= 0 ? 1 : 0;
 }
 
@@ -1233,12 +1303,12 @@ int PAMR_get_g_dim(int *dim) {
 int PAMR_get_g_shape(int *shape) {
   if (!cur_view("PAMR_get_g_shape")) return 0;
   double bb[6];
-  return pmg_amr_grid_info(C->A, plevel(C->it.L, C->it.hier), C->it.g, shape, bb) == 0;
+  return pmg_amr_grid_info(hobj(C->it.hier), plevel(C->it.L, C->it.hier), C->it.g, shape, bb) == 0;
 }
 int PAMR_get_g_bbox(double *bbox) {
   if (!cur_view("PAMR_get_g_bbox")) return 0;
   int shp[3];
-  return pmg_amr_grid_info(C->A, plevel(C->it.L, C->it.hier), C->it.g, shp, bbox) == 0;
+  return pmg_amr_grid_info(hobj(C->it.hier), plevel(C->it.L, C->it.hier), C->it.g, shp, bbox) == 0;
 }
 int PAMR_get_g_ghost_width(int *ghost_width) {
   if (!cur_view("PAMR_get_g_ghost_width")) return 0;
@@ -1247,7 +1317,9 @@ int PAMR_get_g_ghost_width(int *ghost_width) {
 }
 int PAMR_get_g_t(double *t) {
   if (!cur_view("PAMR_get_g_t")) return 0;
-  const int amr_l = plevel(C->it.L, C->it.hier) - C->nbase + 1;  // MG-only levels carry AMR level 1's time
+  // MG-only levels carry the time of the AMR level the MG hierarchy starts at
+  const int Lp = plevel(C->it.L, C->it.hier);
+  const int amr_l = C->it.hier == PAMR_MGH ? C->mg_min + std::max(0, Lp - mg_nbase()) : Lp - C->nbase + 1;
   *t = C->t[std::max(1, amr_l) - 1];
   return 1;
 }
@@ -1270,7 +1342,7 @@ int PAMR_get_g_gfs(double **gfs) {
 }
 int PAMR_get_g_coarsest(int *coarsest) {
   if (!cur_view("PAMR_get_g_coarsest")) return 0;
-  *coarsest = (C->it.hier == PAMR_MGH && C->it.L == 1) ? 1 : 0;
+  *coarsest = C->it.hier == PAMR_MGH ? pmg_amr_grid_coarsest(hobj(PAMR_MGH), C->it.L, C->it.g) == 1 : 0;
   return 1;
 }
 int PAMR_get_g_level(int *L) {
@@ -1470,7 +1542,7 @@ int amrd_mg_extrap(int lev, int method, double tnp1, double dt, double tnp1_lm1,
   const double f2 = -(tnp1 - (tnp1_lm1 - dt_lm1)) / dt_lm1;
   for (const ExtrapGfns &e : E) {
     long long n;
-    double *v1 = pool(Lp, e.v1, n), *v2 = pool(Lp, e.v2, n), *tm1 = pool(Lp, e.tm1, n), *tm2 = pool(Lp, e.tm2, n);
+    double *v1 = pool(Lp, e.v1, n, PAMR_AMRH), *v2 = pool(Lp, e.v2, n, PAMR_AMRH), *tm1 = pool(Lp, e.tm1, n, PAMR_AMRH), *tm2 = pool(Lp, e.tm2, n, PAMR_AMRH);
     if (!v1 || !v2 || !tm1 || !tm2) return 0;
     if (method == 0)
       hipLaunchKernelGGL(k_extrap_etm1, dim3(nblocks(n)), dim3(kNT), 0, C->stream, v1, tm1, tm2, f1, f2, n);
@@ -1493,7 +1565,7 @@ int amrd_set_mg_brs_vars(int L1, int L2) {
     if (!Lp) continue;
     for (const ExtrapGfns &e : E) {
       long long n;
-      double *brs = pool(Lp, e.brs, n), *v2 = pool(Lp, e.v2, n);
+      double *brs = pool(Lp, e.brs, n, PAMR_AMRH), *v2 = pool(Lp, e.v2, n, PAMR_AMRH);
       if (!brs || !v2 || hipMemcpyAsync(brs, v2, n * sizeof(double), hipMemcpyDeviceToDevice, C->stream) != hipSuccess)
         return 0;
     }
@@ -1520,8 +1592,8 @@ int amrd_update_mg_extrap_vars(int L1, int Lf, double eps_c) {
       const double f = eps ? 1.0 - eps_c / ((double)net_rho) : 0.0;
       for (const ExtrapGfns &e : E) {
         long long n;
-        double *tm1 = pool(Lp, e.tm1, n), *tm2 = pool(Lp, e.tm2, n), *v2 = pool(Lp, e.v2, n),
-               *brs = pool(Lp, e.brs, n);
+        double *tm1 = pool(Lp, e.tm1, n, PAMR_AMRH), *tm2 = pool(Lp, e.tm2, n, PAMR_AMRH), *v2 = pool(Lp, e.v2, n, PAMR_AMRH),
+               *brs = pool(Lp, e.brs, n, PAMR_AMRH);
         if (!tm1 || !tm2 || !v2 || !brs) return 0;
         hipLaunchKernelGGL(k_update_extrap, dim3(nblocks(n)), dim3(kNT), 0, C->stream, tm1, tm2, v2, brs, eps, f, n);
         if (hipGetLastError() != hipSuccess) return 0;
@@ -1581,7 +1653,7 @@ double amrd_vcycle_tre(int giter, int save_tre) {
       res[L] = relax_level(L, 0);
       if (!PAMR_sync(L, 0, PAMR_MGH, iter == 0 ? 0 : 1)) return -1.0;
     }
-    if (L == 1) {  // is_coarsest: the grids of MG level 1 (src/gh.c:1352-1353)
+    if (is_coarsest(L)) {  // amrd/mg.c:172-190
       res[L] = ctol + 1;
       set_comm(L, true);
       while (iter < D.max_citer && res[L] > ctol) {

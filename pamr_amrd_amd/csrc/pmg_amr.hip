@@ -185,9 +185,9 @@ __global__ __launch_bounds__(kXT) void k_xfer_interp4(XFields<T> F, const GridGe
 // where the interpolation reads it and the interpolated value is added to u
 // (and its ping-pong twin) at once; the same operations in the same order as
 // the three-step form, without storing either f_res
-template <int ORD>
-__global__ __launch_bounds__(kXT) void k_xfer_cgc(const double *__restrict__ cu, const double *__restrict__ clop,
-                                                  double w0_i, double *fu, double *fua, const GridGeo *gs,
+template <typename T, int ORD>
+__global__ __launch_bounds__(kXT) void k_xfer_cgc(const T *__restrict__ cu, const T *__restrict__ clop,
+                                                  T w0_i, T *fu, T *fua, const GridGeo *gs,
                                                   const GridGeo *gd, const XJob *jobs, const XBlk *blks) {
   const XBlk b = blks[blockIdx.x];
   const XJob J = jobs[b.job];
@@ -195,17 +195,17 @@ __global__ __launch_bounds__(kXT) void k_xfer_cgc(const double *__restrict__ cu,
   if (!job_point(J, b.first + threadIdx.x, i, j, k)) return;
   const GridGeo S = gs[J.sg], D = gd[J.dg];
   const size_t qd = D.off + ix3(i, j, k, D.nx, D.ny);
-  auto C = [&](int a, int bb, int c) -> double {
+  auto C = [&](int a, int bb, int c) -> T {
     const size_t q = S.off + ix3(a, bb, c, S.nx, S.ny);
     return (cu[q] - clop[q]) / w0_i;
   };
-  double v;
+  T v;
   if constexpr (ORD == 2) {
     const int Fx = i + J.sh[0], Fy = j + J.sh[1], Fz = k + J.sh[2];
     const int ai = Fx >> 1, ri = Fx & 1, aj = Fy >> 1, rj = Fy & 1, ak = Fz >> 1, rk = Fz & 1;
-    const double c0 = 1 - (1 * 1.0) / 2, c1 = (1 * 1.0) / 2;
-    auto X = [&](int bb, int c) -> double { return ri == 0 ? C(ai, bb, c) : c0 * C(ai, bb, c) + c1 * C(ai + 1, bb, c); };
-    auto Y = [&](int c) -> double { return rj == 0 ? X(aj, c) : c0 * X(aj, c) + c1 * X(aj + 1, c); };
+    const T c0 = T(1 - (1 * 1.0) / 2), c1 = T((1 * 1.0) / 2);
+    auto X = [&](int bb, int c) -> T { return ri == 0 ? C(ai, bb, c) : c0 * C(ai, bb, c) + c1 * C(ai + 1, bb, c); };
+    auto Y = [&](int c) -> T { return rj == 0 ? X(aj, c) : c0 * X(aj, c) + c1 * X(aj + 1, c); };
     v = rk == 0 ? Y(ak) : c0 * Y(ak) + c1 * Y(ak + 1);
   } else {
     const int fx = i + J.sh[0] - 2 * J.t0[0], fy = j + J.sh[1] - 2 * J.t0[1], fz = k + J.sh[2] - 2 * J.t0[2];
@@ -220,9 +220,9 @@ __global__ __launch_bounds__(kXT) void k_xfer_cgc(const double *__restrict__ cu,
       };
       return line(1, fy, X);
     };
-    v = line(2, fz, Y);
+    v = (T)line(2, fz, Y);
   }
-  const double nu = fu[qd] + v;
+  const T nu = fu[qd] + v;
   fu[qd] = nu;
   fua[qd] = nu;
 }
@@ -346,6 +346,65 @@ __global__ __launch_bounds__(256) void k_add_alt(T *u, T *alt, const T *r, size_
   }
 }
 
+// mixed precision: the FP64 defect r = L[u] (the composite equation has rhs
+// 0 on every AMR level's own points) of every grid of a level, written
+// negated in FP32 as the source of the FP32 correction cycle (0 off the
+// interior), with the RMS partials of the level's grids
+template <int OP, bool P2>
+__global__ __launch_bounds__(PTX * PTY) void k_defect32_batch(PtsBatchP<double> B, float *src32) {
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const BTile tl = B.tiles[t];
+  const GridGeo G = B.grids[tl.g];
+  const int i = tl.x0 + (int)(threadIdx.x % PTX), j = tl.y0 + (int)(threadIdx.x / PTX);
+  const size_t sy = G.nx, sz = (size_t)G.nx * G.ny;
+  double s = 0.0, cnt = 0.0;
+  if (i < G.nx && j < G.ny) {
+    const double *u = B.u + G.off, *rho = B.rho + G.off;
+    const double *a = B.a ? B.a + G.off : nullptr, *c = B.c ? B.c + G.off : nullptr;
+    float *out = src32 + G.off;
+    const bool ixy = i >= 1 && i <= G.nx - 2 && j >= 1 && j <= G.ny - 2;
+#pragma unroll
+    for (int kk = 0; kk < PTZ; kk++) {
+      const int k = tl.z0 + kk;
+      if (k >= G.nz) break;
+      const size_t p = ix3(i, j, k, G.nx, G.ny);
+      float v = 0.0f;
+      if (ixy && k >= 1 && k <= G.nz - 2) {
+        const double r = lop_at<double, OP, P2>(u, rho, a, c, p, sy, sz, B.sp, nullptr) - 0.0;
+        v = (float)(-r);
+        s += r * r;
+        cnt += 1.0;
+      }
+      out[p] = v;
+    }
+  }
+  norm_partial<PTX * PTY>(s, cnt, B.ws, B.slot, t, gridDim.x);
+}
+
+// u += e in FP64 over a level's pool, both ping-pong buffers
+__global__ __launch_bounds__(256) void k_add32_alt(double *u, double *ua, const float *e, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) {
+    const double v = u[q] + (double)e[q];
+    u[q] = v;
+    ua[q] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_to_f32(float *dst, const double *src, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) dst[q] = (float)src[q];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_fill0(T *a, T *b, size_t n) {
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q < n) {
+    a[q] = T(0);
+    b[q] = T(0);
+  }
+}
+
 // per-grid RMS of the partials of one slot, then the max over the grids
 // (call_rr_app: the first grid's value, then max)
 struct FoldSpec {
@@ -396,12 +455,13 @@ __global__ __launch_bounds__(256) void k_clear_ex(double *f, const double *chr, 
   if (q < n && chr[q] == ex) f[q] = 0.0;
 }
 
-__global__ __launch_bounds__(256) void k_clear3(double *a, double *b, double *c, size_t n) {
+template <typename T>
+__global__ __launch_bounds__(256) void k_clear3(T *a, T *b, T *c, size_t n) {
   const size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
-  a[i] = 0.0;
-  b[i] = 0.0;
-  c[i] = 0.0;
+  a[i] = T(0);
+  b[i] = T(0);
+  c[i] = T(0);
 }
 
 inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -466,6 +526,10 @@ struct ALevel {
   uint8_t *child = nullptr;
   GridGeo *d_geo = nullptr;
   Spacing<double> sp;
+  Spacing<float> sp32;
+  // FP32 shadows for the inner cycles of mixed precision (allocated on first
+  // use), indexed like the fields: u, rho, rhs, res, lop, a, c, then ua
+  float *f32[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   int variant = 2, zchunk = 16;
   BTile *d_stiles = nullptr;
   int nstiles = 0;
@@ -501,6 +565,9 @@ struct pmg_amr {
   int use_graph = 1;
   hipGraphExec_t graph = nullptr;
   hipGraphExec_t graph_res = nullptr;
+  hipGraphExec_t graph_mixed = nullptr;  // one mixed-precision iteration
+  bool have_fp32 = false;
+  double mixed_switch = 1e-2;  // mode 1: FP64 V-cycles while ||r|| > mixed_switch ||r0||
   double *xtmp = nullptr;  // interpolation temporary of the excision path
   size_t xtmp_cap = 0;
   // the boxes given for each AMR level (index l-1) in the caller's order
@@ -878,6 +945,20 @@ bool is_pow2(double v) {
   return m == 0.5 && v > 1e-300 && v < 1e300;
 }
 
+Spacing<float> spacing_of32(const double *hdx) {
+  Spacing<float> sp;
+  sp.dx2 = (float)(hdx[0] * hdx[0]);
+  sp.dy2 = (float)(hdx[1] * hdx[1]);
+  sp.dz2 = (float)(hdx[2] * hdx[2]);
+  sp.idx2 = 1.0f / sp.dx2;
+  sp.idy2 = 1.0f / sp.dy2;
+  sp.idz2 = 1.0f / sp.dz2;
+  sp.jac = -2 / sp.dx2 - 2 / sp.dy2 - 2 / sp.dz2;
+  sp.ijac = 1 / sp.jac;
+  sp.pow2 = is_pow2(sp.dx2) && is_pow2(sp.dy2) && is_pow2(sp.dz2);
+  return sp;
+}
+
 Spacing<double> spacing_of(const double *hdx) {
   Spacing<double> sp;
   sp.dx2 = hdx[0] * hdx[0];
@@ -992,47 +1073,82 @@ double *field_pool(ALevel &Lv, int which) {
   return nullptr;
 }
 
+// the level's pools of one precision
+template <typename T>
+struct LvPools {
+  T *u, *ua, *rho, *rhs, *res, *lop, *a, *c;
+};
+template <typename T>
+LvPools<T> pools(ALevel &Lv);
+template <>
+LvPools<double> pools<double>(ALevel &Lv) {
+  return {Lv.u, Lv.ua, Lv.rho, Lv.rhs, Lv.res, Lv.lop, Lv.a, Lv.c};
+}
+template <>
+LvPools<float> pools<float>(ALevel &Lv) {
+  return {Lv.f32[PMG_F_U], Lv.f32[7], Lv.f32[PMG_F_RHO], Lv.f32[PMG_F_RHS], Lv.f32[PMG_F_RES], Lv.f32[PMG_F_LOP],
+          Lv.f32[PMG_F_A], Lv.f32[PMG_F_C]};
+}
+template <typename T>
+const Spacing<T> &spacing_t(const ALevel &Lv);
+template <>
+const Spacing<double> &spacing_t<double>(const ALevel &Lv) { return Lv.sp; }
+template <>
+const Spacing<float> &spacing_t<float>(const ALevel &Lv) { return Lv.sp32; }
+
+template <typename T>
+T *field_pool_t(ALevel &Lv, int which) {
+  if constexpr (std::is_same<T, double>::value) {
+    return field_pool(Lv, which);
+  } else {
+    return (which >= PMG_F_U && which <= PMG_F_C) ? Lv.f32[which] : nullptr;
+  }
+}
+
 bool valid_field(const pmg_amr *A, int which) {
   return (which >= 0 && which <= PMG_F_C) || (which >= PMG_F_EXTRA && which < PMG_F_EXTRA + A->nextra);
 }
 
 // one transfer plan with up to kMaxFields fields; kind 0 copy, 1-3 restrict,
 // 4 interp (second order), 5 interp (fourth order)
+template <typename T = double>
 int run_plan(pmg_amr *A, const XPlan &P, int Ls, int Ld, int kind, const int *which, int nf) {
   if (P.nblocks == 0 || nf == 0) return 0;
   ALevel &S = A->lev[Ls - 1], &D = A->lev[Ld - 1];
-  XFields<double> F;
+  XFields<T> F;
   F.nf = 0;
   for (int f = 0; f < nf; f++) {
-    double *s = field_pool(S, which[f]), *d = field_pool(D, which[f]);
+    T *s = field_pool_t<T>(S, which[f]), *d = field_pool_t<T>(D, which[f]);
     if (!s || !d) continue;
     F.src[F.nf] = s;
     F.dst[F.nf] = d;
-    F.alt[F.nf] = which[f] == PMG_F_U ? D.ua : nullptr;
+    F.alt[F.nf] = which[f] == PMG_F_U ? pools<T>(D).ua : nullptr;
     F.nf++;
   }
   if (F.nf == 0) return 0;
   const dim3 grid(P.nblocks), blk(kXT);
   switch (kind) {
-    case 0: hipLaunchKernelGGL(k_xfer_copy<double>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
-    case 1: hipLaunchKernelGGL((k_xfer_restrict<double, 1>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
-    case 2: hipLaunchKernelGGL((k_xfer_restrict<double, 2>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
-    case 3: hipLaunchKernelGGL((k_xfer_restrict<double, 3>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
-    case 4: hipLaunchKernelGGL(k_xfer_interp<double>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
-    case 5: hipLaunchKernelGGL(k_xfer_interp4<double>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
+    case 0: hipLaunchKernelGGL(k_xfer_copy<T>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
+    case 1: hipLaunchKernelGGL((k_xfer_restrict<T, 1>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
+    case 2: hipLaunchKernelGGL((k_xfer_restrict<T, 2>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
+    case 3: hipLaunchKernelGGL((k_xfer_restrict<T, 3>), grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
+    case 4: hipLaunchKernelGGL(k_xfer_interp<T>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
+    case 5: hipLaunchKernelGGL(k_xfer_interp4<T>, grid, blk, 0, A->stream, F, S.d_geo, D.d_geo, P.d_jobs, P.d_blks); break;
     default: return -1;
   }
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
 
+template <typename T = double>
 int do_sync(pmg_amr *A, int L, int bdy, int which) {
   ALevel &Lv = A->lev[L - 1];
-  return run_plan(A, bdy ? Lv.sync1 : Lv.sync0, L, L, 0, &which, 1);
+  return run_plan<T>(A, bdy ? Lv.sync1 : Lv.sync0, L, L, 0, &which, 1);
 }
 
 // PAMR_inject(Lf): u and u_res; the sources only into pure MG levels
 // (TF_INJECT vs TF_INJECT_TO_MG_LEV, src/pamr.c:677-681,1886-1897)
+template <typename T = double>
 int do_inject_cycle(pmg_amr *A, int Lf) {
   ALevel &C = A->lev[Lf - 2];
   int ops[kMaxFields], ws[kMaxFields], n = 0;
@@ -1045,27 +1161,29 @@ int do_inject_cycle(pmg_amr *A, int Lf) {
   for (int op = 1; op <= 3; op++) {
     int sel[kMaxFields], m = 0;
     for (int f = 0; f < n; f++) if (ops[f] == op) sel[m++] = ws[f];
-    int rc = run_plan(A, C.inject, Lf, Lf - 1, op, sel, m);
+    int rc = run_plan<T>(A, C.inject, Lf, Lf - 1, op, sel, m);
     if (rc) return rc;
   }
   return 0;
 }
 
+template <typename T = double>
 int clear_mg_vars(pmg_amr *A, int L) {
   ALevel &Lv = A->lev[L - 1];
+  const LvPools<T> P = pools<T>(Lv);
   if (!Lv.pool) return 0;
-  if (!Lv.res || !Lv.rhs || !Lv.lop) return -1;
-  hipLaunchKernelGGL(k_clear3, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.res, Lv.rhs, Lv.lop, Lv.pool);
+  if (!P.res || !P.rhs || !P.lop) return -1;
+  hipLaunchKernelGGL(k_clear3<T>, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, P.res, P.rhs, P.lop, Lv.pool);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
 
-template <int OP, bool P2, bool RHS0>
-int launch_sweep_batch(pmg_amr *A, ALevel &Lv, const SweepBatchP<double> &B) {
+template <typename T, int OP, bool P2, bool RHS0>
+int launch_sweep_batch(pmg_amr *A, ALevel &Lv, const SweepBatchP<T> &B) {
   const dim3 grid(Lv.nstiles);
   switch (Lv.variant) {
 #define PMG_LSB(V, TXV, TYV, NTV) \
-  case V: hipLaunchKernelGGL((k_sweep_batch<double, OP, P2, RHS0, TXV, TYV, NTV>), grid, dim3(NTV), 0, A->stream, B); break;
+  case V: hipLaunchKernelGGL((k_sweep_batch<T, OP, P2, RHS0, TXV, TYV, NTV>), grid, dim3(NTV), 0, A->stream, B); break;
     PMG_AMR_VARIANTS(PMG_LSB)
 #undef PMG_LSB
     default: return -1;
@@ -1076,64 +1194,68 @@ int launch_sweep_batch(pmg_amr *A, ALevel &Lv, const SweepBatchP<double> &B) {
 
 // n batched sweeps on level L, each followed by PAMR_sync(L, 0 then 1)
 // (amrd/mg.c:135-141,161-167); sync writes both ping-pong buffers
+template <typename T = double>
 int sweeps(pmg_amr *A, int L, int n, bool do_sync_after) {
   ALevel &Lv = A->lev[L - 1];
+  const LvPools<T> Q = pools<T>(Lv);
   if (n <= 0 || Lv.nstiles == 0) return 0;
   if (Lv.nstiles > A->ws.cap) return -3;
-  SweepBatchP<double> B;
-  B.rho = Lv.rho; B.rhs = Lv.rhs; B.a = Lv.a; B.c = Lv.c;
+  SweepBatchP<T> B;
+  B.rho = Q.rho; B.rhs = Q.rhs; B.a = Q.a; B.c = Q.c;
   B.grids = Lv.d_geo;
   B.tiles = Lv.d_stiles;
   B.zchunk = Lv.zchunk;
-  B.sp = Lv.sp;
+  B.sp = spacing_t<T>(Lv);
   B.ws = A->ws;
   B.slot = AS_RELAX + L - 1;
-  const bool rhs0 = L == A->nlev, p2 = Lv.sp.pow2 != 0;
+  const bool rhs0 = L == A->nlev, p2 = B.sp.pow2 != 0;
   for (int s = 0; s < n; s++) {
-    double *cur = (s % 2 == 0) ? Lv.u : Lv.ua, *nxt = (s % 2 == 0) ? Lv.ua : Lv.u;
+    T *cur = (s % 2 == 0) ? Q.u : Q.ua, *nxt = (s % 2 == 0) ? Q.ua : Q.u;
     B.uin = cur;
     B.uout = nxt;
     int rc;
-#define PMG_SB(OPV) (p2 ? (rhs0 ? launch_sweep_batch<OPV, true, true>(A, Lv, B) : launch_sweep_batch<OPV, true, false>(A, Lv, B)) \
-                        : (rhs0 ? launch_sweep_batch<OPV, false, true>(A, Lv, B) : launch_sweep_batch<OPV, false, false>(A, Lv, B)))
+#define PMG_SB(OPV) (p2 ? (rhs0 ? launch_sweep_batch<T, OPV, true, true>(A, Lv, B) : launch_sweep_batch<T, OPV, true, false>(A, Lv, B)) \
+                        : (rhs0 ? launch_sweep_batch<T, OPV, false, true>(A, Lv, B) : launch_sweep_batch<T, OPV, false, false>(A, Lv, B)))
     rc = A->op == 0 ? PMG_SB(0) : PMG_SB(1);
 #undef PMG_SB
     if (rc) return rc;
     if (do_sync_after) {
       const XPlan &P = s == 0 ? Lv.sync0 : Lv.sync1;
       if (P.nblocks) {
-        XFields<double> F;
+        XFields<T> F;
         F.nf = 1;
         F.src[0] = nxt;
         F.dst[0] = nxt;
         F.alt[0] = cur;
-        hipLaunchKernelGGL(k_xfer_copy<double>, dim3(P.nblocks), dim3(kXT), 0, A->stream, F, Lv.d_geo, Lv.d_geo,
+        hipLaunchKernelGGL(k_xfer_copy<T>, dim3(P.nblocks), dim3(kXT), 0, A->stream, F, Lv.d_geo, Lv.d_geo,
                            P.d_jobs, P.d_blks);
         PMG_HOST_CHECK(hipGetLastError());
       }
     }
   }
   if (n % 2 == 1) {
-    hipLaunchKernelGGL(k_copy<double>, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.u, Lv.ua, Lv.pool);
+    hipLaunchKernelGGL(k_copy<T>, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Q.u, Q.ua, Lv.pool);
     PMG_HOST_CHECK(hipGetLastError());
   }
   return 0;
 }
 
+template <typename T = double>
 int residual(pmg_amr *A, int L, int slot) {
   ALevel &Lv = A->lev[L - 1];
+  const LvPools<T> Q = pools<T>(Lv);
   if (Lv.nptiles > A->ws.cap) return -3;
-  PtsBatchP<double> B{};
-  B.u = Lv.u; B.rho = Lv.rho; B.rhs = Lv.rhs; B.a = Lv.a; B.c = Lv.c;
-  B.res = Lv.res;
+  PtsBatchP<T> B{};
+  B.u = Q.u; B.rho = Q.rho; B.rhs = Q.rhs; B.a = Q.a; B.c = Q.c;
+  B.res = Q.res;
   B.grids = Lv.d_geo;
   B.tiles = Lv.d_ptiles;
-  B.sp = Lv.sp;
+  B.sp = spacing_t<T>(Lv);
   B.ws = A->ws;
   B.slot = slot;
-  const bool rhs0 = L == A->nlev, p2 = Lv.sp.pow2 != 0;
+  const bool rhs0 = L == A->nlev, p2 = B.sp.pow2 != 0;
   const dim3 grid(Lv.nptiles), blk(PTX * PTY);
-#define PMG_RB(OPV, P2V, R0) hipLaunchKernelGGL((k_resid_batch<double, OPV, P2V, R0>), grid, blk, 0, A->stream, B)
+#define PMG_RB(OPV, P2V, R0) hipLaunchKernelGGL((k_resid_batch<T, OPV, P2V, R0>), grid, blk, 0, A->stream, B)
   if (A->op == 0) {
     if (p2) { if (rhs0) PMG_RB(0, true, true); else PMG_RB(0, true, false); }
     else { if (rhs0) PMG_RB(0, false, true); else PMG_RB(0, false, false); }
@@ -1146,20 +1268,22 @@ int residual(pmg_amr *A, int L, int slot) {
   return 0;
 }
 
+template <typename T = double>
 int lop_rhs(pmg_amr *A, int L) {
   ALevel &Lv = A->lev[L - 1];
-  PtsBatchP<double> B{};
-  B.u = Lv.u; B.rho = Lv.rho; B.res = Lv.res; B.a = Lv.a; B.c = Lv.c;
-  B.lop = Lv.lop;
-  B.rhs_out = Lv.rhs;
+  const LvPools<T> Q = pools<T>(Lv);
+  PtsBatchP<T> B{};
+  B.u = Q.u; B.rho = Q.rho; B.res = Q.res; B.a = Q.a; B.c = Q.c;
+  B.lop = Q.lop;
+  B.rhs_out = Q.rhs;
   B.child = Lv.child;
   B.grids = Lv.d_geo;
   B.tiles = Lv.d_ptiles;
-  B.sp = Lv.sp;
-  B.w0_r = A->w0_r;
-  const bool p2 = Lv.sp.pow2 != 0;
+  B.sp = spacing_t<T>(Lv);
+  B.w0_r = (T)A->w0_r;
+  const bool p2 = B.sp.pow2 != 0;
   const dim3 grid(Lv.nptiles), blk(PTX * PTY);
-#define PMG_LB(OPV, P2V) hipLaunchKernelGGL((k_lop_rhs_batch<double, OPV, P2V>), grid, blk, 0, A->stream, B)
+#define PMG_LB(OPV, P2V) hipLaunchKernelGGL((k_lop_rhs_batch<T, OPV, P2V>), grid, blk, 0, A->stream, B)
   if (A->op == 0) { if (p2) PMG_LB(0, true); else PMG_LB(0, false); }
   else { if (p2) PMG_LB(1, true); else PMG_LB(1, false); }
 #undef PMG_LB
@@ -1169,48 +1293,52 @@ int lop_rhs(pmg_amr *A, int L) {
 
 // compute_cgc (amrd/mg.c:323-355): res = (u - lop)/w0_i on L-1, PAMR_interp,
 // u += res on L (whole grids, both ping-pong buffers)
+template <typename T = double>
 int cgc(pmg_amr *A, int L) {
   ALevel &C = A->lev[L - 2], &F = A->lev[L - 1];
+  const LvPools<T> QC = pools<T>(C), QF = pools<T>(F);
   if (A->iord != 2 && A->iord != 4) return -4;
   if (F.interp_once) {
     const XPlan &P = F.interp;
     if (P.nblocks == 0) return 0;
     if (A->iord == 4)
-      hipLaunchKernelGGL(k_xfer_cgc<4>, dim3(P.nblocks), dim3(kXT), 0, A->stream, C.u, C.lop, A->w0_i, F.u, F.ua,
-                         C.d_geo, F.d_geo, P.d_jobs, P.d_blks);
+      hipLaunchKernelGGL((k_xfer_cgc<T, 4>), dim3(P.nblocks), dim3(kXT), 0, A->stream, QC.u, QC.lop, (T)A->w0_i,
+                         QF.u, QF.ua, C.d_geo, F.d_geo, P.d_jobs, P.d_blks);
     else
-      hipLaunchKernelGGL(k_xfer_cgc<2>, dim3(P.nblocks), dim3(kXT), 0, A->stream, C.u, C.lop, A->w0_i, F.u, F.ua,
-                         C.d_geo, F.d_geo, P.d_jobs, P.d_blks);
+      hipLaunchKernelGGL((k_xfer_cgc<T, 2>), dim3(P.nblocks), dim3(kXT), 0, A->stream, QC.u, QC.lop, (T)A->w0_i,
+                         QF.u, QF.ua, C.d_geo, F.d_geo, P.d_jobs, P.d_blks);
     PMG_HOST_CHECK(hipGetLastError());
     return 0;
   }
-  hipLaunchKernelGGL(k_cgc_diff<double>, dim3(nblk(C.pool)), dim3(256), 0, A->stream, C.res, C.u, C.lop, A->w0_i,
+  hipLaunchKernelGGL(k_cgc_diff<T>, dim3(nblk(C.pool)), dim3(256), 0, A->stream, QC.res, QC.u, QC.lop, (T)A->w0_i,
                      C.pool);
   PMG_HOST_CHECK(hipGetLastError());
   const int w = PMG_F_RES;
-  int rc = run_plan(A, F.interp, L - 1, L, A->iord == 4 ? 5 : 4, &w, 1);
+  int rc = run_plan<T>(A, F.interp, L - 1, L, A->iord == 4 ? 5 : 4, &w, 1);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_add_alt<double>, dim3(nblk(F.pool)), dim3(256), 0, A->stream, F.u, F.ua, F.res, F.pool);
+  hipLaunchKernelGGL(k_add_alt<T>, dim3(nblk(F.pool)), dim3(256), 0, A->stream, QF.u, QF.ua, QF.res, F.pool);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
 
+template <typename T = double>
 int coarse_solve(pmg_amr *A) {
   ALevel &Lv = A->lev[0];
+  const LvPools<T> Q = pools<T>(Lv);
   const HGrid &g = Lv.g[0];
-  CoarseP<double> P;
-  P.u = Lv.u; P.rho = Lv.rho; P.rhs = Lv.rhs; P.a = Lv.a; P.c = Lv.c; P.chr = nullptr;
+  CoarseP<T> P;
+  P.u = Q.u; P.rho = Q.rho; P.rhs = Q.rhs; P.a = Q.a; P.c = Q.c; P.chr = nullptr;
   P.nx = g.n[0]; P.ny = g.n[1]; P.nz = g.n[2];
   P.bx0 = P.by0 = P.bz0 = 1;
   P.bx1 = g.n[0] - 2; P.by1 = g.n[1] - 2; P.bz1 = g.n[2] - 2;
-  P.sp = Lv.sp;
+  P.sp = spacing_t<T>(Lv);
   P.npre = A->pre; P.npst = A->pst; P.max_citer = A->max_citer; P.crtol = A->crtol;
   P.norm_pre = A->d_cnorm; P.norm_post = A->d_cnorm + 1; P.citer = A->d_citer;
   const int narr = (A->op == 1) ? 5 : 3;
-  const size_t lds = g.size() * narr * sizeof(double);
+  const size_t lds = g.size() * narr * sizeof(T);
   const bool use_lds = lds <= 60 * 1024, p2 = P.sp.pow2 != 0;
   constexpr int NT = 256;
-#define PMG_CS(OPV, P2V, LDSV) hipLaunchKernelGGL((k_coarse_solve<double, OPV, false, P2V, LDSV, NT>), dim3(1), dim3(NT), LDSV ? lds : 0, A->stream, P)
+#define PMG_CS(OPV, P2V, LDSV) hipLaunchKernelGGL((k_coarse_solve<T, OPV, false, P2V, LDSV, NT>), dim3(1), dim3(NT), LDSV ? lds : 0, A->stream, P)
 #define PMG_CS2(OPV) \
   if (p2) { if (use_lds) PMG_CS(OPV, true, true); else PMG_CS(OPV, true, false); } \
   else { if (use_lds) PMG_CS(OPV, false, true); else PMG_CS(OPV, false, false); }
@@ -1219,7 +1347,7 @@ int coarse_solve(pmg_amr *A) {
 #undef PMG_CS
   PMG_HOST_CHECK(hipGetLastError());
   // u and its ping-pong twin agree after the in-place coarse solve
-  hipLaunchKernelGGL(k_copy<double>, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.ua, Lv.u, Lv.pool);
+  hipLaunchKernelGGL(k_copy<T>, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Q.ua, Q.u, Lv.pool);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
@@ -1231,32 +1359,128 @@ int fold(pmg_amr *A) {
   return 0;
 }
 
-// amrd/mg.c:84-201
-int enqueue_vcycle(pmg_amr *A) {
+// amrd/mg.c:84-201; the FP32 instantiation is the inner cycle of mixed
+// precision (no final residual: the FP64 defect replaces it)
+template <typename T = double>
+int enqueue_vcycle_t(pmg_amr *A) {
   const int Lf = A->nlev;
   int rc;
-  if ((rc = clear_mg_vars(A, Lf))) return rc;
+  if ((rc = clear_mg_vars<T>(A, Lf))) return rc;
   for (int L = Lf; L >= 1; L--) {
     if (L < Lf) {
-      if ((rc = residual(A, L + 1, AS_RESID + L))) return rc;
-      if ((rc = do_sync(A, L + 1, 0, PMG_F_RES))) return rc;  // set_res_sync + PAMR_sync
-      if ((rc = clear_mg_vars(A, L))) return rc;
-      if ((rc = do_inject_cycle(A, L + 1))) return rc;
-      if ((rc = do_sync(A, L, 0, PMG_F_U))) return rc;
-      if ((rc = lop_rhs(A, L))) return rc;
+      if ((rc = residual<T>(A, L + 1, AS_RESID + L))) return rc;
+      if ((rc = do_sync<T>(A, L + 1, 0, PMG_F_RES))) return rc;  // set_res_sync + PAMR_sync
+      if ((rc = clear_mg_vars<T>(A, L))) return rc;
+      if ((rc = do_inject_cycle<T>(A, L + 1))) return rc;
+      if ((rc = do_sync<T>(A, L, 0, PMG_F_U))) return rc;
+      if ((rc = lop_rhs<T>(A, L))) return rc;
     }
     if (L == 1) {
-      if ((rc = coarse_solve(A))) return rc;
+      if ((rc = coarse_solve<T>(A))) return rc;
     } else {
-      if ((rc = sweeps(A, L, A->pre, true))) return rc;
+      if ((rc = sweeps<T>(A, L, A->pre, true))) return rc;
     }
   }
   for (int L = 2; L <= Lf; L++) {
-    if ((rc = cgc(A, L))) return rc;
-    if ((rc = sweeps(A, L, A->pst, true))) return rc;
+    if ((rc = cgc<T>(A, L))) return rc;
+    if ((rc = sweeps<T>(A, L, A->pst, true))) return rc;
   }
-  if ((rc = residual(A, Lf, AS_FINAL))) return rc;
+  if constexpr (std::is_same<T, float>::value) return 0;
+  if ((rc = residual<T>(A, Lf, AS_FINAL))) return rc;
   return fold(A);
+}
+
+int enqueue_vcycle(pmg_amr *A) { return enqueue_vcycle_t<double>(A); }
+
+// FP32 shadows of every level (allocated once; the coefficients converted
+// from FP64 at every call, so that later uploads of a and c are seen)
+int ensure_fp32(pmg_amr *A) {
+  PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
+  for (ALevel &Lv : A->lev) {
+    if (!A->have_fp32)
+      for (int w = 0; w < 8; w++) {
+        if ((w == PMG_F_A || w == PMG_F_C) && A->op == 0) continue;
+        PMG_HOST_CHECK(hipMalloc(&Lv.f32[w], Lv.pool * sizeof(float)));
+        PMG_HOST_CHECK(hipMemset(Lv.f32[w], 0, Lv.pool * sizeof(float)));
+      }
+    if (A->op == 1) {
+      hipLaunchKernelGGL(k_to_f32, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.f32[PMG_F_A], Lv.a, Lv.pool);
+      hipLaunchKernelGGL(k_to_f32, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.f32[PMG_F_C], Lv.c, Lv.pool);
+      PMG_HOST_CHECK(hipGetLastError());
+    }
+  }
+  PMG_HOST_CHECK(hipStreamSynchronize(A->stream));
+  A->have_fp32 = true;
+  return 0;
+}
+
+// u of every coarser AMR level replaced, where a finer level covers it, by
+// the restriction of the finer u (top down), then synchronised: the FAS cycle
+// does this at the start of each level of its V (amrd/mg.c:112-118), so that
+// the coarse values under a finer level -- those at the finer level's
+// boundary included, whose coarse corrections become the finer boundary's --
+// are the finer solution's.  Without it the FP64 u of the coarse points at a
+// finer boundary would keep their own history, and defect correction would
+// converge to another fixed point than the FAS cycle's.
+int inject_u_down(pmg_amr *A) {
+  if (A->rop_u < 1 || A->rop_u > 3) return 0;
+  const int w = PMG_F_U;
+  for (int L = A->nlev - 1; L >= A->nbase; L--) {
+    int rc;
+    if ((rc = run_plan<double>(A, A->lev[L - 1].inject, L + 1, L, A->rop_u, &w, 1))) return rc;
+    if ((rc = do_sync<double>(A, L, 0, PMG_F_U))) return rc;
+  }
+  return 0;
+}
+
+// the FP64 defect of every AMR level into the FP32 sources, after the
+// injection of u above; the finest level's norm goes to the final slot, then
+// the fold
+int enqueue_defect(pmg_amr *A) {
+  int rc;
+  if ((rc = inject_u_down(A))) return rc;
+  for (int L = A->nbase; L <= A->nlev; L++) {
+    ALevel &Lv = A->lev[L - 1];
+    if (Lv.nptiles > A->ws.cap) return -3;
+    PtsBatchP<double> B{};
+    B.u = Lv.u; B.rho = Lv.rho; B.a = Lv.a; B.c = Lv.c;
+    B.grids = Lv.d_geo;
+    B.tiles = Lv.d_ptiles;
+    B.sp = Lv.sp;
+    B.ws = A->ws;
+    B.slot = L == A->nlev ? AS_FINAL : AS_RESID + L - 1;
+    const bool p2 = Lv.sp.pow2 != 0;
+    const dim3 grid(Lv.nptiles), blk(PTX * PTY);
+    float *src = Lv.f32[PMG_F_RHO];
+    if (A->op == 0) {
+      if (p2) hipLaunchKernelGGL((k_defect32_batch<0, true>), grid, blk, 0, A->stream, B, src);
+      else hipLaunchKernelGGL((k_defect32_batch<0, false>), grid, blk, 0, A->stream, B, src);
+    } else {
+      if (p2) hipLaunchKernelGGL((k_defect32_batch<1, true>), grid, blk, 0, A->stream, B, src);
+      else hipLaunchKernelGGL((k_defect32_batch<1, false>), grid, blk, 0, A->stream, B, src);
+    }
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  return fold(A);
+}
+
+// one mixed-precision iteration: the FP32 V-cycle on A e = -r from e = 0,
+// u += e on every AMR level, the new FP64 defect
+int enqueue_mixed(pmg_amr *A) {
+  int rc;
+  for (ALevel &Lv : A->lev) {
+    hipLaunchKernelGGL(k_fill0<float>, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.f32[PMG_F_U], Lv.f32[7],
+                       Lv.pool);
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  if ((rc = enqueue_vcycle_t<float>(A))) return rc;
+  for (int L = A->nbase; L <= A->nlev; L++) {
+    ALevel &Lv = A->lev[L - 1];
+    hipLaunchKernelGGL(k_add32_alt, dim3(nblk(Lv.pool)), dim3(256), 0, A->stream, Lv.u, Lv.ua, Lv.f32[PMG_F_U],
+                       Lv.pool);
+    PMG_HOST_CHECK(hipGetLastError());
+  }
+  return enqueue_defect(A);
 }
 
 int enqueue_residual(pmg_amr *A) {
@@ -1283,7 +1507,7 @@ int capture_or_run(pmg_amr *A, hipGraphExec_t *cache, int (*enq)(pmg_amr *)) {
 }
 
 void drop_graphs(pmg_amr *A) {
-  for (hipGraphExec_t *g : {&A->graph, &A->graph_res})
+  for (hipGraphExec_t *g : {&A->graph, &A->graph_res, &A->graph_mixed})
     if (*g) { (void)hipGraphExecDestroy(*g); *g = nullptr; }
 }
 
@@ -1323,6 +1547,7 @@ int setup_levels(pmg_amr *A, int op) {
       for (int d = 0; d < 3; d++)
         if (g.hdx[d] != Lv.g[0].hdx[d] || g.n[d] < 3) return -1;
     Lv.sp = spacing_of(Lv.g[0].hdx);
+    Lv.sp32 = spacing_of32(Lv.g[0].hdx);
     size_t pool = 0;
     std::vector<GridGeo> geo;
     for (HGrid &g : Lv.g) {
@@ -1498,6 +1723,7 @@ void pmg_amr_destroy(pmg_amr *A) {
     }
     for (double *p : Lv.cm) if (p) (void)hipFree(p);
     if (Lv.child) (void)hipFree(Lv.child);
+    for (float *p : Lv.f32) if (p) (void)hipFree(p);
     if (Lv.d_geo) (void)hipFree(Lv.d_geo);
     for (BTile *p : {Lv.d_stiles, Lv.d_ptiles}) if (p) (void)hipFree(p);
     for (int *p : {Lv.d_sstart, Lv.d_pstart}) if (p) (void)hipFree(p);
@@ -1716,6 +1942,44 @@ int pmg_amr_residual(pmg_amr *A, double *norm_out) {
   int rc = capture_or_run(A, &A->graph_res, enqueue_residual);
   if (rc) return rc;
   return read_norms(A, norm_out, nullptr, nullptr);
+}
+
+int pmg_amr_solve_rel(pmg_amr *A, int mode, double rtol, int max_iter, double *hist, int *iters, double *r0_out) {
+  if (!A || max_iter < 0 || (mode != 0 && mode != 1)) return -1;
+  if (A->parent) {
+    std::fprintf(stderr, "pamr_amr: the built-in V-cycle runs on hierarchies from level 1 only\n");
+    return -1;
+  }
+  int rc, it = 0;
+  double r;
+  if ((rc = pmg_amr_residual(A, &r))) return rc;
+  const double rref = (r0_out && *r0_out > 0) ? *r0_out : r;
+  if (r0_out) *r0_out = rref;
+  if (mode == 1 && (rc = ensure_fp32(A))) return rc;
+  bool defect_ready = false;  // the FP32 sources hold the defect of the current u
+  while (it < max_iter && r > rtol * rref) {
+    if (mode == 0 || r > A->mixed_switch * rref) {
+      if ((rc = pmg_amr_vcycle(A, &r, nullptr, nullptr))) return rc;
+      defect_ready = false;
+    } else {
+      if (!defect_ready) {
+        if ((rc = enqueue_defect(A))) return rc;
+        defect_ready = true;
+      }
+      if ((rc = capture_or_run(A, &A->graph_mixed, enqueue_mixed))) return rc;
+      if ((rc = read_norms(A, &r, nullptr, nullptr))) return rc;
+    }
+    if (hist) hist[it] = r;
+    it++;
+  }
+  if (iters) *iters = it;
+  return 0;
+}
+
+int pmg_amr_set_mixed_switch(pmg_amr *A, double frac) {
+  if (!A || !(frac >= 0.0) || frac > 1.0) return -1;
+  A->mixed_switch = frac;
+  return 0;
 }
 
 int pmg_amr_sync(pmg_amr *A, int L, int bdy, int which) {

@@ -1079,10 +1079,49 @@ struct BottomP {
   int *citer;
 };
 
-template <typename T, int OP, bool P2>
+// LDS: every array of the bottom levels (u, rho, rhs, res, lop, and a, c for
+// op 1) lives in dynamic LDS for the whole V, so the many barrier-separated
+// phases of the small levels read and write LDS instead of L2.  u, rho, res
+// (a, c) come in, u, rhs, res and lop go back out; the arithmetic is the
+// same, so the results are the same bits as without LDS.
+template <typename T, int OP, bool P2, bool LDS = false>
 __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
   constexpr int NT = kBotThreads;
   __shared__ double bres;
+  extern __shared__ unsigned char bot_smem[];
+  __shared__ BotLev<T> slv[LDS ? kBotMax : 1];
+  const BotLev<T> *lv = P.lev;
+  if constexpr (LDS) {
+    T *m = reinterpret_cast<T *>(bot_smem);
+    for (int l = 0; l < P.nb; l++) {
+      const BotLev<T> &G = P.lev[l];
+      const int n = G.nx * G.ny * G.nz;
+      T *u = m, *rho = m + n, *rhs = m + 2 * n, *res = m + 3 * n, *lop = m + 4 * n;
+      m += 5 * n;
+      T *a = nullptr, *c = nullptr;
+      if constexpr (OP == 1) {
+        a = m;
+        c = m + n;
+        m += 2 * n;
+      }
+      for (int q = threadIdx.x; q < n; q += NT) {
+        u[q] = G.u[q];
+        rho[q] = G.rho[q];
+        res[q] = G.res[q];
+        if constexpr (OP == 1) {
+          a[q] = G.a[q];
+          c[q] = G.c[q];
+        }
+      }
+      if (threadIdx.x == 0) {
+        slv[l] = G;
+        slv[l].u = u; slv[l].rho = rho; slv[l].rhs = rhs; slv[l].res = res; slv[l].lop = lop;
+        slv[l].a = a; slv[l].c = c;
+      }
+    }
+    __syncthreads();
+    lv = slv;
+  }
   auto put_norm = [&](int slot, double s, double c) {
     block_reduce2<NT>(s, c);
     if (threadIdx.x == 0) {
@@ -1194,18 +1233,18 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
   const int top = P.nb - 1;
   double s, cnt;
   for (int l = top; l >= 1; l--) {
-    const BotLev<T> &L = P.lev[l];
+    const BotLev<T> &L = lv[l];
     lop_rhs(L);
     for (int it = 0; it < P.npre; it++) sweep(L, s, cnt);
     residual(L, s, cnt);
     put_norm(P.resid_slot0 + l, s, cnt);
-    if (P.rop_u) restrict_to(L.u, P.lev[l - 1].u, L, P.lev[l - 1], P.rop_u);
-    if (P.rop_res) restrict_to(L.res, P.lev[l - 1].res, L, P.lev[l - 1], P.rop_res);
+    if (P.rop_u) restrict_to(L.u, lv[l - 1].u, L, lv[l - 1], P.rop_u);
+    if (P.rop_res) restrict_to(L.res, lv[l - 1].res, L, lv[l - 1], P.rop_res);
     __syncthreads();
   }
   // coarsest level (amrd/mg.c:135-141, 161-190 at L = 1)
   {
-    const BotLev<T> &L = P.lev[0];
+    const BotLev<T> &L = lv[0];
     lop_rhs(L);
     double res = 0.0;
     for (int it = 0; it < P.npre; it++) { sweep(L, s, cnt); res = put_norm(P.relax_slot0, s, cnt); }
@@ -1225,10 +1264,23 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
     }
   }
   for (int l = 1; l <= top; l++) {
-    const BotLev<T> &L = P.lev[l];
-    cgc(L, P.lev[l - 1]);
+    const BotLev<T> &L = lv[l];
+    cgc(L, lv[l - 1]);
     for (int it = 0; it < P.npst; it++) sweep(L, s, cnt);
     if (P.npst > 0) put_norm(P.relax_slot0 + l, s, cnt);
+  }
+  if constexpr (LDS) {
+    __syncthreads();
+    for (int l = 0; l < P.nb; l++) {
+      const BotLev<T> &G = P.lev[l];
+      const int n = G.nx * G.ny * G.nz;
+      for (int q = threadIdx.x; q < n; q += NT) {
+        G.u[q] = lv[l].u[q];
+        G.rhs[q] = lv[l].rhs[q];
+        G.res[q] = lv[l].res[q];
+        G.lop[q] = lv[l].lop[q];
+      }
+    }
   }
 }
 

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "pmg_kernels.h"
@@ -134,6 +135,7 @@ struct pmg_solver {
   int use_graph = 1;
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   int bottom_max_n = 17;  // ... those with at most this many points per side
+  int bottom_lds = 1;     // ... with their arrays in LDS when they fit
   int use_fused_restrict = 1;  // finest residual + injection in one pass
   int rr_czchunk[2] = {8, 8};   // its coarse planes per block, [0] FP64, [1] FP32
   int use_sweep2 = 0;          // pairs of sweeps temporally blocked on large levels
@@ -602,6 +604,48 @@ int bottom_top(const pmg_solver *S) {
   return Lb >= 2 ? Lb : 0;
 }
 
+// whether the LDS bottom kernel can take lds bytes of dynamic LDS: the
+// device's per-workgroup LDS less the kernel's static LDS (level table,
+// reduction slots), with the dynamic limit raised to match once per variant
+template <typename T, int OP, bool P2>
+bool bottom_lds_fits_v(size_t lds) {
+  constexpr int kDev = 64;
+  static std::mutex mu;
+  static int dev_limit[kDev];
+  static size_t static_lds[kDev];
+  static bool known[kDev], raised[kDev];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kDev) return false;
+  std::lock_guard<std::mutex> g(mu);
+  if (!known[dev]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return false;
+    hipFuncAttributes fa;
+    if (hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(&k_bottom<T, OP, P2, true>)) != hipSuccess)
+      return false;
+    static_lds[dev] = fa.sharedSizeBytes;
+    dev_limit[dev] = v;
+    known[dev] = true;
+  }
+  if (lds + static_lds[dev] > (size_t)dev_limit[dev]) return false;
+  if (!raised[dev]) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_bottom<T, OP, P2, true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize,
+                            dev_limit[dev] - (int)static_lds[dev]) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    raised[dev] = true;
+  }
+  return true;
+}
+
+template <typename T>
+bool bottom_lds_fits(int op, bool p2, size_t lds, T *) {
+  if (op == 0) return p2 ? bottom_lds_fits_v<T, 0, true>(lds) : bottom_lds_fits_v<T, 0, false>(lds);
+  return p2 ? bottom_lds_fits_v<T, 1, true>(lds) : bottom_lds_fits_v<T, 1, false>(lds);
+}
+
 template <typename T>
 int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
   BottomP<T> P;
@@ -635,7 +679,19 @@ int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
   const bool p2 = P.lev[0].sp.pow2 != 0;
   for (int L = 2; L <= Lb; L++)
     if ((P.lev[L - 1].sp.pow2 != 0) != p2) return -5;
-#define PMG_BOT(OPV, P2V) hipLaunchKernelGGL((k_bottom<T, OPV, P2V>), dim3(1), dim3(kBotThreads), 0, S->stream, P)
+  // the LDS variant when every array of every bottom level fits beside the
+  // kernel's static LDS
+  size_t lds = 0;
+  for (int l = 0; l < Lb; l++)
+    lds += (size_t)(S->op == 1 ? 7 : 5) * P.lev[l].nx * P.lev[l].ny * P.lev[l].nz * sizeof(T);
+  const bool use_lds = S->bottom_lds && bottom_lds_fits(S->op, p2, lds, (T *)nullptr);
+#define PMG_BOT(OPV, P2V)                                                                                           \
+  do {                                                                                                              \
+    if (use_lds)                                                                                                    \
+      hipLaunchKernelGGL((k_bottom<T, OPV, P2V, true>), dim3(1), dim3(kBotThreads), lds, S->stream, P);            \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_bottom<T, OPV, P2V, false>), dim3(1), dim3(kBotThreads), 0, S->stream, P);             \
+  } while (0)
   if (S->op == 0) { if (p2) PMG_BOT(0, true); else PMG_BOT(0, false); }
   else { if (p2) PMG_BOT(1, true); else PMG_BOT(1, false); }
 #undef PMG_BOT
@@ -1022,6 +1078,13 @@ int pmg_set_bottom_fusion(pmg_solver *S, int on) {
   return 0;
 }
 
+int pmg_set_bottom_lds(pmg_solver *S, int on) {
+  if (!S || on < 0 || on > 1) return -1;
+  S->bottom_lds = on;
+  invalidate_graphs(S);
+  return 0;
+}
+
 int pmg_set_sweep_blocking(pmg_solver *S, int on, int zchunk) {
   if (!S || zchunk < 0) return -1;
   S->use_sweep2 = on ? 1 : 0;
@@ -1263,6 +1326,9 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
         // buffers and weight fragments outside the timed region: the
         // acceptance test compares steady-state costs
         if ((rc = pmg_net_reserve(S->net, (long long)F.n[0] * F.n[1] * F.n[2]))) return rc;
+        // u before the step, in the twin buffer (no FP64 sweep runs in this
+        // mode): a rejected step restores it exactly
+        PMG_HOST_CHECK(hipMemcpyAsync(F.d.ua, F.d.u, F.size * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
         PMG_HOST_CHECK(hipEventRecord(e0, S->stream));
         if ((rc = pmg_net_correct(S->net, F.f.rho, (float)(-1.0 / r), F.d.u, r * h2, F.n[0], F.n[1], F.n[2],
                                   S->stream))) return rc;
@@ -1284,9 +1350,9 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
           it++;
           continue;
         }
-        // undo: u -= coef * y; the defect is recomputed for the V-cycle
+        // undo: u restored from its copy; the defect is recomputed for the V-cycle
         S->net_rejected++;
-        if ((rc = pmg_net_recorrect(S->net, F.d.u, -r * h2, F.n[0], F.n[1], F.n[2], S->stream))) return rc;
+        PMG_HOST_CHECK(hipMemcpyAsync(F.d.u, F.d.ua, F.size * sizeof(double), hipMemcpyDeviceToDevice, S->stream));
         if ((rc = residual<double>(S, S->nlev, NS_OUTER, true, true, F.f.rho))) return rc;
         net_on = false;
         S->net_skip = S->net_retry_period;

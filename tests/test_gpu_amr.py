@@ -229,3 +229,57 @@ def test_sweep_config_rejects(torch_cuda):
             a.set_level_sweep_config(a.nlev, *bad)
         assert a.sweep_config(a.nlev) == v0
     a.close()
+
+
+@pytest.mark.parametrize("op", [0, 1])
+def test_mixed_precision_amr_reaches_1e10(torch_cuda, op):
+    """FP32 V-cycles of the hierarchy in FP64 defect correction
+    (pmg_amr_solve_rel mode 1) reach a 1e-10 relative FP64 residual.  The
+    defect is taken after u is injected down the AMR levels, as the FAS cycle
+    does, so in exact arithmetic the two iterations are the same: their
+    residual histories agree while FP32 rounding is far below the residual.
+    The composite solution -- every finest-level grid, the coarser AMR levels
+    where no finer grid covers them -- is compared with the FP64 solve's to
+    1e-13.  A fine grid's AMR-boundary values are set only by coarse-grid
+    corrections, which no later defect measures, so the FP32 rounding of the
+    corrections stays in them: FP32 from the start agrees to 1e-6 (FP32
+    rounding of the accumulated corrections), FP32 after FP64 cycles down to
+    1e-3 (pmg_amr_set_mixed_switch) to 1e-9.  Measured values are printed."""
+    levels = layout(5, base_n=33, nmin=4, nmax=6, smin=9, smax=17)
+    print(f"case amr mixed op={op} boxes={[len(l) for l in levels]}", flush=True)
+    runs = []
+    for mode, switch, rtol in ((0, 1.0, 1e-13), (1, 1.0, 1e-10), (1, 1e-3, 1e-10)):
+        a = pm.Amr(33, levels, op=op)
+        for L in range(a.base, a.nlev + 1):
+            for g in range(a.ngrids(L)):
+                shape, bb = a.grid(L, g)
+                a.upload(L, g, pm.PMG_F_RHO, -H.gaussian_on(shape, bb, seed=77))
+                if op == 1:
+                    a.upload(L, g, pm.PMG_F_A, H.lognormal_on(shape, bb))
+                    a.upload(L, g, pm.PMG_F_C, H.uniform_on(shape, bb))
+        a.set_params()
+        a.init()
+        a.set_mixed_switch(switch)
+        r0, h = a.solve_rel(mode, rtol, 600)
+        print(f"mode {mode} switch {switch:g}: {len(h)} iterations, final {h[-1] / r0:.3e}", flush=True)
+        assert len(h) and h[-1] <= rtol * r0, (mode, switch, len(h), h[-3:] / r0)
+        sol = {}
+        for L in range(a.base, a.nlev + 1):
+            for g in range(a.ngrids(L)):
+                own = a.cmask(L, g, 1) == 0.0 if L < a.nlev else np.ones(a.grid(L, g)[0], dtype=bool)
+                sol[L, g] = (a.download(L, g, pm.PMG_F_U), own)
+        runs.append((r0, np.asarray(h), sol))
+        a.close()
+    r64, h64, s64 = runs[0]
+    scale = max(np.abs(u[own]).max() for u, own in s64.values() if own.any())
+    for (r32, h32, s32), bound in zip(runs[1:], (1e-6, 1e-9)):
+        assert abs(r32 - r64) <= 1e-12 * r64
+        k = int(np.sum(h64[:len(h32)] > 1e-5 * r64))
+        assert k >= 3
+        rel = np.abs(h32[:k] - h64[:k]) / h64[:k]
+        err = max(np.abs(um[own] - u64[own]).max() for (um, own), (u64, _) in zip(s32.values(), s64.values())
+                  if own.any())
+        print(f"residual histories: first {k} iterations agree to {rel.max():.2e}; composite solution vs FP64: "
+              f"{err / scale:.3e} relative (bound {bound:g})", flush=True)
+        assert rel.max() <= 1e-3
+        assert err <= bound * scale

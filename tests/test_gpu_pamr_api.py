@@ -643,3 +643,158 @@ def test_excised_amr_matches_oracle(torch_cuda, seed):
         for g in range(m.ngrids(Lm)):
             assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, "V"), m.field(Lm, g, O.OA_U)), ("u", Lm, g)
     ctx.close()
+
+
+# ---------------------------------------------------------------------------
+# MG hierarchies built from a refined AMR level (PAMR_build_mgh(min_lev > 1),
+# src/gh.c:1222-1365: islands coarsened from that level's boxes)
+# ---------------------------------------------------------------------------
+def island_layout(seed):
+    return H.random_boxes(base_n=33, nref=2, nmin=3, nmax=5, smin=9, smax=17, seed=seed)
+
+
+def mg_grids(ctx, lev):
+    """[(shape, bbox, coarsest)] of the MG grids of lev, in iterator order"""
+    L = ctx.L
+    out = []
+    valid = L.PAMR_init_s_iter(lev, PAMR_MGH, 0)
+    while valid:
+        sh, bb, _ = _cur_grid(ctx)
+        c = C.c_int()
+        assert L.PAMR_get_g_coarsest(C.byref(c)) == 1
+        out.append((sh, bb.copy(), c.value))
+        valid = L.PAMR_next_g()
+    return out
+
+
+# (layout seed, AMR level the MG hierarchy starts at): seed 1 / 7 from level 2
+# and seed 6 from level 3 hold a level with coarsest and other grids, seed 2
+# a coarsest level of four grids
+@pytest.mark.parametrize("seed,amr_lev", [(1, 2), (7, 2), (6, 3), (2, 2)])
+def test_mgh_from_refined_level_matches_oracle(torch_cuda, seed, amr_lev):
+    """PAMR_build_mgh(amr_lev, finest, 1): the island levels (grids, order,
+    coarsest flags), the initial injection and three V-cycles with the
+    application's hooks -- coarsest loops on levels whose other grids stay
+    idle included -- equal the oracle's hierarchy bit for bit"""
+    levels = island_layout(seed)
+    print(f"case mgh from level {amr_lev} seed={seed} boxes={[len(l) for l in levels]}", flush=True)
+    m = O.AmrMG(33, levels, mcw=3)
+    ctx = NpsContext(33, levels, mcw=3)
+    load_sources(m, ctx, 0, 500 + seed)
+    m.set_params()
+    s = m.sub_mgh(amr_lev)
+    s.init()
+    ctx.set_params(pre=2, pst=2)
+    nl = ctx.L.PAMR_build_mgh(amr_lev, ctx.max_lev, 1)
+    assert nl == s.nlev
+    for Lm in range(1, nl + 1):
+        got = mg_grids(ctx, Lm)
+        assert len(got) == s.ngrids(Lm), Lm
+        for g, (sh, bb, co) in enumerate(got):
+            shape, box = s.grid(Lm, g)
+            assert sh == shape and np.array_equal(bb, box), (Lm, g)
+            assert co == int(s.coarsest(Lm, g)), (Lm, g)
+            assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, "rho"), s.field(Lm, g, O.OA_RHO)), (Lm, g)
+    for c in range(3):
+        rm, nm, cm = s.vcycle()
+        ra, na, ca = ctx.vcycle(c)
+        print(f"cycle {c}: oracle {rm:.15e} drop-in {ra:.15e} citer {cm} {ca}", flush=True)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        assert ca == cm
+        assert np.allclose(na, nm, rtol=1e-12, atol=0)
+    for Lm in range(1, nl + 1):
+        for g in range(s.ngrids(Lm)):
+            assert np.array_equal(ctx.download(PAMR_MGH, Lm, g, "V"), s.field(Lm, g, O.OA_U)), ("u", Lm, g)
+    assert ctx.L.PAMR_destroy_mgh() == 1
+    # built again (the hierarchy is kept): the same levels
+    assert ctx.L.PAMR_build_mgh(amr_lev, ctx.max_lev, 1) == nl
+    ctx.close()
+    del s
+
+
+def test_evo_iterate_on_refined_hierarchy(torch_cuda):
+    """tstep's iteration (amrd/evolve.c:449-500) on the finest level of a
+    two-level hierarchy: amrd_evo_iterate builds PAMR_build_mgh(2, 2, 1)
+    (evolve.c:454), islands below the two disjoint level-2 grids; a toy
+    evolution phi(tl 1) = phi(tl 2) + dt V feeds rho; against the same loop
+    over the oracle's island hierarchy: iteration count, residuals, V and phi
+    bit for bit"""
+    pbt = (C.c_int * 6)()
+    dt = 1.0 / 64.0
+    h = 1.0 / 32
+    levels = [[(2 * h, 18 * h, 2 * h, 18 * h, 4 * h, 12 * h), (20 * h, 28 * h, 6 * h, 22 * h, 16 * h, 28 * h)]]
+
+    def defs(L):
+        assert L.PAMR_def_var_full(b"phi", 1, 0, 2, 0, 0, 0, 1, 0, 0, 0, 0, 0, pbt) > 0
+
+    m = O.AmrMG(17, levels, mcw=3)
+    ctx = NpsContext(17, levels, mcw=3, extra_defs=defs)
+    assert ctx.max_lev == 2
+    phi2 = {}
+    for g, (shape, bb, _) in enumerate(ctx.grids(PAMR_AMRH, 2)):
+        Z, Y, X = _coords(shape, bb)
+        phi2[g] = np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z)
+        for tl in (1, 2):
+            ctx.upload(PAMR_AMRH, 2, g, "phi", phi2[g], tl=tl)
+        ctx.upload(PAMR_AMRH, 2, g, "rho", phi2[g])
+        ctx.upload(PAMR_AMRH, 2, g, "V", np.zeros(shape))
+    L = ctx.L
+
+    def evolve(it):
+        sh, b, gfs = _cur_grid(ctx)
+        a = {}
+        for nm, tl in (("phi", 2), ("V", 1)):
+            a[nm] = np.empty(sh)
+            assert ctx.app.nps_download(pm.P(a[nm]), gfs[ctx.gfn(nm, PAMR_AMRH, tl) - 1], a[nm].size)
+        p1 = np.ascontiguousarray(a["phi"] + dt * a["V"])
+        assert ctx.app.nps_upload(gfs[ctx.gfn("phi", PAMR_AMRH, 1) - 1], pm.P(p1), p1.size)
+        assert ctx.app.nps_upload(gfs[ctx.gfn("rho", PAMR_AMRH, 1) - 1], pm.P(p1), p1.size)
+
+    def evo_res():
+        sh, b, gfs = _cur_grid(ctx)
+        a = {}
+        for nm, tl in (("phi", 1), ("phi", 2), ("V", 1)):
+            a[nm, tl] = np.empty(sh)
+            assert ctx.app.nps_download(pm.P(a[nm, tl]), gfs[ctx.gfn(nm, PAMR_AMRH, tl) - 1], a[nm, tl].size)
+        d = a["phi", 1] - a["phi", 2] - dt * a["V", 1]
+        return float(np.sqrt(np.mean(d * d)))
+
+    ev, er = EVOLVE(evolve), REAL(evo_res)
+    mg_tol = 1e-9
+    ctx.set_params(pre=2, pst=2, tol=mg_tol)
+    r_evo, r_mg = C.c_double(), C.c_double()
+    n_it = L.amrd_evo_iterate(2, C.cast(ev, C.c_void_p), C.cast(er, C.c_void_p), 2, 30, 1e-12, 0,
+                              C.byref(r_evo), C.byref(r_mg))
+    # the oracle's side: the island hierarchy from level 2, the same loop
+    for g in range(m.ngrids(m.base + 1)):
+        m.field(m.base + 1, g, O.OA_RHO)[:] = phi2[g]
+    m.set_params()
+    s = m.sub_mgh(2)
+    s.init()
+    top = s.nlev
+    assert s.nlev > 1
+    it, mg_res, evo = 0, mg_tol + 1, 1.0
+    p1 = {}
+    while (it < 2 or evo > 1e-12 or mg_res > mg_tol) and it < 30:
+        evo = 0.0
+        for g in range(s.ngrids(top)):
+            U = s.field(top, g, O.OA_U)
+            p1[g] = phi2[g] + dt * U
+            s.field(top, g, O.OA_RHO)[:] = p1[g]
+        for g in range(s.ngrids(top)):
+            U = s.field(top, g, O.OA_U)
+            d = p1[g] - phi2[g] - dt * U
+            r = float(np.sqrt(np.mean(d * d)))
+            evo = r if g == 0 else max(evo, r)
+        mg_res = s.vcycle()[0]
+        it += 1
+    print(f"evo iterate (refined): gpu {n_it} iterations mg_res {r_mg.value:.6e}, oracle {it} {mg_res:.6e}",
+          flush=True)
+    assert n_it == it and 2 < it < 30
+    assert abs(r_mg.value - mg_res) <= 1e-12 * mg_res
+    assert r_evo.value == evo
+    for g in range(s.ngrids(top)):
+        assert np.array_equal(ctx.download(PAMR_AMRH, 2, g, "V"), s.field(top, g, O.OA_U)), g
+        assert np.array_equal(ctx.download(PAMR_AMRH, 2, g, "phi", tl=1), p1[g]), g
+    ctx.close()
+    del s

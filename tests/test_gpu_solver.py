@@ -242,6 +242,42 @@ def test_bottom_fusion_matches_unfused(pm, n):
     Su.close()
 
 
+@pytest.mark.parametrize("op", [0, 1])
+def test_bottom_lds_bit_identical(pm, op):
+    """the bottom V with its arrays in LDS gives the same bits as from global
+    memory: FP64 cycles with the bottom up to 9^3 (its arrays fit), FP32
+    inner cycles with the bottom up to 17^3 -- u of every level, the norms,
+    the coarsest iteration counts and the FP64 residual history"""
+    n = 65
+    print(f"case bottom lds op={op}", flush=True)
+    rho = -pm.gaussian_source(n, seed=8)
+    a = c = None
+    if op == 1:
+        rng = np.random.default_rng(3)
+        a = np.exp(0.5 * rng.standard_normal((n, n, n)))
+        c = rng.random((n, n, n))
+    Sl, _ = _pair(pm, n, rho, op=op, a=a, c=c)
+    Sg, _ = _pair(pm, n, rho, op=op, a=a, c=c)
+    Sg.set_bottom_lds(False)
+    for S in (Sl, Sg):
+        S.set_bottom_fusion(True, max_n=9)
+    for _ in range(3):
+        rl, nl, cl = Sl.vcycle()
+        rg, ng, cg = Sg.vcycle()
+        assert rl == rg and cl == cg and np.array_equal(nl, ng)
+    for L in range(1, Sl.nlev + 1):
+        assert np.array_equal(Sl.download(L, pm.PMG_F_U), Sg.download(L, pm.PMG_F_U)), L
+    for S in (Sl, Sg):
+        S.set_bottom_fusion(True)
+        S.zero_solution()
+    r0l, hl = Sl.solve_rel(1, 1e-10, 40)
+    r0g, hg = Sg.solve_rel(1, 1e-10, 40)
+    assert r0l == r0g and np.array_equal(hl, hg)
+    assert np.array_equal(Sl.download(Sl.nlev, pm.PMG_F_U), Sg.download(Sg.nlev, pm.PMG_F_U))
+    Sl.close()
+    Sg.close()
+
+
 def test_mixed_precision_reaches_1e10(pm):
     """FP32 inner V-cycles in FP64 defect correction reach ||r||/||r0|| <= 1e-10;
     the error against a 40-cycle FP64 solution is reported and bounded."""
