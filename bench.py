@@ -61,6 +61,19 @@ def measured_traffic(n, variant, prec="double"):
     return None if entry is None else entry["traffic_bytes"]
 
 
+def measured_resid_restrict_traffic(n, name):
+    """HBM bytes per launch of the fused finest residual + injection from the
+    committed rocprofv3 summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_finest_{n}.json")
+    if not os.path.exists(path):
+        return None
+    kern = {"f64": "pmg::k_resid_restrict<double, 0, true, 16, 8, 256>",
+            "f32": "pmg::k_resid_restrict<float, 0, true, 32, 8, 256>"}[name]
+    with open(path) as f:
+        entry = json.load(f).get("kernels", {}).get(kern)
+    return None if entry is None else entry["traffic_bytes"]
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -240,6 +253,33 @@ def amr_leg(args, pm, torch):
         hist.append(A.vcycle()[0])
     torch.cuda.synchronize()
     ms_cycle = 1e3 * (time.perf_counter() - t0) / ncyc
+    # FP32 inner cycles in FP64 defect correction (defect after u is injected
+    # down the levels, FP32 V-cycle on the error, u += e): a fixed number of
+    # iterations from the current u
+    A.set_mixed_switch(1.0)
+    A.solve_rel(1, 0.0, 2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, hmix = A.solve_rel(1, 0.0, ncyc)
+    torch.cuda.synchronize()
+    ms_mixed = 1e3 * (time.perf_counter() - t0) / len(hmix)
+    # whole solves from u = 0 to ||r|| <= rtol ||r0||: FP64 V-cycles, and
+    # mixed precision with FP64 cycles down to 1e-2 ||r0|| (the default)
+    solves = {}
+    for name, mode, switch in (("fp64", 0, 1.0), ("mixed", 1, 1e-2)):
+        for L in range(A.base, A.nlev + 1):
+            for g in range(A.ngrids(L)):
+                A.upload(L, g, pm.PMG_F_U, np.zeros(A.grid(L, g)[0]))
+        A.init()
+        A.set_mixed_switch(switch)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r0, h = A.solve_rel(mode, args.rtol, 400)
+        torch.cuda.synchronize()
+        dt = 1e3 * (time.perf_counter() - t0)
+        if len(h) == 0 or h[-1] > args.rtol * r0:
+            raise RuntimeError(f"amr {name} solve did not reach {args.rtol}: {len(h)} iterations")
+        solves[name] = {"ms": round(dt, 2), "iterations": len(h), "final_rel_residual": float(h[-1] / r0)}
     Lf = A.nlev
     t_sw = A.time_sweep(Lf, 10)
     interior = sum(int(np.prod([s - 2 for s in A.grid(Lf, g)[0]])) for g in range(A.ngrids(Lf)))
@@ -251,6 +291,8 @@ def amr_leg(args, pm, torch):
         "levels": A.nlev, "grids_per_level": [A.ngrids(L) for L in range(1, A.nlev + 1)],
         "points_per_amr_level": npts,
         "ms_per_vcycle": round(ms_cycle, 4), "vcycles_per_s": round(1e3 / ms_cycle, 1),
+        "ms_per_mixed_iteration": round(ms_mixed, 4),
+        "solve_to_rtol": solves, "rtol": args.rtol,
         "residual_history": [float(f"{h:.6e}") for h in hist],
         "finest_batched_sweep": {"grids": A.ngrids(Lf), "interior_points": interior, "avg_ms": round(t_sw, 5),
                                  "algorithmic_bytes": bytes_sw,
@@ -272,27 +314,35 @@ def excised_leg(args, pm, torch):
     right-hand sides, each solved to ||r|| <= rtol ||r(u0)|| by flexible
     GMRES(8) with the FP32 V-cycle as preconditioner; the first right-hand
     sides also by FP32 inner V-cycles in FP64 defect correction, the first in
-    plain FP64 V-cycles (capped) for comparison."""
+    plain FP64 V-cycles (capped) for comparison; then all right-hand sides
+    again concurrently, spread over c5_streams solver instances (one stream
+    and host thread each)."""
     import numpy as np
     from pamr_amrd_amd import hierarchy as H
     from pamr_amrd_amd import sequence as SQ
     n = args.c5_n
     box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     spheres = H.excision_spheres(seed=21)
-    S = pm.Solver(n, op=1, do_ex=1, ex=1.0)
-    S.set_params(pre=2, pst=2, max_citer=5000, crtol=1e-3, rop_u=3, rop_res=3, rop_src=3)
-    Lf = S.nlev
-    for lev in range(1, Lf + 1):
-        S.set_chr(lev, H.chr_on(S.shape(lev), box, spheres))
     shape = (n, n, n)
     a = H.lognormal_on_device(shape, box, torch)
-    SQ._copy_grid(S.field_ptr(Lf, pm.PMG_F_A), pm.tptr(a), n)
-    del a
     gen = torch.Generator(device="cuda").manual_seed(5)
     c = torch.rand(shape, generator=gen, dtype=torch.float64, device="cuda")
-    SQ._copy_grid(S.field_ptr(Lf, pm.PMG_F_C), pm.tptr(c), n)
-    del c
-    ch = H.chr_on(shape, box, spheres)
+    chrs = {}
+
+    def make_solver():
+        S = pm.Solver(n, op=1, do_ex=1, ex=1.0)
+        S.set_params(pre=2, pst=2, max_citer=5000, crtol=1e-3, rop_u=3, rop_res=3, rop_src=3)
+        for lev in range(1, S.nlev + 1):
+            if lev not in chrs:
+                chrs[lev] = H.chr_on(S.shape(lev), box, spheres)
+            S.set_chr(lev, chrs[lev])
+        SQ._copy_grid(S.field_ptr(S.nlev, pm.PMG_F_A), pm.tptr(a), n)
+        SQ._copy_grid(S.field_ptr(S.nlev, pm.PMG_F_C), pm.tptr(c), n)
+        return S
+
+    S = make_solver()
+    Lf = S.nlev
+    ch = chrs[Lf]
     excised = int((ch == 1.0).sum())
     u0 = torch.from_numpy(np.where(ch == 1.0, H.excision_data(shape, box), 0.0)).cuda()
     del ch
@@ -333,7 +383,31 @@ def excised_leg(args, pm, torch):
                 plain = {"ms": round(dt, 2), "cycles": len(h), "final_rel_residual": float(h[-1] / r0),
                          "converged": bool(h[-1] <= args.rtol * r0),
                          "mean_contraction": float((h[-1] / r0) ** (1.0 / len(h)))}
+    # the same right-hand sides concurrently: one solver per stream, one host
+    # thread each, every solve the single-RHS FGMRES (same bits as alone)
+    conc = None
+    if args.c5_streams > 1:
+        from pamr_amrd_amd import multirhs as MR
+        solvers = [S] + [make_solver() for _ in range(args.c5_streams - 1)]
+        rhs = [SQ.DriftingSource(seed=1234 + r, speed=0.0).rho(n, 0, torch) for r in range(args.c5_rhs)]
+        for X in solvers:  # allocations (FP32 shadows, Krylov basis) before the timed region
+            L.pmg_set_solution_lincomb(X.h, pm.tptr(u0), None, 1.0, 0.0)
+            X.init()
+            X.solve_fgmres(args.c5_restart, args.rtol, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res = MR.solve_fgmres_concurrent(solvers, [pm.tptr(r) for r in rhs], pm.tptr(u0), args.c5_restart,
+                                         args.rtol, args.c5_max_iter)
+        torch.cuda.synchronize()
+        dt = 1e3 * (time.perf_counter() - t0)
+        conc = {"streams": len(solvers), "rhs": len(rhs), "total_ms": round(dt, 1),
+                "ms_per_rhs": round(dt / len(rhs), 2),
+                "preconditioner_applications": [len(h) for _, h in res]}
+        del rhs
+        for X in solvers[1:]:
+            X.close()
     S.close()
+    del a, c, u0
     return {"workload": f"excised{n}_3spheres_varcoef_16rhs", "n": n, "excised_points": excised,
             "rhs": args.c5_rhs,
             "fgmres_ms_per_rhs": round(sum(gtimes) / len(gtimes), 2), "fgmres_total_ms": round(sum(gtimes), 1),
@@ -341,7 +415,8 @@ def excised_leg(args, pm, torch):
             "mixed_rhs_timed": len(times),
             "mixed_ms_per_rhs": round(sum(times) / len(times), 2), "mixed_outer_iters": iters,
             "mixed_mean_contraction": None if not hists else float(np.mean(hists)),
-            "plain_fp64_first_rhs": plain, "rtol": args.rtol, "reference_norm": "||r(u0)||"}
+            "plain_fp64_first_rhs": plain, "fgmres_concurrent": conc,
+            "rtol": args.rtol, "reference_norm": "||r(u0)||"}
 
 
 def probe_kernels(S, n, pm, torch):
@@ -431,6 +506,8 @@ def main():
     ap.add_argument("--c5-max-iter", type=int, default=300)
     ap.add_argument("--c5-restart", type=int, default=8, help="FGMRES restart length of the excised leg")
     ap.add_argument("--c5-mixed-rhs", type=int, default=2, help="right-hand sides also solved by defect correction")
+    ap.add_argument("--c5-streams", type=int, default=4,
+                    help="solver instances (streams, host threads) of the concurrent multi-RHS run; 1 skips it")
     ap.add_argument("--probe-net", action="store_true",
                     help="only time the correction network's inference on the n^3 grid (for a kernel trace)")
     ap.add_argument("--probe-kernels", action="store_true",
@@ -498,6 +575,13 @@ def main():
     ms_plain, r0p, hp = timed(0)
     print("bench: timing mixed FP32/FP64", file=sys.stderr, flush=True)
     ms_mixed, r0m, hm = timed(1)
+    # the mixed solution's error against a 40-cycle FP64 solve of the same problem
+    u_mixed = S.download(Lf, pm.PMG_F_U)
+    S.zero_solution()
+    S.init()
+    S.solve(40, 40)
+    u_ref40 = S.download(Lf, pm.PMG_F_U)
+    err_mixed = float(np.abs(u_mixed - u_ref40).max() / np.abs(u_ref40).max())
 
     # dominant kernel: finest-level fused red-black sweep (rhs == 0 there):
     # reads u and rho, writes u -> 24 B (FP64) / 12 B (FP32) per interior point
@@ -519,6 +603,17 @@ def main():
               "traffic": measured_traffic(n, variant32, "float"), "sweep_variant": variant32, "zchunk": zchunk32,
               "algorithmic_bytes": bytes32,
               "avg_ms": round(t32, 5)}
+    # the finest residual fused with the full-weight injection of u and f_res:
+    # reads u and rho at every fine point, writes the coarse u and f_res
+    nc = (n - 1) // 2 + 1
+    roof_rr = {}
+    for prec, name, b in ((0, "f64", 16), (1, "f32", 8)):
+        t = S.time_kernel(2, Lf, prec, 40)
+        by = b * n ** 3 + b * nc ** 3
+        ach = by / (t * 1e-3) / 1e9
+        roof_rr[name] = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": measured_resid_restrict_traffic(n, name),
+                         "algorithmic_bytes": by, "avg_ms": round(t, 5)}
     cpu = None
     if not args.no_cpu:
         print("bench: CPU baseline (oracle, 1 core)", file=sys.stderr, flush=True)
@@ -527,7 +622,8 @@ def main():
               "mixed_fp32_fp64_ms_per_solve": round(ms_mixed, 4),
               "speedup_mixed_vs_plain_fp64": round(ms_plain / ms_mixed, 3),
               "plain_cycles": len(hp), "mixed_outer_iters": len(hm),
-              "final_rel_residual": {"plain": hp[-1] / r0p, "mixed": hm[-1] / r0m}}
+              "final_rel_residual": {"plain": hp[-1] / r0p, "mixed": hm[-1] / r0m},
+              "mixed_error_vs_fp64_40_cycles": err_mixed}
     amrres = None
     if not args.no_amr:
         print("bench: batched AMR hierarchy leg", file=sys.stderr, flush=True)
@@ -577,6 +673,7 @@ def main():
         "excised": exres,
         "roofline": roof,
         "roofline_fp32_sweep": roof32,
+        "roofline_resid_restrict": roof_rr,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
