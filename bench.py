@@ -67,8 +67,8 @@ def measured_resid_restrict_traffic(n, name):
     path = os.path.join(ROOT, "profiles", f"traffic_finest_{n}.json")
     if not os.path.exists(path):
         return None
-    kern = {"f64": "pmg::k_resid_restrict<double, 0, true, 16, 8, 256>",
-            "f32": "pmg::k_resid_restrict<float, 0, true, 32, 8, 256>"}[name]
+    kern = {"f64": "pmg::k_resid_restrict_s<double, 0, true, 16, 8, 256, 1>",
+            "f32": "pmg::k_resid_restrict_s<float, 0, true, 32, 8, 256, 1>"}[name]
     with open(path) as f:
         entry = json.load(f).get("kernels", {}).get(kern)
     return None if entry is None else entry["traffic_bytes"]

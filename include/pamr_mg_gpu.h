@@ -195,7 +195,11 @@ int pmg_set_bottom_lds(pmg_solver *S, int on);
  * cycle) inside the full-weight injection of u and f_res instead of storing
  * it (default on; no excision, full weighting for both); u, the norms and
  * every coarse-level field are identical either way, but with the fusion on
- * the finest level's f_res (PMG_F_RES) is not stored by the V-cycle */
+ * the finest level's f_res (PMG_F_RES) is not stored by the V-cycle.
+ * on = 1: a ring of three fine residual planes per workgroup; on = 2 (the
+ * default): the restriction carried through the z march in registers; on =
+ * 3: the same with two fine planes of u and rho in flight instead of one
+ * (same results) */
 int pmg_set_restrict_fusion(pmg_solver *S, int on);
 /* coarse planes per workgroup of that fused kernel (prec 0 FP64 cycle, 1 FP32
  * inner cycle; 1..64; default 5 / 12 when the coarse level has at least 129

@@ -416,7 +416,9 @@ class Solver:
         _check(self._L.pmg_set_bottom_lds(self.h, 1 if on else 0), "pmg_set_bottom_lds")
 
     def set_restrict_fusion(self, on):
-        _check(self._L.pmg_set_restrict_fusion(self.h, 1 if on else 0), "pmg_set_restrict_fusion")
+        """0 off, 1 (True) the ring kernel, 2 the restriction streamed through the z march,
+        3 the same with two planes in flight"""
+        _check(self._L.pmg_set_restrict_fusion(self.h, int(on)), "pmg_set_restrict_fusion")
 
     def set_resid_restrict_zchunk(self, prec, czchunk):
         """coarse planes per workgroup of the fused finest residual + injection"""

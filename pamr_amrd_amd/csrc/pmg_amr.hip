@@ -558,6 +558,7 @@ struct pmg_amr {
   NormWs ws{nullptr, nullptr, 0, 0};
   double *d_norms = nullptr;  // [0, kMaxLev): relax per level; [kMaxLev, 2 kMaxLev): residual; 2 kMaxLev: final
   double *d_cnorm = nullptr;  // coarse solve pre / post
+  double *d_snap = nullptr;   // saved state of level 1 for the coarsest loop's cycle watch
   int *d_citer = nullptr;
   FoldSpec *d_fold = nullptr;
   int nfold = 0;
@@ -1334,6 +1335,7 @@ int coarse_solve(pmg_amr *A) {
   P.sp = spacing_t<T>(Lv);
   P.npre = A->pre; P.npst = A->pst; P.max_citer = A->max_citer; P.crtol = A->crtol;
   P.norm_pre = A->d_cnorm; P.norm_post = A->d_cnorm + 1; P.citer = A->d_citer;
+  P.snap = reinterpret_cast<T *>(A->d_snap);
   const int narr = (A->op == 1) ? 5 : 3;
   const size_t lds = g.size() * narr * sizeof(T);
   const bool use_lds = lds <= 60 * 1024, p2 = P.sp.pow2 != 0;
@@ -1648,6 +1650,7 @@ int setup_levels(pmg_amr *A, int op) {
       hipMalloc(&A->ws.slot_nb, sizeof(int) * kAmrSlots) != hipSuccess ||
       hipMalloc(&A->d_norms, sizeof(double) * kAmrSlots) != hipSuccess ||
       hipMalloc(&A->d_cnorm, sizeof(double) * 2) != hipSuccess || hipMalloc(&A->d_citer, sizeof(int) * 2) != hipSuccess ||
+      hipMalloc(&A->d_snap, sizeof(double) * kSnap * std::max<size_t>(A->lev[0].pool, 1)) != hipSuccess ||
       hipHostMalloc(&A->h_pinned, sizeof(double) * (kAmrSlots + 4)) != hipSuccess)
     return -1;
   (void)hipMemset(A->ws.partials, 0, sizeof(double) * 2 * (size_t)cap * kAmrSlots);
@@ -1732,7 +1735,7 @@ void pmg_amr_destroy(pmg_amr *A) {
   if (A->xtmp) (void)hipFree(A->xtmp);
   if (A->ws.partials) (void)hipFree(A->ws.partials);
   if (A->ws.slot_nb) (void)hipFree(A->ws.slot_nb);
-  for (double *p : {A->d_norms, A->d_cnorm}) if (p) (void)hipFree(p);
+  for (double *p : {A->d_norms, A->d_cnorm, A->d_snap}) if (p) (void)hipFree(p);
   if (A->d_citer) (void)hipFree(A->d_citer);
   if (A->d_fold) (void)hipFree(A->d_fold);
   if (A->h_pinned) (void)hipHostFree(A->h_pinned);

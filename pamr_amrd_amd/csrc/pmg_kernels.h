@@ -7,6 +7,7 @@
 // Poisson, 1 variable coefficient) and on excision.
 #pragma once
 #include "pmg_ops.h"
+#include <type_traits>
 
 namespace pmg {
 
@@ -789,6 +790,7 @@ struct CoarseP {
   double crtol;
   double *norm_pre, *norm_post;  // device outputs (res[1] after pre / final)
   int *citer;                    // device output: iteration count at exit
+  T *snap;                       // a saved state of the grid for cycle_watch (or null)
 };
 
 // q / m and q % m for 0 <= q < 2^22: a float reciprocal estimate, off by
@@ -817,6 +819,57 @@ __device__ __forceinline__ int fdivmod(int q, const FastDiv &d, int &rem) {
 __device__ __forceinline__ void box_ijk(int q, const FastDiv &di, const FastDiv &dj, int &i, int &j, int &k) {
   const int t = fdivmod(q, di, i);
   k = fdivmod(t, dj, j);
+}
+
+// The coarsest loop `while (iter < max_citer && res > ctol) relax` of
+// amrd/mg.c:179-186 can reach a state that the relaxation maps back onto
+// itself after lam sweeps (a fixed point or cycle at rounding level, res
+// still above ctol); the reference then relaxes until max_citer.  A sweep is
+// a deterministic function of u (fixed point order, fixed reduction tree), so
+// from there states and residuals repeat with period lam, and the state and
+// residual at max_citer are those after (max_citer - iter) mod lam further
+// sweeps.  The cycle is found with Brent's method (one saved state, replaced
+// whenever the distance to it reaches a power of two; R. P. Brent, BIT 20,
+// 1980), one bitwise comparison of the grid per sweep, watched from the
+// kWatchStart-th sweep of the loop on.  The outcome is the same bits and the
+// same iteration count as every sweep run.
+constexpr int kSnap = 1;        // saved states (of the coarsest grid)
+constexpr int kWatchStart = 16;  // loop sweeps before the watch starts
+__device__ __forceinline__ bool same_bits(double a, double b) {
+  return __double_as_longlong(a) == __double_as_longlong(b);
+}
+__device__ __forceinline__ bool same_bits(float a, float b) { return __float_as_int(a) == __float_as_int(b); }
+
+struct CycleWatch {
+  int power = 1, lam = 0;
+  bool saved = false;
+};
+
+// called after sweep `iter` (uniform across the block); returns -1 to go on,
+// else the number of further sweeps after which u is the state of sweep
+// max_citer
+template <typename T, int NT>
+__device__ int cycle_watch(const T *u, size_t n, T *snap, CycleWatch &cw, int iter, int max_citer) {
+  auto save = [&]() {
+    for (size_t q = threadIdx.x; q < n; q += NT) snap[q] = u[q];
+    __syncthreads();  // before the next sweep writes u
+  };
+  if (!cw.saved) {
+    save();
+    cw.saved = true;
+    return -1;
+  }
+  cw.lam++;
+  int diff = 0;
+  for (size_t q = threadIdx.x; q < n; q += NT) diff |= !same_bits(u[q], snap[q]);
+  diff = __syncthreads_or(diff);
+  if (!diff) return (max_citer - iter) % cw.lam;
+  if (cw.lam == cw.power) {
+    save();
+    cw.power <<= 1;
+    cw.lam = 0;
+  }
+  return -1;
 }
 
 template <typename T, int OP, bool EX, bool P2, bool LDS, int NT>
@@ -874,9 +927,19 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
   int iter = 0;
   for (iter = 0; iter < P.npst; iter++) res = sweep();
   res = ctol + 1;
+  CycleWatch cw;
+  int wit = 0;
   while (iter < P.max_citer && res > ctol) {
     res = sweep();
     iter++;
+    if (P.snap && ++wit >= kWatchStart && res > ctol && iter < P.max_citer) {
+      const int r = cycle_watch<T, NT>(u, n, P.snap, cw, iter, P.max_citer);
+      if (r >= 0) {
+        for (int k = 0; k < r; k++) res = sweep();
+        iter = P.max_citer;
+        break;
+      }
+    }
   }
   if constexpr (LDS) {
     for (size_t q = threadIdx.x; q < n; q += NT) P.u[q] = u[q];
@@ -1039,6 +1102,227 @@ __global__ __launch_bounds__(NT) void k_resid_restrict(ResRestrP<T> P) {
   norm_partial<NT>(s, cnt, P.ws, P.slot, blockIdx.x, P.nb);
 }
 
+// The same fused residual + full-weight injection with the restriction
+// streamed through the z march: dmfwr3d's 27-term sum for an interior coarse
+// point adds its terms in the order plane km, plane k0, plane kp (within a
+// plane row by row), left to right, so it can be carried in a register per
+// coarse column and extended as each fine plane is formed -- the same
+// additions in the same order, hence the same bits.  Only the current fine
+// residual plane is kept (two LDS slots) and u needs a ring of three planes,
+// and each fine plane costs two barriers instead of three.  Coarse points on
+// the x or y faces (lower-dimensional stencils in another order) take
+// dmfwr3d's point formula on the fine u in global memory, their res is zero
+// (the fine res vanishes on the boundary); the z-boundary planes use the
+// single-plane stencil as the plane passes.
+// ---------------------------------------------------------------------------
+template <typename T, typename V>
+__device__ __forceinline__ T fw_plane_chain(T acc, bool first, const V &v, T c) {
+  // terms (di, dj) in dmfwr3d's order: dj = -1, 0, 1 outer, di = -1, 0, 1
+  // inner, weights c * (1 2 1; 2 4 2; 1 2 1)
+  const T w1 = c, w2 = T(2) * c, w4 = T(4) * c;
+  acc = first ? w1 * v(-1, -1) : acc + w1 * v(-1, -1);
+  acc = acc + w2 * v(0, -1);
+  acc = acc + w1 * v(1, -1);
+  acc = acc + w2 * v(-1, 0);
+  acc = acc + w4 * v(0, 0);
+  acc = acc + w2 * v(1, 0);
+  acc = acc + w1 * v(-1, 1);
+  acc = acc + w2 * v(0, 1);
+  acc = acc + w1 * v(1, 1);
+  return acc;
+}
+
+// Tiles cover the coarse columns 0 .. cnx-2 (and rows 0 .. cny-2): on the
+// usual 2^m + 1 grids that is a whole number of tiles, with no tail tile
+// marching for a single column.  The last column and row are x/y faces
+// (point formula, no march) and go to the tiles at the high edges.
+inline __host__ __device__ int rr_tiles(int cn, int c) { return cn > 1 ? (cn - 1 + c - 1) / c : 1; }
+
+template <typename T, int OP, bool P2, int CX, int CY, int NT, int PF>
+__global__ __launch_bounds__(NT) void k_resid_restrict_s(ResRestrP<T> P) {
+  static_assert(NT >= CX * CY && NT >= CX + CY + 1, "one coarse column per thread");
+  static_assert(PF == 1 || PF == 2, "planes in flight");
+  constexpr int UX = 2 * CX + 3, UY = 2 * CY + 3;  // u tile: fine x in [2cx0-2, 2cx0+2CX]
+  constexpr int RX = 2 * CX + 1, RY = 2 * CY + 1;  // res tile: fine x in [2cx0-1, 2cx0+2CX-1]
+  constexpr int NLU = (UX * UY + NT - 1) / NT, NLR = (RX * RY + NT - 1) / NT;
+  __shared__ T sU[3][UY][UX];
+  __shared__ T sR[2][RY][RX];
+  const int nx = P.nx, ny = P.ny, nz = P.nz;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int ntx = rr_tiles(P.cnx, CX), nty = rr_tiles(P.cny, CY);
+  const int ty = tile % nty, tx = (tile / nty) % ntx, tz = tile / (nty * ntx);
+  const int cx0 = tx * CX, cy0 = ty * CY;
+  const int ck0 = tz * P.czchunk, ck1 = min(ck0 + P.czchunk, P.cnz);
+  const int ux0 = 2 * cx0 - 2, uy0 = 2 * cy0 - 2;
+  const int rx0 = 2 * cx0 - 1, ry0 = 2 * cy0 - 1;
+  double s = 0.0, cnt = 0.0;
+  // u and rho of the next PF fine planes in flight, in registers (buffer
+  // plane % PF)
+  T ldU[PF][NLU], ldR[PF][NLR];
+  auto issue_u = [&](int k, T (&buf)[NLU]) {
+#pragma unroll
+    for (int m = 0; m < NLU; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int lx = q % UX, ly = q / UX;
+      const int i = ux0 + lx, j = uy0 + ly;
+      buf[m] = (q < UX * UY && k >= 0 && k < nz && i >= 0 && i < nx && j >= 0 && j < ny)
+                   ? P.u[ix3(i, j, k, nx, ny)] : T(0);
+    }
+  };
+  auto commit_u = [&](int k, const T (&buf)[NLU]) {
+    T(*S)[UX] = sU[(k + 3) % 3];
+#pragma unroll
+    for (int m = 0; m < NLU; m++) {
+      const int q = threadIdx.x + m * NT;
+      if (q < UX * UY) S[q / UX][q % UX] = buf[m];
+    }
+  };
+  auto issue_rho = [&](int k, T (&buf)[NLR]) {
+    const bool kin = k >= 1 && k <= nz - 2;
+#pragma unroll
+    for (int m = 0; m < NLR; m++) {
+      const int q = threadIdx.x + m * NT;
+      const int i = rx0 + q % RX, j = ry0 + q / RX;
+      buf[m] = (q < RX * RY && kin && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2)
+                   ? P.rho[ix3(i, j, k, nx, ny)] : T(0);
+    }
+  };
+  auto res_plane = [&](int k, const T (&rho)[NLR]) {
+    T(*R)[RX] = sR[k & 1];
+    T(*U0)[UX] = sU[(k + 3) % 3];
+    T(*Um)[UX] = sU[(k + 2) % 3];
+    T(*Up)[UX] = sU[(k + 4) % 3];
+    const bool kin = k >= 1 && k <= nz - 2;
+    const bool kown = k >= 2 * ck0 && k < 2 * ck1;
+#pragma unroll
+    for (int m = 0; m < NLR; m++) {
+      const int q = threadIdx.x + m * NT;
+      if (q >= RX * RY) continue;
+      const int lx = q % RX, ly = q / RX;
+      const int i = rx0 + lx, j = ry0 + ly;
+      T r = T(0);
+      if (kin && i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2) {
+        const int ax = lx + 1, ay = ly + 1;
+        if constexpr (OP == 0) {
+          r = lop_poisson<T, P2>(U0[ay][ax], U0[ay][ax + 1], U0[ay][ax - 1], U0[ay + 1][ax], U0[ay - 1][ax],
+                                 Up[ay][ax], Um[ay][ax], rho[m], P.sp) - T(0);
+        } else {
+          const size_t p = ix3(i, j, k, nx, ny);
+          const size_t sy = nx, sz = (size_t)nx * ny;
+          const T *__restrict__ a = P.a;
+          r = lop_varcoef<T, P2>(U0[ay][ax], U0[ay][ax + 1], U0[ay][ax - 1], U0[ay + 1][ax], U0[ay - 1][ax],
+                                 Up[ay][ax], Um[ay][ax], a[p], a[p + 1], a[p - 1], a[p + sy], a[p - sy],
+                                 a[p + sz], a[p - sz], P.c[p], rho[m], P.sp, nullptr) - T(0);
+        }
+        if (kown && i >= 2 * cx0 && i < 2 * cx0 + 2 * CX && j >= 2 * cy0 && j < 2 * cy0 + 2 * CY) {
+          s += (double)r * (double)r;
+          cnt += 1.0;
+        }
+      }
+      R[ly][lx] = r;
+    }
+  };
+  auto put = [&](int ci, int cj, int kc, T vu, T vr) {
+    const size_t q = ix3(ci, cj, kc, P.cnx, P.cny);
+    P.cres[q] = vr;
+    P.cu[q] = vu;
+    if (ci == 0 || cj == 0 || kc == 0 || ci == P.cnx - 1 || cj == P.cny - 1 || kc == P.cnz - 1) P.cua[q] = vu;
+  };
+  // this thread's coarse column (columns 0 .. cnx-2, rows 0 .. cny-2)
+  const int t = threadIdx.x;
+  const int ci = cx0 + t % CX, cj = cy0 + t / CX;
+  const bool mine = t < CX * CY && ci < P.cnx - 1 && cj < P.cny - 1;
+  const bool col_in = ci >= 1 && cj >= 1;
+  const int lx = 2 * (ci - cx0) + 1, ly = 2 * (cj - cy0) + 1;  // fine centre in the res tile
+  T accU = T(0), accR = T(0);
+  auto restrict_plane = [&](int f) {
+    if (!mine || !col_in) return;
+    T(*Uf)[UX] = sU[(f + 3) % 3];
+    T(*Rf)[RX] = sR[f & 1];
+    auto vu = [&](int di, int dj) -> T { return Uf[ly + 1 + dj][lx + 1 + di]; };
+    auto vr = [&](int di, int dj) -> T { return Rf[ly + dj][lx + di]; };
+    if ((f & 1) == 0) {
+      const int kc = f / 2;
+      if (kc < ck0 || kc >= ck1) return;
+      if (kc == 0 || kc == P.cnz - 1) {  // single-plane stencil of the z faces
+        put(ci, cj, kc, fw_plane_chain<T>(T(0), true, vu, T(1)) / T(16.0),
+            fw_plane_chain<T>(T(0), true, vr, T(1)) / T(16.0));
+      } else {
+        accU = fw_plane_chain<T>(accU, false, vu, T(2));
+        accR = fw_plane_chain<T>(accR, false, vr, T(2));
+      }
+    } else {
+      const int kf = (f - 1) / 2, ks = (f + 1) / 2;  // f = kp of kf, km of ks
+      if (kf >= ck0 && kf < ck1 && kf >= 1 && kf <= P.cnz - 2) {
+        accU = fw_plane_chain<T>(accU, false, vu, T(1));
+        accR = fw_plane_chain<T>(accR, false, vr, T(1));
+        put(ci, cj, kf, accU / T(64.0), accR / T(64.0));
+      }
+      if (ks >= ck0 && ks < ck1 && ks >= 1 && ks <= P.cnz - 2) {
+        accU = fw_plane_chain<T>(T(0), true, vu, T(1));
+        accR = fw_plane_chain<T>(T(0), true, vr, T(1));
+      }
+    }
+  };
+  // fine planes: the km plane of the first coarse plane to the kp plane of
+  // the last (the z faces: the plane itself)
+  const int kf0 = max(2 * ck0 - 1, 0), kf1 = min(2 * ck1 - 1, nz - 1);
+  {
+    T tmp[NLU];
+    issue_u(kf0 - 1, tmp);
+    commit_u(kf0 - 1, tmp);
+    issue_u(kf0, tmp);
+    commit_u(kf0, tmp);
+  }
+  for (int d = 1; d <= PF; d++) {
+    if (PF == 1 || ((kf0 + d) & 1) == 0) issue_u(kf0 + d, ldU[0]);
+    else issue_u(kf0 + d, ldU[PF - 1]);
+  }
+  for (int d = 0; d < PF; d++) {
+    if (PF == 1 || ((kf0 + d) & 1) == 0) issue_rho(kf0 + d, ldR[0]);
+    else issue_rho(kf0 + d, ldR[PF - 1]);
+  }
+  // one step of the march; B = f % PF (a compile-time constant, so the
+  // register buffers are indexed statically)
+  auto step = [&](int f, auto parity) {
+    constexpr int B = decltype(parity)::value, B1 = (B + 1) % PF;
+    // slot of plane f+1 held f-2: last read by the residual of f-1 and the
+    // restriction of f-2, both before the previous barrier
+    commit_u(f + 1, ldU[B1]);
+    T rho[NLR];
+#pragma unroll
+    for (int m = 0; m < NLR; m++) rho[m] = ldR[B][m];
+    if (f + 1 + PF <= kf1 + 1) issue_u(f + 1 + PF, ldU[B1]);
+    if (f + PF <= kf1) issue_rho(f + PF, ldR[B]);
+    __syncthreads();
+    res_plane(f, rho);  // res slot f&1 was last read by the restriction of f-2
+    __syncthreads();
+    restrict_plane(f);
+  };
+  for (int f = kf0; f <= kf1; f++) {
+    if (PF == 2 && (f & 1)) step(f, std::integral_constant<int, (PF == 2 ? 1 : 0)>{});
+    else step(f, std::integral_constant<int, 0>{});
+  }
+  // x/y-face columns: dmfwr3d's point formula on the fine u, after the march
+  // (it reads global memory and would otherwise hold registers through it):
+  // columns 0 and rows 0 of the tile, the last column (cnx-1) on the tiles of
+  // the high x edge, the last row (cny-1) on those of the high y edge
+  auto put_face = [&](int fi, int fj) {
+    for (int kc = ck0; kc < ck1; kc++)
+      put(fi, fj, kc, fw_point<T>(P.u, nx, ny, nz, 2 * fi + 1, 2 * fj + 1, 2 * kc + 1), T(0));
+  };
+  if (mine && !col_in) put_face(ci, cj);
+  if (tx == ntx - 1 && t <= CY) {
+    const int fj = t < CY ? cy0 + t : P.cny - 1;
+    if ((t < CY && fj < P.cny - 1) || (t == CY && ty == nty - 1)) put_face(P.cnx - 1, fj);
+  }
+  if (ty == nty - 1 && t > CY && t <= CY + CX) {
+    const int fi = cx0 + t - CY - 1;
+    if (fi < P.cnx - 1) put_face(fi, P.cny - 1);
+  }
+  norm_partial<NT>(s, cnt, P.ws, P.slot, blockIdx.x, P.nb);
+}
+
 // ---------------------------------------------------------------------------
 // Bottom of the V-cycle in one workgroup.  Levels 1..nb (all small) are
 // processed from L_op + compute_MG_rhs on the top bottom level, through its
@@ -1077,6 +1361,7 @@ struct BottomP {
   int resid_slot0;  // level L's residual norm -> slot resid_slot0 + L - 1
   double *norm_pre, *norm_post;
   int *citer;
+  T *snap;  // a saved state of the coarsest level for cycle_watch (or null)
 };
 
 // LDS: every array of the bottom levels (u, rho, rhs, res, lop, and a, c for
@@ -1253,10 +1538,24 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
     int iter;
     for (iter = 0; iter < P.npst; iter++) { sweep(L, s, cnt); res = put_norm(P.relax_slot0, s, cnt); }
     res = ctol + 1;
+    CycleWatch cw;
+    int wit = 0;
+    const size_t n0 = (size_t)L.nx * L.ny * L.nz;
     while (iter < P.max_citer && res > ctol) {
       sweep(L, s, cnt);
       res = put_norm(P.relax_slot0, s, cnt);
       iter++;
+      if (P.snap && ++wit >= kWatchStart && res > ctol && iter < P.max_citer) {
+        const int r = cycle_watch<T, NT>(L.u, n0, P.snap, cw, iter, P.max_citer);
+        if (r >= 0) {
+          for (int k = 0; k < r; k++) {
+            sweep(L, s, cnt);
+            res = put_norm(P.relax_slot0, s, cnt);
+          }
+          iter = P.max_citer;
+          break;
+        }
+      }
     }
     if (threadIdx.x == 0) {
       if (P.norm_post) P.norm_post[0] = res;

@@ -129,6 +129,7 @@ struct pmg_solver {
   NormWs ws{nullptr, nullptr, 0, 0};
   double *d_norms = nullptr;  // folded norms: FP64 slots, FP32 slots, coarse-solve outputs
   int *d_citer = nullptr;     // 2 ints (FP64 cycle, FP32 cycle)
+  double *d_snap = nullptr;   // saved state of level 1 for the coarsest loop's cycle watch
   double *h_pinned = nullptr; // pinned readback
   int sweep_variant = 1, zchunk = 32;
   int lv_variant[2][kMaxLev], lv_zchunk[2][kMaxLev];  // per-level sweep configuration, [0] FP64, [1] FP32
@@ -136,7 +137,7 @@ struct pmg_solver {
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   int bottom_max_n = 17;  // ... those with at most this many points per side
   int bottom_lds = 1;     // ... with their arrays in LDS when they fit
-  int use_fused_restrict = 1;  // finest residual + injection in one pass
+  int use_fused_restrict = 2;  // finest residual + injection in one pass (1: res ring; 2, 3: restriction streamed in z)
   int rr_czchunk[2] = {8, 8};   // its coarse planes per block, [0] FP64, [1] FP32
   int use_sweep2 = 0;          // pairs of sweeps temporally blocked on large levels
                                // (measured slower than single sweeps; off by default)
@@ -556,6 +557,7 @@ int coarse_solve(pmg_solver *S, int L, double *norm_pre, double *norm_post, int 
   P.sp = spacing<T>(Lv);
   P.npre = S->pre; P.npst = S->pst; P.max_citer = max_citer; P.crtol = S->crtol;
   P.norm_pre = norm_pre; P.norm_post = norm_post; P.citer = citer;
+  P.snap = L == 1 ? reinterpret_cast<T *>(S->d_snap) : nullptr;
   const int narr = (S->op == 1) ? 5 : 3;
   const size_t lds = Lv.size * narr * sizeof(T);
   const bool use_lds = lds <= 60 * 1024;
@@ -676,6 +678,7 @@ int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
   P.relax_slot0 = norm_base + NS_RELAX;
   P.resid_slot0 = norm_base + NS_RESID;
   P.norm_pre = cnorm; P.norm_post = cnorm + 1; P.citer = citer;
+  P.snap = reinterpret_cast<T *>(S->d_snap);
   const bool p2 = P.lev[0].sp.pow2 != 0;
   for (int L = 2; L <= Lb; L++)
     if ((P.lev[L - 1].sp.pow2 != 0) != p2) return -5;
@@ -714,21 +717,29 @@ bool can_fuse_resid_restrict(const pmg_solver *S) {
 }
 
 // workgroups of the fused kernel for a coarse level of cn points and a
-// given number of coarse planes per workgroup
+// given number of coarse planes per workgroup: the ring kernel tiles every
+// coarse column, the streamed one columns 0 .. cn-2 (never more workgroups)
 template <typename T>
-int resid_restrict_blocks(const int *cn, int czchunk) {
+int resid_restrict_blocks(const int *cn, int czchunk, bool streamed = false) {
   constexpr int CX = std::is_same<T, double>::value ? 16 : 32, CY = 8;
-  return ((cn[0] + CX - 1) / CX) * ((cn[1] + CY - 1) / CY) * ((cn[2] + czchunk - 1) / czchunk);
+  const int tx = streamed ? rr_tiles(cn[0], CX) : (cn[0] + CX - 1) / CX;
+  const int ty = streamed ? rr_tiles(cn[1], CY) : (cn[1] + CY - 1) / CY;
+  return tx * ty * ((cn[2] + czchunk - 1) / czchunk);
 }
 
 template <typename T, int OP, bool P2>
 int launch_resid_restrict_t(pmg_solver *S, ResRestrP<T> P) {
   constexpr int CX = std::is_same<T, double>::value ? 16 : 32, CY = 8, NT = 256;
   const int cn[3] = {P.cnx, P.cny, P.cnz};
-  const dim3 grid(resid_restrict_blocks<T>(cn, P.czchunk));
+  const dim3 grid(resid_restrict_blocks<T>(cn, P.czchunk, S->use_fused_restrict >= 2));
   P.nb = (int)grid.x;
   if (P.nb > S->ws.cap) return -3;
-  hipLaunchKernelGGL((k_resid_restrict<T, OP, P2, CX, CY, NT>), grid, dim3(NT), 0, S->stream, P);
+  if (S->use_fused_restrict == 2)
+    hipLaunchKernelGGL((k_resid_restrict_s<T, OP, P2, CX, CY, NT, 1>), grid, dim3(NT), 0, S->stream, P);
+  else if (S->use_fused_restrict == 3)
+    hipLaunchKernelGGL((k_resid_restrict_s<T, OP, P2, CX, CY, NT, 2>), grid, dim3(NT), 0, S->stream, P);
+  else
+    hipLaunchKernelGGL((k_resid_restrict<T, OP, P2, CX, CY, NT>), grid, dim3(NT), 0, S->stream, P);
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }
@@ -933,12 +944,12 @@ pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidt
       S->lv_zchunk[pi][L - 1] = n >= 257 ? 32 : (n >= 129 ? 16 : (n >= 65 ? 8 : 4));
     }
   }
-  // coarse planes per block of the fused finest residual + injection: on
-  // coarse levels of 129 planes and more, 5 (FP64) and 12 (FP32) -- fastest at
-  // 257^3 (12: one round of workgroups on the 256 CUs); smaller grids keep 8
-  if (nl >= 2 && S->lev[nl - 2].n[2] >= 129) {
-    S->rr_czchunk[0] = 5;
-    S->rr_czchunk[1] = 12;
+  // coarse planes per block of the fused finest residual + injection
+  // (streamed kernel), the fastest measured: 6 (FP64) and 4 (FP32) on a
+  // coarse level of 129 planes, 8 and 8 from 257 planes; smaller grids 8
+  if (nl >= 2 && S->lev[nl - 2].n[2] >= 129 && S->lev[nl - 2].n[2] < 257) {
+    S->rr_czchunk[0] = 6;
+    S->rr_czchunk[1] = 4;
   }
   // norm workspace sized for the largest launch
   // partial-norm capacity: the largest launch over the finest level
@@ -952,6 +963,7 @@ pmg_solver *pmg_create(int nx, int ny, int nz, const double *bbox, int min_cwidt
       hipMalloc(&S->ws.slot_nb, sizeof(int) * S->ws.nslots) != hipSuccess ||
       hipMalloc(&S->d_norms, sizeof(double) * kNormTotal) != hipSuccess ||
       hipMalloc(&S->d_citer, sizeof(int) * 2) != hipSuccess ||
+      hipMalloc(&S->d_snap, sizeof(double) * kSnap * S->lev[0].size) != hipSuccess ||
       hipHostMalloc(&S->h_pinned, sizeof(double) * (kNormTotal + 4)) != hipSuccess) {
     pmg_destroy(S);
     return nullptr;
@@ -987,6 +999,7 @@ void pmg_destroy(pmg_solver *S) {
   if (S->ws.slot_nb) (void)hipFree(S->ws.slot_nb);
   if (S->d_norms) (void)hipFree(S->d_norms);
   if (S->d_citer) (void)hipFree(S->d_citer);
+  if (S->d_snap) (void)hipFree(S->d_snap);
   if (S->h_pinned) (void)hipHostFree(S->h_pinned);
   if (S->d_hist) (void)hipFree(S->d_hist);
   if (S->h_hist) (void)hipHostFree(S->h_hist);
@@ -1094,8 +1107,8 @@ int pmg_set_sweep_blocking(pmg_solver *S, int on, int zchunk) {
 }
 
 int pmg_set_restrict_fusion(pmg_solver *S, int on) {
-  if (!S) return -1;
-  S->use_fused_restrict = on ? 1 : 0;
+  if (!S || on < 0 || on > 3) return -1;
+  S->use_fused_restrict = on;
   invalidate_graphs(S);
   return 0;
 }

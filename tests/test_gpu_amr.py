@@ -232,6 +232,43 @@ def test_sweep_config_rejects(torch_cuda):
 
 
 @pytest.mark.parametrize("op", [0, 1])
+def test_coarsest_cycle_skip_bit_exact(torch_cuda, op):
+    """66 cycles on a 33^3-base hierarchy: from about cycle 60 the coarsest
+    loop cannot reach crtol and the reference relaxes to max_citer; the
+    periodic states are skipped -- the same iteration counts every cycle and
+    u bit for bit on every grid against the oracle"""
+    levels = layout(5, base_n=33, nmin=4, nmax=6, smin=9, smax=17)
+    print(f"case amr coarsest skip op={op} boxes={[len(l) for l in levels]}", flush=True)
+    m = O.AmrMG(33, levels, op=op)
+    a = pm.Amr(33, levels, op=op)
+    for L in range(m.base, m.nlev + 1):
+        for g in range(m.ngrids(L)):
+            shape, bb = m.grid(L, g)
+            vals = {O.OA_RHO: -H.gaussian_on(shape, bb, seed=77)}
+            if op == 1:
+                vals[O.OA_A] = H.lognormal_on(shape, bb)
+                vals[O.OA_C] = H.uniform_on(shape, bb)
+            for w, v in vals.items():
+                m.field(L, g, w)[:] = v
+                a.upload(L, g, FIELD[w], v)
+    for x in (m, a):
+        x.set_params()
+        x.init()
+    cits = []
+    for c in range(66):
+        rm, nm, cm = m.vcycle()
+        ra, na, ca = a.vcycle()
+        assert cm == ca, (c, cm, ca)
+        assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+        cits.append(ca)
+    print(f"coarsest iterations {cits}", flush=True)
+    assert cits[-1] == 5000
+    for L, g in each_grid(m):
+        assert np.array_equal(a.download(L, g, pm.PMG_F_U), m.field(L, g, O.OA_U)), ("u", L, g)
+    a.close()
+
+
+@pytest.mark.parametrize("op", [0, 1])
 def test_mixed_precision_amr_reaches_1e10(torch_cuda, op):
     """FP32 V-cycles of the hierarchy in FP64 defect correction
     (pmg_amr_solve_rel mode 1) reach a 1e-10 relative FP64 residual.  The

@@ -150,11 +150,13 @@ def _solver_only(pm, n, rho, pre, pst):
     return S
 
 
+@pytest.mark.parametrize("kind", [1, 2, 3])
 @pytest.mark.parametrize("n,op", [(33, 0), (65, 0), (129, 0), (65, 1), (97, 1)])
-def test_restrict_fusion_matches_unfused(pm, n, op):
+def test_restrict_fusion_matches_unfused(pm, n, op, kind):
     """the finest residual computed inside the injection gives the same
-    coarse u and res, the same cycles and the same u as residual + dmfwr3d"""
-    print(f"case restrict fusion n={n} op={op}", flush=True)
+    coarse u and res, the same cycles and the same u as residual + dmfwr3d
+    (kind 1: residual planes in an LDS ring; 2: restriction streamed in z)"""
+    print(f"case restrict fusion n={n} op={op} kind={kind}", flush=True)
     rho = -pm.gaussian_source(n, seed=6)
     a = c = None
     if op == 1:
@@ -163,6 +165,7 @@ def test_restrict_fusion_matches_unfused(pm, n, op):
         c = rng.random((n, n, n))
     Sf, _ = _pair(pm, n, rho, op=op, a=a, c=c)
     Su, _ = _pair(pm, n, rho, op=op, a=a, c=c)
+    Sf.set_restrict_fusion(kind)
     Su.set_restrict_fusion(False)
     for _ in range(3):
         rf, nf, cf = Sf.vcycle()
@@ -188,14 +191,17 @@ def test_restrict_fusion_matches_unfused(pm, n, op):
     Su.close()
 
 
+@pytest.mark.parametrize("kind", [1, 2, 3])
 @pytest.mark.parametrize("cz,n", [(1, 65), (3, 65), (5, 65), (16, 65), (2, 257), (13, 257)])
-def test_resid_restrict_zchunk_bit_exact(pm, cz, n):
-    """every z-chunk of the fused finest residual + injection gives the same
-    cycles (FP64) and the same mixed-precision history (FP32) as the default"""
-    print(f"case resid_restrict zchunk {cz} n={n}", flush=True)
+def test_resid_restrict_zchunk_bit_exact(pm, cz, n, kind):
+    """every z-chunk of the fused finest residual + injection (both kernels)
+    gives the same cycles (FP64) and the same mixed-precision history (FP32)
+    as the default"""
+    print(f"case resid_restrict zchunk {cz} n={n} kind={kind}", flush=True)
     rho = -pm.gaussian_source(n, seed=11)
     Sd, _ = _pair(pm, n, rho)
     Sz, _ = _pair(pm, n, rho)
+    Sz.set_restrict_fusion(kind)
     Sz.set_resid_restrict_zchunk(0, cz)
     Sz.set_resid_restrict_zchunk(1, cz)
     for _ in range(3):
@@ -240,6 +246,38 @@ def test_bottom_fusion_matches_unfused(pm, n):
     assert abs(len(hf) - len(hu)) <= 1
     Sf.close()
     Su.close()
+
+
+@pytest.mark.parametrize("op,fused", [(0, True), (0, False), (1, True), (1, False)])
+def test_coarsest_cycle_skip_bit_exact(pm, op, fused):
+    """once the solution is converged to rounding, the coarsest loop cannot
+    reach crtol and the reference relaxes to max_citer: the periodic states
+    are skipped (same iteration count, same u, same norms) -- 20 cycles at
+    33^3 against the oracle, with the bottom fused and with per-level
+    launches"""
+    n = 33
+    print(f"case coarsest cycle skip op={op} fused={fused}", flush=True)
+    rho = -pm.gaussian_source(n, seed=5)
+    a = c = None
+    if op == 1:
+        rng = np.random.default_rng(3)
+        a = np.exp(0.5 * rng.standard_normal((n, n, n)))
+        c = rng.random((n, n, n))
+    S, m = _pair(pm, n, rho, op=op, a=a, c=c)
+    S.set_bottom_fusion(fused)
+    cits = []
+    for k in range(20):
+        rm, nm, cm = m.vcycle()
+        rs, ns, cs = S.vcycle()
+        assert cm == cs, (k, cm, cs)
+        assert abs(rs - rm) <= 1e-12 * rm, (k, rs, rm)
+        assert np.allclose(ns, nm, rtol=1e-12, atol=0), (k, ns, nm)
+        cits.append(cs)
+    print(f"coarsest iterations {cits}", flush=True)
+    assert cits[-1] == 5000
+    for L in range(1, S.nlev + 1):
+        assert np.array_equal(S.download(L, pm.PMG_F_U), m.field(L, O.F_U)), L
+    S.close()
 
 
 @pytest.mark.parametrize("op", [0, 1])
