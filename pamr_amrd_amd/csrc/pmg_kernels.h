@@ -872,6 +872,135 @@ __device__ int cycle_watch(const T *u, size_t n, T *snap, CycleWatch &cw, int it
   return -1;
 }
 
+// The coarsest loop by one wavefront.  When the relax box has at most 64
+// points, the block kernels give each point to lane q of wave 0 and every
+// other wave contributes zero to the norm reduction, whose first stage is the
+// wave's shuffle tree; so one wave running the same sweeps and the same tree
+// produces the same bits, without a block barrier per half sweep.  Colour
+// passes are ordered by waiting for every outstanding memory operation of
+// the wave (the arrays are in LDS or in the wave's own global writes).
+constexpr int kWaveCoarseMax = 512;  // grid points staged for the wave path
+template <typename T>
+struct WaveCoarse {
+  T *u;
+  const T *rho, *rhs, *a, *c;
+  const uint8_t *chr;
+  int nx, ny, bx0, by0, bz0, mi, mj, npts;
+  Spacing<T> sp;
+  int npre, npst, max_citer;
+  double crtol;
+  double *slot;  // norm partial (sum, count) written after every sweep, or null
+  int *slot_nb;
+  T *snap;       // a saved state for the cycle watch, or null
+  size_t n;      // grid points (for the watch)
+};
+
+// a wavefront-scope fence orders the wave's memory operations for the
+// compiler (the hardware keeps a wave's accesses in order); the wait makes
+// every earlier store of the wave complete before any later load issues
+__device__ __forceinline__ void wave_wait_all() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0);
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+}
+
+// one sweep by lanes 0..63 of wave 0; returns the RMS of the residuals
+// (lane-uniform) and writes the norm partial
+template <typename T, int OP, bool EX, bool P2, bool IJAC>
+__device__ double wave_sweep(const WaveCoarse<T> &W) {
+  const int lane = (int)threadIdx.x;
+  double s = 0.0, cnt = 0.0;
+  const size_t sy = W.nx, sz = (size_t)W.nx * W.ny;
+  const FastDiv dmi = make_fdiv(W.mi), dmj = make_fdiv(W.mj);
+  for (int pass = 0; pass <= 1; pass++) {
+    if (lane < W.npts) {
+      int i, j, k;
+      box_ijk(lane, dmi, dmj, i, j, k);
+      i += W.bx0; j += W.by0; k += W.bz0;
+      const size_t p = ix3(i, j, k, W.nx, W.ny);
+      if (((i + j + k + 3 + pass) & 1) == 0 && !(EX && W.chr[p])) {
+        T jac;
+        const T r = lop_at<T, OP, P2>(W.u, W.rho, W.a, W.c, p, sy, sz, W.sp, &jac) - W.rhs[p];
+        if constexpr (IJAC) W.u[p] = W.u[p] - r * W.sp.ijac;
+        else W.u[p] = W.u[p] - r / jac;
+        s += (double)r * (double)r;
+        cnt += 1.0;
+      }
+    }
+    wave_wait_all();
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    s += __shfl_down(s, off, kWave);
+    cnt += __shfl_down(cnt, off, kWave);
+  }
+  double res = 0.0;
+  if (lane == 0) {
+    res = sqrt(s / cnt);
+    if (W.slot) {
+      W.slot[0] = s;
+      W.slot[1] = cnt;
+      *W.slot_nb = 1;
+    }
+  }
+  return __shfl(res, 0, kWave);
+}
+
+// cycle_watch for one wave (the saved state in global memory; each lane
+// reads back only what it wrote)
+template <typename T>
+__device__ int cycle_watch_wave(const WaveCoarse<T> &W, CycleWatch &cw, int iter) {
+  const int lane = (int)threadIdx.x;
+  auto save = [&]() {
+    for (size_t q = lane; q < W.n; q += kWave) W.snap[q] = W.u[q];
+    wave_wait_all();
+  };
+  if (!cw.saved) {
+    save();
+    cw.saved = true;
+    return -1;
+  }
+  cw.lam++;
+  int diff = 0;
+  for (size_t q = lane; q < W.n; q += kWave) diff |= !same_bits(W.u[q], W.snap[q]);
+  if (!__any(diff)) return (W.max_citer - iter) % cw.lam;
+  if (cw.lam == cw.power) {
+    save();
+    cw.power <<= 1;
+    cw.lam = 0;
+  }
+  return -1;
+}
+
+// pre sweeps, post sweeps and the loop of amrd/mg.c:135-141, 161-190 at the
+// coarsest level, by wave 0; returns (lane-uniform) the final residual and
+// sets iter and the pre-loop residual
+template <typename T, int OP, bool EX, bool P2, bool IJAC>
+__device__ double wave_coarsest(const WaveCoarse<T> &W, int &iter_out, double &res_pre) {
+  double res = 0.0;
+  for (int it = 0; it < W.npre; it++) res = wave_sweep<T, OP, EX, P2, IJAC>(W);
+  const double ctol = res * W.crtol;
+  res_pre = res;
+  int iter;
+  for (iter = 0; iter < W.npst; iter++) res = wave_sweep<T, OP, EX, P2, IJAC>(W);
+  res = ctol + 1;
+  CycleWatch cw;
+  int wit = 0;
+  while (iter < W.max_citer && res > ctol) {
+    res = wave_sweep<T, OP, EX, P2, IJAC>(W);
+    iter++;
+    if (W.snap && ++wit >= kWatchStart && res > ctol && iter < W.max_citer) {
+      const int r = cycle_watch_wave<T>(W, cw, iter);
+      if (r >= 0) {
+        for (int k = 0; k < r; k++) res = wave_sweep<T, OP, EX, P2, IJAC>(W);
+        iter = W.max_citer;
+        break;
+      }
+    }
+  }
+  iter_out = iter;
+  return res;
+}
+
 template <typename T, int OP, bool EX, bool P2, bool LDS, int NT>
 __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
   extern __shared__ unsigned char smem[];
@@ -920,6 +1049,25 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
     __syncthreads();
     return bres;
   };
+  if constexpr (LDS) {
+    if (npts <= kWave) {  // one wave, same bits (see wave_coarsest)
+      if (threadIdx.x < kWave) {
+        WaveCoarse<T> W{u, rho, rhs, a, c, P.chr, nx, ny, P.bx0, P.by0, P.bz0, mi, mj, npts, P.sp,
+                        P.npre, P.npst, P.max_citer, P.crtol, nullptr, nullptr, P.snap, n};
+        int iter;
+        double rpre;
+        const double res = wave_coarsest<T, OP, EX, P2, false>(W, iter, rpre);
+        if (threadIdx.x == 0) {
+          if (P.norm_pre) P.norm_pre[0] = rpre;
+          if (P.norm_post) P.norm_post[0] = res;
+          if (P.citer) P.citer[0] = iter;
+        }
+      }
+      __syncthreads();
+      for (size_t q = threadIdx.x; q < n; q += NT) P.u[q] = u[q];
+      return;
+    }
+  }
   double res = 0.0;
   for (int it = 0; it < P.npre; it++) res = sweep();
   const double ctol = res * P.crtol;
@@ -1528,7 +1676,53 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
     __syncthreads();
   }
   // coarsest level (amrd/mg.c:135-141, 161-190 at L = 1)
-  {
+  constexpr bool IJAC = std::is_same<T, float>::value && OP == 0;
+  __shared__ T cb[LDS ? 1 : (OP == 1 ? 5 : 3) * kWaveCoarseMax];
+  int cmi, cmj, cmk;
+  const int cnpts = box_pts(lv[0], cmi, cmj, cmk);
+  const size_t cn0 = (size_t)lv[0].nx * lv[0].ny * lv[0].nz;
+  if (cnpts <= kWave && cn0 <= (size_t)kWaveCoarseMax) {
+    // one wave, same bits (see wave_coarsest); staged into LDS unless there
+    const BotLev<T> &L = lv[0];
+    lop_rhs(L);
+    T *wu = L.u;
+    const T *wrho = L.rho, *wrhs = L.rhs, *wa = L.a, *wc = L.c;
+    if constexpr (!LDS) {
+      T *cu = cb, *crho = cb + cn0, *crhs = cb + 2 * cn0, *ca = nullptr, *cc = nullptr;
+      if constexpr (OP == 1) {
+        ca = cb + 3 * cn0;
+        cc = cb + 4 * cn0;
+      }
+      for (size_t q = threadIdx.x; q < cn0; q += NT) {
+        cu[q] = L.u[q];
+        crho[q] = L.rho[q];
+        crhs[q] = L.rhs[q];
+        if constexpr (OP == 1) {
+          ca[q] = L.a[q];
+          cc[q] = L.c[q];
+        }
+      }
+      __syncthreads();
+      wu = cu; wrho = crho; wrhs = crhs; wa = ca; wc = cc;
+    }
+    if (threadIdx.x < kWave) {
+      WaveCoarse<T> W{wu, wrho, wrhs, wa, wc, nullptr, L.nx, L.ny, L.bx0, L.by0, L.bz0, cmi, cmj, cnpts, L.sp,
+                      P.npre, P.npst, P.max_citer, P.crtol,
+                      P.ws.partials + (size_t)P.relax_slot0 * P.ws.cap * 2, P.ws.slot_nb + P.relax_slot0,
+                      P.snap, cn0};
+      int iter;
+      double rpre;
+      const double res = wave_coarsest<T, OP, false, P2, IJAC>(W, iter, rpre);
+      if (threadIdx.x == 0) {
+        if (P.norm_pre) P.norm_pre[0] = rpre;
+        if (P.norm_post) P.norm_post[0] = res;
+        if (P.citer) P.citer[0] = iter;
+      }
+      if constexpr (!LDS)
+        for (size_t q = threadIdx.x; q < cn0; q += kWave) L.u[q] = wu[q];
+    }
+    __syncthreads();
+  } else {
     const BotLev<T> &L = lv[0];
     lop_rhs(L);
     double res = 0.0;
