@@ -149,7 +149,9 @@ int pmg_residual(pmg_solver *S, double *norm_out);
 int pmg_solve(pmg_solver *S, int min_iter, int max_iter, double tol, double *norms, int *iters);
 /* relative solve ||r_k|| <= rtol ||r_0||; mode 0 FP64 V-cycles, 1 FP32
  * inner V-cycles in FP64 defect correction, 2 as 1 with the learned
- * correction (pmg_set_net; -1 on a solver with excision); hist[k] = ||r_{k+1}||.  r0_out: on entry, a
+ * correction (pmg_set_net; -1 on a solver with excision), 3 as 1 with the
+ * learned band correction next to the excised region (pmg_band_setup,
+ * pmg_set_band_net); hist[k] = ||r_{k+1}||.  r0_out: on entry, a
  * value > 0 is taken as the reference norm instead of ||r_0|| of the
  * initial u (warm starts measure against the residual of u = 0); on exit
  * the reference norm used. */
@@ -275,6 +277,82 @@ int pmg_set_solution_lincomb(pmg_solver *S, const double *a, const double *b, do
  * pmg_init afterwards, as after any new initial guess. */
 int pmg_history_warm_start(pmg_solver *S, int K, const double *const *F, const double *const *U, double *c_out,
                            double *rel_out);
+/* ---- 4b. learned correction next to excised regions (no reference
+ * counterpart).  With excision the reference's transfers (renormalised
+ * restriction, excised interpolation, clear_ex: src/m_util_r8.f:835-949,
+ * 368-421, src/transfer.c:930-935) leave the coarse-grid correction short
+ * in a thin layer of fine points next to each excised region.  The network
+ * corrects u only there: the "band" of updatable points within Euclidean
+ * distance `depth` (grid points) of an excised point, covered by cubic
+ * cores; a correction gathers a crop of side core + 6 around each core that
+ * holds band points, runs the network on the crops as one batch, and adds
+ * ts * ||r|| * N(r / ||r||) to u at the band points of each core. */
+/* band mask on the host (no GPU): band[p] = 1 at interior, non-excised
+ * points (chr != ex) within distance depth of an excised point; returns the
+ * number of band points (-1 on bad arguments) */
+long long pmg_band_mask(const double *chr, int nx, int ny, int nz, double ex, int depth, unsigned char *band);
+/* crop origins (3 ints each, core corner - margin) of the side-`core` cubes
+ * tiling the grid from index 0 that hold band points; returns their number */
+int pmg_band_cores(const unsigned char *band, int nx, int ny, int nz, int core, int margin, int *org, int cap);
+/* band plan of a solver with excision, from its finest level's chr (set
+ * pmg_set_chr first); returns the number of cores (or -1) */
+int pmg_band_setup(pmg_solver *S, int depth, int core);
+int pmg_band_info(pmg_solver *S, long long *npts, int *ncore, int *side);
+int pmg_band_download(pmg_solver *S, unsigned char *host);
+/* attach the network (NULL detaches): use_dc enables it in pmg_solve_rel
+ * mode 3, use_fgmres in pmg_solve_fgmres's preconditioner.  Mode 3 judges
+ * every iteration by the FP64 residual and drops the correction for the rest
+ * of the solve when an iteration's contraction per unit time is worse than
+ * the plain iteration's; FGMRES compares whole solves (the first solve runs
+ * plain to measure the plain rate; after a worse solve with the correction
+ * the next four run plain). */
+int pmg_set_band_net(pmg_solver *S, pmg_net *N, double ts, int use_dc, int use_fgmres);
+/* accepted / rejected trials; out6 = plain and corrected contraction and ms
+ * per defect-correction iteration, plain and corrected FGMRES rates
+ * (log10 of the residual reduction per ms) */
+int pmg_band_stats(pmg_solver *S, int *accepted, int *rejected, double *out6);
+/* training crops (device fp32 [ncrop][side^3] each) centred on seeded random
+ * band points: x = xs * (the finest FP32 source, -r in defect correction),
+ * t = ts * (ustar - u), w = 1 at band points, 0 elsewhere (zero outside the
+ * grid); ustar device FP64 of the finest level's shape */
+int pmg_band_sample(pmg_solver *S, const double *ustar, double xs, double ts, int ncrop, unsigned long long seed,
+                    float *x, float *t, float *w);
+/* one correction now: which 0 the defect-correction form (needs the state
+ * a mode-1/3 pmg_solve_rel leaves: FP32 source -r, its norm), 1 the
+ * preconditioner form (FP32 u against the FP32 source) */
+int pmg_band_correct(pmg_solver *S, int which);
+/* FP32 arrays of the mixed-precision cycles: which 0 u, 1 rho (the source),
+ * 2 rhs, 3 res, 4 lop (download); 0 and 1 (upload) */
+int pmg_download32(pmg_solver *S, int L, int which, float *host);
+int pmg_upload32(pmg_solver *S, int L, int which, const float *host);
+/* weighted forms of pmg_net_loss_grad / pmg_net_train_step: the mean runs
+ * over points with w != 0 (device fp32 [nb][P]) past the margin; w NULL in
+ * the train step = every point past the margin */
+int pmg_net_loss_grad_w(pmg_net *N, const float *x, const float *t, const float *w, int nx, int ny, int nz,
+                        int nb, int margin, double *loss, double *gw1, double *gb1, double *gw2, double *gb2,
+                        double *gw3, double *gb3);
+int pmg_net_train_step_w(pmg_net *N, const float *x, const float *t, const float *w, int nx, int ny, int nz,
+                         int nb, int margin, float lr, double *loss);
+/* y = N(scale * x) on nb boxes, enqueued on `stream` without
+ * synchronisation; returns the network's output buffer (fp32 [nb][P], valid
+ * until its next inference) or NULL */
+float *pmg_net_infer_async(pmg_net *N, const float *x, float scale, int nx, int ny, int nz, int nb, void *stream);
+/* device launchers of the band correction on raw pointers (org: device
+ * crop origins): gather out = mul / *norm * src (mul alone when norm is
+ * NULL), zero outside the grid; scatter u += mul * *norm * y at band points
+ * of each core (into u and u2, or into u32 when u is NULL); resid: -L64[u]
+ * in FP32 at the updatable points of each core grown by one point */
+int pmg_band_gather_f32(const float *src, int nx, int ny, int nz, const int *org, int ncrop, int side, float mul,
+                        const double *norm, float *out, void *stream);
+int pmg_band_scatter(double *u, double *u2, float *u32, const float *y, const unsigned char *band, int nx, int ny,
+                     int nz, const int *org, int ncrop, int side, int margin, double mul, const double *norm,
+                     void *stream);
+int pmg_band_resid(int op, int do_ex, const double *u, const double *rho, const double *a, const double *c,
+                   const unsigned char *chr, float *neg32, int nx, int ny, int nz, const int *org, int ncrop, int side,
+                   int margin, double dx, double dy, double dz, void *stream);
+int pmg_band_pairs(const float *xsrc, float xs, const double *ustar, const double *u, double ts,
+                   const unsigned char *band, int nx, int ny, int nz, const int *org, int ncrop, int side, float *xo,
+                   float *to, float *wo, void *stream);
 /* lane-layout self-test of the BF16 16x16x16 MFMA used by the network:
  * D = bf16(A) * bf16(B) (host 16x16 row-major) */
 int pmg_net_mfma_selftest(const float *A, const float *B, float *D_out);

@@ -112,6 +112,24 @@ def _declare(L):
     L.pmg_set_solution_lincomb.argtypes = [_vp, _vp, _vp, d, d]
     L.pmg_solve_fgmres.argtypes = [_vp, i, d, i, _dp, _ip, _dp]
     L.pmg_history_warm_start.argtypes = [_vp, i, C.POINTER(_vp), C.POINTER(_vp), _dp, _dp]
+    # learned band correction next to excised regions
+    u8p = C.POINTER(C.c_ubyte)
+    L.pmg_band_mask.argtypes = [_dp, i, i, i, d, i, u8p]
+    L.pmg_band_mask.restype = C.c_longlong
+    L.pmg_band_cores.argtypes = [u8p, i, i, i, i, i, _ip, i]
+    L.pmg_band_setup.argtypes = [_vp, i, i]
+    L.pmg_band_info.argtypes = [_vp, C.POINTER(C.c_longlong), _ip, _ip]
+    L.pmg_band_download.argtypes = [_vp, u8p]
+    L.pmg_set_band_net.argtypes = [_vp, _vp, d, i, i]
+    L.pmg_band_stats.argtypes = [_vp, _ip, _ip, _dp]
+    L.pmg_band_sample.argtypes = [_vp, _vp, d, d, i, C.c_ulonglong, _vp, _vp, _vp]
+    L.pmg_band_correct.argtypes = [_vp, i]
+    L.pmg_download32.argtypes = [_vp, i, i, C.POINTER(C.c_float)]
+    L.pmg_upload32.argtypes = [_vp, i, i, C.POINTER(C.c_float)]
+    L.pmg_net_loss_grad_w.argtypes = [_vp, _vp, _vp, _vp, i, i, i, i, i, _dp] + [_dp] * 6
+    L.pmg_net_train_step_w.argtypes = [_vp, _vp, _vp, _vp, i, i, i, i, i, C.c_float, _dp]
+    L.pmg_net_infer_async.argtypes = [_vp, _vp, C.c_float, i, i, i, i, _vp]
+    L.pmg_net_infer_async.restype = _vp
     # batched AMR hierarchy (include/pamr_amr_gpu.h)
     L.pmg_amr_create.argtypes = [_ip, _dp, i, _ip, _dp, i, i]
     L.pmg_amr_create.restype = _vp
@@ -437,6 +455,52 @@ class Solver:
         _check(self._L.pmg_net_stats(self.h, C.byref(a), C.byref(r), C.byref(tn), C.byref(tv)), "pmg_net_stats")
         return {"accepted": a.value, "rejected": r.value, "t_net_ms": tn.value, "t_vcycle_ms": tv.value}
 
+    # -- learned band correction (excision) --
+    def band_setup(self, depth=3, core=26):
+        nc = self._L.pmg_band_setup(self.h, depth, core)
+        if nc < 0:
+            raise RuntimeError("pmg_band_setup failed")
+        return nc
+
+    def band_info(self):
+        np_, nc, side = C.c_longlong(0), C.c_int(0), C.c_int(0)
+        _check(self._L.pmg_band_info(self.h, C.byref(np_), C.byref(nc), C.byref(side)), "pmg_band_info")
+        return np_.value, nc.value, side.value
+
+    def band_mask(self):
+        out = np.zeros(self.shape(self.nlev), dtype=np.uint8)
+        _check(self._L.pmg_band_download(self.h, out.ctypes.data_as(C.POINTER(C.c_ubyte))), "pmg_band_download")
+        return out
+
+    def set_band_net(self, net, ts, use_dc=True, use_fgmres=False):
+        self._band_net = net
+        _check(self._L.pmg_set_band_net(self.h, net.h if net is not None else None, ts, int(use_dc),
+                                        int(use_fgmres)), "pmg_set_band_net")
+
+    def band_stats(self):
+        a, r = C.c_int(0), C.c_int(0)
+        out = np.zeros(6)
+        _check(self._L.pmg_band_stats(self.h, C.byref(a), C.byref(r), P(out)), "pmg_band_stats")
+        return {"accepted": a.value, "rejected": r.value, "rho_plain": out[0], "ms_plain": out[1],
+                "rho_net": out[2], "ms_net": out[3], "fgmres_rate_plain": out[4], "fgmres_rate_net": out[5]}
+
+    def band_sample(self, ustar_ptr, xs, ts, ncrop, seed, x_ptr, t_ptr, w_ptr):
+        _check(self._L.pmg_band_sample(self.h, ustar_ptr, xs, ts, ncrop, seed, x_ptr, t_ptr, w_ptr),
+               "pmg_band_sample")
+
+    def band_correct(self, which=0):
+        _check(self._L.pmg_band_correct(self.h, which), "pmg_band_correct")
+
+    def download32(self, lev, which):
+        out = np.zeros(self.shape(lev), dtype=np.float32)
+        _check(self._L.pmg_download32(self.h, lev, which, out.ctypes.data_as(C.POINTER(C.c_float))),
+               "pmg_download32")
+        return out
+
+    def upload32(self, lev, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        _check(self._L.pmg_upload32(self.h, lev, which, a.ctypes.data_as(C.POINTER(C.c_float))), "pmg_upload32")
+
     def field_ptr(self, lev, which):
         return self._L.pmg_field(self.h, lev, which)
 
@@ -616,12 +680,28 @@ class Net:
                "pmg_net_loss_grad")
         return loss.value, gs
 
-    def train_step(self, x_dev, t_dev, shape, nb, lr, margin=0):
+    def train_step(self, x_dev, t_dev, shape, nb, lr, margin=0, w_dev=None):
+        """One Adam step; w_dev (optional): 0/1 weights selecting the points
+        the mean loss runs over (besides the margin rule)."""
         nz, ny, nx = shape
         loss = C.c_double(0)
-        _check(self._L.pmg_net_train_step(self.h, tptr(x_dev), tptr(t_dev), nx, ny, nz, nb, margin,
-                                          lr, C.byref(loss)), "pmg_net_train_step")
+        if w_dev is None:
+            _check(self._L.pmg_net_train_step(self.h, tptr(x_dev), tptr(t_dev), nx, ny, nz, nb, margin,
+                                              lr, C.byref(loss)), "pmg_net_train_step")
+        else:
+            _check(self._L.pmg_net_train_step_w(self.h, tptr(x_dev), tptr(t_dev), tptr(w_dev), nx, ny, nz, nb,
+                                                margin, lr, C.byref(loss)), "pmg_net_train_step_w")
         return loss.value
+
+    def loss_grad_w(self, x_dev, t_dev, w_dev, shape, nb=1, margin=0):
+        nz, ny, nx = shape
+        loss = C.c_double(0)
+        gs = {k: np.zeros(v) for k, v in self.SHAPES.items()}
+        _check(self._L.pmg_net_loss_grad_w(self.h, tptr(x_dev), tptr(t_dev), tptr(w_dev), nx, ny, nz, nb, margin,
+                                           C.byref(loss),
+                                           *[P(gs[k]) for k in ("w1", "b1", "w2", "b2", "w3", "b3")]),
+               "pmg_net_loss_grad_w")
+        return loss.value, gs
 
 
 def mfma_selftest(A, B):

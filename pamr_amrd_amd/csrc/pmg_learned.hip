@@ -444,17 +444,22 @@ __global__ __launch_bounds__(256) void k_fold_wgrad(const float *__restrict__ pa
 // (Z - T)/count (bf16), other channels 0.  Points within `margin` of a box
 // face are not counted.  Per-block partial sums of the squared error.
 __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ Z, const float *__restrict__ T,
+                                              const float *__restrict__ W, const double *__restrict__ wcount,
                                               uint16_t *__restrict__ dZ, double *__restrict__ part,
                                               Geom g, int margin, double inv_count) {
+  // W (optional): per-point weights 0/1 on top of the margin rule; the
+  // mean then runs over the weighted points, whose count *wcount holds
   __shared__ double red[256];
   const size_t box = (size_t)g.nx * g.ny * g.nz;
   const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (wcount) inv_count = *wcount > 0 ? 1.0 / *wcount : 0.0;
   double e2 = 0.0;
   if (p < box * g.nb) {
     const size_t q = p % box;
     const int x = (int)(q % g.nx), y = (int)((q / g.nx) % g.ny), z = (int)(q / ((size_t)g.nx * g.ny));
-    const bool on = x >= margin && x < g.nx - margin && y >= margin && y < g.ny - margin &&
-                    z >= margin && z < g.nz - margin;
+    bool on = x >= margin && x < g.nx - margin && y >= margin && y < g.ny - margin &&
+              z >= margin && z < g.nz - margin;
+    if (W) on = on && W[p] != 0.f;
     float d = 0.f;
     if (on) {
       d = Z[p * NC] - T[p];
@@ -465,6 +470,29 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ Z, const
     for (int c = 1; c < NC; c++) dZ[p * NC + c] = 0;
   }
   red[threadIdx.x] = e2;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// partial counts of the weighted points (margin rule and W != 0), one per block
+__global__ __launch_bounds__(256) void k_wcount(const float *__restrict__ W, Geom g, int margin,
+                                                double *__restrict__ part) {
+  __shared__ double red[256];
+  const size_t box = (size_t)g.nx * g.ny * g.nz;
+  const size_t p = (size_t)blockIdx.x * 256 + threadIdx.x;
+  double v = 0.0;
+  if (p < box * g.nb) {
+    const size_t q = p % box;
+    const int x = (int)(q % g.nx), y = (int)((q / g.nx) % g.ny), z = (int)(q / ((size_t)g.nx * g.ny));
+    if (x >= margin && x < g.nx - margin && y >= margin && y < g.ny - margin && z >= margin &&
+        z < g.nz - margin && W[p] != 0.f)
+      v = 1.0;
+  }
+  red[threadIdx.x] = v;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -707,7 +735,7 @@ int net_wgrad(pmg_net *N, const Geom &g, const uint16_t *X, const uint16_t *dZ, 
 }
 
 // forward + loss + backward; gradients into N->grad (3 * LAYER_FLOATS)
-int net_loss_backward(pmg_net *N, const Geom &g, const float *T, int margin, double *loss_out) {
+int net_loss_backward(pmg_net *N, const Geom &g, const float *T, const float *W, int margin, double *loss_out) {
   int rc;
   if ((rc = net_forward(N, g))) return rc;
   const size_t total = (size_t)g.nx * g.ny * g.nz * g.nb;
@@ -719,9 +747,14 @@ int net_loss_backward(pmg_net *N, const Geom &g, const float *T, int margin, dou
     N->lpart_cap = nblk;
   }
   const long long cx = g.nx - 2 * margin, cy = g.ny - 2 * margin, cz = g.nz - 2 * margin;
-  const double count = (double)(cx > 0 ? cx : 0) * (cy > 0 ? cy : 0) * (cz > 0 ? cz : 0) * g.nb;
-  hipLaunchKernelGGL(k_loss, dim3(nblk), dim3(256), 0, N->stream, N->z3, T, N->d16, N->lpart, g, margin,
-                     1.0 / count);
+  double count = (double)(cx > 0 ? cx : 0) * (cy > 0 ? cy : 0) * (cz > 0 ? cz : 0) * g.nb;
+  if (W) {
+    // the weighted count, folded on the device into lsum[1] before the loss uses it
+    hipLaunchKernelGGL(k_wcount, dim3(nblk), dim3(256), 0, N->stream, W, g, margin, N->lpart);
+    hipLaunchKernelGGL(k_fold_d, dim3(1), dim3(256), 0, N->stream, N->lpart, nblk, N->lsum + 1);
+  }
+  hipLaunchKernelGGL(k_loss, dim3(nblk), dim3(256), 0, N->stream, N->z3, T, W, W ? N->lsum + 1 : (const double *)nullptr,
+                     N->d16, N->lpart, g, margin, 1.0 / count);
   hipLaunchKernelGGL(k_fold_d, dim3(1), dim3(256), 0, N->stream, N->lpart, nblk, N->lsum);
   PMG_HOST_CHECK(hipGetLastError());
   const dim3 gr = conv_grid(g);
@@ -737,10 +770,11 @@ int net_loss_backward(pmg_net *N, const Geom &g, const float *T, int margin, dou
   if ((rc = net_wgrad(N, g, N->x0, N->d16, N->grad))) return rc;
   PMG_HOST_CHECK(hipGetLastError());
   if (loss_out) {
-    double l;
-    PMG_HOST_CHECK(hipMemcpyAsync(&l, N->lsum, sizeof(double), hipMemcpyDeviceToHost, N->stream));
+    double l[2];
+    PMG_HOST_CHECK(hipMemcpyAsync(l, N->lsum, 2 * sizeof(double), hipMemcpyDeviceToHost, N->stream));
     PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
-    *loss_out = 0.5 * l / count;
+    if (W) count = l[1];
+    *loss_out = count > 0 ? 0.5 * l[0] / count : 0.0;
   }
   return 0;
 }
@@ -825,7 +859,7 @@ pmg_net *pmg_net_create(int hidden, unsigned long long seed) {
             hipMalloc(&N->wb, 3 * NTAP * 256 * sizeof(uint16_t)) == hipSuccess &&
             hipMalloc(&N->wfi, 64 * 8 * sizeof(uint16_t)) == hipSuccess &&
             hipMalloc(&N->wp, 2 * NPAIR * 512 * sizeof(uint16_t)) == hipSuccess &&
-            hipMalloc(&N->lsum, sizeof(double)) == hipSuccess;
+            hipMalloc(&N->lsum, 2 * sizeof(double)) == hipSuccess;
   if (!ok) { pmg_net_destroy(N); return nullptr; }
   (void)hipMemcpy(N->w, w.data(), nf * 4, hipMemcpyHostToDevice);
   (void)hipMemcpy(N->mask, mask.data(), nf, hipMemcpyHostToDevice);
@@ -925,11 +959,53 @@ int pmg_net_loss_grad(pmg_net *N, const float *x, const float *t, int nx, int ny
   Geom g{nx, ny, nz, nb};
   hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, N->stream, x, N->x0,
                      P, 1.0f);
-  int rc = net_loss_backward(N, g, t, margin, loss);
+  int rc = net_loss_backward(N, g, t, nullptr, margin, loss);
   if (rc) return rc;
   std::vector<float> gr(3 * (size_t)LAYER_FLOATS);
   PMG_HOST_CHECK(hipMemcpy(gr.data(), N->grad, gr.size() * 4, hipMemcpyDeviceToHost));
   from_master(gr, gw1, gb1, gw2, gb2, gw3, gb3);
+  return 0;
+}
+
+// Loss and gradients as pmg_net_loss_grad, the mean taken over the points
+// with w != 0 (w: device fp32 [nb][P]) that also satisfy the margin rule.
+int pmg_net_loss_grad_w(pmg_net *N, const float *x, const float *t, const float *w, int nx, int ny, int nz,
+                        int nb, int margin, double *loss, double *gw1, double *gb1, double *gw2, double *gb2,
+                        double *gw3, double *gb3) {
+  if (!N || !x || !t || !w) return -1;
+  const size_t P = (size_t)nx * ny * nz * nb;
+  if (net_ensure(N, P)) return -1;
+  Geom g{nx, ny, nz, nb};
+  hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, N->stream, x, N->x0,
+                     P, 1.0f);
+  int rc = net_loss_backward(N, g, t, w, margin, loss);
+  if (rc) return rc;
+  std::vector<float> gr(3 * (size_t)LAYER_FLOATS);
+  PMG_HOST_CHECK(hipMemcpy(gr.data(), N->grad, gr.size() * 4, hipMemcpyDeviceToHost));
+  from_master(gr, gw1, gb1, gw2, gb2, gw3, gb3);
+  return 0;
+}
+
+// One Adam step on a batch (device fp32 inputs/targets; w optional weights
+// as in pmg_net_loss_grad_w, NULL for every point past the margin).
+int pmg_net_train_step_w(pmg_net *N, const float *x, const float *t, const float *w, int nx, int ny, int nz,
+                         int nb, int margin, float lr, double *loss) {
+  if (!N || !x || !t) return -1;
+  const size_t P = (size_t)nx * ny * nz * nb;
+  if (net_ensure(N, P)) return -1;
+  Geom g{nx, ny, nz, nb};
+  hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, N->stream, x, N->x0,
+                     P, 1.0f);
+  int rc = net_loss_backward(N, g, t, w, margin, loss);
+  if (rc) return rc;
+  N->adam_t++;
+  const float b1 = 0.9f, b2 = 0.999f;
+  const float c1 = 1.f - std::pow(b1, (float)N->adam_t), c2 = 1.f - std::pow(b2, (float)N->adam_t);
+  const int nf = 3 * LAYER_FLOATS;
+  hipLaunchKernelGGL(k_adam, dim3((nf + 255) / 256), dim3(256), 0, N->stream, N->w, N->m1, N->m2, N->grad,
+                     N->mask, nf, lr, b1, b2, c1, c2, 1e-8f);
+  PMG_HOST_CHECK(hipGetLastError());
+  N->frags_valid = false;
   return 0;
 }
 
@@ -942,7 +1018,7 @@ int pmg_net_train_step(pmg_net *N, const float *x, const float *t, int nx, int n
   Geom g{nx, ny, nz, nb};
   hipLaunchKernelGGL(k_pack_input<float>, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, N->stream, x, N->x0,
                      P, 1.0f);
-  int rc = net_loss_backward(N, g, t, margin, loss);
+  int rc = net_loss_backward(N, g, t, nullptr, margin, loss);
   if (rc) return rc;
   N->adam_t++;
   const float b1 = 0.9f, b2 = 0.999f;
@@ -992,6 +1068,22 @@ int pmg_net_reserve(pmg_net *N, long long points) {
   if (net_frags(N)) return -1;
   PMG_HOST_CHECK(hipStreamSynchronize(N->stream));
   return 0;
+}
+
+// y = N(scale * x) on nb boxes, enqueued on `stream` with no host
+// synchronisation (capturable in a graph once pmg_net_reserve has sized the
+// buffers and packed the weights: then nothing is allocated or packed
+// here).  Returns the network-owned output buffer (fp32 [nb][P], valid until
+// the next inference), or NULL on error.
+float *pmg_net_infer_async(pmg_net *N, const float *x, float scale, int nx, int ny, int nz, int nb, void *stream) {
+  if (!N || !x || nx < 1 || ny < 1 || nz < 1 || nb < 1) return nullptr;
+  const size_t P = (size_t)nx * ny * nz * nb;
+  if (net_ensure_infer(N, P)) return nullptr;
+  hipStream_t keep = N->stream;
+  N->stream = (hipStream_t)stream;
+  const int rc = net_infer(N, x, scale, Geom{nx, ny, nz, nb});
+  N->stream = keep;
+  return rc ? nullptr : N->y;
 }
 
 int pmg_net_infer(pmg_net *N, const float *x, float scale, float *y, int nx, int ny, int nz, int nb) {

@@ -10,6 +10,7 @@
 // replayed.  Mixed precision (FP32 inner V-cycles inside FP64 defect
 // correction) reuses the same kernels instantiated for float.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -162,6 +163,25 @@ struct pmg_solver {
   double *kry_red = nullptr;
   double *kry_host = nullptr;
   hipGraphExec_t gprec = nullptr;  // captured FP32 V-cycle (the preconditioner)
+  // learned correction in the band next to the excised region (pmg_band.hip)
+  struct Band {
+    int depth = 0, core = 0, side = 0, margin = 3, ncore = 0;
+    long long npts = 0;
+    int *d_org = nullptr;          // crop origins, 3 ints per core
+    uint8_t *d_band = nullptr;     // band mask of the finest level
+    float *d_x = nullptr;          // network input crops [ncore][side^3]
+    std::vector<long long> h_pts;  // linear indices of the band points (training sampler)
+    pmg_net *net = nullptr;
+    double ts = 1.0;               // output scale: correction = ts * ||r|| * N(r / ||r||)
+    int use_dc = 0, use_fgmres = 0;
+    double rho_plain = -1, t_plain = -1;   // defect correction: plain iteration (EMA)
+    double rho_net = -1, t_net = -1;       // ... iterations with the correction (EMA)
+    double fg_rate_plain = 0, fg_rate_net = 0;  // FGMRES: log10 reduction per ms of a whole solve
+    int fg_skip = 0;
+    int accepted = 0, rejected = 0;
+    hipGraphExec_t gdc = nullptr;   // captured correction of the defect-correction iteration
+    hipGraphExec_t gpc = nullptr;   // captured correction of the preconditioner output
+  } band;
 };
 
 // norm slot layout in d_norms
@@ -871,10 +891,50 @@ int ensure_fp32(pmg_solver *S) {
   return 0;
 }
 
+// The learned band correction of the defect-correction iteration: on entry
+// the FP32 source of the finest level holds -r and d_norms[NS_OUTER] its
+// RMS norm; u += ts * ||r|| * N(r / ||r||) at the band points, then -r is
+// recomputed where that changed it (each core grown by one point).
+int enq_band_dc(pmg_solver *S) {
+  Level &F = S->lev[S->nlev - 1];
+  auto &B = S->band;
+  const double *nrm = S->d_norms + NS_OUTER;
+  if (pmg_band_gather_f32(F.f.rho, F.n[0], F.n[1], F.n[2], B.d_org, B.ncore, B.side, -1.0f, nrm, B.d_x, S->stream))
+    return -1;
+  const float *y = pmg_net_infer_async(B.net, B.d_x, 1.0f, B.side, B.side, B.side, B.ncore, S->stream);
+  if (!y) return -1;
+  if (pmg_band_scatter(F.d.u, nullptr, nullptr, y, B.d_band, F.n[0], F.n[1], F.n[2], B.d_org, B.ncore, B.side,
+                       B.margin, B.ts, nrm, S->stream))
+    return -1;
+  return pmg_band_resid(S->op, S->do_ex, F.d.u, F.d.rho, F.d.a, F.d.c, F.chr, F.f.rho, F.n[0], F.n[1], F.n[2],
+                        B.d_org, B.ncore, B.side, B.margin, F.hdx[0], F.hdx[1], F.hdx[2], S->stream);
+}
+
+// The same correction of the preconditioner's output (flexible GMRES): the
+// FP32 residual of the V-cycle's z for its source, its norm folded on the
+// device, z += ts * ||r|| * N(r / ||r||) at the band points.
+int enq_band_pc(pmg_solver *S) {
+  Level &F = S->lev[S->nlev - 1];
+  auto &B = S->band;
+  const int slot = NS_F32 + NS_FINAL;
+  int rc;
+  if ((rc = residual<float>(S, S->nlev, slot, true, true))) return rc;
+  if ((rc = fold_norms(S, slot, 1))) return rc;
+  const double *nrm = S->d_norms + slot;
+  if (pmg_band_gather_f32(F.f.res, F.n[0], F.n[1], F.n[2], B.d_org, B.ncore, B.side, 1.0f, nrm, B.d_x, S->stream))
+    return -1;
+  const float *y = pmg_net_infer_async(B.net, B.d_x, 1.0f, B.side, B.side, B.side, B.ncore, S->stream);
+  if (!y) return -1;
+  return pmg_band_scatter(nullptr, nullptr, ucur<float>(F), y, B.d_band, F.n[0], F.n[1], F.n[2], B.d_org, B.ncore,
+                          B.side, B.margin, B.ts, nrm, S->stream);
+}
+
 void invalidate_graphs(pmg_solver *S) {
   if (S->g64) { (void)hipGraphExecDestroy(S->g64); S->g64 = nullptr; }
   if (S->g32) { (void)hipGraphExecDestroy(S->g32); S->g32 = nullptr; }
   if (S->gprec) { (void)hipGraphExecDestroy(S->gprec); S->gprec = nullptr; }
+  if (S->band.gdc) { (void)hipGraphExecDestroy(S->band.gdc); S->band.gdc = nullptr; }
+  if (S->band.gpc) { (void)hipGraphExecDestroy(S->band.gpc); S->band.gpc = nullptr; }
 }
 
 }  // namespace
@@ -1006,6 +1066,9 @@ void pmg_destroy(pmg_solver *S) {
   if (S->kry) (void)hipFree(S->kry);
   if (S->kry_red) (void)hipFree(S->kry_red);
   if (S->kry_host) (void)hipHostFree(S->kry_host);
+  if (S->band.d_org) (void)hipFree(S->band.d_org);
+  if (S->band.d_band) (void)hipFree(S->band.d_band);
+  if (S->band.d_x) (void)hipFree(S->band.d_x);
   if (S->stream) (void)hipStreamDestroy(S->stream);
   delete S;
 }
@@ -1277,7 +1340,7 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
   if (!S || max_iter < 0) return -1;
   double r0;
   int rc;
-  if (mode == 1 || mode == 2) {
+  if (mode == 1 || mode == 2 || mode == 3) {
     // the FP64 defect of the initial u: its norm is r0 and, in FP32, it is
     // the source of the first inner cycle
     if ((rc = ensure_fp32(S))) return rc;
@@ -1380,6 +1443,50 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
       PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));
       S->t_vc_ms = S->t_vc_ms < 0 ? ms : 0.5 * (S->t_vc_ms + ms);
       if (r > 0 && rn > 0) S->rho_vc = 0.5 * S->rho_vc + 0.5 * (rn / r);
+      r = rn;
+      if (hist) hist[it] = r;
+      it++;
+    }
+  } else if (mode == 3) {
+    // mixed precision with the learned band correction (excision) before
+    // each FP32 V-cycle.  Every iteration is judged by the FP64 residual:
+    // its contraction per unit time must beat the plain iteration's (both
+    // measured, running means); the first iteration that does not ends the
+    // correction for the rest of the solve (plain iterations from there on).
+    auto &B = S->band;
+    if (!B.net || B.ncore < 1) return -1;
+    if ((rc = pmg_net_reserve(B.net, (long long)B.ncore * B.side * B.side * B.side))) return rc;
+    EventPair ev;
+    PMG_HOST_CHECK(ev.create());
+    bool net_on = B.use_dc != 0;
+    while (it < max_iter && r > rtol * r0) {
+      const bool with_net = net_on && B.rho_plain > 0 && B.t_plain > 0;
+      const bool fresh = with_net ? (!B.gdc || !S->g32) : !S->g32;  // graph capture in this iteration
+      PMG_HOST_CHECK(hipEventRecord(ev.e0, S->stream));
+      if (with_net && (rc = capture_or_run(S, &B.gdc, enq_band_dc))) return rc;
+      if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
+      PMG_HOST_CHECK(hipEventRecord(ev.e1, S->stream));
+      PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost,
+                                    S->stream));
+      PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+      const double rn = S->h_pinned[0];
+      float ms = 0.f;
+      PMG_HOST_CHECK(hipEventElapsedTime(&ms, ev.e0, ev.e1));
+      const double rho = r > 0 ? rn / r : 1.0;
+      if (with_net) {
+        if (!fresh) B.t_net = B.t_net < 0 ? ms : 0.5 * (B.t_net + ms);
+        B.rho_net = B.rho_net < 0 ? rho : 0.5 * (B.rho_net + rho);
+        const double t = fresh ? (B.t_net > 0 ? B.t_net : B.t_plain) : (double)ms;
+        if (rho < 1.0 && std::log(rho) / t < std::log(B.rho_plain) / B.t_plain) {
+          B.accepted++;
+        } else {
+          B.rejected++;
+          net_on = false;
+        }
+      } else if (rho > 0 && rho < 1.0) {
+        if (!fresh) B.t_plain = B.t_plain < 0 ? ms : 0.5 * (B.t_plain + ms);
+        B.rho_plain = B.rho_plain < 0 ? rho : 0.5 * (B.rho_plain + rho);
+      }
       r = rn;
       if (hist) hist[it] = r;
       it++;
@@ -1943,6 +2050,18 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
   if (r0_out) *r0_out = rref;
   int it = 0;
   std::vector<double> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+  // learned band correction of every preconditioner output (excision): on
+  // once a plain solve has measured the plain rate, kept while a solve with
+  // it reduces the residual faster per unit time than the plain solves do
+  auto &B = S->band;
+  bool band_on = B.use_fgmres && B.net && B.ncore > 0 && B.fg_rate_plain > 0;
+  if (band_on && B.fg_skip > 0) {
+    B.fg_skip--;
+    band_on = false;
+  }
+  if (band_on && (rc = pmg_net_reserve(B.net, (long long)B.ncore * B.side * B.side * B.side))) return rc;
+  const double r_start = r_rms;
+  const auto t_start = std::chrono::steady_clock::now();
   while (it < max_iter && r_rms > rtol * rref) {
     // b = -R(u) (in w): 2-norm, and the RMS-to-2-norm factor of this grid
     VecPtrs Pw{};
@@ -1961,6 +2080,7 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
       hipLaunchKernelGGL(k_kry_prep, dim3(nb), dim3(256), 0, S->stream, F.f.rho, F.f.u, V[j], n);
       PMG_HOST_CHECK(hipGetLastError());
       if ((rc = capture_or_run(S, &S->gprec, enq_prec))) return rc;
+      if (band_on && (rc = capture_or_run(S, &B.gpc, enq_band_pc))) return rc;
       hipLaunchKernelGGL(k_kry_to64, dim3(nb), dim3(256), 0, S->stream, Z[j], F.f.u, n);
       PMG_HOST_CHECK(hipGetLastError());
       // V_{j+1} = A Z_j, orthogonalised against V_0..V_j by classical
@@ -2019,6 +2139,22 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
     if ((rc = true_residual(&r_rms))) return rc;
     if (hist && it > 0) hist[it - 1] = r_rms;
   }
+  if (B.use_fgmres && B.net && B.ncore > 0 && it > 0 && r_rms > 0 && r_start > r_rms) {
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    const double rate = std::log10(r_start / r_rms) / ms;
+    if (!band_on) {
+      B.fg_rate_plain = B.fg_rate_plain > 0 ? 0.5 * (B.fg_rate_plain + rate) : rate;
+    } else {
+      B.fg_rate_net = B.fg_rate_net > 0 ? 0.5 * (B.fg_rate_net + rate) : rate;
+      if (rate > B.fg_rate_plain) {
+        B.accepted++;
+      } else {
+        B.rejected++;
+        B.fg_skip = 4;  // plain solves re-measure the plain rate before the next trial
+      }
+    }
+  }
   if (iters) *iters = it;
   return 0;
 }
@@ -2070,6 +2206,183 @@ int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double 
   PMG_HOST_CHECK(hipEventElapsedTime(&ms, e0, e1));
   *ms_avg = (double)ms / nrep;
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Learned band correction (excision; no reference counterpart)
+// ---------------------------------------------------------------------------
+int pmg_band_setup(pmg_solver *S, int depth, int core) {
+  if (!S || !S->do_ex || depth < 1 || depth > 16 || core < 4 || core > 64) return -1;
+  Level &F = S->lev[S->nlev - 1];
+  auto &B = S->band;
+  const int nx = F.n[0], ny = F.n[1], nz = F.n[2];
+  std::vector<double> chr(F.size);
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  PMG_HOST_CHECK(hipMemcpy(chr.data(), F.chrd, F.size * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<unsigned char> band(F.size);
+  const long long np = pmg_band_mask(chr.data(), nx, ny, nz, S->ex, depth, band.data());
+  if (np < 0) return -1;
+  std::vector<double>().swap(chr);
+  const int margin = 3;
+  const int maxc = ((nx + core - 1) / core) * ((ny + core - 1) / core) * ((nz + core - 1) / core);
+  std::vector<int> org(3 * (size_t)maxc);
+  const int nc = pmg_band_cores(band.data(), nx, ny, nz, core, margin, org.data(), maxc);
+  if (nc < 0) return -1;
+  if (B.d_org) PMG_HOST_CHECK(hipFree(B.d_org));
+  if (B.d_band) PMG_HOST_CHECK(hipFree(B.d_band));
+  if (B.d_x) PMG_HOST_CHECK(hipFree(B.d_x));
+  B.d_org = nullptr;
+  B.d_band = nullptr;
+  B.d_x = nullptr;
+  const int side = core + 2 * margin;
+  PMG_HOST_CHECK(hipMalloc(&B.d_band, F.size));
+  PMG_HOST_CHECK(hipMemcpy(B.d_band, band.data(), F.size, hipMemcpyHostToDevice));
+  PMG_HOST_CHECK(hipMalloc(&B.d_org, sizeof(int) * 3 * (size_t)std::max(nc, 1)));
+  if (nc > 0) {
+    PMG_HOST_CHECK(hipMemcpy(B.d_org, org.data(), sizeof(int) * 3 * (size_t)nc, hipMemcpyHostToDevice));
+    PMG_HOST_CHECK(hipMalloc(&B.d_x, sizeof(float) * (size_t)nc * side * side * side));
+  }
+  B.h_pts.clear();
+  B.h_pts.reserve((size_t)np);
+  for (size_t p = 0; p < F.size; p++)
+    if (band[p]) B.h_pts.push_back((long long)p);
+  B.depth = depth;
+  B.core = core;
+  B.side = side;
+  B.margin = margin;
+  B.ncore = nc;
+  B.npts = np;
+  if (B.gdc) { (void)hipGraphExecDestroy(B.gdc); B.gdc = nullptr; }
+  if (B.gpc) { (void)hipGraphExecDestroy(B.gpc); B.gpc = nullptr; }
+  return nc;
+}
+
+int pmg_band_info(pmg_solver *S, long long *npts, int *ncore, int *side) {
+  if (!S) return -1;
+  if (npts) *npts = S->band.npts;
+  if (ncore) *ncore = S->band.ncore;
+  if (side) *side = S->band.side;
+  return 0;
+}
+
+int pmg_band_download(pmg_solver *S, unsigned char *host) {
+  if (!S || !host || !S->band.d_band) return -1;
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  PMG_HOST_CHECK(hipMemcpy(host, S->band.d_band, S->lev[S->nlev - 1].size, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmg_set_band_net(pmg_solver *S, pmg_net *N, double ts, int use_dc, int use_fgmres) {
+  if (!S || (N && !(ts > 0))) return -1;
+  auto &B = S->band;
+  B.net = N;
+  B.ts = ts;
+  B.use_dc = use_dc;
+  B.use_fgmres = use_fgmres;
+  B.rho_net = B.t_net = -1;
+  B.fg_rate_net = 0;
+  B.fg_skip = 0;
+  B.accepted = B.rejected = 0;
+  if (B.gdc) { (void)hipGraphExecDestroy(B.gdc); B.gdc = nullptr; }
+  if (B.gpc) { (void)hipGraphExecDestroy(B.gpc); B.gpc = nullptr; }
+  return 0;
+}
+
+int pmg_band_stats(pmg_solver *S, int *accepted, int *rejected, double *out6) {
+  if (!S) return -1;
+  const auto &B = S->band;
+  if (accepted) *accepted = B.accepted;
+  if (rejected) *rejected = B.rejected;
+  if (out6) {
+    out6[0] = B.rho_plain;
+    out6[1] = B.t_plain;
+    out6[2] = B.rho_net;
+    out6[3] = B.t_net;
+    out6[4] = B.fg_rate_plain;
+    out6[5] = B.fg_rate_net;
+  }
+  return 0;
+}
+
+int pmg_band_sample(pmg_solver *S, const double *ustar, double xs, double ts, int ncrop, unsigned long long seed,
+                    float *x, float *t, float *w) {
+  // training crops centred on random band points of the current state:
+  // x = xs * (FP32 finest source), t = ts * (ustar - u), w = band indicator
+  if (!S || !ustar || !x || !t || !w || ncrop < 1) return -1;
+  auto &B = S->band;
+  if (B.h_pts.empty() || !S->fp32_ready) return -1;
+  Level &F = S->lev[S->nlev - 1];
+  const int nx = F.n[0], ny = F.n[1];
+  std::vector<int> org(3 * (size_t)ncrop);
+  unsigned long long z = seed;
+  for (int c = 0; c < ncrop; c++) {
+    // splitmix64 (S. Vigna): one draw per crop
+    z += 0x9E3779B97F4A7C15ull;
+    unsigned long long v = z;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    v ^= v >> 31;
+    const long long p = B.h_pts[(size_t)(v % B.h_pts.size())];
+    const int i = (int)(p % nx), j = (int)((p / nx) % ny), k = (int)(p / ((long long)nx * ny));
+    org[3 * c] = i - B.side / 2;
+    org[3 * c + 1] = j - B.side / 2;
+    org[3 * c + 2] = k - B.side / 2;
+  }
+  int *dorg = nullptr;
+  PMG_HOST_CHECK(hipMalloc(&dorg, sizeof(int) * org.size()));
+  PMG_HOST_CHECK(hipMemcpyAsync(dorg, org.data(), sizeof(int) * org.size(), hipMemcpyHostToDevice, S->stream));
+  int rc = pmg_band_pairs(F.f.rho, (float)xs, ustar, F.d.u, ts, B.d_band, F.n[0], F.n[1], F.n[2], dorg, ncrop,
+                          B.side, x, t, w, S->stream);
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  PMG_HOST_CHECK(hipFree(dorg));
+  return rc;
+}
+
+int pmg_band_correct(pmg_solver *S, int which) {
+  // one band correction now: which 0 = the defect-correction form (FP64 u,
+  // from the FP32 source -r and the outer norm, then -r updated), 1 = the
+  // preconditioner form (FP32 u from its FP32 residual)
+  if (!S || !S->band.net || S->band.ncore < 1 || !S->fp32_ready) return -1;
+  auto &B = S->band;
+  int rc;
+  if ((rc = pmg_net_reserve(B.net, (long long)B.ncore * B.side * B.side * B.side))) return rc;
+  rc = which == 0 ? enq_band_dc(S) : enq_band_pc(S);
+  if (rc) return rc;
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
+}
+
+// FP32 arrays of a level (mixed precision): which as pmg_field (u, rho, rhs, res, lop)
+int pmg_download32(pmg_solver *S, int L, int which, float *host) {
+  if (!S || L < 1 || L > S->nlev || !host || !S->fp32_ready) return -1;
+  Level &Lv = S->lev[L - 1];
+  const float *src = nullptr;
+  switch (which) {
+    case 0: src = ucur<float>(Lv); break;
+    case 1: src = Lv.f.rho; break;
+    case 2: src = Lv.f.rhs; break;
+    case 3: src = Lv.f.res; break;
+    case 4: src = Lv.f.lop; break;
+    default: return -1;
+  }
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  PMG_HOST_CHECK(hipMemcpy(host, src, Lv.size * sizeof(float), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int pmg_upload32(pmg_solver *S, int L, int which, const float *host) {
+  if (!S || L < 1 || L > S->nlev || !host) return -1;
+  if (ensure_fp32(S)) return -1;
+  Level &Lv = S->lev[L - 1];
+  float *dst = nullptr;
+  switch (which) {
+    case 0: dst = ucur<float>(Lv); break;
+    case 1: dst = Lv.f.rho; break;
+    default: return -1;
+  }
+  PMG_HOST_CHECK(hipMemcpyAsync(dst, host, Lv.size * sizeof(float), hipMemcpyHostToDevice, S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  return 0;
 }
 
 int pmg_last_citer(pmg_solver *S, int which) {
