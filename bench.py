@@ -311,15 +311,25 @@ def excised_leg(args, pm, torch):
     """Large excised problem: div(a grad u) + c u on n^3 (default 513^3), a
     lognormal, c uniform in [0,1], 3 seeded excised spheres (chr on every
     level, Dirichlet data g held on the excised points), 16 seeded Gaussian
-    right-hand sides, each solved to ||r|| <= rtol ||r(u0)|| by flexible
-    GMRES(8) with the FP32 V-cycle as preconditioner; the first right-hand
-    sides also by FP32 inner V-cycles in FP64 defect correction, the first in
-    plain FP64 V-cycles (capped) for comparison; then all right-hand sides
-    again concurrently, spread over c5_streams solver instances (one stream
-    and host thread each)."""
+    right-hand sides, each solved to ||r|| <= rtol ||r(u0)||.
+
+    Untimed set-up: the band plan of the learned correction (updatable points
+    within 3 grid points of an excised point, in cores of 26^3) and its
+    network, trained on (residual, error) crops of the plain mixed iteration
+    of c5_band_train extra right-hand sides.  Timed per right-hand side:
+    flexible GMRES(8) with the FP32 V-cycle as preconditioner, and FP32 inner
+    V-cycles in FP64 defect correction with the learned band correction
+    (pmg_solve_rel mode 3, every iteration checked against the plain one);
+    the first c5_mixed_rhs right-hand sides also by plain defect correction
+    and by FGMRES with the correction in its preconditioner, the first in
+    plain FP64 V-cycles (capped).  Then all right-hand sides again,
+    concurrently, by the faster of the two methods, spread over c5_streams
+    solver instances (one stream and host thread each)."""
     import numpy as np
+    from pamr_amrd_amd import band as BD
     from pamr_amrd_amd import hierarchy as H
     from pamr_amrd_amd import sequence as SQ
+    log = lambda m: print(f"bench: excised: {m}", file=sys.stderr, flush=True)
     n = args.c5_n
     box = (0.0, 1.0, 0.0, 1.0, 0.0, 1.0)
     spheres = H.excision_spheres(seed=21)
@@ -347,76 +357,171 @@ def excised_leg(args, pm, torch):
     u0 = torch.from_numpy(np.where(ch == 1.0, H.excision_data(shape, box), 0.0)).cuda()
     del ch
     L = pm.lib()
-    times, iters, hists = [], [], []
-    gtimes, giters = [], []
+
+    # ---- learned band correction: plan and training (not timed) ----
+    t0 = time.perf_counter()
+    ncore = S.band_setup(3, 26)
+    npts, _, side = S.band_info()
+    t_plan = time.perf_counter() - t0
+    log(f"band: {npts} points in {ncore} cores of 26^3 ({t_plan:.1f} s)")
+    t0 = time.perf_counter()
+    train_rhos = [SQ.DriftingSource(seed=900 + k, speed=0.0).rho(n, 0, torch) for k in range(args.c5_band_train)]
+    ustars = [BD.converged_solution(S, r, u0, torch, rtol=args.rtol) for r in train_rhos]
+    t_sol = time.perf_counter() - t0
+    # two placements, each with its own network: after the V-cycle (on the
+    # FP64 defect) and inside the FP32 V-cycle after the finest coarse-grid
+    # correction
+    t0 = time.perf_counter()
+    x, t, w = BD.collect_pairs(S, train_rhos, ustars, u0, torch, iters=10, crops=48)
+    t_collect = time.perf_counter() - t0
+    net, ts, tinfo = BD.train(x, t, w, torch, steps=args.c5_band_steps, log=log)
+    del x, t, w
+    t0 = time.perf_counter()
+    x, t, w = BD.collect_pairs_inner(S, train_rhos, ustars, u0, torch, iters=10, crops=48)
+    t_collect_in = time.perf_counter() - t0
+    net_in, ts_in, tinfo_in = BD.train(x, t, w, torch, steps=args.c5_band_steps, log=log)
+    del x, t, w, ustars, train_rhos
+    torch.cuda.empty_cache()
+    tinfo["collect_s"] = round(t_collect, 2)
+    tinfo["solutions_s"] = round(t_sol, 2)
+    tinfo_in["collect_s"] = round(t_collect_in, 2)
+    S.set_band_net(net, ts, use_dc=True, use_fgmres=False)
+    S.set_band_net_inner(net_in, ts_in)
+    # the network's inference on the crops of one correction, timed alone
+    xin = torch.randn((ncore, side, side, side), dtype=torch.float32, device="cuda")
+    yout = torch.empty_like(xin)
+    net.infer(xin, yout, (side, side, side), ncore)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        net.infer(xin, yout, (side, side, side), ncore)
+    e1.record()
+    e1.synchronize()
+    t_inf = e0.elapsed_time(e1) / 10
+    del xin, yout
+    flops = 15552.0 * ncore * side ** 3
+    roof_net = {"bound": "mfma", "achieved": round(flops / (t_inf * 1e-3) / 1e12, 2), "peak": 2500.0,
+                "unit": "TFLOP/s", "frac": round(flops / (t_inf * 1e-3) / 1e12 / 2500.0, 4),
+                "flop_per_application": flops, "ms_per_application": round(t_inf, 4),
+                "crop_points": ncore * side ** 3, "band_points": npts,
+                "note": "3x3x3 convs 1->16->16->1 over the crops of one correction (15552 flop per crop point), "
+                        "BF16 MFMA with FP32 accumulation; pmg_net_infer timed with HIP events"}
+
+    times = {"fgmres": [], "dc_band": [], "dc_band_in": [], "fgmres_band_in": [], "dc": [], "fgmres_band": []}
+    iters = {k: [] for k in times}
     plain = None
     for r in range(args.c5_rhs):
         src = SQ.DriftingSource(seed=1234 + r, speed=0.0)
         rho = src.rho(n, 0, torch)
         SQ._copy_grid(S.field_ptr(Lf, pm.PMG_F_RHO), pm.tptr(rho), n)
         del rho
-        for mode in ((-1, 1, 0) if r == 0 else (-1, 1) if r < args.c5_mixed_rhs else (-1,)):
+        modes = ["fgmres", "dc_band"]
+        if r < args.c5_mixed_rhs:
+            modes += ["dc", "fgmres_band", "dc_band_in", "fgmres_band_in"]
+        if r == 0:
+            modes += ["fp64"]
+        for mode in modes:
+            S.band_set_inner(mode.endswith("_in"))
+            S.band_enable(True, 2 if mode.startswith("fgmres_band") else False)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             L.pmg_set_solution_lincomb(S.h, pm.tptr(u0), None, 1.0, 0.0)
             S.init()
-            if mode < 0:
+            if mode in ("fgmres", "fgmres_band"):
                 r0, h = S.solve_fgmres(args.c5_restart, args.rtol, args.c5_max_iter)
+            elif mode == "fp64":
+                r0, h = S.solve_rel(0, args.rtol, args.c5_plain_cycles)
             else:
-                r0, h = S.solve_rel(mode, args.rtol, args.c5_max_iter if mode == 1 else args.c5_plain_cycles)
+                r0, h = S.solve_rel(3 if mode == "dc_band" else 1, args.rtol, args.c5_max_iter)
             torch.cuda.synchronize()
             dt = 1e3 * (time.perf_counter() - t0)
-            if mode < 0:
-                if len(h) == 0 or h[-1] > args.rtol * r0:
-                    raise RuntimeError(f"excised rhs {r} (fgmres) did not converge in {len(h)} steps: {h[-3:]}")
-                gtimes.append(dt)
-                giters.append(len(h))
-            elif mode == 1:
-                if len(h) == 0 or h[-1] > args.rtol * r0:
-                    raise RuntimeError(f"excised rhs {r} did not converge in {len(h)} iterations: {h[-3:]}")
-                times.append(dt)
-                iters.append(len(h))
-                hists.append((h[-1] / r0) ** (1.0 / len(h)))
-            else:
+            if mode == "fp64":
                 # the reference's excision transfers make plain FP64 V-cycles
                 # contract slowly here; report where the capped run ended
                 plain = {"ms": round(dt, 2), "cycles": len(h), "final_rel_residual": float(h[-1] / r0),
                          "converged": bool(h[-1] <= args.rtol * r0),
                          "mean_contraction": float((h[-1] / r0) ** (1.0 / len(h)))}
-    # the same right-hand sides concurrently: one solver per stream, one host
-    # thread each, every solve the single-RHS FGMRES (same bits as alone)
+                continue
+            if len(h) == 0 or h[-1] > args.rtol * r0:
+                raise RuntimeError(f"excised rhs {r} ({mode}) did not converge in {len(h)} steps: {h[-3:]}")
+            times[mode].append(dt)
+            iters[mode].append(len(h))
+        log(f"rhs {r}: " + ", ".join(f"{m} {times[m][-1]:.1f} ms/{iters[m][-1]}" for m in modes if m != "fp64"))
+    S.band_enable(True, False)
+    S.band_set_inner(False)
+    mean = {k: (sum(v) / len(v) if v else None) for k, v in times.items()}
+    best = min(("fgmres", "dc_band"), key=lambda k: mean[k])
+    # the same right-hand sides concurrently by the faster method: one solver
+    # per stream, one host thread each, every solve the single-RHS solve
     conc = None
     if args.c5_streams > 1:
         from pamr_amrd_amd import multirhs as MR
         solvers = [S] + [make_solver() for _ in range(args.c5_streams - 1)]
+        nets = []
+        inner = best.endswith("_in")
+        S.band_set_inner(inner)
+        S.band_enable(True, 2 if best.startswith("fgmres_band") else False)
+        if best != "fgmres":
+            src = net_in if inner else net
+            wts = src.get_weights()
+            for X in solvers[1:]:
+                X.band_setup(3, 26)
+                nx_ = pm.Net(seed=1)
+                nx_.set_weights(wts)
+                if inner:
+                    X.set_band_net_inner(nx_, ts_in)
+                    X.set_band_net(None, 1.0, use_dc=True, use_fgmres=False)
+                else:
+                    X.set_band_net(nx_, ts, use_dc=True, use_fgmres=False)
+                X.band_set_inner(inner)
+                X.band_enable(True, 2 if best.startswith("fgmres_band") else False)
+                nets.append(nx_)
         rhs = [SQ.DriftingSource(seed=1234 + r, speed=0.0).rho(n, 0, torch) for r in range(args.c5_rhs)]
-        for X in solvers:  # allocations (FP32 shadows, Krylov basis) before the timed region
+
+        def solve(X):
+            if best.startswith("dc_band"):
+                return X.solve_rel(3, args.rtol, args.c5_max_iter)
+            return X.solve_fgmres(args.c5_restart, args.rtol, args.c5_max_iter)
+
+        for X in solvers:  # allocations (FP32 shadows, Krylov basis, graphs) before the timed region
             L.pmg_set_solution_lincomb(X.h, pm.tptr(u0), None, 1.0, 0.0)
             X.init()
-            X.solve_fgmres(args.c5_restart, args.rtol, 1)
+            if best.startswith("dc_band"):
+                X.solve_rel(3, args.rtol, 3)
+            else:
+                X.solve_fgmres(args.c5_restart, args.rtol, 2)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        res = MR.solve_fgmres_concurrent(solvers, [pm.tptr(r) for r in rhs], pm.tptr(u0), args.c5_restart,
-                                         args.rtol, args.c5_max_iter)
+        res = MR.solve_concurrent(solvers, [pm.tptr(r) for r in rhs], pm.tptr(u0), solve, args.rtol)
         torch.cuda.synchronize()
         dt = 1e3 * (time.perf_counter() - t0)
-        conc = {"streams": len(solvers), "rhs": len(rhs), "total_ms": round(dt, 1),
-                "ms_per_rhs": round(dt / len(rhs), 2),
-                "preconditioner_applications": [len(h) for _, h in res]}
+        conc = {"method": best, "streams": len(solvers), "rhs": len(rhs), "total_ms": round(dt, 1),
+                "ms_per_rhs": round(dt / len(rhs), 2), "iterations": [len(h) for _, h in res]}
         del rhs
         for X in solvers[1:]:
             X.close()
+        for nx_ in nets:
+            nx_.close()
+    bstats = S.band_stats()
     S.close()
+    net.close()
+    net_in.close()
     del a, c, u0
+    rd = lambda v: None if v is None else round(v, 2)
     return {"workload": f"excised{n}_3spheres_varcoef_16rhs", "n": n, "excised_points": excised,
-            "rhs": args.c5_rhs,
-            "fgmres_ms_per_rhs": round(sum(gtimes) / len(gtimes), 2), "fgmres_total_ms": round(sum(gtimes), 1),
-            "fgmres_restart": args.c5_restart, "fgmres_preconditioner_applications": giters,
-            "mixed_rhs_timed": len(times),
-            "mixed_ms_per_rhs": round(sum(times) / len(times), 2), "mixed_outer_iters": iters,
-            "mixed_mean_contraction": None if not hists else float(np.mean(hists)),
-            "plain_fp64_first_rhs": plain, "fgmres_concurrent": conc,
-            "rtol": args.rtol, "reference_norm": "||r(u0)||"}
+            "rhs": args.c5_rhs, "rtol": args.rtol, "reference_norm": "||r(u0)||",
+            "ms_per_rhs": {k: rd(v) for k, v in mean.items()},
+            "iterations": {k: v for k, v in iters.items()},
+            "speedup_dc_band_vs_dc": (round(mean["dc"] / mean["dc_band"], 3) if mean["dc"] else None),
+            "speedup_dc_band_vs_fgmres": round(mean["fgmres"] / mean["dc_band"], 3),
+            "fastest": best,
+            "fgmres_restart": args.c5_restart,
+            "band": {"depth": 3, "core": 26, "crop": side, "cores": ncore, "points": npts,
+                     "plan_s": round(t_plan, 2), "training": tinfo, "training_in_cycle": tinfo_in,
+                     "stats": bstats,
+                     "roofline_mfma": roof_net},
+            "plain_fp64_first_rhs": plain, "concurrent": conc}
 
 
 def probe_kernels(S, n, pm, torch):
@@ -506,6 +611,9 @@ def main():
     ap.add_argument("--c5-max-iter", type=int, default=300)
     ap.add_argument("--c5-restart", type=int, default=8, help="FGMRES restart length of the excised leg")
     ap.add_argument("--c5-mixed-rhs", type=int, default=2, help="right-hand sides also solved by defect correction")
+    ap.add_argument("--c5-band-train", type=int, default=2,
+                    help="extra right-hand sides whose plain iterations train the band correction")
+    ap.add_argument("--c5-band-steps", type=int, default=1500, help="Adam steps of the band correction")
     ap.add_argument("--c5-streams", type=int, default=4,
                     help="solver instances (streams, host threads) of the concurrent multi-RHS run; 1 skips it")
     ap.add_argument("--probe-net", action="store_true",

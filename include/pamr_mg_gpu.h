@@ -307,6 +307,27 @@ int pmg_band_download(pmg_solver *S, unsigned char *host);
  * plain to measure the plain rate; after a worse solve with the correction
  * the next four run plain). */
 int pmg_set_band_net(pmg_solver *S, pmg_net *N, double ts, int use_dc, int use_fgmres);
+/* switch the two uses on or off without resetting the measurements (plain
+ * FGMRES solves measure the plain rate whenever a network is attached);
+ * use_fgmres = 2 applies the correction in every FGMRES solve regardless of
+ * the measured rates (to measure its effect) */
+int pmg_band_enable(pmg_solver *S, int use_dc, int use_fgmres);
+/* place the correction inside the FP32 V-cycle, after the finest level's
+ * coarse-grid correction and before its post-smoothing (on = 1), instead of
+ * after the cycle (0, the default): u32 += ts * |src| * N(r32 / |src|) with
+ * r32 the FP32 residual at that moment and |src| the RMS of the cycle's
+ * finest source (the outer defect, or 1/sqrt(updatable points) for the unit
+ * Krylov vectors of FGMRES) */
+int pmg_band_set_inner(pmg_solver *S, int on);
+/* the network of the in-cycle placement and its output scale (NULL detaches) */
+int pmg_set_band_net_inner(pmg_solver *S, pmg_net *N, double ts);
+/* training crops for the in-cycle placement: one mixed iteration whose FP32
+ * cycle writes, after the finest coarse-grid correction, x = r32 / |r| and
+ * t = (ustar - u - u32) / |r| (|r| the outer defect's RMS), w = band, on
+ * crops centred on seeded random band points; *rnorm = the outer defect's
+ * RMS after the iteration.  Needs the state of a mode-1 pmg_solve_rel. */
+int pmg_band_sample_inner(pmg_solver *S, const double *ustar, int ncrop, unsigned long long seed, float *x, float *t,
+                          float *w, double *rnorm);
 /* accepted / rejected trials; out6 = plain and corrected contraction and ms
  * per defect-correction iteration, plain and corrected FGMRES rates
  * (log10 of the residual reduction per ms) */
@@ -353,6 +374,16 @@ int pmg_band_resid(int op, int do_ex, const double *u, const double *rho, const 
 int pmg_band_pairs(const float *xsrc, float xs, const double *ustar, const double *u, double ts,
                    const unsigned char *band, int nx, int ny, int nz, const int *org, int ncrop, int side, float *xo,
                    float *to, float *wo, void *stream);
+/* the same inside the FP32 cycle: gather x = mul / *norm * r32 (r32 the
+ * FP32 residual of u with source rho, rhs 0; 0 off the updatable points);
+ * pairs x = r32 / *norm, t = (ustar - u64 - u) / *norm, w = band */
+int pmg_band_gather_res32(int op, int do_ex, const float *u, const float *rho, const float *a, const float *c,
+                          const unsigned char *chr, int nx, int ny, int nz, const int *org, int ncrop, int side,
+                          float mul, const double *norm, double dx, double dy, double dz, float *out, void *stream);
+int pmg_band_pairs_res32(int op, int do_ex, const float *u, const float *rho, const float *a, const float *c,
+                         const unsigned char *chr, const double *ustar, const double *u64, const unsigned char *band,
+                         int nx, int ny, int nz, const int *org, int ncrop, int side, const double *norm, double dx,
+                         double dy, double dz, float *xo, float *to, float *wo, void *stream);
 /* lane-layout self-test of the BF16 16x16x16 MFMA used by the network:
  * D = bf16(A) * bf16(B) (host 16x16 row-major) */
 int pmg_net_mfma_selftest(const float *A, const float *B, float *D_out);

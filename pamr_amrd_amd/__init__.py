@@ -122,6 +122,19 @@ def _declare(L):
     L.pmg_band_download.argtypes = [_vp, u8p]
     L.pmg_set_band_net.argtypes = [_vp, _vp, d, i, i]
     L.pmg_band_stats.argtypes = [_vp, _ip, _ip, _dp]
+    L.pmg_band_enable.argtypes = [_vp, i, i]
+    L.pmg_band_set_inner.argtypes = [_vp, i]
+    f = C.c_float
+    L.pmg_band_gather_f32.argtypes = [_vp, i, i, i, _vp, i, i, f, _vp, _vp, _vp]
+    L.pmg_band_scatter.argtypes = [_vp, _vp, _vp, _vp, _vp, i, i, i, _vp, i, i, i, d, _vp, _vp]
+    L.pmg_band_resid.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, _vp, i, i, i, _vp, i, i, i, d, d, d, _vp]
+    L.pmg_band_pairs.argtypes = [_vp, f, _vp, _vp, d, _vp, i, i, i, _vp, i, i, _vp, _vp, _vp, _vp]
+    L.pmg_band_gather_res32.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, i, i, i, _vp, i, i, f, _vp, d, d, d, _vp,
+                                        _vp]
+    L.pmg_band_pairs_res32.argtypes = [i, i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, i, i, i, _vp, i, i, _vp, d, d,
+                                       d, _vp, _vp, _vp, _vp]
+    L.pmg_set_band_net_inner.argtypes = [_vp, _vp, d]
+    L.pmg_band_sample_inner.argtypes = [_vp, _vp, i, C.c_ulonglong, _vp, _vp, _vp, _dp]
     L.pmg_band_sample.argtypes = [_vp, _vp, d, d, i, C.c_ulonglong, _vp, _vp, _vp]
     L.pmg_band_correct.argtypes = [_vp, i]
     L.pmg_download32.argtypes = [_vp, i, i, C.POINTER(C.c_float)]
@@ -476,6 +489,24 @@ class Solver:
         self._band_net = net
         _check(self._L.pmg_set_band_net(self.h, net.h if net is not None else None, ts, int(use_dc),
                                         int(use_fgmres)), "pmg_set_band_net")
+
+    def band_enable(self, use_dc, use_fgmres):
+        """use_fgmres: False, True (policy: kept while faster), or 2 (always)"""
+        _check(self._L.pmg_band_enable(self.h, int(use_dc), int(use_fgmres)), "pmg_band_enable")
+
+    def set_band_net_inner(self, net, ts):
+        self._band_net_in = net
+        _check(self._L.pmg_set_band_net_inner(self.h, net.h if net is not None else None, ts),
+               "pmg_set_band_net_inner")
+
+    def band_set_inner(self, on):
+        _check(self._L.pmg_band_set_inner(self.h, int(on)), "pmg_band_set_inner")
+
+    def band_sample_inner(self, ustar_ptr, ncrop, seed, x_ptr, t_ptr, w_ptr):
+        r = C.c_double(0)
+        _check(self._L.pmg_band_sample_inner(self.h, ustar_ptr, ncrop, seed, x_ptr, t_ptr, w_ptr, C.byref(r)),
+               "pmg_band_sample_inner")
+        return r.value
 
     def band_stats(self):
         a, r = C.c_int(0), C.c_int(0)

@@ -65,6 +65,31 @@ def collect_pairs(S, rhos, ustars, u0, torch, iters=10, crops=48, seed=77):
     return x, t, w
 
 
+def collect_pairs_inner(S, rhos, ustars, u0, torch, iters=10, crops=48, seed=91):
+    """(x, t, w) crop tensors for the in-cycle placement: inside each of
+    `iters` mixed iterations of every training right-hand side, after the
+    finest coarse-grid correction of the FP32 cycle (pmg_band_sample_inner);
+    t is the error of the cycle's correction at that moment over the outer
+    defect's RMS (not yet divided by ts)."""
+    _, ncore, side = S.band_info()
+    if side <= 0:
+        raise RuntimeError("band_setup first")
+    total = len(rhos) * iters * crops
+    x = torch.empty((total, side, side, side), dtype=torch.float32, device="cuda")
+    t = torch.empty_like(x)
+    w = torch.empty_like(x)
+    at = 0
+    for k, (rp, up) in enumerate(zip(rhos, ustars)):
+        set_state(S, rp, u0)
+        S.solve_rel(1, 0.0, 0)  # the outer defect of u0 as the first FP32 source
+        for it in range(iters):
+            torch.cuda.synchronize()
+            S.band_sample_inner(tptr(up), crops, seed + 1000 * k + it, tptr(x[at:]), tptr(t[at:]), tptr(w[at:]))
+            at += crops
+    torch.cuda.synchronize()
+    return x, t, w
+
+
 def train(x, t, w, torch, steps=1500, batch=16, lr=2e-3, seed=5, margin=3, net=None, log=None):
     """Train a network on crop tensors (targets scaled here by 1/ts, ts the
     RMS of t over the weighted core points).  Returns (net, ts, info)."""

@@ -137,6 +137,91 @@ __global__ __launch_bounds__(256) void k_band_pairs(const float *__restrict__ xs
   wo[o] = wv;
 }
 
+// The FP32 residual hook r = L32[u] (source rho, finest rhs = 0) at crop
+// point p, or 0 where p is outside the grid or not updatable.
+template <int OP, bool EX, bool P2>
+__device__ __forceinline__ float band_res32(const float *__restrict__ u, const float *__restrict__ rho,
+                                            const float *__restrict__ a, const float *__restrict__ cc,
+                                            const uint8_t *__restrict__ chr, int nx, int ny, int nz, int i, int j,
+                                            int k, const Spacing<float> &sp) {
+  if (i < 1 || i > nx - 2 || j < 1 || j > ny - 2 || k < 1 || k > nz - 2) return 0.f;
+  const size_t p = ix3(i, j, k, nx, ny);
+  if constexpr (EX) { if (chr[p]) return 0.f; }
+  const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+  const float L = lop_at<float, OP, P2>(u, rho, a, cc, p, sy, sz, sp, (float *)nullptr);
+  return L - 0.0f;
+}
+
+// network input inside the FP32 V-cycle: out[c][q] = mul / *norm * r32(p)
+template <int OP, bool EX, bool P2>
+__global__ __launch_bounds__(256) void k_band_gather_res(const float *__restrict__ u, const float *__restrict__ rho,
+                                                         const float *__restrict__ a, const float *__restrict__ cc,
+                                                         const uint8_t *__restrict__ chr, int nx, int ny, int nz,
+                                                         const int *__restrict__ org, int side, float mul,
+                                                         const double *__restrict__ norm, Spacing<float> sp,
+                                                         float *__restrict__ out) {
+  const int c = blockIdx.y;
+  const size_t n = (size_t)side * side * side;
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  int x, y, z;
+  crop_xyz(q, side, x, y, z);
+  const int i = org[3 * c] + x, j = org[3 * c + 1] + y, k = org[3 * c + 2] + z;
+  const float r = band_res32<OP, EX, P2>(u, rho, a, cc, chr, nx, ny, nz, i, j, k, sp);
+  out[(size_t)c * n + q] = (float)(((double)mul / *norm) * (double)r);
+}
+
+// training pairs inside the FP32 V-cycle: x = r32 / *norm, t = (ustar - u64 -
+// u32) / *norm (the error of the cycle's current correction), w = band
+template <int OP, bool EX, bool P2>
+__global__ __launch_bounds__(256) void k_band_pairs_res(const float *__restrict__ u, const float *__restrict__ rho,
+                                                        const float *__restrict__ a, const float *__restrict__ cc,
+                                                        const uint8_t *__restrict__ chr,
+                                                        const double *__restrict__ ustar,
+                                                        const double *__restrict__ u64,
+                                                        const uint8_t *__restrict__ band, int nx, int ny, int nz,
+                                                        const int *__restrict__ org, int side,
+                                                        const double *__restrict__ norm, Spacing<float> sp,
+                                                        float *__restrict__ xo, float *__restrict__ to,
+                                                        float *__restrict__ wo) {
+  const int c = blockIdx.y;
+  const size_t n = (size_t)side * side * side;
+  const size_t q = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (q >= n) return;
+  int x, y, z;
+  crop_xyz(q, side, x, y, z);
+  const int i = org[3 * c] + x, j = org[3 * c + 1] + y, k = org[3 * c + 2] + z;
+  const double s = 1.0 / *norm;
+  const float r = band_res32<OP, EX, P2>(u, rho, a, cc, chr, nx, ny, nz, i, j, k, sp);
+  float tv = 0.f, wv = 0.f;
+  if (i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz) {
+    const size_t p = ix3(i, j, k, nx, ny);
+    tv = (float)(s * (ustar[p] - u64[p] - (double)u[p]));
+    wv = band[p] ? 1.f : 0.f;
+  }
+  const size_t o = (size_t)c * n + q;
+  xo[o] = (float)(s * (double)r);
+  to[o] = tv;
+  wo[o] = wv;
+}
+
+// FP32 spacing as the solver builds it for its FP32 cycles
+Spacing<float> spacing32(double hx, double hy, double hz) {
+  Spacing<float> sp;
+  const float dx = (float)hx, dy = (float)hy, dz = (float)hz;
+  sp.dx2 = dx * dx;
+  sp.dy2 = dy * dy;
+  sp.dz2 = dz * dz;
+  sp.idx2 = 1.0f / sp.dx2;
+  sp.idy2 = 1.0f / sp.dy2;
+  sp.idz2 = 1.0f / sp.dz2;
+  sp.jac = -2 / sp.dx2 - 2 / sp.dy2 - 2 / sp.dz2;
+  sp.ijac = 1 / sp.jac;
+  auto p2 = [](double v) { int e; return std::frexp(v, &e) == 0.5 && v > 1e-300 && v < 1e300; };
+  sp.pow2 = p2((double)sp.dx2) && p2((double)sp.dy2) && p2((double)sp.dz2);
+  return sp;
+}
+
 inline unsigned nblk(size_t n) { return (unsigned)((n + 255) / 256); }
 
 }  // namespace
@@ -264,6 +349,55 @@ int pmg_band_resid(int op, int do_ex, const double *u, const double *rho, const 
     else { if (sp.pow2) PMG_BR(1, false, true); else PMG_BR(1, false, false); }
   }
 #undef PMG_BR
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int pmg_band_gather_res32(int op, int do_ex, const float *u, const float *rho, const float *a, const float *c,
+                          const unsigned char *chr, int nx, int ny, int nz, const int *org, int ncrop, int side,
+                          float mul, const double *norm, double dx, double dy, double dz, float *out, void *stream) {
+  if (!u || !rho || !org || !norm || !out || ncrop < 1 || (op == 1 && (!a || !c)) || (do_ex && !chr)) return -1;
+  const Spacing<float> sp = spacing32(dx, dy, dz);
+  const dim3 gr(nblk((size_t)side * side * side), ncrop);
+  hipStream_t st = (hipStream_t)stream;
+  const uint8_t *ch = (const uint8_t *)chr;
+#define PMG_BG(OPV, EXV, P2V)                                                                                     \
+  hipLaunchKernelGGL((k_band_gather_res<OPV, EXV, P2V>), gr, dim3(256), 0, st, u, rho, a, c, ch, nx, ny, nz, org, side, \
+                     mul, norm, sp, out)
+  if (op == 0) {
+    if (do_ex) { if (sp.pow2) PMG_BG(0, true, true); else PMG_BG(0, true, false); }
+    else { if (sp.pow2) PMG_BG(0, false, true); else PMG_BG(0, false, false); }
+  } else {
+    if (do_ex) { if (sp.pow2) PMG_BG(1, true, true); else PMG_BG(1, true, false); }
+    else { if (sp.pow2) PMG_BG(1, false, true); else PMG_BG(1, false, false); }
+  }
+#undef PMG_BG
+  PMG_HOST_CHECK(hipGetLastError());
+  return 0;
+}
+
+int pmg_band_pairs_res32(int op, int do_ex, const float *u, const float *rho, const float *a, const float *c,
+                         const unsigned char *chr, const double *ustar, const double *u64, const unsigned char *band,
+                         int nx, int ny, int nz, const int *org, int ncrop, int side, const double *norm, double dx,
+                         double dy, double dz, float *xo, float *to, float *wo, void *stream) {
+  if (!u || !rho || !ustar || !u64 || !band || !org || !norm || !xo || !to || !wo || ncrop < 1 ||
+      (op == 1 && (!a || !c)) || (do_ex && !chr))
+    return -1;
+  const Spacing<float> sp = spacing32(dx, dy, dz);
+  const dim3 gr(nblk((size_t)side * side * side), ncrop);
+  hipStream_t st = (hipStream_t)stream;
+  const uint8_t *ch = (const uint8_t *)chr, *bd = (const uint8_t *)band;
+#define PMG_BP(OPV, EXV, P2V)                                                                                    \
+  hipLaunchKernelGGL((k_band_pairs_res<OPV, EXV, P2V>), gr, dim3(256), 0, st, u, rho, a, c, ch, ustar, u64, bd, nx, ny, \
+                     nz, org, side, norm, sp, xo, to, wo)
+  if (op == 0) {
+    if (do_ex) { if (sp.pow2) PMG_BP(0, true, true); else PMG_BP(0, true, false); }
+    else { if (sp.pow2) PMG_BP(0, false, true); else PMG_BP(0, false, false); }
+  } else {
+    if (do_ex) { if (sp.pow2) PMG_BP(1, true, true); else PMG_BP(1, true, false); }
+    else { if (sp.pow2) PMG_BP(1, false, true); else PMG_BP(1, false, false); }
+  }
+#undef PMG_BP
   PMG_HOST_CHECK(hipGetLastError());
   return 0;
 }

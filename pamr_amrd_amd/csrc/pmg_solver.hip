@@ -181,6 +181,21 @@ struct pmg_solver {
     int accepted = 0, rejected = 0;
     hipGraphExec_t gdc = nullptr;   // captured correction of the defect-correction iteration
     hipGraphExec_t gpc = nullptr;   // captured correction of the preconditioner output
+    // the correction inside the FP32 V-cycle, after the finest level's
+    // coarse-grid correction (instead of after the cycle): on when inner = 1
+    int inner = 0;
+    pmg_net *net_in = nullptr;      // its network and output scale (trained on in-cycle crops)
+    double ts_in = 1.0;
+    long long n_upd = 0;            // updatable finest points (the RMS norms' count)
+    bool enq_inner = false;         // set while an FP32 cycle with it is enqueued
+    const double *cur_norm = nullptr;  // RMS of that cycle's finest source (device)
+    hipGraphExec_t g32i = nullptr;     // mixed iteration with the in-cycle correction
+    hipGraphExec_t gpreci = nullptr;   // preconditioner with the in-cycle correction
+    // training crops taken inside the cycle instead of correcting (pmg_band_sample_inner)
+    const double *samp_ustar = nullptr;
+    int *samp_org = nullptr;
+    int samp_n = 0;
+    float *samp_x = nullptr, *samp_t = nullptr, *samp_w = nullptr;
   } band;
 };
 
@@ -195,6 +210,7 @@ enum {
   kNormSlots = 2 * kMaxLev + 8,
   NS_F32 = kNormSlots,          // FP32 inner-cycle slots start here
   NS_COARSE = 2 * kNormSlots,   // coarse solve outputs: pre64, post64, pre32, post32
+  NS_PSRC = 2 * kNormSlots + 4, // RMS of the preconditioner's source (unit 2-norm Krylov vector)
   kNormTotal = 2 * kNormSlots + 8
 };
 
@@ -783,6 +799,8 @@ int resid_restrict(pmg_solver *S, int slot) {
   return p2 ? launch_resid_restrict_t<T, 1, true>(S, P) : launch_resid_restrict_t<T, 1, false>(S, P);
 }
 
+int band_inner(pmg_solver *S);
+
 template <typename T>
 int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cnorm, int *citer) {
   const int Lf = S->nlev;
@@ -815,6 +833,10 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
   }
   for (int L = Lb ? Lb + 1 : 2; L <= Lf; L++) {
     if ((rc = cgc<T>(S, L))) return rc;
+    if constexpr (std::is_same<T, float>::value) {
+      // the learned band correction of the finest level's coarse-grid correction
+      if (L == Lf && S->band.enq_inner && (rc = band_inner(S))) return rc;
+    }
     if ((rc = sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
     if ((rc = settle<T>(S, L))) return rc;
   }
@@ -929,12 +951,57 @@ int enq_band_pc(pmg_solver *S) {
                           B.side, B.margin, B.ts, nrm, S->stream);
 }
 
+// Inside the FP32 V-cycle, after the finest coarse-grid correction: u32 +=
+// ts * |src| * N(r32 / |src|) at the band points, r32 the FP32 residual at
+// that moment and |src| the RMS of the cycle's finest source (the cycle is a
+// linear map of its source, so this scaling is exact); or, while training
+// crops are requested, gather them instead.
+int band_inner(pmg_solver *S) {
+  Level &F = S->lev[S->nlev - 1];
+  auto &B = S->band;
+  if (!B.cur_norm || B.ncore < 1) return -1;
+  float *u32 = ucur<float>(F);
+  if (B.samp_x)
+    return pmg_band_pairs_res32(S->op, S->do_ex, u32, F.f.rho, F.f.a, F.f.c, F.chr, B.samp_ustar, F.d.u, B.d_band,
+                                F.n[0], F.n[1], F.n[2], B.samp_org, B.samp_n, B.side, B.cur_norm, F.hdx[0], F.hdx[1],
+                                F.hdx[2], B.samp_x, B.samp_t, B.samp_w, S->stream);
+  if (!B.net_in) return -1;
+  if (pmg_band_gather_res32(S->op, S->do_ex, u32, F.f.rho, F.f.a, F.f.c, F.chr, F.n[0], F.n[1], F.n[2], B.d_org,
+                            B.ncore, B.side, 1.0f, B.cur_norm, F.hdx[0], F.hdx[1], F.hdx[2], B.d_x, S->stream))
+    return -1;
+  const float *y = pmg_net_infer_async(B.net_in, B.d_x, 1.0f, B.side, B.side, B.side, B.ncore, S->stream);
+  if (!y) return -1;
+  return pmg_band_scatter(nullptr, nullptr, u32, y, B.d_band, F.n[0], F.n[1], F.n[2], B.d_org, B.ncore, B.side,
+                          B.margin, B.ts_in, B.cur_norm, S->stream);
+}
+
+int enq_mixed_inner(pmg_solver *S) {
+  S->band.enq_inner = true;
+  S->band.cur_norm = S->d_norms + NS_OUTER;
+  const int rc = enq_mixed(S);
+  S->band.enq_inner = false;
+  return rc;
+}
+
+int enq_prec_inner(pmg_solver *S) {
+  S->band.enq_inner = true;
+  S->band.cur_norm = S->d_norms + NS_PSRC;
+  const int rc = enqueue_vcycle<float>(S, false, NS_F32, S->d_norms + NS_COARSE + 2, S->d_citer + 1);
+  S->band.enq_inner = false;
+  return rc;
+}
+
+void band_graphs_reset(pmg_solver *S) {
+  auto &B = S->band;
+  for (hipGraphExec_t *g : {&B.gdc, &B.gpc, &B.g32i, &B.gpreci})
+    if (*g) { (void)hipGraphExecDestroy(*g); *g = nullptr; }
+}
+
 void invalidate_graphs(pmg_solver *S) {
   if (S->g64) { (void)hipGraphExecDestroy(S->g64); S->g64 = nullptr; }
   if (S->g32) { (void)hipGraphExecDestroy(S->g32); S->g32 = nullptr; }
   if (S->gprec) { (void)hipGraphExecDestroy(S->gprec); S->gprec = nullptr; }
-  if (S->band.gdc) { (void)hipGraphExecDestroy(S->band.gdc); S->band.gdc = nullptr; }
-  if (S->band.gpc) { (void)hipGraphExecDestroy(S->band.gpc); S->band.gpc = nullptr; }
+  band_graphs_reset(S);
 }
 
 }  // namespace
@@ -1454,17 +1521,29 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
     // measured, running means); the first iteration that does not ends the
     // correction for the rest of the solve (plain iterations from there on).
     auto &B = S->band;
-    if (!B.net || B.ncore < 1) return -1;
-    if ((rc = pmg_net_reserve(B.net, (long long)B.ncore * B.side * B.side * B.side))) return rc;
+    pmg_net *act = B.inner ? B.net_in : B.net;
+    if (!act || B.ncore < 1) return -1;
+    if ((rc = pmg_net_reserve(act, (long long)B.ncore * B.side * B.side * B.side))) return rc;
     EventPair ev;
     PMG_HOST_CHECK(ev.create());
     bool net_on = B.use_dc != 0;
     while (it < max_iter && r > rtol * r0) {
-      const bool with_net = net_on && B.rho_plain > 0 && B.t_plain > 0;
-      const bool fresh = with_net ? (!B.gdc || !S->g32) : !S->g32;  // graph capture in this iteration
+      // the first iteration of a solve is plain and not measured: its
+      // defect (of the initial guess) is unlike the post-cycle defects the
+      // network was trained on, and its contraction unlike the later ones
+      const bool first = it == 0;
+      const bool with_net = !first && net_on && B.rho_plain > 0 && B.t_plain > 0;
+      bool fresh;  // graph capture in this iteration
+      if (with_net && B.inner) fresh = !B.g32i;
+      else if (with_net) fresh = !B.gdc || !S->g32;
+      else fresh = !S->g32;
       PMG_HOST_CHECK(hipEventRecord(ev.e0, S->stream));
-      if (with_net && (rc = capture_or_run(S, &B.gdc, enq_band_dc))) return rc;
-      if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
+      if (with_net && B.inner) {
+        if ((rc = capture_or_run(S, &B.g32i, enq_mixed_inner))) return rc;
+      } else {
+        if (with_net && (rc = capture_or_run(S, &B.gdc, enq_band_dc))) return rc;
+        if ((rc = capture_or_run(S, &S->g32, enq_mixed))) return rc;
+      }
       PMG_HOST_CHECK(hipEventRecord(ev.e1, S->stream));
       PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost,
                                     S->stream));
@@ -1483,7 +1562,7 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
           B.rejected++;
           net_on = false;
         }
-      } else if (rho > 0 && rho < 1.0) {
+      } else if (!first && rho > 0 && rho < 1.0) {
         if (!fresh) B.t_plain = B.t_plain < 0 ? ms : 0.5 * (B.t_plain + ms);
         B.rho_plain = B.rho_plain < 0 ? rho : 0.5 * (B.rho_plain + rho);
       }
@@ -2054,12 +2133,22 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
   // once a plain solve has measured the plain rate, kept while a solve with
   // it reduces the residual faster per unit time than the plain solves do
   auto &B = S->band;
-  bool band_on = B.use_fgmres && B.net && B.ncore > 0 && B.fg_rate_plain > 0;
-  if (band_on && B.fg_skip > 0) {
+  pmg_net *act = B.inner ? B.net_in : B.net;
+  // use_fgmres 2: always on (a measurement of the correction's effect, no policy)
+  bool band_on = B.use_fgmres && act && B.ncore > 0 && (B.fg_rate_plain > 0 || B.use_fgmres == 2);
+  if (band_on && B.use_fgmres != 2 && B.fg_skip > 0) {
     B.fg_skip--;
     band_on = false;
   }
-  if (band_on && (rc = pmg_net_reserve(B.net, (long long)B.ncore * B.side * B.side * B.side))) return rc;
+  if (band_on && (rc = pmg_net_reserve(act, (long long)B.ncore * B.side * B.side * B.side))) return rc;
+  if (band_on && B.inner) {
+    // the preconditioner's source is a Krylov vector of unit 2-norm: its RMS
+    // over the updatable points scales the in-cycle correction
+    S->h_pinned[1] = B.n_upd > 0 ? 1.0 / std::sqrt((double)B.n_upd) : 1.0;
+    PMG_HOST_CHECK(hipMemcpyAsync(S->d_norms + NS_PSRC, S->h_pinned + 1, sizeof(double), hipMemcpyHostToDevice,
+                                  S->stream));
+    PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  }
   const double r_start = r_rms;
   const auto t_start = std::chrono::steady_clock::now();
   while (it < max_iter && r_rms > rtol * rref) {
@@ -2079,8 +2168,12 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
       // Z_j = M V_j: the FP32 V-cycle from zero with V_j as its source
       hipLaunchKernelGGL(k_kry_prep, dim3(nb), dim3(256), 0, S->stream, F.f.rho, F.f.u, V[j], n);
       PMG_HOST_CHECK(hipGetLastError());
-      if ((rc = capture_or_run(S, &S->gprec, enq_prec))) return rc;
-      if (band_on && (rc = capture_or_run(S, &B.gpc, enq_band_pc))) return rc;
+      if (band_on && B.inner) {
+        if ((rc = capture_or_run(S, &B.gpreci, enq_prec_inner))) return rc;
+      } else {
+        if ((rc = capture_or_run(S, &S->gprec, enq_prec))) return rc;
+        if (band_on && (rc = capture_or_run(S, &B.gpc, enq_band_pc))) return rc;
+      }
       hipLaunchKernelGGL(k_kry_to64, dim3(nb), dim3(256), 0, S->stream, Z[j], F.f.u, n);
       PMG_HOST_CHECK(hipGetLastError());
       // V_{j+1} = A Z_j, orthogonalised against V_0..V_j by classical
@@ -2139,7 +2232,7 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
     if ((rc = true_residual(&r_rms))) return rc;
     if (hist && it > 0) hist[it - 1] = r_rms;
   }
-  if (B.use_fgmres && B.net && B.ncore > 0 && it > 0 && r_rms > 0 && r_start > r_rms) {
+  if ((B.net || B.net_in) && B.ncore > 0 && it > 0 && r_rms > 0 && r_start > r_rms) {
     const double ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     const double rate = std::log10(r_start / r_rms) / ms;
@@ -2222,7 +2315,6 @@ int pmg_band_setup(pmg_solver *S, int depth, int core) {
   std::vector<unsigned char> band(F.size);
   const long long np = pmg_band_mask(chr.data(), nx, ny, nz, S->ex, depth, band.data());
   if (np < 0) return -1;
-  std::vector<double>().swap(chr);
   const int margin = 3;
   const int maxc = ((nx + core - 1) / core) * ((ny + core - 1) / core) * ((nz + core - 1) / core);
   std::vector<int> org(3 * (size_t)maxc);
@@ -2242,6 +2334,12 @@ int pmg_band_setup(pmg_solver *S, int depth, int core) {
     PMG_HOST_CHECK(hipMemcpy(B.d_org, org.data(), sizeof(int) * 3 * (size_t)nc, hipMemcpyHostToDevice));
     PMG_HOST_CHECK(hipMalloc(&B.d_x, sizeof(float) * (size_t)nc * side * side * side));
   }
+  long long nupd = 0;
+  for (int k = 1; k < nz - 1; k++)
+    for (int j = 1; j < ny - 1; j++)
+      for (int i = 1; i < nx - 1; i++)
+        if (chr[ix3(i, j, k, nx, ny)] != S->ex) nupd++;
+  B.n_upd = nupd;
   B.h_pts.clear();
   B.h_pts.reserve((size_t)np);
   for (size_t p = 0; p < F.size; p++)
@@ -2252,8 +2350,7 @@ int pmg_band_setup(pmg_solver *S, int depth, int core) {
   B.margin = margin;
   B.ncore = nc;
   B.npts = np;
-  if (B.gdc) { (void)hipGraphExecDestroy(B.gdc); B.gdc = nullptr; }
-  if (B.gpc) { (void)hipGraphExecDestroy(B.gpc); B.gpc = nullptr; }
+  band_graphs_reset(S);
   return nc;
 }
 
@@ -2283,8 +2380,78 @@ int pmg_set_band_net(pmg_solver *S, pmg_net *N, double ts, int use_dc, int use_f
   B.fg_rate_net = 0;
   B.fg_skip = 0;
   B.accepted = B.rejected = 0;
-  if (B.gdc) { (void)hipGraphExecDestroy(B.gdc); B.gdc = nullptr; }
-  if (B.gpc) { (void)hipGraphExecDestroy(B.gpc); B.gpc = nullptr; }
+  band_graphs_reset(S);
+  return 0;
+}
+
+int pmg_set_band_net_inner(pmg_solver *S, pmg_net *N, double ts) {
+  if (!S || (N && !(ts > 0))) return -1;
+  S->band.net_in = N;
+  S->band.ts_in = ts;
+  band_graphs_reset(S);
+  return 0;
+}
+
+int pmg_band_set_inner(pmg_solver *S, int on) {
+  if (!S) return -1;
+  S->band.inner = on ? 1 : 0;
+  return 0;
+}
+
+int pmg_band_sample_inner(pmg_solver *S, const double *ustar, int ncrop, unsigned long long seed, float *x, float *t,
+                          float *w, double *rnorm) {
+  // one mixed iteration (no graph) whose FP32 cycle, after the finest
+  // coarse-grid correction, writes training crops centred on seeded random
+  // band points: x = r32 / |r|, t = (ustar - u - u32) / |r| (the error of the
+  // cycle's correction at that moment), w = band; |r| = the outer defect's
+  // RMS.  Needs the state a mode-1 pmg_solve_rel leaves (FP32 source -r).
+  // *rnorm: the outer defect's RMS after the iteration.
+  if (!S || !ustar || !x || !t || !w || ncrop < 1 || !S->fp32_ready) return -1;
+  auto &B = S->band;
+  if (B.h_pts.empty()) return -1;
+  Level &F = S->lev[S->nlev - 1];
+  const int nx = F.n[0], ny = F.n[1];
+  std::vector<int> org(3 * (size_t)ncrop);
+  unsigned long long z = seed;
+  for (int c = 0; c < ncrop; c++) {
+    z += 0x9E3779B97F4A7C15ull;  // splitmix64
+    unsigned long long v = z;
+    v = (v ^ (v >> 30)) * 0xBF58476D1CE4E5B9ull;
+    v = (v ^ (v >> 27)) * 0x94D049BB133111EBull;
+    v ^= v >> 31;
+    const long long p = B.h_pts[(size_t)(v % B.h_pts.size())];
+    org[3 * c] = (int)(p % nx) - B.side / 2;
+    org[3 * c + 1] = (int)((p / nx) % ny) - B.side / 2;
+    org[3 * c + 2] = (int)(p / ((long long)nx * ny)) - B.side / 2;
+  }
+  int *dorg = nullptr;
+  PMG_HOST_CHECK(hipMalloc(&dorg, sizeof(int) * org.size()));
+  PMG_HOST_CHECK(hipMemcpyAsync(dorg, org.data(), sizeof(int) * org.size(), hipMemcpyHostToDevice, S->stream));
+  B.samp_ustar = ustar;
+  B.samp_org = dorg;
+  B.samp_n = ncrop;
+  B.samp_x = x;
+  B.samp_t = t;
+  B.samp_w = w;
+  B.enq_inner = true;
+  B.cur_norm = S->d_norms + NS_OUTER;
+  int rc = enq_mixed(S);
+  B.enq_inner = false;
+  B.samp_x = B.samp_t = B.samp_w = nullptr;
+  B.samp_ustar = nullptr;
+  B.samp_org = nullptr;
+  PMG_HOST_CHECK(hipMemcpyAsync(S->h_pinned, S->d_norms + NS_OUTER, sizeof(double), hipMemcpyDeviceToHost,
+                                S->stream));
+  PMG_HOST_CHECK(hipStreamSynchronize(S->stream));
+  PMG_HOST_CHECK(hipFree(dorg));
+  if (rnorm) *rnorm = S->h_pinned[0];
+  return rc;
+}
+
+int pmg_band_enable(pmg_solver *S, int use_dc, int use_fgmres) {
+  if (!S) return -1;
+  S->band.use_dc = use_dc;
+  S->band.use_fgmres = use_fgmres;
   return 0;
 }
 

@@ -40,8 +40,15 @@ def _copy(dst, src, n, stream):
 
 
 def solve_fgmres_concurrent(solvers, rhs, u0, m, rtol, max_iter, out=None):
+    """Flexible GMRES(m) with the FP32 V-cycle as preconditioner for every
+    right-hand side (solve_concurrent with that solve)."""
+    return solve_concurrent(solvers, rhs, u0, lambda S: S.solve_fgmres(m, rtol, max_iter), rtol, out)
+
+
+def solve_concurrent(solvers, rhs, u0, solve, rtol, out=None):
     """Solve finest-level sources ``rhs`` (device pointers to n^3 FP64 grids)
-    by flexible GMRES(m) with the FP32 V-cycle as preconditioner, each to
+    with ``solve(S)`` (a relative solve of the solver's current source and
+    initial guess returning (reference norm, residual history)), each to
     ||r|| <= rtol ||r(u0)||.  Right-hand side k runs on ``solvers[k %
     len(solvers)]``; all solvers must hold the same problem (shape,
     coefficients, excision mask, parameters).  ``u0``: device pointer of the
@@ -72,7 +79,7 @@ def solve_fgmres_concurrent(solvers, rhs, u0, m, rtol, max_iter, out=None):
                 elif L.pmg_set_solution_lincomb(S.h, u0, None, 1.0, 0.0) != 0:
                     raise RuntimeError("pmg_set_solution_lincomb failed")
                 S.init()
-                r0, h = S.solve_fgmres(m, rtol, max_iter)
+                r0, h = solve(S)
                 if len(h) == 0 or h[-1] > rtol * r0:
                     raise RuntimeError(f"right-hand side {k} did not converge in {len(h)} steps")
                 if out is not None:
