@@ -428,12 +428,12 @@ def excised_leg(args, pm, torch):
             t0 = time.perf_counter()
             L.pmg_set_solution_lincomb(S.h, pm.tptr(u0), None, 1.0, 0.0)
             S.init()
-            if mode in ("fgmres", "fgmres_band"):
+            if mode.startswith("fgmres"):
                 r0, h = S.solve_fgmres(args.c5_restart, args.rtol, args.c5_max_iter)
             elif mode == "fp64":
                 r0, h = S.solve_rel(0, args.rtol, args.c5_plain_cycles)
             else:
-                r0, h = S.solve_rel(3 if mode == "dc_band" else 1, args.rtol, args.c5_max_iter)
+                r0, h = S.solve_rel(3 if mode.startswith("dc_band") else 1, args.rtol, args.c5_max_iter)
             torch.cuda.synchronize()
             dt = 1e3 * (time.perf_counter() - t0)
             if mode == "fp64":

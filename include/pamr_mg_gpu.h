@@ -167,6 +167,10 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
 int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hist, int *iters,
                      double *r0_out);
 int pmg_last_citer(pmg_solver *S, int which);
+/* FP32 V-cycles per FGMRES preconditioner application (1..4, default 1);
+ * with the band correction on (and more than one cycle) it runs between
+ * the cycles instead of after the single one */
+int pmg_set_fgmres_prec_cycles(pmg_solver *S, int cycles);
 /* initial guess u = 0 on the finest level (device-side) */
 int pmg_zero_solution(pmg_solver *S);
 /* average duration (ms) of one launch of a solver kernel on level L, timed

@@ -111,6 +111,7 @@ def _declare(L):
     L.pmg_net_stats.argtypes = [_vp, _ip, _ip, _dp, _dp]
     L.pmg_set_solution_lincomb.argtypes = [_vp, _vp, _vp, d, d]
     L.pmg_solve_fgmres.argtypes = [_vp, i, d, i, _dp, _ip, _dp]
+    L.pmg_set_fgmres_prec_cycles.argtypes = [_vp, i]
     L.pmg_history_warm_start.argtypes = [_vp, i, C.POINTER(_vp), C.POINTER(_vp), _dp, _dp]
     # learned band correction next to excised regions
     u8p = C.POINTER(C.c_ubyte)
@@ -412,6 +413,9 @@ class Solver:
         _check(self._L.pmg_solve_fgmres(self.h, m, rtol, max_iter, P(hist), C.byref(it), C.byref(r0)),
                "pmg_solve_fgmres")
         return r0.value, hist[: it.value]
+
+    def set_fgmres_prec_cycles(self, cycles):
+        _check(self._L.pmg_set_fgmres_prec_cycles(self.h, cycles), "pmg_set_fgmres_prec_cycles")
 
     def history_warm_start(self, F, U, misfit=True):
         """u = sum c_k U[k] minimising ||rho - sum c_k F[k]|| (device

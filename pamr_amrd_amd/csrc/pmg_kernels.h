@@ -62,11 +62,20 @@ __device__ __forceinline__ bool sweep_updatable(const SweepP<T> &P, int i, int j
   return i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1 && k >= P.bz0 && k <= P.bz1;
 }
 
+// The variable-coefficient sweep keeps the coefficient a in an LDS ring of
+// the same shape as u's when both rings fit (its seven reads per point then
+// come from LDS, like u's); otherwise a is read from global memory.
+template <typename T, int OP, int TX, int TY>
+constexpr bool sweep_a_ring() {
+  return OP == 1 && 2 * 4 * (TY + 4) * (TX + 4) * sizeof(T) <= 128 * 1024;
+}
+
 // One colour update at LDS position (lx, ly) of plane k.  rho/rhs are the
-// values at that point (prefetched by the caller).
-template <typename T, int OP, bool P2, bool RHS0, int TX, int TY>
+// values at that point (prefetched by the caller); sA is a's ring (AR).
+template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, bool AR = false>
 __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int lx,
-                                         int ly, int k, size_t p, T rho, T rhs) {
+                                         int ly, int k, size_t p, T rho, T rhs,
+                                         T (*sA)[TY + 4][TX + 4] = nullptr) {
   T(*S0)[TX + 4] = sU[k & 3];
   T(*Sm)[TX + 4] = sU[(k - 1) & 3];
   T(*Sp)[TX + 4] = sU[(k + 1) & 3];
@@ -76,6 +85,13 @@ __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX 
     L = lop_poisson<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
                            Sp[ly][lx], Sm[ly][lx], rho, P.sp);
     jac = P.sp.jac;
+  } else if constexpr (AR) {
+    T(*A0)[TX + 4] = sA[k & 3];
+    T(*Am)[TX + 4] = sA[(k - 1) & 3];
+    T(*Ap)[TX + 4] = sA[(k + 1) & 3];
+    L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
+                           Sp[ly][lx], Sm[ly][lx], A0[ly][lx], A0[ly][lx + 1], A0[ly][lx - 1],
+                           A0[ly + 1][lx], A0[ly - 1][lx], Ap[ly][lx], Am[ly][lx], P.c[p], rho, P.sp, &jac);
   } else {
     const T *__restrict__ a = P.a;
     const size_t sy = (size_t)P.nx, sz = (size_t)P.nx * P.ny;
@@ -94,10 +110,10 @@ __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX 
 
 // Update every slot of one phase on plane k with values fetched directly
 // (prologue only).
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, bool HALO>
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, bool HALO, bool AR>
 __device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4],
-                                                   int x0, int y0, int k, int pass, bool count,
-                                                   double &s, double &cnt) {
+                                                   T (*sA)[TY + 4][TX + 4], int x0, int y0, int k, int pass,
+                                                   bool count, double &s, double &cnt) {
   constexpr int W = HALO ? TX + 2 : TX, H = HALO ? TY + 2 : TY;
   for (int q = threadIdx.x; q < ((W + 1) / 2) * H; q += NT) {
     int lx, ly;
@@ -107,7 +123,7 @@ __device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[T
     const size_t p = ix3(i, j, k, P.nx, P.ny);
     if constexpr (EX) { if (P.chr[p]) continue; }
     const T rhs = RHS0 ? T(0) : P.rhs[p];
-    const T res = sweep_point<T, OP, P2, RHS0, TX, TY>(P, sU, lx, ly, k, p, P.rho[p], rhs);
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR>(P, sU, lx, ly, k, p, P.rho[p], rhs, sA);
     if (count && lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2) {
       s += (double)res * (double)res;
       cnt += 1.0;
@@ -173,15 +189,15 @@ __device__ __forceinline__ void phase_prefetch_t(const SweepP<T> &P, int k, cons
   }
 }
 
-template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, int N>
-__device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int k,
-                                          const PhaseRegs<T, N> &R, bool count, double &s,
-                                          double &cnt) {
+template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, int N, bool AR>
+__device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4],
+                                          T (*sA)[TY + 4][TX + 4], int k, const PhaseRegs<T, N> &R, bool count,
+                                          double &s, double &cnt) {
 #pragma unroll
   for (int m = 0; m < N; m++) {
     if (R.p[m] < 0) continue;
-    const T res = sweep_point<T, OP, P2, RHS0, TX, TY>(P, sU, R.lx[m], R.ly[m], k, (size_t)R.p[m],
-                                                       R.rho[m], RHS0 ? T(0) : R.rhs[m]);
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR>(P, sU, R.lx[m], R.ly[m], k, (size_t)R.p[m],
+                                                           R.rho[m], RHS0 ? T(0) : R.rhs[m], sA);
     if (count && R.lx[m] >= 2 && R.lx[m] < TX + 2 && R.ly[m] >= 2 && R.ly[m] < TY + 2) {
       s += (double)res * (double)res;
       cnt += 1.0;
@@ -207,9 +223,13 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   constexpr int NLD = (LX * LY + NT - 1) / NT;
   constexpr int N0 = (((TX + 3) / 2) * (TY + 2) + NT - 1) / NT;
   constexpr int N1 = (((TX + 1) / 2) * TY + NT - 1) / NT;
+  constexpr bool AR = sweep_a_ring<T, OP, TX, TY>();
   __shared__ T sU[4][LY][LX];
+  __shared__ T sAbuf[AR ? 4 * LY * LX : 1];
+  T(*sA)[LY][LX] = AR ? reinterpret_cast<T(*)[LY][LX]>(sAbuf) : nullptr;
   const int nx = P.nx, ny = P.ny, nz = P.nz;
   const T *__restrict__ uin = P.uin;
+  const T *__restrict__ ain = P.a;
   T *__restrict__ uout = P.uout;
   double s = 0.0, cnt = 0.0;
 
@@ -219,24 +239,36 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
     for (int q = threadIdx.x; q < LX * LY; q += NT) {
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      if (i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = uin[ix3(i, j, k, nx, ny)];
+      if (i >= 0 && i < nx && j >= 0 && j < ny) {
+        const size_t p = ix3(i, j, k, nx, ny);
+        S[ly][lx] = uin[p];
+        if constexpr (AR) sA[k & 3][ly][lx] = ain[p];
+      }
     }
   };
-  // u planes are prefetched into rotating register sets
-  T ldA[NLD], ldB[NLD], ldC[NLD];
+  // u (and a) planes are prefetched into rotating register sets
+  struct Ld {
+    T u[NLD];
+    T a[AR ? NLD : 1];
+  };
+  Ld ldA, ldB, ldC;
   // the last plane the chunk reads: pass 0 also runs on plane z1 (pass 1 on
   // z1-1 needs its updated points) and reads z1+1
   const int kmax = min(z1 + 1, nz - 1);
-  auto load_issue = [&](T(&ld)[NLD], int k) {
+  auto load_issue = [&](Ld &ld, int k) {
 #pragma unroll
     for (int m = 0; m < NLD; m++) {
       const int q = threadIdx.x + m * NT;
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      if (k <= kmax && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) ld[m] = uin[ix3(i, j, k, nx, ny)];
+      if (k <= kmax && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) {
+        const size_t p = ix3(i, j, k, nx, ny);
+        ld.u[m] = uin[p];
+        if constexpr (AR) ld.a[m] = ain[p];
+      }
     }
   };
-  auto load_commit = [&](const T(&ld)[NLD], int k) {
+  auto load_commit = [&](const Ld &ld, int k) {
     if (k > kmax) return;
     T(*S)[LX] = sU[k & 3];
 #pragma unroll
@@ -244,7 +276,10 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       const int q = threadIdx.x + m * NT;
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) S[ly][lx] = ld[m];
+      if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) {
+        S[ly][lx] = ld.u[m];
+        if constexpr (AR) sA[k & 3][ly][lx] = ld.a[m];
+      }
     }
   };
   auto store = [&](int k) {
@@ -263,9 +298,9 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   load_direct(z0 + 1);
   __syncthreads();
   if (z0 - 1 >= P.bz0 && z0 - 1 <= P.bz1)
-    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, false>(P, sU, x0, y0, z0 - 1, 0, false, s, cnt);
+    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, false, AR>(P, sU, sA, x0, y0, z0 - 1, 0, false, s, cnt);
   if (z0 >= P.bz0 && z0 <= P.bz1)
-    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, true>(P, sU, x0, y0, z0, 0, true, s, cnt);
+    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, true, AR>(P, sU, sA, x0, y0, z0, 0, true, s, cnt);
   __syncthreads();
   load_direct(z0 + 2);
   load_issue(ldA, z0 + 3);
@@ -281,13 +316,13 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   __syncthreads();
   // step k: issue plane k+3+PF and the next step's rho/rhs; pass 0 on k+1;
   // pass 1 on k; store k; commit plane k+3 (issued PF steps earlier)
-  auto step = [&](int k, const T(&ldc)[NLD], T(&ldi)[NLD]) {
+  auto step = [&](int k, const Ld &ldc, Ld &ldi) {
     load_issue(ldi, k + 3 + PF);
     phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0);
     phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1);
-    phase_run<T, OP, P2, RHS0, TX, TY, N0>(P, sU, k + 1, cur0, k + 1 < z1, s, cnt);
+    phase_run<T, OP, P2, RHS0, TX, TY, N0, AR>(P, sU, sA, k + 1, cur0, k + 1 < z1, s, cnt);
     __syncthreads();
-    phase_run<T, OP, P2, RHS0, TX, TY, N1>(P, sU, k, cur1, true, s, cnt);
+    phase_run<T, OP, P2, RHS0, TX, TY, N1, AR>(P, sU, sA, k, cur1, true, s, cnt);
     __syncthreads();
     store(k);
     load_commit(ldc, k + 3);

@@ -163,6 +163,7 @@ struct pmg_solver {
   double *kry_red = nullptr;
   double *kry_host = nullptr;
   hipGraphExec_t gprec = nullptr;  // captured FP32 V-cycle (the preconditioner)
+  int prec_cycles = 1;             // FP32 V-cycles per preconditioner application
   // learned correction in the band next to the excised region (pmg_band.hip)
   struct Band {
     int depth = 0, core = 0, side = 0, margin = 3, ncore = 0;
@@ -2168,12 +2169,18 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
       // Z_j = M V_j: the FP32 V-cycle from zero with V_j as its source
       hipLaunchKernelGGL(k_kry_prep, dim3(nb), dim3(256), 0, S->stream, F.f.rho, F.f.u, V[j], n);
       PMG_HOST_CHECK(hipGetLastError());
-      if (band_on && B.inner) {
-        if ((rc = capture_or_run(S, &B.gpreci, enq_prec_inner))) return rc;
-      } else {
-        if ((rc = capture_or_run(S, &S->gprec, enq_prec))) return rc;
-        if (band_on && (rc = capture_or_run(S, &B.gpc, enq_band_pc))) return rc;
+      // prec_cycles FP32 V-cycles on the source V_j, the first from zero;
+      // the band correction (when on) after a single cycle, or between
+      // cycles, where the next cycle smooths what it adds
+      for (int cyc = 0; cyc < S->prec_cycles; cyc++) {
+        if (cyc > 0 && band_on && !B.inner && (rc = capture_or_run(S, &B.gpc, enq_band_pc))) return rc;
+        if (band_on && B.inner) {
+          if ((rc = capture_or_run(S, &B.gpreci, enq_prec_inner))) return rc;
+        } else if ((rc = capture_or_run(S, &S->gprec, enq_prec))) {
+          return rc;
+        }
       }
+      if (band_on && !B.inner && S->prec_cycles == 1 && (rc = capture_or_run(S, &B.gpc, enq_band_pc))) return rc;
       hipLaunchKernelGGL(k_kry_to64, dim3(nb), dim3(256), 0, S->stream, Z[j], F.f.u, n);
       PMG_HOST_CHECK(hipGetLastError());
       // V_{j+1} = A Z_j, orthogonalised against V_0..V_j by classical
@@ -2249,6 +2256,12 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
     }
   }
   if (iters) *iters = it;
+  return 0;
+}
+
+int pmg_set_fgmres_prec_cycles(pmg_solver *S, int cycles) {
+  if (!S || cycles < 1 || cycles > 4) return -1;
+  S->prec_cycles = cycles;
   return 0;
 }
 
