@@ -408,7 +408,8 @@ def excised_leg(args, pm, torch):
                 "note": "3x3x3 convs 1->16->16->1 over the crops of one correction (15552 flop per crop point), "
                         "BF16 MFMA with FP32 accumulation; pmg_net_infer timed with HIP events"}
 
-    times = {"fgmres": [], "dc_band": [], "dc_band_in": [], "fgmres_band_in": [], "dc": [], "fgmres_band": []}
+    times = {"fgmres": [], "dc_band": [], "dc_band_in": [], "fgmres_band_in": [], "dc": [], "fgmres_band": [],
+             "fgmres2": [], "fgmres2_band": []}
     iters = {k: [] for k in times}
     plain = None
     for r in range(args.c5_rhs):
@@ -418,12 +419,15 @@ def excised_leg(args, pm, torch):
         del rho
         modes = ["fgmres", "dc_band"]
         if r < args.c5_mixed_rhs:
-            modes += ["dc", "fgmres_band", "dc_band_in", "fgmres_band_in"]
+            modes += ["dc", "fgmres_band", "dc_band_in", "fgmres_band_in", "fgmres2", "fgmres2_band"]
         if r == 0:
             modes += ["fp64"]
         for mode in modes:
             S.band_set_inner(mode.endswith("_in"))
-            S.band_enable(True, 2 if mode.startswith("fgmres_band") else False)
+            S.band_enable(True, 2 if mode.startswith(("fgmres_band", "fgmres2_band")) else False)
+            # fgmres2*: two FP32 V-cycles per preconditioner application, the band
+            # correction (when on) between them
+            S.set_fgmres_prec_cycles(2 if mode.startswith("fgmres2") else 1)
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             L.pmg_set_solution_lincomb(S.h, pm.tptr(u0), None, 1.0, 0.0)
@@ -450,6 +454,7 @@ def excised_leg(args, pm, torch):
         log(f"rhs {r}: " + ", ".join(f"{m} {times[m][-1]:.1f} ms/{iters[m][-1]}" for m in modes if m != "fp64"))
     S.band_enable(True, False)
     S.band_set_inner(False)
+    S.set_fgmres_prec_cycles(1)
     mean = {k: (sum(v) / len(v) if v else None) for k, v in times.items()}
     best = min(("fgmres", "dc_band"), key=lambda k: mean[k])
     # the same right-hand sides concurrently by the faster method: one solver

@@ -314,15 +314,17 @@ class Solver:
         if not self.h:
             raise RuntimeError("pmg_create failed")
         self.nlev = L.pmg_nlev(self.h)
+        self.op, self.do_ex = op, do_ex
         self.apply_tuning()
 
     def apply_tuning(self):
         """Per-level sweep tile/z-chunk of the FP64 and the FP32 sweeps from
-        the autotuner table (tuning.json)."""
+        the autotuner table (tuning.json): the entry tuned for this operator
+        and excision when there is one, else the plain Poisson entry."""
         from .autotune import tuned_level_config
         for lev in range(1, self.nlev + 1):
             for pi, prec in enumerate(("f64", "f32")):
-                cfg = tuned_level_config(self.shape(lev)[2], prec)
+                cfg = tuned_level_config(self.shape(lev)[2], prec, self.op, self.do_ex)
                 if cfg is not None:
                     _check(self._L.pmg_set_level_sweep_config_prec(self.h, lev, pi, *cfg),
                            "pmg_set_level_sweep_config_prec")

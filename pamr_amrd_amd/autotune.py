@@ -16,7 +16,8 @@ TABLE = os.path.join(HERE, "tuning.json")
 DEFAULT = (1, 32)
 VARIANTS = {0: (64, 8, 256), 1: (64, 16, 512), 2: (32, 16, 512), 3: (128, 8, 512), 4: (64, 32, 512),
             5: (128, 16, 1024), 6: (128, 16, 1024, "2 planes in flight"), 7: (64, 16, 512, "2 planes in flight"),
-            8: (128, 8, 512, "2 planes in flight")}
+            8: (128, 8, 512, "2 planes in flight"), 9: (128, 16, 1024, "2 planes in flight, 5-plane ring"),
+            10: (128, 16, 1024, "5-plane ring"), 11: (128, 8, 512, "5-plane ring")}
 
 
 def load_table():
@@ -26,9 +27,18 @@ def load_table():
         return json.load(f)
 
 
-def tuned_level_config(n, prec="f64"):
-    """(variant, zchunk) tuned for a level of n^3 points, None when untuned."""
-    t = load_table().get(str(n), {}).get(prec)
+def table_key(n, op=0, do_ex=0):
+    """Table key: the grid size, with the operator and excision when they are
+    not the plain Poisson operator (the variable-coefficient sweep moves more
+    bytes per point and may prefer another tile)."""
+    return str(n) if (op == 0 and not do_ex) else f"{n}/op{op}{'ex' if do_ex else ''}"
+
+
+def tuned_level_config(n, prec="f64", op=0, do_ex=0):
+    """(variant, zchunk) tuned for a level of n^3 points (and this operator
+    and excision when tuned for them), None when untuned."""
+    tab = load_table()
+    t = tab.get(table_key(n, op, do_ex), {}).get(prec) or tab.get(str(n), {}).get(prec)
     if not t:
         return None
     return int(t["variant"]), int(t["zchunk"])
@@ -38,14 +48,20 @@ def tuned_sweep_config(n, prec="f64"):
     return tuned_level_config(n, prec) or DEFAULT
 
 
-def autotune(n, zchunks=(2, 4, 8, 16, 32, 64), nrep=20, write=True, verbose=True):
+def autotune(n, zchunks=(2, 4, 8, 16, 32, 64), nrep=20, write=True, verbose=True, solver=None):
+    """Tune the finest level of an n^3 Poisson solver, or of `solver` (set up
+    by the caller: operator, coefficients, excision; stored under its own
+    table key)."""
     import numpy as np
 
     import pamr_amrd_amd as pm
-    S = pm.Solver(n)
-    rho = -pm.gaussian_source(n, seed=1234) if n <= 129 else np.zeros((n, n, n))
-    S.upload(S.nlev, pm.PMG_F_RHO, rho)
-    S.init()
+    S = solver
+    if S is None:
+        S = pm.Solver(n)
+        rho = -pm.gaussian_source(n, seed=1234) if n <= 129 else np.zeros((n, n, n))
+        S.upload(S.nlev, pm.PMG_F_RHO, rho)
+        S.init()
+    key = table_key(n, getattr(S, "op", 0), getattr(S, "do_ex", 0))
     results = {}
     for prec, name in ((0, "f64"), (1, "f32")):
         best = None
@@ -68,10 +84,13 @@ def autotune(n, zchunks=(2, 4, 8, 16, 32, 64), nrep=20, write=True, verbose=True
                     best = (v, z, ms)
         results[name] = {"variant": best[0], "zchunk": best[1], "ms": best[2],
                          "all": [list(r) for r in rows]}
-    S.close()
+    if solver is None:
+        S.close()
+    else:
+        S.apply_tuning()
     if write:
         tab = load_table()
-        tab[str(n)] = results
+        tab[key] = results
         with open(TABLE, "w") as f:
             json.dump(tab, f, indent=1)
     return results

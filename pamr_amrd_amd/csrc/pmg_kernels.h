@@ -65,20 +65,26 @@ __device__ __forceinline__ bool sweep_updatable(const SweepP<T> &P, int i, int j
 // The variable-coefficient sweep keeps the coefficient a in an LDS ring of
 // the same shape as u's when both rings fit (its seven reads per point then
 // come from LDS, like u's); otherwise a is read from global memory.
-template <typename T, int OP, int TX, int TY>
+template <typename T, int OP, int TX, int TY, int RING = 4>
 constexpr bool sweep_a_ring() {
-  return OP == 1 && 2 * 4 * (TY + 4) * (TX + 4) * sizeof(T) <= 128 * 1024;
+  return OP == 1 && 2 * RING * (TY + 4) * (TX + 4) * sizeof(T) <= 128 * 1024;
+}
+
+// Slot of plane k (k >= -2) in an LDS ring of RING planes.
+template <int RING>
+__device__ __forceinline__ int ring_ix(int k) {
+  if constexpr (RING == 4) return k & 3; else return (k + 2 * RING) % RING;
 }
 
 // One colour update at LDS position (lx, ly) of plane k.  rho/rhs are the
 // values at that point (prefetched by the caller); sA is a's ring (AR).
-template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, bool AR = false>
+template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, bool AR = false, int RING = 4>
 __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int lx,
                                          int ly, int k, size_t p, T rho, T rhs,
                                          T (*sA)[TY + 4][TX + 4] = nullptr) {
-  T(*S0)[TX + 4] = sU[k & 3];
-  T(*Sm)[TX + 4] = sU[(k - 1) & 3];
-  T(*Sp)[TX + 4] = sU[(k + 1) & 3];
+  T(*S0)[TX + 4] = sU[ring_ix<RING>(k)];
+  T(*Sm)[TX + 4] = sU[ring_ix<RING>(k - 1)];
+  T(*Sp)[TX + 4] = sU[ring_ix<RING>(k + 1)];
   const T uc = S0[ly][lx];
   T jac, L;
   if constexpr (OP == 0) {
@@ -86,9 +92,9 @@ __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX 
                            Sp[ly][lx], Sm[ly][lx], rho, P.sp);
     jac = P.sp.jac;
   } else if constexpr (AR) {
-    T(*A0)[TX + 4] = sA[k & 3];
-    T(*Am)[TX + 4] = sA[(k - 1) & 3];
-    T(*Ap)[TX + 4] = sA[(k + 1) & 3];
+    T(*A0)[TX + 4] = sA[ring_ix<RING>(k)];
+    T(*Am)[TX + 4] = sA[ring_ix<RING>(k - 1)];
+    T(*Ap)[TX + 4] = sA[ring_ix<RING>(k + 1)];
     L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
                            Sp[ly][lx], Sm[ly][lx], A0[ly][lx], A0[ly][lx + 1], A0[ly][lx - 1],
                            A0[ly + 1][lx], A0[ly - 1][lx], Ap[ly][lx], Am[ly][lx], P.c[p], rho, P.sp, &jac);
@@ -110,7 +116,7 @@ __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX 
 
 // Update every slot of one phase on plane k with values fetched directly
 // (prologue only).
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, bool HALO, bool AR>
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, bool HALO, bool AR, int RING>
 __device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4],
                                                    T (*sA)[TY + 4][TX + 4], int x0, int y0, int k, int pass,
                                                    bool count, double &s, double &cnt) {
@@ -123,7 +129,7 @@ __device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[T
     const size_t p = ix3(i, j, k, P.nx, P.ny);
     if constexpr (EX) { if (P.chr[p]) continue; }
     const T rhs = RHS0 ? T(0) : P.rhs[p];
-    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR>(P, sU, lx, ly, k, p, P.rho[p], rhs, sA);
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR, RING>(P, sU, lx, ly, k, p, P.rho[p], rhs, sA);
     if (count && lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2) {
       s += (double)res * (double)res;
       cnt += 1.0;
@@ -189,15 +195,15 @@ __device__ __forceinline__ void phase_prefetch_t(const SweepP<T> &P, int k, cons
   }
 }
 
-template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, int N, bool AR>
+template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, int N, bool AR, int RING>
 __device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4],
                                           T (*sA)[TY + 4][TX + 4], int k, const PhaseRegs<T, N> &R, bool count,
                                           double &s, double &cnt) {
 #pragma unroll
   for (int m = 0; m < N; m++) {
     if (R.p[m] < 0) continue;
-    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR>(P, sU, R.lx[m], R.ly[m], k, (size_t)R.p[m],
-                                                           R.rho[m], RHS0 ? T(0) : R.rhs[m], sA);
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR, RING>(P, sU, R.lx[m], R.ly[m], k, (size_t)R.p[m],
+                                                                 R.rho[m], RHS0 ? T(0) : R.rhs[m], sA);
     if (count && R.lx[m] >= 2 && R.lx[m] < TX + 2 && R.ly[m] >= 2 && R.ly[m] < TY + 2) {
       s += (double)res * (double)res;
       cnt += 1.0;
@@ -216,16 +222,22 @@ __device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX
 // PF: u planes in flight ahead of the march (1: one plane issued per step and
 // committed the next; 2: two, for latency-bound tiles such as FP32).  Planes
 // past z1+1, which no pass of the chunk reads, are not fetched.
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, int PF = 1>
+// RING: LDS planes.  With 4, the slot plane k+3 is committed into is that of
+// plane k-1, which pass 1 on plane k reads, so a step needs three barriers
+// (after pass 0, after pass 1, after the commit).  With 5 the commit touches
+// no slot the step reads: it follows pass 1 directly and one barrier covers
+// both, two barriers per step.
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, int PF = 1, int RING = 4>
 __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, int z0, int z1, int bid) {
   static_assert(PF == 1 || PF == 2, "prefetch depth");
+  static_assert(RING == 4 || RING == 5, "LDS ring");
   constexpr int LX = TX + 4, LY = TY + 4;
   constexpr int NLD = (LX * LY + NT - 1) / NT;
   constexpr int N0 = (((TX + 3) / 2) * (TY + 2) + NT - 1) / NT;
   constexpr int N1 = (((TX + 1) / 2) * TY + NT - 1) / NT;
-  constexpr bool AR = sweep_a_ring<T, OP, TX, TY>();
-  __shared__ T sU[4][LY][LX];
-  __shared__ T sAbuf[AR ? 4 * LY * LX : 1];
+  constexpr bool AR = sweep_a_ring<T, OP, TX, TY, RING>();
+  __shared__ T sU[RING][LY][LX];
+  __shared__ T sAbuf[AR ? RING * LY * LX : 1];
   T(*sA)[LY][LX] = AR ? reinterpret_cast<T(*)[LY][LX]>(sAbuf) : nullptr;
   const int nx = P.nx, ny = P.ny, nz = P.nz;
   const T *__restrict__ uin = P.uin;
@@ -235,14 +247,14 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
 
   auto load_direct = [&](int k) {
     if (k < 0 || k >= nz) return;
-    T(*S)[LX] = sU[k & 3];
+    T(*S)[LX] = sU[ring_ix<RING>(k)];
     for (int q = threadIdx.x; q < LX * LY; q += NT) {
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
       if (i >= 0 && i < nx && j >= 0 && j < ny) {
         const size_t p = ix3(i, j, k, nx, ny);
         S[ly][lx] = uin[p];
-        if constexpr (AR) sA[k & 3][ly][lx] = ain[p];
+        if constexpr (AR) sA[ring_ix<RING>(k)][ly][lx] = ain[p];
       }
     }
   };
@@ -270,7 +282,7 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   };
   auto load_commit = [&](const Ld &ld, int k) {
     if (k > kmax) return;
-    T(*S)[LX] = sU[k & 3];
+    T(*S)[LX] = sU[ring_ix<RING>(k)];
 #pragma unroll
     for (int m = 0; m < NLD; m++) {
       const int q = threadIdx.x + m * NT;
@@ -278,12 +290,12 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
       if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) {
         S[ly][lx] = ld.u[m];
-        if constexpr (AR) sA[k & 3][ly][lx] = ld.a[m];
+        if constexpr (AR) sA[ring_ix<RING>(k)][ly][lx] = ld.a[m];
       }
     }
   };
   auto store = [&](int k) {
-    T(*S)[LX] = sU[k & 3];
+    T(*S)[LX] = sU[ring_ix<RING>(k)];
     for (int q = threadIdx.x; q < TX * TY; q += NT) {
       const int tx = q % TX, ty = q / TX;
       const int i = x0 + tx, j = y0 + ty;
@@ -298,9 +310,10 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   load_direct(z0 + 1);
   __syncthreads();
   if (z0 - 1 >= P.bz0 && z0 - 1 <= P.bz1)
-    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, false, AR>(P, sU, sA, x0, y0, z0 - 1, 0, false, s, cnt);
+    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, false, AR, RING>(P, sU, sA, x0, y0, z0 - 1, 0, false, s,
+                                                                         cnt);
   if (z0 >= P.bz0 && z0 <= P.bz1)
-    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, true, AR>(P, sU, sA, x0, y0, z0, 0, true, s, cnt);
+    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, true, AR, RING>(P, sU, sA, x0, y0, z0, 0, true, s, cnt);
   __syncthreads();
   load_direct(z0 + 2);
   load_issue(ldA, z0 + 3);
@@ -320,15 +333,21 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
     load_issue(ldi, k + 3 + PF);
     phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0);
     phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1);
-    phase_run<T, OP, P2, RHS0, TX, TY, N0, AR>(P, sU, sA, k + 1, cur0, k + 1 < z1, s, cnt);
+    phase_run<T, OP, P2, RHS0, TX, TY, N0, AR, RING>(P, sU, sA, k + 1, cur0, k + 1 < z1, s, cnt);
     __syncthreads();
-    phase_run<T, OP, P2, RHS0, TX, TY, N1, AR>(P, sU, sA, k, cur1, true, s, cnt);
-    __syncthreads();
-    store(k);
-    load_commit(ldc, k + 3);
+    phase_run<T, OP, P2, RHS0, TX, TY, N1, AR, RING>(P, sU, sA, k, cur1, true, s, cnt);
+    if constexpr (RING == 5) {
+      load_commit(ldc, k + 3);
+      __syncthreads();
+      store(k);
+    } else {
+      __syncthreads();
+      store(k);
+      load_commit(ldc, k + 3);
+      __syncthreads();
+    }
     cur0 = nxt0;
     cur1 = nxt1;
-    __syncthreads();
   };
   if constexpr (PF == 1) {
     for (int k = z0; k < z1; k += 2) {
@@ -347,14 +366,14 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   norm_partial<NT>(s, cnt, P.ws, P.slot, bid, P.nb);
 }
 
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, int PF = 1>
+template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, int PF = 1, int RING = 4>
 __global__ __launch_bounds__(NT) void k_sweep(SweepP<T> P) {
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int nty = (P.ny - 2 + TY - 1) / TY, ntx = (P.nx - 2 + TX - 1) / TX;
   const int ty = tile % nty, tx = (tile / nty) % ntx, tz = tile / (nty * ntx);
   const int z0 = 1 + tz * P.zchunk;
-  sweep_tile<T, OP, EX, P2, RHS0, TX, TY, NT, PF>(P, 1 + tx * TX, 1 + ty * TY, z0, min(z0 + P.zchunk, P.nz - 1),
-                                                  blockIdx.x);
+  sweep_tile<T, OP, EX, P2, RHS0, TX, TY, NT, PF, RING>(P, 1 + tx * TX, 1 + ty * TY, z0,
+                                                        min(z0 + P.zchunk, P.nz - 1), blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------

@@ -283,18 +283,19 @@ T *ucur(Level &Lv) { LevelArrays<T> &A = arrays<T>(Lv); return curbuf<T>(Lv) ? A
 template <typename T>
 T *uoth(Level &Lv) { LevelArrays<T> &A = arrays<T>(Lv); return curbuf<T>(Lv) ? A.u : A.ua; }
 
-// sweep tile variants (TX, TY, NT, planes in flight)
-#define PMG_SWEEP_VARIANTS(X)                                                                        \
-  X(0, 64, 8, 256, 1) X(1, 64, 16, 512, 1) X(2, 32, 16, 512, 1) X(3, 128, 8, 512, 1) X(4, 64, 32, 512, 1) \
-  X(5, 128, 16, 1024, 1) X(6, 128, 16, 1024, 2) X(7, 64, 16, 512, 2) X(8, 128, 8, 512, 2)
-constexpr int kNumSweepVariants = 9;
+// sweep tile variants (TX, TY, NT, planes in flight, LDS ring planes)
+#define PMG_SWEEP_VARIANTS(X)                                                                                   \
+  X(0, 64, 8, 256, 1, 4) X(1, 64, 16, 512, 1, 4) X(2, 32, 16, 512, 1, 4) X(3, 128, 8, 512, 1, 4)             \
+  X(4, 64, 32, 512, 1, 4) X(5, 128, 16, 1024, 1, 4) X(6, 128, 16, 1024, 2, 4) X(7, 64, 16, 512, 2, 4)        \
+  X(8, 128, 8, 512, 2, 4) X(9, 128, 16, 1024, 2, 5) X(10, 128, 16, 1024, 1, 5) X(11, 128, 8, 512, 1, 5)
+constexpr int kNumSweepVariants = 12;
 
 template <typename T, int OP, bool EX, bool P2, bool RHS0>
 int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
   P.zchunk = std::max(1, std::min(P.zchunk, P.nz - 2));
   int tx = 64, ty = 8;
   switch (variant) {
-#define PMG_TXY(V, TXV, TYV, NTV, PFV) case V: tx = TXV; ty = TYV; break;
+#define PMG_TXY(V, TXV, TYV, NTV, PFV, RGV) case V: tx = TXV; ty = TYV; break;
     PMG_SWEEP_VARIANTS(PMG_TXY)
 #undef PMG_TXY
     default: return -1;
@@ -303,10 +304,10 @@ int launch_sweep_v(pmg_solver *S, int variant, SweepP<T> P) {
   P.nb = (int)grid.x;
   if (P.nb > S->ws.cap) return -3;
   switch (variant) {
-#define PMG_LSW(V, TXV, TYV, NTV, PFV)                                                         \
-  case V:                                                                                      \
-    hipLaunchKernelGGL((k_sweep<T, OP, EX, P2, RHS0, TXV, TYV, NTV, PFV>), grid, dim3(NTV), 0, \
-                       S->stream, P);                                                          \
+#define PMG_LSW(V, TXV, TYV, NTV, PFV, RGV)                                                         \
+  case V:                                                                                           \
+    hipLaunchKernelGGL((k_sweep<T, OP, EX, P2, RHS0, TXV, TYV, NTV, PFV, RGV>), grid, dim3(NTV), 0, \
+                       S->stream, P);                                                               \
     break;
     PMG_SWEEP_VARIANTS(PMG_LSW)
 #undef PMG_LSW

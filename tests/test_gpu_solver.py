@@ -36,7 +36,7 @@ def _pair(pm, n, rho, op=0, a=None, c=None, pre=2, pst=2, variant=1, zchunk=32, 
     return S, m
 
 
-@pytest.mark.parametrize("variant,zchunk", [(v, 32) for v in range(9)] + [(6, 5), (7, 4), (8, 7)])
+@pytest.mark.parametrize("variant,zchunk", [(v, 32) for v in range(12)] + [(6, 5), (7, 4), (8, 7), (9, 5), (11, 3)])
 def test_c1_ten_cycles_65(pm, variant, zchunk):
     """65^3 Poisson, seeded Gaussian source, V(2,2), FW, 10 cycles, every
     sweep tile variant (z-chunks that leave every remainder of the
@@ -70,14 +70,18 @@ def test_cycles_other_sizes(pm, n):
     S.close()
 
 
-def test_varcoef_cycles(pm):
-    print("case varcoef 33", flush=True)
-    n = 33
+@pytest.mark.parametrize("n,variant,zchunk", [(33, 1, 32), (65, 5, 32), (65, 6, 5), (65, 9, 7), (65, 10, 32),
+                                               (65, 11, 3), (65, 2, 8)])
+def test_varcoef_cycles(pm, n, variant, zchunk):
+    """Variable-coefficient cycles, the coefficient a in its LDS ring (4 or 5
+    planes) where both rings fit, read from global memory otherwise (the
+    FP64 128 x 16 tiles)."""
+    print(f"case varcoef {n} variant={variant} zchunk={zchunk}", flush=True)
     rng = np.random.default_rng(5)
     rho = -pm.gaussian_source(n, seed=5)
     a = np.exp(0.5 * rng.standard_normal((n, n, n)))
     c = rng.random((n, n, n))
-    S, m = _pair(pm, n, rho, op=1, a=a, c=c)
+    S, m = _pair(pm, n, rho, op=1, a=a, c=c, variant=variant, zchunk=zchunk)
     for cyc in range(4):
         r_gpu, _, _ = S.vcycle()
         r_ref, _, _ = m.vcycle()
