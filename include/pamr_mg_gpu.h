@@ -197,6 +197,12 @@ int pmg_set_bottom_fusion(pmg_solver *S, int on);
  * global memory (0); results are identical either way.  No counterpart in
  * the reference (a tuning switch of this implementation). */
 int pmg_set_bottom_lds(pmg_solver *S, int on);
+/* FP32 inner cycles (mixed precision, FGMRES preconditioner) on the plain
+ * Poisson operator without excision: solve a 17^3 top bottom level exactly
+ * by the discrete sine transform instead of running the V-cycle below it
+ * (on = 1, the default; no counterpart in the reference, which relaxes
+ * there; the FP64 cycles keep the reference's method) */
+int pmg_set_fp32_exact_bottom(pmg_solver *S, int on);
 /* compute the finest-level residual (of the FP64 cycle and of the FP32 inner
  * cycle) inside the full-weight injection of u and f_res instead of storing
  * it (default on; no excision, full weighting for both); u, the norms and

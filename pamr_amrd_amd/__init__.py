@@ -84,6 +84,7 @@ def _declare(L):
     L.pmg_set_graph.argtypes = [_vp, i]
     L.pmg_set_bottom_fusion.argtypes = [_vp, i]
     L.pmg_set_bottom_lds.argtypes = [_vp, i]
+    L.pmg_set_fp32_exact_bottom.argtypes = [_vp, i]
     L.pmg_set_restrict_fusion.argtypes = [_vp, i]
     L.pmg_set_resid_restrict_zchunk.argtypes = [_vp, i, i]
     L.pmg_set_sweep_blocking.argtypes = [_vp, i, i]
@@ -447,6 +448,9 @@ class Solver:
         """fuse the bottom levels into one workgroup (max_n: largest side fused)"""
         v = (max_n if max_n is not None else 1) if on else 0
         _check(self._L.pmg_set_bottom_fusion(self.h, v), "pmg_set_bottom_fusion")
+
+    def set_fp32_exact_bottom(self, on):
+        _check(self._L.pmg_set_fp32_exact_bottom(self.h, 1 if on else 0), "pmg_set_fp32_exact_bottom")
 
     def set_bottom_lds(self, on):
         """keep the fused bottom levels' arrays in LDS when they fit"""

@@ -138,6 +138,7 @@ struct pmg_solver {
   int use_bottom = 1;  // fuse the small bottom levels into one workgroup
   int bottom_max_n = 17;  // ... those with at most this many points per side
   int bottom_lds = 1;     // ... with their arrays in LDS when they fit
+  int fp32_exact_bottom = 1;  // FP32 inner cycles: exact sine-transform solve of a 17^3 bottom level
   int use_fused_restrict = 2;  // finest residual + injection in one pass (1: res ring; 2, 3: restriction streamed in z)
   int rr_czchunk[2] = {8, 8};   // its coarse planes per block, [0] FP64, [1] FP32
   int use_sweep2 = 0;          // pairs of sweeps temporally blocked on large levels
@@ -686,8 +687,126 @@ bool bottom_lds_fits(int op, bool p2, size_t lds, T *) {
   return p2 ? bottom_lds_fits_v<T, 1, true>(lds) : bottom_lds_fits_v<T, 1, false>(lds);
 }
 
+// ---------------------------------------------------------------------------
+// Exact solve of a 17^3 level for the FP32 inner cycles (plain Poisson, no
+// excision; no parity claim: the reference relaxes there).  The FAS
+// equation of the level, L[u] = lap_h u - rho = rhs, is solved for the 15^3
+// interior with u's boundary values as Dirichlet data, by the discrete sine
+// transform that diagonalises the 7-point Laplacian: with Q the orthonormal
+// DST-I basis of each axis (Q = S / sqrt(8), S[i][m] = sin((i+1)(m+1) pi / 16)),
+// lap_h = Q Lambda Q^T, lambda = -(4/h^2) (sin^2(i' pi/32) + sin^2(j' pi/32) +
+// sin^2(k' pi/32)), so u = (1/512) S3 (Lambda^-1 (S3 b)), b = rhs + rho - (the
+// boundary neighbours' u / h^2).  One workgroup, 15^3 values in LDS, six
+// 15-point transforms.
+// ---------------------------------------------------------------------------
+constexpr int kDstM = 15;  // interior points per axis (a level of 17)
+
+__global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, float *__restrict__ ua,
+                                                     const float *__restrict__ rho, const float *__restrict__ rhs,
+                                                     float h2inv, double *cnorm, int *citer) {
+  constexpr int M = kDstM, N = M + 2, V = M * M * M;
+  __shared__ float X[V], Y[V];
+  __shared__ float sn[M][M];
+  __shared__ float lam[M];
+  const int t = threadIdx.x;
+  for (int q = t; q < M * M; q += 256) {
+    const int i = q / M, m = q % M;
+    sn[i][m] = (float)std::sin((double)((i + 1) * (m + 1)) * M_PI / 16.0);
+  }
+  if (t < M) {
+    const double sv = std::sin((double)(t + 1) * M_PI / 32.0);
+    lam[t] = (float)(4.0 * sv * sv);
+  }
+  for (int q = t; q < V; q += 256) {
+    const int i = q % M, j = (q / M) % M, k = q / (M * M);
+    const int I = i + 1, J = j + 1, K = k + 1;
+    const size_t p = ix3(I, J, K, N, N);
+    float b = rhs[p] + rho[p];
+    if (I == 1) b -= u[p - 1] * h2inv;
+    if (I == M) b -= u[p + 1] * h2inv;
+    if (J == 1) b -= u[p - N] * h2inv;
+    if (J == M) b -= u[p + N] * h2inv;
+    if (K == 1) b -= u[p - N * N] * h2inv;
+    if (K == M) b -= u[p + N * N] * h2inv;
+    X[q] = b;
+  }
+  __syncthreads();
+  // Y = S along x, X = S along y, Y = S along z (and the same back)
+  auto tx = [&](const float *in, float *out) {
+    for (int q = t; q < V; q += 256) {
+      const int i = q % M, row = q - i;
+      float acc = 0.f;
+      for (int m = 0; m < M; m++) acc += sn[i][m] * in[row + m];
+      out[q] = acc;
+    }
+  };
+  auto ty = [&](const float *in, float *out) {
+    for (int q = t; q < V; q += 256) {
+      const int i = q % M, j = (q / M) % M, k = q / (M * M);
+      float acc = 0.f;
+      for (int m = 0; m < M; m++) acc += sn[j][m] * in[(k * M + m) * M + i];
+      out[q] = acc;
+    }
+  };
+  auto tz = [&](const float *in, float *out) {
+    for (int q = t; q < V; q += 256) {
+      const int ij = q % (M * M), k = q / (M * M);
+      float acc = 0.f;
+      for (int m = 0; m < M; m++) acc += sn[k][m] * in[m * M * M + ij];
+      out[q] = acc;
+    }
+  };
+  tx(X, Y);
+  __syncthreads();
+  ty(Y, X);
+  __syncthreads();
+  tz(X, Y);
+  __syncthreads();
+  for (int q = t; q < V; q += 256) {
+    const int i = q % M, j = (q / M) % M, k = q / (M * M);
+    // lap_h eigenvalue -(lam_i + lam_j + lam_k) / h^2; 1/512 normalises S3 S3
+    Y[q] = Y[q] / (-(lam[i] + lam[j] + lam[k]) * h2inv * 512.0f);
+  }
+  __syncthreads();
+  tx(Y, X);
+  __syncthreads();
+  ty(X, Y);
+  __syncthreads();
+  tz(Y, X);
+  __syncthreads();
+  for (int q = t; q < V; q += 256) {
+    const int i = q % M, j = (q / M) % M, k = q / (M * M);
+    const size_t p = ix3(i + 1, j + 1, k + 1, N, N);
+    u[p] = X[q];
+    ua[p] = X[q];
+  }
+  if (t == 0) {
+    cnorm[0] = 0.0;
+    cnorm[1] = 0.0;
+    *citer = 0;
+  }
+}
+
 template <typename T>
 int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
+  if constexpr (std::is_same<T, float>::value) {
+    // FP32 inner cycles: an exact solve of a 17^3 top bottom level instead
+    // of the single-workgroup V below it (plain Poisson, no excision)
+    Level &Lv = S->lev[Lb - 1];
+    const bool cube17 = Lv.n[0] == kDstM + 2 && Lv.n[1] == kDstM + 2 && Lv.n[2] == kDstM + 2;
+    const bool full = Lv.rbox.ok && Lv.rbox.lo[0] == 1 && Lv.rbox.lo[1] == 1 && Lv.rbox.lo[2] == 1 &&
+                      Lv.rbox.hi[0] == kDstM && Lv.rbox.hi[1] == kDstM && Lv.rbox.hi[2] == kDstM;
+    const bool iso = Lv.hdx[0] == Lv.hdx[1] && Lv.hdx[1] == Lv.hdx[2];
+    if (S->fp32_exact_bottom && S->op == 0 && !S->do_ex && cube17 && full && iso) {
+      int rc;
+      if ((rc = lop_rhs<float>(S, Lb))) return rc;
+      LevelArrays<float> &A = arrays<float>(Lv);
+      hipLaunchKernelGGL(k_dst_solve17, dim3(1), dim3(256), 0, S->stream, A.u, A.ua, A.rho, A.rhs,
+                         (float)(1.0 / (Lv.hdx[0] * Lv.hdx[0])), cnorm, citer);
+      PMG_HOST_CHECK(hipGetLastError());
+      return 0;
+    }
+  }
   BottomP<T> P;
   P.nb = Lb;
   for (int L = 1; L <= Lb; L++) {
@@ -1219,6 +1338,13 @@ int pmg_set_bottom_fusion(pmg_solver *S, int on) {
   if (!S || on < 0 || on > kBottomMaxN) return -1;
   S->use_bottom = on ? 1 : 0;
   S->bottom_max_n = on > 1 ? on : kBottomMaxN;
+  invalidate_graphs(S);
+  return 0;
+}
+
+int pmg_set_fp32_exact_bottom(pmg_solver *S, int on) {
+  if (!S) return -1;
+  S->fp32_exact_bottom = on ? 1 : 0;
   invalidate_graphs(S);
   return 0;
 }

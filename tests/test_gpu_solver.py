@@ -346,3 +346,31 @@ def test_plain_fp64_rel_solve(pm):
     assert hist[-1] <= 1e-10 * r0
     assert len(hist) < 15
     S.close()
+
+
+@pytest.mark.parametrize("n", [33, 129])
+def test_fp32_exact_bottom(pm, n):
+    """FP32 inner cycles with the 17^3 level solved exactly by the sine
+    transform (the default) against the relaxed bottom V: both reach 1e-10,
+    the exact one in no more iterations, and the two solutions agree with a
+    40-cycle FP64 solve to 1e-9."""
+    print(f"case fp32 exact bottom n={n}", flush=True)
+    rho = -pm.gaussian_source(n, seed=77)
+    S, m = _pair(pm, n, rho)
+    S.init()
+    S.solve(40, 40)
+    u64 = S.download(S.nlev, pm.PMG_F_U)
+    out = {}
+    for on in (False, True):
+        S.set_fp32_exact_bottom(on)
+        S.zero_solution()
+        S.init()
+        r0, h = S.solve_rel(1, 1e-10, 60)
+        assert h[-1] <= 1e-10 * r0, (on, h[-3:])
+        u = S.download(S.nlev, pm.PMG_F_U)
+        err = np.abs(u - u64).max() / np.abs(u64).max()
+        out[on] = (len(h), err, (h[-1] / r0) ** (1.0 / len(h)))
+        assert err < 1e-9, (on, err)
+    print(f"relaxed bottom: {out[False]}, exact bottom: {out[True]}", flush=True)
+    assert out[True][0] <= out[False][0]
+    S.close()
