@@ -110,7 +110,7 @@ __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX 
   if constexpr (std::is_same<T, float>::value && OP == 0)
     S0[ly][lx] = uc - res * P.sp.ijac;  // FP32 inner cycles: no IEEE division
   else
-    S0[ly][lx] = uc - res / jac;  // FP64: the reference's V - res/Jac, bit for bit
+    S0[ly][lx] = uc - div_jac<T, OP>(res, jac, P.sp);  // FP64: the reference's V - res/Jac, bit for bit
   return res;
 }
 
@@ -502,7 +502,7 @@ __device__ __forceinline__ void sweep2_tile(const SweepP<T> &P, int x0, int y0, 
       T res;
       if constexpr (RHS0) res = L - T(0); else res = L - P.rhs[p];
       if constexpr (std::is_same<T, float>::value && OP == 0) S0[ly][lx] = uc - res * P.sp.ijac;
-      else S0[ly][lx] = uc - res / jac;
+      else S0[ly][lx] = uc - div_jac<T, OP>(res, jac, P.sp);
       if (count && lx >= M && lx < M + TX && ly >= M && ly < M + TY) {
         s += (double)res * (double)res;
         cnt += 1.0;
@@ -976,7 +976,7 @@ __device__ double wave_sweep(const WaveCoarse<T> &W) {
         T jac;
         const T r = lop_at<T, OP, P2>(W.u, W.rho, W.a, W.c, p, sy, sz, W.sp, &jac) - W.rhs[p];
         if constexpr (IJAC) W.u[p] = W.u[p] - r * W.sp.ijac;
-        else W.u[p] = W.u[p] - r / jac;
+        else W.u[p] = W.u[p] - div_jac<T, OP>(r, jac, W.sp);
         s += (double)r * (double)r;
         cnt += 1.0;
       }
@@ -1092,7 +1092,7 @@ __global__ __launch_bounds__(NT) void k_coarse_solve(CoarseP<T> P) {
         if constexpr (EX) { if (P.chr[p]) continue; }
         T jac;
         const T r = lop_at<T, OP, P2>(u, rho, a, c, p, sy, sz, P.sp, &jac) - rhs[p];
-        u[p] = u[p] - r / jac;
+        u[p] = u[p] - div_jac<T, OP>(r, jac, P.sp);
         s += (double)r * (double)r;
         cnt += 1.0;
       }
@@ -1642,7 +1642,7 @@ __global__ __launch_bounds__(kBotThreads) void k_bottom(BottomP<T> P) {
         T jac;
         const T r = lop_at<T, OP, P2>(L.u, L.rho, L.a, L.c, p, sy, sz, L.sp, &jac) - L.rhs[p];
         if constexpr (std::is_same<T, float>::value && OP == 0) L.u[p] = L.u[p] - r * L.sp.ijac;
-        else L.u[p] = L.u[p] - r / jac;
+        else L.u[p] = L.u[p] - div_jac<T, OP>(r, jac, L.sp);
         s += (double)r * (double)r;
         cnt += 1.0;
       }

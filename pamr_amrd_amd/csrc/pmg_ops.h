@@ -14,6 +14,31 @@ __device__ __forceinline__ T divd(T x, T d2, T inv) {
   if constexpr (P2) return x * inv; else return x / d2;
 }
 
+// a / b rounded to nearest from y = RN(1/b), without a division: q0 =
+// RN(a y) is within 2 ulps of a/b; one correction with the exact remainder
+// (an FMA), q1 = RN(q0 + (a - b q0) y), is faithful; a second one with a
+// faithful quotient and a correctly rounded reciprocal returns RN(a/b)
+// (P. Markstein, IA-64 and Elementary Functions, 2000, Thm. 8.10; J.-M.
+// Muller et al., Handbook of Floating-Point Arithmetic, 2018, Sec. 4.7),
+// absent underflow and overflow.  5 FP64 operations against the ~12 of the
+// IEEE division sequence.
+__device__ __forceinline__ double div_rn(double a, double b, double y) {
+  const double q0 = a * y;
+  const double r0 = __builtin_fma(-b, q0, a);
+  const double q1 = __builtin_fma(r0, y, q0);
+  const double r1 = __builtin_fma(-b, q1, a);
+  return __builtin_fma(r1, y, q1);
+}
+
+// res / jac of the relaxation update V - res/Jac: with the plain operator
+// in FP64, Jac is the level's constant and sp.ijac = RN(1/Jac), so div_rn
+// gives the IEEE quotient's bits; otherwise the division itself.
+template <typename T, int OP>
+__device__ __forceinline__ T div_jac(T r, T jac, const Spacing<T> &sp) {
+  if constexpr (OP == 0 && std::is_same<T, double>::value) return div_rn(r, jac, sp.ijac);
+  else return r / jac;
+}
+
 // Plain Poisson operator L[V] = lap_h V - rho at one point
 // (w/examples/nps/num_w.f:42-45; relax/residual terms at :136-140).
 // Neighbour naming: e/w = i+1/i-1, n/s = j+1/j-1, t/b = k+1/k-1.
