@@ -700,8 +700,9 @@ bool bottom_lds_fits(int op, bool p2, size_t lds, T *) {
 // 15-point transforms.
 // ---------------------------------------------------------------------------
 constexpr int kDstM = 15;  // interior points per axis (a level of 17)
+constexpr int kDstThreads = 1024;
 
-__global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, float *__restrict__ ua,
+__global__ __launch_bounds__(kDstThreads) void k_dst_solve17(float *__restrict__ u, float *__restrict__ ua,
                                                      const float *__restrict__ rho, const float *__restrict__ rhs,
                                                      float h2inv, double *cnorm, int *citer) {
   constexpr int M = kDstM, N = M + 2, V = M * M * M;
@@ -709,7 +710,7 @@ __global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, floa
   __shared__ float sn[M][M];
   __shared__ float lam[M];
   const int t = threadIdx.x;
-  for (int q = t; q < M * M; q += 256) {
+  for (int q = t; q < M * M; q += kDstThreads) {
     const int i = q / M, m = q % M;
     sn[i][m] = (float)std::sin((double)((i + 1) * (m + 1)) * M_PI / 16.0);
   }
@@ -717,7 +718,7 @@ __global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, floa
     const double sv = std::sin((double)(t + 1) * M_PI / 32.0);
     lam[t] = (float)(4.0 * sv * sv);
   }
-  for (int q = t; q < V; q += 256) {
+  for (int q = t; q < V; q += kDstThreads) {
     const int i = q % M, j = (q / M) % M, k = q / (M * M);
     const int I = i + 1, J = j + 1, K = k + 1;
     const size_t p = ix3(I, J, K, N, N);
@@ -733,7 +734,7 @@ __global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, floa
   __syncthreads();
   // Y = S along x, X = S along y, Y = S along z (and the same back)
   auto tx = [&](const float *in, float *out) {
-    for (int q = t; q < V; q += 256) {
+    for (int q = t; q < V; q += kDstThreads) {
       const int i = q % M, row = q - i;
       float acc = 0.f;
       for (int m = 0; m < M; m++) acc += sn[i][m] * in[row + m];
@@ -741,7 +742,7 @@ __global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, floa
     }
   };
   auto ty = [&](const float *in, float *out) {
-    for (int q = t; q < V; q += 256) {
+    for (int q = t; q < V; q += kDstThreads) {
       const int i = q % M, j = (q / M) % M, k = q / (M * M);
       float acc = 0.f;
       for (int m = 0; m < M; m++) acc += sn[j][m] * in[(k * M + m) * M + i];
@@ -749,7 +750,7 @@ __global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, floa
     }
   };
   auto tz = [&](const float *in, float *out) {
-    for (int q = t; q < V; q += 256) {
+    for (int q = t; q < V; q += kDstThreads) {
       const int ij = q % (M * M), k = q / (M * M);
       float acc = 0.f;
       for (int m = 0; m < M; m++) acc += sn[k][m] * in[m * M * M + ij];
@@ -762,7 +763,7 @@ __global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, floa
   __syncthreads();
   tz(X, Y);
   __syncthreads();
-  for (int q = t; q < V; q += 256) {
+  for (int q = t; q < V; q += kDstThreads) {
     const int i = q % M, j = (q / M) % M, k = q / (M * M);
     // lap_h eigenvalue -(lam_i + lam_j + lam_k) / h^2; 1/512 normalises S3 S3
     Y[q] = Y[q] / (-(lam[i] + lam[j] + lam[k]) * h2inv * 512.0f);
@@ -774,7 +775,7 @@ __global__ __launch_bounds__(256) void k_dst_solve17(float *__restrict__ u, floa
   __syncthreads();
   tz(Y, X);
   __syncthreads();
-  for (int q = t; q < V; q += 256) {
+  for (int q = t; q < V; q += kDstThreads) {
     const int i = q % M, j = (q / M) % M, k = q / (M * M);
     const size_t p = ix3(i + 1, j + 1, k + 1, N, N);
     u[p] = X[q];
@@ -801,7 +802,7 @@ int bottom(pmg_solver *S, int Lb, int norm_base, double *cnorm, int *citer) {
       int rc;
       if ((rc = lop_rhs<float>(S, Lb))) return rc;
       LevelArrays<float> &A = arrays<float>(Lv);
-      hipLaunchKernelGGL(k_dst_solve17, dim3(1), dim3(256), 0, S->stream, A.u, A.ua, A.rho, A.rhs,
+      hipLaunchKernelGGL(k_dst_solve17, dim3(1), dim3(kDstThreads), 0, S->stream, A.u, A.ua, A.rho, A.rhs,
                          (float)(1.0 / (Lv.hdx[0] * Lv.hdx[0])), cnorm, citer);
       PMG_HOST_CHECK(hipGetLastError());
       return 0;
