@@ -177,7 +177,9 @@ int pmg_zero_solution(pmg_solver *S);
  * with HIP events on the solver's stream: which 0 = red-black sweep,
  * 1 = residual, 2 = the finest residual fused with the full-weight
  * injection into level L-1 (L the finest level; -1 where that fusion does
- * not apply; it overwrites the coarse u and f_res); prec 0 = FP64, 1 = FP32 */
+ * not apply; it overwrites the coarse u and f_res), 3 = the finest FP64 sweep
+ * inside nrep ungraphed V-cycles (events around each pre- and post-smoothing
+ * group; the cycles advance u); prec 0 = FP64, 1 = FP32 */
 int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double *ms_avg);
 /* execution tuning: sweep tile variant and z chunk; graph capture on/off */
 int pmg_set_sweep_config(pmg_solver *S, int variant, int zchunk);

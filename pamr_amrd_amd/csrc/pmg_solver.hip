@@ -165,6 +165,10 @@ struct pmg_solver {
   double *kry_host = nullptr;
   hipGraphExec_t gprec = nullptr;  // captured FP32 V-cycle (the preconditioner)
   int prec_cycles = 1;             // FP32 V-cycles per preconditioner application
+  // kernel timing inside V-cycles (pmg_time_kernel which 3): HIP events
+  // around the finest level's FP64 sweep launches of ungraphed cycles
+  hipEvent_t *tk_ev = nullptr;
+  int tk_cap = 0, tk_n = 0, tk_sweeps = 0;
   // learned correction in the band next to the excised region (pmg_band.hip)
   struct Band {
     int depth = 0, core = 0, side = 0, margin = 3, ncore = 0;
@@ -923,6 +927,22 @@ int resid_restrict(pmg_solver *S, int slot) {
 
 int band_inner(pmg_solver *S);
 
+// sweeps(), with HIP events around the launches of the finest FP64 level
+// while pmg_time_kernel (which 3) collects in-cycle durations
+template <typename T>
+int timed_sweeps(pmg_solver *S, int L, int n, int slot, bool rhs0) {
+  const bool tk = std::is_same<T, double>::value && L == S->nlev && S->tk_ev && n > 0 &&
+                  S->tk_n + 2 <= S->tk_cap;
+  if (tk) PMG_HOST_CHECK(hipEventRecord(S->tk_ev[S->tk_n], S->stream));
+  const int rc = sweeps<T>(S, L, n, slot, rhs0);
+  if (tk) {
+    PMG_HOST_CHECK(hipEventRecord(S->tk_ev[S->tk_n + 1], S->stream));
+    S->tk_n += 2;
+    S->tk_sweeps += n;
+  }
+  return rc;
+}
+
 template <typename T>
 int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cnorm, int *citer) {
   const int Lf = S->nlev;
@@ -950,7 +970,7 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
     if (L == 1) {
       if ((rc = coarse_solve<T>(S, 1, cnorm, cnorm + 1, citer, S->max_citer))) return rc;
     } else {
-      if ((rc = sweeps<T>(S, L, S->pre, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
+      if ((rc = timed_sweeps<T>(S, L, S->pre, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
     }
   }
   for (int L = Lb ? Lb + 1 : 2; L <= Lf; L++) {
@@ -959,7 +979,7 @@ int enqueue_vcycle(pmg_solver *S, bool final_residual, int norm_base, double *cn
       // the learned band correction of the finest level's coarse-grid correction
       if (L == Lf && S->band.enq_inner && (rc = band_inner(S))) return rc;
     }
-    if ((rc = sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
+    if ((rc = timed_sweeps<T>(S, L, S->pst, norm_base + NS_RELAX + L - 1, L == Lf))) return rc;
     if ((rc = settle<T>(S, L))) return rc;
   }
   if (final_residual) {
@@ -2411,6 +2431,37 @@ int pmg_time_kernel(pmg_solver *S, int which, int L, int prec, int nrep, double 
   // to its buffer), 1 = residual.  prec: 0 FP64, 1 FP32.
   if (!S || L < 1 || L > S->nlev || nrep < 2 || !ms_avg) return -1;
   if (prec == 1 && ensure_fp32(S)) return -1;
+  if (which == 3) {
+    // the finest FP64 sweep inside nrep ungraphed V-cycles: events around
+    // each pre- and post-smoothing group, the average per launch
+    if (prec != 0 || L != S->nlev) return -1;
+    std::vector<hipEvent_t> evs((size_t)4 * nrep, nullptr);
+    int rc = 0;
+    for (auto &e : evs)
+      if (hipEventCreate(&e) != hipSuccess) { rc = -1; break; }
+    const int keep = S->use_graph;
+    if (!rc) {
+      S->tk_ev = evs.data();
+      S->tk_cap = (int)evs.size();
+      S->tk_n = S->tk_sweeps = 0;
+      S->use_graph = 0;
+      for (int r = 0; r < nrep && !rc; r++) rc = enq_v64(S);
+      S->use_graph = keep;
+      if (!rc && hipStreamSynchronize(S->stream) != hipSuccess) rc = -1;
+      double tot = 0.0;
+      for (int q = 0; !rc && q + 1 < S->tk_n; q += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, evs[q], evs[q + 1]) != hipSuccess) rc = -1;
+        tot += ms;
+      }
+      if (!rc) *ms_avg = S->tk_sweeps > 0 ? tot / S->tk_sweeps : 0.0;
+    }
+    S->tk_ev = nullptr;
+    S->tk_cap = S->tk_n = S->tk_sweeps = 0;
+    for (auto &e : evs)
+      if (e) (void)hipEventDestroy(e);
+    return rc;
+  }
   EventPair ev;
   PMG_HOST_CHECK(ev.create());
   hipEvent_t e0 = ev.e0, e1 = ev.e1;

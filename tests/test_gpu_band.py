@@ -344,7 +344,7 @@ def test_gather_res32_bit_exact(pm, torch_cuda):
     assert np.array_equal(out.cpu().numpy(), want)
 
 
-def test_in_cycle_band_net_is_judged(pm, torch_cuda):
+def test_in_cycle_band_net_is_checked(pm, torch_cuda):
     """The correction placed inside the FP32 V-cycle, after the finest
     coarse-grid correction (trained on crops taken there), on held-out
     right-hand sides (65^3, variable coefficient, three excised spheres).

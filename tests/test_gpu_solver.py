@@ -374,3 +374,19 @@ def test_fp32_exact_bottom(pm, n):
     print(f"relaxed bottom: {out[False]}, exact bottom: {out[True]}", flush=True)
     assert out[True][0] <= out[False][0]
     S.close()
+
+
+def test_time_kernel_in_cycle(pm):
+    """pmg_time_kernel which 3: the finest FP64 sweep timed inside ungraphed
+    V-cycles -- positive and of the order of the back-to-back figure; FP32
+    is refused."""
+    n = 65
+    rho = -pm.gaussian_source(n, seed=1234)
+    S, m = _pair(pm, n, rho)
+    t_in = S.time_kernel(3, S.nlev, 0, 4)
+    t_b2b = S.time_kernel(0, S.nlev, 0, 4)
+    print(f"in-cycle {t_in:.4f} ms, back to back {t_b2b:.4f} ms", flush=True)
+    assert 0 < t_in < 10 * t_b2b
+    with pytest.raises(RuntimeError):
+        S.time_kernel(3, S.nlev, 1, 4)
+    S.close()
