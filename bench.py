@@ -360,7 +360,7 @@ def excised_leg(args, pm, torch):
 
     # ---- learned band correction: plan and training (not timed) ----
     t0 = time.perf_counter()
-    ncore = S.band_setup(3, 26)
+    ncore = S.band_setup(args.c5_band_depth, 26)
     npts, _, side = S.band_info()
     t_plan = time.perf_counter() - t0
     log(f"band: {npts} points in {ncore} cores of 26^3 ({t_plan:.1f} s)")
@@ -372,12 +372,14 @@ def excised_leg(args, pm, torch):
     # FP64 defect) and inside the FP32 V-cycle after the finest coarse-grid
     # correction
     t0 = time.perf_counter()
-    x, t, w = BD.collect_pairs(S, train_rhos, ustars, u0, torch, iters=10, crops=48)
+    x, t, w = BD.collect_pairs(S, train_rhos, ustars, u0, torch, iters=args.c5_band_iters,
+                               crops=args.c5_band_crops)
     t_collect = time.perf_counter() - t0
     net, ts, tinfo = BD.train(x, t, w, torch, steps=args.c5_band_steps, log=log)
     del x, t, w
     t0 = time.perf_counter()
-    x, t, w = BD.collect_pairs_inner(S, train_rhos, ustars, u0, torch, iters=10, crops=48)
+    x, t, w = BD.collect_pairs_inner(S, train_rhos, ustars, u0, torch, iters=args.c5_band_iters,
+                                     crops=args.c5_band_crops)
     t_collect_in = time.perf_counter() - t0
     net_in, ts_in, tinfo_in = BD.train(x, t, w, torch, steps=args.c5_band_steps, log=log)
     del x, t, w, ustars, train_rhos
@@ -471,7 +473,7 @@ def excised_leg(args, pm, torch):
             src = net_in if inner else net
             wts = src.get_weights()
             for X in solvers[1:]:
-                X.band_setup(3, 26)
+                X.band_setup(args.c5_band_depth, 26)
                 nx_ = pm.Net(seed=1)
                 nx_.set_weights(wts)
                 if inner:
@@ -522,7 +524,7 @@ def excised_leg(args, pm, torch):
             "speedup_dc_band_vs_fgmres": round(mean["fgmres"] / mean["dc_band"], 3),
             "fastest": best,
             "fgmres_restart": args.c5_restart,
-            "band": {"depth": 3, "core": 26, "crop": side, "cores": ncore, "points": npts,
+            "band": {"depth": args.c5_band_depth, "core": 26, "crop": side, "cores": ncore, "points": npts,
                      "plan_s": round(t_plan, 2), "training": tinfo, "training_in_cycle": tinfo_in,
                      "stats": bstats,
                      "roofline_mfma": roof_net},
@@ -618,7 +620,10 @@ def main():
     ap.add_argument("--c5-mixed-rhs", type=int, default=2, help="right-hand sides also solved by defect correction")
     ap.add_argument("--c5-band-train", type=int, default=2,
                     help="extra right-hand sides whose plain iterations train the band correction")
-    ap.add_argument("--c5-band-steps", type=int, default=1500, help="Adam steps of the band correction")
+    ap.add_argument("--c5-band-steps", type=int, default=4000, help="Adam steps of the band correction")
+    ap.add_argument("--c5-band-iters", type=int, default=10, help="plain iterations sampled per training RHS")
+    ap.add_argument("--c5-band-crops", type=int, default=48, help="training crops per sampled iteration")
+    ap.add_argument("--c5-band-depth", type=int, default=3, help="band: points within this distance of the excision")
     ap.add_argument("--c5-streams", type=int, default=4,
                     help="solver instances (streams, host threads) of the concurrent multi-RHS run; 1 skips it")
     ap.add_argument("--probe-net", action="store_true",

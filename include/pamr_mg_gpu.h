@@ -163,7 +163,7 @@ int pmg_solve_rel(pmg_solver *S, int mode, double rtol, int max_iter, double *hi
  * preconditioner one FP32 V-cycle from zero (the mixed mode's inner cycle).
  * hist[k]: the residual estimate after preconditioner application k+1, the
  * FP64 residual at the end of each restart; iters counts preconditioner
- * applications.  Holds 2m + 3 finest-level FP64 vectors. */
+ * applications.  Holds 2m + 2 finest-level FP64 vectors. */
 int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hist, int *iters,
                      double *r0_out);
 int pmg_last_citer(pmg_solver *S, int which);

@@ -22,6 +22,7 @@ holding the crop tensors.
 """
 from __future__ import annotations
 
+import math
 import time
 
 from . import Net, PMG_F_RHO, PMG_F_U, lib, tptr
@@ -90,9 +91,12 @@ def collect_pairs_inner(S, rhos, ustars, u0, torch, iters=10, crops=48, seed=91)
     return x, t, w
 
 
-def train(x, t, w, torch, steps=1500, batch=16, lr=2e-3, seed=5, margin=3, net=None, log=None):
+def train(x, t, w, torch, steps=4000, batch=16, lr=2e-3, seed=5, margin=3, net=None, log=None):
     """Train a network on crop tensors (targets scaled here by 1/ts, ts the
-    RMS of t over the weighted core points).  Returns (net, ts, info)."""
+    RMS of t over the weighted core points), Adam with the learning rate
+    decaying as a half cosine to zero over the steps (the last weights are
+    then a converged state, not a sample of the constant-rate noise).
+    Returns (net, ts, info)."""
     side = x.shape[-1]
     core = torch.zeros_like(w[0])
     core[margin:side - margin, margin:side - margin, margin:side - margin] = 1.0
@@ -108,7 +112,8 @@ def train(x, t, w, torch, steps=1500, batch=16, lr=2e-3, seed=5, margin=3, net=N
         idx = torch.randint(0, x.shape[0], (batch,), generator=g, device="cuda")
         xb, tb, wb = x[idx].contiguous(), tn[idx].contiguous(), w[idx].contiguous()
         torch.cuda.synchronize()
-        loss = net.train_step(xb, tb, (side, side, side), batch, lr, margin=margin, w_dev=wb)
+        lr_s = 0.5 * lr * (1.0 + math.cos(math.pi * s / steps))
+        loss = net.train_step(xb, tb, (side, side, side), batch, lr_s, margin=margin, w_dev=wb)
         losses.append(loss)
         if log and (s % 250 == 0 or s == steps - 1):
             log(f"band net step {s}: loss {loss:.4e}")

@@ -634,7 +634,7 @@ __global__ __launch_bounds__(RTX * RTY) void k_residual(ResidP<T> P) {
     bool on = inxy && k >= P.bz0 && k <= P.bz1;
     if constexpr (EX) { if (interior) on = on && !P.chr[p]; }
     if (on) {
-      rhov = P.rho[p];
+      rhov = P.rho ? P.rho[p] : T(0);  // no source array: a zero source (FGMRES's operator)
       if constexpr (!RHS0) rhsv = P.rhs[p];
     }
     __syncthreads();  // the previous plane's readers are done

@@ -2233,7 +2233,7 @@ int ensure_krylov(pmg_solver *S, int m) {
   if (S->kry && S->kry_m >= m) return 0;
   if (S->kry) { PMG_HOST_CHECK(hipStreamSynchronize(S->stream)); (void)hipFree(S->kry); S->kry = nullptr; }
   const size_t n = S->lev[S->nlev - 1].size;
-  const size_t nvec = (size_t)(2 * m + 3);
+  const size_t nvec = (size_t)(2 * m + 2);
   PMG_HOST_CHECK(hipMalloc(&S->kry, nvec * n * sizeof(double)));
   PMG_HOST_CHECK(hipMemsetAsync(S->kry, 0, nvec * n * sizeof(double), S->stream));
   if (!S->kry_red) {
@@ -2261,7 +2261,6 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
   for (int i = 0; i <= m; i++) V[i] = S->kry + (size_t)i * n;
   for (int i = 0; i < m; i++) Z[i] = S->kry + (size_t)(m + 1 + i) * n;
   double *w = S->kry + (size_t)(2 * m + 1) * n;
-  const double *zero = S->kry + (size_t)(2 * m + 2) * n;
   double *hh = S->kry_host;  // [0, kKryMax]: dots / norms
   auto true_residual = [&](double *rms) -> int {
     int r;
@@ -2335,7 +2334,7 @@ int pmg_solve_fgmres(pmg_solver *S, int m, double rtol, int max_iter, double *hi
       // Gram-Schmidt, repeated when the first pass removed more than 1 - 1/2
       // of w's squared norm (the criterion of Daniel, Gragg, Kaufman and
       // Stewart, Math. Comp. 30 (1976) 772-795)
-      if ((rc = resid_vec(S, Z[j], zero, V[j + 1], NS_OUTER + 1))) return rc;
+      if ((rc = resid_vec(S, Z[j], nullptr, V[j + 1], NS_OUTER + 1))) return rc;
       VecPtrs PV{};
       for (int i = 0; i <= j; i++) PV.p[i] = V[i];
       PV.p[j + 1] = V[j + 1];
