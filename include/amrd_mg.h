@@ -160,6 +160,33 @@ int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out);
 void solve_elliptics(int L, int tl, int save_tre);
 double vcycle(int giter, int save_tre);
 
+/* AMRD's MG tracing parameters (amrd/io.c:80-83,256-259): MG_trace 1-4
+ * prints, on stdout and in the reference's format, the solve's level range
+ * and cycle count (1), the residual after every cycle (2), every level's
+ * norm after its sweeps (3) and after every sweep (4) (amrd/mg.c:35-41,71,76,
+ * 138,142,164,195).  MG_DV_trace != 0 makes every V-cycle dump its MG
+ * grid functions with mg_dump at its four stages (amrd/mg.c:133,144,159,197)
+ * while MG_DV_trace_t_on <= t <= MG_DV_trace_t_off, t the time of the finest
+ * AMR level.  Defaults 0, -1, 1e-15, 1e-15. */
+void amrd_mg_set_trace(int MG_trace, int MG_DV_trace, double MG_DV_trace_t_on, double MG_DV_trace_t_off);
+
+/* The variable lists mg_dump writes with which = 0 after the elliptic
+ * variables and their _lop/_rhs/_res (amrd/mg.c:757-763): hyperbolic_vars,
+ * then "cmask", then MGH_work_vars, then AMRH_work_in_MGH_vars
+ * (AMRD_hyperbolic_vars, AMRD_MGH_work_vars, AMRD_AMRH_work_in_MGH_vars,
+ * read from the parameter file at amrd/io.c:318-352).  Space- or
+ * comma-separated names; NULL or "" for none.  Returns 1. */
+int amrd_mg_def_dump_vars(const char *hyperbolic_vars, const char *MGH_work_vars, const char *AMRH_work_in_MGH_vars);
+
+/* mg_dump(L, tag, giter, which) (amrd/mg.c:707-766, the reference's name
+ * and signature): PAMR_save_gfn of the MG grid functions of level L with
+ * pre_tag `tag` at time lcount + giter / (max_iter + 2), lcount counting the
+ * solves (giter returning to 0); which 2: the elliptic variables, 1: also
+ * their _lop, _rhs, _res, 0: also the amrd_mg_def_dump_vars lists.  Does
+ * nothing unless MG_DV_trace is set and the finest AMR level's time is in
+ * its window. */
+void mg_dump(int L, char *tag, int giter, int which);
+
 /* last norms of amrd_vcycle: lev_norms[L-1] for MG levels 1..Lf (the
  * reference's res[] array) and the coarsest-level sweep count */
 int amrd_mg_last_norms(double *lev_norms, int *citer);

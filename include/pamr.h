@@ -154,6 +154,33 @@ int PAMR_get_min_lev(int hier);
 int PAMR_get_g_comm(int *comm);
 int PAMR_set_g_comm(int comm);
 
+/* ---- grid-function output (include/pamr.h:202-203; src/pamr.c:1908-1978)
+ * The grid function `name` of time level tl (AMR hierarchy) or the MG
+ * hierarchy, every grid of level L (L < 0: every level), copied from the
+ * device after the stream's earlier work and written as one record per grid
+ * to "<pre_tag><name>_tl<tl><post_tag>_0" (AMR) or
+ * "<pre_tag><name>_MG<post_tag>_0" (MG), with time t (-1.0: the level's
+ * time).  The _close form closes that output afterwards, so the next save
+ * to the name starts it again.  Records go to the writer registered with
+ * PAMR_set_gft_writer; without one to the built-in record file
+ * "<output name>.pgf" (below).  Returns 1 / 0. */
+int PAMR_save_gfn(char *name, int hier, int tl, int L, double t, char *pre_tag, char *post_tag);
+int PAMR_save_gfn_close(char *name, int hier, int tl, int L, double t, char *pre_tag, char *post_tag);
+
+/* No counterpart in the reference, which calls bbhutil's SDF writer
+ * gft_out_bbox / gft_close directly (src/pamr.c:1946,1951).  An application
+ * that links bbhutil passes those two functions here and gets the
+ * reference's SDF files; close may be NULL.  out_bbox(NULL, NULL) restores
+ * the built-in writer: records appended to "<output name>.pgf" (the file is
+ * truncated by its first record after the process starts or after a close),
+ * each one, in the host's byte order,
+ *   char magic[4] = "PGF1"; int32 rank; float64 time; int32 shape[rank];
+ *   float64 bbox[2 * rank]; float64 data[shape[0] * ... * shape[rank-1]]
+ * with data in Fortran order (x fastest), as gft_out_bbox takes it. */
+typedef int (*PAMR_gft_out_bbox_fn)(char *name, double time, int *shape, int rank, double *bbox, double *data);
+typedef int (*PAMR_gft_close_fn)(char *name);
+void PAMR_set_gft_writer(PAMR_gft_out_bbox_fn out_bbox, PAMR_gft_close_fn close);
+
 #ifdef __cplusplus
 }
 #endif

@@ -285,6 +285,15 @@ def _declare_pamr(L):
     L.amrd_mg_last_norms.argtypes = [_dp, _ip]
     L.amrd_gpu_stream.argtypes = []
     L.amrd_gpu_stream.restype = _vp
+    L.PAMR_save_gfn.argtypes = [s, i, i, i, d, s, s]
+    L.PAMR_save_gfn_close.argtypes = [s, i, i, i, d, s, s]
+    L.PAMR_set_gft_writer.argtypes = [_vp, _vp]
+    L.PAMR_set_gft_writer.restype = None
+    L.amrd_mg_set_trace.argtypes = [i, i, d, d]
+    L.amrd_mg_set_trace.restype = None
+    L.amrd_mg_def_dump_vars.argtypes = [s, s, s]
+    L.mg_dump.argtypes = [i, s, i, i]
+    L.mg_dump.restype = None
 
 
 def _check(rc, what):

@@ -174,6 +174,15 @@ int plevel(int l, int hier) {
   return 0;
 }
 
+// time of level l of a hierarchy: MG-only levels carry the time of the AMR
+// level the MG hierarchy starts at (amr_to_mgh / build_mgh copy the AMR
+// level's t, src/gh.c:1108,1312)
+double level_time(int l, int hier) {
+  const int Lp = plevel(l, hier);
+  const int amr_l = hier == PAMR_MGH ? C->mg_min + std::max(0, Lp - mg_nbase()) : Lp - C->nbase + 1;
+  return C->t[std::max(1, amr_l) - 1];
+}
+
 std::vector<std::vector<View>> &views_of(int hier) {
   return (hier == PAMR_MGH && C->M && C->M != C->A) ? C->mviews : C->views;
 }
@@ -365,8 +374,20 @@ struct Amrd {
   double last[kMaxLevs + 2];
   int last_nlev = 0, last_citer = 0;
   int cwarn = 10;
+  // tracing (amrd/io.c:80-83): MG_trace, MG_DV_trace and its time window;
+  // mg_dump's solve counter and last giter (static in amrd/mg.c:713)
+  int trace = 0, dv_trace = -1;
+  double dv_t_on = 1e-15, dv_t_off = 1e-15;
+  double dv_lcount = -1, dv_pgiter = -1;
+  std::vector<std::string> hyp, mgh_work, amrh_work_mg;  // mg_dump's which = 0 lists
 };
 Amrd D;
+
+// the reference's indentation of the per-level trace lines (amrd/mg.c:138)
+void trace_indent(int L, int Lf) {
+  for (int i = L - 1; i < Lf; i++) std::printf("   ");
+}
+
 
 std::vector<std::string> split_list(const char *s) {
   std::vector<std::string> out;
@@ -386,6 +407,42 @@ std::vector<std::string> split_list(const char *s) {
 
 bool in_list(const std::string &n, const std::vector<std::string> &l) {
   return std::find(l.begin(), l.end(), n) != l.end();
+}
+
+// ---------------------------------------------------------------------------
+// grid-function output behind PAMR_save_gfn: the application's writer (for
+// example bbhutil's gft_out_bbox / gft_close) or the built-in record file
+// ---------------------------------------------------------------------------
+PAMR_gft_out_bbox_fn gft_out_app = nullptr;
+PAMR_gft_close_fn gft_close_app = nullptr;
+std::vector<std::string> pgf_open;  // built-in outputs written since their last close
+
+int pgf_out_bbox(char *name, double t, int *shape, int rank, double *bbox, double *data) {
+  const std::string path = std::string(name) + ".pgf";
+  const bool fresh = !in_list(name, pgf_open);
+  FILE *f = std::fopen(path.c_str(), fresh ? "wb" : "ab");
+  if (!f) {
+    std::printf("PAMR_save_gfn: error ... cannot open %s\n", path.c_str());
+    return 0;
+  }
+  if (fresh) pgf_open.push_back(name);
+  size_t n = 1;
+  for (int d = 0; d < rank; d++) n *= (size_t)shape[d];
+  const int32_t r = rank;
+  bool ok = std::fwrite("PGF1", 1, 4, f) == 4 && std::fwrite(&r, 4, 1, f) == 1 && std::fwrite(&t, 8, 1, f) == 1;
+  for (int d = 0; d < rank && ok; d++) {
+    const int32_t s = shape[d];
+    ok = std::fwrite(&s, 4, 1, f) == 1;
+  }
+  ok = ok && std::fwrite(bbox, 8, 2 * (size_t)rank, f) == 2 * (size_t)rank && std::fwrite(data, 8, n, f) == n;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) std::printf("PAMR_save_gfn: error ... writing %s failed\n", path.c_str());
+  return ok ? 1 : 0;
+}
+
+int pgf_close(char *name) {
+  pgf_open.erase(std::remove(pgf_open.begin(), pgf_open.end(), std::string(name)), pgf_open.end());
+  return 1;
 }
 
 struct EllGfns {
@@ -1317,10 +1374,7 @@ int PAMR_get_g_ghost_width(int *ghost_width) {
 }
 int PAMR_get_g_t(double *t) {
   if (!cur_view("PAMR_get_g_t")) return 0;
-  // MG-only levels carry the time of the AMR level the MG hierarchy starts at
-  const int Lp = plevel(C->it.L, C->it.hier);
-  const int amr_l = C->it.hier == PAMR_MGH ? C->mg_min + std::max(0, Lp - mg_nbase()) : Lp - C->nbase + 1;
-  *t = C->t[std::max(1, amr_l) - 1];
+  *t = level_time(C->it.L, C->it.hier);
   return 1;
 }
 int PAMR_get_g_ngfs(int *ngfs) {
@@ -1377,6 +1431,74 @@ int PAMR_get_g_attribs(int *rank, int *dim, int *shape, double *bbox, int *ghost
   ret *= PAMR_get_g_x(x);
   ret *= PAMR_get_g_gfs(gfs);
   return ret;
+}
+
+// ---- grid-function output (src/pamr.c:1908-1978) ---------------------------
+static int do_save_gfn(const char *who, char *name, int hier, int tl, int L, double t, char *pre_tag,
+                       char *post_tag, int close) {
+  if (!need_ctx(who)) return 0;
+  const int gfn = PAMR_get_gfn(name, hier, tl);
+  if (!gfn) return 0;
+  char out_name[512];
+  if (hier == PAMR_AMRH) {
+    if (!C->A) { std::printf("%s: AMR hierarchy does not exist\n", who); return 0; }
+    std::snprintf(out_name, sizeof out_name, "%s%s_tl%i%s_%i", pre_tag ? pre_tag : "", name, tl,
+                  post_tag ? post_tag : "", 0);
+  } else {
+    if (!C->mgh) { std::printf("%s: MG hierarchy does not exist\n", who); return 0; }
+    std::snprintf(out_name, sizeof out_name, "%s%s_MG%s_%i", pre_tag ? pre_tag : "", name,
+                  post_tag ? post_tag : "", 0);
+  }
+  const int lmin = PAMR_get_min_lev(hier), lmax = PAMR_get_max_lev(hier);
+  const int ls = L < 0 ? lmin : L, le = L < 0 ? lmax : L;
+  if (ls < lmin || le > lmax) {
+    std::printf("%s: error ... no level %i in the %s hierarchy\n", who, L, hier == PAMR_MGH ? "MG" : "AMR");
+    return 0;
+  }
+  if (hipStreamSynchronize(C->stream) != hipSuccess) {
+    std::printf("%s: error ... device stream failed\n", who);
+    return 0;
+  }
+  PAMR_gft_out_bbox_fn out = gft_out_app ? gft_out_app : pgf_out_bbox;
+  std::vector<double> host;
+  for (int l = ls; l <= le; l++) {
+    const int Lp = plevel(l, hier);
+    const std::vector<View> &vs = views_of(hier)[Lp - 1];
+    for (int g = 0; g < (int)vs.size(); g++) {
+      const double *dev = vs[g].gfs[gfn - 1];
+      if (!dev) continue;  // the grid function does not exist on this grid
+      int shp[3];
+      double bb[6];
+      pmg_amr_grid_info(hobj(hier), Lp, g, shp, bb);
+      host.resize((size_t)shp[0] * shp[1] * shp[2]);
+      if (hipMemcpy(host.data(), dev, host.size() * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) {
+        std::printf("%s: error ... device copy failed\n", who);
+        return 0;
+      }
+      out(out_name, t == -1.0 ? level_time(l, hier) : t, shp, 3, bb, host.data());
+    }
+  }
+  if (close) {
+    if (gft_out_app) {
+      if (gft_close_app) gft_close_app(out_name);
+    } else {
+      pgf_close(out_name);
+    }
+  }
+  return 1;
+}
+
+int PAMR_save_gfn(char *name, int hier, int tl, int L, double t, char *pre_tag, char *post_tag) {
+  return do_save_gfn("PAMR_save_gfn", name, hier, tl, L, t, pre_tag, post_tag, 0);
+}
+
+int PAMR_save_gfn_close(char *name, int hier, int tl, int L, double t, char *pre_tag, char *post_tag) {
+  return do_save_gfn("PAMR_save_gfn_close", name, hier, tl, L, t, pre_tag, post_tag, 1);
+}
+
+void PAMR_set_gft_writer(PAMR_gft_out_bbox_fn out_bbox, PAMR_gft_close_fn close) {
+  gft_out_app = out_bbox;
+  gft_close_app = out_bbox ? close : nullptr;
 }
 
 // ===========================================================================
@@ -1469,6 +1591,49 @@ void amrd_mg_set_params(int pre_swp, int pst_swp, int min_iter, int max_iter, do
   D.max_citer = max_citer;
   D.w0_r = w0_r;
   D.w0_i = w0_i;
+}
+
+void amrd_mg_set_trace(int MG_trace, int MG_DV_trace, double MG_DV_trace_t_on, double MG_DV_trace_t_off) {
+  D.trace = MG_trace;
+  D.dv_trace = MG_DV_trace;
+  D.dv_t_on = MG_DV_trace_t_on;
+  D.dv_t_off = MG_DV_trace_t_off;
+}
+
+int amrd_mg_def_dump_vars(const char *hyperbolic_vars, const char *MGH_work_vars, const char *AMRH_work_in_MGH_vars) {
+  D.hyp = split_list(hyperbolic_vars);
+  D.mgh_work = split_list(MGH_work_vars);
+  D.amrh_work_mg = split_list(AMRH_work_in_MGH_vars);
+  return 1;
+}
+
+// amrd/mg.c:707-766
+void mg_dump(int L, char *tag, int giter, int which) {
+  if (!D.dv_trace || !C) return;
+  const double t_lf = PAMR_get_time(PAMR_get_max_lev(PAMR_AMRH));
+  if (D.dv_t_on > t_lf || D.dv_t_off < t_lf) return;
+  if (giter == 0 && D.dv_pgiter != giter) D.dv_lcount++;
+  D.dv_pgiter = giter;
+  const double t = D.dv_lcount + giter / (D.max_iter + 2.0);
+  std::vector<std::string> ell = D.ell;
+  if (D.is_t0) ell.insert(ell.end(), D.ell_t0.begin(), D.ell_t0.end());
+  auto save = [&](const std::string &v) {
+    PAMR_save_gfn((char *)v.c_str(), PAMR_MGH, 0, L, t, tag, (char *)"");
+  };
+  for (const std::string &v : ell) save(v);
+  if (which < 2) {
+    for (const std::string &v : ell) {
+      save(v + "_lop");
+      save(v + "_rhs");
+      save(v + "_res");
+    }
+    if (which < 1) {
+      for (const std::string &v : D.hyp) save(v);
+      save("cmask");
+      for (const std::string &v : D.mgh_work) save(v);
+      for (const std::string &v : D.amrh_work_mg) save(v);
+    }
+  }
 }
 
 // amrd/mg.c:84-201
@@ -1607,7 +1772,6 @@ int amrd_update_mg_extrap_vars(int L1, int Lf, double eps_c) {
 double amrd_vcycle(int giter) { return amrd_vcycle_tre(giter, 0); }
 
 double amrd_vcycle_tre(int giter, int save_tre) {
-  (void)giter;
   if (!C || !C->mgh) {
     std::printf("amrd_vcycle: error ... no MG hierarchy\n");
     return -1.0;
@@ -1641,16 +1805,22 @@ double amrd_vcycle_tre(int giter, int save_tre) {
       if (!compute_mg_rhs(L, E, cmask, save_tre)) return -1.0;
     }
     if (!set_cmask(L, 0, cmask)) return -1.0;
+    mg_dump(L, (char *)"before_preswp_", giter, 0);
     for (iter = 0; iter < D.pre; iter++) {
       res[L] = relax_level(L, 0);
+      if (D.trace > 3) { trace_indent(L, Lf); std::printf("(L%i(%i):%12.8e)\n", L, iter, res[L]); }
       if (!PAMR_sync(L, 0, PAMR_MGH, iter == 0 ? 0 : 1)) return -1.0;
     }
+    if (D.trace > 2) { trace_indent(L, Lf); std::printf("L%i(%i):%12.8e\n", L, iter, res[L]); }
+    mg_dump(L, (char *)"after_preswp_", giter, 0);
   }
   for (int L = 1; L <= Lf; L++) {
     const double ctol = res[L] * D.crtol;
     if (L > 1 && !compute_cgc(L, E)) return -1.0;
+    mg_dump(L, (char *)"after_cgc_", giter, 1);
     for (iter = 0; iter < D.pst; iter++) {
       res[L] = relax_level(L, 0);
+      if (D.trace > 3) { trace_indent(L, Lf); std::printf("(L%i(%i):%12.8e)\n", L, iter, res[L]); }
       if (!PAMR_sync(L, 0, PAMR_MGH, iter == 0 ? 0 : 1)) return -1.0;
     }
     if (is_coarsest(L)) {  // amrd/mg.c:172-190
@@ -1670,6 +1840,8 @@ double amrd_vcycle_tre(int giter, int save_tre) {
       D.last_citer = iter;
     }
     if (L == Lf) res[Lf] = residual_level(L);
+    if (D.trace > 2) { trace_indent(L, Lf); std::printf("L%i(%i):%12.8e\n", L, iter, res[L]); }
+    mg_dump(L, (char *)"after_pstswp_", giter, 1);
   }
   for (int L = 1; L <= Lf; L++) D.last[L - 1] = res[L];
   D.last_nlev = Lf;
@@ -1748,6 +1920,12 @@ int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out) {
     return -1;
   }
   const int Lf = PAMR_get_max_lev(PAMR_MGH);
+  if (D.trace) {  // mg.c:35-41
+    std::printf("\n\n");
+    for (int l = 0; l < L - 1; l++) std::printf(" ");
+    std::printf("MG L = %i - %i, MG Lf=%i, tl=%i ...", L, Lf_amr, Lf, tl);
+  }
+  if (D.trace > 1) std::printf("\n");
   if (D.t0_init && D.is_t0 && !t0_pass(D.t0_init, D.ell_t0, Lf)) {  // mg.c:45-52
     PAMR_destroy_mgh();
     return -1;
@@ -1765,10 +1943,14 @@ int amrd_solve_elliptics_tre(int L, int tl, int save_tre, double *res_out) {
       return -1;
     }
     it++;
+    if (D.trace > 1) std::printf("iter %i\t\tres(Lf)=%12.8e\n", it, res);
   }
   if (it == D.max_iter && res > D.tol)
     std::printf("\nWARNING ... failed to solve elliptics to within %12.8e in %i iterations; res=%12.8e\n", D.tol,
                 D.max_iter, res);
+  else if (D.trace)
+    std::printf("  iters=%i\n", it);
+  std::fflush(stdout);
   PAMR_destroy_mgh();
   if (res_out) *res_out = res;
   return it;

@@ -798,3 +798,147 @@ def test_evo_iterate_on_refined_hierarchy(torch_cuda):
         assert np.array_equal(ctx.download(PAMR_AMRH, 2, g, "phi", tl=1), p1[g]), g
     ctx.close()
     del s
+
+
+# ---------------------------------------------------------------------------
+# grid-function output and MG tracing (src/pamr.c:1908-1978, amrd/mg.c:707-766)
+# ---------------------------------------------------------------------------
+GFT_OUT = C.CFUNCTYPE(C.c_int, C.c_char_p, C.c_double, C.POINTER(C.c_int), C.c_int, C.POINTER(C.c_double),
+                      C.POINTER(C.c_double))
+GFT_CLOSE = C.CFUNCTYPE(C.c_int, C.c_char_p)
+
+
+def test_save_gfn_writers(torch_cuda, tmp_path, monkeypatch):
+    """PAMR_save_gfn(_close): every grid of the selected levels, the device
+    data at call time, through an application writer (gft_out_bbox's
+    signature) and through the built-in record file (append; close, then
+    restart)."""
+    from pamr_amrd_amd.gfdump import read_pgf
+    monkeypatch.chdir(tmp_path)
+    m, ctx = pair(7)
+    load_sources(m, ctx, 0, 5)
+    L = ctx.L
+    got, closed = [], []
+
+    def out(name, t, shape, rank, bbox, data):
+        shp = [shape[k] for k in range(rank)]
+        arr = np.ctypeslib.as_array(data, shape=(int(np.prod(shp)),)).copy().reshape(shp[::-1])
+        got.append((name.decode(), t, tuple(shp), [bbox[k] for k in range(2 * rank)], arr))
+        return 1
+
+    out_cb, close_cb = GFT_OUT(out), GFT_CLOSE(lambda name: closed.append(name.decode()) or 1)
+    L.PAMR_set_gft_writer(C.cast(out_cb, C.c_void_p), C.cast(close_cb, C.c_void_p))
+    try:
+        assert L.PAMR_save_gfn_close(b"rho", PAMR_AMRH, 1, -1, -1.0, b"p_", b"_q") == 1
+    finally:
+        L.PAMR_set_gft_writer(None, None)
+    want = [(l, g) for l in range(1, ctx.max_lev + 1) for g in range(len(ctx.grids(PAMR_AMRH, l)))]
+    assert len(got) == len(want) and closed == ["p_rho_tl1_q_0"]
+    for (name, t, shp, bb, arr), (l, g) in zip(got, want):
+        shape, bbox, _ = ctx.grids(PAMR_AMRH, l)[g]
+        assert name == "p_rho_tl1_q_0" and t == L.PAMR_get_time(l)
+        assert shp == shape[::-1] and np.array_equal(bb, bbox)
+        assert np.array_equal(arr, ctx.download(PAMR_AMRH, l, g, "rho")), (l, g)
+
+    n2 = len(ctx.grids(PAMR_AMRH, 2))
+    assert L.PAMR_save_gfn(b"rho", PAMR_AMRH, 1, 1, 1.5, b"", b"") == 1
+    assert L.PAMR_save_gfn(b"rho", PAMR_AMRH, 1, 2, 2.5, b"", b"") == 1
+    recs = read_pgf("rho_tl1_0.pgf")
+    assert [r.time for r in recs] == [1.5] + [2.5] * n2
+    assert np.array_equal(recs[0].data, ctx.download(PAMR_AMRH, 1, 0, "rho"))
+    assert np.array_equal(recs[-1].data, ctx.download(PAMR_AMRH, 2, n2 - 1, "rho"))
+    assert L.PAMR_save_gfn_close(b"rho", PAMR_AMRH, 1, 1, 3.5, b"", b"") == 1
+    assert [r.time for r in read_pgf("rho_tl1_0.pgf")] == [1.5] + [2.5] * n2 + [3.5]
+    assert L.PAMR_save_gfn(b"rho", PAMR_AMRH, 1, 1, 4.5, b"", b"") == 1     # closed: starts again
+    assert [r.time for r in read_pgf("rho_tl1_0.pgf")] == [4.5]
+    assert L.PAMR_save_gfn(b"nosuch", PAMR_AMRH, 1, 1, 0.0, b"", b"") == 0
+    assert L.PAMR_save_gfn(b"V_res", PAMR_MGH, 0, 1, 0.0, b"", b"") == 0     # no MG hierarchy yet
+    assert L.PAMR_save_gfn(b"rho", PAMR_AMRH, 1, ctx.max_lev + 1, 0.0, b"", b"") == 0
+    ctx.close()
+
+
+def test_mg_dumps_follow_the_cycle(torch_cuda, tmp_path, monkeypatch):
+    """MG_DV_trace: mg_dump at the V-cycle's four stages writes every MG
+    level in the cycle's order at time lcount + giter / (max_iter + 2); the
+    cycles still match the oracle bit for bit; the first finest-level
+    record is the initial solution, the last cycle's after_pstswp_ records
+    are the final one on every level, and level 1 after_cgc_ equals
+    after_preswp_ (no correction on the coarsest level)."""
+    from pamr_amrd_amd.gfdump import read_pgf
+    monkeypatch.chdir(tmp_path)
+    m, ctx = pair(6)
+    load_sources(m, ctx, 0, 99)
+    m.set_params()
+    m.init()
+    ctx.set_params(pre=2, pst=2, max_iter=6)
+    L = ctx.L
+    nl = ctx.build_mgh()
+    init = ctx.download(PAMR_MGH, nl, 0, "V")
+    assert L.amrd_mg_def_dump_vars(b"", None, b"") == 1
+    L.amrd_mg_set_trace(0, 1, -1.0, 1.0)
+    try:
+        for c in range(2):
+            rm, _, _ = m.vcycle()
+            ra, _, _ = ctx.vcycle(c)
+            assert abs(ra - rm) <= 1e-12 * rm, (c, ra, rm)
+    finally:
+        L.amrd_mg_set_trace(0, -1, 1e-15, 1e-15)
+    down = [(Lm, g) for Lm in range(nl, 0, -1) for g in range(m.ngrids(Lm))]
+    up = [(Lm, g) for Lm in range(1, nl + 1) for g in range(m.ngrids(Lm))]
+    R = {}
+    for tag, order, which in (("before_preswp_", down, 0), ("after_preswp_", down, 0), ("after_cgc_", up, 1),
+                              ("after_pstswp_", up, 1)):
+        names = ["V", "V_lop", "V_rhs", "V_res"] + (["cmask"] if which == 0 else [])
+        for nm in names:
+            recs = read_pgf(f"{tag}{nm}_MG_0.pgf")
+            assert len(recs) == 2 * len(order), (tag, nm)
+            for r, (Lm, g) in zip(recs, order + order):
+                shape, bbox, _ = ctx.grids(PAMR_MGH, Lm)[g]
+                assert r.shape == shape[::-1] and np.array_equal(r.bbox, bbox), (tag, nm, Lm, g)
+            R[tag, nm] = (recs, order)
+        if which == 1:
+            assert not (tmp_path / f"{tag}cmask_MG_0.pgf").exists()
+    t0 = R["before_preswp_", "V"][0][0].time
+    assert t0 == int(t0)
+    for (tag, nm), (recs, order) in R.items():
+        n = len(order)
+        assert all(r.time == t0 for r in recs[:n]) and all(r.time == t0 + 1 / 8 for r in recs[n:]), (tag, nm)
+    assert np.array_equal(R["before_preswp_", "V"][0][0].data, init)
+    recs, order = R["after_pstswp_", "V"]
+    for r, (Lm, g) in zip(recs[len(order):], order):
+        assert np.array_equal(r.data, m.field(Lm, g, O.OA_U)), ("final", Lm, g)
+    pre, order = R["after_preswp_", "V"]
+    cgc, _ = R["after_cgc_", "V"]
+    k1 = [i for i, (Lm, _) in enumerate(order) if Lm == 1]
+    for i, j in zip(k1, range(len(k1))):
+        assert np.array_equal(pre[i].data, cgc[j].data)
+    ctx.close()
+
+
+def test_mg_trace_lines(torch_cuda, capfd):
+    """MG_trace 4: solve_elliptics prints the reference's header, per-sweep
+    and per-level norm lines, per-cycle residuals and the cycle count."""
+    m, ctx = pair(3)
+    load_sources(m, ctx, 0, 77)
+    ctx.set_params(pre=2, pst=2, min_iter=2, max_iter=5, tol=1e30)
+    L = ctx.L
+    L.amrd_mg_set_trace(4, 0, 1e-15, 1e-15)
+    res = C.c_double()
+    try:
+        capfd.readouterr()
+        it = L.amrd_solve_elliptics(1, 1, C.byref(res))
+        out = capfd.readouterr().out
+    finally:
+        L.amrd_mg_set_trace(0, -1, 1e-15, 1e-15)
+    assert it == 2
+    nl = m.nlev
+    norms = np.zeros(nl)
+    L.amrd_mg_last_norms(pm.P(norms), None)
+    lines = out.splitlines()
+    assert f"MG L = 1 - {ctx.max_lev}, MG Lf={nl}, tl=1 ..." in out
+    assert sum(ln.startswith("iter ") for ln in lines) == 2
+    assert f"iter 2\t\tres(Lf)={res.value:12.8e}" in lines
+    assert "  iters=2" in lines
+    assert f"   L{nl}(2):{norms[nl - 1]:12.8e}" in lines
+    assert sum(ln.startswith(f"   (L{nl}(0):") for ln in lines) == 4    # pre- and post-sweep 0, two cycles
+    ctx.close()
