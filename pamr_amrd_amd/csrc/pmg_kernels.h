@@ -70,47 +70,67 @@ constexpr bool sweep_a_ring() {
   return OP == 1 && 2 * RING * (TY + 4) * (TX + 4) * sizeof(T) <= 128 * 1024;
 }
 
+// LDS column of local x lx in a row of LX (even) values stored colour-split:
+// even lx in the first half of the row, odd lx in the second.  The points of
+// one colour in a row all have the same lx parity, so a wave updating them
+// reads and writes consecutive 8-byte words (no bank conflicts), and so do
+// their x, y and z neighbours.
+template <int LX>
+__device__ __forceinline__ int csx(int lx) {
+  static_assert(LX % 2 == 0, "colour-split rows need an even length");
+  return (lx & 1) * (LX / 2) + (lx >> 1);
+}
+
 // Slot of plane k (k >= -2) in an LDS ring of RING planes.
 template <int RING>
 __device__ __forceinline__ int ring_ix(int k) {
   if constexpr (RING == 4) return k & 3; else return (k + 2 * RING) % RING;
 }
 
-// One colour update at LDS position (lx, ly) of plane k.  rho/rhs are the
-// values at that point (prefetched by the caller); sA is a's ring (AR).
+// Flat LDS offset (within a plane) of local position (lx, ly), and the
+// offset from it to its east neighbour in the colour-split row: +LX/2 for an
+// even lx, 1 - LX/2 for an odd one (the west neighbour is one word before
+// the east one).
+template <int LX>
+__device__ __forceinline__ int cs_off(int lx, int ly) { return ly * LX + csx<LX>(lx); }
+template <int LX>
+__device__ __forceinline__ int cs_east(int lx) { return (lx & 1) ? 1 - LX / 2 : LX / 2; }
+
+// One colour update at flat LDS offset `lo` of plane k (`de`: cs_east).
+// rho/rhs are the values at that point (prefetched by the caller); sA is a's
+// ring (AR).
 template <typename T, int OP, bool P2, bool RHS0, int TX, int TY, bool AR = false, int RING = 4>
-__device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int lx,
-                                         int ly, int k, size_t p, T rho, T rhs,
+__device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4], int lo,
+                                         int de, int k, size_t p, T rho, T rhs,
                                          T (*sA)[TY + 4][TX + 4] = nullptr) {
-  T(*S0)[TX + 4] = sU[ring_ix<RING>(k)];
-  T(*Sm)[TX + 4] = sU[ring_ix<RING>(k - 1)];
-  T(*Sp)[TX + 4] = sU[ring_ix<RING>(k + 1)];
-  const T uc = S0[ly][lx];
+  constexpr int LX = TX + 4;
+  T *S0 = &sU[ring_ix<RING>(k)][0][0];
+  const T *Sm = &sU[ring_ix<RING>(k - 1)][0][0];
+  const T *Sp = &sU[ring_ix<RING>(k + 1)][0][0];
+  const int ce = lo + de, cw = ce - 1;
+  const T uc = S0[lo];
   T jac, L;
   if constexpr (OP == 0) {
-    L = lop_poisson<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
-                           Sp[ly][lx], Sm[ly][lx], rho, P.sp);
+    L = lop_poisson<T, P2>(uc, S0[ce], S0[cw], S0[lo + LX], S0[lo - LX], Sp[lo], Sm[lo], rho, P.sp);
     jac = P.sp.jac;
   } else if constexpr (AR) {
-    T(*A0)[TX + 4] = sA[ring_ix<RING>(k)];
-    T(*Am)[TX + 4] = sA[ring_ix<RING>(k - 1)];
-    T(*Ap)[TX + 4] = sA[ring_ix<RING>(k + 1)];
-    L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
-                           Sp[ly][lx], Sm[ly][lx], A0[ly][lx], A0[ly][lx + 1], A0[ly][lx - 1],
-                           A0[ly + 1][lx], A0[ly - 1][lx], Ap[ly][lx], Am[ly][lx], P.c[p], rho, P.sp, &jac);
+    const T *A0 = &sA[ring_ix<RING>(k)][0][0];
+    const T *Am = &sA[ring_ix<RING>(k - 1)][0][0];
+    const T *Ap = &sA[ring_ix<RING>(k + 1)][0][0];
+    L = lop_varcoef<T, P2>(uc, S0[ce], S0[cw], S0[lo + LX], S0[lo - LX], Sp[lo], Sm[lo], A0[lo], A0[ce],
+                           A0[cw], A0[lo + LX], A0[lo - LX], Ap[lo], Am[lo], P.c[p], rho, P.sp, &jac);
   } else {
     const T *__restrict__ a = P.a;
     const size_t sy = (size_t)P.nx, sz = (size_t)P.nx * P.ny;
-    L = lop_varcoef<T, P2>(uc, S0[ly][lx + 1], S0[ly][lx - 1], S0[ly + 1][lx], S0[ly - 1][lx],
-                           Sp[ly][lx], Sm[ly][lx], a[p], a[p + 1], a[p - 1], a[p + sy], a[p - sy],
-                           a[p + sz], a[p - sz], P.c[p], rho, P.sp, &jac);
+    L = lop_varcoef<T, P2>(uc, S0[ce], S0[cw], S0[lo + LX], S0[lo - LX], Sp[lo], Sm[lo], a[p], a[p + 1],
+                           a[p - 1], a[p + sy], a[p - sy], a[p + sz], a[p - sz], P.c[p], rho, P.sp, &jac);
   }
   T res;
   if constexpr (RHS0) res = L - T(0); else res = L - rhs;
   if constexpr (std::is_same<T, float>::value && OP == 0)
-    S0[ly][lx] = uc - res * P.sp.ijac;  // FP32 inner cycles: no IEEE division
+    S0[lo] = uc - res * P.sp.ijac;  // FP32 inner cycles: no IEEE division
   else
-    S0[ly][lx] = uc - div_jac<T, OP>(res, jac, P.sp);  // FP64: the reference's V - res/Jac, bit for bit
+    S0[lo] = uc - div_jac<T, OP>(res, jac, P.sp);  // FP64: the reference's V - res/Jac, bit for bit
   return res;
 }
 
@@ -129,7 +149,8 @@ __device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[T
     const size_t p = ix3(i, j, k, P.nx, P.ny);
     if constexpr (EX) { if (P.chr[p]) continue; }
     const T rhs = RHS0 ? T(0) : P.rhs[p];
-    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR, RING>(P, sU, lx, ly, k, p, P.rho[p], rhs, sA);
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR, RING>(P, sU, cs_off<TX + 4>(lx, ly), cs_east<TX + 4>(lx),
+                                                                 k, p, P.rho[p], rhs, sA);
     if (count && lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2) {
       s += (double)res * (double)res;
       cnt += 1.0;
@@ -142,16 +163,18 @@ __device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[T
 template <typename T, int N>
 struct PhaseRegs {
   long long p[N];
-  int lx[N], ly[N];
+  int lo[N], fl[N];  // as in SlotTab
   T rho[N], rhs[N];
 };
 
 // Per-thread slot table of one phase for both plane parities, computed once
-// per block: LDS position, xy offset of the point, and whether the slot is
-// inside the tile region and the updatable x/y range.
+// per block: LDS position (flat colour-split offset `lo`; `fl` = 2 * cs_east
+// + 1 when the slot lies in the tile itself, whose residuals the norm
+// counts), xy offset of the point, and whether the slot is inside the tile
+// region and the updatable x/y range.
 template <int N>
 struct SlotTab {
-  int lx[2][N], ly[2][N];
+  int lo[2][N], fl[2][N];
   int off[2][N];  // i + nx*j, or -1
 };
 
@@ -166,8 +189,8 @@ __device__ __forceinline__ void slot_table(const SweepP<T> &P, int x0, int y0, i
       // plane parity par: the colour test uses (j + k + 3 + pass), k = par mod 2
       if (!sweep_pos<TX, TY, HALO>(threadIdx.x + m * NT, x0, y0, par, pass, lx, ly)) continue;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      S.lx[par][m] = lx;
-      S.ly[par][m] = ly;
+      S.lo[par][m] = cs_off<TX + 4>(lx, ly);
+      S.fl[par][m] = 2 * cs_east<TX + 4>(lx) + ((lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2) ? 1 : 0);
       if (i >= P.bx0 && i <= P.bx1 && j >= P.by0 && j <= P.by1) S.off[par][m] = i + P.nx * j;
     }
 }
@@ -188,8 +211,8 @@ __device__ __forceinline__ void phase_prefetch_t(const SweepP<T> &P, int k, cons
     const size_t p = base + (size_t)off;
     if constexpr (EX) { if (P.chr[p]) continue; }
     R.p[m] = (long long)p;
-    R.lx[m] = odd ? S.lx[1][m] : S.lx[0][m];
-    R.ly[m] = odd ? S.ly[1][m] : S.ly[0][m];
+    R.lo[m] = odd ? S.lo[1][m] : S.lo[0][m];
+    R.fl[m] = odd ? S.fl[1][m] : S.fl[0][m];
     R.rho[m] = P.rho[p];
     if constexpr (!RHS0) R.rhs[m] = P.rhs[p];
   }
@@ -202,9 +225,9 @@ __device__ __forceinline__ void phase_run(const SweepP<T> &P, T (*sU)[TY + 4][TX
 #pragma unroll
   for (int m = 0; m < N; m++) {
     if (R.p[m] < 0) continue;
-    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR, RING>(P, sU, R.lx[m], R.ly[m], k, (size_t)R.p[m],
+    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR, RING>(P, sU, R.lo[m], R.fl[m] >> 1, k, (size_t)R.p[m],
                                                                  R.rho[m], RHS0 ? T(0) : R.rhs[m], sA);
-    if (count && R.lx[m] >= 2 && R.lx[m] < TX + 2 && R.ly[m] >= 2 && R.ly[m] < TY + 2) {
+    if (count && (R.fl[m] & 1)) {
       s += (double)res * (double)res;
       cnt += 1.0;
     }
@@ -253,8 +276,8 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
       if (i >= 0 && i < nx && j >= 0 && j < ny) {
         const size_t p = ix3(i, j, k, nx, ny);
-        S[ly][lx] = uin[p];
-        if constexpr (AR) sA[ring_ix<RING>(k)][ly][lx] = ain[p];
+        S[ly][csx<LX>(lx)] = uin[p];
+        if constexpr (AR) sA[ring_ix<RING>(k)][ly][csx<LX>(lx)] = ain[p];
       }
     }
   };
@@ -267,17 +290,18 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   // the last plane the chunk reads: pass 0 also runs on plane z1 (pass 1 on
   // z1-1 needs its updated points) and reads z1+1
   const int kmax = min(z1 + 1, nz - 1);
+  // unconditional loads (positions outside the grid or the chunk read
+  // element 0, never committed): see phase_prefetch_t
   auto load_issue = [&](Ld &ld, int k) {
 #pragma unroll
     for (int m = 0; m < NLD; m++) {
       const int q = threadIdx.x + m * NT;
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      if (k <= kmax && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) {
-        const size_t p = ix3(i, j, k, nx, ny);
-        ld.u[m] = uin[p];
-        if constexpr (AR) ld.a[m] = ain[p];
-      }
+      const bool ok = k <= kmax && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny;
+      const size_t p = ok ? ix3(i, j, k, nx, ny) : 0;
+      ld.u[m] = uin[p];
+      if constexpr (AR) ld.a[m] = ain[p];
     }
   };
   auto load_commit = [&](const Ld &ld, int k) {
@@ -289,17 +313,21 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
       if (q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny) {
-        S[ly][lx] = ld.u[m];
-        if constexpr (AR) sA[ring_ix<RING>(k)][ly][lx] = ld.a[m];
+        S[ly][csx<LX>(lx)] = ld.u[m];
+        if constexpr (AR) sA[ring_ix<RING>(k)][ly][csx<LX>(lx)] = ld.a[m];
       }
     }
   };
+  // fixed trip count (unrolled): no loop of global stores inside the march
+  constexpr int NST = (TX * TY + NT - 1) / NT;
   auto store = [&](int k) {
     T(*S)[LX] = sU[ring_ix<RING>(k)];
-    for (int q = threadIdx.x; q < TX * TY; q += NT) {
+#pragma unroll
+    for (int m = 0; m < NST; m++) {
+      const int q = threadIdx.x + m * NT;
       const int tx = q % TX, ty = q / TX;
       const int i = x0 + tx, j = y0 + ty;
-      if (i <= nx - 2 && j <= ny - 2) uout[ix3(i, j, k, nx, ny)] = S[ty + 2][tx + 2];
+      if (q < TX * TY && i <= nx - 2 && j <= ny - 2) uout[ix3(i, j, k, nx, ny)] = S[ty + 2][csx<LX>(tx + 2)];
     }
   };
 
@@ -322,14 +350,17 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   SlotTab<N1> tab1;
   slot_table<T, TX, TY, NT, true, N0>(P, x0, y0, 0, tab0);
   slot_table<T, TX, TY, NT, false, N1>(P, x0, y0, 1, tab1);
-  PhaseRegs<T, N0> cur0, nxt0;
-  PhaseRegs<T, N1> cur1, nxt1;
-  phase_prefetch_t<T, EX, RHS0, N0>(P, z0 + 1, tab0, cur0);
-  phase_prefetch_t<T, EX, RHS0, N1>(P, z0, tab1, cur1);
+  // the rho/rhs prefetch sets alternate with the steps like the u sets (no
+  // register copies of values still in flight)
+  PhaseRegs<T, N0> pa0, pb0, pc0;
+  PhaseRegs<T, N1> pa1, pb1, pc1;
+  phase_prefetch_t<T, EX, RHS0, N0>(P, z0 + 1, tab0, pa0);
+  phase_prefetch_t<T, EX, RHS0, N1>(P, z0, tab1, pa1);
   __syncthreads();
   // step k: issue plane k+3+PF and the next step's rho/rhs; pass 0 on k+1;
   // pass 1 on k; store k; commit plane k+3 (issued PF steps earlier)
-  auto step = [&](int k, const Ld &ldc, Ld &ldi) {
+  auto step = [&](int k, const Ld &ldc, Ld &ldi, const PhaseRegs<T, N0> &cur0, const PhaseRegs<T, N1> &cur1,
+                  PhaseRegs<T, N0> &nxt0, PhaseRegs<T, N1> &nxt1) {
     load_issue(ldi, k + 3 + PF);
     phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0);
     phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1);
@@ -346,21 +377,19 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       load_commit(ldc, k + 3);
       __syncthreads();
     }
-    cur0 = nxt0;
-    cur1 = nxt1;
   };
   if constexpr (PF == 1) {
     for (int k = z0; k < z1; k += 2) {
-      step(k, ldA, ldB);
-      if (k + 1 < z1) step(k + 1, ldB, ldA);
+      step(k, ldA, ldB, pa0, pa1, pb0, pb1);
+      if (k + 1 < z1) step(k + 1, ldB, ldA, pb0, pb1, pa0, pa1);
     }
   } else {
     // set (k - z0) % 3 holds plane k+3; the set committed one step earlier
     // takes plane k+5
     for (int k = z0; k < z1; k += 3) {
-      step(k, ldA, ldC);
-      if (k + 1 < z1) step(k + 1, ldB, ldA);
-      if (k + 2 < z1) step(k + 2, ldC, ldB);
+      step(k, ldA, ldC, pa0, pa1, pb0, pb1);
+      if (k + 1 < z1) step(k + 1, ldB, ldA, pb0, pb1, pc0, pc1);
+      if (k + 2 < z1) step(k + 2, ldC, ldB, pc0, pc1, pa0, pa1);
     }
   }
   norm_partial<NT>(s, cnt, P.ws, P.slot, bid, P.nb);
