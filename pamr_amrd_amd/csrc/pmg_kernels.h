@@ -134,30 +134,6 @@ __device__ __forceinline__ T sweep_point(const SweepP<T> &P, T (*sU)[TY + 4][TX 
   return res;
 }
 
-// Update every slot of one phase on plane k with values fetched directly
-// (prologue only).
-template <typename T, int OP, bool EX, bool P2, bool RHS0, int TX, int TY, int NT, bool HALO, bool AR, int RING>
-__device__ __forceinline__ void sweep_phase_direct(const SweepP<T> &P, T (*sU)[TY + 4][TX + 4],
-                                                   T (*sA)[TY + 4][TX + 4], int x0, int y0, int k, int pass,
-                                                   bool count, double &s, double &cnt) {
-  constexpr int W = HALO ? TX + 2 : TX, H = HALO ? TY + 2 : TY;
-  for (int q = threadIdx.x; q < ((W + 1) / 2) * H; q += NT) {
-    int lx, ly;
-    if (!sweep_pos<TX, TY, HALO>(q, x0, y0, k, pass, lx, ly)) continue;
-    const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-    if (!sweep_updatable(P, i, j, k)) continue;
-    const size_t p = ix3(i, j, k, P.nx, P.ny);
-    if constexpr (EX) { if (P.chr[p]) continue; }
-    const T rhs = RHS0 ? T(0) : P.rhs[p];
-    const T res = sweep_point<T, OP, P2, RHS0, TX, TY, AR, RING>(P, sU, cs_off<TX + 4>(lx, ly), cs_east<TX + 4>(lx),
-                                                                 k, p, P.rho[p], rhs, sA);
-    if (count && lx >= 2 && lx < TX + 2 && ly >= 2 && ly < TY + 2) {
-      s += (double)res * (double)res;
-      cnt += 1.0;
-    }
-  }
-}
-
 // Prefetch state of one phase: per slot the point index (or -1) and its
 // rho/rhs values, loaded one step before use.
 template <typename T, int N>
@@ -195,22 +171,30 @@ __device__ __forceinline__ void slot_table(const SweepP<T> &P, int x0, int y0, i
     }
 }
 
+// kpar: the plane whose parity selects the table (k itself, except for the
+// prologue's pass 0 over the tile, which takes the pass-1 table of the plane
+// above: the colours of pass 0 on plane k are those of pass 1 on k + 1).
 template <typename T, bool EX, bool RHS0, int N>
 __device__ __forceinline__ void phase_prefetch_t(const SweepP<T> &P, int k, const SlotTab<N> &S,
-                                                 PhaseRegs<T, N> &R) {
+                                                 PhaseRegs<T, N> &R, int kpar) {
   // the plane parity selects between the two register tables with a
   // select, not an index (a run-time index would move them to scratch)
-  const bool odd = (k & 1) != 0;
+  //
+  // The loads are unconditional (slots outside the region read element 0 and
+  // are marked p = -1): a load behind a branch makes the compiler's wait
+  // counting fall back to waiting for every load in flight, which would
+  // stall each march step on the plane prefetched for the steps after it.
+  const bool odd = (kpar & 1) != 0;
   const bool zok = k >= P.bz0 && k <= P.bz1;
   const size_t base = (size_t)k * P.nx * P.ny;
 #pragma unroll
   for (int m = 0; m < N; m++) {
-    R.p[m] = -1;
     const int off = odd ? S.off[1][m] : S.off[0][m];
-    if (!zok || off < 0) continue;
-    const size_t p = base + (size_t)off;
-    if constexpr (EX) { if (P.chr[p]) continue; }
-    R.p[m] = (long long)p;
+    const bool ok = zok && off >= 0;
+    const size_t p = ok ? base + (size_t)off : 0;
+    bool act = ok;
+    if constexpr (EX) act = act && !P.chr[p];
+    R.p[m] = act ? (long long)p : -1;
     R.lo[m] = odd ? S.lo[1][m] : S.lo[0][m];
     R.fl[m] = odd ? S.fl[1][m] : S.fl[0][m];
     R.rho[m] = P.rho[p];
@@ -268,19 +252,6 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   T *__restrict__ uout = P.uout;
   double s = 0.0, cnt = 0.0;
 
-  auto load_direct = [&](int k) {
-    if (k < 0 || k >= nz) return;
-    T(*S)[LX] = sU[ring_ix<RING>(k)];
-    for (int q = threadIdx.x; q < LX * LY; q += NT) {
-      const int lx = q % LX, ly = q / LX;
-      const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      if (i >= 0 && i < nx && j >= 0 && j < ny) {
-        const size_t p = ix3(i, j, k, nx, ny);
-        S[ly][csx<LX>(lx)] = uin[p];
-        if constexpr (AR) sA[ring_ix<RING>(k)][ly][csx<LX>(lx)] = ain[p];
-      }
-    }
-  };
   // u (and a) planes are prefetched into rotating register sets
   struct Ld {
     T u[NLD];
@@ -298,14 +269,14 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
       const int q = threadIdx.x + m * NT;
       const int lx = q % LX, ly = q / LX;
       const int i = x0 - 2 + lx, j = y0 - 2 + ly;
-      const bool ok = k <= kmax && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny;
+      const bool ok = k >= 0 && k <= kmax && q < LX * LY && i >= 0 && i < nx && j >= 0 && j < ny;
       const size_t p = ok ? ix3(i, j, k, nx, ny) : 0;
       ld.u[m] = uin[p];
       if constexpr (AR) ld.a[m] = ain[p];
     }
   };
   auto load_commit = [&](const Ld &ld, int k) {
-    if (k > kmax) return;
+    if (k < 0 || k > kmax) return;
     T(*S)[LX] = sU[ring_ix<RING>(k)];
 #pragma unroll
     for (int m = 0; m < NLD; m++) {
@@ -331,21 +302,9 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
     }
   };
 
-  // prologue: planes z0-2 .. z0+1, pass 0 on z0-1 (tile) and z0 (tile+1)
-  load_direct(z0 - 2);
-  load_direct(z0 - 1);
-  load_direct(z0);
-  load_direct(z0 + 1);
-  __syncthreads();
-  if (z0 - 1 >= P.bz0 && z0 - 1 <= P.bz1)
-    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, false, AR, RING>(P, sU, sA, x0, y0, z0 - 1, 0, false, s,
-                                                                         cnt);
-  if (z0 >= P.bz0 && z0 <= P.bz1)
-    sweep_phase_direct<T, OP, EX, P2, RHS0, TX, TY, NT, true, AR, RING>(P, sU, sA, x0, y0, z0, 0, true, s, cnt);
-  __syncthreads();
-  load_direct(z0 + 2);
-  load_issue(ldA, z0 + 3);
-  if constexpr (PF == 2) load_issue(ldB, z0 + 4);
+  // prologue: planes z0-2 .. z0+2, pass 0 on z0-1 (tile) and z0 (tile+1).
+  // Every load of the prologue and of the first march step is requested
+  // before any is used (one memory round trip instead of one per plane).
   SlotTab<N0> tab0;
   SlotTab<N1> tab1;
   slot_table<T, TX, TY, NT, true, N0>(P, x0, y0, 0, tab0);
@@ -354,16 +313,39 @@ __device__ __forceinline__ void sweep_tile(const SweepP<T> &P, int x0, int y0, i
   // register copies of values still in flight)
   PhaseRegs<T, N0> pa0, pb0, pc0;
   PhaseRegs<T, N1> pa1, pb1, pc1;
-  phase_prefetch_t<T, EX, RHS0, N0>(P, z0 + 1, tab0, pa0);
-  phase_prefetch_t<T, EX, RHS0, N1>(P, z0, tab1, pa1);
-  __syncthreads();
+  {
+    Ld q0, q1, q2, q3, q4;
+    load_issue(q0, z0 - 2);
+    load_issue(q1, z0 - 1);
+    load_issue(q2, z0);
+    load_issue(q3, z0 + 1);
+    load_issue(q4, z0 + 2);
+    PhaseRegs<T, N1> pro1;  // pass 0 on plane z0-1 over the tile
+    PhaseRegs<T, N0> pro0;  // pass 0 on plane z0 over tile+1
+    phase_prefetch_t<T, EX, RHS0, N1>(P, z0 - 1, tab1, pro1, z0);
+    phase_prefetch_t<T, EX, RHS0, N0>(P, z0, tab0, pro0, z0);
+    load_issue(ldA, z0 + 3);
+    if constexpr (PF == 2) load_issue(ldB, z0 + 4);
+    phase_prefetch_t<T, EX, RHS0, N0>(P, z0 + 1, tab0, pa0, z0 + 1);
+    phase_prefetch_t<T, EX, RHS0, N1>(P, z0, tab1, pa1, z0);
+    load_commit(q0, z0 - 2);
+    load_commit(q1, z0 - 1);
+    load_commit(q2, z0);
+    load_commit(q3, z0 + 1);
+    __syncthreads();
+    phase_run<T, OP, P2, RHS0, TX, TY, N1, AR, RING>(P, sU, sA, z0 - 1, pro1, false, s, cnt);
+    phase_run<T, OP, P2, RHS0, TX, TY, N0, AR, RING>(P, sU, sA, z0, pro0, true, s, cnt);
+    __syncthreads();
+    load_commit(q4, z0 + 2);
+    __syncthreads();
+  }
   // step k: issue plane k+3+PF and the next step's rho/rhs; pass 0 on k+1;
   // pass 1 on k; store k; commit plane k+3 (issued PF steps earlier)
   auto step = [&](int k, const Ld &ldc, Ld &ldi, const PhaseRegs<T, N0> &cur0, const PhaseRegs<T, N1> &cur1,
                   PhaseRegs<T, N0> &nxt0, PhaseRegs<T, N1> &nxt1) {
     load_issue(ldi, k + 3 + PF);
-    phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0);
-    phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1);
+    phase_prefetch_t<T, EX, RHS0, N0>(P, k + 2, tab0, nxt0, k + 2);
+    phase_prefetch_t<T, EX, RHS0, N1>(P, k + 1, tab1, nxt1, k + 1);
     phase_run<T, OP, P2, RHS0, TX, TY, N0, AR, RING>(P, sU, sA, k + 1, cur0, k + 1 < z1, s, cnt);
     __syncthreads();
     phase_run<T, OP, P2, RHS0, TX, TY, N1, AR, RING>(P, sU, sA, k, cur1, true, s, cnt);
