@@ -41,10 +41,11 @@ METRIC = "time to 1e-10 rel. residual per 257^3 solve; learned+FP32 vs plain FP6
 HBM_PEAK_GBS = 8000.0
 
 
-# tile variants of the fused sweep: (TX, TY, threads, u planes in flight)
-SWEEP_SHAPES = {0: (64, 8, 256, 1), 1: (64, 16, 512, 1), 2: (32, 16, 512, 1), 3: (128, 8, 512, 1),
-                4: (64, 32, 512, 1), 5: (128, 16, 1024, 1), 6: (128, 16, 1024, 2), 7: (64, 16, 512, 2),
-                8: (128, 8, 512, 2)}
+# tile variants of the fused sweep: (TX, TY, threads, u planes in flight, LDS ring planes)
+SWEEP_SHAPES = {0: (64, 8, 256, 1, 4), 1: (64, 16, 512, 1, 4), 2: (32, 16, 512, 1, 4), 3: (128, 8, 512, 1, 4),
+                4: (64, 32, 512, 1, 4), 5: (128, 16, 1024, 1, 4), 6: (128, 16, 1024, 2, 4), 7: (64, 16, 512, 2, 4),
+                8: (128, 8, 512, 2, 4), 9: (128, 16, 1024, 2, 5), 10: (128, 16, 1024, 1, 5),
+                11: (128, 8, 512, 1, 5)}
 
 
 def measured_traffic(n, variant, prec="double"):
@@ -54,8 +55,8 @@ def measured_traffic(n, variant, prec="double"):
     path = os.path.join(ROOT, "profiles", f"traffic_finest_{n}.json")
     if not os.path.exists(path) or variant not in SWEEP_SHAPES:
         return None
-    tx, ty, nt, pf = SWEEP_SHAPES[variant]
-    name = f"pmg::k_sweep<{prec}, 0, false, true, true, {tx}, {ty}, {nt}, {pf}>"
+    tx, ty, nt, pf, ring = SWEEP_SHAPES[variant]
+    name = f"pmg::k_sweep<{prec}, 0, false, true, true, {tx}, {ty}, {nt}, {pf}, {ring}>"
     with open(path) as f:
         entry = json.load(f).get("kernels", {}).get(name)
     return None if entry is None else entry["traffic_bytes"]
