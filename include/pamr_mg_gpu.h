@@ -216,8 +216,8 @@ int pmg_set_fp32_exact_bottom(pmg_solver *S, int on);
  * (same results) */
 int pmg_set_restrict_fusion(pmg_solver *S, int on);
 /* coarse planes per workgroup of that fused kernel (prec 0 FP64 cycle, 1 FP32
- * inner cycle; 1..64; default 5 / 12 when the coarse level has at least 129
- * planes, else 8); results are identical for every value.  -3 (value kept)
+ * inner cycle; 1..64; default 6 / 4 when the coarse level has 129 to 256
+ * planes, else 8 / 8); results are identical for every value.  -3 (value kept)
  * when the launch would need more workgroups than the solver's partial-norm
  * workspace holds (small values on large grids) */
 int pmg_set_resid_restrict_zchunk(pmg_solver *S, int prec, int czchunk);
